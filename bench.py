@@ -1,0 +1,212 @@
+"""Benchmark: kinematic sum throughput (atom-pixel pairs/s) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Workload (default): the precision-form sample -- SrTiO3-type perovskite,
+100^3 cells = 5,000,000 atoms (3 species, 64 spatial chunks) with a screw
+dislocation, 10 keV, 512 x 512 pixels at 1 m on the (200) reflection:
+1.31e12 atom-pixel pairs per step.  The sample is resident in HBM before
+the timed region; one step stages it (Morton sort into tiles), runs the
+pixel prologue and the sum.
+
+Prints one JSON line.  ``roofline`` prices the dominant kernel (k_sum) at the
+algorithmic 20 FP32 flop per pair against the 157.3 TFLOP/s FP32 vector peak;
+its duration comes from HIP events around the mosaic_kin_sum call on the
+stream it runs on.  ``cpu_baseline`` times the reference's own C kernel
+(compute_scattering_cffi, built into oracle/_ref) on a bounded pixel sample
+of the same workload, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 20.0
+PEAK_FP32 = 157.3e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="perovskite_screw_5M_512px")
+    ap.add_argument("--cpu-pixels", type=int, default=48)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--err-pixels", type=int, default=48)
+    return ap.parse_args()
+
+
+def build_workload(name):
+    from mosaic_amd import workloads as W
+    if name == "perovskite_screw_5M_512px":
+        return W.perovskite_screw_bragg200(n=100, npix=512)
+    if name == "fcc_au_32k_128px":
+        return W.fcc_au_bragg111(n=20, npix=128)
+    if name.startswith("perovskite_screw_n"):
+        n, p = name[len("perovskite_screw_n"):].split("_")
+        return W.perovskite_screw_bragg200(n=int(n), npix=int(p))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def gather(w):
+    smp = w["sample"]
+    labels, index, pos, codes = [], {}, [], []
+    for cid in range(1, smp.chunk_total + 1):
+        p = smp.load_chunk_positions(cid)
+        c, lab = smp.load_chunk_species(cid, codes=True)
+        rm = []
+        for el in lab:
+            el = str(el)
+            if el not in index:
+                index[el] = len(labels)
+                labels.append(el)
+            rm.append(index[el])
+        pos.append(p)
+        codes.append(np.asarray(rm, dtype=np.uint8)[np.asarray(c, dtype=np.int64)])
+    return np.concatenate(pos), np.concatenate(codes), labels
+
+
+def cpu_baseline(w, n_pix, seed=0):
+    from mosaic_amd import workloads as W
+    from oracle import kinematic as O
+    if not O.ref_available():
+        return None
+    a = W.oracle_inputs(w)
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(a["Ny"] * a["Nz"], size=n_pix, replace=False)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    kw = dict(pos_m=a["pos_m"], f0p=a["f0p"], f0z=a["f0z"], sanr=a["sanr"], sani=a["sani"],
+              amp_r=a["amp_r"], amp_i=a["amp_i"], cx=a["cx"][sel], cy=a["cy"][sel],
+              cz=a["cz"][sel], k_val=a["k_val"], remove_forward=False, Ny=n_pix, Nz=1,
+              apply_pol=True, pol_deg=a["pol_deg"], pol_e=a["pol_e"], apply_decay=False)
+    O.ref_kin_sum(**{**kw, "pos_m": a["pos_m"][:2000], "f0p": a["f0p"][:2000],
+                     "f0z": a["f0z"][:2000], "sanr": a["sanr"][:2000], "sani": a["sani"][:2000],
+                     "amp_r": a["amp_r"][:2000], "amp_i": a["amp_i"][:2000]},
+                  threads=threads)  # warm-up
+    t = time.perf_counter()
+    O.ref_kin_sum(**kw, threads=threads)
+    dt = time.perf_counter() - t
+    pairs = a["pos_m"].shape[0] * n_pix
+    return {"value": pairs / dt, "unit": "atom*pixel pairs/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"compute_scattering_cffi (oracle/_ref, gcc -O3) on all "
+                      f"{a['pos_m'].shape[0]} atoms x {n_pix} random pixels, "
+                      f"{threads} threads over >=1000-atom slices, {dt:.1f} s"}
+
+
+def accuracy(w, field, n_pix, seed=1):
+    """Relative error of the GPU field against the FP64 oracle on random pixels."""
+    from mosaic_amd import workloads as W
+    from oracle import kinematic as O
+    a = W.oracle_inputs(w)
+    rng = np.random.default_rng(seed)
+    sel = rng.choice(a["Ny"] * a["Nz"], size=n_pix, replace=False)
+    R = O.kin_sum_c(a["pos_m"], a["f0p"], a["f0z"], a["sanr"], a["sani"], a["amp_r"],
+                    a["amp_i"], a["cx"][sel], a["cy"][sel], a["cz"][sel], a["k_val"], False,
+                    n_pix, 1, True, a["pol_deg"], a["pol_e"], False, precision="f64",
+                    threads=min(16, os.cpu_count() or 1))
+    A = field.ravel()[sel]
+    return float(np.linalg.norm(A - R) / np.linalg.norm(R))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the kinematic sum is "
+              "not sharded across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mosaic_amd import kinematic as K
+    from mosaic_amd import tables
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    w = build_workload(args.workload)
+    b, d, st = w["beam"], w["detector"], w["stage"]
+    pos, codes, labels = gather(w)
+    n_atoms = pos.shape[0]
+    Ny, Nz = d.shape
+    npix = Ny * Nz
+    pairs = float(n_atoms) * npix
+    k_val = 2.0 * np.pi / b.wavelength
+    consts = tables.species_constants(labels, b.energy)
+    spc = K.species_table(consts)
+    opts = K.kin_opts(k_val, False, True, b._polarization_state(), False)
+    rot = np.asarray(st.rotation, dtype=np.float32)[None]
+    tr = np.asarray(st.translation, dtype=np.float32)[None]
+    counts = np.bincount(codes, minlength=len(labels)).astype(np.int64)
+
+    d_pos = torch.from_numpy(pos).to(dev)
+    d_cod = torch.from_numpy(codes).to(dev)
+    coords = torch.from_numpy(np.ascontiguousarray(d.pixel_coordinates / 1e10)).to(dev)
+    E0 = torch.from_numpy(b._beam_E0_map).to(dev)
+    amp = K.ein_bilinear(d_pos, rot[0], tr[0], E0, b._beam_du, b._beam_dv, b._beam_uc,
+                         b._beam_vc)
+    out = torch.zeros((1, Nz, Ny), dtype=torch.complex128, device=dev)
+    ws = [None]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sum_ms = []
+
+    def step(timed):
+        out.zero_()
+        staged = K.stage(d_pos, d_cod, amp, counts, k_val, 1, Ny, Nz, workspace=ws[0])
+        ws[0] = staged.workspace
+        K.kin_pixels(staged, coords, k_val)
+        ev0.record()
+        K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+        ev1.record()
+        if timed:
+            ev1.synchronize()
+            sum_ms.append(ev0.elapsed_time(ev1))
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        print(f"[bench] step {i + 1}/{args.steps} k_sum {sum_ms[-1]:.2f} ms", file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    ksum_ms = float(np.mean(sum_ms))
+    field = out[0].cpu().numpy()
+    achieved = pairs * FLOP_PER_PAIR / (ksum_ms * 1e-3)
+    res = {
+        "metric": "atom*pixel pairs/s (kinematic sum)",
+        "value": pairs / (ms_step * 1e-3),
+        "unit": "atom*pixel pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "atoms": int(n_atoms), "pixels": [Ny, Nz],
+                   "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_sum", "kernel_ms": ksum_ms,
+                     "kernel_pairs_per_s": pairs / (ksum_ms * 1e-3)},
+    }
+    if args.err_pixels > 0:
+        res["rel_err_vs_fp64_oracle"] = accuracy(w, field, args.err_pixels)
+        res["rel_err_pixels"] = args.err_pixels
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(w, args.cpu_pixels)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

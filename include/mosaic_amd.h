@@ -1,0 +1,134 @@
+/*
+ * mosaic_amd -- C ABI of the MI355X (gfx950) kinematic-scattering and
+ * propagation path.
+ *
+ * Conventions shared by every entry point:
+ *   - plain pointers and sizes only; "device" pointers are HIP device
+ *     memory, "host" pointers are ordinary host memory;
+ *   - every call is ordered on the HIP stream it is given (0 = the null
+ *     stream) and returns an int status (MOSAIC_OK or an error below);
+ *   - the library keeps no state between calls: scratch memory comes from
+ *     the caller as a workspace whose size a *_workspace query returns;
+ *   - complex arrays are interleaved (re, im) pairs.
+ *
+ * Pixel layout follows the reference: p = iz * Ny + iy, fields (Nz, Ny)
+ * row-major (Beam.py:704, 719-720).
+ */
+#ifndef MOSAIC_AMD_H
+#define MOSAIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOSAIC_OK 0
+#define MOSAIC_ERR_ARGS 1          /* a size or pointer is invalid */
+#define MOSAIC_ERR_HIP 2           /* a HIP runtime call failed */
+#define MOSAIC_ERR_UNSUPPORTED 3   /* declined: outside the fast path's error bound */
+#define MOSAIC_ERR_WORKSPACE 4     /* workspace smaller than the query returned */
+
+/* Opaque stream handle (a hipStream_t). */
+typedef void* mosaic_stream_t;
+
+/* Per-species constants of the kinematic sum. */
+typedef struct {
+    float wk[11];     /* Waasmaier-Kirfel [a1..a5, c, b1..b5] (Beam.py:503-527) */
+    float f0_zero;    /* f0(0) = c + sum a_i (Beam.py:600-615) */
+    float fp;         /* f'  at the beam energy (Beam.py:570-598) */
+    float fpp;        /* f'' at the beam energy */
+} mosaic_species_t;
+
+/* Flags and beam constants of the kinematic sum. */
+typedef struct {
+    double k;               /* wavenumber 2 pi / lambda [rad/m] */
+    int remove_forward;     /* subtract f0(0)+f'+if'' below the pixel's Q cut */
+    int apply_pol;          /* scale by sqrt(P) */
+    int apply_decay;        /* scale by R0 / r */
+    int pad_;
+    double pol_deg;         /* degree of linear polarization p */
+    double pol_e[3];        /* its unit vector (lab) */
+} mosaic_kin_opts_t;
+
+/*
+ * Workspace bytes for mosaic_kin_stage + mosaic_kin_sum on a sample of
+ * `atom_count` atoms split by species as `species_counts[n_species]`, for
+ * `n_frames` poses of an Ny x Nz detector.
+ */
+int mosaic_kin_workspace(int64_t atom_count, int n_species,
+                         const int64_t* species_counts /* host */,
+                         int n_frames, int Ny, int Nz, size_t* bytes);
+
+/*
+ * Stage a sample into the workspace: sort the atoms by (species, Morton
+ * code), cut them into 256-atom single-species tiles, and store each tile as
+ * an FP64 origin plus FP32 offsets in wavelengths with the incident
+ * amplitude.  Replaces the reference's per-species partition, Morton order
+ * and per-chunk origins (_partition/_stage, Beam.py:9265-9319;
+ * _morton_order_gpu/_chunk_origins_gpu, Beam.py:1205-1300).
+ *
+ *   pos      device (N, 3) float32, sample frame, Angstrom
+ *   species  device (N,) uint8 species codes (< n_species)
+ *   amp      device (N,) complex64 incident amplitude, or NULL for 1
+ *   max_tile_radius_wl  host out: largest tile radius in wavelengths
+ */
+int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
+                     const float* amp, int n_species,
+                     const int64_t* species_counts /* host */, double k,
+                     int n_frames, int Ny, int Nz, void* workspace,
+                     size_t workspace_bytes, double* max_tile_radius_wl,
+                     mosaic_stream_t stream);
+
+/*
+ * Pixel prologue of the kinematic sum, once per detector geometry: writes
+ * per-pixel rho0 = |T_p|, the phase frac(rho0 / lambda) in double-double and
+ * the forward-removal Q cut (Beam.py:718-746) into the workspace, and
+ * returns the smallest pixel distance `rho_min` [m] (host out; the call
+ * synchronizes the stream for it).  Must precede mosaic_kin_sum on the same
+ * workspace.  `coords` is device (3, Ny*Nz) float64 metres.
+ */
+int mosaic_kin_pixels(int Ny, int Nz, const double* coords, double k, void* workspace,
+                      size_t workspace_bytes, double* rho_min, mosaic_stream_t stream);
+
+/*
+ * Kinematic sum of a staged sample for `n_frames` sample poses
+ *   lab = rot[f] @ sample + trans[f]   (rot row-major float32 3x3, trans Angstrom)
+ * onto pixels `coords` (device (3, Ny*Nz) float64 metres), accumulated into
+ * `out` (device (n_frames, Nz, Ny) complex64, or complex128 when
+ * out_is_c128).  Replaces compute_scattering_cffi (Beam.py:629-816) and the
+ * mosaic_fast launch (Beam.py:8246-8632, 9336-9391): same per-pair
+ * semantics, with form factor, polarization and decay evaluated once per
+ * (tile, pixel) and the phase split into an FP64 (tile, pixel) origin term
+ * and an FP32 per-atom offset.  Stream-ordered, no host synchronization.
+ * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
+ * close for that split's error bound -- the reference's "decline"
+ * (Beam.py:9466-9468).
+ */
+int mosaic_kin_sum(int64_t atom_count, int n_species,
+                   const int64_t* species_counts /* host */,
+                   const mosaic_species_t* species_tab /* host [n_species] */,
+                   const mosaic_kin_opts_t* opts /* host */,
+                   int n_frames, const float* rot /* host */,
+                   const float* trans /* host */, int Ny, int Nz,
+                   const double* coords, void* out, int out_is_c128,
+                   double max_tile_radius_wl, double rho_min, void* workspace,
+                   size_t workspace_bytes, mosaic_stream_t stream);
+
+/*
+ * Incident amplitude per atom by bilinear sampling of the beam's entrance
+ * field E0 (NyB x NzB complex64, 'ij' layout) at the lab position
+ * rot @ pos + trans; zero outside the grid.  The kinematic case of
+ * ein_bilinear_kernel / _ein_bilinear_cpu with tau = phi = 0
+ * (Beam.py:956-1045, 2439-2578).
+ */
+int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const float* rot /* host */,
+                        const float* trans /* host */, const float* E0 /* device */,
+                        int NyB, int NzB, double du, double dv, double uc,
+                        double vc, float* amp_out, mosaic_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOSAIC_AMD_H */

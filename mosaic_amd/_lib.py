@@ -1,0 +1,92 @@
+"""ctypes binding of ``libmosaic_amd.so`` (the C ABI in include/mosaic_amd.h).
+
+The library must be built (``python -m mosaic_amd.build``); a missing
+library raises ``MosaicLibraryError`` -- there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmosaic_amd.so")
+
+MOSAIC_OK = 0
+MOSAIC_ERR_ARGS = 1
+MOSAIC_ERR_HIP = 2
+MOSAIC_ERR_UNSUPPORTED = 3
+MOSAIC_ERR_WORKSPACE = 4
+
+_ERRORS = {
+    MOSAIC_ERR_ARGS: "invalid argument",
+    MOSAIC_ERR_HIP: "HIP runtime error",
+    MOSAIC_ERR_UNSUPPORTED: "declined: geometry outside the fast path's error bound",
+    MOSAIC_ERR_WORKSPACE: "workspace too small",
+}
+
+# Every symbol include/mosaic_amd.h declares.
+EXPORTS = (
+    "mosaic_kin_workspace",
+    "mosaic_kin_stage",
+    "mosaic_kin_pixels",
+    "mosaic_kin_sum",
+    "mosaic_ein_bilinear",
+)
+
+
+class MosaicLibraryError(RuntimeError):
+    pass
+
+
+class MosaicDeclined(RuntimeError):
+    """The fast path declined (MOSAIC_ERR_UNSUPPORTED); nothing was written."""
+
+
+class Species(ctypes.Structure):
+    _fields_ = [("wk", ctypes.c_float * 11), ("f0_zero", ctypes.c_float),
+                ("fp", ctypes.c_float), ("fpp", ctypes.c_float)]
+
+
+class KinOpts(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_double), ("remove_forward", ctypes.c_int),
+                ("apply_pol", ctypes.c_int), ("apply_decay", ctypes.c_int),
+                ("pad_", ctypes.c_int), ("pol_deg", ctypes.c_double),
+                ("pol_e", ctypes.c_double * 3)]
+
+
+_LIB = None
+
+_vp, _i, _i64, _d, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                          ctypes.c_size_t)
+_SIGS = {
+    "mosaic_kin_workspace": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
+    "mosaic_kin_stage": [_i64, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz,
+                         ctypes.POINTER(_d), _vp],
+    "mosaic_kin_pixels": [_i, _i, _vp, _d, _vp, _sz, ctypes.POINTER(_d), _vp],
+    "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _d, _d,
+                       _vp, _sz, _vp],
+    "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
+}
+
+
+def lib():
+    """The loaded library; raises MosaicLibraryError when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MosaicLibraryError(
+                f"{LIB_PATH} is missing: build it with `python -m mosaic_amd.build` "
+                "(there is no non-HIP fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def check(status, what):
+    if status == MOSAIC_OK:
+        return
+    if status == MOSAIC_ERR_UNSUPPORTED:
+        raise MosaicDeclined(f"{what}: {_ERRORS[status]}")
+    raise MosaicLibraryError(f"{what} failed: {_ERRORS.get(status, status)}")

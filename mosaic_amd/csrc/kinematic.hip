@@ -1,0 +1,821 @@
+// Kinematic scattering sum on gfx950.
+//
+// Semantics: compute_scattering_cffi, Beam.py:663-816 -- per atom j and
+// pixel p the term
+//     amp_j * (f0(Q) + f' + i f'') * exp(i k (x_j + |T_p - r_j|)) * r_e
+//           * [R0/r] * [sqrt(P)]
+// accumulated over atoms.  The MI355X formulation:
+//
+//  * Staging (mosaic_kin_stage): atoms sorted by (species, Morton code) and
+//    cut into 256-atom single-species tiles.  A tile stores its origin O_t
+//    in FP64 and each atom as an FP32 offset sigma_j = (r_j - O_t) / lambda
+//    (in wavelengths) with its complex amplitude, 32 B per atom.
+//  * Pixel prologue: per pixel, rho0 = |T_p| and Theta0 = frac(rho0/lambda)
+//    in double-double, and the reference's forward-removal Q cut.
+//  * Sum: per (tile, pixel) an FP64 prologue gives the origin phase
+//        Theta = frac(Theta0 + (X_t + |T_p - O_t| - rho0) / lambda),
+//    the direction v = R^T (x_hat - u_hat) (u_hat towards the pixel) and
+//    the hoisted factor G = (f0(Q)+f'+if'') r_e [R0/rho] [sqrt P] e^{2 pi i
+//    Theta}.  Per pair only FP32 work remains:
+//        psi = v . sigma_j (+ second-order term when the geometry needs it),
+//        S  += w_j * (cos 2 pi psi, sin 2 pi psi)
+//    with sin/cos on the transcendental unit (input in revolutions), the
+//    atom record read once per wave from the scalar cache (the tile address
+//    is wave-uniform).  After the tile, acc += G * S in FP64.
+//  * Atom slices over gridDim.y keep ~2k workgroups in flight for small
+//    detectors; slice partials (FP64) are summed in slice order by a second
+//    kernel, so results do not depend on scheduling.
+//
+// The hoist is exact to first order in (tile radius / distance); the host
+// declines (MOSAIC_ERR_UNSUPPORTED) when that ratio exceeds 1e-6 or the
+// dropped third-order phase term exceeds 1e-7 revolutions.
+
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace mosaic {
+namespace {
+
+constexpr int TILE = 256;            // atoms per tile
+constexpr int SUM_THREADS = 256;     // threads per sum workgroup
+constexpr int MAX_SPECIES = 16;
+constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
+constexpr double R_E = 2.81794092e-15;
+constexpr double PI = 3.14159265358979323846;
+
+struct AtomRec {      // 32 B, read whole by one scalar load
+    float sx, sy, sz, s2;
+    float wr, wi, pad0, pad1;
+};
+
+struct TileRec {      // 32 B
+    double ox, oy, oz;   // sample-frame origin [m]
+    int species;
+    int count;
+};
+
+struct FrameRec {     // 128 B
+    double R[9];     // sample -> lab, row-major
+    double T[3];     // lab translation [m]
+    double xs[3];    // R^T x_hat
+    double pad;
+};
+
+struct Layout {
+    double* theta0;
+    double* rho0;
+    double* qcut;
+    double* rho_min_blocks;
+    double* rho_min;
+    TileRec* tiles;
+    AtomRec* atoms;
+    uint64_t* keys_a;
+    uint64_t* keys_b;
+    uint32_t* vals_a;
+    uint32_t* vals_b;
+    void* sort_tmp;
+    size_t sort_bytes;
+    double* bbox_blocks;
+    double* bbox;
+    unsigned int* radius_bits;
+    FrameRec* frames;
+    double2* partial;
+    size_t total;
+    int n_tiles;
+    int n_slices;
+    int tiles_per_slice;
+    int pix_blocks;
+};
+
+constexpr int PRO_THREADS = 256;
+constexpr int BBOX_THREADS = 256;
+constexpr int BBOX_MAX_BLOCKS = 1024;
+
+int pix_per_thread(int npix) { return npix >= 4 * SUM_THREADS * 64 ? 8 : 4; }
+
+int count_tiles(int n_species, const int64_t* counts) {
+    int64_t t = 0;
+    for (int s = 0; s < n_species; ++s) t += (counts[s] + TILE - 1) / TILE;
+    return static_cast<int>(t);
+}
+
+// Workspace layout; also the size query when base is null.
+int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
+                int n_frames, int npix, Layout& L) {
+    Carver c(base);
+    L.theta0 = c.take<double>(npix);
+    L.rho0 = c.take<double>(npix);
+    L.qcut = c.take<double>(npix);
+    L.rho_min_blocks = c.take<double>(blocks_for(npix, PRO_THREADS));
+    L.rho_min = c.take<double>(1);
+    L.n_tiles = count_tiles(n_species, counts);
+    L.tiles = c.take<TileRec>(std::max(1, L.n_tiles));
+    L.atoms = c.take<AtomRec>(static_cast<size_t>(std::max(1, L.n_tiles)) * TILE);
+    size_t n = static_cast<size_t>(std::max<int64_t>(1, n_atoms));
+    L.keys_a = c.take<uint64_t>(n);
+    L.keys_b = c.take<uint64_t>(n);
+    L.vals_a = c.take<uint32_t>(n);
+    L.vals_b = c.take<uint32_t>(n);
+    size_t sort_bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  static_cast<unsigned int>(n), 0, 64) != hipSuccess)
+        return MOSAIC_ERR_HIP;
+    L.sort_bytes = sort_bytes;
+    L.sort_tmp = c.take<char>(sort_bytes);
+    L.bbox_blocks = c.take<double>(6 * BBOX_MAX_BLOCKS);
+    L.bbox = c.take<double>(6);
+    L.radius_bits = c.take<unsigned int>(1);
+    L.frames = c.take<FrameRec>(std::max(1, n_frames));
+    const int ppt = pix_per_thread(npix);
+    L.pix_blocks = blocks_for(npix, SUM_THREADS * ppt);
+    const int64_t grid_px = static_cast<int64_t>(L.pix_blocks) * std::max(1, n_frames);
+    int slices = static_cast<int>((TARGET_BLOCKS + grid_px - 1) / grid_px);
+    slices = std::max(1, std::min(slices, std::max(1, L.n_tiles)));
+    L.tiles_per_slice = (std::max(1, L.n_tiles) + slices - 1) / slices;
+    L.n_slices = (std::max(1, L.n_tiles) + L.tiles_per_slice - 1) / L.tiles_per_slice;
+    L.partial = c.take<double2>(static_cast<size_t>(L.n_slices) * std::max(1, n_frames) * npix);
+    L.total = c.off + 256;
+    return MOSAIC_OK;
+}
+
+// ----------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_min(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide min of one value per thread (blockDim multiple of 64, <= 1024).
+__device__ double block_min(double v, double* lds) {
+    v = wave_min(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+    double r = lds[0];
+    for (int i = 1; i < nw; ++i) r = fmin(r, lds[i]);
+    __syncthreads();
+    return r;
+}
+__device__ double block_max(double v, double* lds) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+    double r = lds[0];
+    for (int i = 1; i < nw; ++i) r = fmax(r, lds[i]);
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------- pixel prologue
+// Per pixel: rho0 = |T| and Theta0 = frac(K rho0) in double-double
+// (K = k / 2 pi), and the forward-removal cut-off (Beam.py:718-746).
+__global__ __launch_bounds__(PRO_THREADS) void k_pixels(
+    const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ cz, int Ny, int Nz, double k, double* __restrict__ theta0,
+    double* __restrict__ rho0, double* __restrict__ qcut, double* __restrict__ rho_min_blocks) {
+    __shared__ double lds[PRO_THREADS / 64];
+    const int npix = Ny * Nz;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double rmin = 1e300;
+    if (p < npix) {
+        const double tx = cx[p], ty = cy[p], tz = cz[p];
+        // |T|^2 in double-double
+        double a, ae, b, be, c, ce, s, se, s2, se2;
+        two_prod(tx, tx, a, ae);
+        two_prod(ty, ty, b, be);
+        two_prod(tz, tz, c, ce);
+        two_sum(a, b, s, se);
+        two_sum(s, c, s2, se2);
+        const double sq_hi = s2, sq_lo = (se + se2) + (ae + be + ce);
+        double r = sqrt(sq_hi);
+        double rr, rre;
+        two_prod(r, r, rr, rre);
+        const double resid = ((sq_hi - rr) - rre) + sq_lo;
+        const double r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
+        // K = k / (2 pi) in double-double
+        double K_hi, K_lo;
+        two_prod(k, INV_2PI_HI, K_hi, K_lo);
+        K_lo = fma(k, INV_2PI_LO, K_lo);
+        // Theta = K * rho in double-double, then fractional part
+        double th, the;
+        two_prod(K_hi, r, th, the);
+        the += K_hi * r_lo + K_lo * r;
+        const double n = floor(th);
+        double f = (th - n) + the;
+        f -= floor(f);
+        theta0[p] = f;
+        rho0[p] = r + r_lo;
+        rmin = r;
+
+        // Q cut from the right and upper neighbours (inner one at the edge)
+        const int ix = p % Ny, iy = p / Ny;
+        double ux = 0.0, uy = 0.0, uz = 0.0;
+        if (r > 0.0) { ux = tx / r; uy = ty / r; uz = tz / r; }
+        const int nr = (ix + 1 < Ny) ? (p + 1) : ((ix > 0) ? (p - 1) : p);
+        const int nu = (iy + 1 < Nz) ? (p + Ny) : ((iy > 0) ? (p - Ny) : p);
+        double q2[2];
+        const int nb[2] = {nr, nu};
+        for (int t = 0; t < 2; ++t) {
+            const double rx = cx[nb[t]], ry = cy[nb[t]], rz = cz[nb[t]];
+            const double Rr = sqrt(rx * rx + ry * ry + rz * rz);
+            double cd = (Rr > 0.0) ? (ux * rx + uy * ry + uz * rz) / Rr : 1.0;
+            cd = fmin(1.0, fmax(-1.0, cd));
+            q2[t] = k * sqrt(fmax(0.0, 2.0 * (1.0 - cd)));
+        }
+        qcut[p] = 0.5 * sqrt(q2[0] * q2[0] + q2[1] * q2[1]);
+    }
+    rmin = block_min(rmin, lds);
+    if (threadIdx.x == 0) rho_min_blocks[blockIdx.x] = rmin;
+}
+
+__global__ void k_min_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
+    __shared__ double lds[16];
+    double v = 1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, in[i]);
+    v = block_min(v, lds);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+// ------------------------------------------------------------------- staging
+__global__ __launch_bounds__(BBOX_THREADS) void k_bbox_partial(
+    const float* __restrict__ pos, int64_t n, double* __restrict__ out) {
+    __shared__ double lds[BBOX_THREADS / 64];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        for (int d = 0; d < 3; ++d) {
+            const double x = static_cast<double>(pos[3 * i + d]);
+            lo[d] = fmin(lo[d], x);
+            hi[d] = fmax(hi[d], x);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double a = block_min(lo[d], lds);
+        const double b = block_max(hi[d], lds);
+        if (threadIdx.x == 0) {
+            out[6 * blockIdx.x + d] = a;
+            out[6 * blockIdx.x + 3 + d] = b;
+        }
+    }
+}
+
+__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ bbox,
+                             unsigned int* __restrict__ radius_bits) {
+    __shared__ double lds[16];
+    for (int d = 0; d < 3; ++d) {
+        double a = 1e300, b = -1e300;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            a = fmin(a, part[6 * i + d]);
+            b = fmax(b, part[6 * i + 3 + d]);
+        }
+        a = block_min(a, lds);
+        b = block_max(b, lds);
+        if (threadIdx.x == 0) { bbox[d] = a; bbox[3 + d] = b; }
+    }
+    if (threadIdx.x == 0) radius_bits[0] = 0u;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+    x &= 0xFFFFFull;                         // 20 bits
+    x = (x | (x << 32)) & 0x00FF00000000FFFFull;
+    x = (x | (x << 16)) & 0x00FF0000FF0000FFull;
+    x = (x | (x << 8)) & 0xF00F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x30C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x9249249249249249ull;
+    return x;
+}
+
+// Sort key: species in bits 60..63, 3 x 20-bit Morton code below.
+__global__ void k_keys(const float* __restrict__ pos, const uint8_t* __restrict__ spc, int64_t n,
+                       const double* __restrict__ bbox, uint64_t* __restrict__ keys,
+                       uint32_t* __restrict__ vals) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t code = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double lo = bbox[d], ext = bbox[3 + d] - lo;
+        double q = ext > 0.0 ? (static_cast<double>(pos[3 * i + d]) - lo) / ext : 0.0;
+        q = fmin(fmax(q, 0.0), 1.0) * 1048575.0;
+        code |= spread3(static_cast<uint64_t>(q)) << d;
+    }
+    keys[i] = (static_cast<uint64_t>(spc[i] & 0xF) << 60) | code;
+    vals[i] = static_cast<uint32_t>(i);
+}
+
+struct SpeciesSplit {
+    int n_species;
+    int64_t atom_start[MAX_SPECIES + 1];
+    int tile_start[MAX_SPECIES + 1];
+};
+
+// One workgroup per tile: FP64 origin at the tile's bounding-box centre, FP32
+// offsets in wavelengths, amplitudes; padding lanes get zero weight.
+__global__ __launch_bounds__(TILE) void k_tiles(
+    const float* __restrict__ pos, const float2* __restrict__ amp,
+    const uint32_t* __restrict__ order, SpeciesSplit split, double K,
+    TileRec* __restrict__ tiles, AtomRec* __restrict__ atoms,
+    unsigned int* __restrict__ radius_bits) {
+    __shared__ double lds[TILE / 64];
+    const int t = blockIdx.x;
+    int s = 0;
+    while (s + 1 < split.n_species && t >= split.tile_start[s + 1]) ++s;
+    const int64_t begin = split.atom_start[s] + static_cast<int64_t>(t - split.tile_start[s]) * TILE;
+    const int64_t end = split.atom_start[s + 1];
+    const int count = static_cast<int>(min<int64_t>(TILE, end - begin));
+    const int i = threadIdx.x;
+    const bool live = i < count;
+    uint32_t idx = live ? order[begin + i] : 0u;
+    double r[3] = {0.0, 0.0, 0.0};
+    if (live)
+        for (int d = 0; d < 3; ++d) r[d] = static_cast<double>(pos[3 * (int64_t)idx + d]) / 1e10;
+    double o[3];
+    for (int d = 0; d < 3; ++d) {
+        const double a = block_min(live ? r[d] : 1e300, lds);
+        const double b = block_max(live ? r[d] : -1e300, lds);
+        o[d] = 0.5 * (a + b);
+    }
+    AtomRec rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float rad = 0.f;
+    if (live) {
+        rec.sx = static_cast<float>((r[0] - o[0]) * K);
+        rec.sy = static_cast<float>((r[1] - o[1]) * K);
+        rec.sz = static_cast<float>((r[2] - o[2]) * K);
+        rec.s2 = rec.sx * rec.sx + rec.sy * rec.sy + rec.sz * rec.sz;
+        if (amp) {
+            const float2 w = amp[idx];
+            rec.wr = w.x;
+            rec.wi = w.y;
+        } else {
+            rec.wr = 1.f;
+        }
+        rad = sqrtf(rec.s2);
+    }
+    atoms[static_cast<int64_t>(t) * TILE + i] = rec;
+    const double rmax = block_max(static_cast<double>(rad), lds);
+    if (i == 0) {
+        TileRec tr;
+        tr.ox = o[0];
+        tr.oy = o[1];
+        tr.oz = o[2];
+        tr.species = s;
+        tr.count = count;
+        tiles[t] = tr;
+        atomicMax(radius_bits, __float_as_uint(static_cast<float>(rmax)));
+    }
+}
+
+// ---------------------------------------------------------------------- sum
+struct SpeciesConst {
+    float a[5], b[5], c, f0z, fp, fpp;
+};
+
+struct SumArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double* cx;
+    const double* cy;
+    const double* cz;
+    const double* theta0;
+    const double* rho0;
+    const double* qcut;
+    double2* partial;
+    int n_tiles;
+    int tiles_per_slice;
+    int n_slices;
+    int npix;
+    double K;         // 1 / lambda
+    double k;         // 2 pi / lambda
+    double pol_deg;
+    double pol_e[3];
+    int remove_forward;
+    int apply_pol;
+    int apply_decay;
+    int n_species;
+    SpeciesConst spc[MAX_SPECIES];
+};
+
+template <int PIX, bool SECOND>
+__global__ __launch_bounds__(SUM_THREADS) void k_sum(const SumArgs A) {
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+
+    int pix[PIX];
+    bool valid[PIX];
+#pragma unroll
+    for (int i = 0; i < PIX; ++i) {
+        const int p = blockIdx.x * (SUM_THREADS * PIX) + i * SUM_THREADS + threadIdx.x;
+        valid[i] = p < A.npix;
+        pix[i] = valid[i] ? p : A.npix - 1;
+    }
+    double acc_r[PIX], acc_i[PIX];
+#pragma unroll
+    for (int i = 0; i < PIX; ++i) { acc_r[i] = 0.0; acc_i[i] = 0.0; }
+
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    const float xs0 = static_cast<float>(F.xs[0]);
+    const float xs1 = static_cast<float>(F.xs[1]);
+    const float xs2 = static_cast<float>(F.xs[2]);
+
+    for (int t = t0; t < t1; ++t) {
+        const TileRec tr = A.tiles[t];
+        const SpeciesConst& sc = A.spc[tr.species];
+        // tile origin in the lab frame
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+
+        float vx[PIX], vy[PIX], vz[PIX], gr[PIX], gi[PIX], hh[PIX];
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) {
+            const int p = pix[i];
+            const double tx = A.cx[p], ty = A.cy[p], tz = A.cz[p];
+            const double Dx = tx - Ox, Dy = ty - Oy, Dz = tz - Oz;
+            const double rho = sqrt(Dx * Dx + Dy * Dy + Dz * Dz);
+            const double r0 = A.rho0[p];
+            // rho - rho0 without cancellation: (|O|^2 - 2 T.O) / (rho + rho0)
+            const double dR = (O2 - 2.0 * (tx * Ox + ty * Oy + tz * Oz)) / (rho + r0);
+            double th = A.theta0[p] + A.K * (Ox + dR);
+            th -= rint(th);                          // [-0.5, 0.5]
+            const double inv = 1.0 / rho;
+            const double ux = Dx * inv, uy = Dy * inv, uz = Dz * inv;
+            const double wx = 1.0 - ux, wy = -uy, wz = -uz;  // x_hat - u_hat (lab)
+            vx[i] = static_cast<float>(R00 * wx + R10 * wy + R20 * wz);
+            vy[i] = static_cast<float>(R01 * wx + R11 * wy + R21 * wz);
+            vz[i] = static_cast<float>(R02 * wx + R12 * wy + R22 * wz);
+            hh[i] = static_cast<float>(0.5 / (A.K * rho));
+            // 1 - cos(2 theta) without cancellation for forward pixels
+            const double omc = (Dx > 0.0) ? (Dy * Dy + Dz * Dz) * inv / (rho + Dx) : 1.0 - ux;
+            const double Q = A.k * sqrt(fmax(0.0, 2.0 * omc));
+            const float sq = static_cast<float>(Q * (0.25e-10 / PI));
+            const float ss = sq * sq;
+            float f0 = sc.c;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) f0 += sc.a[m] * __expf(-sc.b[m] * ss);
+            float s_re = f0 + sc.fp, s_im = sc.fpp;
+            if (A.remove_forward && Q < A.qcut[p]) {
+                s_re -= sc.f0z + sc.fp;
+                s_im -= sc.fpp;
+            }
+            double scale = R_E;
+            if (A.apply_decay) scale *= r0 * inv;
+            if (A.apply_pol) {
+                const double er = A.pol_e[0] * ux + A.pol_e[1] * uy + A.pol_e[2] * uz;
+                const double dv = ux;
+                double P = 0.5 * (1.0 - A.pol_deg) * (1.0 + dv * dv) + A.pol_deg * (1.0 - er * er);
+                P = fmin(1.0, fmax(0.0, P));
+                scale *= sqrt(P);
+            }
+            float sn, cs;
+            sincospif(2.0f * static_cast<float>(th), &sn, &cs);
+            const float scf = static_cast<float>(scale);
+            const float br = s_re * scf, bi = s_im * scf;
+            gr[i] = br * cs - bi * sn;
+            gi[i] = br * sn + bi * cs;
+        }
+
+        float S_r[PIX], S_i[PIX];
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) { S_r[i] = 0.f; S_i[i] = 0.f; }
+        const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
+        const int cnt = tr.count;
+#pragma unroll 2
+        for (int j = 0; j < cnt; ++j) {
+            const AtomRec a = at[j];
+            float xsd = 0.f;
+            if (SECOND) xsd = fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx));
+#pragma unroll
+            for (int i = 0; i < PIX; ++i) {
+                float psi = fmaf(vz[i], a.sz, fmaf(vy[i], a.sy, vx[i] * a.sx));
+                if (SECOND) {
+                    const float u = xsd - psi;          // u_hat . sigma (sample frame)
+                    psi = fmaf(hh[i], fmaf(-u, u, a.s2), psi);
+                }
+                const float r = __builtin_amdgcn_fractf(psi);
+                const float sn = __builtin_amdgcn_sinf(r);
+                const float cs = __builtin_amdgcn_cosf(r);
+                S_r[i] = fmaf(a.wr, cs, S_r[i]);
+                S_r[i] = fmaf(-a.wi, sn, S_r[i]);
+                S_i[i] = fmaf(a.wr, sn, S_i[i]);
+                S_i[i] = fmaf(a.wi, cs, S_i[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) {
+            const double g_r = gr[i], g_i = gi[i], s_r = S_r[i], s_i = S_i[i];
+            acc_r[i] = fma(g_r, s_r, fma(-g_i, s_i, acc_r[i]));
+            acc_i[i] = fma(g_r, s_i, fma(g_i, s_r, acc_i[i]));
+        }
+    }
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+#pragma unroll
+    for (int i = 0; i < PIX; ++i)
+        if (valid[i]) out[pix[i]] = make_double2(acc_r[i], acc_i[i]);
+}
+
+// Sum slice partials in slice order and add into the output field.
+template <bool C128>
+__global__ void k_reduce(const double2* __restrict__ partial, int n_slices, int npix,
+                         int n_frames, void* __restrict__ out) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= static_cast<int64_t>(npix) * n_frames) return;
+    const int64_t f = q / npix, p = q % npix;
+    const double2* src = partial + f * n_slices * (int64_t)npix + p;
+    double r = 0.0, im = 0.0;
+    for (int s = 0; s < n_slices; ++s) {
+        const double2 v = src[static_cast<int64_t>(s) * npix];
+        r += v.x;
+        im += v.y;
+    }
+    if (C128) {
+        double2* o = static_cast<double2*>(out) + q;
+        double2 cur = *o;
+        cur.x += r;
+        cur.y += im;
+        *o = cur;
+    } else {
+        float2* o = static_cast<float2*>(out) + q;
+        float2 cur = *o;
+        cur.x += static_cast<float>(r);
+        cur.y += static_cast<float>(im);
+        *o = cur;
+    }
+}
+
+// ------------------------------------------------------------------ Ein
+__global__ void k_ein(const float* __restrict__ pos, int64_t n, float R00, float R01, float R02,
+                      float R10, float R11, float R12, float R20, float R21, float R22,
+                      float T0, float T1, float T2, const float2* __restrict__ E0, int NyB,
+                      int NzB, double du, double dv, double uc, double vc,
+                      float2* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    // lab y and z (float32, separate roundings); e1 = +y, e2 = +z for a +x beam
+    const float ly = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(x, R10), __fmul_rn(y, R11)),
+                                         __fmul_rn(z, R12)), T1);
+    const float lz = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(x, R20), __fmul_rn(y, R21)),
+                                         __fmul_rn(z, R22)), T2);
+    (void)R00; (void)R01; (void)R02; (void)T0;
+    const float iu = __fadd_rn(__fdiv_rn(ly, static_cast<float>(du)), static_cast<float>(uc));
+    const float iv = __fadd_rn(__fdiv_rn(lz, static_cast<float>(dv)), static_cast<float>(vc));
+    float2 res = make_float2(0.f, 0.f);
+    if (iu >= 0.f && iu <= static_cast<float>(NyB - 1) && iv >= 0.f &&
+        iv <= static_cast<float>(NzB - 1)) {
+        const int i0 = static_cast<int>(floorf(iu));
+        const int j0 = static_cast<int>(floorf(iv));
+        const int i1 = min(i0 + 1, NyB - 1);
+        const int j1 = min(j0 + 1, NzB - 1);
+        const float fu = iu - static_cast<float>(i0);
+        const float fv = iv - static_cast<float>(j0);
+        const float w00 = __fmul_rn(1.f - fu, 1.f - fv);
+        const float w01 = __fmul_rn(1.f - fu, fv);
+        const float w10 = __fmul_rn(fu, 1.f - fv);
+        const float w11 = __fmul_rn(fu, fv);
+        const float2 a = E0[i0 * NzB + j0], b = E0[i0 * NzB + j1];
+        const float2 c = E0[i1 * NzB + j0], d = E0[i1 * NzB + j1];
+        res.x = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a.x, w00), __fmul_rn(b.x, w01)),
+                                    __fmul_rn(c.x, w10)), __fmul_rn(d.x, w11));
+        res.y = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a.y, w00), __fmul_rn(b.y, w01)),
+                                    __fmul_rn(c.y, w10)), __fmul_rn(d.y, w11));
+    }
+    out[i] = res;
+}
+
+template <int PIX, bool SECOND>
+hipError_t launch_sum(const SumArgs& A, int pix_blocks, int n_slices, int n_frames,
+                      hipStream_t st) {
+    dim3 grid(pix_blocks, n_slices, n_frames);
+    hipLaunchKernelGGL((k_sum<PIX, SECOND>), grid, dim3(SUM_THREADS), 0, st, A);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace mosaic
+
+using namespace mosaic;
+
+extern "C" int mosaic_kin_workspace(int64_t atom_count, int n_species, const int64_t* counts,
+                                    int n_frames, int Ny, int Nz, size_t* bytes) {
+    if (!bytes || atom_count < 0 || n_species < 1 || n_species > MAX_SPECIES || !counts ||
+        n_frames < 1 || Ny < 1 || Nz < 1 || atom_count >= (int64_t(1) << 32))
+        return MOSAIC_ERR_ARGS;
+    int64_t tot = 0;
+    for (int s = 0; s < n_species; ++s) {
+        if (counts[s] < 0) return MOSAIC_ERR_ARGS;
+        tot += counts[s];
+    }
+    if (tot != atom_count) return MOSAIC_ERR_ARGS;
+    Layout L;
+    const int rc = make_layout(nullptr, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    *bytes = L.total;
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
+                                const float* amp, int n_species, const int64_t* counts, double k,
+                                int n_frames, int Ny, int Nz, void* workspace,
+                                size_t workspace_bytes, double* max_tile_radius_wl,
+                                mosaic_stream_t stream) {
+    size_t need = 0;
+    int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
+    if (atom_count > 0 && (!pos || !species)) return MOSAIC_ERR_ARGS;
+    if (!(k > 0.0) || !max_tile_radius_wl) return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Layout L;
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    *max_tile_radius_wl = 0.0;
+    if (atom_count == 0) return MOSAIC_OK;
+
+    const int nb = std::min(BBOX_MAX_BLOCKS, blocks_for(atom_count, BBOX_THREADS));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(BBOX_THREADS), 0, st, pos, atom_count,
+                       L.bbox_blocks);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, L.bbox_blocks, nb, L.bbox,
+                       L.radius_bits);
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(atom_count, 256)), dim3(256), 0, st, pos,
+                       species, atom_count, L.bbox, L.keys_a, L.vals_a);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    size_t sb = L.sort_bytes;
+    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
+                                               L.vals_b, static_cast<unsigned int>(atom_count),
+                                               0, 64, st));
+    SpeciesSplit split;
+    split.n_species = n_species;
+    split.atom_start[0] = 0;
+    split.tile_start[0] = 0;
+    for (int s = 0; s < n_species; ++s) {
+        split.atom_start[s + 1] = split.atom_start[s] + counts[s];
+        split.tile_start[s + 1] =
+            split.tile_start[s] + static_cast<int>((counts[s] + TILE - 1) / TILE);
+    }
+    const double K = k * INV_2PI_HI;
+    if (L.n_tiles > 0)
+        hipLaunchKernelGGL(k_tiles, dim3(L.n_tiles), dim3(TILE), 0, st, pos,
+                           reinterpret_cast<const float2*>(amp), L.vals_b, split, K, L.tiles,
+                           L.atoms, L.radius_bits);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    unsigned int bits = 0;
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(&bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
+    float rf;
+    std::memcpy(&rf, &bits, sizeof(rf));
+    *max_tile_radius_wl = static_cast<double>(rf);
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_kin_pixels(int Ny, int Nz, const double* coords, double k,
+                                 void* workspace, size_t workspace_bytes, double* rho_min,
+                                 mosaic_stream_t stream) {
+    if (Ny < 1 || Nz < 1 || !coords || !(k > 0.0) || !workspace || !rho_min)
+        return MOSAIC_ERR_ARGS;
+    const int npix = Ny * Nz;
+    Carver c(workspace);
+    double* theta0 = c.take<double>(npix);
+    double* rho0 = c.take<double>(npix);
+    double* qcut = c.take<double>(npix);
+    const int pb = blocks_for(npix, PRO_THREADS);
+    double* rmb = c.take<double>(pb);
+    double* rm = c.take<double>(1);
+    if (c.off > workspace_bytes) return MOSAIC_ERR_WORKSPACE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_pixels, dim3(pb), dim3(PRO_THREADS), 0, st, coords, coords + npix,
+                       coords + 2 * (int64_t)npix, Ny, Nz, k, theta0, rho0, qcut, rmb);
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, st, rmb, pb, rm);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(rho_min, rm, sizeof(double), hipMemcpyDeviceToHost, st));
+    MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* counts,
+                              const mosaic_species_t* species_tab, const mosaic_kin_opts_t* opts,
+                              int n_frames, const float* rot, const float* trans, int Ny, int Nz,
+                              const double* coords, void* out, int out_is_c128,
+                              double max_tile_radius_wl, double rho_min, void* workspace,
+                              size_t workspace_bytes, mosaic_stream_t stream) {
+    size_t need = 0;
+    int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
+    if (!species_tab || !opts || !rot || !trans || !coords || !out) return MOSAIC_ERR_ARGS;
+    if (!(opts->k > 0.0) || !(max_tile_radius_wl >= 0.0)) return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Layout L;
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    const int npix = Ny * Nz;
+    const double K = opts->k * INV_2PI_HI;
+    if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
+
+    // error bounds of the (tile, pixel) split: the tile radius against the
+    // nearest pixel distance (tiles move rigidly with the sample pose)
+    const double rad_m = max_tile_radius_wl / K;
+    if (!(rho_min > 0.0) || rad_m / rho_min > 1e-6) return MOSAIC_ERR_UNSUPPORTED;
+    const double third = K * rad_m * rad_m * rad_m / (rho_min * rho_min);
+    if (third > 1e-7) return MOSAIC_ERR_UNSUPPORTED;
+    const double second = max_tile_radius_wl * max_tile_radius_wl / (2.0 * K * rho_min);
+    const bool use_second = second > 1e-8;
+
+    std::vector<FrameRec> fr(n_frames);
+    for (int f = 0; f < n_frames; ++f) {
+        for (int q = 0; q < 9; ++q) fr[f].R[q] = static_cast<double>(rot[9 * f + q]);
+        for (int d = 0; d < 3; ++d) fr[f].T[d] = static_cast<double>(trans[3 * f + d]) / 1e10;
+        for (int d = 0; d < 3; ++d) fr[f].xs[d] = fr[f].R[d];
+        fr[f].pad = 0.0;
+    }
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
+                                    hipMemcpyHostToDevice, st));
+
+    SumArgs A;
+    A.atoms = L.atoms;
+    A.tiles = L.tiles;
+    A.frames = L.frames;
+    A.cx = coords;
+    A.cy = coords + npix;
+    A.cz = coords + 2 * (int64_t)npix;
+    A.theta0 = L.theta0;
+    A.rho0 = L.rho0;
+    A.qcut = L.qcut;
+    A.partial = L.partial;
+    A.n_tiles = L.n_tiles;
+    A.tiles_per_slice = L.tiles_per_slice;
+    A.n_slices = L.n_slices;
+    A.npix = npix;
+    A.K = K;
+    A.k = opts->k;
+    A.pol_deg = opts->pol_deg;
+    for (int d = 0; d < 3; ++d) A.pol_e[d] = opts->pol_e[d];
+    A.remove_forward = opts->remove_forward ? 1 : 0;
+    A.apply_pol = opts->apply_pol ? 1 : 0;
+    A.apply_decay = opts->apply_decay ? 1 : 0;
+    A.n_species = n_species;
+    for (int s = 0; s < n_species; ++s) {
+        const mosaic_species_t& sp = species_tab[s];
+        for (int m = 0; m < 5; ++m) {
+            A.spc[s].a[m] = sp.wk[m];
+            A.spc[s].b[m] = sp.wk[6 + m];
+        }
+        A.spc[s].c = sp.wk[5];
+        A.spc[s].f0z = sp.f0_zero;
+        A.spc[s].fp = sp.fp;
+        A.spc[s].fpp = sp.fpp;
+    }
+    const int ppt = pix_per_thread(npix);
+    hipError_t e;
+    if (ppt == 8)
+        e = use_second ? launch_sum<8, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
+                       : launch_sum<8, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
+    else
+        e = use_second ? launch_sum<4, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
+                       : launch_sum<4, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
+    if (e != hipSuccess) return MOSAIC_ERR_HIP;
+    const int64_t nq = static_cast<int64_t>(npix) * n_frames;
+    if (out_is_c128)
+        hipLaunchKernelGGL(k_reduce<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, L.partial,
+                           L.n_slices, npix, n_frames, out);
+    else
+        hipLaunchKernelGGL(k_reduce<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st,
+                           L.partial, L.n_slices, npix, n_frames, out);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const float* rot,
+                                   const float* trans, const float* E0, int NyB, int NzB,
+                                   double du, double dv, double uc, double vc, float* amp_out,
+                                   mosaic_stream_t stream) {
+    if (atom_count < 0 || !rot || !trans || NyB < 1 || NzB < 1) return MOSAIC_ERR_ARGS;
+    if (atom_count == 0) return MOSAIC_OK;
+    if (!pos || !E0 || !amp_out || !(du > 0.0) || !(dv > 0.0)) return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_ein, dim3(blocks_for(atom_count, 256)), dim3(256), 0, st, pos,
+                       atom_count, rot[0], rot[1], rot[2], rot[3], rot[4], rot[5], rot[6], rot[7],
+                       rot[8], trans[0], trans[1], trans[2],
+                       reinterpret_cast<const float2*>(E0), NyB, NzB, du, dv, uc, vc,
+                       reinterpret_cast<float2*>(amp_out));
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}

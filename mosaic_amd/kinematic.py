@@ -1,0 +1,180 @@
+"""Operator-level kinematic sum: torch device buffers in, HIP kernels out.
+
+This is the seam the reference calls ``dispatch`` (Beam.py:9394-9496): the
+caller owns every buffer, the output is accumulated into, and a decline
+(``MosaicDeclined``) writes nothing.  Device memory and streams come from
+PyTorch; all arithmetic runs in ``libmosaic_amd.so``.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import KinOpts, Species, check
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(int(s.cuda_stream))
+
+
+def _ptr(t):
+    return ctypes.c_void_p(int(t.data_ptr())) if t is not None else ctypes.c_void_p(0)
+
+
+def species_table(consts):
+    """ctypes array of mosaic_species_t from tables.species_constants()."""
+    S = consts["wk"].shape[0]
+    arr = (Species * S)()
+    for s in range(S):
+        for m in range(11):
+            arr[s].wk[m] = float(consts["wk"][s, m])
+        arr[s].f0_zero = float(consts["f0z"][s])
+        arr[s].fp = float(consts["fp"][s])
+        arr[s].fpp = float(consts["fpp"][s])
+    return arr
+
+
+def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
+             apply_decay=False):
+    o = KinOpts()
+    o.k = float(k)
+    o.remove_forward = int(bool(remove_forward))
+    o.apply_pol = int(bool(apply_pol))
+    o.apply_decay = int(bool(apply_decay))
+    o.pad_ = 0
+    o.pol_deg = float(pol[0])
+    for d in range(3):
+        o.pol_e[d] = float(pol[1 + d])
+    return o
+
+
+def _counts_arr(counts):
+    c = np.ascontiguousarray(np.asarray(counts, dtype=np.int64))
+    return c, c.ctypes.data_as(ctypes.c_void_p)
+
+
+def workspace_bytes(n_atoms, counts, n_frames, Ny, Nz):
+    c, cp_ = _counts_arr(counts)
+    out = ctypes.c_size_t(0)
+    check(_lib.lib().mosaic_kin_workspace(int(n_atoms), int(c.shape[0]), cp_, int(n_frames),
+                                          int(Ny), int(Nz), ctypes.byref(out)),
+          "mosaic_kin_workspace")
+    return int(out.value)
+
+
+class StagedSample:
+    """A sample staged into tiles inside a caller-owned workspace."""
+
+    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, radius_wl):
+        self.n_atoms = int(n_atoms)
+        self.counts = np.asarray(counts, dtype=np.int64)
+        self.n_frames = int(n_frames)
+        self.Ny, self.Nz = int(Ny), int(Nz)
+        self.workspace = workspace
+        self.radius_wl = float(radius_wl)
+
+
+def ensure_workspace(ws, nbytes, device):
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+    return ws
+
+
+def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=None):
+    """Stage (N,3) float32 Angstrom sample-frame positions on the device.
+
+    ``codes`` are uint8 species codes on the device, ``counts[s]`` the number
+    of atoms with code s (host), ``amp`` an optional (N,) complex64 device
+    tensor of incident amplitudes.
+    """
+    if not pos.is_cuda:
+        raise ValueError("positions must be a device tensor")
+    if pos.dtype != torch.float32 or pos.dim() != 2 or pos.shape[1] != 3:
+        raise ValueError("positions must be (N, 3) float32")
+    pos = pos.contiguous()
+    codes = codes.contiguous()
+    n = int(pos.shape[0])
+    if codes.dtype != torch.uint8 or codes.shape[0] != n:
+        raise ValueError("species codes must be (N,) uint8")
+    if amp is not None:
+        if amp.dtype != torch.complex64 or amp.shape[0] != n:
+            raise ValueError("amplitudes must be (N,) complex64")
+        amp = torch.view_as_real(amp.contiguous())
+    c, cp_ = _counts_arr(counts)
+    if int(c.sum()) != n:
+        raise ValueError("species counts do not add up to the atom count")
+    nbytes = workspace_bytes(n, c, n_frames, Ny, Nz)
+    ws = ensure_workspace(workspace, nbytes, pos.device)
+    rad = ctypes.c_double(0.0)
+    check(_lib.lib().mosaic_kin_stage(n, _ptr(pos), _ptr(codes), _ptr(amp), int(c.shape[0]),
+                                      cp_, float(k), int(n_frames), int(Ny), int(Nz), _ptr(ws),
+                                      int(ws.numel()), ctypes.byref(rad),
+                                      _stream_handle(stream)),
+          "mosaic_kin_stage")
+    return StagedSample(n, c, n_frames, Ny, Nz, ws, rad.value)
+
+
+def kin_pixels(staged, coords, k, stream=None):
+    """Pixel prologue into the staged sample's workspace; returns rho_min [m]."""
+    npix = staged.Ny * staged.Nz
+    if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
+        raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
+    coords = coords.contiguous()
+    rm = ctypes.c_double(0.0)
+    check(_lib.lib().mosaic_kin_pixels(staged.Ny, staged.Nz, _ptr(coords), float(k),
+                                       _ptr(staged.workspace), int(staged.workspace.numel()),
+                                       ctypes.byref(rm), _stream_handle(stream)),
+          "mosaic_kin_pixels")
+    staged.rho_min = rm.value
+    staged.pixels_key = (int(coords.data_ptr()), float(k))
+    return rm.value
+
+
+def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
+    """Accumulate the kinematic field of a staged sample into ``out``.
+
+    rot (F,3,3) and trans (F,3) are float32 host arrays (sample -> lab,
+    Angstrom); coords a (3, Ny*Nz) float64 device tensor in metres; out a
+    (F, Nz, Ny) complex64 or complex128 device tensor.
+    """
+    F = staged.n_frames
+    rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(F, 3, 3))
+    trans = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(F, 3))
+    npix = staged.Ny * staged.Nz
+    if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
+        raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
+    coords = coords.contiguous()
+    if out.dtype not in (torch.complex64, torch.complex128) or out.numel() != F * npix:
+        raise ValueError("out must hold (frames, Nz, Ny) complex64/complex128")
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    if getattr(staged, "pixels_key", None) != (int(coords.data_ptr()), float(opts.k)):
+        kin_pixels(staged, coords, opts.k, stream)
+    c, cp_ = _counts_arr(staged.counts)
+    check(_lib.lib().mosaic_kin_sum(
+        staged.n_atoms, int(c.shape[0]), cp_, ctypes.cast(species_tab, ctypes.c_void_p),
+        ctypes.byref(opts), F, rot.ctypes.data_as(ctypes.c_void_p),
+        trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, _ptr(coords), _ptr(out),
+        int(out.dtype == torch.complex128), staged.radius_wl, staged.rho_min,
+        _ptr(staged.workspace),
+        int(staged.workspace.numel()), _stream_handle(stream)), "mosaic_kin_sum")
+    return out
+
+
+def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
+    """Incident amplitude per atom sampled from the beam grid E0 (NyB, NzB)."""
+    n = int(pos.shape[0])
+    if out is None:
+        out = torch.empty(n, dtype=torch.complex64, device=pos.device)
+    rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(3, 3))
+    trans = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(3))
+    E0 = E0.contiguous()
+    NyB, NzB = int(E0.shape[0]), int(E0.shape[1])
+    check(_lib.lib().mosaic_ein_bilinear(
+        n, _ptr(pos.contiguous()), rot.ctypes.data_as(ctypes.c_void_p),
+        trans.ctypes.data_as(ctypes.c_void_p), _ptr(torch.view_as_real(E0)), NyB, NzB,
+        float(du), float(dv), float(uc), float(vc), _ptr(torch.view_as_real(out)),
+        _stream_handle(stream)), "mosaic_ein_bilinear")
+    return out

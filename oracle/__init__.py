@@ -1,0 +1,16 @@
+"""CPU oracle for the kinematic scattering sum and the propagation path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``mosaic_amd`` imports this package:
+only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg use it, as the checker.
+
+Contents:
+  kinematic.py   -- NumPy restatement of compute_scattering_cffi
+                    (Beam.py:663-816) in float64 or long double, plus the
+                    ctypes binding of the C restatement (kin_sum.c) and of
+                    the reference's own C kernel built into oracle/_ref/.
+  propagation.py -- NumPy restatement of the angular-spectrum propagator,
+                    the pad rule and the thin lens (Beam.py:4622-4900,
+                    7945-8022; Optics.py:49-130, 513-639).
+  closed_forms.py -- analytic results used to pin the restatements.
+"""
