@@ -128,6 +128,44 @@ int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const float* rot /
                         int NyB, int NzB, double du, double dv, double uc,
                         double vc, float* amp_out, mosaic_stream_t stream);
 
+/* ----------------------------------------------------------- propagation */
+
+/* Thin lens exp(-i k r^2 / 2f) with optional CRL absorption
+ * (_apply_thin_lens_box, Optics.py:49-130). */
+typedef struct {
+    double focal_length_m;  /* f [m] of the whole stack */
+    double parab_coef;      /* mu N / (2 R_apex) [1/m^2]; 0 = no parabolic absorption */
+    float uniform_amp;      /* exp(-N t / sigma); 1 = no uniform absorption */
+    float pad_;
+} mosaic_lens_t;
+
+/* Workspace bytes for one angular-spectrum step on an (Nz2, Ny2) padded grid
+ * (the padded field plus the FFT work area). */
+int mosaic_asp_workspace(int Ny2, int Nz2, size_t* bytes);
+
+/*
+ * One angular-spectrum propagation step over z [m] of the (Nz, Ny) complex64
+ * device field `field_in` (pitches dy along columns, dz along rows, metres):
+ * optional thin lens (NULL = none) applied first, pad to (Nz2, Ny2) (edge
+ * replication when pad_edge, else the constant pad_re + i pad_im), FFT,
+ * transfer function, inverse FFT, centre crop into `field_out` (Nz, Ny).
+ * Replaces one pass of _angular_spectrum_propagate_gpu/_cpu
+ * (Beam.py:7829-8022) with prop_mul_kernel / prop_mul_cpu
+ * (Beam.py:4739-4900); the pad size and sub-stepping rules stay with the
+ * caller (_choose_optimal_pad / _alias_free_step, Beam.py:4632-4736).
+ * field_in and field_out may alias.
+ */
+int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, double dz,
+                         double wavelength, double z, int Ny2, int Nz2, int pad_edge,
+                         float pad_re, float pad_im, const mosaic_lens_t* lens /* host */,
+                         float* field_out, void* workspace, size_t workspace_bytes,
+                         mosaic_stream_t stream);
+
+/* Thin lens in place on the (Nz, Ny) device field, pitches dx (columns), dy
+ * (rows) in metres (_apply_thin_lens_box, Optics.py:49-130). */
+int mosaic_lens_apply(float* field, int Ny, int Nz, double dx, double dy, double wavelength,
+                      const mosaic_lens_t* lens /* host */, mosaic_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

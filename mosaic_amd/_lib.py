@@ -29,6 +29,9 @@ EXPORTS = (
     "mosaic_kin_pixels",
     "mosaic_kin_sum",
     "mosaic_ein_bilinear",
+    "mosaic_asp_workspace",
+    "mosaic_asp_propagate",
+    "mosaic_lens_apply",
 )
 
 
@@ -52,6 +55,11 @@ class KinOpts(ctypes.Structure):
                 ("pol_e", ctypes.c_double * 3)]
 
 
+class Lens(ctypes.Structure):
+    _fields_ = [("focal_length_m", ctypes.c_double), ("parab_coef", ctypes.c_double),
+                ("uniform_amp", ctypes.c_float), ("pad_", ctypes.c_float)]
+
+
 _LIB = None
 
 _vp, _i, _i64, _d, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -64,6 +72,10 @@ _SIGS = {
     "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _d, _d,
                        _vp, _sz, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
+    "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
+    "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
+                             ctypes.c_float, _vp, _vp, _vp, _sz, _vp],
+    "mosaic_lens_apply": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
 }
 
 

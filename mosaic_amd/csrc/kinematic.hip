@@ -567,6 +567,9 @@ __global__ void k_ein(const float* __restrict__ pos, int64_t n, float R00, float
                       float T0, float T1, float T2, const float2* __restrict__ E0, int NyB,
                       int NzB, double du, double dv, double uc, double vc,
                       float2* __restrict__ out) {
+    // Same float32 roundings, in the same order, as the reference's NumPy
+    // sampler: no multiply-add contraction here.
+#pragma clang fp contract(off)
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
