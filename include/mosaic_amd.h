@@ -47,7 +47,7 @@ typedef struct {
     int remove_forward;     /* subtract f0(0)+f'+if'' below the pixel's Q cut */
     int apply_pol;          /* scale by sqrt(P) */
     int apply_decay;        /* scale by R0 / r */
-    int pad_;
+    int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
 } mosaic_kin_opts_t;
@@ -72,25 +72,32 @@ int mosaic_kin_workspace(int64_t atom_count, int n_species,
  *   pos      device (N, 3) float32, sample frame, Angstrom
  *   species  device (N,) uint8 species codes (< n_species)
  *   amp      device (N,) complex64 incident amplitude, or NULL for 1
- *   max_tile_radius_wl  host out: largest tile radius in wavelengths
+ *   stage_stats  host out [2]: largest tile radius in wavelengths, largest
+ *                tile-origin distance from the sample origin in metres
+ * The call synchronizes the stream to return stage_stats.
  */
 int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
                      const float* amp, int n_species,
                      const int64_t* species_counts /* host */, double k,
                      int n_frames, int Ny, int Nz, void* workspace,
-                     size_t workspace_bytes, double* max_tile_radius_wl,
+                     size_t workspace_bytes, double* stage_stats,
                      mosaic_stream_t stream);
 
 /*
- * Pixel prologue of the kinematic sum, once per detector geometry: writes
- * per-pixel rho0 = |T_p|, the phase frac(rho0 / lambda) in double-double and
- * the forward-removal Q cut (Beam.py:718-746) into the workspace, and
- * returns the smallest pixel distance `rho_min` [m] (host out; the call
- * synchronizes the stream for it).  Must precede mosaic_kin_sum on the same
- * workspace.  `coords` is device (3, Ny*Nz) float64 metres.
+ * Pixel prologue of the kinematic sum, once per detector geometry, species
+ * table and flags: writes into the head of the workspace, per pixel,
+ * rho0 = |T_p|, frac(rho0 / lambda) in double-double, the forward-removal
+ * Q cut (Beam.py:718-746), the unit vector and 1/(2 rho0), and per species
+ * the factor (f0 + f' + i f'') r_e sqrt(P) in the pixel's direction.
+ * Returns pixel_geom (host out [3]): the smallest pixel distance [m] and the
+ * largest first and second differences of the pixel direction along a row;
+ * the call synchronizes the stream for them.  Must precede mosaic_kin_sum
+ * on the same workspace.  `coords` is device (3, Ny*Nz) float64 metres.
  */
-int mosaic_kin_pixels(int Ny, int Nz, const double* coords, double k, void* workspace,
-                      size_t workspace_bytes, double* rho_min, mosaic_stream_t stream);
+int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host */,
+                      const mosaic_kin_opts_t* opts /* host */, int Ny, int Nz,
+                      const double* coords, void* workspace, size_t workspace_bytes,
+                      double* pixel_geom, mosaic_stream_t stream);
 
 /*
  * Kinematic sum of a staged sample for `n_frames` sample poses
@@ -101,9 +108,16 @@ int mosaic_kin_pixels(int Ny, int Nz, const double* coords, double k, void* work
  * mosaic_fast launch (Beam.py:8246-8632, 9336-9391): same per-pair
  * semantics, with form factor, polarization and decay evaluated once per
  * (tile, pixel) and the phase split into an FP64 (tile, pixel) origin term
- * and an FP32 per-atom offset.  Stream-ordered, no host synchronization.
+ * and an FP32 per-atom offset.  Two kernels: per-tile factors (exact
+ * distance per tile and pixel, optional second-order offset term) and, in
+ * the far field, a row recurrence (per-pixel factors, far-field origin phase,
+ * one sincos per atom per KP pixels); opts->path = 0 picks the recurrence
+ * whenever its error bounds hold.  `stage_stats` and `pixel_geom` are the
+ * outputs of mosaic_kin_stage and mosaic_kin_pixels; `path_used` (host out,
+ * may be NULL) reports 11/12 (per-tile, first/second order) or 24/28 (row
+ * recurrence over 4/8 pixels).  Stream-ordered, no host synchronization.
  * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
- * close for that split's error bound -- the reference's "decline"
+ * close for the split's error bound -- the reference's "decline"
  * (Beam.py:9466-9468).
  */
 int mosaic_kin_sum(int64_t atom_count, int n_species,
@@ -113,8 +127,28 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    int n_frames, const float* rot /* host */,
                    const float* trans /* host */, int Ny, int Nz,
                    const double* coords, void* out, int out_is_c128,
-                   double max_tile_radius_wl, double rho_min, void* workspace,
-                   size_t workspace_bytes, mosaic_stream_t stream);
+                   const double* stage_stats /* host [2] */,
+                   const double* pixel_geom /* host [3] */, void* workspace,
+                   size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
+
+/*
+ * Adjoint of mosaic_kin_sum with respect to the atoms' sample-frame
+ * positions: given grad_field = dL/dRe(A) + i dL/dIm(A) (device
+ * (n_frames, Nz, Ny) complex64) for a real loss L of the field A that
+ * mosaic_kin_sum produced from the same workspace, adds dL/du (device (N, 3)
+ * float32, per Angstrom, original atom order) into grad_pos.  Phases are
+ * recomputed, not stored.  No counterpart in the reference, which has no
+ * gradient path; it serves displacement-field fitting.  Declines
+ * (MOSAIC_ERR_UNSUPPORTED) outside the far-field bounds of the row
+ * recurrence.
+ */
+int mosaic_kin_adjoint(int64_t atom_count, int n_species,
+                       const int64_t* species_counts /* host */, double k, int n_frames,
+                       const float* rot /* host */, const float* trans /* host */, int Ny,
+                       int Nz, const double* stage_stats /* host [2] */,
+                       const double* pixel_geom /* host [3] */, const float* grad_field,
+                       float* grad_pos, void* workspace, size_t workspace_bytes,
+                       mosaic_stream_t stream);
 
 /*
  * Incident amplitude per atom by bilinear sampling of the beam's entrance

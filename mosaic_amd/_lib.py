@@ -28,6 +28,7 @@ EXPORTS = (
     "mosaic_kin_stage",
     "mosaic_kin_pixels",
     "mosaic_kin_sum",
+    "mosaic_kin_adjoint",
     "mosaic_ein_bilinear",
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
@@ -51,7 +52,7 @@ class Species(ctypes.Structure):
 class KinOpts(ctypes.Structure):
     _fields_ = [("k", ctypes.c_double), ("remove_forward", ctypes.c_int),
                 ("apply_pol", ctypes.c_int), ("apply_decay", ctypes.c_int),
-                ("pad_", ctypes.c_int), ("pol_deg", ctypes.c_double),
+                ("path", ctypes.c_int), ("pol_deg", ctypes.c_double),
                 ("pol_e", ctypes.c_double * 3)]
 
 
@@ -66,11 +67,12 @@ _vp, _i, _i64, _d, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.
                           ctypes.c_size_t)
 _SIGS = {
     "mosaic_kin_workspace": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
-    "mosaic_kin_stage": [_i64, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz,
-                         ctypes.POINTER(_d), _vp],
-    "mosaic_kin_pixels": [_i, _i, _vp, _d, _vp, _sz, ctypes.POINTER(_d), _vp],
-    "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _d, _d,
-                       _vp, _sz, _vp],
+    "mosaic_kin_stage": [_i64, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz, _vp, _vp],
+    "mosaic_kin_pixels": [_i, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp, _vp],
+    "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _vp, _vp,
+                       _vp, _sz, _vp, _vp],
+    "mosaic_kin_adjoint": [_i64, _i, _vp, _d, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
+                           _sz, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,

@@ -50,6 +50,8 @@ class beam:
         self._q = tables.Q_E
         self._hq = self._h / self._q
         self._workspace = None
+        # kinematic kernel selection: "auto", "tile" or "rows" (tests pin it)
+        self._kin_path = "auto"
 
     # -------------------------------------------------------------- tables
     parse_f0_db_all = staticmethod(lambda database_name="f0_WaasKirf.dat": tables.parse_f0_db_all())
@@ -193,7 +195,7 @@ class beam:
         consts = tables.species_constants(labels, self._energy)
         spc_tab = K.species_table(consts)
         opts = K.kin_opts(k_val, remove_forward, apply_polarization, self._polarization_state(),
-                          spherical_decay)
+                          spherical_decay, path=self._kin_path)
         E0 = torch.from_numpy(self._beam_E0_map).to(dev)
         compute = torch.cuda.current_stream()
         copy = torch.cuda.Stream()
@@ -230,13 +232,13 @@ class beam:
                                  self._beam_uc, self._beam_vc)
             staged = K.stage(d_pos, d_cod, amp, counts, k_val, F, Ny, Nz,
                              workspace=self._workspace)
-            if (staged_prev is not None and staged.workspace is staged_prev.workspace
-                    and hasattr(staged_prev, "pixels_key")):
+            if staged_prev is not None and staged.workspace is staged_prev.workspace:
                 # pixel tables sit at the head of the workspace and survive staging
                 staged.pixels_key = staged_prev.pixels_key
-                staged.rho_min = staged_prev.rho_min
+                staged.pixel_geom = staged_prev.pixel_geom
             self._workspace = staged.workspace
             K.kin_sum(staged, spc_tab, opts, rot, trans, coords, out)
+            self._last_path = staged.path_used
             staged_prev = staged
             del hold
         del staged_prev
@@ -321,3 +323,49 @@ class beam:
                               (rot, trans), remove_forward, apply_polarization, spherical_decay,
                               out_dtype=torch.complex128)
         return out.to(out_dtype)
+
+    # --------------------------------------------------------- propagation
+    def free_space_propagator(self, step_max=None, pad_factor=1.0, padding_mode="edge",
+                              pad_constant=0.0):
+        """Callback ``(F, dy, dz, z, lens=None) -> F`` for optics.apply_stack.
+
+        F is a (Nz, Ny) complex64 device tensor; the reference's closure
+        (Beam.py:8062-8072) with the automatic alias-free sub-stepping.
+        """
+        from . import propagation as P
+        wl = self._wavelength
+
+        def _propagate_free_space(F, dym, dzm, z, lens=None):
+            if not isinstance(F, torch.Tensor):
+                F = torch.as_tensor(np.ascontiguousarray(F, dtype=np.complex64),
+                                    device=torch.device("cuda", torch.cuda.current_device()))
+            return P.propagate(F.to(torch.complex64).contiguous(), dym, dzm, wl, z,
+                               step_max=step_max, pad_factor=pad_factor,
+                               padding_mode=padding_mode, pad_constant=pad_constant, lens=lens)
+
+        _propagate_free_space.accepts_lens = True
+        return _propagate_free_space
+
+    def wavefield_propagation(self, detector, optics, use_gpu=True, step_max=None,
+                              pad_factor=1.0, padding_mode="edge", pad_constant=0.0,
+                              save_field=True):
+        """Propagate the detector field through an optics stack (Beam.py:8024-8095)."""
+        if not use_gpu:
+            raise NotImplementedError("the CPU propagation path is not part of this build")
+        _require_gpu()
+        dy, dz = np.asarray(detector.pixel_size, dtype=np.float64) * 1e-10
+        E = detector.pixel_values
+        if E is None:
+            raise ValueError("detector.pixel_values is None. Run atomic_direct_interaction() "
+                             "first to populate the detector field before calling "
+                             "wavefield_propagation().")
+        if self._wavelength is None:
+            raise ValueError("Beam wavelength is not set. Call beam.create_beam() first.")
+        prop = self.free_space_propagator(step_max, pad_factor, padding_mode, pad_constant)
+        E = optics.apply_stack(field=E, dx=dy, dy=dz, wavelength=self._wavelength,
+                               propagate_free_space=prop, use_gpu=True)
+        if save_field is True:
+            detector.input_pixel_values(E.astype(np.complex64))
+        else:
+            return E.astype(np.complex64)
+

@@ -51,7 +51,7 @@ constexpr double PI = 3.14159265358979323846;
 
 struct AtomRec {      // 32 B, read whole by one scalar load
     float sx, sy, sz, s2;
-    float wr, wi, pad0, pad1;
+    float wr, wi, pad0, pad1;   // pad0 carries the original atom index (bits)
 };
 
 struct TileRec {      // 32 B
@@ -67,12 +67,34 @@ struct FrameRec {     // 128 B
     double pad;
 };
 
+// Per-pixel tables written by the pixel prologue; they sit at the head of the
+// workspace and depend only on the pixel count and the number of species.
+struct PixelTables {
+    double* theta0;       // frac(rho0 / lambda)
+    double* rho0;         // |T_p|
+    double* qcut;         // forward-removal cut [1/m]
+    double4* u0h;         // (T_p / rho0, 1 / (2 rho0))
+    float2* gtab;         // [species][pixel] (f0+f'+if'') r_e sqrt(P), origin direction
+    double* stat_blocks;  // per prologue block: rho_min, max |du|, max |d2u|
+    double* stats;        // reduced: rho_min, max |du|, max |d2u|
+};
+
+struct SpeciesConst {
+    float a[5], b[5], c, f0z, fp, fpp;
+};
+
+struct PixOpts {
+    double k;
+    double pol_deg;
+    double pol_e[3];
+    int remove_forward;
+    int apply_pol;
+    int n_species;
+    SpeciesConst spc[MAX_SPECIES];
+};
+
 struct Layout {
-    double* theta0;
-    double* rho0;
-    double* qcut;
-    double* rho_min_blocks;
-    double* rho_min;
+    PixelTables px;
     TileRec* tiles;
     AtomRec* atoms;
     uint64_t* keys_a;
@@ -99,6 +121,16 @@ constexpr int BBOX_MAX_BLOCKS = 1024;
 
 int pix_per_thread(int npix) { return npix >= 4 * SUM_THREADS * 64 ? 8 : 4; }
 
+void take_pixel_tables(Carver& c, int npix, int n_species, PixelTables& P) {
+    P.theta0 = c.take<double>(npix);
+    P.rho0 = c.take<double>(npix);
+    P.qcut = c.take<double>(npix);
+    P.u0h = c.take<double4>(npix);
+    P.gtab = c.take<float2>(static_cast<size_t>(n_species) * npix);
+    P.stat_blocks = c.take<double>(3 * static_cast<size_t>(blocks_for(npix, PRO_THREADS)));
+    P.stats = c.take<double>(3);
+}
+
 int count_tiles(int n_species, const int64_t* counts) {
     int64_t t = 0;
     for (int s = 0; s < n_species; ++s) t += (counts[s] + TILE - 1) / TILE;
@@ -109,11 +141,7 @@ int count_tiles(int n_species, const int64_t* counts) {
 int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
                 int n_frames, int npix, Layout& L) {
     Carver c(base);
-    L.theta0 = c.take<double>(npix);
-    L.rho0 = c.take<double>(npix);
-    L.qcut = c.take<double>(npix);
-    L.rho_min_blocks = c.take<double>(blocks_for(npix, PRO_THREADS));
-    L.rho_min = c.take<double>(1);
+    take_pixel_tables(c, npix, n_species, L.px);
     L.n_tiles = count_tiles(n_species, counts);
     L.tiles = c.take<TileRec>(std::max(1, L.n_tiles));
     L.atoms = c.take<AtomRec>(static_cast<size_t>(std::max(1, L.n_tiles)) * TILE);
@@ -131,7 +159,7 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.sort_tmp = c.take<char>(sort_bytes);
     L.bbox_blocks = c.take<double>(6 * BBOX_MAX_BLOCKS);
     L.bbox = c.take<double>(6);
-    L.radius_bits = c.take<unsigned int>(1);
+    L.radius_bits = c.take<unsigned int>(2);
     L.frames = c.take<FrameRec>(std::max(1, n_frames));
     const int ppt = pix_per_thread(npix);
     L.pix_blocks = blocks_for(npix, SUM_THREADS * ppt);
@@ -180,16 +208,35 @@ __device__ double block_max(double v, double* lds) {
 }
 
 // ------------------------------------------------------------- pixel prologue
-// Per pixel: rho0 = |T| and Theta0 = frac(K rho0) in double-double
-// (K = k / 2 pi), and the forward-removal cut-off (Beam.py:718-746).
+// Unit vector from the lab origin to pixel p.
+__device__ __forceinline__ void pixel_dir(const double* cx, const double* cy, const double* cz,
+                                          int p, double& ux, double& uy, double& uz) {
+    const double tx = cx[p], ty = cy[p], tz = cz[p];
+    const double r = sqrt(tx * tx + ty * ty + tz * tz);
+    const double inv = r > 0.0 ? 1.0 / r : 0.0;
+    ux = tx * inv;
+    uy = ty * inv;
+    uz = tz * inv;
+}
+
+// Per pixel:
+//  * rho0 = |T| and Theta0 = frac(K rho0) in double-double (K = k / 2 pi);
+//  * the forward-removal cut-off from the right/upper neighbours
+//    (Beam.py:718-746);
+//  * u0 = T / rho0 and 1 / (2 rho0), for the far-field expansion of the
+//    (tile, pixel) origin phase;
+//  * per species, G = (f0(Q) + f' + i f'' [- forward term]) r_e sqrt(P) in
+//    the direction u0 (Beam.py:748-807 with the atom at the lab origin);
+//  * block statistics: nearest pixel distance and the largest first and
+//    second differences of u0 between neighbours along a row.
 __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const double* __restrict__ cx, const double* __restrict__ cy,
-    const double* __restrict__ cz, int Ny, int Nz, double k, double* __restrict__ theta0,
-    double* __restrict__ rho0, double* __restrict__ qcut, double* __restrict__ rho_min_blocks) {
+    const double* __restrict__ cz, int Ny, int Nz, const PixOpts O, PixelTables T) {
     __shared__ double lds[PRO_THREADS / 64];
     const int npix = Ny * Nz;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    double rmin = 1e300;
+    const double k = O.k;
+    double rmin = 1e300, du1 = 0.0, du2 = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
         // |T|^2 in double-double
@@ -205,22 +252,19 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
         two_prod(r, r, rr, rre);
         const double resid = ((sq_hi - rr) - rre) + sq_lo;
         const double r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
-        // K = k / (2 pi) in double-double
         double K_hi, K_lo;
         two_prod(k, INV_2PI_HI, K_hi, K_lo);
         K_lo = fma(k, INV_2PI_LO, K_lo);
-        // Theta = K * rho in double-double, then fractional part
         double th, the;
         two_prod(K_hi, r, th, the);
         the += K_hi * r_lo + K_lo * r;
         const double n = floor(th);
         double f = (th - n) + the;
         f -= floor(f);
-        theta0[p] = f;
-        rho0[p] = r + r_lo;
+        T.theta0[p] = f;
+        T.rho0[p] = r + r_lo;
         rmin = r;
 
-        // Q cut from the right and upper neighbours (inner one at the edge)
         const int ix = p % Ny, iy = p / Ny;
         double ux = 0.0, uy = 0.0, uz = 0.0;
         if (r > 0.0) { ux = tx / r; uy = ty / r; uz = tz / r; }
@@ -235,18 +279,69 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
             cd = fmin(1.0, fmax(-1.0, cd));
             q2[t] = k * sqrt(fmax(0.0, 2.0 * (1.0 - cd)));
         }
-        qcut[p] = 0.5 * sqrt(q2[0] * q2[0] + q2[1] * q2[1]);
+        const double qcut = 0.5 * sqrt(q2[0] * q2[0] + q2[1] * q2[1]);
+        T.qcut[p] = qcut;
+        T.u0h[p] = make_double4(ux, uy, uz, r > 0.0 ? 0.5 / r : 0.0);
+
+        // form factor, polarization in the origin direction
+        const double omc = (tx > 0.0) ? (ty * ty + tz * tz) / (r * (r + tx)) : 1.0 - ux;
+        const double Q = k * sqrt(fmax(0.0, 2.0 * omc));
+        const double sq = Q * (0.25e-10 / PI);
+        const double ss = sq * sq;
+        double scale = R_E;
+        if (O.apply_pol) {
+            const double er = O.pol_e[0] * ux + O.pol_e[1] * uy + O.pol_e[2] * uz;
+            double P = 0.5 * (1.0 - O.pol_deg) * (1.0 + ux * ux) + O.pol_deg * (1.0 - er * er);
+            P = fmin(1.0, fmax(0.0, P));
+            scale *= sqrt(P);
+        }
+        const bool cut = O.remove_forward && Q < qcut;
+        for (int sp = 0; sp < O.n_species; ++sp) {
+            const SpeciesConst& S = O.spc[sp];
+            double f0 = S.c;
+            for (int m = 0; m < 5; ++m) f0 += static_cast<double>(S.a[m]) * exp(-static_cast<double>(S.b[m]) * ss);
+            double sr = f0 + S.fp, si = S.fpp;
+            if (cut) { sr -= static_cast<double>(S.f0z) + S.fp; si -= S.fpp; }
+            T.gtab[static_cast<int64_t>(sp) * npix + p] =
+                make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
+        }
+
+        // direction differences along the row
+        if (ix + 1 < Ny) {
+            double vx, vy, vz;
+            pixel_dir(cx, cy, cz, p + 1, vx, vy, vz);
+            du1 = sqrt((vx - ux) * (vx - ux) + (vy - uy) * (vy - uy) + (vz - uz) * (vz - uz));
+            if (ix > 0) {
+                double wx, wy, wz;
+                pixel_dir(cx, cy, cz, p - 1, wx, wy, wz);
+                const double ax = vx - 2.0 * ux + wx, ay = vy - 2.0 * uy + wy,
+                             az = vz - 2.0 * uz + wz;
+                du2 = sqrt(ax * ax + ay * ay + az * az);
+            }
+        }
     }
     rmin = block_min(rmin, lds);
-    if (threadIdx.x == 0) rho_min_blocks[blockIdx.x] = rmin;
+    du1 = block_max(du1, lds);
+    du2 = block_max(du2, lds);
+    if (threadIdx.x == 0) {
+        T.stat_blocks[3 * blockIdx.x] = rmin;
+        T.stat_blocks[3 * blockIdx.x + 1] = du1;
+        T.stat_blocks[3 * blockIdx.x + 2] = du2;
+    }
 }
 
-__global__ void k_min_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
+__global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double lds[16];
-    double v = 1e300;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmin(v, in[i]);
-    v = block_min(v, lds);
-    if (threadIdx.x == 0) out[0] = v;
+    double a = 1e300, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a = fmin(a, in[3 * i]);
+        b = fmax(b, in[3 * i + 1]);
+        c = fmax(c, in[3 * i + 2]);
+    }
+    a = block_min(a, lds);
+    b = block_max(b, lds);
+    c = block_max(c, lds);
+    if (threadIdx.x == 0) { out[0] = a; out[1] = b; out[2] = c; }
 }
 
 // ------------------------------------------------------------------- staging
@@ -285,7 +380,7 @@ __global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __
         b = block_max(b, lds);
         if (threadIdx.x == 0) { bbox[d] = a; bbox[3 + d] = b; }
     }
-    if (threadIdx.x == 0) radius_bits[0] = 0u;
+    if (threadIdx.x == 0) { radius_bits[0] = 0u; radius_bits[1] = 0u; }
 }
 
 __device__ __forceinline__ uint64_t spread3(uint64_t x) {
@@ -354,6 +449,7 @@ __global__ __launch_bounds__(TILE) void k_tiles(
         rec.sy = static_cast<float>((r[1] - o[1]) * K);
         rec.sz = static_cast<float>((r[2] - o[2]) * K);
         rec.s2 = rec.sx * rec.sx + rec.sy * rec.sy + rec.sz * rec.sz;
+        rec.pad0 = __uint_as_float(idx);     // original atom index (adjoint scatter)
         if (amp) {
             const float2 w = amp[idx];
             rec.wr = w.x;
@@ -374,13 +470,12 @@ __global__ __launch_bounds__(TILE) void k_tiles(
         tr.count = count;
         tiles[t] = tr;
         atomicMax(radius_bits, __float_as_uint(static_cast<float>(rmax)));
+        const float on = static_cast<float>(sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+        atomicMax(radius_bits + 1, __float_as_uint(on * 1.0000002f));
     }
 }
 
 // ---------------------------------------------------------------------- sum
-struct SpeciesConst {
-    float a[5], b[5], c, f0z, fp, fpp;
-};
 
 struct SumArgs {
     const AtomRec* atoms;
@@ -532,6 +627,356 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum(const SumArgs A) {
         if (valid[i]) out[pix[i]] = make_double2(acc_r[i], acc_i[i]);
 }
 
+// Recurrence form of the sum, used when every tile origin is close to the lab
+// origin compared with the pixel distance (far field).  One lane owns KP
+// consecutive pixels of a detector row.  Across those pixels the direction
+// v = R^T (x_hat - u_hat) is a straight line in the pixel index to within the
+// row curvature (the host bounds it), so per atom
+//     e^{2 pi i v_m . sigma} = e^{2 pi i v_0 . sigma} (e^{2 pi i g . sigma})^m
+// and one FP32 sincos plus a short Taylor series for the small step phasor
+// replace KP sincos pairs.  Per (tile, pixel) the origin phase is the
+// far-field expansion
+//     Theta = Theta0 + K (X_t - u0 . O + (|O|^2 - (u0 . O)^2) / (2 rho0)),
+// exact to third order in |O| / rho0, and the factor G comes from the
+// per-pixel table.
+struct RecArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float2* gtab;
+    double2* partial;
+    int n_tiles;
+    int tiles_per_slice;
+    int n_slices;
+    int Ny;
+    int Nz;
+    int npix;
+    int nbx;          // lane blocks per detector row
+    double K;         // 1 / lambda
+};
+
+template <int KP>
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+    const int lanes = A.Nz * A.nbx;
+    const int lb = blockIdx.x * SUM_THREADS + threadIdx.x;
+    const bool lane_ok = lb < lanes;
+    const int lbc = lane_ok ? lb : 0;
+    const int iz = lbc / A.nbx;
+    const int iy0 = (lbc % A.nbx) * KP;
+    const int nv = min(KP, A.Ny - iy0);           // valid pixels of this lane (>= 1)
+    const int pbase = iz * A.Ny + iy0;
+
+    // FP64 accumulators live in LDS: the atom loop keeps only the FP32 tile
+    // sums in registers, which doubles the waves a SIMD can hold.
+    __shared__ double2 acc[KP][SUM_THREADS];
+#pragma unroll
+    for (int m = 0; m < KP; ++m) acc[m][threadIdx.x] = make_double2(0.0, 0.0);
+
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    const int m_last = nv - 1, m_a = (nv - 1) / 2, m_b = nv / 2;
+
+    for (int t = t0; t < t1; ++t) {
+        const TileRec tr = A.tiles[t];
+        // Re-read the per-pixel tables every tile instead of holding 40 B per
+        // pixel in registers for the whole loop: tying the index to the tile
+        // record (count <= 256, so the shift is 0) keeps the loads inside the
+        // loop, and occupancy is worth more than the reloads.
+        const int pb = pbase + (tr.count >> 16);
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+
+        // v in the sample frame at pixel offset m of this lane
+        auto vdir = [&](int m, double& sx, double& sy, double& sz) {
+            const double4 u = A.u0h[pb + m];
+            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+            const double h2 = 2.0 * u.w;                     // 1 / rho0
+            const double wx = 1.0 - u.x + (Ox - uo * u.x) * h2;
+            const double wy = -u.y + (Oy - uo * u.y) * h2;
+            const double wz = -u.z + (Oz - uo * u.z) * h2;
+            sx = R00 * wx + R10 * wy + R20 * wz;
+            sy = R01 * wx + R11 * wy + R21 * wz;
+            sz = R02 * wx + R12 * wy + R22 * wz;
+        };
+        double a0x, a0y, a0z, a1x, a1y, a1z, max_, may_, maz_, mbx_, mby_, mbz_;
+        vdir(0, a0x, a0y, a0z);
+        vdir(m_last, a1x, a1y, a1z);
+        vdir(m_a, max_, may_, maz_);
+        vdir(m_b, mbx_, mby_, mbz_);
+        const double inv_n = m_last > 0 ? 1.0 / m_last : 0.0;
+        // chord through the end pixels, lowered by half its gap to the row's
+        // middle (minimax line for a quadratic)
+        const double gapx = 0.5 * (a0x + a1x) - 0.5 * (max_ + mbx_);
+        const double gapy = 0.5 * (a0y + a1y) - 0.5 * (may_ + mby_);
+        const double gapz = 0.5 * (a0z + a1z) - 0.5 * (maz_ + mbz_);
+        const float v0x = static_cast<float>(a0x - 0.5 * gapx);
+        const float v0y = static_cast<float>(a0y - 0.5 * gapy);
+        const float v0z = static_cast<float>(a0z - 0.5 * gapz);
+        const float gx = static_cast<float>((a1x - a0x) * inv_n);
+        const float gy = static_cast<float>((a1y - a0y) * inv_n);
+        const float gz = static_cast<float>((a1z - a0z) * inv_n);
+
+        float S_r[KP], S_i[KP];
+#pragma unroll
+        for (int m = 0; m < KP; ++m) { S_r[m] = 0.f; S_i[m] = 0.f; }
+        const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
+        const int cnt = tr.count;
+        constexpr float TWO_PI_F = 6.2831853071795865f;
+        AtomRec nxt = at[0];
+        for (int j = 0; j < cnt; ++j) {
+            const AtomRec a = nxt;                 // next record in flight meanwhile
+            if (j + 1 < cnt) nxt = at[j + 1];
+            const float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
+            const float d = fmaf(gz, a.sz, fmaf(gy, a.sy, gx * a.sx));
+            const float r = __builtin_amdgcn_fractf(psi);
+            const float sn = __builtin_amdgcn_sinf(r);
+            const float cs = __builtin_amdgcn_cosf(r);
+            float er = fmaf(a.wr, cs, -a.wi * sn);
+            float ei = fmaf(a.wr, sn, a.wi * cs);
+            const float x = d * TWO_PI_F;
+            const float x2 = x * x;
+            const float ar = fmaf(x2, fmaf(x2, 0.041666668f, -0.5f), 1.0f);
+            const float ai = x * fmaf(x2, -0.16666667f, 1.0f);
+#pragma unroll
+            for (int m = 0; m < KP; ++m) {
+                S_r[m] += er;
+                S_i[m] += ei;
+                if (m + 1 < KP) {
+                    const float tr_ = fmaf(er, ar, -ei * ai);
+                    ei = fmaf(er, ai, ei * ar);
+                    er = tr_;
+                }
+            }
+        }
+        const float2* __restrict__ G = A.gtab + static_cast<int64_t>(tr.species) * A.npix;
+#pragma unroll
+        for (int m = 0; m < KP; ++m) {
+            const int p = pb + min(m, m_last);
+            const double4 u = A.u0h[p];
+            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+            double th = A.theta0[p] + A.K * (Ox - uo + (O2 - uo * uo) * u.w);
+            th -= rint(th);
+            const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
+            const float sn = __builtin_amdgcn_sinf(thf);
+            const float cs = __builtin_amdgcn_cosf(thf);
+            const float2 g = G[p];
+            const float zr = g.x * cs - g.y * sn, zi = g.x * sn + g.y * cs;
+            double2 c = acc[m][threadIdx.x];
+            c.x += static_cast<double>(zr * S_r[m] - zi * S_i[m]);
+            c.y += static_cast<double>(zr * S_i[m] + zi * S_r[m]);
+            acc[m][threadIdx.x] = c;
+        }
+    }
+    if (!lane_ok) return;
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+#pragma unroll
+    for (int m = 0; m < KP; ++m)
+        if (m < nv) out[pbase + m] = acc[m][threadIdx.x];
+}
+
+template <int KP>
+hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t st) {
+    const int lanes = A.Nz * A.nbx;
+    dim3 grid(blocks_for(lanes, SUM_THREADS), n_slices, n_frames);
+    hipLaunchKernelGGL((k_sum_rec<KP>), grid, dim3(SUM_THREADS), 0, st, A);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ adjoint
+// Gradient of a real loss L(A) with respect to the sample-frame displacement
+// of every atom, given g = dL/dRe(A) + i dL/dIm(A) per frame and pixel.  With
+// c_j the atom's term of A and phi its phase,
+//     dL/du_j = sum_{f,p} Re(conj(g) i c_j) dphi/du_j
+//             = -2 pi K' Im(w_j Z_j),   Z_j = sum_{f,p} v B e^{2 pi i v.sigma_j},
+//     B = conj(g) G e^{2 pi i Theta},  K' = 1e-10 / lambda (per Angstrom),
+// where v = R^T (x_hat - u_hat) is also the phase gradient direction.  One
+// workgroup owns one tile (its 256 atoms arrive as wave-uniform scalar loads);
+// lanes walk the detector in KP-pixel row blocks with the same line model as
+// the forward recurrence, so per atom a lane forms S0 = sum_m B_m e_m and
+// S1 = sum_m m B_m e_m and Z = v0 S0 + g S1.  The 6 floats of Z are summed over
+// the wave with DPP row shifts/broadcasts and accumulated per wave in FP64 in
+// LDS; the tile's atoms are written once at the end (no atomics, fixed order).
+struct AdjArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float2* gtab;
+    const float2* gin;     // [frame][pixel] dL/dA in the Re + i Im convention
+    float* grad;           // [atom][3], accumulated into
+    int n_frames;
+    int Ny;
+    int Nz;
+    int npix;
+    int nbx;
+    double K;
+};
+
+// Sum over the 64 lanes of a wave; the total lands in lane 63.
+__device__ __forceinline__ float wave_sum_to_63(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xa, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xc, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+template <int KP>
+__global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
+    __shared__ double acc[SUM_THREADS / 64][TILE][6];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < (SUM_THREADS / 64) * TILE * 6; i += SUM_THREADS)
+        (&acc[0][0][0])[i] = 0.0;
+    __syncthreads();
+    const TileRec tr = A.tiles[t];
+    const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
+    const int cnt = tr.count;
+    const float2* __restrict__ G = A.gtab + static_cast<int64_t>(tr.species) * A.npix;
+    const int lanes = A.Nz * A.nbx;
+    const int passes = (lanes + SUM_THREADS - 1) / SUM_THREADS;
+    constexpr float TWO_PI_F = 6.2831853071795865f;
+
+    for (int f = 0; f < A.n_frames; ++f) {
+        const FrameRec& F = A.frames[f];
+        const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+        const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+        const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const float2* __restrict__ gin = A.gin + static_cast<int64_t>(f) * A.npix;
+        for (int pass = 0; pass < passes; ++pass) {
+            const int lb = pass * SUM_THREADS + tid;
+            const bool lane_ok = lb < lanes;
+            const int lbc = lane_ok ? lb : 0;
+            const int iz = lbc / A.nbx;
+            const int iy0 = (lbc % A.nbx) * KP;
+            const int nv = min(KP, A.Ny - iy0);
+            const int pbase = iz * A.Ny + iy0;
+            const int m_last = nv - 1, m_a = (nv - 1) / 2, m_b = nv / 2;
+            auto vdir = [&](int m, double& sx, double& sy, double& sz) {
+                const double4 u = A.u0h[pbase + m];
+                const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+                const double h2 = 2.0 * u.w;
+                const double wx = 1.0 - u.x + (Ox - uo * u.x) * h2;
+                const double wy = -u.y + (Oy - uo * u.y) * h2;
+                const double wz = -u.z + (Oz - uo * u.z) * h2;
+                sx = R00 * wx + R10 * wy + R20 * wz;
+                sy = R01 * wx + R11 * wy + R21 * wz;
+                sz = R02 * wx + R12 * wy + R22 * wz;
+            };
+            double a0x, a0y, a0z, a1x, a1y, a1z, max_, may_, maz_, mbx_, mby_, mbz_;
+            vdir(0, a0x, a0y, a0z);
+            vdir(m_last, a1x, a1y, a1z);
+            vdir(m_a, max_, may_, maz_);
+            vdir(m_b, mbx_, mby_, mbz_);
+            const double inv_n = m_last > 0 ? 1.0 / m_last : 0.0;
+            const double gapx = 0.5 * (a0x + a1x) - 0.5 * (max_ + mbx_);
+            const double gapy = 0.5 * (a0y + a1y) - 0.5 * (may_ + mby_);
+            const double gapz = 0.5 * (a0z + a1z) - 0.5 * (maz_ + mbz_);
+            const float v0x = static_cast<float>(a0x - 0.5 * gapx);
+            const float v0y = static_cast<float>(a0y - 0.5 * gapy);
+            const float v0z = static_cast<float>(a0z - 0.5 * gapz);
+            const float gx = static_cast<float>((a1x - a0x) * inv_n);
+            const float gy = static_cast<float>((a1y - a0y) * inv_n);
+            const float gz = static_cast<float>((a1z - a0z) * inv_n);
+            // B_m = conj(g) G e^{2 pi i Theta}; zero for pixels past the row end
+            float br[KP], bi[KP];
+#pragma unroll
+            for (int m = 0; m < KP; ++m) {
+                const int p = pbase + min(m, m_last);
+                const double4 u = A.u0h[p];
+                const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+                double th = A.theta0[p] + A.K * (Ox - uo + (O2 - uo * uo) * u.w);
+                th -= rint(th);
+                const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
+                const float sn = __builtin_amdgcn_sinf(thf);
+                const float cs = __builtin_amdgcn_cosf(thf);
+                const float2 gg = G[p];
+                const float zr = gg.x * cs - gg.y * sn, zi = gg.x * sn + gg.y * cs;
+                const float2 q = gin[p];                       // conj(q) * z
+                const bool live = lane_ok && m < nv;
+                br[m] = live ? q.x * zr + q.y * zi : 0.f;
+                bi[m] = live ? q.x * zi - q.y * zr : 0.f;
+            }
+            AtomRec nxt = at[0];
+            for (int j = 0; j < cnt; ++j) {
+                const AtomRec a = nxt;
+                if (j + 1 < cnt) nxt = at[j + 1];
+                const float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
+                const float d = fmaf(gz, a.sz, fmaf(gy, a.sy, gx * a.sx));
+                const float r = __builtin_amdgcn_fractf(psi);
+                float er = __builtin_amdgcn_cosf(r);
+                float ei = __builtin_amdgcn_sinf(r);
+                const float x = d * TWO_PI_F;
+                const float x2 = x * x;
+                const float ar = fmaf(x2, fmaf(x2, 0.041666668f, -0.5f), 1.0f);
+                const float ai = x * fmaf(x2, -0.16666667f, 1.0f);
+                float s0r = 0.f, s0i = 0.f, s1r = 0.f, s1i = 0.f;
+#pragma unroll
+                for (int m = 0; m < KP; ++m) {
+                    const float tr_ = fmaf(br[m], er, -bi[m] * ei);
+                    const float ti_ = fmaf(br[m], ei, bi[m] * er);
+                    s0r += tr_;
+                    s0i += ti_;
+                    if (m > 0) {
+                        s1r = fmaf(static_cast<float>(m), tr_, s1r);
+                        s1i = fmaf(static_cast<float>(m), ti_, s1i);
+                    }
+                    if (m + 1 < KP) {
+                        const float nr = fmaf(er, ar, -ei * ai);
+                        ei = fmaf(er, ai, ei * ar);
+                        er = nr;
+                    }
+                }
+                float z[6];
+                z[0] = fmaf(v0x, s0r, gx * s1r);
+                z[1] = fmaf(v0x, s0i, gx * s1i);
+                z[2] = fmaf(v0y, s0r, gy * s1r);
+                z[3] = fmaf(v0y, s0i, gy * s1i);
+                z[4] = fmaf(v0z, s0r, gz * s1r);
+                z[5] = fmaf(v0z, s0i, gz * s1i);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const float tot = wave_sum_to_63(z[c]);
+                    if (lane == 0) acc[wave][j][c] += static_cast<double>(tot);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < cnt) {
+        const AtomRec a = at[tid];
+        double Z[6];
+        for (int c = 0; c < 6; ++c) {
+            Z[c] = 0.0;
+            for (int w = 0; w < SUM_THREADS / 64; ++w) Z[c] += acc[w][tid][c];
+        }
+        const double scale = -2.0 * PI * A.K * 1e-10;
+        const uint32_t idx = __float_as_uint(a.pad0);
+        for (int c = 0; c < 3; ++c) {
+            const double g = scale * (static_cast<double>(a.wr) * Z[2 * c + 1] +
+                                      static_cast<double>(a.wi) * Z[2 * c]);
+            A.grad[3 * static_cast<int64_t>(idx) + c] += static_cast<float>(g);
+        }
+    }
+}
+
 // Sum slice partials in slice order and add into the output field.
 template <bool C128>
 __global__ void k_reduce(const double2* __restrict__ partial, int n_slices, int npix,
@@ -638,19 +1083,20 @@ extern "C" int mosaic_kin_workspace(int64_t atom_count, int n_species, const int
 extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
                                 const float* amp, int n_species, const int64_t* counts, double k,
                                 int n_frames, int Ny, int Nz, void* workspace,
-                                size_t workspace_bytes, double* max_tile_radius_wl,
+                                size_t workspace_bytes, double* stage_stats,
                                 mosaic_stream_t stream) {
     size_t need = 0;
     int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
     if (rc != MOSAIC_OK) return rc;
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
     if (atom_count > 0 && (!pos || !species)) return MOSAIC_ERR_ARGS;
-    if (!(k > 0.0) || !max_tile_radius_wl) return MOSAIC_ERR_ARGS;
+    if (!(k > 0.0) || !stage_stats) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
     rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
     if (rc != MOSAIC_OK) return rc;
-    *max_tile_radius_wl = 0.0;
+    stage_stats[0] = 0.0;
+    stage_stats[1] = 0.0;
     if (atom_count == 0) return MOSAIC_OK;
 
     const int nb = std::min(BBOX_MAX_BLOCKS, blocks_for(atom_count, BBOX_THREADS));
@@ -680,120 +1126,187 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
                            reinterpret_cast<const float2*>(amp), L.vals_b, split, K, L.tiles,
                            L.atoms, L.radius_bits);
     MOSAIC_HIP_CHECK(hipGetLastError());
-    unsigned int bits = 0;
-    MOSAIC_HIP_CHECK(hipMemcpyAsync(&bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    unsigned int bits[2] = {0u, 0u};
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
     MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
-    float rf;
-    std::memcpy(&rf, &bits, sizeof(rf));
-    *max_tile_radius_wl = static_cast<double>(rf);
+    float rf[2];
+    std::memcpy(rf, bits, sizeof(rf));
+    stage_stats[0] = static_cast<double>(rf[0]);
+    stage_stats[1] = static_cast<double>(rf[1]);
     return MOSAIC_OK;
 }
 
-extern "C" int mosaic_kin_pixels(int Ny, int Nz, const double* coords, double k,
-                                 void* workspace, size_t workspace_bytes, double* rho_min,
-                                 mosaic_stream_t stream) {
-    if (Ny < 1 || Nz < 1 || !coords || !(k > 0.0) || !workspace || !rho_min)
+namespace {
+void fill_species(const mosaic_species_t* tab, int n, SpeciesConst* out) {
+    for (int s = 0; s < n; ++s) {
+        for (int m = 0; m < 5; ++m) {
+            out[s].a[m] = tab[s].wk[m];
+            out[s].b[m] = tab[s].wk[6 + m];
+        }
+        out[s].c = tab[s].wk[5];
+        out[s].f0z = tab[s].f0_zero;
+        out[s].fp = tab[s].fp;
+        out[s].fpp = tab[s].fpp;
+    }
+}
+}  // namespace
+
+extern "C" int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab,
+                                 const mosaic_kin_opts_t* opts, int Ny, int Nz,
+                                 const double* coords, void* workspace, size_t workspace_bytes,
+                                 double* pixel_geom, mosaic_stream_t stream) {
+    if (Ny < 1 || Nz < 1 || !coords || !opts || !(opts->k > 0.0) || !workspace ||
+        !pixel_geom || n_species < 1 || n_species > MAX_SPECIES || !species_tab)
         return MOSAIC_ERR_ARGS;
     const int npix = Ny * Nz;
     Carver c(workspace);
-    double* theta0 = c.take<double>(npix);
-    double* rho0 = c.take<double>(npix);
-    double* qcut = c.take<double>(npix);
-    const int pb = blocks_for(npix, PRO_THREADS);
-    double* rmb = c.take<double>(pb);
-    double* rm = c.take<double>(1);
+    PixelTables T;
+    take_pixel_tables(c, npix, n_species, T);
     if (c.off > workspace_bytes) return MOSAIC_ERR_WORKSPACE;
+    PixOpts O;
+    O.k = opts->k;
+    O.pol_deg = opts->pol_deg;
+    for (int d = 0; d < 3; ++d) O.pol_e[d] = opts->pol_e[d];
+    O.remove_forward = opts->remove_forward ? 1 : 0;
+    O.apply_pol = opts->apply_pol ? 1 : 0;
+    O.n_species = n_species;
+    fill_species(species_tab, n_species, O.spc);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    const int pb = blocks_for(npix, PRO_THREADS);
     hipLaunchKernelGGL(k_pixels, dim3(pb), dim3(PRO_THREADS), 0, st, coords, coords + npix,
-                       coords + 2 * (int64_t)npix, Ny, Nz, k, theta0, rho0, qcut, rmb);
-    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, st, rmb, pb, rm);
+                       coords + 2 * (int64_t)npix, Ny, Nz, O, T);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, st, T.stat_blocks, pb, T.stats);
     MOSAIC_HIP_CHECK(hipGetLastError());
-    MOSAIC_HIP_CHECK(hipMemcpyAsync(rho_min, rm, sizeof(double), hipMemcpyDeviceToHost, st));
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(pixel_geom, T.stats, 3 * sizeof(double),
+                                    hipMemcpyDeviceToHost, st));
     MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
     return MOSAIC_OK;
 }
+
+namespace {
+// Row-recurrence bound: the deviation of v from the fitted line over KP
+// pixels (second difference d2u), times the largest offset, in revolutions.
+double rec_curv_bound(int kp, double d2u, double radius_wl) {
+    const double h = 0.5 * (kp - 1);
+    return 0.5 * (h * h / 2.0) * d2u * radius_wl;
+}
+}  // namespace
 
 extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* counts,
                               const mosaic_species_t* species_tab, const mosaic_kin_opts_t* opts,
                               int n_frames, const float* rot, const float* trans, int Ny, int Nz,
                               const double* coords, void* out, int out_is_c128,
-                              double max_tile_radius_wl, double rho_min, void* workspace,
-                              size_t workspace_bytes, mosaic_stream_t stream) {
+                              const double* stage_stats, const double* pixel_geom,
+                              void* workspace, size_t workspace_bytes, int* path_used,
+                              mosaic_stream_t stream) {
     size_t need = 0;
     int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
     if (rc != MOSAIC_OK) return rc;
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
-    if (!species_tab || !opts || !rot || !trans || !coords || !out) return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(max_tile_radius_wl >= 0.0)) return MOSAIC_ERR_ARGS;
+    if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
+        !pixel_geom)
+        return MOSAIC_ERR_ARGS;
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 2)
+        return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
     rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
     if (rc != MOSAIC_OK) return rc;
     const int npix = Ny * Nz;
     const double K = opts->k * INV_2PI_HI;
+    if (path_used) *path_used = 0;
     if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
 
-    // error bounds of the (tile, pixel) split: the tile radius against the
-    // nearest pixel distance (tiles move rigidly with the sample pose)
-    const double rad_m = max_tile_radius_wl / K;
+    // Error bounds.  Per-tile path: tile radius against the nearest pixel
+    // distance (tiles move rigidly with the sample pose).
+    const double radius_wl = stage_stats[0];
+    const double rho_min = pixel_geom[0], du1 = pixel_geom[1], du2 = pixel_geom[2];
+    const double rad_m = radius_wl / K;
     if (!(rho_min > 0.0) || rad_m / rho_min > 1e-6) return MOSAIC_ERR_UNSUPPORTED;
     const double third = K * rad_m * rad_m * rad_m / (rho_min * rho_min);
     if (third > 1e-7) return MOSAIC_ERR_UNSUPPORTED;
-    const double second = max_tile_radius_wl * max_tile_radius_wl / (2.0 * K * rho_min);
+    const double second = radius_wl * radius_wl / (2.0 * K * rho_min);
     const bool use_second = second > 1e-8;
 
     std::vector<FrameRec> fr(n_frames);
+    double tmax = 0.0;
     for (int f = 0; f < n_frames; ++f) {
         for (int q = 0; q < 9; ++q) fr[f].R[q] = static_cast<double>(rot[9 * f + q]);
         for (int d = 0; d < 3; ++d) fr[f].T[d] = static_cast<double>(trans[3 * f + d]) / 1e10;
         for (int d = 0; d < 3; ++d) fr[f].xs[d] = fr[f].R[d];
         fr[f].pad = 0.0;
+        tmax = std::max(tmax, std::sqrt(fr[f].T[0] * fr[f].T[0] + fr[f].T[1] * fr[f].T[1] +
+                                        fr[f].T[2] * fr[f].T[2]));
     }
+    // Recurrence path: every tile origin within 1e-6 rad of the lab origin as
+    // seen from the nearest pixel (per-pixel form factor), the dropped third
+    // order of the origin phase below 1e-8 rev, the row line fit below 1e-6
+    // rev and the step phasor inside its Taylor range.
+    const double omax = stage_stats[1] + tmax;
+    const bool rec_geom = !use_second && omax / rho_min <= 1e-6 &&
+                          K * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
+                          6.2831853 * du1 * radius_wl <= 0.05;
+    int kp = 0;
+    if (rec_geom && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
+    else if (rec_geom && rec_curv_bound(4, du2, radius_wl) <= 1e-6) kp = 4;
+    int path = opts->path == 0 ? (kp ? 2 : 1) : opts->path;
+    if (path == 2 && kp == 0) return MOSAIC_ERR_UNSUPPORTED;
+
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
-
-    SumArgs A;
-    A.atoms = L.atoms;
-    A.tiles = L.tiles;
-    A.frames = L.frames;
-    A.cx = coords;
-    A.cy = coords + npix;
-    A.cz = coords + 2 * (int64_t)npix;
-    A.theta0 = L.theta0;
-    A.rho0 = L.rho0;
-    A.qcut = L.qcut;
-    A.partial = L.partial;
-    A.n_tiles = L.n_tiles;
-    A.tiles_per_slice = L.tiles_per_slice;
-    A.n_slices = L.n_slices;
-    A.npix = npix;
-    A.K = K;
-    A.k = opts->k;
-    A.pol_deg = opts->pol_deg;
-    for (int d = 0; d < 3; ++d) A.pol_e[d] = opts->pol_e[d];
-    A.remove_forward = opts->remove_forward ? 1 : 0;
-    A.apply_pol = opts->apply_pol ? 1 : 0;
-    A.apply_decay = opts->apply_decay ? 1 : 0;
-    A.n_species = n_species;
-    for (int s = 0; s < n_species; ++s) {
-        const mosaic_species_t& sp = species_tab[s];
-        for (int m = 0; m < 5; ++m) {
-            A.spc[s].a[m] = sp.wk[m];
-            A.spc[s].b[m] = sp.wk[6 + m];
-        }
-        A.spc[s].c = sp.wk[5];
-        A.spc[s].f0z = sp.f0_zero;
-        A.spc[s].fp = sp.fp;
-        A.spc[s].fpp = sp.fpp;
-    }
-    const int ppt = pix_per_thread(npix);
     hipError_t e;
-    if (ppt == 8)
-        e = use_second ? launch_sum<8, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
-                       : launch_sum<8, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
-    else
-        e = use_second ? launch_sum<4, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
-                       : launch_sum<4, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
+    if (path == 2) {
+        RecArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.u0h = L.px.u0h;
+        A.theta0 = L.px.theta0;
+        A.gtab = L.px.gtab;
+        A.partial = L.partial;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.Ny = Ny;
+        A.Nz = Nz;
+        A.npix = npix;
+        A.nbx = (Ny + kp - 1) / kp;
+        A.K = K;
+        e = kp == 8 ? launch_rec<8>(A, L.n_slices, n_frames, st)
+                    : launch_rec<4>(A, L.n_slices, n_frames, st);
+    } else {
+        SumArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.cx = coords;
+        A.cy = coords + npix;
+        A.cz = coords + 2 * (int64_t)npix;
+        A.theta0 = L.px.theta0;
+        A.rho0 = L.px.rho0;
+        A.qcut = L.px.qcut;
+        A.partial = L.partial;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.npix = npix;
+        A.K = K;
+        A.k = opts->k;
+        A.pol_deg = opts->pol_deg;
+        for (int d = 0; d < 3; ++d) A.pol_e[d] = opts->pol_e[d];
+        A.remove_forward = opts->remove_forward ? 1 : 0;
+        A.apply_pol = opts->apply_pol ? 1 : 0;
+        A.apply_decay = opts->apply_decay ? 1 : 0;
+        A.n_species = n_species;
+        fill_species(species_tab, n_species, A.spc);
+        const int ppt = pix_per_thread(npix);
+        if (ppt == 8)
+            e = use_second ? launch_sum<8, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
+                           : launch_sum<8, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
+        else
+            e = use_second ? launch_sum<4, true>(A, L.pix_blocks, L.n_slices, n_frames, st)
+                           : launch_sum<4, false>(A, L.pix_blocks, L.n_slices, n_frames, st);
+    }
     if (e != hipSuccess) return MOSAIC_ERR_HIP;
     const int64_t nq = static_cast<int64_t>(npix) * n_frames;
     if (out_is_c128)
@@ -803,6 +1316,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         hipLaunchKernelGGL(k_reduce<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st,
                            L.partial, L.n_slices, npix, n_frames, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
+    if (path_used) *path_used = path == 2 ? (kp == 8 ? 28 : 24) : (use_second ? 12 : 11);
     return MOSAIC_OK;
 }
 
@@ -822,3 +1336,72 @@ extern "C" int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const f
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }
+
+extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64_t* counts,
+                                  double k, int n_frames, const float* rot, const float* trans,
+                                  int Ny, int Nz, const double* stage_stats,
+                                  const double* pixel_geom,
+                                  const float* grad_field, float* grad_pos, void* workspace,
+                                  size_t workspace_bytes, mosaic_stream_t stream) {
+    size_t need = 0;
+    int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
+    if (!rot || !trans || !stage_stats || !pixel_geom || !grad_field || !grad_pos)
+        return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Layout L;
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
+    // The adjoint reads the staged sample and the pixel tables of the forward
+    // and uses the row line model: it declines where the forward recurrence
+    // would.
+    const double radius_wl = stage_stats[0];
+    const double rho_min = pixel_geom[0], du1 = pixel_geom[1], du2 = pixel_geom[2];
+    std::vector<FrameRec> fr(n_frames);
+    double tmax = 0.0;
+    for (int f = 0; f < n_frames; ++f) {
+        for (int q = 0; q < 9; ++q) fr[f].R[q] = static_cast<double>(rot[9 * f + q]);
+        for (int d = 0; d < 3; ++d) fr[f].T[d] = static_cast<double>(trans[3 * f + d]) / 1e10;
+        for (int d = 0; d < 3; ++d) fr[f].xs[d] = fr[f].R[d];
+        fr[f].pad = 0.0;
+        tmax = std::max(tmax, std::sqrt(fr[f].T[0] * fr[f].T[0] + fr[f].T[1] * fr[f].T[1] +
+                                        fr[f].T[2] * fr[f].T[2]));
+    }
+    const double Kk = k * INV_2PI_HI;
+    if (!(Kk > 0.0) || !(rho_min > 0.0)) return MOSAIC_ERR_ARGS;
+    const double second = radius_wl * radius_wl / (2.0 * Kk * rho_min);
+    const double omax = stage_stats[1] + tmax;
+    const bool ok = second <= 1e-8 && omax / rho_min <= 1e-6 &&
+                    Kk * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
+                    6.2831853 * du1 * radius_wl <= 0.05;
+    int kp = 0;
+    if (ok && rec_curv_bound(16, du2, radius_wl) <= 1e-6) kp = 16;
+    else if (ok && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
+    if (kp == 0) return MOSAIC_ERR_UNSUPPORTED;
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
+                                    hipMemcpyHostToDevice, st));
+    AdjArgs A;
+    A.atoms = L.atoms;
+    A.tiles = L.tiles;
+    A.frames = L.frames;
+    A.u0h = L.px.u0h;
+    A.theta0 = L.px.theta0;
+    A.gtab = L.px.gtab;
+    A.gin = reinterpret_cast<const float2*>(grad_field);
+    A.grad = grad_pos;
+    A.n_frames = n_frames;
+    A.Ny = Ny;
+    A.Nz = Nz;
+    A.npix = Ny * Nz;
+    A.nbx = (Ny + kp - 1) / kp;
+    A.K = Kk;
+    if (kp == 16)
+        hipLaunchKernelGGL(k_adjoint<16>, dim3(L.n_tiles), dim3(SUM_THREADS), 0, st, A);
+    else
+        hipLaunchKernelGGL(k_adjoint<8>, dim3(L.n_tiles), dim3(SUM_THREADS), 0, st, A);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+

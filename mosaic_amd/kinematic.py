@@ -36,14 +36,19 @@ def species_table(consts):
     return arr
 
 
+PATHS = {"auto": 0, "tile": 1, "rows": 2}
+
+
 def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
-             apply_decay=False):
+             apply_decay=False, path="auto"):
+    """mosaic_kin_opts_t; ``path`` is "auto", "tile" (per-tile factors) or
+    "rows" (row recurrence, declines when its bounds do not hold)."""
     o = KinOpts()
     o.k = float(k)
     o.remove_forward = int(bool(remove_forward))
     o.apply_pol = int(bool(apply_pol))
     o.apply_decay = int(bool(apply_decay))
-    o.pad_ = 0
+    o.path = PATHS[path]
     o.pol_deg = float(pol[0])
     for d in range(3):
         o.pol_e[d] = float(pol[1 + d])
@@ -67,13 +72,17 @@ def workspace_bytes(n_atoms, counts, n_frames, Ny, Nz):
 class StagedSample:
     """A sample staged into tiles inside a caller-owned workspace."""
 
-    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, radius_wl):
+    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, stats):
         self.n_atoms = int(n_atoms)
         self.counts = np.asarray(counts, dtype=np.int64)
         self.n_frames = int(n_frames)
         self.Ny, self.Nz = int(Ny), int(Nz)
         self.workspace = workspace
-        self.radius_wl = float(radius_wl)
+        self.stage_stats = (ctypes.c_double * 2)(*stats)
+        self.radius_wl = float(stats[0])
+        self.pixel_geom = None
+        self.pixels_key = None
+        self.path_used = None
 
 
 def ensure_workspace(ws, nbytes, device):
@@ -107,29 +116,34 @@ def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=N
         raise ValueError("species counts do not add up to the atom count")
     nbytes = workspace_bytes(n, c, n_frames, Ny, Nz)
     ws = ensure_workspace(workspace, nbytes, pos.device)
-    rad = ctypes.c_double(0.0)
+    stats = (ctypes.c_double * 2)()
     check(_lib.lib().mosaic_kin_stage(n, _ptr(pos), _ptr(codes), _ptr(amp), int(c.shape[0]),
                                       cp_, float(k), int(n_frames), int(Ny), int(Nz), _ptr(ws),
-                                      int(ws.numel()), ctypes.byref(rad),
-                                      _stream_handle(stream)),
+                                      int(ws.numel()), stats, _stream_handle(stream)),
           "mosaic_kin_stage")
-    return StagedSample(n, c, n_frames, Ny, Nz, ws, rad.value)
+    return StagedSample(n, c, n_frames, Ny, Nz, ws, list(stats))
 
 
-def kin_pixels(staged, coords, k, stream=None):
-    """Pixel prologue into the staged sample's workspace; returns rho_min [m]."""
+def kin_pixels(staged, species_tab, opts, coords, stream=None):
+    """Pixel prologue into the staged sample's workspace; returns
+    (rho_min [m], max row step of the pixel direction, max row second difference)."""
     npix = staged.Ny * staged.Nz
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
         raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
     coords = coords.contiguous()
-    rm = ctypes.c_double(0.0)
-    check(_lib.lib().mosaic_kin_pixels(staged.Ny, staged.Nz, _ptr(coords), float(k),
+    geom = (ctypes.c_double * 3)()
+    check(_lib.lib().mosaic_kin_pixels(len(species_tab), ctypes.cast(species_tab, ctypes.c_void_p),
+                                       ctypes.byref(opts), staged.Ny, staged.Nz, _ptr(coords),
                                        _ptr(staged.workspace), int(staged.workspace.numel()),
-                                       ctypes.byref(rm), _stream_handle(stream)),
+                                       geom, _stream_handle(stream)),
           "mosaic_kin_pixels")
-    staged.rho_min = rm.value
-    staged.pixels_key = (int(coords.data_ptr()), float(k))
-    return rm.value
+    staged.pixel_geom = geom
+    staged.pixels_key = _pixels_key(species_tab, opts, coords)
+    return tuple(geom)
+
+
+def _pixels_key(species_tab, opts, coords):
+    return (int(coords.data_ptr()), bytes(species_tab), bytes(opts))
 
 
 def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
@@ -137,7 +151,9 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
 
     rot (F,3,3) and trans (F,3) are float32 host arrays (sample -> lab,
     Angstrom); coords a (3, Ny*Nz) float64 device tensor in metres; out a
-    (F, Nz, Ny) complex64 or complex128 device tensor.
+    (F, Nz, Ny) complex64 or complex128 device tensor.  Runs the pixel
+    prologue first unless it already ran for these pixels, species and flags
+    on this workspace.
     """
     F = staged.n_frames
     rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(F, 3, 3))
@@ -150,16 +166,18 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
         raise ValueError("out must hold (frames, Nz, Ny) complex64/complex128")
     if not out.is_contiguous():
         raise ValueError("out must be contiguous")
-    if getattr(staged, "pixels_key", None) != (int(coords.data_ptr()), float(opts.k)):
-        kin_pixels(staged, coords, opts.k, stream)
+    if staged.pixels_key != _pixels_key(species_tab, opts, coords):
+        kin_pixels(staged, species_tab, opts, coords, stream)
     c, cp_ = _counts_arr(staged.counts)
+    used = ctypes.c_int(0)
     check(_lib.lib().mosaic_kin_sum(
         staged.n_atoms, int(c.shape[0]), cp_, ctypes.cast(species_tab, ctypes.c_void_p),
         ctypes.byref(opts), F, rot.ctypes.data_as(ctypes.c_void_p),
         trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, _ptr(coords), _ptr(out),
-        int(out.dtype == torch.complex128), staged.radius_wl, staged.rho_min,
-        _ptr(staged.workspace),
-        int(staged.workspace.numel()), _stream_handle(stream)), "mosaic_kin_sum")
+        int(out.dtype == torch.complex128), staged.stage_stats, staged.pixel_geom,
+        _ptr(staged.workspace), int(staged.workspace.numel()), ctypes.byref(used),
+        _stream_handle(stream)), "mosaic_kin_sum")
+    staged.path_used = used.value
     return out
 
 
@@ -178,3 +196,32 @@ def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
         float(du), float(dv), float(uc), float(vc), _ptr(torch.view_as_real(out)),
         _stream_handle(stream)), "mosaic_ein_bilinear")
     return out
+
+
+def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, stream=None):
+    """Add dL/du (per Angstrom, original atom order) into grad_pos (N, 3) float32.
+
+    ``grad_field`` is dL/dRe(A) + i dL/dIm(A) for the field the last
+    ``kin_sum`` on ``staged`` produced, (F, Nz, Ny) complex (cast to
+    complex64).  Raises MosaicDeclined outside the recurrence's bounds.
+    """
+    F = staged.n_frames
+    if staged.pixel_geom is None:
+        raise RuntimeError("kin_adjoint needs the pixel tables of a preceding kin_sum")
+    rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(F, 3, 3))
+    trans = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(F, 3))
+    g = grad_field.to(torch.complex64).contiguous()
+    if g.numel() != F * staged.Ny * staged.Nz:
+        raise ValueError("grad_field must hold (frames, Nz, Ny) values")
+    if grad_pos.dtype != torch.float32 or tuple(grad_pos.shape) != (staged.n_atoms, 3):
+        raise ValueError("grad_pos must be (N, 3) float32")
+    if not grad_pos.is_contiguous():
+        raise ValueError("grad_pos must be contiguous")
+    c, cp_ = _counts_arr(staged.counts)
+    check(_lib.lib().mosaic_kin_adjoint(
+        staged.n_atoms, int(c.shape[0]), cp_, float(k), F, rot.ctypes.data_as(ctypes.c_void_p),
+        trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, staged.stage_stats,
+        staged.pixel_geom, _ptr(torch.view_as_real(g)), _ptr(grad_pos),
+        _ptr(staged.workspace), int(staged.workspace.numel()), _stream_handle(stream)),
+        "mosaic_kin_adjoint")
+    return grad_pos

@@ -79,12 +79,13 @@ def perovskite_screw_bragg200(n=100, npix=512, blocks=(4, 4, 4), b=4.0, seed=0):
 
 
 def fcc_fault_scan111(cells=(50, 50, 100), npix=256, n_angles=64, half_range_deg=0.05,
-                      sigma=0.0, seed=0):
+                      sigma=0.0, seed=0, fault=True):
     a = 4.08
     lam = tables.wavelength_from_energy_ev(ENERGY_EV)
     th = S.bragg_angle(a / np.sqrt(3.0), lam)
     pos, _ = S.lattice(a, cells, S.FCC_BASIS)
-    pos = S.stacking_fault_shift(pos, a, plane_offset=0.0)
+    if fault:
+        pos = S.stacking_fault_shift(pos, a, plane_offset=0.0)
     R = S.orient_for_reflection((1, 1, 1), th)
     pos = (pos @ R.T).astype(np.float32)
     smp = S.sample([pos], [np.zeros(pos.shape[0], dtype=np.uint8)], ["Au"])
@@ -137,3 +138,33 @@ def oracle_inputs(w, frame_rot=None, frame_trans=None, amp=None, atom_subset=Non
                 cx=pc[0], cy=pc[1], cz=pc[2], k_val=2.0 * np.pi / bm.wavelength,
                 Ny=Ny, Nz=Nz, pol_deg=bm._polarization_state()[0],
                 pol_e=bm._polarization_state()[1:])
+
+
+def gaussian_bump_field(pos, amplitude=0.2, width=200.0, rng=None, centre=None, direction=None):
+    """Per-atom displacement u = amplitude exp(-|r - r0|^2 / (2 width^2)) e_hat (float32).
+
+    r0 and e_hat are drawn from ``rng`` when not given: r0 uniformly in the
+    middle half of the sample's bounding box, e_hat uniformly on the sphere.
+    """
+    from .sample import gaussian_bump
+    rng = np.random.default_rng(0) if rng is None else rng
+    lo, hi = pos.min(axis=0), pos.max(axis=0)
+    if centre is None:
+        centre = lo + (hi - lo) * (0.25 + 0.5 * rng.random(3))
+    if direction is None:
+        direction = rng.standard_normal(3)
+    return gaussian_bump(pos.astype(np.float64), centre, width, amplitude,
+                         direction).astype(np.float32)
+
+
+def fcc_strain_fit(cells=(63, 63, 63), npix=256, n_angles=16, half_range_deg=0.05, seed=0):
+    """The displacement-fitting workload: FCC Au, (111) rocking poses, a smooth
+    Gaussian strain bump (0.2 A, width 200 A) as the current field and a second
+    seeded bump as the target."""
+    w = fcc_fault_scan111(cells=cells, npix=npix, n_angles=n_angles,
+                          half_range_deg=half_range_deg, fault=False)
+    pos = w["sample"].load_chunk_positions(1)
+    w["u"] = gaussian_bump_field(pos, 0.2, 200.0, np.random.default_rng(seed))
+    w["u_target"] = gaussian_bump_field(pos, 0.2, 200.0, np.random.default_rng(seed + 1))
+    w["name"] = f"fcc_strain_fit_{cells[0]}x{cells[1]}x{cells[2]}_{npix}px_{n_angles}ang"
+    return w

@@ -24,8 +24,9 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def gpu_field(w, remove_forward=False, apply_pol=True, decay=False):
+def gpu_field(w, remove_forward=False, apply_pol=True, decay=False, path="auto"):
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    b._kin_path = path
     rot = np.asarray(st.rotation, dtype=np.float32)[None]
     tr = np.asarray(st.translation, dtype=np.float32)[None]
     out = b._kin_field(s, d.pixel_coordinates, d.shape, st, (rot, tr), remove_forward,
@@ -51,26 +52,30 @@ def report(name, A, R):
     return e, ei
 
 
+@pytest.mark.parametrize("path", ["tile", "rows"])
 @pytest.mark.parametrize("flags", [
     dict(apply_pol=True, decay=False, remove_forward=False),
     dict(apply_pol=False, decay=True, remove_forward=False),
     dict(apply_pol=True, decay=True, remove_forward=True),
 ])
-def test_fcc_small_vs_oracle(device, flags):
+def test_fcc_small_vs_oracle(device, flags, path):
     w = W.fcc_au_bragg111(n=8, npix=32, seed=3)
-    A = gpu_field(w, **flags)
+    A = gpu_field(w, path=path, **flags)
     R80 = oracle_field(w, "f80", **flags)
     R64 = oracle_field(w, "f64", **flags)
-    e80, _ = report(f"fcc8 {flags}", A, R80)
-    e64, _ = report(f"fcc8 {flags} vs f64", A, R64)
+    e80, _ = report(f"fcc8 {path} {flags}", A, R80)
+    e64, _ = report(f"fcc8 {path} {flags} vs f64", A, R64)
     assert e80 <= TOL and e64 <= TOL
 
 
-def test_perovskite_multispecies_chunks(device):
-    w = W.perovskite_screw_bragg200(n=8, npix=24, blocks=(2, 2, 2))
-    A = gpu_field(w)
+@pytest.mark.parametrize("path", ["tile", "rows"])
+def test_perovskite_multispecies_chunks(device, path):
+    # 24 pixels per row: the 8-pixel row blocks tile it exactly; 21 leaves a
+    # ragged block at the row end
+    w = W.perovskite_screw_bragg200(n=8, npix=21, blocks=(2, 2, 2))
+    A = gpu_field(w, path=path)
     R = oracle_field(w, "f80")
-    e, _ = report("perovskite8 3 species 8 chunks", A, R)
+    e, _ = report(f"perovskite8 {path} 3 species 8 chunks", A, R)
     assert e <= TOL
 
 
@@ -106,15 +111,17 @@ def test_ein_bilinear_bitexact(device):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-def test_scan_frames_match_single_poses(device):
+@pytest.mark.parametrize("path", ["tile", "rows"])
+def test_scan_frames_match_single_poses(device, path):
     w = W.fcc_fault_scan111(cells=(6, 6, 8), npix=24, n_angles=5, half_range_deg=0.5)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    b._kin_path = path
     frames = b.kinematic_scan(s, d, st, "phi", w["angles_deg"], apply_polarization=True,
                               out_dtype=torch.complex128).cpu().numpy()
     for i in (0, 2, 4):
         st.set_single_motor_value_absolute("phi", float(w["angles_deg"][i]))
         R = oracle_field(w, "f80")
-        e, _ = report(f"scan frame {i}", frames[i].ravel(), R)
+        e, _ = report(f"scan {path} frame {i}", frames[i].ravel(), R)
         assert e <= TOL
     st.set_single_motor_value_absolute("phi", 0.0)
 
@@ -146,9 +153,29 @@ def test_empty_and_ragged(device):
 
 
 def test_c1_full_vs_fp64_oracle(device):
-    """32,000 atoms x 128^2 pixels (5.2e8 pairs) against the FP64 restatement."""
+    """32,000 atoms x 128^2 pixels (5.2e8 pairs) against the FP64 restatement,
+    both kernels; the automatic choice is the row recurrence here."""
     w = W.fcc_au_bragg111(n=20, npix=128, seed=0)
-    A = gpu_field(w)
     R = oracle_field(w, "f64")
-    e, ei = report("C1 full vs f64", A, R)
-    assert e <= TOL and ei <= TOL
+    for path in ("tile", "rows", "auto"):
+        A = gpu_field(w, path=path)
+        e, ei = report(f"C1 full {path} vs f64", A, R)
+        assert e <= TOL and ei <= TOL
+
+
+def test_auto_path_choice(device):
+    """Far field picks the recurrence; a 5 cm detector with 2 mm pixels bends
+    the rows too much for it and falls back to per-tile factors."""
+    w = W.fcc_au_bragg111(n=4, npix=16)
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    b._kin_path = "auto"
+    gpu_field(w)
+    staged_path = b._last_path
+    assert staged_path in (24, 28)
+    d.create_detector((16, 16), (2.0e7, 2.0e7))
+    d.position_detector_absolute(5.0e8, np.rad2deg(2 * w["theta_b"]), 0.0)
+    A = gpu_field(w)
+    assert b._last_path in (11, 12)
+    R = oracle_field(w, "f80")
+    e, _ = report("near detector auto", A, R)
+    assert e <= TOL
