@@ -200,6 +200,55 @@ int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, doubl
 int mosaic_lens_apply(float* field, int Ny, int Nz, double dx, double dy, double wavelength,
                       const mosaic_lens_t* lens /* host */, mosaic_stream_t stream);
 
+/* --------------------------------------------------------------- surrogate */
+/* Kernels of the learned rocking-scan surrogate.  None has a counterpart in
+ * the reference, which has no learned model; they serve the surrogate's
+ * training and inference (mosaic_amd/surrogate.py).  Arrays are float32
+ * device memory; `batch` runs independent problems (ensemble members) at the
+ * given element strides. */
+
+/* C = alpha op(A) op(B) + beta C, BF16 MFMA products with FP32 accumulation;
+ * op(A)[m][k] = A[m*lda+k] (transA = 0) or A[k*lda+m]; op(B)[k][n] =
+ * B[k*ldb+n] (transB = 0) or B[n*ldb+k]. */
+int mosaic_sg_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda,
+                   long long strideA, const float* B, int ldb, long long strideB, float* C,
+                   int ldc, long long strideC, int batch, float alpha, float beta,
+                   mosaic_stream_t stream);
+
+/* Z += bias[col % bias_period] over (rows, cols); Y = SiLU(Z) (act 1) or Z (act 0). */
+int mosaic_sg_bias_act(float* Z, float* Y, const float* bias, long long rows, int cols,
+                       int bias_period, long long strideZ, long long strideBias, int batch,
+                       int act, mosaic_stream_t stream);
+
+/* dZ = dY * act'(Z) over n elements. */
+int mosaic_sg_act_bwd(const float* Z, const float* dY, float* dZ, long long n, int act,
+                      mosaic_stream_t stream);
+
+/* dbias[p] = sum of dZ over rows and over the columns c with c % period = p. */
+int mosaic_sg_bias_grad(const float* dZ, float* dbias, long long rows, int cols, int period,
+                        long long strideZ, long long strideBias, int batch,
+                        mosaic_stream_t stream);
+
+/* Pixel shuffle of a 2x2 stride-2 transposed convolution, channels last:
+ * G[(n H + h) W + w][(dy 2 + dx) Co + c] <-> Y[n][2h+dy][2w+dx][c]
+ * (backward = 0: G -> Y; 1: Y -> G). */
+int mosaic_sg_shuffle2(float* G, float* Y, int nimg, int H, int W, int Co, long long stride,
+                       int batch, int backward, mosaic_stream_t stream);
+
+/* loss[b] += mean((P_b - T)^2); dP_b = 2 (P_b - T) / n. */
+int mosaic_sg_mse(const float* P, const float* T, float* dP, float* loss, long long n,
+                  long long strideP, int batch, mosaic_stream_t stream);
+
+/* Adam step with decoupled weight decay (bias-corrected, 1-based step). */
+int mosaic_sg_adam(float* w, const float* g, float* m, float* v, long long n, float lr,
+                   float beta1, float beta2, float eps, int step, float weight_decay,
+                   mosaic_stream_t stream);
+
+/* Per frame: ensemble mean (frames, npix) and mean-over-pixels of the
+ * members' standard deviation (frames). */
+int mosaic_sg_ensemble_stats(const float* P, float* mean, float* spread, int members,
+                             int frames, int npix, long long strideP, mosaic_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

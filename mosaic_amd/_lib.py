@@ -33,6 +33,14 @@ EXPORTS = (
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
     "mosaic_lens_apply",
+    "mosaic_sg_gemm",
+    "mosaic_sg_bias_act",
+    "mosaic_sg_act_bwd",
+    "mosaic_sg_bias_grad",
+    "mosaic_sg_shuffle2",
+    "mosaic_sg_mse",
+    "mosaic_sg_adam",
+    "mosaic_sg_ensemble_stats",
 )
 
 
@@ -65,6 +73,7 @@ _LIB = None
 
 _vp, _i, _i64, _d, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
                           ctypes.c_size_t)
+_ll, _f = ctypes.c_longlong, ctypes.c_float
 _SIGS = {
     "mosaic_kin_workspace": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
     "mosaic_kin_stage": [_i64, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz, _vp, _vp],
@@ -78,6 +87,15 @@ _SIGS = {
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
                              ctypes.c_float, _vp, _vp, _vp, _sz, _vp],
     "mosaic_lens_apply": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
+    "mosaic_sg_gemm": [_i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _i, _f, _f,
+                       _vp],
+    "mosaic_sg_bias_act": [_vp, _vp, _vp, _ll, _i, _i, _ll, _ll, _i, _i, _vp],
+    "mosaic_sg_act_bwd": [_vp, _vp, _vp, _ll, _i, _vp],
+    "mosaic_sg_bias_grad": [_vp, _vp, _ll, _i, _i, _ll, _ll, _i, _vp],
+    "mosaic_sg_shuffle2": [_vp, _vp, _i, _i, _i, _i, _ll, _i, _i, _vp],
+    "mosaic_sg_mse": [_vp, _vp, _vp, _vp, _ll, _ll, _i, _vp],
+    "mosaic_sg_adam": [_vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _f, _vp],
+    "mosaic_sg_ensemble_stats": [_vp, _vp, _vp, _i, _i, _i, _ll, _vp],
 }
 
 
