@@ -42,6 +42,12 @@ def parse():
     ap.add_argument("--err-pixels", type=int, default=48)
     ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows"])
     ap.add_argument("--no-propagation", action="store_true")
+    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="run the surrogate workload (data generation, training, gated scan)")
+    ap.add_argument("--sg-images", type=int, default=20000)
+    ap.add_argument("--sg-epochs", type=int, default=40)
     return ap.parse_args()
 
 
@@ -164,6 +170,102 @@ def bench_propagation(steps, warmup, wavelength):
             "out_energy": float(torch.sum(torch.abs(out) ** 2).item())}
 
 
+def bench_scan(steps, warmup):
+    """Batched rocking scan: 1,000,000-atom FCC Au with a (111) stacking fault,
+    64 rotations in +-0.05 deg, 256^2 pixels, all poses in one launch."""
+    import torch
+    from mosaic_amd import kinematic as K
+    from mosaic_amd import tables
+    from mosaic_amd import workloads as W
+    w = W.fcc_fault_scan111(cells=(50, 50, 100), npix=256, n_angles=64)
+    b, d, st = w["beam"], w["detector"], w["stage"]
+    dev = torch.device("cuda", 0)
+    pos = torch.from_numpy(w["sample"].load_chunk_positions(1)).to(dev)
+    n = pos.shape[0]
+    codes = torch.zeros(n, dtype=torch.uint8, device=dev)
+    rots, trs = [], []
+    for a in w["angles_deg"]:
+        st.set_single_motor_value_absolute("phi", float(a))
+        rots.append(np.asarray(st.rotation, np.float32))
+        trs.append(np.asarray(st.translation, np.float32))
+    st.set_single_motor_value_absolute("phi", 0.0)
+    rot, tr = np.stack(rots), np.stack(trs)
+    F = rot.shape[0]
+    Ny, Nz = d.shape
+    k = 2 * np.pi / b.wavelength
+    spc = K.species_table(tables.species_constants(["Au"], b.energy))
+    opts = K.kin_opts(k, False, True, b._polarization_state(), False)
+    coords = torch.from_numpy(np.ascontiguousarray(d.pixel_coordinates / 1e10)).to(dev)
+    out = torch.zeros((F, Nz, Ny), dtype=torch.complex64, device=dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ws = [None]
+
+    def run():
+        out.zero_()
+        sg = K.stage(pos, codes, None, np.array([n]), k, F, Ny, Nz, workspace=ws[0])
+        ws[0] = sg.workspace
+        K.kin_sum(sg, spc, opts, rot, tr, coords, out)
+        return sg
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+        ev0.record()
+        sg = run()
+        ev1.record()
+        ev1.synchronize()
+        ts.append(ev0.elapsed_time(ev1))
+    ms = float(np.mean(ts))
+    pairs = float(n) * Ny * Nz * F
+    return {"workload": w["name"], "atoms": n, "frames": F, "pixels": [Ny, Nz],
+            "ms_per_scan": ms, "frames_per_s": F / (ms * 1e-3),
+            "pairs_per_s": pairs / (ms * 1e-3), "kernel_path": sg.path_used}
+
+
+def bench_adjoint(steps):
+    """Displacement gradient: 1,000,188-atom FCC Au (63^3 cells) with a Gaussian
+    strain bump, 16 poses x 256^2 pixels, L = sum (|A|^2 - I_t)^2."""
+    import torch
+    from mosaic_amd import workloads as W
+    from mosaic_amd.fitting import KinematicModel
+    w = W.fcc_strain_fit(cells=(63, 63, 63), npix=256, n_angles=16)
+    b, d, st = w["beam"], w["detector"], w["stage"]
+    dev = torch.device("cuda", 0)
+    pos = w["sample"].load_chunk_positions(1)
+    rots, trs = [], []
+    for a in w["angles_deg"]:
+        st.set_single_motor_value_absolute("phi", float(a))
+        rots.append(np.asarray(st.rotation, np.float32))
+        trs.append(np.asarray(st.translation, np.float32))
+    st.set_single_motor_value_absolute("phi", 0.0)
+    m = KinematicModel(pos, np.zeros(pos.shape[0], np.uint8), ["Au"], b, d, np.stack(rots),
+                       np.stack(trs))
+    with torch.no_grad():
+        It = (m.field(torch.from_numpy(w["u_target"]).to(dev)).abs() ** 2)
+    u = torch.from_numpy(w["u"]).to(dev).requires_grad_(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tf, tb = [], []
+    for i in range(steps + 1):
+        u.grad = None
+        ev[0].record()
+        A = m.field(u)
+        L = torch.sum((A.abs() ** 2 - It) ** 2)
+        ev[1].record()
+        L.backward()
+        ev[2].record()
+        ev[2].synchronize()
+        if i:
+            tf.append(ev[0].elapsed_time(ev[1]))
+            tb.append(ev[1].elapsed_time(ev[2]))
+    pairs = float(pos.shape[0]) * 16 * 256 * 256
+    return {"workload": w["name"], "atoms": int(pos.shape[0]), "poses": 16,
+            "forward_ms": float(np.mean(tf)), "backward_ms": float(np.mean(tb)),
+            "backward_pairs_per_s": pairs / (np.mean(tb) * 1e-3),
+            "grad_norm": float(torch.linalg.norm(u.grad).item())}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -254,6 +356,10 @@ def main():
     }
     if not args.no_propagation:
         res["propagation"] = bench_propagation(args.steps, args.warmup, b.wavelength)
+    if not args.no_scan:
+        res["scan"] = bench_scan(max(1, args.steps - 1), 1)
+    if not args.no_adjoint:
+        res["adjoint"] = bench_adjoint(1)
     if args.err_pixels > 0:
         res["rel_err_vs_fp64_oracle"] = accuracy(w, field, args.err_pixels)
         res["rel_err_pixels"] = args.err_pixels

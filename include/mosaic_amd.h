@@ -200,6 +200,16 @@ int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, doubl
 int mosaic_lens_apply(float* field, int Ny, int Nz, double dx, double dy, double wavelength,
                       const mosaic_lens_t* lens /* host */, mosaic_stream_t stream);
 
+/* Defected crystal for the surrogate's training images: base (N, 3) float32
+ * crystal-frame positions [A] plus one dislocation (isotropic elasticity),
+ * a hydrostatic strain and one stacking fault, rotated into the sample frame,
+ * into out (N, 3).  params (host, 32 floats): core[3], line[3], e1[3], e2[3],
+ * b_screw, b_edge, nu, strain, fault_normal[3], fault_shift[3], R[9]
+ * (row-major crystal -> sample), fault_offset (plane fault_normal . r = offset).  Device-side counterpart of the reference's
+ * host defect fields (Defects.py:3951-3990 for the fault rule). */
+int mosaic_defect_positions(int64_t n, const float* base, const float* params /* host */,
+                            float* out, mosaic_stream_t stream);
+
 /* --------------------------------------------------------------- surrogate */
 /* Kernels of the learned rocking-scan surrogate.  None has a counterpart in
  * the reference, which has no learned model; they serve the surrogate's

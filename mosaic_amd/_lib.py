@@ -33,6 +33,7 @@ EXPORTS = (
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
     "mosaic_lens_apply",
+    "mosaic_defect_positions",
     "mosaic_sg_gemm",
     "mosaic_sg_bias_act",
     "mosaic_sg_act_bwd",
@@ -87,6 +88,7 @@ _SIGS = {
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
                              ctypes.c_float, _vp, _vp, _vp, _sz, _vp],
     "mosaic_lens_apply": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
+    "mosaic_defect_positions": [_i64, _vp, _vp, _vp, _vp],
     "mosaic_sg_gemm": [_i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _i, _f, _f,
                        _vp],
     "mosaic_sg_bias_act": [_vp, _vp, _vp, _ll, _i, _i, _ll, _ll, _i, _i, _vp],

@@ -1,0 +1,373 @@
+"""Learned surrogate of the kinematic rocking scan, with fallback to the plain sum.
+
+Model: an ensemble of ``members`` networks, each an MLP followed by a
+convolutional decoder of 2x2 stride-2 transposed convolutions,
+
+    9 inputs -> 256 -> 512 -> 8x8x64 -> 16x16x64 -> 32x32x32 -> 64x64x16 -> 128x128x1
+
+(SiLU between layers, none at the output).  Every layer is one batched GEMM
+(``mosaic_sg_gemm``: BF16 MFMA, FP32 accumulation; the ensemble is the GEMM
+batch) plus hand-written elementwise kernels; the backward pass and the Adam
+update run on the same kernels.  The output is the normalised log-intensity
+patch  y = (ln(I / I0 + 1e-4) - mu) / sd.
+
+Inputs (``DefectParams``): dislocation core x, y [nm, +-50], Burgers vector
+polar/azimuth angles, line direction polar/azimuth angles, hydrostatic strain
+(0..1e-3), stacking-fault shift fraction (0..1), rocking angle [deg, +-0.05].
+
+Training images come from the GPU kinematic sum itself
+(``make_training_set``): the defect field is applied on the device
+(``mosaic_defect_positions``), the crystal staged and summed at its rocking
+angle.  At scan time (``SurrogateScan``) the ensemble predicts every frame;
+frames whose ensemble spread exceeds a threshold calibrated on held-out data
+are recomputed with the plain sum, which also serves as the error judge.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import kinematic as K
+from . import sg_ops as S
+from . import tables
+from ._lib import check
+
+N_IN = 9
+LOG_EPS = 1e-4
+A_AU = 4.08
+NU_AU = 0.44
+
+# ranges of the 9 inputs (low, high)
+RANGES = np.array([
+    [-50.0, 50.0],        # core x [nm]
+    [-50.0, 50.0],        # core y [nm]
+    [0.0, math.pi],       # Burgers polar angle
+    [0.0, 2 * math.pi],   # Burgers azimuth
+    [0.0, math.pi],       # line polar angle
+    [0.0, 2 * math.pi],   # line azimuth
+    [0.0, 1e-3],          # hydrostatic strain
+    [0.0, 1.0],           # stacking-fault shift fraction
+    [-0.05, 0.05],        # rocking angle [deg]
+])
+
+
+def sample_params(n, rng):
+    u = rng.random((n, N_IN))
+    return RANGES[:, 0] + u * (RANGES[:, 1] - RANGES[:, 0])
+
+
+def normalise(params):
+    return (2.0 * (np.asarray(params) - RANGES[:, 0]) / (RANGES[:, 1] - RANGES[:, 0]) - 1.0
+            ).astype(np.float32)
+
+
+def _unit(theta, phi):
+    return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi),
+                     math.cos(theta)])
+
+
+def defect_args(p, orient):
+    """The 32-float argument block of mosaic_defect_positions for one draw.
+
+    The fault plane sits a quarter (111) spacing off the origin, between atomic
+    planes of the centred lattice, so no atom lies on it."""
+    line = _unit(p[4], p[5])
+    b = (A_AU / math.sqrt(2.0)) * _unit(p[2], p[3])
+    bs = float(b @ line)
+    be_vec = b - bs * line
+    be = float(np.linalg.norm(be_vec))
+    if be > 1e-9:
+        e1 = be_vec / be
+    else:
+        t = np.array([1.0, 0.0, 0.0]) if abs(line[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        e1 = t - (t @ line) * line
+        e1 /= np.linalg.norm(e1)
+    e2 = np.cross(line, e1)
+    n = np.array([1.0, 1.0, 1.0]) / math.sqrt(3.0)
+    shift = p[7] * (A_AU / 6.0) * np.array([1.0, 1.0, -2.0])
+    core = np.array([p[0] * 10.0, p[1] * 10.0, 0.0])
+    arr = np.concatenate([core, line, e1, e2, [bs, be, NU_AU, p[6]], n, shift,
+                          np.asarray(orient, dtype=np.float64).reshape(9),
+                          [0.25 * A_AU / math.sqrt(3.0)]])
+    return np.ascontiguousarray(arr, dtype=np.float32)
+
+
+def defect_positions(base, p, orient, out=None):
+    """Displaced, oriented positions (N, 3) float32 on the device."""
+    if out is None:
+        out = torch.empty_like(base)
+    a = defect_args(p, orient)
+    s = torch.cuda.current_stream()
+    check(_lib.lib().mosaic_defect_positions(int(base.shape[0]), ctypes.c_void_p(base.data_ptr()),
+                                             a.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.c_void_p(out.data_ptr()),
+                                             ctypes.c_void_p(int(s.cuda_stream))),
+          "mosaic_defect_positions")
+    return out
+
+
+class ImageGenerator:
+    """Kinematic images of one defected FCC Au crystal family at (111)."""
+
+    def __init__(self, cells=(37, 37, 37), npix=128, device=None):
+        from . import workloads as W
+        from .sample import FCC_BASIS, lattice, orient_for_reflection, bragg_angle
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        w = W.fcc_fault_scan111(cells=cells, npix=npix, n_angles=1, fault=False)
+        self.w = w
+        lam = w["beam"].wavelength
+        th = bragg_angle(A_AU / math.sqrt(3.0), lam)
+        base, _ = lattice(A_AU, cells, FCC_BASIS)
+        self.orient = orient_for_reflection((1, 1, 1), th)
+        self.base = torch.as_tensor(base.astype(np.float32), device=self.device)
+        self.n = base.shape[0]
+        self.codes = torch.zeros(self.n, dtype=torch.uint8, device=self.device)
+        self.counts = np.array([self.n], dtype=np.int64)
+        b, d = w["beam"], w["detector"]
+        self.k = 2.0 * np.pi / lam
+        self.Ny, self.Nz = d.shape
+        self.spc = K.species_table(tables.species_constants(["Au"], b.energy))
+        self.opts = K.kin_opts(self.k, False, True, b._polarization_state(), False)
+        self.coords = torch.as_tensor(np.ascontiguousarray(d.pixel_coordinates / 1e10),
+                                      device=self.device)
+        self.stage = w["stage"]
+        self._ws = None
+        self._pos = torch.empty_like(self.base)
+
+    def pose(self, angle_deg):
+        self.stage.set_single_motor_value_absolute("phi", float(angle_deg))
+        R = np.asarray(self.stage.rotation, np.float32)
+        T = np.asarray(self.stage.translation, np.float32)
+        self.stage.set_single_motor_value_absolute("phi", 0.0)
+        return R, T
+
+    def stage_crystal(self, p):
+        defect_positions(self.base, p, self.orient, out=self._pos)
+        return self._pos
+
+    def intensities(self, p, angles_deg):
+        """|A|^2 (F, Nz*Ny) float32 for defect parameters p at the given angles."""
+        pos = self.stage_crystal(p)
+        F = len(angles_deg)
+        poses = [self.pose(a) for a in angles_deg]
+        rot = np.stack([q[0] for q in poses])
+        tr = np.stack([q[1] for q in poses])
+        staged = K.stage(pos, self.codes, None, self.counts, self.k, F, self.Ny, self.Nz,
+                         workspace=self._ws)
+        if self._ws is not None and staged.workspace is self._ws and hasattr(self, "_pix"):
+            staged.pixels_key, staged.pixel_geom = self._pix
+        self._ws = staged.workspace
+        out = torch.zeros((F, self.Nz, self.Ny), dtype=torch.complex64, device=self.device)
+        K.kin_sum(staged, self.spc, self.opts, rot, tr, self.coords, out)
+        self._pix = (staged.pixels_key, staged.pixel_geom)
+        return (out.abs() ** 2).reshape(F, -1)
+
+
+def make_training_set(gen, n, seed, report_every=2000):
+    """n (params, intensity) pairs from the GPU kinematic sum."""
+    rng = np.random.default_rng(seed)
+    P = sample_params(n, rng)
+    I = torch.empty((n, gen.Ny * gen.Nz), dtype=torch.float32, device=gen.device)
+    t0 = time.perf_counter()
+    for i in range(n):
+        I[i] = gen.intensities(P[i], [P[i, 8]])[0]
+        if report_every and (i + 1) % report_every == 0:
+            torch.cuda.synchronize()
+            print(f"[surrogate] generated {i + 1}/{n} images "
+                  f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    return P, I
+
+
+class Layer:
+    def __init__(self, E, n_in, n_out, bias_n, act, rng, device):
+        std = math.sqrt(2.0 / n_in) if act else math.sqrt(1.0 / n_in)
+        self.W = torch.as_tensor(rng.standard_normal((E, n_in, n_out)).astype(np.float32) * std,
+                                 device=device)
+        self.b = torch.zeros((E, bias_n), dtype=torch.float32, device=device)
+        self.act = act
+        self.params = [self.W, self.b]
+        self.grads = [torch.zeros_like(self.W), torch.zeros_like(self.b)]
+        self.m = [torch.zeros_like(self.W), torch.zeros_like(self.b)]
+        self.v = [torch.zeros_like(self.W), torch.zeros_like(self.b)]
+
+
+class SurrogateEnsemble:
+    """Ensemble MLP + transposed-conv decoder, all on the surrogate kernels."""
+
+    MLP = (N_IN, 256, 512, 8 * 8 * 64)
+    CHANNELS = (64, 64, 32, 16, 16, 16)
+
+    def __init__(self, members=5, seed=0, device=None, out_hw=128):
+        if out_hw < 16 or out_hw & (out_hw - 1):
+            raise ValueError("out_hw must be a power of two >= 16")
+        self.out_hw = out_hw
+        n_stage = int(round(math.log2(out_hw // 8)))
+        # decoder stages (H, Cin, Cout): 8x8x64 doubling up to out_hw, one channel out
+        self.DEC = tuple((8 << i, self.CHANNELS[i],
+                          1 if i == n_stage - 1 else self.CHANNELS[i + 1]) for i in range(n_stage))
+        self.E = members
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        rng = np.random.default_rng(seed)
+        self.mlp = [Layer(members, a, b, b, 1, rng, self.device)
+                    for a, b in zip(self.MLP[:-1], self.MLP[1:])]
+        self.dec = [Layer(members, ci, 4 * co, co, int(i < len(self.DEC) - 1), rng, self.device)
+                    for i, (h, ci, co) in enumerate(self.DEC)]
+        self.step_count = 0
+        self.norm = None     # (I0, mu, sd)
+
+    @property
+    def layers(self):
+        return self.mlp + self.dec
+
+    def n_params(self):
+        """Parameters of one member."""
+        return sum(int(p[0].numel()) for L in self.layers for p in L.params)
+
+    # ------------------------------------------------------------ forward
+    def forward(self, X):
+        """X (B, 9) normalised inputs on the device -> (E, B, 128*128) outputs."""
+        B = X.shape[0]
+        E = self.E
+        self._X = X.reshape(1, B, N_IN).contiguous()
+        self._acts = []
+        h = self._X
+        for L in self.mlp:
+            n_out = L.W.shape[2]
+            Z = torch.empty((E, B, n_out), dtype=torch.float32, device=self.device)
+            S.gemm(h, L.W, Z)
+            Y = torch.empty_like(Z)
+            S.bias_act(Z, Y, L.b, n_out, L.act)
+            self._acts.append((h, Z))
+            h = Y
+        img = h.reshape(E, B * 64, 64)
+        for i, (L, (H, ci, co)) in enumerate(zip(self.dec, self.DEC)):
+            G = torch.empty((E, B * H * H, 4 * co), dtype=torch.float32, device=self.device)
+            S.gemm(img, L.W, G)
+            Ya = torch.empty_like(G)
+            S.bias_act(G, Ya, L.b, co, L.act)
+            out = torch.empty((E, B * 4 * H * H, co), dtype=torch.float32, device=self.device)
+            S.shuffle2(Ya, out, B, H, H, co)
+            self._acts.append((img, G))
+            img = out
+        return img.reshape(E, B, -1)
+
+    # ----------------------------------------------------------- backward
+    def backward(self, dP):
+        """dP (E, B, 128*128) -> parameter gradients (summed over the batch)."""
+        E = self.E
+        B = dP.shape[1]
+        d = dP.reshape(E, -1, 1).contiguous()
+        for i in reversed(range(len(self.dec))):
+            L = self.dec[i]
+            H, ci, co = self.DEC[i]
+            img, G = self._acts[len(self.mlp) + i]
+            dYa = torch.empty_like(G)
+            S.shuffle2(dYa, d.reshape(E, -1).contiguous(), B, H, H, co, backward=True)
+            dG = torch.empty_like(G)
+            S.act_bwd(G, dYa, dG, L.act)
+            S.bias_grad(dG, L.grads[1], co)
+            S.gemm(img, dG, L.grads[0], transA=True)
+            d = torch.empty_like(img)
+            S.gemm(dG, L.W, d, transB=True)
+        d = d.reshape(E, B, -1).contiguous()
+        for i in reversed(range(len(self.mlp))):
+            L = self.mlp[i]
+            h, Z = self._acts[i]
+            dZ = torch.empty_like(Z)
+            S.act_bwd(Z, d, dZ, L.act)
+            S.bias_grad(dZ, L.grads[1], Z.shape[2])
+            if h.shape[0] == 1 and E > 1:
+                hb = h.expand(E, -1, -1).contiguous()
+            else:
+                hb = h
+            S.gemm(hb, dZ, L.grads[0], transA=True)
+            if i > 0:
+                d = torch.empty_like(h)
+                S.gemm(dZ, L.W, d, transB=True)
+
+    def adam_step(self, lr, weight_decay=0.0):
+        self.step_count += 1
+        for L in self.layers:
+            for p, g, m, v in zip(L.params, L.grads, L.m, L.v):
+                S.adam(p, g, m, v, lr, self.step_count, weight_decay=weight_decay)
+
+    # ------------------------------------------------------------- fitting
+    def set_normalisation(self, I):
+        I0 = float(I.max().item())
+        y = torch.log(I / I0 + LOG_EPS)
+        self.norm = (I0, float(y.mean().item()), float(y.std().item()))
+
+    def targets(self, I):
+        I0, mu, sd = self.norm
+        return ((torch.log(I / I0 + LOG_EPS) - mu) / sd).contiguous()
+
+    def intensity(self, y):
+        I0, mu, sd = self.norm
+        return I0 * (torch.exp(y * sd + mu) - LOG_EPS)
+
+    def train(self, P, I, epochs, batch=128, lr=2e-3, seed=0, log=print, lr_min=1e-5):
+        """Adam on the mean squared error of the normalised log-intensity."""
+        if self.norm is None:
+            self.set_normalisation(I)
+        n = P.shape[0]
+        X = torch.as_tensor(normalise(P), device=self.device)
+        Y = self.targets(I)
+        rng = np.random.default_rng(seed)
+        loss = torch.zeros(self.E, dtype=torch.float32, device=self.device)
+        steps = epochs * (n // batch)
+        hist = []
+        s = 0
+        for ep in range(epochs):
+            perm = torch.as_tensor(rng.permutation(n), device=self.device)
+            loss.zero_()
+            nb = 0
+            for b0 in range(0, n - batch + 1, batch):
+                idx = perm[b0:b0 + batch]
+                out = self.forward(X[idx])
+                dP = torch.empty_like(out)
+                # loss and gradient of the mean over the batch's pixels
+                S.mse(out, Y[idx].reshape(-1).contiguous(), dP, loss)
+                self.backward(dP)
+                cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * s / max(1, steps)))
+                self.adam_step(cur)
+                s += 1
+                nb += 1
+            ep_loss = (loss / nb).cpu().numpy()
+            hist.append(ep_loss)
+            if log:
+                log(f"[surrogate] epoch {ep + 1}/{epochs} mse per member "
+                    f"{np.array2string(ep_loss, precision=4)}")
+        return np.array(hist)
+
+    def predict(self, P):
+        """Ensemble mean intensity (B, npix) and per-frame spread (B,)."""
+        X = torch.as_tensor(normalise(P), device=self.device)
+        out = self.forward(X)
+        B = out.shape[1]
+        mean = torch.empty((B, out.shape[2]), dtype=torch.float32, device=self.device)
+        spread = torch.empty(B, dtype=torch.float32, device=self.device)
+        S.ensemble_stats(out.contiguous(), mean, spread)
+        return self.intensity(mean), spread
+
+
+def rel_l2(a, b):
+    return (torch.linalg.norm(a - b, dim=-1) / torch.linalg.norm(b, dim=-1))
+
+
+class SurrogateScan:
+    """A rocking scan predicted by the ensemble, gated frames recomputed."""
+
+    def __init__(self, model, gen, threshold):
+        self.model, self.gen, self.threshold = model, gen, float(threshold)
+
+    def run(self, p, angles_deg):
+        P = np.repeat(np.asarray(p, dtype=np.float64)[None], len(angles_deg), axis=0)
+        P[:, 8] = angles_deg
+        I, spread = self.model.predict(P)
+        bad = torch.nonzero(spread > self.threshold).flatten().cpu().numpy()
+        if bad.size:
+            I[bad] = self.gen.intensities(p, [angles_deg[i] for i in bad])
+        return I, spread, bad

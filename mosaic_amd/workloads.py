@@ -15,7 +15,7 @@ Each builder returns a dict with ``sample``, ``detector``, ``stage``,
   reflection; npix^2 detector at 1 m.  n = 100, npix = 512 is the
   5,000,000-atom precision workload.
 * ``fcc_fault_scan111`` -- FCC Au (nx, ny, nz) cells with one intrinsic
-  stacking fault on the (111) plane through the centre (atoms above shift by
+  stacking fault on a (111) plane at mid-height, between atomic planes (atoms above shift by
   a/6 [1 1 -2]); ``n_angles`` phi rotations uniformly in +-0.05 deg around the
   (111) Bragg condition; npix^2 detector.  (50, 50, 100) cells = 1,000,000
   atoms, 64 angles, 256^2 pixels is the batched rocking-scan workload.
@@ -85,7 +85,8 @@ def fcc_fault_scan111(cells=(50, 50, 100), npix=256, n_angles=64, half_range_deg
     th = S.bragg_angle(a / np.sqrt(3.0), lam)
     pos, _ = S.lattice(a, cells, S.FCC_BASIS)
     if fault:
-        pos = S.stacking_fault_shift(pos, a, plane_offset=0.0)
+        # plane a quarter (111) spacing off the centre: between atomic planes
+        pos = S.stacking_fault_shift(pos, a, plane_offset=0.25 * a / np.sqrt(3.0))
     R = S.orient_for_reflection((1, 1, 1), th)
     pos = (pos @ R.T).astype(np.float32)
     smp = S.sample([pos], [np.zeros(pos.shape[0], dtype=np.uint8)], ["Au"])

