@@ -62,8 +62,8 @@ int mosaic_kin_workspace(int64_t atom_count, int n_species,
                          int n_frames, int Ny, int Nz, size_t* bytes);
 
 /*
- * Stage a sample into the workspace: sort the atoms by (species, Morton
- * code), cut them into 256-atom single-species tiles, and store each tile as
+ * Stage a sample into the workspace: sort the atoms by (species, Hilbert
+ * index), cut them into 256-atom single-species tiles, and store each tile as
  * an FP64 origin plus FP32 offsets in wavelengths with the incident
  * amplitude.  Replaces the reference's per-species partition, Morton order
  * and per-chunk origins (_partition/_stage, Beam.py:9265-9319;
@@ -115,7 +115,8 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * whenever its error bounds hold.  `stage_stats` and `pixel_geom` are the
  * outputs of mosaic_kin_stage and mosaic_kin_pixels; `path_used` (host out,
  * may be NULL) reports 11/12 (per-tile, first/second order) or 24/28 (row
- * recurrence over 4/8 pixels).  Stream-ordered, no host synchronization.
+ * recurrence over 4/8 pixels; 124/128 with the atoms' second-order phase).
+ * Stream-ordered, no host synchronization.
  * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
  * close for the split's error bound -- the reference's "decline"
  * (Beam.py:9466-9468).

@@ -6,7 +6,7 @@
 //           * [R0/r] * [sqrt(P)]
 // accumulated over atoms.  The MI355X formulation:
 //
-//  * Staging (mosaic_kin_stage): atoms sorted by (species, Morton code) and
+//  * Staging (mosaic_kin_stage): atoms sorted by (species, Hilbert index) and
 //    cut into 256-atom single-species tiles.  A tile stores its origin O_t
 //    in FP64 and each atom as an FP32 offset sigma_j = (r_j - O_t) / lambda
 //    (in wavelengths) with its complex amplitude, 32 B per atom.
@@ -43,6 +43,7 @@ namespace mosaic {
 namespace {
 
 constexpr int TILE = 256;            // atoms per tile
+constexpr uint32_t NO_SPLIT = 0xFFFFFFFFu;   // k-d node that does not split
 constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
@@ -103,9 +104,12 @@ struct Layout {
     uint32_t* vals_b;
     void* sort_tmp;
     size_t sort_bytes;
-    double* bbox_blocks;
-    double* bbox;
+    uint32_t* seg;          // k-d node of every position
+    unsigned int* kd_box;   // [node][6] bounding boxes of one level
+    uint32_t* kd_split;     // cut positions of every level
+    int kd_levels;
     unsigned int* radius_bits;
+    float* amax_blocks;     // adjoint: per-block max |dL/dA| and max |G|
     FrameRec* frames;
     double2* partial;
     size_t total;
@@ -116,8 +120,7 @@ struct Layout {
 };
 
 constexpr int PRO_THREADS = 256;
-constexpr int BBOX_THREADS = 256;
-constexpr int BBOX_MAX_BLOCKS = 1024;
+constexpr int AMAX_BLOCKS = 256;
 
 int pix_per_thread(int npix) { return npix >= 4 * SUM_THREADS * 64 ? 8 : 4; }
 
@@ -138,6 +141,57 @@ int count_tiles(int n_species, const int64_t* counts) {
 }
 
 // Workspace layout; also the size query when base is null.
+// Levels of the k-d bisection: enough to bring the largest species down to
+// one tile per leaf.
+int kd_levels(int n_species, const int64_t* counts) {
+    int64_t tmax = 1;
+    for (int s = 0; s < n_species; ++s) tmax = std::max<int64_t>(tmax, (counts[s] + TILE - 1) / TILE);
+    int levels = 0;
+    while ((int64_t(1) << levels) < tmax) ++levels;
+    return levels;
+}
+
+// Nodes over all levels (level l holds n_species << l of them).
+size_t kd_split_count(int n_species, int levels) {
+    return static_cast<size_t>(n_species) * ((size_t(1) << levels) - 1);
+}
+
+// Cut positions of the k-d bisection, level by level, in heap numbering
+// (node h of a level has children 2h and 2h + 1 on the next): a node of n > T
+// atoms keeps ceil(n / 2T) T on the left; NO_SPLIT where a node holds one
+// tile or less (it passes whole to its left child).
+void kd_plan(int n_species, const int64_t* counts, int levels, std::vector<uint32_t>& split) {
+    std::vector<int64_t> b(n_species), e(n_species);
+    int64_t a = 0;
+    for (int s = 0; s < n_species; ++s) {
+        b[s] = a;
+        a += counts[s];
+        e[s] = a;
+    }
+    split.clear();
+    split.reserve(kd_split_count(n_species, levels));
+    for (int l = 0; l < levels; ++l) {
+        const size_t nodes = b.size();
+        std::vector<int64_t> nb(2 * nodes), ne(2 * nodes);
+        for (size_t h = 0; h < nodes; ++h) {
+            const int64_t n = e[h] - b[h];
+            int64_t cut = e[h];
+            if (n > TILE) {
+                cut = b[h] + (n + 2 * TILE - 1) / (2 * TILE) * TILE;
+                split.push_back(static_cast<uint32_t>(cut));
+            } else {
+                split.push_back(NO_SPLIT);
+            }
+            nb[2 * h] = b[h];
+            ne[2 * h] = cut;
+            nb[2 * h + 1] = cut;
+            ne[2 * h + 1] = e[h];
+        }
+        b.swap(nb);
+        e.swap(ne);
+    }
+}
+
 int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
                 int n_frames, int npix, Layout& L) {
     Carver c(base);
@@ -157,9 +211,13 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
         return MOSAIC_ERR_HIP;
     L.sort_bytes = sort_bytes;
     L.sort_tmp = c.take<char>(sort_bytes);
-    L.bbox_blocks = c.take<double>(6 * BBOX_MAX_BLOCKS);
-    L.bbox = c.take<double>(6);
+    L.seg = c.take<uint32_t>(n);
+    L.kd_levels = kd_levels(n_species, counts);
+    const size_t top = static_cast<size_t>(n_species) << std::max(0, L.kd_levels - 1);
+    L.kd_box = c.take<unsigned int>(6 * top);
+    L.kd_split = c.take<uint32_t>(kd_split_count(n_species, L.kd_levels) + 1);
     L.radius_bits = c.take<unsigned int>(2);
+    L.amax_blocks = c.take<float>(2 * AMAX_BLOCKS);
     L.frames = c.take<FrameRec>(std::max(1, n_frames));
     const int ppt = pix_per_thread(npix);
     L.pix_blocks = blocks_for(npix, SUM_THREADS * ppt);
@@ -345,69 +403,153 @@ __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __r
 }
 
 // ------------------------------------------------------------------- staging
-__global__ __launch_bounds__(BBOX_THREADS) void k_bbox_partial(
-    const float* __restrict__ pos, int64_t n, double* __restrict__ out) {
-    __shared__ double lds[BBOX_THREADS / 64];
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        for (int d = 0; d < 3; ++d) {
-            const double x = static_cast<double>(pos[3 * i + d]);
-            lo[d] = fmin(lo[d], x);
-            hi[d] = fmax(hi[d], x);
-        }
-    }
-    for (int d = 0; d < 3; ++d) {
-        const double a = block_min(lo[d], lds);
-        const double b = block_max(hi[d], lds);
-        if (threadIdx.x == 0) {
-            out[6 * blockIdx.x + d] = a;
-            out[6 * blockIdx.x + 3 + d] = b;
-        }
-    }
+// Tiles come from a k-d bisection of each species' atoms.  A node of n atoms
+// is cut across the longest side of its bounding box so that the left part
+// holds ceil(n / 2T) T atoms (T = 256); after ceil(log2(tiles)) levels every
+// leaf is one tile, all full except the last of each species, in the same
+// places as a plain cut of the sorted order into runs of T.  The cells are
+// boxes of near-equal count and bounded aspect, so the tile radius stays near
+// that of a cube of 256 atoms whatever the crystal's shape or orientation (a
+// space-filling-curve order leaves runs that wander through the empty corners
+// of the bounding box: radius up to ~120 wavelengths instead of ~20).  One
+// level = per-node bounding boxes (integer atomics on order-preserving images
+// of the floats), a key (node, coordinate along the node's longest side), one
+// stable radix sort over all nodes at once, and the nodes of the next level.
+
+constexpr int KD_QBITS = 20;       // coordinate bits of a k-d sort key
+
+__device__ __forceinline__ unsigned int f2ord(float f) {
+    const unsigned int b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned int u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ unsigned int wave_umin(unsigned int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, static_cast<unsigned int>(__shfl_xor(static_cast<int>(v), o, 64)));
+    return v;
+}
+__device__ __forceinline__ unsigned int wave_umax(unsigned int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned int>(__shfl_xor(static_cast<int>(v), o, 64)));
+    return v;
 }
 
-__global__ void k_bbox_final(const double* __restrict__ part, int nb, double* __restrict__ bbox,
-                             unsigned int* __restrict__ radius_bits) {
-    __shared__ double lds[16];
-    for (int d = 0; d < 3; ++d) {
-        double a = 1e300, b = -1e300;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-            a = fmin(a, part[6 * i + d]);
-            b = fmax(b, part[6 * i + 3 + d]);
-        }
-        a = block_min(a, lds);
-        b = block_max(b, lds);
-        if (threadIdx.x == 0) { bbox[d] = a; bbox[3 + d] = b; }
-    }
-    if (threadIdx.x == 0) { radius_bits[0] = 0u; radius_bits[1] = 0u; }
+__global__ void k_stage_init(unsigned int* __restrict__ radius_bits) {
+    if (threadIdx.x < 2) radius_bits[threadIdx.x] = 0u;
 }
 
-__device__ __forceinline__ uint64_t spread3(uint64_t x) {
-    x &= 0xFFFFFull;                         // 20 bits
-    x = (x | (x << 32)) & 0x00FF00000000FFFFull;
-    x = (x | (x << 16)) & 0x00FF0000FF0000FFull;
-    x = (x | (x << 8)) & 0xF00F00F00F00F00Full;
-    x = (x | (x << 4)) & 0x30C30C30C30C30C3ull;
-    x = (x | (x << 2)) & 0x9249249249249249ull;
-    return x;
-}
-
-// Sort key: species in bits 60..63, 3 x 20-bit Morton code below.
-__global__ void k_keys(const float* __restrict__ pos, const uint8_t* __restrict__ spc, int64_t n,
-                       const double* __restrict__ bbox, uint64_t* __restrict__ keys,
-                       uint32_t* __restrict__ vals) {
+// Group by species (stable): key = species code.
+__global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
+                               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t code = 0;
-    for (int d = 0; d < 3; ++d) {
-        const double lo = bbox[d], ext = bbox[3 + d] - lo;
-        double q = ext > 0.0 ? (static_cast<double>(pos[3 * i + d]) - lo) / ext : 0.0;
-        q = fmin(fmax(q, 0.0), 1.0) * 1048575.0;
-        code |= spread3(static_cast<uint64_t>(q)) << d;
-    }
-    keys[i] = (static_cast<uint64_t>(spc[i] & 0xF) << 60) | code;
+    keys[i] = spc[i] & 0xF;
     vals[i] = static_cast<uint32_t>(i);
+}
+
+constexpr int KD_THREADS = 256;
+
+// Bounding boxes of the nodes that split at this level.  Node ids do not
+// decrease along the order, so a block (or a wave) whose first and last
+// positions share a node reduces locally and issues 6 atomics.
+__global__ __launch_bounds__(KD_THREADS) void k_kd_box(
+    const float* __restrict__ pos, const uint32_t* __restrict__ order,
+    const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split, int64_t n,
+    unsigned int* __restrict__ box) {
+    __shared__ unsigned int red[KD_THREADS / 64][6];
+    const int64_t i0 = blockIdx.x * (int64_t)KD_THREADS;
+    const int64_t i = i0 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool in = i < n;
+    const uint32_t s = in ? seg[i] : NO_SPLIT;
+    const bool active = in && split[s] != NO_SPLIT;
+    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    if (active) {
+        const uint32_t a = order[i];
+        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
+    }
+    const int64_t i_last = min(i0 + KD_THREADS, n) - 1;
+    const uint32_t sb0 = seg[i0], sb1 = seg[i_last];
+    const bool block_uniform = sb0 == sb1 && i0 + KD_THREADS <= n;
+    const uint32_t w0 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 0, 64));
+    const uint32_t w1 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 63, 64));
+    if (block_uniform || w0 == w1) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = wave_umin(lo[d]);
+            hi[d] = wave_umax(hi[d]);
+        }
+    }
+    if (block_uniform) {
+        if (lane == 0)
+            for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
+        __syncthreads();
+        if (threadIdx.x == 0 && split[sb0] != NO_SPLIT) {
+            for (int d = 0; d < 3; ++d) {
+                unsigned int a = red[0][d], b = red[0][3 + d];
+                for (int w = 1; w < KD_THREADS / 64; ++w) {
+                    a = min(a, red[w][d]);
+                    b = max(b, red[w][3 + d]);
+                }
+                atomicMin(box + 6 * static_cast<int64_t>(sb0) + d, a);
+                atomicMax(box + 6 * static_cast<int64_t>(sb0) + 3 + d, b);
+            }
+        }
+    } else if (w0 == w1) {
+        if (lane == 0 && active)
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
+                atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
+            }
+    } else if (active) {
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
+            atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
+        }
+    }
+}
+
+__global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < words) box[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
+}
+
+// Key (node, coordinate along the node's longest side); nodes that do not
+// split keep their order (coordinate 0, stable sort).
+__global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restrict__ order,
+                          const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split,
+                          const unsigned int* __restrict__ box, int64_t n,
+                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = seg[i];
+    const uint32_t a = order[i];
+    uint32_t q = 0;
+    if (split[s] != NO_SPLIT) {
+        const unsigned int* b = box + 6 * static_cast<int64_t>(s);
+        float lo[3], ext[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = ord2f(b[d]);
+            ext[d] = ord2f(b[3 + d]) - lo[d];
+        }
+        int ax = 0;
+        if (ext[1] > ext[ax]) ax = 1;
+        if (ext[2] > ext[ax]) ax = 2;
+        if (ext[ax] > 0.f) {
+            const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) / ext[ax];
+            q = static_cast<uint32_t>(fminf(fmaxf(t, 0.f), 1.f) * static_cast<float>((1u << KD_QBITS) - 1u));
+        }
+    }
+    keys[i] = (static_cast<uint64_t>(s) << KD_QBITS) | q;
+    vals[i] = a;
+}
+
+// Node ids of the next level: children 2s (left of the cut) and 2s + 1.
+__global__ void k_kd_next(const uint32_t* __restrict__ split, int64_t n,
+                          uint32_t* __restrict__ seg) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = seg[i];
+    seg[i] = 2u * s + (static_cast<uint64_t>(i) >= split[s] ? 1u : 0u);
 }
 
 struct SpeciesSplit {
@@ -415,6 +557,15 @@ struct SpeciesSplit {
     int64_t atom_start[MAX_SPECIES + 1];
     int tile_start[MAX_SPECIES + 1];
 };
+
+// Level-0 node of every position of the species-grouped order: its species.
+__global__ void k_kd_root(int64_t n, SpeciesSplit split, uint32_t* __restrict__ seg) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = 0;
+    while (static_cast<int>(s) + 1 < split.n_species && i >= split.atom_start[s + 1]) ++s;
+    seg[i] = s;
+}
 
 // One workgroup per tile: FP64 origin at the tile's bounding-box centre, FP32
 // offsets in wavelengths, amplitudes; padding lanes get zero weight.
@@ -638,7 +789,9 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum(const SumArgs A) {
 // far-field expansion
 //     Theta = Theta0 + K (X_t - u0 . O + (|O|^2 - (u0 . O)^2) / (2 rho0)),
 // exact to third order in |O| / rho0, and the factor G comes from the
-// per-pixel table.
+// per-pixel table.  With SECOND the atom's own second-order phase
+// (|sigma|^2 - (u . sigma)^2) / (2 K rho) is added, evaluated once per lane at
+// the middle of its KP pixels (across them it changes by ~1e-10 rev).
 struct RecArgs {
     const AtomRec* atoms;
     const TileRec* tiles;
@@ -657,7 +810,7 @@ struct RecArgs {
     double K;         // 1 / lambda
 };
 
-template <int KP>
+template <int KP, bool SECOND>
 __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
     const int frame = blockIdx.z;
     const int slice = blockIdx.y;
@@ -683,6 +836,10 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
     const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
     const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
     const int m_last = nv - 1, m_a = (nv - 1) / 2, m_b = nv / 2;
+    const float xs0 = static_cast<float>(F.xs[0]);
+    const float xs1 = static_cast<float>(F.xs[1]);
+    const float xs2 = static_cast<float>(F.xs[2]);
+    const float hmid = 0.5f * static_cast<float>(m_last);
 
     for (int t = t0; t < t1; ++t) {
         const TileRec tr = A.tiles[t];
@@ -725,6 +882,8 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
         const float gx = static_cast<float>((a1x - a0x) * inv_n);
         const float gy = static_cast<float>((a1y - a0y) * inv_n);
         const float gz = static_cast<float>((a1z - a0z) * inv_n);
+        // 1 / (2 K rho) at the lane's middle pixel
+        const float hh = SECOND ? static_cast<float>(A.u0h[pb + m_a].w / A.K) : 0.f;
 
         float S_r[KP], S_i[KP];
 #pragma unroll
@@ -736,8 +895,13 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
         for (int j = 0; j < cnt; ++j) {
             const AtomRec a = nxt;                 // next record in flight meanwhile
             if (j + 1 < cnt) nxt = at[j + 1];
-            const float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
+            float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
             const float d = fmaf(gz, a.sz, fmaf(gy, a.sy, gx * a.sx));
+            if (SECOND) {
+                // u . sigma at the middle pixel = x_hat . sigma - v_mid . sigma
+                const float u = fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx)) - fmaf(hmid, d, psi);
+                psi = fmaf(hh, fmaf(-u, u, a.s2), psi);
+            }
             const float r = __builtin_amdgcn_fractf(psi);
             const float sn = __builtin_amdgcn_sinf(r);
             const float cs = __builtin_amdgcn_cosf(r);
@@ -784,11 +948,11 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
         if (m < nv) out[pbase + m] = acc[m][threadIdx.x];
 }
 
-template <int KP>
+template <int KP, bool SECOND>
 hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t st) {
     const int lanes = A.Nz * A.nbx;
     dim3 grid(blocks_for(lanes, SUM_THREADS), n_slices, n_frames);
-    hipLaunchKernelGGL((k_sum_rec<KP>), grid, dim3(SUM_THREADS), 0, st, A);
+    hipLaunchKernelGGL((k_sum_rec<KP, SECOND>), grid, dim3(SUM_THREADS), 0, st, A);
     return hipGetLastError();
 }
 
@@ -814,6 +978,7 @@ struct AdjArgs {
     const double* theta0;
     const float2* gtab;
     const float2* gin;     // [frame][pixel] dL/dA in the Re + i Im convention
+    const float* amax;     // [2][AMAX_BLOCKS] block maxima of |gin| and |gtab|
     float* grad;           // [atom][3], accumulated into
     int n_frames;
     int Ny;
@@ -834,13 +999,50 @@ __device__ __forceinline__ float wave_sum_to_63(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-template <int KP>
+// Block maxima of |x| over the components of dL/dA and of the per-species
+// factors G.  Their product is ~1e-40 for fields in r_e units, below FP32's
+// normal range, so the adjoint scales both to O(1) by powers of two.
+__global__ __launch_bounds__(256) void k_amax(const float* __restrict__ gin, int64_t n_gin,
+                                              const float* __restrict__ gtab, int64_t n_gtab,
+                                              float* __restrict__ out) {
+    __shared__ double lds[4];
+    float a = 0.f, b = 0.f;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_gin; i += stride)
+        a = fmaxf(a, fabsf(gin[i]));
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_gtab; i += stride)
+        b = fmaxf(b, fabsf(gtab[i]));
+    const double ma = block_max(static_cast<double>(a), lds);
+    const double mb = block_max(static_cast<double>(b), lds);
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = static_cast<float>(ma);
+        out[AMAX_BLOCKS + blockIdx.x] = static_cast<float>(mb);
+    }
+}
+
+// 2^-e with x 2^-e in [0.5, 1); 1 for x = 0 (or non-finite x).
+__device__ __forceinline__ float pow2_scale(float x) {
+    if (!(x > 0.f) || !isfinite(x)) return 1.f;
+    int e;
+    frexpf(x, &e);
+    return ldexpf(1.f, -e);
+}
+
+template <int KP, bool SECOND>
 __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
     __shared__ double acc[SUM_THREADS / 64][TILE][6];
+    __shared__ double red[SUM_THREADS / 64];
     const int t = blockIdx.x;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int i = tid; i < (SUM_THREADS / 64) * TILE * 6; i += SUM_THREADS)
         (&acc[0][0][0])[i] = 0.0;
+    float mg = 0.f, mG = 0.f;
+    for (int i = tid; i < AMAX_BLOCKS; i += SUM_THREADS) {
+        mg = fmaxf(mg, A.amax[i]);
+        mG = fmaxf(mG, A.amax[AMAX_BLOCKS + i]);
+    }
+    const float sg = pow2_scale(static_cast<float>(block_max(static_cast<double>(mg), red)));
+    const float sG = pow2_scale(static_cast<float>(block_max(static_cast<double>(mG), red)));
     __syncthreads();
     const TileRec tr = A.tiles[t];
     const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
@@ -859,6 +1061,9 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
         const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
         const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
         const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const float xs0 = static_cast<float>(F.xs[0]);
+        const float xs1 = static_cast<float>(F.xs[1]);
+        const float xs2 = static_cast<float>(F.xs[2]);
         const float2* __restrict__ gin = A.gin + static_cast<int64_t>(f) * A.npix;
         for (int pass = 0; pass < passes; ++pass) {
             const int lb = pass * SUM_THREADS + tid;
@@ -895,7 +1100,10 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
             const float gx = static_cast<float>((a1x - a0x) * inv_n);
             const float gy = static_cast<float>((a1y - a0y) * inv_n);
             const float gz = static_cast<float>((a1z - a0z) * inv_n);
-            // B_m = conj(g) G e^{2 pi i Theta}; zero for pixels past the row end
+            const float hh = SECOND ? static_cast<float>(A.u0h[pbase + m_a].w / A.K) : 0.f;
+            const float hmid = 0.5f * static_cast<float>(m_last);
+            // B_m = conj(g) G e^{2 pi i Theta} (scaled by sg sG); zero for
+            // pixels past the row end
             float br[KP], bi[KP];
 #pragma unroll
             for (int m = 0; m < KP; ++m) {
@@ -907,9 +1115,11 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
                 const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
                 const float sn = __builtin_amdgcn_sinf(thf);
                 const float cs = __builtin_amdgcn_cosf(thf);
-                const float2 gg = G[p];
+                const float2 g0 = G[p];
+                const float2 gg = make_float2(g0.x * sG, g0.y * sG);
                 const float zr = gg.x * cs - gg.y * sn, zi = gg.x * sn + gg.y * cs;
-                const float2 q = gin[p];                       // conj(q) * z
+                const float2 q0 = gin[p];                      // conj(q) * z
+                const float2 q = make_float2(q0.x * sg, q0.y * sg);
                 const bool live = lane_ok && m < nv;
                 br[m] = live ? q.x * zr + q.y * zi : 0.f;
                 bi[m] = live ? q.x * zi - q.y * zr : 0.f;
@@ -918,8 +1128,13 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
             for (int j = 0; j < cnt; ++j) {
                 const AtomRec a = nxt;
                 if (j + 1 < cnt) nxt = at[j + 1];
-                const float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
+                float psi = fmaf(v0z, a.sz, fmaf(v0y, a.sy, v0x * a.sx));
                 const float d = fmaf(gz, a.sz, fmaf(gy, a.sy, gx * a.sx));
+                if (SECOND) {
+                    const float u =
+                        fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx)) - fmaf(hmid, d, psi);
+                    psi = fmaf(hh, fmaf(-u, u, a.s2), psi);
+                }
                 const float r = __builtin_amdgcn_fractf(psi);
                 float er = __builtin_amdgcn_cosf(r);
                 float ei = __builtin_amdgcn_sinf(r);
@@ -967,7 +1182,8 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
             Z[c] = 0.0;
             for (int w = 0; w < SUM_THREADS / 64; ++w) Z[c] += acc[w][tid][c];
         }
-        const double scale = -2.0 * PI * A.K * 1e-10;
+        const double scale = -2.0 * PI * A.K * 1e-10 /
+                             (static_cast<double>(sg) * static_cast<double>(sG));
         const uint32_t idx = __float_as_uint(a.pad0);
         for (int c = 0; c < 3; ++c) {
             const double g = scale * (static_cast<double>(a.wr) * Z[2 * c + 1] +
@@ -1095,18 +1311,6 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     stage_stats[1] = 0.0;
     if (atom_count == 0) return MOSAIC_OK;
 
-    const int nb = std::min(BBOX_MAX_BLOCKS, blocks_for(atom_count, BBOX_THREADS));
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(BBOX_THREADS), 0, st, pos, atom_count,
-                       L.bbox_blocks);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, L.bbox_blocks, nb, L.bbox,
-                       L.radius_bits);
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(atom_count, 256)), dim3(256), 0, st, pos,
-                       species, atom_count, L.bbox, L.keys_a, L.vals_a);
-    MOSAIC_HIP_CHECK(hipGetLastError());
-    size_t sb = L.sort_bytes;
-    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
-                                               L.vals_b, static_cast<unsigned int>(atom_count),
-                                               0, 64, st));
     SpeciesSplit split;
     split.n_species = n_species;
     split.atom_start[0] = 0;
@@ -1115,6 +1319,43 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         split.atom_start[s + 1] = split.atom_start[s] + counts[s];
         split.tile_start[s + 1] =
             split.tile_start[s] + static_cast<int>((counts[s] + TILE - 1) / TILE);
+    }
+    const unsigned int n32 = static_cast<unsigned int>(atom_count);
+    const int nb = blocks_for(atom_count, KD_THREADS);
+    hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
+    // group by species (stable), then bisect each species level by level
+    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
+                       L.keys_a, L.vals_a);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    size_t sb = L.sort_bytes;
+    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
+                                               L.vals_b, n32, 0, 4, st));
+    std::vector<uint32_t> plan;       // alive until the synchronisation below
+    if (L.kd_levels > 0) {
+        kd_plan(n_species, counts, L.kd_levels, plan);
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_split, plan.data(), plan.size() * sizeof(uint32_t),
+                                        hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_kd_root, dim3(nb), dim3(KD_THREADS), 0, st, atom_count, split, L.seg);
+        int sbits = 0;
+        while ((1 << sbits) < n_species) ++sbits;
+        for (int l = 0; l < L.kd_levels; ++l) {
+            const int64_t nodes = static_cast<int64_t>(n_species) << l;
+            const uint32_t* cut = L.kd_split + kd_split_count(n_species, l);
+            hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0, st,
+                               L.kd_box, 6 * nodes);
+            hipLaunchKernelGGL(k_kd_box, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg,
+                               cut, atom_count, L.kd_box);
+            hipLaunchKernelGGL(k_kd_keys, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b,
+                               L.seg, cut, L.kd_box, atom_count, L.keys_a, L.vals_a);
+            MOSAIC_HIP_CHECK(hipGetLastError());
+            sb = L.sort_bytes;
+            MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b,
+                                                       L.vals_a, L.vals_b, n32, 0,
+                                                       KD_QBITS + l + sbits, st));
+            if (l + 1 < L.kd_levels)
+                hipLaunchKernelGGL(k_kd_next, dim3(nb), dim3(KD_THREADS), 0, st, cut, atom_count,
+                                   L.seg);
+        }
     }
     const double K = k * INV_2PI_HI;
     if (L.n_tiles > 0)
@@ -1239,7 +1480,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // order of the origin phase below 1e-8 rev, the row line fit below 1e-6
     // rev and the step phasor inside its Taylor range.
     const double omax = stage_stats[1] + tmax;
-    const bool rec_geom = !use_second && omax / rho_min <= 1e-6 &&
+    const bool rec_geom = omax / rho_min <= 1e-6 &&
                           K * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
                           6.2831853 * du1 * radius_wl <= 0.05;
     int kp = 0;
@@ -1268,8 +1509,12 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         A.npix = npix;
         A.nbx = (Ny + kp - 1) / kp;
         A.K = K;
-        e = kp == 8 ? launch_rec<8>(A, L.n_slices, n_frames, st)
-                    : launch_rec<4>(A, L.n_slices, n_frames, st);
+        if (kp == 8)
+            e = use_second ? launch_rec<8, true>(A, L.n_slices, n_frames, st)
+                           : launch_rec<8, false>(A, L.n_slices, n_frames, st);
+        else
+            e = use_second ? launch_rec<4, true>(A, L.n_slices, n_frames, st)
+                           : launch_rec<4, false>(A, L.n_slices, n_frames, st);
     } else {
         SumArgs A;
         A.atoms = L.atoms;
@@ -1312,7 +1557,9 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         hipLaunchKernelGGL(k_reduce<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, st,
                            L.partial, L.n_slices, npix, n_frames, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
-    if (path_used) *path_used = path == 2 ? (kp == 8 ? 28 : 24) : (use_second ? 12 : 11);
+    if (path_used)
+        *path_used = path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
+                               : (use_second ? 12 : 11);
     return MOSAIC_OK;
 }
 
@@ -1367,9 +1614,13 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     }
     const double Kk = k * INV_2PI_HI;
     if (!(Kk > 0.0) || !(rho_min > 0.0)) return MOSAIC_ERR_ARGS;
+    const double rad_m = radius_wl / Kk;
+    if (rad_m / rho_min > 1e-6 || Kk * rad_m * rad_m * rad_m / (rho_min * rho_min) > 1e-7)
+        return MOSAIC_ERR_UNSUPPORTED;
     const double second = radius_wl * radius_wl / (2.0 * Kk * rho_min);
+    const bool use_second = second > 1e-7;
     const double omax = stage_stats[1] + tmax;
-    const bool ok = second <= 1e-7 && omax / rho_min <= 1e-6 &&
+    const bool ok = omax / rho_min <= 1e-6 &&
                     Kk * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
                     6.2831853 * du1 * radius_wl <= 0.05;
     int kp = 0;
@@ -1386,6 +1637,7 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     A.theta0 = L.px.theta0;
     A.gtab = L.px.gtab;
     A.gin = reinterpret_cast<const float2*>(grad_field);
+    A.amax = L.amax_blocks;
     A.grad = grad_pos;
     A.n_frames = n_frames;
     A.Ny = Ny;
@@ -1393,10 +1645,18 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     A.npix = Ny * Nz;
     A.nbx = (Ny + kp - 1) / kp;
     A.K = Kk;
-    if (kp == 16)
-        hipLaunchKernelGGL(k_adjoint<16>, dim3(L.n_tiles), dim3(SUM_THREADS), 0, st, A);
-    else
-        hipLaunchKernelGGL(k_adjoint<8>, dim3(L.n_tiles), dim3(SUM_THREADS), 0, st, A);
+    const int64_t n_gin = 2 * static_cast<int64_t>(n_frames) * Ny * Nz;
+    const int64_t n_gtab = 2 * static_cast<int64_t>(n_species) * Ny * Nz;
+    hipLaunchKernelGGL(k_amax, dim3(AMAX_BLOCKS), dim3(256), 0, st, grad_field, n_gin,
+                       reinterpret_cast<const float*>(L.px.gtab), n_gtab, L.amax_blocks);
+    const dim3 grid(L.n_tiles), blk(SUM_THREADS);
+    if (kp == 16) {
+        if (use_second) hipLaunchKernelGGL((k_adjoint<16, true>), grid, blk, 0, st, A);
+        else hipLaunchKernelGGL((k_adjoint<16, false>), grid, blk, 0, st, A);
+    } else {
+        if (use_second) hipLaunchKernelGGL((k_adjoint<8, true>), grid, blk, 0, st, A);
+        else hipLaunchKernelGGL((k_adjoint<8, false>), grid, blk, 0, st, A);
+    }
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

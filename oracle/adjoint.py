@@ -4,8 +4,8 @@ For a real loss L(A) of the kinematic field and the PyTorch gradient
 convention g = dL/dRe(A) + i dL/dIm(A):
 
 * ``atom_terms`` -- c_j(f, p), one atom's term of A for every pose and pixel
-  (the C float64 restatement of compute_scattering_cffi with one atom, lab
-  position R_f (s_j + u_j) + T_f evaluated in float64);
+  (the C restatement of compute_scattering_cffi with one atom, long double by
+  default; lab position R_f (s_j + u_j) + T_f evaluated in float64);
 * ``reference_gradient`` -- dL/du_j = sum_{f,p} Re(conj(g) i c_j) dphi/du_j with
   the exact per-pair phase gradient dphi/du = k 1e-10 R^T (x_hat - u_hat)
   [per Angstrom], in float64;
@@ -19,8 +19,14 @@ import numpy as np
 from . import kinematic as O
 
 
-def atom_terms(p_sample, f0p, f0z, fp, fpp, amp, rots, trans, pc, k, pol, threads=0):
-    """(F, npix) complex128 terms of one atom at sample-frame position p (Angstrom)."""
+def atom_terms(p_sample, f0p, f0z, fp, fpp, amp, rots, trans, pc, k, pol, threads=0,
+               precision="f80"):
+    """(F, npix) complex128 terms of one atom at sample-frame position p (Angstrom).
+
+    The phase k (x + |T - r|) is ~5e10 rad at 1 m, so FP64 leaves ~1e-5 rad of
+    rounding in every term: enough to swamp a finite difference over 1e-3 A
+    (a 3e-3 rad change).  The default evaluates it in x87 long double.
+    """
     F = rots.shape[0]
     out = np.zeros((F, pc.shape[1]), dtype=np.complex128)
     for f in range(F):
@@ -29,7 +35,7 @@ def atom_terms(p_sample, f0p, f0z, fp, fpp, amp, rots, trans, pc, k, pol, thread
         out[f] = O.kin_sum_c(lab[None, :], f0p[None, :], np.array([f0z]), np.array([fp]),
                              np.array([fpp]), np.array([amp.real]), np.array([amp.imag]),
                              pc[0], pc[1], pc[2], k, False, pc.shape[1], 1, True, pol[0],
-                             pol[1:], False, precision="f64", threads=threads)
+                             pol[1:], False, precision=precision, threads=threads)
     return out
 
 
