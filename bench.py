@@ -226,7 +226,7 @@ def bench_scan(steps, warmup):
 
 def bench_adjoint(steps):
     """Displacement gradient: 1,000,188-atom FCC Au (63^3 cells) with a Gaussian
-    strain bump, 16 poses x 256^2 pixels, L = sum (|A|^2 - I_t)^2."""
+    strain bump, 16 poses x 256^2 pixels, L = sum ((|A|^2 - I_t) / max I_t)^2."""
     import torch
     from mosaic_amd import workloads as W
     from mosaic_amd.fitting import KinematicModel
@@ -244,6 +244,7 @@ def bench_adjoint(steps):
                        np.stack(trs))
     with torch.no_grad():
         It = (m.field(torch.from_numpy(w["u_target"]).to(dev)).abs() ** 2)
+    scale = float(It.max())
     u = torch.from_numpy(w["u"]).to(dev).requires_grad_(True)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     tf, tb = [], []
@@ -251,7 +252,7 @@ def bench_adjoint(steps):
         u.grad = None
         ev[0].record()
         A = m.field(u)
-        L = torch.sum((A.abs() ** 2 - It) ** 2)
+        L = torch.sum(((A.abs() ** 2 - It) / scale) ** 2)
         ev[1].record()
         L.backward()
         ev[2].record()
