@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-pixels", type=int, default=48)
-    ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows"])
+    ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows", "mma"])
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -222,6 +222,12 @@ def bench_scan(steps, warmup):
     return {"workload": w["name"], "atoms": n, "frames": F, "pixels": [Ny, Nz],
             "ms_per_scan": ms, "frames_per_s": F / (ms * 1e-3),
             "pairs_per_s": pairs / (ms * 1e-3), "kernel_path": sg.path_used}
+
+
+def kernel_name(path_used):
+    """Sum kernel behind a mosaic_kin_sum path code."""
+    base = path_used % 100
+    return {32: "k_sum_mma", 24: "k_sum_rec", 28: "k_sum_rec"}.get(base, "k_sum")
 
 
 def bench_adjoint(steps):
@@ -417,7 +423,7 @@ def main():
                    "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "k_sum_rec" if path_used[0] % 100 in (24, 28) else "k_sum",
+                     "kernel": kernel_name(path_used[0]),
                      "kernel_path": path_used[0], "kernel_ms": ksum_ms,
                      "kernel_pairs_per_s": pairs / (ksum_ms * 1e-3)},
     }

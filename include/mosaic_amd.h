@@ -47,7 +47,8 @@ typedef struct {
     int remove_forward;     /* subtract f0(0)+f'+if'' below the pixel's Q cut */
     int apply_pol;          /* scale by sqrt(P) */
     int apply_decay;        /* scale by R0 / r */
-    int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence */
+    int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence,
+                               3 matrix-core patches */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
 } mosaic_kin_opts_t;
@@ -89,9 +90,11 @@ int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* specie
  * rho0 = |T_p|, frac(rho0 / lambda) in double-double, the forward-removal
  * Q cut (Beam.py:718-746), the unit vector and 1/(2 rho0), and per species
  * the factor (f0 + f' + i f'') r_e sqrt(P) in the pixel's direction.
- * Returns pixel_geom (host out [3]): the smallest pixel distance [m] and the
- * largest first and second differences of the pixel direction along a row;
- * the call synchronizes the stream for them.  Must precede mosaic_kin_sum
+ * Returns pixel_geom (host out [4]): the smallest pixel distance [m], the
+ * largest first and second differences of the pixel direction along a row,
+ * and the largest deviation of the pixel direction from the matrix-core
+ * sum's per-patch quadratic model; the call synchronizes the stream for
+ * them.  Must precede mosaic_kin_sum
  * on the same workspace.  `coords` is device (3, Ny*Nz) float64 metres.
  */
 int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host */,
@@ -108,15 +111,18 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * mosaic_fast launch (Beam.py:8246-8632, 9336-9391): same per-pair
  * semantics, with form factor, polarization and decay evaluated once per
  * (tile, pixel) and the phase split into an FP64 (tile, pixel) origin term
- * and an FP32 per-atom offset.  Two kernels: per-tile factors (exact
+ * and an FP32 per-atom offset.  Three kernels: per-tile factors (exact
  * distance per tile and pixel, optional second-order offset term) and, in
  * the far field, a row recurrence (per-pixel factors, far-field origin phase,
- * one sincos per atom per KP pixels); opts->path = 0 picks the recurrence
- * whenever its error bounds hold.  `stage_stats` and `pixel_geom` are the
- * outputs of mosaic_kin_stage and mosaic_kin_pixels; `path_used` (host out,
- * may be NULL) reports 11/12 (per-tile, first/second order) or 24/28 (row
- * recurrence over 4/8 pixels; 124/128 with the atoms' second-order phase).
- * Stream-ordered, no host synchronization.
+ * one sincos per atom per KP pixels) and a matrix-core form over 16 x 16
+ * pixel patches (the tile sum as a complex product of per-row and
+ * per-column phasors on v_mfma_f32_32x32x2_f32); opts->path = 0 picks the
+ * matrix-core form, else the recurrence, whenever their error bounds hold.
+ * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
+ * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
+ * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels) or
+ * 32 (matrix-core), plus 100 for the recurrence and matrix-core forms with
+ * the atoms' second-order phase.  Stream-ordered, no host synchronization.
  * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
  * close for the split's error bound -- the reference's "decline"
  * (Beam.py:9466-9468).
@@ -129,7 +135,7 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const float* trans /* host */, int Ny, int Nz,
                    const double* coords, void* out, int out_is_c128,
                    const double* stage_stats /* host [2] */,
-                   const double* pixel_geom /* host [3] */, void* workspace,
+                   const double* pixel_geom /* host [4] */, void* workspace,
                    size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
 
 /*
@@ -147,7 +153,7 @@ int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
                        const float* rot /* host */, const float* trans /* host */, int Ny,
                        int Nz, const double* stage_stats /* host [2] */,
-                       const double* pixel_geom /* host [3] */, const float* grad_field,
+                       const double* pixel_geom /* host [4] */, const float* grad_field,
                        float* grad_pos, void* workspace, size_t workspace_bytes,
                        mosaic_stream_t stream);
 

@@ -47,12 +47,15 @@ constexpr uint32_t NO_SPLIT = 0xFFFFFFFFu;   // k-d node that does not split
 constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
+constexpr int MMA_P = 16;            // pixel patch edge of the matrix-core sum
+constexpr int N_GEOM = 4;            // pixel geometry statistics (mosaic_kin_pixels)
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
 
 struct AtomRec {      // 32 B, read whole by one scalar load
-    float sx, sy, sz, s2;
-    float wr, wi, pad0, pad1;   // pad0 carries the original atom index (bits)
+    float sx, sy, sz, s2;       // offset from the tile origin [wavelengths], |offset|^2
+    float mag, ph;              // incident amplitude |w| and arg(w) / 2 pi
+    float pad0, pad1;           // pad0 carries the original atom index (bits)
 };
 
 struct TileRec {      // 32 B
@@ -130,8 +133,8 @@ void take_pixel_tables(Carver& c, int npix, int n_species, PixelTables& P) {
     P.qcut = c.take<double>(npix);
     P.u0h = c.take<double4>(npix);
     P.gtab = c.take<float2>(static_cast<size_t>(n_species) * npix);
-    P.stat_blocks = c.take<double>(3 * static_cast<size_t>(blocks_for(npix, PRO_THREADS)));
-    P.stats = c.take<double>(3);
+    P.stat_blocks = c.take<double>(N_GEOM * static_cast<size_t>(blocks_for(npix, PRO_THREADS)));
+    P.stats = c.take<double>(N_GEOM);
 }
 
 int count_tiles(int n_species, const int64_t* counts) {
@@ -277,6 +280,35 @@ __device__ __forceinline__ void pixel_dir(const double* cx, const double* cy, co
     uz = tz * inv;
 }
 
+// Slope g and curvature h of the quadratic f(a) - f(0) = g a + h a^2 through
+// the values d1 = f(a1) - f(0) and d2 = f(a2) - f(0), a1 <= 0 <= a2; a line
+// when a2 = 0 (two points), zero for a single point.
+__device__ __forceinline__ void quad_fit(double d1, double d2, int a1, int a2, double& g,
+                                         double& h) {
+    if (a1 == 0) { g = 0.0; h = 0.0; return; }
+    if (a2 == 0) { g = d1 / a1; h = 0.0; return; }
+    h = (d2 / a2 - d1 / a1) / (a2 - a1);
+    g = d1 / a1 - h * a1;
+}
+
+// The matrix-core sum's model of a direction field over one MMA_P x MMA_P
+// pixel patch: f(a, b) = f_c + a gy + a^2 hy + b gz + b^2 hz, a and b the
+// offsets from the patch's middle pixel (my, mz) = (vy / 2, vz / 2), fitted
+// through the middle pixel and the ends of the middle row and column.
+struct PatchGeom {
+    int y0, z0, vy, vz, my, mz;
+};
+__device__ __forceinline__ PatchGeom patch_of(int py, int pz, int Ny, int Nz) {
+    PatchGeom g;
+    g.y0 = py * MMA_P;
+    g.z0 = pz * MMA_P;
+    g.vy = min(MMA_P, Ny - g.y0);
+    g.vz = min(MMA_P, Nz - g.z0);
+    g.my = g.vy / 2;
+    g.mz = g.vz / 2;
+    return g;
+}
+
 // Per pixel:
 //  * rho0 = |T| and Theta0 = frac(K rho0) in double-double (K = k / 2 pi);
 //  * the forward-removal cut-off from the right/upper neighbours
@@ -285,8 +317,9 @@ __device__ __forceinline__ void pixel_dir(const double* cx, const double* cy, co
 //    (tile, pixel) origin phase;
 //  * per species, G = (f0(Q) + f' + i f'' [- forward term]) r_e sqrt(P) in
 //    the direction u0 (Beam.py:748-807 with the atom at the lab origin);
-//  * block statistics: nearest pixel distance and the largest first and
-//    second differences of u0 between neighbours along a row.
+//  * block statistics: nearest pixel distance, the largest first and second
+//    differences of u0 between neighbours along a row, and the largest
+//    deviation of u0 from the patch model of the matrix-core sum.
 __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int Ny, int Nz, const PixOpts O, PixelTables T) {
@@ -294,7 +327,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const int npix = Ny * Nz;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const double k = O.k;
-    double rmin = 1e300, du1 = 0.0, du2 = 0.0;
+    double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
         // |T|^2 in double-double
@@ -377,29 +410,51 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
                 du2 = sqrt(ax * ax + ay * ay + az * az);
             }
         }
+
+        // deviation from the patch model
+        const PatchGeom G = patch_of(ix / MMA_P, iy / MMA_P, Ny, Nz);
+        const int pm = (G.z0 + G.mz) * Ny + G.y0 + G.my;
+        double mc[3], e1[3], e2[3], e3[3], e4[3];
+        pixel_dir(cx, cy, cz, pm, mc[0], mc[1], mc[2]);
+        pixel_dir(cx, cy, cz, pm - G.my, e1[0], e1[1], e1[2]);
+        pixel_dir(cx, cy, cz, pm - G.my + G.vy - 1, e2[0], e2[1], e2[2]);
+        pixel_dir(cx, cy, cz, pm - G.mz * Ny, e3[0], e3[1], e3[2]);
+        pixel_dir(cx, cy, cz, pm + (G.vz - 1 - G.mz) * Ny, e4[0], e4[1], e4[2]);
+        const int oa = ix - G.y0 - G.my, ob = iy - G.z0 - G.mz;
+        const double u0v[3] = {ux, uy, uz};
+        double dev2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double gy, hy, gz, hz;
+            quad_fit(e1[d] - mc[d], e2[d] - mc[d], -G.my, G.vy - 1 - G.my, gy, hy);
+            quad_fit(e3[d] - mc[d], e4[d] - mc[d], -G.mz, G.vz - 1 - G.mz, gz, hz);
+            const double model = mc[d] + oa * (gy + oa * hy) + ob * (gz + ob * hz);
+            dev2 += (u0v[d] - model) * (u0v[d] - model);
+        }
+        dpatch = sqrt(dev2);
     }
     rmin = block_min(rmin, lds);
     du1 = block_max(du1, lds);
     du2 = block_max(du2, lds);
+    dpatch = block_max(dpatch, lds);
     if (threadIdx.x == 0) {
-        T.stat_blocks[3 * blockIdx.x] = rmin;
-        T.stat_blocks[3 * blockIdx.x + 1] = du1;
-        T.stat_blocks[3 * blockIdx.x + 2] = du2;
+        T.stat_blocks[N_GEOM * blockIdx.x] = rmin;
+        T.stat_blocks[N_GEOM * blockIdx.x + 1] = du1;
+        T.stat_blocks[N_GEOM * blockIdx.x + 2] = du2;
+        T.stat_blocks[N_GEOM * blockIdx.x + 3] = dpatch;
     }
 }
 
 __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double lds[16];
-    double a = 1e300, b = 0.0, c = 0.0;
+    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        a = fmin(a, in[3 * i]);
-        b = fmax(b, in[3 * i + 1]);
-        c = fmax(c, in[3 * i + 2]);
+        v[0] = fmin(v[0], in[N_GEOM * i]);
+        for (int q = 1; q < N_GEOM; ++q) v[q] = fmax(v[q], in[N_GEOM * i + q]);
     }
-    a = block_min(a, lds);
-    b = block_max(b, lds);
-    c = block_max(c, lds);
-    if (threadIdx.x == 0) { out[0] = a; out[1] = b; out[2] = c; }
+    v[0] = block_min(v[0], lds);
+    for (int q = 1; q < N_GEOM; ++q) v[q] = block_max(v[q], lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < N_GEOM; ++q) out[q] = v[q];
 }
 
 // ------------------------------------------------------------------- staging
@@ -603,10 +658,10 @@ __global__ __launch_bounds__(TILE) void k_tiles(
         rec.pad0 = __uint_as_float(idx);     // original atom index (adjoint scatter)
         if (amp) {
             const float2 w = amp[idx];
-            rec.wr = w.x;
-            rec.wi = w.y;
+            rec.mag = sqrtf(w.x * w.x + w.y * w.y);
+            rec.ph = atan2f(w.y, w.x) * 0.15915494309189535f;
         } else {
-            rec.wr = 1.f;
+            rec.mag = 1.f;
         }
         rad = sqrtf(rec.s2);
     }
@@ -756,13 +811,11 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum(const SumArgs A) {
                     const float u = xsd - psi;          // u_hat . sigma (sample frame)
                     psi = fmaf(hh[i], fmaf(-u, u, a.s2), psi);
                 }
-                const float r = __builtin_amdgcn_fractf(psi);
+                const float r = __builtin_amdgcn_fractf(psi + a.ph);
                 const float sn = __builtin_amdgcn_sinf(r);
                 const float cs = __builtin_amdgcn_cosf(r);
-                S_r[i] = fmaf(a.wr, cs, S_r[i]);
-                S_r[i] = fmaf(-a.wi, sn, S_r[i]);
-                S_i[i] = fmaf(a.wr, sn, S_i[i]);
-                S_i[i] = fmaf(a.wi, cs, S_i[i]);
+                S_r[i] = fmaf(a.mag, cs, S_r[i]);
+                S_i[i] = fmaf(a.mag, sn, S_i[i]);
             }
         }
 #pragma unroll
@@ -902,11 +955,11 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum_rec(const RecArgs A) {
                 const float u = fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx)) - fmaf(hmid, d, psi);
                 psi = fmaf(hh, fmaf(-u, u, a.s2), psi);
             }
-            const float r = __builtin_amdgcn_fractf(psi);
+            const float r = __builtin_amdgcn_fractf(psi + a.ph);
             const float sn = __builtin_amdgcn_sinf(r);
             const float cs = __builtin_amdgcn_cosf(r);
-            float er = fmaf(a.wr, cs, -a.wi * sn);
-            float ei = fmaf(a.wr, sn, a.wi * cs);
+            float er = a.mag * cs;
+            float ei = a.mag * sn;
             const float x = d * TWO_PI_F;
             const float x2 = x * x;
             const float ar = fmaf(x2, fmaf(x2, 0.041666668f, -0.5f), 1.0f);
@@ -953,6 +1006,195 @@ hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t 
     const int lanes = A.Nz * A.nbx;
     dim3 grid(blocks_for(lanes, SUM_THREADS), n_slices, n_frames);
     hipLaunchKernelGGL((k_sum_rec<KP, SECOND>), grid, dim3(SUM_THREADS), 0, st, A);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- matrix-core sum
+// Far-field sum on the matrix cores.  One wave owns a patch of MMA_P x MMA_P
+// pixels (a along the row, b down the columns, both offsets from the patch's
+// middle pixel).  Across the patch the direction is modelled per axis to
+// second order,
+//     v(a, b) = vc + a gy + a^2 hy + b gz + b^2 hz,
+// exact along the middle row and column (the host bounds the deviation
+// elsewhere from the measured pixel grid), so per atom the phasor separates:
+//     e^{2 pi i v.sigma} = A(a) B(b),
+//     A(a) = w e^{2 pi i (vc.sigma + a gy.sigma + a^2 hy.sigma)},
+//     B(b) = e^{2 pi i (b gz.sigma + b^2 hz.sigma)},
+// and the tile sum S(a, b) = sum_j A_j(a) B_j(b) is a complex matrix product
+// over the atoms.  Stacking Re and Im turns it into one real 32 x 32 product,
+//     [Re A; Im A] (32 x atoms) . [Re B, Im B] (atoms x 32),
+// from which S = (C_rr - C_ii) + i (C_ri + C_ir): v_mfma_f32_32x32x2_f32 with
+// two atoms per instruction.  Each lane forms one element of each operand
+// with one v_cos (an Im part is the cos of the phase less a quarter turn).
+// Per (tile, pixel) the FP64 origin phase and the factor G are those of the
+// row recurrence.
+struct MmaArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float2* gtab;
+    double2* partial;
+    int n_tiles;
+    int tiles_per_slice;
+    int n_slices;
+    int Ny;
+    int Nz;
+    int npix;
+    int pby;          // patches along a row
+    int n_patches;
+    double K;         // 1 / lambda
+};
+
+constexpr int MMA_WAVES = 4;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <bool SECOND>
+__global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int patch = blockIdx.x * MMA_WAVES + wave;
+    if (patch >= A.n_patches) return;          // whole wave; no block barriers below
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+    const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
+    const int Ny = A.Ny;
+
+    // operand roles: atom kk of each pair; A row / B column q = (Re|Im, offset)
+    const int kk = lane >> 5;
+    const int q = lane & 31;
+    const float quarter = (q & 16) ? -0.25f : 0.f;
+    const float fa = static_cast<float>((q & 15) - P.my);
+    const float fb = static_cast<float>((q & 15) - P.mz);
+    // result roles: D register r (< 8) of this lane holds row m(r), column
+    // q; its partner lane ^ 16 holds the other half of the same pixels.
+    const bool re_lane = (lane & 16) == 0;
+    const int n = lane & 15;
+    const int mbase = 4 * (lane >> 5);
+    // pixel of D register r: row m(r) = 8 (r / 4) + mbase + r % 4, column n
+    const int prow = (P.z0 + min(n, P.vz - 1)) * Ny + P.y0;
+    auto pix_of = [&](int r) { return prow + min(8 * (r >> 2) + mbase + (r & 3), P.vy - 1); };
+    double acc[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[r] = 0.0;
+
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    const float xs0 = static_cast<float>(F.xs[0]);
+    const float xs1 = static_cast<float>(F.xs[1]);
+    const float xs2 = static_cast<float>(F.xs[2]);
+    const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+    const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                         pmid + (P.vz - 1 - P.mz) * Ny};
+
+    for (int t = t0; t < t1; ++t) {
+        const TileRec tr = A.tiles[t];
+        // Per-pixel table loads stay inside the loop: tying their index to
+        // the tile record (count <= 256, so the shift is 0) stops the
+        // compiler from hoisting ~150 registers of tables out of it.
+        const int tie = tr.count >> 16;
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+
+        // exact v (sample frame) at the middle pixel and the ends of the
+        // middle row and column, then the per-axis quadratic model
+        double vs[5][3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double4 u = A.u0h[samp[k] + tie];
+            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+            const double h2 = 2.0 * u.w;                     // 1 / rho0
+            const double wx = 1.0 - u.x + (Ox - uo * u.x) * h2;
+            const double wy = -u.y + (Oy - uo * u.y) * h2;
+            const double wz = -u.z + (Oz - uo * u.z) * h2;
+            vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
+            vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
+            vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
+        }
+        float vc[3], gy[3], hy[3], gz[3], hz[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double g, h;
+            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, g, h);
+            gy[d] = static_cast<float>(g);
+            hy[d] = static_cast<float>(h);
+            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, g, h);
+            gz[d] = static_cast<float>(g);
+            hz[d] = static_cast<float>(h);
+            vc[d] = static_cast<float>(vs[0][d]);
+        }
+        const float hh = SECOND ? static_cast<float>(A.u0h[pmid + tie].w / A.K) : 0.f;
+
+        floatx16 cacc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) cacc[r] = 0.f;
+        const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
+        const int cnt = tr.count;
+        // records past the count are zero (mag = 0), so an odd count reads
+        // one; two pairs are in flight ahead of the one in use
+        AtomRec a0 = at[kk], a1 = at[2 + kk];
+#pragma unroll 1
+        for (int j = 0; j < cnt; j += 2) {
+            const AtomRec a = a0;
+            a0 = a1;
+            a1 = at[min(j + 4 + kk, TILE - 1)];
+            const float psi = fmaf(vc[2], a.sz, fmaf(vc[1], a.sy, vc[0] * a.sx));
+            const float dy = fmaf(gy[2], a.sz, fmaf(gy[1], a.sy, gy[0] * a.sx));
+            const float ey = fmaf(hy[2], a.sz, fmaf(hy[1], a.sy, hy[0] * a.sx));
+            const float dz = fmaf(gz[2], a.sz, fmaf(gz[1], a.sy, gz[0] * a.sx));
+            const float ez = fmaf(hz[2], a.sz, fmaf(hz[1], a.sy, hz[0] * a.sx));
+            float pa = fmaf(fa, fmaf(fa, ey, dy), psi);
+            if (SECOND) {
+                const float u = fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx)) - psi;
+                pa = fmaf(hh, fmaf(-u, u, a.s2), pa);
+            }
+            pa += a.ph + quarter;
+            const float pb = fmaf(fb, fmaf(fb, ez, dz), quarter);
+            const float av = a.mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
+            const float bv = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb));
+            cacc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, cacc, 0, 0, 0);
+        }
+
+        // S = (C_rr - C_ii) + i (C_ri + C_ir): the Re lane keeps Re S, its
+        // partner Im S; then both form their half of G e^{2 pi i Theta} S.
+        const float2* __restrict__ G = A.gtab + static_cast<int64_t>(tr.species) * A.npix;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const float x = __shfl_xor(cacc[r + 8], 16, 64);
+            const float v = re_lane ? cacc[r] - x : cacc[r] + x;
+            const float o = __shfl_xor(v, 16, 64);
+            const float s_re = re_lane ? v : o, s_im = re_lane ? o : v;
+            const int p = pix_of(r) + tie;
+            const double4 u = A.u0h[p];
+            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+            double th = A.theta0[p] + A.K * (Ox - uo + (O2 - uo * uo) * u.w);
+            th -= rint(th);
+            const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
+            const float sn = __builtin_amdgcn_sinf(thf);
+            const float cs = __builtin_amdgcn_cosf(thf);
+            const float2 g = G[p];
+            const float zr = g.x * cs - g.y * sn, zi = g.x * sn + g.y * cs;
+            acc[r] += static_cast<double>(re_lane ? zr * s_re - zi * s_im : zr * s_im + zi * s_re);
+        }
+    }
+    double* out = reinterpret_cast<double*>(
+        A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix);
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+        if (8 * (r >> 2) + mbase + (r & 3) < P.vy && n < P.vz)
+            out[2 * static_cast<int64_t>(pix_of(r)) + (re_lane ? 0 : 1)] = acc[r];
+}
+
+template <bool SECOND>
+hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t st) {
+    dim3 grid((A.n_patches + MMA_WAVES - 1) / MMA_WAVES, n_slices, n_frames);
+    hipLaunchKernelGGL((k_sum_mma<SECOND>), grid, dim3(64 * MMA_WAVES), 0, st, A);
     return hipGetLastError();
 }
 
@@ -1185,9 +1427,12 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
         const double scale = -2.0 * PI * A.K * 1e-10 /
                              (static_cast<double>(sg) * static_cast<double>(sG));
         const uint32_t idx = __float_as_uint(a.pad0);
+        double wi, wr;
+        sincospi(2.0 * static_cast<double>(a.ph), &wi, &wr);
+        wr *= a.mag;
+        wi *= a.mag;
         for (int c = 0; c < 3; ++c) {
-            const double g = scale * (static_cast<double>(a.wr) * Z[2 * c + 1] +
-                                      static_cast<double>(a.wi) * Z[2 * c]);
+            const double g = scale * (wr * Z[2 * c + 1] + wi * Z[2 * c]);
             A.grad[3 * static_cast<int64_t>(idx) + c] += static_cast<float>(g);
         }
     }
@@ -1414,7 +1659,7 @@ extern "C" int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_
                        coords + 2 * (int64_t)npix, Ny, Nz, O, T);
     hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, st, T.stat_blocks, pb, T.stats);
     MOSAIC_HIP_CHECK(hipGetLastError());
-    MOSAIC_HIP_CHECK(hipMemcpyAsync(pixel_geom, T.stats, 3 * sizeof(double),
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(pixel_geom, T.stats, N_GEOM * sizeof(double),
                                     hipMemcpyDeviceToHost, st));
     MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
     return MOSAIC_OK;
@@ -1443,7 +1688,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
         !pixel_geom)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 2)
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 3)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
@@ -1458,6 +1703,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // distance (tiles move rigidly with the sample pose).
     const double radius_wl = stage_stats[0];
     const double rho_min = pixel_geom[0], du1 = pixel_geom[1], du2 = pixel_geom[2];
+    const double dpatch = pixel_geom[3];
     const double rad_m = radius_wl / K;
     if (!(rho_min > 0.0) || rad_m / rho_min > 1e-6) return MOSAIC_ERR_UNSUPPORTED;
     const double third = K * rad_m * rad_m * rad_m / (rho_min * rho_min);
@@ -1480,19 +1726,42 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // order of the origin phase below 1e-8 rev, the row line fit below 1e-6
     // rev and the step phasor inside its Taylor range.
     const double omax = stage_stats[1] + tmax;
-    const bool rec_geom = omax / rho_min <= 1e-6 &&
-                          K * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
-                          6.2831853 * du1 * radius_wl <= 0.05;
+    const bool far = omax / rho_min <= 1e-6 &&
+                     K * omax * omax * omax / (rho_min * rho_min) <= 1e-8;
+    const bool rec_geom = far && 6.2831853 * du1 * radius_wl <= 0.05;
     int kp = 0;
     if (rec_geom && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
     else if (rec_geom && rec_curv_bound(4, du2, radius_wl) <= 1e-6) kp = 4;
-    int path = opts->path == 0 ? (kp ? 2 : 1) : opts->path;
-    if (path == 2 && kp == 0) return MOSAIC_ERR_UNSUPPORTED;
+    // matrix-core sum: the patch model's deviation from the pixel grid,
+    // times the largest offset, below 1e-6 rev
+    const bool mma_ok = far && dpatch * radius_wl <= 1e-6;
+    int path = opts->path == 0 ? (mma_ok ? 3 : (kp ? 2 : 1)) : opts->path;
+    if ((path == 2 && kp == 0) || (path == 3 && !mma_ok)) return MOSAIC_ERR_UNSUPPORTED;
 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
     hipError_t e;
-    if (path == 2) {
+    if (path == 3) {
+        MmaArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.u0h = L.px.u0h;
+        A.theta0 = L.px.theta0;
+        A.gtab = L.px.gtab;
+        A.partial = L.partial;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.Ny = Ny;
+        A.Nz = Nz;
+        A.npix = npix;
+        A.pby = (Ny + MMA_P - 1) / MMA_P;
+        A.n_patches = A.pby * ((Nz + MMA_P - 1) / MMA_P);
+        A.K = K;
+        e = use_second ? launch_mma<true>(A, L.n_slices, n_frames, st)
+                       : launch_mma<false>(A, L.n_slices, n_frames, st);
+    } else if (path == 2) {
         RecArgs A;
         A.atoms = L.atoms;
         A.tiles = L.tiles;
@@ -1558,8 +1827,9 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
                            L.partial, L.n_slices, npix, n_frames, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
     if (path_used)
-        *path_used = path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
-                               : (use_second ? 12 : 11);
+        *path_used = path == 3   ? 32 + (use_second ? 100 : 0)
+                     : path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
+                                 : (use_second ? 12 : 11);
     return MOSAIC_OK;
 }
 

@@ -36,13 +36,14 @@ def species_table(consts):
     return arr
 
 
-PATHS = {"auto": 0, "tile": 1, "rows": 2}
+PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3}
 
 
 def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
              apply_decay=False, path="auto"):
-    """mosaic_kin_opts_t; ``path`` is "auto", "tile" (per-tile factors) or
-    "rows" (row recurrence, declines when its bounds do not hold)."""
+    """mosaic_kin_opts_t; ``path`` is "auto", "tile" (per-tile factors),
+    "rows" (row recurrence) or "mma" (matrix-core patches); the last two
+    decline when their bounds do not hold."""
     o = KinOpts()
     o.k = float(k)
     o.remove_forward = int(bool(remove_forward))
@@ -126,12 +127,13 @@ def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=N
 
 def kin_pixels(staged, species_tab, opts, coords, stream=None):
     """Pixel prologue into the staged sample's workspace; returns
-    (rho_min [m], max row step of the pixel direction, max row second difference)."""
+    (rho_min [m], max row step of the pixel direction, max row second
+    difference, max deviation from the matrix-core patch model)."""
     npix = staged.Ny * staged.Nz
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
         raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
     coords = coords.contiguous()
-    geom = (ctypes.c_double * 3)()
+    geom = (ctypes.c_double * 4)()
     check(_lib.lib().mosaic_kin_pixels(len(species_tab), ctypes.cast(species_tab, ctypes.c_void_p),
                                        ctypes.byref(opts), staged.Ny, staged.Nz, _ptr(coords),
                                        _ptr(staged.workspace), int(staged.workspace.numel()),
