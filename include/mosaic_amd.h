@@ -117,7 +117,7 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * one sincos per atom per KP pixels) and a matrix-core form over 16 x 16
  * pixel patches (the tile sum as a complex product of per-row and
  * per-column phasors on v_mfma_f32_32x32x2_f32); opts->path = 0 picks the
- * matrix-core form, else the recurrence, whenever their error bounds hold.
+ * recurrence, else the matrix-core form, whenever their error bounds hold.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels) or

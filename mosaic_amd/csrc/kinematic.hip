@@ -1084,9 +1084,6 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
     const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
     const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
     const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
-    const float xs0 = static_cast<float>(F.xs[0]);
-    const float xs1 = static_cast<float>(F.xs[1]);
-    const float xs2 = static_cast<float>(F.xs[2]);
     const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
     const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
                          pmid + (P.vz - 1 - P.mz) * Ny};
@@ -1117,17 +1114,17 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
             vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
             vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
         }
-        float vc[3], gy[3], hy[3], gz[3], hz[3];
+        // this lane's row and column directions: A phase = va . sigma, B
+        // phase = wb . sigma (FP64 model, rounded once)
+        float va[3], wb[3], xc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double g, h;
-            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, g, h);
-            gy[d] = static_cast<float>(g);
-            hy[d] = static_cast<float>(h);
-            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, g, h);
-            gz[d] = static_cast<float>(g);
-            hz[d] = static_cast<float>(h);
-            vc[d] = static_cast<float>(vs[0][d]);
+            double gyd, hyd, gzd, hzd;
+            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gyd, hyd);
+            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gzd, hzd);
+            va[d] = static_cast<float>(vs[0][d] + fa * (gyd + fa * hyd));
+            wb[d] = static_cast<float>(fb * (gzd + fb * hzd));
+            xc[d] = static_cast<float>(F.xs[d] - vs[0][d]);    // u_hat . sigma at the middle
         }
         const float hh = SECOND ? static_cast<float>(A.u0h[pmid + tie].w / A.K) : 0.f;
 
@@ -1136,27 +1133,30 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
         for (int r = 0; r < 16; ++r) cacc[r] = 0.f;
         const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
         const int cnt = tr.count;
-        // records past the count are zero (mag = 0), so an odd count reads
-        // one; two pairs are in flight ahead of the one in use
-        AtomRec a0 = at[kk], a1 = at[2 + kk];
+        // The pair's records are wave-uniform (scalar loads, one pair in
+        // flight ahead); records past the count are zero (mag = 0), so an odd
+        // count reads one of them.
+        AtomRec n0 = at[0], n1 = at[1];
 #pragma unroll 1
         for (int j = 0; j < cnt; j += 2) {
-            const AtomRec a = a0;
-            a0 = a1;
-            a1 = at[min(j + 4 + kk, TILE - 1)];
-            const float psi = fmaf(vc[2], a.sz, fmaf(vc[1], a.sy, vc[0] * a.sx));
-            const float dy = fmaf(gy[2], a.sz, fmaf(gy[1], a.sy, gy[0] * a.sx));
-            const float ey = fmaf(hy[2], a.sz, fmaf(hy[1], a.sy, hy[0] * a.sx));
-            const float dz = fmaf(gz[2], a.sz, fmaf(gz[1], a.sy, gz[0] * a.sx));
-            const float ez = fmaf(hz[2], a.sz, fmaf(hz[1], a.sy, hz[0] * a.sx));
-            float pa = fmaf(fa, fmaf(fa, ey, dy), psi);
-            if (SECOND) {
-                const float u = fmaf(xs2, a.sz, fmaf(xs1, a.sy, xs0 * a.sx)) - psi;
-                pa = fmaf(hh, fmaf(-u, u, a.s2), pa);
+            const AtomRec a0 = n0, a1 = n1;
+            if (j + 2 < cnt) {
+                n0 = at[j + 2];
+                n1 = at[j + 3];
             }
-            pa += a.ph + quarter;
-            const float pb = fmaf(fb, fmaf(fb, ez, dz), quarter);
-            const float av = a.mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
+            const float sx = kk ? a1.sx : a0.sx;
+            const float sy = kk ? a1.sy : a0.sy;
+            const float sz = kk ? a1.sz : a0.sz;
+            const float mag = kk ? a1.mag : a0.mag;
+            float pa = fmaf(va[2], sz, fmaf(va[1], sy, va[0] * sx));
+            if (SECOND) {
+                const float s2 = kk ? a1.s2 : a0.s2;
+                const float u = fmaf(xc[2], sz, fmaf(xc[1], sy, xc[0] * sx));
+                pa = fmaf(hh, fmaf(-u, u, s2), pa);
+            }
+            pa += (kk ? a1.ph : a0.ph) + quarter;
+            const float pb = fmaf(wb[2], sz, fmaf(wb[1], sy, fmaf(wb[0], sx, quarter)));
+            const float av = mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
             const float bv = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb));
             cacc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, cacc, 0, 0, 0);
         }
@@ -1735,7 +1735,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // matrix-core sum: the patch model's deviation from the pixel grid,
     // times the largest offset, below 1e-6 rev
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6;
-    int path = opts->path == 0 ? (mma_ok ? 3 : (kp ? 2 : 1)) : opts->path;
+    int path = opts->path == 0 ? (kp ? 2 : (mma_ok ? 3 : 1)) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok)) return MOSAIC_ERR_UNSUPPORTED;
 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,

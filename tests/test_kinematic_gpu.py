@@ -169,14 +169,14 @@ def test_empty_and_ragged(device):
 
 def test_c1_full_vs_fp64_oracle(device):
     """32,000 atoms x 128^2 pixels (5.2e8 pairs) against the FP64 restatement,
-    all three kernels; the automatic choice is the matrix-core form here."""
+    all three kernels; the automatic choice is the row recurrence here."""
     w = W.fcc_au_bragg111(n=20, npix=128, seed=0)
     R = oracle_field(w, "f64")
     for path in ("tile", "rows", "mma", "auto"):
         A = gpu_field(w, path=path)
         e, ei = report(f"C1 full {path} vs f64", A, R)
         assert e <= TOL and ei <= TOL
-    assert w["beam"]._last_path % 100 == 32
+    assert w["beam"]._last_path % 100 in (24, 28)
 
 
 def test_rows_second_order_close_detector(device):
@@ -199,15 +199,15 @@ def test_rows_second_order_close_detector(device):
 
 
 def test_auto_path_choice(device):
-    """Far field picks the matrix-core form; a 5 cm detector with 2 mm pixels
-    bends the rows and patches too much for it and for the recurrence, and
+    """Far field picks the recurrence; a 5 cm detector with 2 mm pixels bends
+    the rows and patches too much for it and for the matrix-core form, and
     falls back to per-tile factors."""
     w = W.fcc_au_bragg111(n=4, npix=16)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
     b._kin_path = "auto"
     gpu_field(w)
     staged_path = b._last_path
-    assert staged_path == 32
+    assert staged_path % 100 in (24, 28)
     d.create_detector((16, 16), (2.0e7, 2.0e7))
     d.position_detector_absolute(5.0e8, np.rad2deg(2 * w["theta_b"]), 0.0)
     A = gpu_field(w)
