@@ -63,12 +63,13 @@ int mosaic_kin_workspace(int64_t atom_count, int n_species,
                          int n_frames, int Ny, int Nz, size_t* bytes);
 
 /*
- * Stage a sample into the workspace: sort the atoms by (species, Hilbert
- * index), cut them into 256-atom single-species tiles, and store each tile as
- * an FP64 origin plus FP32 offsets in wavelengths with the incident
- * amplitude.  Replaces the reference's per-species partition, Morton order
- * and per-chunk origins (_partition/_stage, Beam.py:9265-9319;
- * _morton_order_gpu/_chunk_origins_gpu, Beam.py:1205-1300).
+ * Stage a sample into the workspace: group the atoms by species, cut each
+ * species into 256-atom tiles by a k-d bisection (compact boxes of equal
+ * count), and store each tile as an FP64 origin plus FP32 offsets in
+ * wavelengths with the incident amplitude.  Replaces the reference's
+ * per-species partition, Morton order and per-chunk origins
+ * (_partition/_stage, Beam.py:9265-9319; _morton_order_gpu/_chunk_origins_gpu,
+ * Beam.py:1205-1300).
  *
  *   pos      device (N, 3) float32, sample frame, Angstrom
  *   species  device (N,) uint8 species codes (< n_species)
@@ -83,6 +84,23 @@ int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* specie
                      int n_frames, int Ny, int Nz, void* workspace,
                      size_t workspace_bytes, double* stage_stats,
                      mosaic_stream_t stream);
+
+/*
+ * Re-stage moved atoms without sorting: rebuild the tiles of a workspace that
+ * mosaic_kin_stage filled for the same atoms (same count, species codes and
+ * order), from new positions and amplitudes, keeping the tile assignment.
+ * For displacements small against a tile (strain fields, defects, thermal
+ * draws, fitting steps) the tiles stay compact; stage_stats reports their
+ * radius, which the sum's bounds check as usual.  The counterpart of the
+ * reference's per-step re-staging (reload, Morton order and origins per scan
+ * step, Experiment.py:305-310; Beam.py:1205-1300), minus the sort.
+ * Synchronizes the stream to return stage_stats.
+ */
+int mosaic_kin_restage(int64_t atom_count, const float* pos, const float* amp,
+                       int n_species, const int64_t* species_counts /* host */, double k,
+                       int n_frames, int Ny, int Nz, void* workspace,
+                       size_t workspace_bytes, double* stage_stats,
+                       mosaic_stream_t stream);
 
 /*
  * Pixel prologue of the kinematic sum, once per detector geometry, species
@@ -114,10 +132,11 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * and an FP32 per-atom offset.  Three kernels: per-tile factors (exact
  * distance per tile and pixel, optional second-order offset term) and, in
  * the far field, a row recurrence (per-pixel factors, far-field origin phase,
- * one sincos per atom per KP pixels) and a matrix-core form over 16 x 16
+ * one sincos per atom per KP pixels) and a matrix-core form over 32 x 32
  * pixel patches (the tile sum as a complex product of per-row and
- * per-column phasors on v_mfma_f32_32x32x2_f32); opts->path = 0 picks the
- * recurrence, else the matrix-core form, whenever their error bounds hold.
+ * per-column phasors, FP16-split MFMA with FP32 accumulation);
+ * opts->path = 0 picks the matrix-core form, else the recurrence, whenever
+ * their error bounds hold.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels) or

@@ -26,6 +26,7 @@ _ERRORS = {
 EXPORTS = (
     "mosaic_kin_workspace",
     "mosaic_kin_stage",
+    "mosaic_kin_restage",
     "mosaic_kin_pixels",
     "mosaic_kin_sum",
     "mosaic_kin_adjoint",
@@ -78,6 +79,7 @@ _ll, _f = ctypes.c_longlong, ctypes.c_float
 _SIGS = {
     "mosaic_kin_workspace": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
     "mosaic_kin_stage": [_i64, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz, _vp, _vp],
+    "mosaic_kin_restage": [_i64, _vp, _vp, _i, _vp, _d, _i, _i, _i, _vp, _sz, _vp, _vp],
     "mosaic_kin_pixels": [_i, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp, _vp],
     "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _vp, _vp,
                        _vp, _sz, _vp, _vp],

@@ -47,7 +47,7 @@ constexpr uint32_t NO_SPLIT = 0xFFFFFFFFu;   // k-d node that does not split
 constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
-constexpr int MMA_P = 16;            // pixel patch edge of the matrix-core sum
+constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int N_GEOM = 4;            // pixel geometry statistics (mosaic_kin_pixels)
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
@@ -79,8 +79,10 @@ struct PixelTables {
     double* qcut;         // forward-removal cut [1/m]
     double4* u0h;         // (T_p / rho0, 1 / (2 rho0))
     float2* gtab;         // [species][pixel] (f0+f'+if'') r_e sqrt(P), origin direction
-    double* stat_blocks;  // per prologue block: rho_min, max |du|, max |d2u|
-    double* stats;        // reduced: rho_min, max |du|, max |d2u|
+    float4* dup;          // u0 - u0(middle pixel of the pixel's MMA_P patch)
+    float2* gmma;         // [species][pixel] gtab e^{2 pi i (Theta0 - Theta0(middle))}
+    double* stat_blocks;  // per prologue block: the N_GEOM statistics
+    double* stats;        // reduced statistics
 };
 
 struct SpeciesConst {
@@ -133,6 +135,8 @@ void take_pixel_tables(Carver& c, int npix, int n_species, PixelTables& P) {
     P.qcut = c.take<double>(npix);
     P.u0h = c.take<double4>(npix);
     P.gtab = c.take<float2>(static_cast<size_t>(n_species) * npix);
+    P.dup = c.take<float4>(npix);
+    P.gmma = c.take<float2>(static_cast<size_t>(n_species) * npix);
     P.stat_blocks = c.take<double>(N_GEOM * static_cast<size_t>(blocks_for(npix, PRO_THREADS)));
     P.stats = c.take<double>(N_GEOM);
 }
@@ -309,6 +313,33 @@ __device__ __forceinline__ PatchGeom patch_of(int py, int pz, int Ny, int Nz) {
     return g;
 }
 
+// frac(K |T|) in double-double (K = k / 2 pi), with |T| = r + r_lo.
+__device__ __forceinline__ void origin_phase(double tx, double ty, double tz, double k,
+                                             double& f, double& r, double& r_lo) {
+    // |T|^2 in double-double
+    double a, ae, b, be, c, ce, s, se, s2, se2;
+    two_prod(tx, tx, a, ae);
+    two_prod(ty, ty, b, be);
+    two_prod(tz, tz, c, ce);
+    two_sum(a, b, s, se);
+    two_sum(s, c, s2, se2);
+    const double sq_hi = s2, sq_lo = (se + se2) + (ae + be + ce);
+    r = sqrt(sq_hi);
+    double rr, rre;
+    two_prod(r, r, rr, rre);
+    const double resid = ((sq_hi - rr) - rre) + sq_lo;
+    r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
+    double K_hi, K_lo;
+    two_prod(k, INV_2PI_HI, K_hi, K_lo);
+    K_lo = fma(k, INV_2PI_LO, K_lo);
+    double th, the;
+    two_prod(K_hi, r, th, the);
+    the += K_hi * r_lo + K_lo * r;
+    const double n = floor(th);
+    f = (th - n) + the;
+    f -= floor(f);
+}
+
 // Per pixel:
 //  * rho0 = |T| and Theta0 = frac(K rho0) in double-double (K = k / 2 pi);
 //  * the forward-removal cut-off from the right/upper neighbours
@@ -330,28 +361,8 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
-        // |T|^2 in double-double
-        double a, ae, b, be, c, ce, s, se, s2, se2;
-        two_prod(tx, tx, a, ae);
-        two_prod(ty, ty, b, be);
-        two_prod(tz, tz, c, ce);
-        two_sum(a, b, s, se);
-        two_sum(s, c, s2, se2);
-        const double sq_hi = s2, sq_lo = (se + se2) + (ae + be + ce);
-        double r = sqrt(sq_hi);
-        double rr, rre;
-        two_prod(r, r, rr, rre);
-        const double resid = ((sq_hi - rr) - rre) + sq_lo;
-        const double r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
-        double K_hi, K_lo;
-        two_prod(k, INV_2PI_HI, K_hi, K_lo);
-        K_lo = fma(k, INV_2PI_LO, K_lo);
-        double th, the;
-        two_prod(K_hi, r, th, the);
-        the += K_hi * r_lo + K_lo * r;
-        const double n = floor(th);
-        double f = (th - n) + the;
-        f -= floor(f);
+        double f, r, r_lo;
+        origin_phase(tx, ty, tz, k, f, r, r_lo);
         T.theta0[p] = f;
         T.rho0[p] = r + r_lo;
         rmin = r;
@@ -386,6 +397,18 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
             P = fmin(1.0, fmax(0.0, P));
             scale *= sqrt(P);
         }
+        // the pixel's matrix-core patch and its middle pixel
+        const PatchGeom G = patch_of(ix / MMA_P, iy / MMA_P, Ny, Nz);
+        const int pm = (G.z0 + G.mz) * Ny + G.y0 + G.my;
+        double mc[3];
+        pixel_dir(cx, cy, cz, pm, mc[0], mc[1], mc[2]);
+        T.dup[p] = make_float4(static_cast<float>(ux - mc[0]), static_cast<float>(uy - mc[1]),
+                               static_cast<float>(uz - mc[2]), 0.f);
+        double fm, rm, rm_lo;
+        origin_phase(cx[pm], cy[pm], cz[pm], k, fm, rm, rm_lo);
+        double dsn, dcs;
+        sincospi(2.0 * (f - fm), &dsn, &dcs);
+
         const bool cut = O.remove_forward && Q < qcut;
         for (int sp = 0; sp < O.n_species; ++sp) {
             const SpeciesConst& S = O.spc[sp];
@@ -395,6 +418,9 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
             if (cut) { sr -= static_cast<double>(S.f0z) + S.fp; si -= S.fpp; }
             T.gtab[static_cast<int64_t>(sp) * npix + p] =
                 make_float2(static_cast<float>(sr * scale), static_cast<float>(si * scale));
+            T.gmma[static_cast<int64_t>(sp) * npix + p] =
+                make_float2(static_cast<float>((sr * dcs - si * dsn) * scale),
+                            static_cast<float>((sr * dsn + si * dcs) * scale));
         }
 
         // direction differences along the row
@@ -412,10 +438,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
         }
 
         // deviation from the patch model
-        const PatchGeom G = patch_of(ix / MMA_P, iy / MMA_P, Ny, Nz);
-        const int pm = (G.z0 + G.mz) * Ny + G.y0 + G.my;
-        double mc[3], e1[3], e2[3], e3[3], e4[3];
-        pixel_dir(cx, cy, cz, pm, mc[0], mc[1], mc[2]);
+        double e1[3], e2[3], e3[3], e4[3];
         pixel_dir(cx, cy, cz, pm - G.my, e1[0], e1[1], e1[2]);
         pixel_dir(cx, cy, cz, pm - G.my + G.vy - 1, e2[0], e2[1], e2[2]);
         pixel_dir(cx, cy, cz, pm - G.mz * Ny, e3[0], e3[1], e3[2]);
@@ -1012,29 +1035,40 @@ hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t 
 // ------------------------------------------------------- matrix-core sum
 // Far-field sum on the matrix cores.  One wave owns a patch of MMA_P x MMA_P
 // pixels (a along the row, b down the columns, both offsets from the patch's
-// middle pixel).  Across the patch the direction is modelled per axis to
-// second order,
-//     v(a, b) = vc + a gy + a^2 hy + b gz + b^2 hz,
+// middle pixel).  Across the patch the atom-offset direction is modelled per
+// axis to second order,
+//     v(a, b) = va(a) + wb(b),  va(a) = vc + a gy + a^2 hy,  wb(b) = b gz + b^2 hz,
 // exact along the middle row and column (the host bounds the deviation
-// elsewhere from the measured pixel grid), so per atom the phasor separates:
-//     e^{2 pi i v.sigma} = A(a) B(b),
-//     A(a) = w e^{2 pi i (vc.sigma + a gy.sigma + a^2 hy.sigma)},
-//     B(b) = e^{2 pi i (b gz.sigma + b^2 hz.sigma)},
+// elsewhere from the measured pixel grid), so per atom the phasor separates,
+//     e^{2 pi i v.sigma} = A(a) B(b),  A(a) = w e^{2 pi i va(a).sigma},
+//     B(b) = e^{2 pi i wb(b).sigma},
 // and the tile sum S(a, b) = sum_j A_j(a) B_j(b) is a complex matrix product
-// over the atoms.  Stacking Re and Im turns it into one real 32 x 32 product,
-//     [Re A; Im A] (32 x atoms) . [Re B, Im B] (atoms x 32),
-// from which S = (C_rr - C_ii) + i (C_ri + C_ir): v_mfma_f32_32x32x2_f32 with
-// two atoms per instruction.  Each lane forms one element of each operand
-// with one v_cos (an Im part is the cos of the phase less a quarter turn).
-// Per (tile, pixel) the FP64 origin phase and the factor G are those of the
-// row recurrence.
+// over the atoms.  With A = Ar + i Ai and B = Br + i Bi, two two-block
+// products give it: [Ar; Ai] x [Br; Bi] (blocks Ar.Br and Ai.Bi) and
+// [Ar; Ai] x [Bi; Br] (blocks Ar.Bi and Ai.Br), so S = (C1_0 - C1_1) +
+// i (C2_0 + C2_1), register by register in the same lane.  The instruction is
+// v_mfma_f32_32x32x4_2b_f16: both 32 x 32 blocks take the same four atoms
+// (wave-uniform records, scalar loads), and each operand value is split into
+// two FP16 halves (22 significant bits), so three products per sum keep FP32
+// accuracy: hi.hi + hi.lo + lo.hi.  Each lane forms, per atom, one A value
+// and two B values with one v_cos each (an Im part is the cos of the phase
+// less a quarter turn).
+//
+// The per-(tile, pixel) origin phase is Theta0(p) + K (X_t - u_p.O +
+// (|O|^2 - (u_p.O)^2) / (2 rho)).  Its pixel-only part, Theta0(p) -
+// Theta0(middle), sits in the table gmma with the factor G; the tile part at
+// the middle pixel is one FP64 value per tile; what remains,
+// -K (u_p - u_mid).O (< 1 rev), is an FP32 dot product per pixel and tile.
+// Tile results are summed in FP32 over MMA_FLUSH tiles, then added to the
+// FP64 slice partial (owned by this wave).
 struct MmaArgs {
     const AtomRec* atoms;
     const TileRec* tiles;
     const FrameRec* frames;
     const double4* u0h;
     const double* theta0;
-    const float2* gtab;
+    const float4* dup;
+    const float2* gmma;
     double2* partial;
     int n_tiles;
     int tiles_per_slice;
@@ -1048,7 +1082,9 @@ struct MmaArgs {
 };
 
 constexpr int MMA_WAVES = 4;
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int MMA_FLUSH = 16;
+typedef float floatx32 __attribute__((ext_vector_type(32)));
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
 
 template <bool SECOND>
 __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
@@ -1063,132 +1099,178 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
     const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
     const int Ny = A.Ny;
 
-    // operand roles: atom kk of each pair; A row / B column q = (Re|Im, offset)
-    const int kk = lane >> 5;
+    // operand roles: block h = lane / 32 holds Re (h = 0) or Im (h = 1) rows
+    // of A; row a and column b of this lane are both q
+    const int h = lane >> 5;
     const int q = lane & 31;
-    const float quarter = (q & 16) ? -0.25f : 0.f;
-    const float fa = static_cast<float>((q & 15) - P.my);
-    const float fb = static_cast<float>((q & 15) - P.mz);
-    // result roles: D register r (< 8) of this lane holds row m(r), column
-    // q; its partner lane ^ 16 holds the other half of the same pixels.
-    const bool re_lane = (lane & 16) == 0;
-    const int n = lane & 15;
-    const int mbase = 4 * (lane >> 5);
-    // pixel of D register r: row m(r) = 8 (r / 4) + mbase + r % 4, column n
-    const int prow = (P.z0 + min(n, P.vz - 1)) * Ny + P.y0;
-    auto pix_of = [&](int r) { return prow + min(8 * (r >> 2) + mbase + (r & 3), P.vy - 1); };
-    double acc[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) acc[r] = 0.0;
-
+    const float qa = h ? -0.25f : 0.f;          // A value: Re or Im part
+    const float qb1 = qa;                        // B in product 1: the same part
+    const float qb2 = h ? 0.f : -0.25f;          // B in product 2: the other part
     const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
     const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
     const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
     const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
-    const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
-                         pmid + (P.vz - 1 - P.mz) * Ny};
+
+    // direction model for the lab origin, exact at the middle pixel and the
+    // ends of the middle row and column (sample frame)
+    double va0[3], vc0[3];
+    float wb[3];
+    {
+        const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                             pmid + (P.vz - 1 - P.mz) * Ny};
+        double vs[5][3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double4 u = A.u0h[samp[k]];
+            const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
+            vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
+            vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
+            vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
+        }
+        const double fa = q - P.my, fb = q - P.mz;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double g, hq;
+            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, g, hq);
+            va0[d] = vs[0][d] + fa * (g + fa * hq);
+            vc0[d] = vs[0][d];
+            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, g, hq);
+            wb[d] = static_cast<float>(fb * (g + fb * hq));
+        }
+    }
+    const double4 um = A.u0h[pmid];              // middle pixel: direction, 1 / (2 rho)
+    const double th_mid = A.theta0[pmid];
+    // u_hat . sigma at the middle pixel = (x_hat - v_mid) . sigma (second order)
+    const float xc0 = static_cast<float>(F.xs[0] - vc0[0]);
+    const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
+    const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
+    const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
+
+    // result registers r of this lane: pixel row a(r) = 8 (r / 4) + 4 h + r % 4,
+    // column q
+    const int prow = (P.z0 + min(q, P.vz - 1)) * Ny + P.y0;
+    auto arow = [&](int r) { return 8 * (r >> 2) + 4 * h + (r & 3); };
+    // FP32 running sums of the lane's 16 pixels live in LDS (registers go
+    // to the matrix accumulators)
+    __shared__ float2 lacc[MMA_WAVES][16][64];
+    float2 (*acc)[64] = lacc[wave];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r][lane] = make_float2(0.f, 0.f);
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+    bool first = true;
+    int since = 0;
 
     for (int t = t0; t < t1; ++t) {
         const TileRec tr = A.tiles[t];
-        // Per-pixel table loads stay inside the loop: tying their index to
-        // the tile record (count <= 256, so the shift is 0) stops the
-        // compiler from hoisting ~150 registers of tables out of it.
+        // keeps the table loads below inside the loop (count <= 256: shift 0)
         const int tie = tr.count >> 16;
         const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
         const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
         const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
         const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+        double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+        ph_t -= floor(ph_t);
+        // first-order change of the direction for this tile origin, constant
+        // over the patch: R^T (O - (u.O) u) / rho
+        const double h2 = 2.0 * um.w;
+        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                     czl = (Oz - uo * um.z) * h2;
+        const float va_x = static_cast<float>(va0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+        const float va_y = static_cast<float>(va0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+        const float va_z = static_cast<float>(va0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                    koz = static_cast<float>(A.K * Oz);
+        const float pht = static_cast<float>(ph_t);
 
-        // exact v (sample frame) at the middle pixel and the ends of the
-        // middle row and column, then the per-axis quadratic model
-        double vs[5][3];
+        floatx32 c1, c2;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const double4 u = A.u0h[samp[k] + tie];
-            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
-            const double h2 = 2.0 * u.w;                     // 1 / rho0
-            const double wx = 1.0 - u.x + (Ox - uo * u.x) * h2;
-            const double wy = -u.y + (Oy - uo * u.y) * h2;
-            const double wz = -u.z + (Oz - uo * u.z) * h2;
-            vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
-            vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
-            vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
-        }
-        // this lane's row and column directions: A phase = va . sigma, B
-        // phase = wb . sigma (FP64 model, rounded once)
-        float va[3], wb[3], xc[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double gyd, hyd, gzd, hzd;
-            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gyd, hyd);
-            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gzd, hzd);
-            va[d] = static_cast<float>(vs[0][d] + fa * (gyd + fa * hyd));
-            wb[d] = static_cast<float>(fb * (gzd + fb * hzd));
-            xc[d] = static_cast<float>(F.xs[d] - vs[0][d]);    // u_hat . sigma at the middle
-        }
-        const float hh = SECOND ? static_cast<float>(A.u0h[pmid + tie].w / A.K) : 0.f;
-
-        floatx16 cacc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) cacc[r] = 0.f;
+        for (int r = 0; r < 32; ++r) { c1[r] = 0.f; c2[r] = 0.f; }
         const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
         const int cnt = tr.count;
-        // The pair's records are wave-uniform (scalar loads, one pair in
-        // flight ahead); records past the count are zero (mag = 0), so an odd
-        // count reads one of them.
-        AtomRec n0 = at[0], n1 = at[1];
+        // four wave-uniform records per step; records past the count are zero
+        // (mag = 0) and j + 3 <= 255
 #pragma unroll 1
-        for (int j = 0; j < cnt; j += 2) {
-            const AtomRec a0 = n0, a1 = n1;
-            if (j + 2 < cnt) {
-                n0 = at[j + 2];
-                n1 = at[j + 3];
+        for (int j = 0; j < cnt; j += 4) {
+            halfx4 ah, al, b1h, b1l, b2h, b2l;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const AtomRec a = at[j + u];
+                float pa = fmaf(va_z, a.sz, fmaf(va_y, a.sy, va_x * a.sx));
+                if (SECOND) {
+                    const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
+                    pa = fmaf(hh, fmaf(-w, w, a.s2), pa);
+                }
+                pa += a.ph + qa;
+                const float pb = fmaf(wb[2], a.sz, fmaf(wb[1], a.sy, wb[0] * a.sx));
+                const float av = a.mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
+                const float bv1 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb + qb1));
+                const float bv2 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb + qb2));
+                ah[u] = static_cast<_Float16>(av);
+                al[u] = static_cast<_Float16>(av - static_cast<float>(ah[u]));
+                b1h[u] = static_cast<_Float16>(bv1);
+                b1l[u] = static_cast<_Float16>(bv1 - static_cast<float>(b1h[u]));
+                b2h[u] = static_cast<_Float16>(bv2);
+                b2l[u] = static_cast<_Float16>(bv2 - static_cast<float>(b2h[u]));
             }
-            const float sx = kk ? a1.sx : a0.sx;
-            const float sy = kk ? a1.sy : a0.sy;
-            const float sz = kk ? a1.sz : a0.sz;
-            const float mag = kk ? a1.mag : a0.mag;
-            float pa = fmaf(va[2], sz, fmaf(va[1], sy, va[0] * sx));
-            if (SECOND) {
-                const float s2 = kk ? a1.s2 : a0.s2;
-                const float u = fmaf(xc[2], sz, fmaf(xc[1], sy, xc[0] * sx));
-                pa = fmaf(hh, fmaf(-u, u, s2), pa);
-            }
-            pa += (kk ? a1.ph : a0.ph) + quarter;
-            const float pb = fmaf(wb[2], sz, fmaf(wb[1], sy, fmaf(wb[0], sx, quarter)));
-            const float av = mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
-            const float bv = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb));
-            cacc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, cacc, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b1h, c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b2h, c2, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b1l, c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b2l, c2, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(al, b1h, c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(al, b2h, c2, 0, 0, 0);
         }
 
-        // S = (C_rr - C_ii) + i (C_ri + C_ir): the Re lane keeps Re S, its
-        // partner Im S; then both form their half of G e^{2 pi i Theta} S.
-        const float2* __restrict__ G = A.gtab + static_cast<int64_t>(tr.species) * A.npix;
+        // S(a(r), q) = (C1_0 - C1_1) + i (C2_0 + C2_1); times the origin
+        // phasor and G of the pixel
+        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+        float s_re[16], s_im[16];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const float x = __shfl_xor(cacc[r + 8], 16, 64);
-            const float v = re_lane ? cacc[r] - x : cacc[r] + x;
-            const float o = __shfl_xor(v, 16, 64);
-            const float s_re = re_lane ? v : o, s_im = re_lane ? o : v;
-            const int p = pix_of(r) + tie;
-            const double4 u = A.u0h[p];
-            const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
-            double th = A.theta0[p] + A.K * (Ox - uo + (O2 - uo * uo) * u.w);
-            th -= rint(th);
-            const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
-            const float sn = __builtin_amdgcn_sinf(thf);
-            const float cs = __builtin_amdgcn_cosf(thf);
-            const float2 g = G[p];
+        for (int r = 0; r < 16; ++r) {
+            s_re[r] = c1[r] - c1[16 + r];
+            s_im[r] = c2[r] + c2[16 + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            // four pixels at a time: bounds the table loads in flight
+            if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+            const float sr = s_re[r], si = s_im[r];
+            const int p = prow + min(arow(r), P.vy - 1) + tie;
+            const float4 du = A.dup[p];
+            const float phs = __builtin_amdgcn_fractf(
+                pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+            const float sn = __builtin_amdgcn_sinf(phs);
+            const float cs = __builtin_amdgcn_cosf(phs);
+            const float2 g = Gm[p];
             const float zr = g.x * cs - g.y * sn, zi = g.x * sn + g.y * cs;
-            acc[r] += static_cast<double>(re_lane ? zr * s_re - zi * s_im : zr * s_im + zi * s_re);
+            float2 av = acc[r][lane];
+            av.x = fmaf(zr, sr, fmaf(-zi, si, av.x));
+            av.y = fmaf(zr, si, fmaf(zi, sr, av.y));
+            acc[r][lane] = av;
+        }
+        if (++since == MMA_FLUSH || t + 1 == t1) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                const float2 av = acc[r][lane];
+                if (arow(r) < P.vy && q < P.vz) {
+                    double2* o = out + prow + arow(r);
+                    double2 v = first ? make_double2(0.0, 0.0) : *o;
+                    v.x += static_cast<double>(av.x);
+                    v.y += static_cast<double>(av.y);
+                    *o = v;
+                }
+                acc[r][lane] = make_float2(0.f, 0.f);
+            }
+            first = false;
+            since = 0;
         }
     }
-    double* out = reinterpret_cast<double*>(
-        A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix);
+    if (first) {                                  // a slice without tiles
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
-        if (8 * (r >> 2) + mbase + (r & 3) < P.vy && n < P.vz)
-            out[2 * static_cast<int64_t>(pix_of(r)) + (re_lane ? 0 : 1)] = acc[r];
+        for (int r = 0; r < 16; ++r)
+            if (arow(r) < P.vy && q < P.vz) out[prow + arow(r)] = make_double2(0.0, 0.0);
+    }
 }
 
 template <bool SECOND>
@@ -1618,6 +1700,48 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     return MOSAIC_OK;
 }
 
+extern "C" int mosaic_kin_restage(int64_t atom_count, const float* pos, const float* amp,
+                                  int n_species, const int64_t* counts, double k, int n_frames,
+                                  int Ny, int Nz, void* workspace, size_t workspace_bytes,
+                                  double* stage_stats, mosaic_stream_t stream) {
+    size_t need = 0;
+    int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
+    if (atom_count > 0 && !pos) return MOSAIC_ERR_ARGS;
+    if (!(k > 0.0) || !stage_stats) return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Layout L;
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    stage_stats[0] = 0.0;
+    stage_stats[1] = 0.0;
+    if (atom_count == 0) return MOSAIC_OK;
+    SpeciesSplit split;
+    split.n_species = n_species;
+    split.atom_start[0] = 0;
+    split.tile_start[0] = 0;
+    for (int s = 0; s < n_species; ++s) {
+        split.atom_start[s + 1] = split.atom_start[s] + counts[s];
+        split.tile_start[s + 1] =
+            split.tile_start[s] + static_cast<int>((counts[s] + TILE - 1) / TILE);
+    }
+    hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
+    // the order of the last mosaic_kin_stage on this workspace is in vals_b
+    hipLaunchKernelGGL(k_tiles, dim3(L.n_tiles), dim3(TILE), 0, st, pos,
+                       reinterpret_cast<const float2*>(amp), L.vals_b, split, k * INV_2PI_HI,
+                       L.tiles, L.atoms, L.radius_bits);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    unsigned int bits[2] = {0u, 0u};
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
+    float rf[2];
+    std::memcpy(rf, bits, sizeof(rf));
+    stage_stats[0] = static_cast<double>(rf[0]);
+    stage_stats[1] = static_cast<double>(rf[1]);
+    return MOSAIC_OK;
+}
+
 namespace {
 void fill_species(const mosaic_species_t* tab, int n, SpeciesConst* out) {
     for (int s = 0; s < n; ++s) {
@@ -1734,8 +1858,11 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     else if (rec_geom && rec_curv_bound(4, du2, radius_wl) <= 1e-6) kp = 4;
     // matrix-core sum: the patch model's deviation from the pixel grid,
     // times the largest offset, below 1e-6 rev
-    const bool mma_ok = far && dpatch * radius_wl <= 1e-6;
-    int path = opts->path == 0 ? (kp ? 2 : (mma_ok ? 3 : 1)) : opts->path;
+    // and the patch-constant second-order origin term: its change across a
+    // patch, K |O|^2 |du| / rho, below 1e-7 rev
+    const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
+                        K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7;
+    int path = opts->path == 0 ? (mma_ok ? 3 : (kp ? 2 : 1)) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok)) return MOSAIC_ERR_UNSUPPORTED;
 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
@@ -1748,7 +1875,8 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         A.frames = L.frames;
         A.u0h = L.px.u0h;
         A.theta0 = L.px.theta0;
-        A.gtab = L.px.gtab;
+        A.dup = L.px.dup;
+        A.gmma = L.px.gmma;
         A.partial = L.partial;
         A.n_tiles = L.n_tiles;
         A.tiles_per_slice = L.tiles_per_slice;

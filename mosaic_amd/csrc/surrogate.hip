@@ -37,12 +37,14 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
                                                    float* __restrict__ C, int M, int N, int K,
                                                    int lda, int ldb, int ldc, long long sA,
                                                    long long sB, long long sC, float alpha,
-                                                   float beta) {
+                                                   float beta, int splits, int kchunk) {
     // LDS tiles as bf16: As[m][k], Bs[n][k] (k contiguous, padded against
     // bank conflicts)
     __shared__ unsigned short As[GBM][GBK + 4];
     __shared__ unsigned short Bs[GBN][GBK + 4];
-    const int z = blockIdx.z;
+    // grid.z = batch x K-splits; a split adds its partial product atomically
+    const int z = blockIdx.z / splits, split = blockIdx.z % splits;
+    const int kb = split * kchunk, ke = min(K, kb + kchunk);
     A += z * sA;
     B += z * sB;
     C += z * sC;
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
 
-    for (int k0 = 0; k0 < K; k0 += GBK) {
+    for (int k0 = kb; k0 < ke; k0 += GBK) {
         // stage: 64 x 32 elements of each operand, 8 per thread
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
@@ -62,14 +64,14 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
             if (TA) { kk = e / GBM; mm = e % GBM; } else { mm = e / GBK; kk = e % GBK; }
             const int gm = m0 + mm, gk = k0 + kk;
             float v = 0.f;
-            if (gm < M && gk < K) v = TA ? A[static_cast<long long>(gk) * lda + gm]
+            if (gm < M && gk < ke) v = TA ? A[static_cast<long long>(gk) * lda + gm]
                                          : A[static_cast<long long>(gm) * lda + gk];
             As[mm][kk] = to_bf16(v);
             int nn, kb;
             if (TB) { nn = e / GBK; kb = e % GBK; } else { kb = e / GBN; nn = e % GBN; }
             const int gn = n0 + nn, gkb = k0 + kb;
             float w = 0.f;
-            if (gn < N && gkb < K) w = TB ? B[static_cast<long long>(gn) * ldb + gkb]
+            if (gn < N && gkb < ke) w = TB ? B[static_cast<long long>(gn) * ldb + gkb]
                                           : B[static_cast<long long>(gkb) * ldb + gn];
             Bs[nn][kb] = to_bf16(w);
         }
@@ -96,9 +98,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
         if (row < M && col < N) {
             float* c = C + static_cast<long long>(row) * ldc + col;
             const float v = alpha * acc[j];
-            *c = (beta == 0.f) ? v : fmaf(beta, *c, v);
+            if (splits > 1) atomicAdd(c, v);        // C was scaled by beta first
+            else *c = (beta == 0.f) ? v : fmaf(beta, *c, v);
         }
     }
+}
+
+// C[b][m][n] *= beta (0 writes zeros), ahead of a split-K product
+__global__ void k_scale(float* __restrict__ C, int M, int N, int ldc, long long sC, float beta) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i >= static_cast<long long>(M) * N) return;
+    float* c = C + blockIdx.y * sC + (i / N) * ldc + (i % N);
+    *c = (beta == 0.f) ? 0.f : beta * *c;
 }
 
 __device__ __forceinline__ float silu(float x) { return x / (1.f + __expf(-x)); }
@@ -249,11 +260,27 @@ extern "C" int mosaic_sg_gemm(int transA, int transB, int M, int N, int K, const
     if (M < 0 || N < 0 || K < 0 || batch < 1 || !A || !B || !C) return MOSAIC_ERR_ARGS;
     if (M == 0 || N == 0) return MOSAIC_OK;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM, batch);
+    // Few output tiles over a long K (weight gradients: K = batch x pixels):
+    // split K across workgroups so the chip fills, adding partials atomically.
+    const long long tiles = static_cast<long long>((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM);
+    int splits = 1;
+    if (tiles * batch < 1024 && K > 4 * GBK) {
+        const long long want = (2048 + tiles * batch - 1) / (tiles * batch);
+        splits = static_cast<int>(std::min<long long>(want, (K + 4 * GBK - 1) / (4 * GBK)));
+        splits = std::max(1, std::min(splits, 65535 / batch));
+    }
+    const int kchunk = ((K + splits - 1) / splits + GBK - 1) / GBK * GBK;
+    splits = (K + kchunk - 1) / kchunk;
+    dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM, batch * splits);
     if (grid.y > 65535) return MOSAIC_ERR_ARGS;
+    if (splits > 1) {
+        const long long mn = static_cast<long long>(M) * N;
+        hipLaunchKernelGGL(k_scale, dim3(static_cast<unsigned>((mn + 255) / 256), batch), dim3(256),
+                           0, st, C, M, N, ldc, strideC, beta);
+    }
 #define MOSAIC_GEMM(ta, tb)                                                                   \
     hipLaunchKernelGGL((k_gemm_bf16<ta, tb>), grid, dim3(256), 0, st, A, B, C, M, N, K, lda, \
-                       ldb, ldc, strideA, strideB, strideC, alpha, beta)
+                       ldb, ldc, strideA, strideB, strideC, alpha, beta, splits, kchunk)
     if (!transA && !transB) MOSAIC_GEMM(0, 0);
     else if (!transA && transB) MOSAIC_GEMM(0, 1);
     else if (transA && !transB) MOSAIC_GEMM(1, 0);

@@ -73,8 +73,9 @@ def workspace_bytes(n_atoms, counts, n_frames, Ny, Nz):
 class StagedSample:
     """A sample staged into tiles inside a caller-owned workspace."""
 
-    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, stats):
+    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, stats, k):
         self.n_atoms = int(n_atoms)
+        self.k = float(k)
         self.counts = np.asarray(counts, dtype=np.int64)
         self.n_frames = int(n_frames)
         self.Ny, self.Nz = int(Ny), int(Nz)
@@ -122,7 +123,31 @@ def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=N
                                       cp_, float(k), int(n_frames), int(Ny), int(Nz), _ptr(ws),
                                       int(ws.numel()), stats, _stream_handle(stream)),
           "mosaic_kin_stage")
-    return StagedSample(n, c, n_frames, Ny, Nz, ws, list(stats))
+    return StagedSample(n, c, n_frames, Ny, Nz, ws, list(stats), k)
+
+
+def restage(staged, pos, amp=None, stream=None):
+    """Rebuild the tiles of ``staged`` from moved positions (same atoms, same
+    species codes and order as its last ``stage``) without sorting; updates
+    its stage statistics in place and returns it."""
+    if not pos.is_cuda or pos.dtype != torch.float32 or tuple(pos.shape) != (staged.n_atoms, 3):
+        raise ValueError("positions must be a (N, 3) float32 device tensor of the staged atoms")
+    pos = pos.contiguous()
+    if amp is not None:
+        if amp.dtype != torch.complex64 or amp.shape[0] != staged.n_atoms:
+            raise ValueError("amplitudes must be (N,) complex64")
+        amp = torch.view_as_real(amp.contiguous())
+    c, cp_ = _counts_arr(staged.counts)
+    stats = (ctypes.c_double * 2)()
+    check(_lib.lib().mosaic_kin_restage(staged.n_atoms, _ptr(pos), _ptr(amp), int(c.shape[0]),
+                                        cp_, float(staged.k), staged.n_frames, staged.Ny,
+                                        staged.Nz, _ptr(staged.workspace),
+                                        int(staged.workspace.numel()), stats,
+                                        _stream_handle(stream)),
+          "mosaic_kin_restage")
+    staged.stage_stats = stats
+    staged.radius_wl = float(stats[0])
+    return staged
 
 
 def kin_pixels(staged, species_tab, opts, coords, stream=None):

@@ -134,7 +134,7 @@ class ImageGenerator:
         self.coords = torch.as_tensor(np.ascontiguousarray(d.pixel_coordinates / 1e10),
                                       device=self.device)
         self.stage = w["stage"]
-        self._ws = None
+        self._staged = {}
         self._pos = torch.empty_like(self.base)
 
     def pose(self, angle_deg):
@@ -155,14 +155,16 @@ class ImageGenerator:
         poses = [self.pose(a) for a in angles_deg]
         rot = np.stack([q[0] for q in poses])
         tr = np.stack([q[1] for q in poses])
-        staged = K.stage(pos, self.codes, None, self.counts, self.k, F, self.Ny, self.Nz,
-                         workspace=self._ws)
-        if self._ws is not None and staged.workspace is self._ws and hasattr(self, "_pix"):
-            staged.pixels_key, staged.pixel_geom = self._pix
-        self._ws = staged.workspace
+        staged = self._staged.get(F)
+        if staged is None:
+            # k-d tiles of the undisplaced lattice, kept for every defect: the
+            # displacements (a few A) only widen the tiles slightly
+            staged = K.stage(self.base, self.codes, None, self.counts, self.k, F, self.Ny,
+                             self.Nz)
+            self._staged[F] = staged
+        K.restage(staged, pos)
         out = torch.zeros((F, self.Nz, self.Ny), dtype=torch.complex64, device=self.device)
         K.kin_sum(staged, self.spc, self.opts, rot, tr, self.coords, out)
-        self._pix = (staged.pixels_key, staged.pixel_geom)
         return (out.abs() ** 2).reshape(F, -1)
 
 

@@ -38,3 +38,9 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_LIB", None)
     with pytest.raises(_lib.MosaicLibraryError):
         _lib.lib()
+
+
+def test_every_export_has_a_signature():
+    """ctypes converts Python floats only with declared argument types."""
+    from mosaic_amd import _lib
+    assert set(_lib._SIGS) == set(_lib.EXPORTS)

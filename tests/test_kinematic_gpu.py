@@ -141,6 +141,39 @@ def test_mma_ragged_patches(device, shape):
     assert e <= TOL
 
 
+def test_restage_moved_atoms(device):
+    """Tiles rebuilt from displaced positions with the original k-d order
+    give the field of the displaced crystal (long-double oracle), with tile
+    radii close to a fresh staging's."""
+    from mosaic_amd import tables
+    w = W.fcc_au_bragg111(n=10, npix=24, seed=5)
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    pos0 = s.load_chunk_positions(1).astype(np.float32)
+    rng = np.random.default_rng(9)
+    pos1 = (pos0 + W.gaussian_bump_field(pos0, 0.8, 8.0, rng)).astype(np.float32)
+    k = 2 * np.pi / b.wavelength
+    counts = np.array([pos0.shape[0]], np.int64)
+    codes = torch.zeros(pos0.shape[0], dtype=torch.uint8, device=device)
+    Ny, Nz = d.shape
+    spc = K.species_table(tables.species_constants(["Au"], b.energy))
+    opts = K.kin_opts(k, False, True, b._polarization_state(), False)
+    coords = torch.as_tensor(np.ascontiguousarray(d.pixel_coordinates / 1e10), device=device)
+    rot = np.asarray(st.rotation, np.float32)[None]
+    tr = np.asarray(st.translation, np.float32)[None]
+    staged = K.stage(torch.as_tensor(pos0, device=device), codes, None, counts, k, 1, Ny, Nz)
+    r0 = staged.radius_wl
+    K.restage(staged, torch.as_tensor(pos1, device=device))
+    out = torch.zeros((1, Nz, Ny), dtype=torch.complex128, device=device)
+    K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+    fresh = K.stage(torch.as_tensor(pos1, device=device), codes, None, counts, k, 1, Ny, Nz)
+    assert staged.radius_wl <= fresh.radius_wl + 2.0 and staged.radius_wl >= r0 - 1.0
+    from mosaic_amd.sample import sample
+    w["sample"] = sample([pos1], [np.zeros(pos1.shape[0], np.uint8)], ["Au"])
+    R = oracle_field(w, "f80")
+    e, _ = report("restaged displaced crystal", out[0].cpu().numpy().ravel(), R)
+    assert e <= TOL
+
+
 def test_declines_near_field(device):
     w = W.fcc_au_bragg111(n=4, npix=8)
     d = w["detector"]
@@ -169,14 +202,14 @@ def test_empty_and_ragged(device):
 
 def test_c1_full_vs_fp64_oracle(device):
     """32,000 atoms x 128^2 pixels (5.2e8 pairs) against the FP64 restatement,
-    all three kernels; the automatic choice is the row recurrence here."""
+    all three kernels; the automatic choice is the matrix-core form here."""
     w = W.fcc_au_bragg111(n=20, npix=128, seed=0)
     R = oracle_field(w, "f64")
     for path in ("tile", "rows", "mma", "auto"):
         A = gpu_field(w, path=path)
         e, ei = report(f"C1 full {path} vs f64", A, R)
         assert e <= TOL and ei <= TOL
-    assert w["beam"]._last_path % 100 in (24, 28)
+    assert w["beam"]._last_path % 100 == 32
 
 
 def test_rows_second_order_close_detector(device):
@@ -199,15 +232,15 @@ def test_rows_second_order_close_detector(device):
 
 
 def test_auto_path_choice(device):
-    """Far field picks the recurrence; a 5 cm detector with 2 mm pixels bends
-    the rows and patches too much for it and for the matrix-core form, and
+    """Far field picks the matrix-core form; a 5 cm detector with 2 mm pixels
+    bends the rows and patches too much for it and for the recurrence, and
     falls back to per-tile factors."""
     w = W.fcc_au_bragg111(n=4, npix=16)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
     b._kin_path = "auto"
     gpu_field(w)
     staged_path = b._last_path
-    assert staged_path % 100 in (24, 28)
+    assert staged_path == 32
     d.create_detector((16, 16), (2.0e7, 2.0e7))
     d.position_detector_absolute(5.0e8, np.rad2deg(2 * w["theta_b"]), 0.0)
     A = gpu_field(w)
