@@ -48,6 +48,9 @@ def parse():
                     help="run the surrogate workload (data generation, training, gated scan)")
     ap.add_argument("--sg-images", type=int, default=20000)
     ap.add_argument("--sg-epochs", type=int, default=40)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device 3 times (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to HBM bytes)")
     return ap.parse_args()
 
 
@@ -350,6 +353,16 @@ def main():
     if args.surrogate:
         print(json.dumps({"surrogate": bench_surrogate(args.sg_images, args.sg_epochs)}),
               flush=True)
+        return
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)      # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 1 << 30,
+                                               "bytes_written_per_copy": 1 << 30,
+                                               "copies": 3}}), flush=True)
         return
     w = build_workload(args.workload)
     b, d, st = w["beam"], w["detector"], w["stage"]

@@ -1082,30 +1082,40 @@ struct MmaArgs {
 };
 
 constexpr int MMA_WAVES = 4;
-constexpr int MMA_FLUSH = 16;
-typedef float floatx32 __attribute__((ext_vector_type(32)));
-typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+constexpr int MMA_FLUSH = 32;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+// x = hi + lo, both FP16: 22 significant bits
+__device__ __forceinline__ void split16(float x, halfx8& hi, halfx8& lo, int u) {
+    const _Float16 h = static_cast<_Float16>(x);
+    hi[u] = h;
+    lo[u] = static_cast<_Float16>(x - static_cast<float>(h));
+}
 
 template <bool SECOND>
 __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the workgroup's current and next tile (all its waves walk the same
+    // tiles), and the FP32 running sums of each lane's 16 pixels
+    __shared__ AtomRec tile_buf[2][TILE];
+    __shared__ float2 lacc[MMA_WAVES][16][64];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     const int patch = blockIdx.x * MMA_WAVES + wave;
-    if (patch >= A.n_patches) return;          // whole wave; no block barriers below
+    const bool live = patch < A.n_patches;       // idle waves still take part in the barriers
     const int frame = blockIdx.z;
     const int slice = blockIdx.y;
     const FrameRec& F = A.frames[frame];
     const int t0 = slice * A.tiles_per_slice;
     const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
-    const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
+    const int pc = live ? patch : 0;
+    const PatchGeom P = patch_of(pc % A.pby, pc / A.pby, A.Ny, A.Nz);
     const int Ny = A.Ny;
 
-    // operand roles: block h = lane / 32 holds Re (h = 0) or Im (h = 1) rows
-    // of A; row a and column b of this lane are both q
-    const int h = lane >> 5;
+    // operand roles: A row a = q and B column b = q; this lane's atoms in a
+    // 16-atom step are 8 g .. 8 g + 7
+    const int g = lane >> 5;
     const int q = lane & 31;
-    const float qa = h ? -0.25f : 0.f;          // A value: Re or Im part
-    const float qb1 = qa;                        // B in product 1: the same part
-    const float qb2 = h ? 0.f : -0.25f;          // B in product 2: the other part
     const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
     const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
     const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
@@ -1130,12 +1140,12 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
         const double fa = q - P.my, fb = q - P.mz;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double g, hq;
-            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, g, hq);
-            va0[d] = vs[0][d] + fa * (g + fa * hq);
+            double gq, hq;
+            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gq, hq);
+            va0[d] = vs[0][d] + fa * (gq + fa * hq);
             vc0[d] = vs[0][d];
-            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, g, hq);
-            wb[d] = static_cast<float>(fb * (g + fb * hq));
+            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gq, hq);
+            wb[d] = static_cast<float>(fb * (gq + fb * hq));
         }
     }
     const double4 um = A.u0h[pmid];              // middle pixel: direction, 1 / (2 rho)
@@ -1146,13 +1156,10 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
     const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
     const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
 
-    // result registers r of this lane: pixel row a(r) = 8 (r / 4) + 4 h + r % 4,
+    // result register r of this lane: pixel row a(r) = 8 (r / 4) + 4 g + r % 4,
     // column q
     const int prow = (P.z0 + min(q, P.vz - 1)) * Ny + P.y0;
-    auto arow = [&](int r) { return 8 * (r >> 2) + 4 * h + (r & 3); };
-    // FP32 running sums of the lane's 16 pixels live in LDS (registers go
-    // to the matrix accumulators)
-    __shared__ float2 lacc[MMA_WAVES][16][64];
+    auto arow = [&](int r) { return 8 * (r >> 2) + 4 * g + (r & 3); };
     float2 (*acc)[64] = lacc[wave];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r][lane] = make_float2(0.f, 0.f);
@@ -1160,113 +1167,132 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
     bool first = true;
     int since = 0;
 
+    if (t0 < t1) tile_buf[0][tid] = A.atoms[static_cast<int64_t>(t0) * TILE + tid];
+    __syncthreads();
     for (int t = t0; t < t1; ++t) {
+        const int buf = (t - t0) & 1;
+        const bool has_next = t + 1 < t1;
+        // this thread's record of the next tile, in flight during this one
+        const float4* nsrc = reinterpret_cast<const float4*>(
+            A.atoms + static_cast<int64_t>(has_next ? t + 1 : t) * TILE + tid);
+        const float4 n0 = nsrc[0], n1 = nsrc[1];
         const TileRec tr = A.tiles[t];
-        // keeps the table loads below inside the loop (count <= 256: shift 0)
-        const int tie = tr.count >> 16;
-        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
-        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
-        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
-        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
-        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
-        double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
-        ph_t -= floor(ph_t);
-        // first-order change of the direction for this tile origin, constant
-        // over the patch: R^T (O - (u.O) u) / rho
-        const double h2 = 2.0 * um.w;
-        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
-                     czl = (Oz - uo * um.z) * h2;
-        const float va_x = static_cast<float>(va0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
-        const float va_y = static_cast<float>(va0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
-        const float va_z = static_cast<float>(va0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
-        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
-                    koz = static_cast<float>(A.K * Oz);
-        const float pht = static_cast<float>(ph_t);
+        if (live) {
+            // keeps the table loads below inside the loop (count <= 256: shift 0)
+            const int tie = tr.count >> 16;
+            const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+            const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+            const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+            const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+            const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+            double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+            ph_t -= floor(ph_t);
+            // first-order change of the direction for this tile origin,
+            // constant over the patch: R^T (O - (u.O) u) / rho
+            const double h2 = 2.0 * um.w;
+            const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                         czl = (Oz - uo * um.z) * h2;
+            const float va_x = static_cast<float>(va0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+            const float va_y = static_cast<float>(va0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+            const float va_z = static_cast<float>(va0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+            const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                        koz = static_cast<float>(A.K * Oz);
+            const float pht = static_cast<float>(ph_t);
 
-        floatx32 c1, c2;
+            floatx16 crr, cii, cri, cir;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) { c1[r] = 0.f; c2[r] = 0.f; }
-        const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
-        const int cnt = tr.count;
-        // four wave-uniform records per step; records past the count are zero
-        // (mag = 0) and j + 3 <= 255
+            for (int r = 0; r < 16; ++r) { crr[r] = 0.f; cii[r] = 0.f; cri[r] = 0.f; cir[r] = 0.f; }
+            const AtomRec* at = tile_buf[buf] + 8 * g;
+            const int cnt = tr.count;
+            // records past the count are zero (mag = 0); j + 15 <= 255
 #pragma unroll 1
-        for (int j = 0; j < cnt; j += 4) {
-            halfx4 ah, al, b1h, b1l, b2h, b2l;
+            for (int j = 0; j < cnt; j += 16) {
+                halfx8 arh, arl, aih, ail, brh, brl, bih, bil;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const AtomRec a = at[j + u];
-                float pa = fmaf(va_z, a.sz, fmaf(va_y, a.sy, va_x * a.sx));
-                if (SECOND) {
-                    const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
-                    pa = fmaf(hh, fmaf(-w, w, a.s2), pa);
+                for (int u = 0; u < 8; ++u) {
+                    const AtomRec a = at[j + u];
+                    float pa = fmaf(va_z, a.sz, fmaf(va_y, a.sy, va_x * a.sx));
+                    if (SECOND) {
+                        const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
+                        pa = fmaf(hh, fmaf(-w, w, a.s2), pa);
+                    }
+                    const float fa = __builtin_amdgcn_fractf(pa + a.ph);
+                    const float fb = __builtin_amdgcn_fractf(
+                        fmaf(wb[2], a.sz, fmaf(wb[1], a.sy, wb[0] * a.sx)));
+                    split16(a.mag * __builtin_amdgcn_cosf(fa), arh, arl, u);
+                    split16(a.mag * __builtin_amdgcn_sinf(fa), aih, ail, u);
+                    split16(__builtin_amdgcn_cosf(fb), brh, brl, u);
+                    split16(__builtin_amdgcn_sinf(fb), bih, bil, u);
                 }
-                pa += a.ph + qa;
-                const float pb = fmaf(wb[2], a.sz, fmaf(wb[1], a.sy, wb[0] * a.sx));
-                const float av = a.mag * __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pa));
-                const float bv1 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb + qb1));
-                const float bv2 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(pb + qb2));
-                ah[u] = static_cast<_Float16>(av);
-                al[u] = static_cast<_Float16>(av - static_cast<float>(ah[u]));
-                b1h[u] = static_cast<_Float16>(bv1);
-                b1l[u] = static_cast<_Float16>(bv1 - static_cast<float>(b1h[u]));
-                b2h[u] = static_cast<_Float16>(bv2);
-                b2l[u] = static_cast<_Float16>(bv2 - static_cast<float>(b2h[u]));
+                crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brh, crr, 0, 0, 0);
+                cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bih, cii, 0, 0, 0);
+                cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, bih, cri, 0, 0, 0);
+                cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brh, cir, 0, 0, 0);
+                crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brl, crr, 0, 0, 0);
+                cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bil, cii, 0, 0, 0);
+                cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, bil, cri, 0, 0, 0);
+                cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, brl, cir, 0, 0, 0);
+                crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl, brh, crr, 0, 0, 0);
+                cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, bih, cii, 0, 0, 0);
+                cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl, bih, cri, 0, 0, 0);
+                cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail, brh, cir, 0, 0, 0);
             }
-            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b1h, c1, 0, 0, 0);
-            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b2h, c2, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b1l, c1, 0, 0, 0);
-            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(ah, b2l, c2, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x4f16(al, b1h, c1, 0, 0, 0);
-            c2 = __builtin_amdgcn_mfma_f32_32x32x4f16(al, b2h, c2, 0, 0, 0);
-        }
 
-        // S(a(r), q) = (C1_0 - C1_1) + i (C2_0 + C2_1); times the origin
-        // phasor and G of the pixel
-        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
-        float s_re[16], s_im[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            s_re[r] = c1[r] - c1[16 + r];
-            s_im[r] = c2[r] + c2[16 + r];
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            // four pixels at a time: bounds the table loads in flight
-            if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-            const float sr = s_re[r], si = s_im[r];
-            const int p = prow + min(arow(r), P.vy - 1) + tie;
-            const float4 du = A.dup[p];
-            const float phs = __builtin_amdgcn_fractf(
-                pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
-            const float sn = __builtin_amdgcn_sinf(phs);
-            const float cs = __builtin_amdgcn_cosf(phs);
-            const float2 g = Gm[p];
-            const float zr = g.x * cs - g.y * sn, zi = g.x * sn + g.y * cs;
-            float2 av = acc[r][lane];
-            av.x = fmaf(zr, sr, fmaf(-zi, si, av.x));
-            av.y = fmaf(zr, si, fmaf(zi, sr, av.y));
-            acc[r][lane] = av;
-        }
-        if (++since == MMA_FLUSH || t + 1 == t1) {
+            // S(a(r), q) = (C_rr - C_ii) + i (C_ri + C_ir); times the origin
+            // phasor and G of the pixel
+            const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+            float s_re[16], s_im[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
-                const float2 av = acc[r][lane];
-                if (arow(r) < P.vy && q < P.vz) {
-                    double2* o = out + prow + arow(r);
-                    double2 v = first ? make_double2(0.0, 0.0) : *o;
-                    v.x += static_cast<double>(av.x);
-                    v.y += static_cast<double>(av.y);
-                    *o = v;
-                }
-                acc[r][lane] = make_float2(0.f, 0.f);
+                s_re[r] = crr[r] - cii[r];
+                s_im[r] = cri[r] + cir[r];
             }
-            first = false;
-            since = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                // four pixels at a time: bounds the table loads in flight
+                if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                const float sr = s_re[r], si = s_im[r];
+                const int p = prow + min(arow(r), P.vy - 1) + tie;
+                const float4 du = A.dup[p];
+                const float phs = __builtin_amdgcn_fractf(
+                    pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+                const float sn = __builtin_amdgcn_sinf(phs);
+                const float cs = __builtin_amdgcn_cosf(phs);
+                const float2 gg = Gm[p];
+                const float zr = gg.x * cs - gg.y * sn, zi = gg.x * sn + gg.y * cs;
+                float2 av = acc[r][lane];
+                av.x = fmaf(zr, sr, fmaf(-zi, si, av.x));
+                av.y = fmaf(zr, si, fmaf(zi, sr, av.y));
+                acc[r][lane] = av;
+            }
+            if (++since == MMA_FLUSH || t + 1 == t1) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                    const float2 av = acc[r][lane];
+                    if (arow(r) < P.vy && q < P.vz) {
+                        double2* o = out + prow + arow(r);
+                        double2 v = first ? make_double2(0.0, 0.0) : *o;
+                        v.x += static_cast<double>(av.x);
+                        v.y += static_cast<double>(av.y);
+                        *o = v;
+                    }
+                    acc[r][lane] = make_float2(0.f, 0.f);
+                }
+                first = false;
+                since = 0;
+            }
         }
+        // the other buffer was last read in the previous tile, which every
+        // wave finished before the previous barrier
+        if (has_next) {
+            float4* dst = reinterpret_cast<float4*>(&tile_buf[buf ^ 1][tid]);
+            dst[0] = n0;
+            dst[1] = n1;
+        }
+        __syncthreads();
     }
-    if (first) {                                  // a slice without tiles
+    if (live && first) {                          // a slice without tiles
 #pragma unroll
         for (int r = 0; r < 16; ++r)
             if (arow(r) < P.vy && q < P.vz) out[prow + arow(r)] = make_double2(0.0, 0.0);
