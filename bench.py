@@ -27,8 +27,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLOP_PER_PAIR = 20.0
-PEAK_FP32 = 157.3e12
+FLOP_PER_PAIR = 20.0          # algorithmic FP32 work per atom-pixel pair (20 flop + 2 trans.)
+PEAK_FP32 = 157.3e12          # MI355X FP32 (vector and dense matrix)
+PEAK_FP16_MFMA = 2.5e15       # MI355X dense FP16 matrix
+# The matrix-core sum executes 3 split products x 4 real multiply-adds per pair.
+MFMA_FLOP_PER_PAIR = 24.0
+# HBM bytes per launch of the matrix-core sum on the 5M-atom, 512^2 workload,
+# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (the copy-kernel calibration
+# reads FETCH_SIZE x 2 and WRITE_SIZE x 1; profiles/k_sum_mma_perovskite5M_512px.md).
+MMA_TRAFFIC_BYTES = (2 * 1831133.0 + 2542443.8) * 1024
 
 
 def parse():
@@ -227,6 +234,27 @@ def bench_scan(steps, warmup):
             "pairs_per_s": pairs / (ms * 1e-3), "kernel_path": sg.path_used}
 
 
+def roofline(path_used, pairs, ksum_ms, workload):
+    """Roofline object of the dominant (sum) kernel: algorithmic flops at 20 per
+    pair against the peak of the unit the kernel runs on."""
+    t = ksum_ms * 1e-3
+    achieved = pairs * FLOP_PER_PAIR / t
+    name = kernel_name(path_used)
+    r = {"kernel": name, "kernel_path": path_used, "kernel_ms": ksum_ms,
+         "kernel_pairs_per_s": pairs / t, "unit": "TFLOP/s", "achieved": achieved / 1e12}
+    if name == "k_sum_mma":
+        r.update({"bound": "mfma", "peak": PEAK_FP16_MFMA / 1e12,
+                  "frac": achieved / PEAK_FP16_MFMA,
+                  "mfma_executed_tflops": pairs * MFMA_FLOP_PER_PAIR / t / 1e12,
+                  "fp32_equivalent_frac": achieved / PEAK_FP32,
+                  "traffic": MMA_TRAFFIC_BYTES if workload == "perovskite_screw_5M_512px"
+                  else None})
+    else:
+        r.update({"bound": "valu", "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32,
+                  "traffic": None})
+    return r
+
+
 def kernel_name(path_used):
     """Sum kernel behind a mosaic_kin_sum path code."""
     base = path_used % 100
@@ -418,7 +446,6 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     ksum_ms = float(np.mean(sum_ms))
     field = out[0].cpu().numpy()
-    achieved = pairs * FLOP_PER_PAIR / (ksum_ms * 1e-3)
     res = {
         "metric": "atom*pixel pairs/s (kinematic sum)",
         "value": pairs / (ms_step * 1e-3),
@@ -434,11 +461,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "atoms": int(n_atoms), "pixels": [Ny, Nz],
                    "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": kernel_name(path_used[0]),
-                     "kernel_path": path_used[0], "kernel_ms": ksum_ms,
-                     "kernel_pairs_per_s": pairs / (ksum_ms * 1e-3)},
+        "roofline": roofline(path_used[0], pairs, ksum_ms, args.workload),
     }
     if not args.no_propagation:
         res["propagation"] = bench_propagation(args.steps, args.warmup, b.wavelength)

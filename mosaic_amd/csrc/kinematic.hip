@@ -528,62 +528,99 @@ __global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
 constexpr int KD_THREADS = 256;
 
 // Bounding boxes of the nodes that split at this level.  Node ids do not
-// decrease along the order, so a block (or a wave) whose first and last
-// positions share a node reduces locally and issues 6 atomics.
+// decrease along the order, so a 256-atom chunk whose first and last
+// positions share a node reduces locally: a workgroup walks KD_CHUNKS chunks
+// and keeps a running box while the node stays the same, issuing 6 atomics
+// per node it leaves (the top levels have a handful of nodes, so atomics per
+// chunk would serialise on them); chunks that straddle nodes fall back to
+// per-wave or per-lane atomics.
+constexpr int KD_CHUNKS = 16;
+
+__device__ __forceinline__ void kd_box_flush(uint32_t node, unsigned int (&lo)[3],
+                                             unsigned int (&hi)[3], unsigned int (*red)[6],
+                                             const uint32_t* __restrict__ split,
+                                             unsigned int* __restrict__ box) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_umin(lo[d]);
+        hi[d] = wave_umax(hi[d]);
+    }
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
+    __syncthreads();
+    if (threadIdx.x == 0 && split[node] != NO_SPLIT) {
+        for (int d = 0; d < 3; ++d) {
+            unsigned int a = red[0][d], b = red[0][3 + d];
+            for (int w = 1; w < KD_THREADS / 64; ++w) {
+                a = min(a, red[w][d]);
+                b = max(b, red[w][3 + d]);
+            }
+            atomicMin(box + 6 * static_cast<int64_t>(node) + d, a);
+            atomicMax(box + 6 * static_cast<int64_t>(node) + 3 + d, b);
+        }
+    }
+    __syncthreads();
+    for (int d = 0; d < 3; ++d) { lo[d] = 0xFFFFFFFFu; hi[d] = 0u; }
+}
+
 __global__ __launch_bounds__(KD_THREADS) void k_kd_box(
     const float* __restrict__ pos, const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split, int64_t n,
     unsigned int* __restrict__ box) {
     __shared__ unsigned int red[KD_THREADS / 64][6];
-    const int64_t i0 = blockIdx.x * (int64_t)KD_THREADS;
-    const int64_t i = i0 + threadIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool in = i < n;
-    const uint32_t s = in ? seg[i] : NO_SPLIT;
-    const bool active = in && split[s] != NO_SPLIT;
-    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-    if (active) {
-        const uint32_t a = order[i];
-        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
-    }
-    const int64_t i_last = min(i0 + KD_THREADS, n) - 1;
-    const uint32_t sb0 = seg[i0], sb1 = seg[i_last];
-    const bool block_uniform = sb0 == sb1 && i0 + KD_THREADS <= n;
-    const uint32_t w0 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 0, 64));
-    const uint32_t w1 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 63, 64));
-    if (block_uniform || w0 == w1) {
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = wave_umin(lo[d]);
-            hi[d] = wave_umax(hi[d]);
+    const int lane = threadIdx.x & 63;
+    const int64_t base = blockIdx.x * static_cast<int64_t>(KD_THREADS) * KD_CHUNKS;
+    uint32_t cur = NO_SPLIT;                      // node of the running box (block-uniform)
+    unsigned int rlo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, rhi[3] = {0u, 0u, 0u};
+    for (int c = 0; c < KD_CHUNKS; ++c) {
+        const int64_t i0 = base + static_cast<int64_t>(c) * KD_THREADS;
+        if (i0 >= n) break;
+        const int64_t i = i0 + threadIdx.x;
+        const bool in = i < n;
+        const uint32_t s = in ? seg[i] : NO_SPLIT;
+        const bool active = in && split[s] != NO_SPLIT;
+        unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+        if (active) {
+            const uint32_t a = order[i];
+            for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
         }
-    }
-    if (block_uniform) {
-        if (lane == 0)
-            for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
-        __syncthreads();
-        if (threadIdx.x == 0 && split[sb0] != NO_SPLIT) {
-            for (int d = 0; d < 3; ++d) {
-                unsigned int a = red[0][d], b = red[0][3 + d];
-                for (int w = 1; w < KD_THREADS / 64; ++w) {
-                    a = min(a, red[w][d]);
-                    b = max(b, red[w][3 + d]);
-                }
-                atomicMin(box + 6 * static_cast<int64_t>(sb0) + d, a);
-                atomicMax(box + 6 * static_cast<int64_t>(sb0) + 3 + d, b);
+        const int64_t i_last = min(i0 + KD_THREADS, n) - 1;
+        const uint32_t sb0 = seg[i0], sb1 = seg[i_last];
+        if (sb0 == sb1 && i0 + KD_THREADS <= n) {
+            if (sb0 != cur) {
+                if (cur != NO_SPLIT) kd_box_flush(cur, rlo, rhi, red, split, box);
+                cur = sb0;
             }
+            for (int d = 0; d < 3; ++d) {
+                rlo[d] = min(rlo[d], lo[d]);
+                rhi[d] = max(rhi[d], hi[d]);
+            }
+            continue;
         }
-    } else if (w0 == w1) {
-        if (lane == 0 && active)
+        if (cur != NO_SPLIT) {
+            kd_box_flush(cur, rlo, rhi, red, split, box);
+            cur = NO_SPLIT;
+        }
+        const uint32_t w0 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 0, 64));
+        const uint32_t w1 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 63, 64));
+        if (w0 == w1) {
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = wave_umin(lo[d]);
+                hi[d] = wave_umax(hi[d]);
+            }
+            if (lane == 0 && active)
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
+                    atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
+                }
+        } else if (active) {
             for (int d = 0; d < 3; ++d) {
                 atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
                 atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
             }
-    } else if (active) {
-        for (int d = 0; d < 3; ++d) {
-            atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
-            atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
         }
     }
+    if (cur != NO_SPLIT) kd_box_flush(cur, rlo, rhi, red, split, box);
 }
 
 __global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
@@ -1696,8 +1733,9 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
             const uint32_t* cut = L.kd_split + kd_split_count(n_species, l);
             hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0, st,
                                L.kd_box, 6 * nodes);
-            hipLaunchKernelGGL(k_kd_box, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg,
-                               cut, atom_count, L.kd_box);
+            hipLaunchKernelGGL(k_kd_box, dim3(blocks_for(atom_count, KD_THREADS * KD_CHUNKS)),
+                               dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg, cut, atom_count,
+                               L.kd_box);
             hipLaunchKernelGGL(k_kd_keys, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b,
                                L.seg, cut, L.kd_box, atom_count, L.keys_a, L.vals_a);
             MOSAIC_HIP_CHECK(hipGetLastError());

@@ -141,6 +141,30 @@ def test_mma_ragged_patches(device, shape):
     assert e <= TOL
 
 
+def test_kd_tiles_are_compact(device):
+    """The k-d bisection keeps every 256-atom tile near the radius of a cube
+    of 256 atoms (17 wavelengths for the 32,000-atom crystal, 21 for the
+    perovskite's sparsest species; a Morton order gives 46 and 95)."""
+    for w, bound in ((W.fcc_au_bragg111(n=20, npix=8), 20.0),
+                     (W.perovskite_screw_bragg200(n=24, npix=8, blocks=(2, 2, 2)), 25.0)):
+        b, d = w["beam"], w["detector"]
+        pos, codes, counts = [], [], None
+        smp = w["sample"]
+        for cid in range(1, smp.chunk_total + 1):
+            pos.append(smp.load_chunk_positions(cid))
+            codes.append(smp.load_chunk_species(cid, codes=True)[0])
+        pos = np.concatenate(pos).astype(np.float32)
+        codes = np.concatenate(codes).astype(np.uint8)
+        order = np.argsort(codes, kind="stable")
+        pos, codes = pos[order], codes[order]
+        counts = np.bincount(codes).astype(np.int64)
+        k = 2 * np.pi / b.wavelength
+        st = K.stage(torch.as_tensor(pos, device=device), torch.as_tensor(codes, device=device),
+                     None, counts, k, 1, 8, 8)
+        print(f"[kd] {pos.shape[0]} atoms: largest tile radius {st.radius_wl:.1f} wavelengths")
+        assert 0.0 < st.radius_wl <= bound
+
+
 def test_restage_moved_atoms(device):
     """Tiles rebuilt from displaced positions with the original k-d order
     give the field of the displaced crystal (long-double oracle), with tile
