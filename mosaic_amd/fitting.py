@@ -51,7 +51,7 @@ class KinematicModel:
         self.k = 2.0 * np.pi / beam.wavelength
         self.spc = K.species_table(tables.species_constants(list(labels), beam.energy))
         self.opts = K.kin_opts(self.k, remove_forward, apply_polarization,
-                               beam._polarization_state(), False, path="rows")
+                               beam._polarization_state(), False, path="auto")
         self.coords = torch.as_tensor(np.ascontiguousarray(detector.pixel_coordinates / 1e10),
                                       device=self.device)
         E0 = torch.as_tensor(beam._beam_E0_map, device=self.device)
