@@ -53,8 +53,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--surrogate", action="store_true",
                     help="run the surrogate workload (data generation, training, gated scan)")
-    ap.add_argument("--sg-images", type=int, default=20000)
-    ap.add_argument("--sg-epochs", type=int, default=40)
+    ap.add_argument("--sg-images", type=int, default=40000)
+    ap.add_argument("--sg-epochs", type=int, default=150)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to HBM bytes)")

@@ -164,16 +164,19 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
  * mosaic_kin_sum produced from the same workspace, adds dL/du (device (N, 3)
  * float32, per Angstrom, original atom order) into grad_pos.  Phases are
  * recomputed, not stored.  No counterpart in the reference, which has no
- * gradient path; it serves displacement-field fitting.  Declines
- * (MOSAIC_ERR_UNSUPPORTED) outside the far-field bounds of the row
- * recurrence.
+ * gradient path; it serves displacement-field fitting.  Two forms, chosen
+ * like the forward's (`path` 0 automatic, 2 row recurrence, 3 matrix-core):
+ * the matrix-core form contracts per-patch weight matrices with the atoms'
+ * separable phasors on the FP16-split MFMA; the row form walks row blocks
+ * on the VALU.  Declines (MOSAIC_ERR_UNSUPPORTED) outside the far-field
+ * bounds of the chosen form.
  */
 int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
                        const float* rot /* host */, const float* trans /* host */, int Ny,
                        int Nz, const double* stage_stats /* host [2] */,
                        const double* pixel_geom /* host [4] */, const float* grad_field,
-                       float* grad_pos, void* workspace, size_t workspace_bytes,
+                       float* grad_pos, void* workspace, size_t workspace_bytes, int path,
                        mosaic_stream_t stream);
 
 /*

@@ -1583,6 +1583,285 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
     }
 }
 
+// Matrix-core adjoint.  With the forward's separable patch model the atom
+// phasor is A_j(a) Bz_j(b), A_j(a) = e^{2 pi i va(a).sigma_j} (+ the atom's
+// second-order phase), Bz_j(b) = e^{2 pi i wb(b).sigma_j}, and the gradient
+// direction is v(a, b) = va(a) + wb(b).  Per frame, patch and tile, with the
+// pixel weights W(a, b) = conj(g) G' e^{2 pi i (tile origin phase)},
+//     Z_j = sum_ab v(a, b) W(a, b) A_j(a) Bz_j(b)
+//         = vc S0 + gy S1 + hy S2 + gz U1 + hz U2,
+//     S_n = sum_a a^n A_j(a) Y0_j(a),  U_n = sum_a A_j(a) Y_n_j(a),
+//     Y_n_j(a) = sum_b W(a, b) b^n Bz_j(b)   (a, b offsets from the middle),
+// so three complex products per block of 32 atoms, W (32 x 32, the same for
+// every atom) times [Bz, b Bz, b^2 Bz] (32 x 32 atoms), on the FP16-split
+// matrix cores as in the forward, followed by a contraction over a in the
+// lanes that hold each atom's column.  One workgroup owns one tile; its four
+// waves split the (frame, patch) pairs and accumulate Z per atom in FP64 in
+// LDS; the tile's atoms are written once at the end.
+struct AdjMmaArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float4* dup;
+    const float2* gmma;
+    const float2* gin;     // [frame][pixel] dL/dA in the Re + i Im convention
+    const float* amax;     // [2][AMAX_BLOCKS] block maxima of |gin| and |G|
+    float* grad;           // [atom][3], accumulated into
+    int n_frames;
+    int Ny;
+    int Nz;
+    int npix;
+    int pby;
+    int n_patches;
+    double K;
+};
+
+template <bool SECOND>
+__global__ __launch_bounds__(64 * MMA_WAVES) void k_adjoint_mma(const AdjMmaArgs A) {
+    __shared__ AtomRec tile[TILE];
+    __shared__ double zacc[MMA_WAVES][TILE][6];
+    __shared__ double red[MMA_WAVES];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int t = blockIdx.x;
+    tile[tid] = A.atoms[static_cast<int64_t>(t) * TILE + tid];
+    for (int i = tid; i < MMA_WAVES * TILE * 6; i += 64 * MMA_WAVES) (&zacc[0][0][0])[i] = 0.0;
+    float mg = 0.f, mG = 0.f;
+    for (int i = tid; i < AMAX_BLOCKS; i += 64 * MMA_WAVES) {
+        mg = fmaxf(mg, A.amax[i]);
+        mG = fmaxf(mG, A.amax[AMAX_BLOCKS + i]);
+    }
+    const float sg = pow2_scale(static_cast<float>(block_max(static_cast<double>(mg), red)));
+    const float sG = pow2_scale(static_cast<float>(block_max(static_cast<double>(mG), red)));
+    __syncthreads();
+
+    const TileRec tr = A.tiles[t];
+    const int cnt = tr.count;
+    const int nblk = (cnt + 31) >> 5;
+    const float2* __restrict__ Gt = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+    const int h = lane >> 5, q = lane & 31;
+    const int Ny = A.Ny;
+    const int npairs = A.n_frames * A.n_patches;
+
+    for (int pr = wave; pr < npairs; pr += MMA_WAVES) {
+        const int f = pr / A.n_patches, patch = pr % A.n_patches;
+        const FrameRec& F = A.frames[f];
+        const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
+        const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+        const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+        const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+        const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+
+        // direction model (sample frame) from the exact direction at the
+        // middle pixel and the ends of the middle row and column
+        float vc[3], gy[3], hy[3], gz[3], hz[3];
+        double vc0[3];
+        {
+            const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                                 pmid + (P.vz - 1 - P.mz) * Ny};
+            double vs[5][3];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const double4 u = A.u0h[samp[k]];
+                const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
+                vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
+                vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
+                vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double gq, hq;
+                quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gq, hq);
+                gy[d] = static_cast<float>(gq);
+                hy[d] = static_cast<float>(hq);
+                quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gq, hq);
+                gz[d] = static_cast<float>(gq);
+                hz[d] = static_cast<float>(hq);
+                vc0[d] = vs[0][d];
+            }
+        }
+        // this tile's origin: phase at the middle pixel, direction shift
+        const double4 um = A.u0h[pmid];
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+        double ph_t = A.theta0[pmid] + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+        ph_t -= floor(ph_t);
+        const double h2 = 2.0 * um.w;
+        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                     czl = (Oz - uo * um.z) * h2;
+        vc[0] = static_cast<float>(vc0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+        vc[1] = static_cast<float>(vc0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+        vc[2] = static_cast<float>(vc0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+        const float xc0 = static_cast<float>(F.xs[0] - vc0[0]);
+        const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
+        const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
+        const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
+        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                    koz = static_cast<float>(A.K * Oz);
+        const float pht = static_cast<float>(ph_t);
+
+        // W(a = q, b) for this lane's b = 16 s + 8 h + u, split into FP16
+        // halves (zero outside the detector)
+        halfx8 wrh[2], wrl[2], wih[2], wil[2];
+        const float2* __restrict__ gf = A.gin + static_cast<int64_t>(f) * A.npix;
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if ((u & 3) == 0) __builtin_amdgcn_sched_barrier(0);   // bounds loads in flight
+                const int b = 16 * st + 8 * h + u;
+                const bool okp = q < P.vy && b < P.vz;
+                const int p = (P.z0 + min(b, P.vz - 1)) * Ny + P.y0 + min(q, P.vy - 1);
+                float wr = 0.f, wi = 0.f;
+                if (okp) {
+                    const float4 du = A.dup[p];
+                    const float phs = __builtin_amdgcn_fractf(
+                        pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+                    const float sn = __builtin_amdgcn_sinf(phs), cs = __builtin_amdgcn_cosf(phs);
+                    const float2 g0 = Gt[p], q0 = gf[p];
+                    const float gr = g0.x * sG, gi = g0.y * sG, qr = q0.x * sg, qi = q0.y * sg;
+                    const float zr = gr * cs - gi * sn, zi = gr * sn + gi * cs;
+                    wr = qr * zr + qi * zi;          // conj(g) z
+                    wi = qr * zi - qi * zr;
+                }
+                split16(wr, wrh[st], wrl[st], u);
+                split16(wi, wih[st], wil[st], u);
+            }
+        }
+
+        for (int blk = 0; blk < nblk; ++blk) {
+            // B operand lanes: atom blk * 32 + q (records past the count are
+            // zero and their Z is not kept)
+            // per-lane row and column offsets are rebuilt per block (tied to
+            // blk, which is < 65536): hoisted, they would hold ~50 registers
+            const int tie = blk >> 16;
+            const AtomRec a = tile[blk * 32 + q];
+            const float d1 = fmaf(gz[2], a.sz, fmaf(gz[1], a.sy, gz[0] * a.sx));
+            const float d2 = fmaf(hz[2], a.sz, fmaf(hz[1], a.sy, hz[0] * a.sx));
+            float bzr[2][8], bzi[2][8];
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const float bo = static_cast<float>(16 * st + 8 * h + u - P.mz + tie);
+                    const float fb = __builtin_amdgcn_fractf(bo * fmaf(bo, d2, d1));
+                    bzr[st][u] = __builtin_amdgcn_cosf(fb);
+                    bzi[st][u] = __builtin_amdgcn_sinf(fb);
+                }
+            // A_j(a) at this lane's result rows a(r) = 8 (r / 4) + 4 h + r % 4
+            const float e0 = fmaf(vc[2], a.sz, fmaf(vc[1], a.sy, vc[0] * a.sx));
+            const float e1 = fmaf(gy[2], a.sz, fmaf(gy[1], a.sy, gy[0] * a.sx));
+            const float e2 = fmaf(hy[2], a.sz, fmaf(hy[1], a.sy, hy[0] * a.sx));
+            float e2nd = 0.f;
+            if (SECOND) {
+                const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx)) - e0;
+                e2nd = hh * fmaf(-w, w, a.s2);
+            }
+            float ar[16], ai[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float ao = static_cast<float>(8 * (r >> 2) + 4 * h + (r & 3) - P.my + tie);
+                const float fa = __builtin_amdgcn_fractf(fmaf(ao, fmaf(ao, e2, e1), e0 + e2nd));
+                ar[r] = __builtin_amdgcn_cosf(fa);
+                ai[r] = __builtin_amdgcn_sinf(fa);
+            }
+            float s0r = 0.f, s0i = 0.f, s1r = 0.f, s1i = 0.f, s2r = 0.f, s2i = 0.f;
+            float u1r = 0.f, u1i = 0.f, u2r = 0.f, u2i = 0.f;
+#pragma unroll 1
+            for (int n = 0; n < 3; ++n) {
+                floatx16 crr, cii, cri, cir;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) { crr[r] = 0.f; cii[r] = 0.f; cri[r] = 0.f; cir[r] = 0.f; }
+#pragma unroll
+                for (int st = 0; st < 2; ++st) {
+                    halfx8 brh, brl, bih, bil;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float bo = static_cast<float>(16 * st + 8 * h + u - P.mz + tie + n * tie);
+                        const float m = n == 0 ? 1.f : (n == 1 ? bo : bo * bo);
+                        split16(m * bzr[st][u], brh, brl, u);
+                        split16(m * bzi[st][u], bih, bil, u);
+                    }
+                    crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrh[st], brh, crr, 0, 0, 0);
+                    cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(wih[st], bih, cii, 0, 0, 0);
+                    cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrh[st], bih, cri, 0, 0, 0);
+                    cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(wih[st], brh, cir, 0, 0, 0);
+                    crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrh[st], brl, crr, 0, 0, 0);
+                    cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(wih[st], bil, cii, 0, 0, 0);
+                    cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrh[st], bil, cri, 0, 0, 0);
+                    cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(wih[st], brl, cir, 0, 0, 0);
+                    crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrl[st], brh, crr, 0, 0, 0);
+                    cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(wil[st], bih, cii, 0, 0, 0);
+                    cri = __builtin_amdgcn_mfma_f32_32x32x16_f16(wrl[st], bih, cri, 0, 0, 0);
+                    cir = __builtin_amdgcn_mfma_f32_32x32x16_f16(wil[st], brh, cir, 0, 0, 0);
+                }
+                // Y_n(a(r)) = (C_rr - C_ii) + i (C_ri + C_ir); P = A Y_n
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float yr = crr[r] - cii[r], yi = cri[r] + cir[r];
+                    const float pr_ = ar[r] * yr - ai[r] * yi, pi_ = ar[r] * yi + ai[r] * yr;
+                    if (n == 0) {
+                        const float ao =
+                            static_cast<float>(8 * (r >> 2) + 4 * h + (r & 3) - P.my + tie);
+                        s0r += pr_;
+                        s0i += pi_;
+                        s1r = fmaf(ao, pr_, s1r);
+                        s1i = fmaf(ao, pi_, s1i);
+                        s2r = fmaf(ao * ao, pr_, s2r);
+                        s2i = fmaf(ao * ao, pi_, s2i);
+                    } else if (n == 1) {
+                        u1r += pr_;
+                        u1i += pi_;
+                    } else {
+                        u2r += pr_;
+                        u2i += pi_;
+                    }
+                }
+            }
+            // the other 16 rows of this atom's column are in lane ^ 32
+            s0r += __shfl_xor(s0r, 32, 64); s0i += __shfl_xor(s0i, 32, 64);
+            s1r += __shfl_xor(s1r, 32, 64); s1i += __shfl_xor(s1i, 32, 64);
+            s2r += __shfl_xor(s2r, 32, 64); s2i += __shfl_xor(s2i, 32, 64);
+            u1r += __shfl_xor(u1r, 32, 64); u1i += __shfl_xor(u1i, 32, 64);
+            u2r += __shfl_xor(u2r, 32, 64); u2i += __shfl_xor(u2i, 32, 64);
+            const int j = blk * 32 + q;
+            if (h == 0 && j < cnt) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const float zr = vc[d] * s0r + gy[d] * s1r + hy[d] * s2r + gz[d] * u1r + hz[d] * u2r;
+                    const float zi = vc[d] * s0i + gy[d] * s1i + hy[d] * s2i + gz[d] * u1i + hz[d] * u2i;
+                    zacc[wave][j][2 * d] += static_cast<double>(zr);
+                    zacc[wave][j][2 * d + 1] += static_cast<double>(zi);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < cnt) {
+        const AtomRec a = tile[tid];
+        double Z[6];
+        for (int c = 0; c < 6; ++c) {
+            Z[c] = 0.0;
+            for (int w = 0; w < MMA_WAVES; ++w) Z[c] += zacc[w][tid][c];
+        }
+        const double scale = -2.0 * PI * A.K * 1e-10 /
+                             (static_cast<double>(sg) * static_cast<double>(sG));
+        const uint32_t idx = __float_as_uint(a.pad0);
+        double wi, wr;
+        sincospi(2.0 * static_cast<double>(a.ph), &wi, &wr);
+        wr *= a.mag;
+        wi *= a.mag;
+        for (int c = 0; c < 3; ++c) {
+            const double g = scale * (wr * Z[2 * c + 1] + wi * Z[2 * c]);
+            A.grad[3 * static_cast<int64_t>(idx) + c] += static_cast<float>(g);
+        }
+    }
+}
+
 // Sum slice partials in slice order and add into the output field.
 template <bool C128>
 __global__ void k_reduce(const double2* __restrict__ partial, int n_slices, int npix,
@@ -2047,13 +2326,14 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
                                   int Ny, int Nz, const double* stage_stats,
                                   const double* pixel_geom,
                                   const float* grad_field, float* grad_pos, void* workspace,
-                                  size_t workspace_bytes, mosaic_stream_t stream) {
+                                  size_t workspace_bytes, int path, mosaic_stream_t stream) {
     size_t need = 0;
     int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
     if (rc != MOSAIC_OK) return rc;
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
     if (!rot || !trans || !stage_stats || !pixel_geom || !grad_field || !grad_pos)
         return MOSAIC_ERR_ARGS;
+    if (path != 0 && path != 2 && path != 3) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
     rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
@@ -2082,15 +2362,51 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     const double second = radius_wl * radius_wl / (2.0 * Kk * rho_min);
     const bool use_second = second > 1e-7;
     const double omax = stage_stats[1] + tmax;
-    const bool ok = omax / rho_min <= 1e-6 &&
-                    Kk * omax * omax * omax / (rho_min * rho_min) <= 1e-8 &&
-                    6.2831853 * du1 * radius_wl <= 0.05;
+    const bool far = omax / rho_min <= 1e-6 &&
+                     Kk * omax * omax * omax / (rho_min * rho_min) <= 1e-8;
+    const bool ok = far && 6.2831853 * du1 * radius_wl <= 0.05;
+    // the matrix-core form under the forward's bounds, else the recurrence
+    const bool mma_ok = far && pixel_geom[3] * radius_wl <= 1e-6 &&
+                        Kk * omax * omax * (MMA_P * du1) / rho_min <= 1e-7;
     int kp = 0;
     if (ok && rec_curv_bound(16, du2, radius_wl) <= 1e-6) kp = 16;
     else if (ok && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
-    if (kp == 0) return MOSAIC_ERR_UNSUPPORTED;
+    const bool use_mma = path == 3 || (path == 0 && mma_ok);
+    if ((use_mma && !mma_ok) || (!use_mma && kp == 0)) return MOSAIC_ERR_UNSUPPORTED;
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
+    const int64_t n_gin0 = 2 * static_cast<int64_t>(n_frames) * Ny * Nz;
+    const int64_t n_gtab0 = 2 * static_cast<int64_t>(n_species) * Ny * Nz;
+    if (use_mma) {
+        AdjMmaArgs M;
+        M.atoms = L.atoms;
+        M.tiles = L.tiles;
+        M.frames = L.frames;
+        M.u0h = L.px.u0h;
+        M.theta0 = L.px.theta0;
+        M.dup = L.px.dup;
+        M.gmma = L.px.gmma;
+        M.gin = reinterpret_cast<const float2*>(grad_field);
+        M.amax = L.amax_blocks;
+        M.grad = grad_pos;
+        M.n_frames = n_frames;
+        M.Ny = Ny;
+        M.Nz = Nz;
+        M.npix = Ny * Nz;
+        M.pby = (Ny + MMA_P - 1) / MMA_P;
+        M.n_patches = M.pby * ((Nz + MMA_P - 1) / MMA_P);
+        M.K = Kk;
+        hipLaunchKernelGGL(k_amax, dim3(AMAX_BLOCKS), dim3(256), 0, st, grad_field, n_gin0,
+                           reinterpret_cast<const float*>(L.px.gmma), n_gtab0, L.amax_blocks);
+        if (use_second)
+            hipLaunchKernelGGL((k_adjoint_mma<true>), dim3(L.n_tiles), dim3(64 * MMA_WAVES), 0,
+                               st, M);
+        else
+            hipLaunchKernelGGL((k_adjoint_mma<false>), dim3(L.n_tiles), dim3(64 * MMA_WAVES), 0,
+                               st, M);
+        MOSAIC_HIP_CHECK(hipGetLastError());
+        return MOSAIC_OK;
+    }
     AdjArgs A;
     A.atoms = L.atoms;
     A.tiles = L.tiles;

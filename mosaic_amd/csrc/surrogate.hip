@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
     A += z * sA;
     B += z * sB;
     C += z * sC;
-    const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;   // M blocks on x (up to 2^31)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     f32x16 acc;
@@ -271,8 +271,8 @@ extern "C" int mosaic_sg_gemm(int transA, int transB, int M, int N, int K, const
     }
     const int kchunk = ((K + splits - 1) / splits + GBK - 1) / GBK * GBK;
     splits = (K + kchunk - 1) / kchunk;
-    dim3 grid((N + GBN - 1) / GBN, (M + GBM - 1) / GBM, batch * splits);
-    if (grid.y > 65535) return MOSAIC_ERR_ARGS;
+    dim3 grid((M + GBM - 1) / GBM, (N + GBN - 1) / GBN, batch * splits);
+    if (grid.y > 65535 || grid.z > 65535) return MOSAIC_ERR_ARGS;
     if (splits > 1) {
         const long long mn = static_cast<long long>(M) * N;
         hipLaunchKernelGGL(k_scale, dim3(static_cast<unsigned>((mn + 255) / 256), batch), dim3(256),

@@ -30,14 +30,16 @@ class _KinField(torch.autograd.Function):
     def backward(ctx, grad_A):
         m = ctx.model
         grad = torch.zeros((ctx.n, 3), dtype=torch.float32, device=grad_A.device)
-        K.kin_adjoint(ctx.staged, m.k, m.rot, m.trans, grad_A, grad)
+        K.kin_adjoint(ctx.staged, m.k, m.rot, m.trans, grad_A, grad, path=m.adjoint_path)
         return grad, None
 
 
 class KinematicModel:
     def __init__(self, positions, codes, labels, beam, detector, rot, trans,
-                 apply_polarization=True, remove_forward=False, device=None):
+                 apply_polarization=True, remove_forward=False, device=None,
+                 adjoint_path="auto"):
         self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.adjoint_path = adjoint_path
         self.pos0 = torch.as_tensor(np.ascontiguousarray(positions, dtype=np.float32),
                                     device=self.device)
         self.codes = torch.as_tensor(np.ascontiguousarray(codes, dtype=np.uint8),

@@ -225,12 +225,13 @@ def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
     return out
 
 
-def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, stream=None):
+def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream=None):
     """Add dL/du (per Angstrom, original atom order) into grad_pos (N, 3) float32.
 
     ``grad_field`` is dL/dRe(A) + i dL/dIm(A) for the field the last
     ``kin_sum`` on ``staged`` produced, (F, Nz, Ny) complex (cast to
-    complex64).  Raises MosaicDeclined outside the recurrence's bounds.
+    complex64).  ``path`` is "auto", "rows" or "mma"; raises MosaicDeclined
+    outside the chosen form's bounds.
     """
     F = staged.n_frames
     if staged.pixel_geom is None:
@@ -249,6 +250,6 @@ def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, stream=None):
         staged.n_atoms, int(c.shape[0]), cp_, float(k), F, rot.ctypes.data_as(ctypes.c_void_p),
         trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, staged.stage_stats,
         staged.pixel_geom, _ptr(torch.view_as_real(g)), _ptr(grad_pos),
-        _ptr(staged.workspace), int(staged.workspace.numel()), _stream_handle(stream)),
-        "mosaic_kin_adjoint")
+        _ptr(staged.workspace), int(staged.workspace.numel()), PATHS[path],
+        _stream_handle(stream)), "mosaic_kin_adjoint")
     return grad_pos

@@ -344,15 +344,20 @@ class SurrogateEnsemble:
                     f"{np.array2string(ep_loss, precision=4)}")
         return np.array(hist)
 
-    def predict(self, P):
-        """Ensemble mean intensity (B, npix) and per-frame spread (B,)."""
-        X = torch.as_tensor(normalise(P), device=self.device)
-        out = self.forward(X)
-        B = out.shape[1]
-        mean = torch.empty((B, out.shape[2]), dtype=torch.float32, device=self.device)
-        spread = torch.empty(B, dtype=torch.float32, device=self.device)
-        S.ensemble_stats(out.contiguous(), mean, spread)
-        return self.intensity(mean), spread
+    def predict(self, P, chunk=256):
+        """Ensemble mean intensity (B, npix) and per-frame spread (B,), in
+        chunks of ``chunk`` frames (the decoder GEMMs have B x H x W rows)."""
+        Xall = torch.as_tensor(normalise(P), device=self.device)
+        means, spreads = [], []
+        for i in range(0, Xall.shape[0], chunk):
+            out = self.forward(Xall[i:i + chunk].contiguous())
+            B = out.shape[1]
+            mean = torch.empty((B, out.shape[2]), dtype=torch.float32, device=self.device)
+            spread = torch.empty(B, dtype=torch.float32, device=self.device)
+            S.ensemble_stats(out.contiguous(), mean, spread)
+            means.append(self.intensity(mean))
+            spreads.append(spread)
+        return torch.cat(means), torch.cat(spreads)
 
 
 def rel_l2(a, b):

@@ -19,7 +19,7 @@ from oracle import adjoint as AO
 pytestmark = pytest.mark.gpu
 
 
-def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0):
+def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0, adjoint_path="auto"):
     w = W.fcc_fault_scan111(cells=cells, npix=npix, n_angles=n_angles, half_range_deg=0.05)
     b, d, st = w["beam"], w["detector"], w["stage"]
     pos = w["sample"].load_chunk_positions(1)
@@ -33,12 +33,14 @@ def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0):
     u = W.gaussian_bump_field(pos, amplitude=0.2, width=6.0, rng=rng)
     u_t = W.gaussian_bump_field(pos, amplitude=0.2, width=6.0, rng=np.random.default_rng(seed + 1))
     codes = np.zeros(pos.shape[0], np.uint8)
-    m = KinematicModel(pos, codes, ["Au"], b, d, np.stack(rots), np.stack(trs))
+    m = KinematicModel(pos, codes, ["Au"], b, d, np.stack(rots), np.stack(trs),
+                       adjoint_path=adjoint_path)
     return w, m, pos, u.astype(np.float32), u_t.astype(np.float32), np.stack(rots), np.stack(trs)
 
 
-def test_adjoint_vs_fp64_reference_and_fd(device):
-    w, m, pos, u, u_t, rots, trs = make_problem()
+@pytest.mark.parametrize("path", ["rows", "mma"])
+def test_adjoint_vs_fp64_reference_and_fd(device, path):
+    w, m, pos, u, u_t, rots, trs = make_problem(adjoint_path=path)
     ut = torch.from_numpy(u_t).to(device)
     with torch.no_grad():
         It = (m.field(ut).abs() ** 2).detach()
@@ -81,7 +83,7 @@ def test_adjoint_vs_fp64_reference_and_fd(device):
     ref, fd, gpu = np.array(ref), np.array(fd), np.array(gpu)
     e_ref = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
     e_fd = np.linalg.norm(ref - fd) / np.linalg.norm(ref)
-    print(f"[parity] adjoint vs long-double reference: rel={e_ref:.3e}; "
+    print(f"[parity] adjoint ({path}) vs long-double reference: rel={e_ref:.3e}; "
           f"reference vs FD: rel={e_fd:.3e}")
     assert e_fd < 1e-4          # the analytic formula is the derivative
     assert e_ref < 1e-3         # the GPU adjoint matches it
@@ -89,9 +91,10 @@ def test_adjoint_vs_fp64_reference_and_fd(device):
     assert np.all(np.isfinite(grad))
 
 
-def test_adjoint_gradient_descends(device):
+@pytest.mark.parametrize("path", ["rows", "mma"])
+def test_adjoint_gradient_descends(device, path):
     """A few gradient steps on u reduce the loss towards the target field."""
-    w, m, pos, u, u_t, rots, trs = make_problem(cells=(6, 6, 6), npix=24)
+    w, m, pos, u, u_t, rots, trs = make_problem(cells=(6, 6, 6), npix=24, adjoint_path=path)
     ut = torch.from_numpy(u_t).to(device)
     with torch.no_grad():
         It = (m.field(ut).abs() ** 2).detach()
@@ -108,5 +111,5 @@ def test_adjoint_gradient_descends(device):
             gn = torch.linalg.norm(uu.grad)
             assert torch.isfinite(gn) and gn > 0
             uu -= 0.02 * uu.grad / gn
-    print("[adjoint] losses", losses)
+    print(f"[adjoint] {path} losses", losses)
     assert losses[-1] < losses[0]
