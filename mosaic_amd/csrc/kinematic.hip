@@ -48,7 +48,7 @@ constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
-constexpr int N_GEOM = 4;            // pixel geometry statistics (mosaic_kin_pixels)
+constexpr int N_GEOM = 5;            // pixel geometry statistics (mosaic_kin_pixels)
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
 
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const int npix = Ny * Nz;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const double k = O.k;
-    double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0;
+    double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0, du2z = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
         double f, r, r_lo;
@@ -436,6 +436,15 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
                 du2 = sqrt(ax * ax + ay * ay + az * az);
             }
         }
+        // and along the column
+        if (iy + 1 < Nz && iy > 0) {
+            double vx, vy, vz, wx, wy, wz;
+            pixel_dir(cx, cy, cz, p + Ny, vx, vy, vz);
+            pixel_dir(cx, cy, cz, p - Ny, wx, wy, wz);
+            const double ax = vx - 2.0 * ux + wx, ay = vy - 2.0 * uy + wy,
+                         az = vz - 2.0 * uz + wz;
+            du2z = sqrt(ax * ax + ay * ay + az * az);
+        }
 
         // deviation from the patch model
         double e1[3], e2[3], e3[3], e4[3];
@@ -459,17 +468,19 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     du1 = block_max(du1, lds);
     du2 = block_max(du2, lds);
     dpatch = block_max(dpatch, lds);
+    du2z = block_max(du2z, lds);
     if (threadIdx.x == 0) {
         T.stat_blocks[N_GEOM * blockIdx.x] = rmin;
         T.stat_blocks[N_GEOM * blockIdx.x + 1] = du1;
         T.stat_blocks[N_GEOM * blockIdx.x + 2] = du2;
         T.stat_blocks[N_GEOM * blockIdx.x + 3] = dpatch;
+        T.stat_blocks[N_GEOM * blockIdx.x + 4] = du2z;
     }
 }
 
 __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double lds[16];
-    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0};
+    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         v[0] = fmin(v[0], in[N_GEOM * i]);
         for (int q = 1; q < N_GEOM; ++q) v[q] = fmax(v[q], in[N_GEOM * i + q]);
@@ -1592,12 +1603,14 @@ __global__ __launch_bounds__(SUM_THREADS) void k_adjoint(const AdjArgs A) {
 //         = vc S0 + gy S1 + hy S2 + gz U1 + hz U2,
 //     S_n = sum_a a^n A_j(a) Y0_j(a),  U_n = sum_a A_j(a) Y_n_j(a),
 //     Y_n_j(a) = sum_b W(a, b) b^n Bz_j(b)   (a, b offsets from the middle),
-// so three complex products per block of 32 atoms, W (32 x 32, the same for
+// so NV complex products per block of 32 atoms, W (32 x 32, the same for
 // every atom) times [Bz, b Bz, b^2 Bz] (32 x 32 atoms), on the FP16-split
 // matrix cores as in the forward, followed by a contraction over a in the
 // lanes that hold each atom's column.  One workgroup owns one tile; its four
 // waves split the (frame, patch) pairs and accumulate Z per atom in FP64 in
-// LDS; the tile's atoms are written once at the end.
+// LDS; the tile's atoms are written once at the end.  NV = 2 leaves out
+// hz U2 (the host takes it where |hz| b^2 <= 1e-6 over the patch, a column
+// curvature that fine-pitch detectors keep far below that); NV = 3 keeps it.
 struct AdjMmaArgs {
     const AtomRec* atoms;
     const TileRec* tiles;
@@ -1618,8 +1631,9 @@ struct AdjMmaArgs {
     double K;
 };
 
-template <bool SECOND>
-__global__ __launch_bounds__(64 * MMA_WAVES) void k_adjoint_mma(const AdjMmaArgs A) {
+template <bool SECOND, int NV>
+__global__ __launch_bounds__(64 * MMA_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_adjoint_mma(const AdjMmaArgs A) {
     __shared__ AtomRec tile[TILE];
     __shared__ double zacc[MMA_WAVES][TILE][6];
     __shared__ double red[MMA_WAVES];
@@ -1772,7 +1786,7 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_adjoint_mma(const AdjMmaArgs
             float s0r = 0.f, s0i = 0.f, s1r = 0.f, s1i = 0.f, s2r = 0.f, s2i = 0.f;
             float u1r = 0.f, u1i = 0.f, u2r = 0.f, u2i = 0.f;
 #pragma unroll 1
-            for (int n = 0; n < 3; ++n) {
+            for (int n = 0; n < NV; ++n) {
                 floatx16 crr, cii, cri, cir;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) { crr[r] = 0.f; cii[r] = 0.f; cri[r] = 0.f; cir[r] = 0.f; }
@@ -1827,13 +1841,19 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_adjoint_mma(const AdjMmaArgs
             s1r += __shfl_xor(s1r, 32, 64); s1i += __shfl_xor(s1i, 32, 64);
             s2r += __shfl_xor(s2r, 32, 64); s2i += __shfl_xor(s2i, 32, 64);
             u1r += __shfl_xor(u1r, 32, 64); u1i += __shfl_xor(u1i, 32, 64);
-            u2r += __shfl_xor(u2r, 32, 64); u2i += __shfl_xor(u2i, 32, 64);
+            if (NV > 2) {
+                u2r += __shfl_xor(u2r, 32, 64); u2i += __shfl_xor(u2i, 32, 64);
+            }
             const int j = blk * 32 + q;
             if (h == 0 && j < cnt) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    const float zr = vc[d] * s0r + gy[d] * s1r + hy[d] * s2r + gz[d] * u1r + hz[d] * u2r;
-                    const float zi = vc[d] * s0i + gy[d] * s1i + hy[d] * s2i + gz[d] * u1i + hz[d] * u2i;
+                    float zr = vc[d] * s0r + gy[d] * s1r + hy[d] * s2r + gz[d] * u1r;
+                    float zi = vc[d] * s0i + gy[d] * s1i + hy[d] * s2i + gz[d] * u1i;
+                    if (NV > 2) {
+                        zr = fmaf(hz[d], u2r, zr);
+                        zi = fmaf(hz[d], u2i, zi);
+                    }
                     zacc[wave][j][2 * d] += static_cast<double>(zr);
                     zacc[wave][j][2 * d + 1] += static_cast<double>(zi);
                 }
@@ -2398,12 +2418,17 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
         M.K = Kk;
         hipLaunchKernelGGL(k_amax, dim3(AMAX_BLOCKS), dim3(256), 0, st, grad_field, n_gin0,
                            reinterpret_cast<const float*>(L.px.gmma), n_gtab0, L.amax_blocks);
-        if (use_second)
-            hipLaunchKernelGGL((k_adjoint_mma<true>), dim3(L.n_tiles), dim3(64 * MMA_WAVES), 0,
-                               st, M);
-        else
-            hipLaunchKernelGGL((k_adjoint_mma<false>), dim3(L.n_tiles), dim3(64 * MMA_WAVES), 0,
-                               st, M);
+        // the column curvature of the gradient direction, |hz| b^2 with
+        // hz ~ du2z / 2 and |b| <= MMA_P / 2, may be left out below 1e-6
+        const bool flat = 0.125 * MMA_P * MMA_P * pixel_geom[4] <= 1e-6;
+        const dim3 grid(L.n_tiles), blk(64 * MMA_WAVES);
+        if (use_second) {
+            if (flat) hipLaunchKernelGGL((k_adjoint_mma<true, 2>), grid, blk, 0, st, M);
+            else hipLaunchKernelGGL((k_adjoint_mma<true, 3>), grid, blk, 0, st, M);
+        } else {
+            if (flat) hipLaunchKernelGGL((k_adjoint_mma<false, 2>), grid, blk, 0, st, M);
+            else hipLaunchKernelGGL((k_adjoint_mma<false, 3>), grid, blk, 0, st, M);
+        }
         MOSAIC_HIP_CHECK(hipGetLastError());
         return MOSAIC_OK;
     }

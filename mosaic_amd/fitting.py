@@ -69,6 +69,8 @@ class KinematicModel:
                          self.Ny, self.Nz)
         self._out.zero_()
         K.kin_sum(staged, self.spc, self.opts, self.rot, self.trans, self.coords, self._out)
+        self.path_used = staged.path_used
+        self.pixel_geom = tuple(staged.pixel_geom)
         return staged
 
     def field(self, u):

@@ -19,8 +19,11 @@ from oracle import adjoint as AO
 pytestmark = pytest.mark.gpu
 
 
-def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0, adjoint_path="auto"):
+def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0, adjoint_path="auto",
+                 distance_a=W.DISTANCE_A):
     w = W.fcc_fault_scan111(cells=cells, npix=npix, n_angles=n_angles, half_range_deg=0.05)
+    if distance_a != W.DISTANCE_A:
+        w["detector"] = W._detector(npix, 2 * w["theta_b"], distance_a=distance_a)
     b, d, st = w["beam"], w["detector"], w["stage"]
     pos = w["sample"].load_chunk_positions(1)
     rots, trs = [], []
@@ -38,12 +41,24 @@ def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0, adjoint_path="aut
     return w, m, pos, u.astype(np.float32), u_t.astype(np.float32), np.stack(rots), np.stack(trs)
 
 
-@pytest.mark.parametrize("path", ["rows", "mma"])
+# "mma-curved": a detector at 0.375 m, where the column curvature of the
+# pixel direction over a 32-pixel patch (|hz| b^2 ~ 4e-6) is kept in the
+# matrix-core adjoint's gradient direction
+@pytest.mark.parametrize("path", ["rows", "mma", "mma-curved"])
 def test_adjoint_vs_fp64_reference_and_fd(device, path):
-    w, m, pos, u, u_t, rots, trs = make_problem(adjoint_path=path)
+    curved = path == "mma-curved"
+    w, m, pos, u, u_t, rots, trs = make_problem(
+        adjoint_path="mma" if curved else path, distance_a=3.75e9 if curved else W.DISTANCE_A)
     ut = torch.from_numpy(u_t).to(device)
     with torch.no_grad():
         It = (m.field(ut).abs() ** 2).detach()
+    # the matrix-core forward takes this geometry too; the flat variant of the
+    # adjoint applies below 1e-6 of column curvature over a patch
+    curv = 0.125 * 32 * 32 * m.pixel_geom[4]
+    print(f"[adjoint] {path}: forward path {m.path_used}, patch column curvature {curv:.2e}")
+    if path != "rows":
+        assert m.path_used % 100 == 32
+        assert (curv > 1e-6) == curved
     s = float(It.max())
     uu = torch.from_numpy(u).to(device).requires_grad_(True)
     A = m.field(uu)
