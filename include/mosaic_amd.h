@@ -216,12 +216,18 @@ int mosaic_asp_workspace(int Ny2, int Nz2, size_t* bytes);
  * (Beam.py:7829-8022) with prop_mul_kernel / prop_mul_cpu
  * (Beam.py:4739-4900); the pad size and sub-stepping rules stay with the
  * caller (_choose_optimal_pad / _alias_free_step, Beam.py:4632-4736).
+ * `method`: 0 automatic (the pruned transform where Ny2 and Nz2 are powers
+ * of two and Ny, Nz <= 8192, else the padded hipFFT round trip), 1 padded
+ * hipFFT round trip, 2 pruned transform (MOSAIC_ERR_UNSUPPORTED where its
+ * size rules fail).  The pruned transform never forms the padded grid: it
+ * transforms the data rows, then each frequency column (transfer function
+ * and inverse onto the crop rows fused), then the crop rows.
  * field_in and field_out may alias.
  */
 int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, double dz,
                          double wavelength, double z, int Ny2, int Nz2, int pad_edge,
                          float pad_re, float pad_im, const mosaic_lens_t* lens /* host */,
-                         float* field_out, void* workspace, size_t workspace_bytes,
+                         int method, float* field_out, void* workspace, size_t workspace_bytes,
                          mosaic_stream_t stream);
 
 /* Thin lens in place on the (Nz, Ny) device field, pitches dx (columns), dy
@@ -274,17 +280,20 @@ int mosaic_sg_bias_grad(const float* dZ, float* dbias, long long rows, int cols,
 int mosaic_sg_shuffle2(float* G, float* Y, int nimg, int H, int W, int Co, long long stride,
                        int batch, int backward, mosaic_stream_t stream);
 
-/* loss[b] += mean((P_b - T)^2); dP_b = 2 (P_b - T) / n. */
+/* loss[b] += mean(w (P_b - T)^2); dP_b = 2 w (P_b - T) / n, with w = 1, or
+ * w = row_weight[i / row_len] for element i when row_weight is given (device,
+ * n / row_len entries; row_len must divide n). */
 int mosaic_sg_mse(const float* P, const float* T, float* dP, float* loss, long long n,
-                  long long strideP, int batch, mosaic_stream_t stream);
+                  long long strideP, int batch, const float* row_weight, long long row_len,
+                  mosaic_stream_t stream);
 
 /* Adam step with decoupled weight decay (bias-corrected, 1-based step). */
 int mosaic_sg_adam(float* w, const float* g, float* m, float* v, long long n, float lr,
                    float beta1, float beta2, float eps, int step, float weight_decay,
                    mosaic_stream_t stream);
 
-/* Per frame: ensemble mean (frames, npix) and mean-over-pixels of the
- * members' standard deviation (frames). */
+/* Per frame: ensemble mean (frames, npix) and the members' spread, the
+ * root mean over pixels of their variance (frames). */
 int mosaic_sg_ensemble_stats(const float* P, float* mean, float* spread, int members,
                              int frames, int npix, long long strideP, mosaic_stream_t stream);
 

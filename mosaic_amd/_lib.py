@@ -88,7 +88,7 @@ _SIGS = {
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
-                             ctypes.c_float, _vp, _vp, _vp, _sz, _vp],
+                             ctypes.c_float, _vp, _i, _vp, _vp, _sz, _vp],
     "mosaic_lens_apply": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
     "mosaic_defect_positions": [_i64, _vp, _vp, _vp, _vp],
     "mosaic_sg_gemm": [_i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _i, _f, _f,
@@ -97,7 +97,7 @@ _SIGS = {
     "mosaic_sg_act_bwd": [_vp, _vp, _vp, _ll, _i, _vp],
     "mosaic_sg_bias_grad": [_vp, _vp, _ll, _i, _i, _ll, _ll, _i, _vp],
     "mosaic_sg_shuffle2": [_vp, _vp, _i, _i, _i, _i, _ll, _i, _i, _vp],
-    "mosaic_sg_mse": [_vp, _vp, _vp, _vp, _ll, _ll, _i, _vp],
+    "mosaic_sg_mse": [_vp, _vp, _vp, _vp, _ll, _ll, _i, _vp, _ll, _vp],
     "mosaic_sg_adam": [_vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _f, _vp],
     "mosaic_sg_ensemble_stats": [_vp, _vp, _vp, _i, _i, _i, _ll, _vp],
 }

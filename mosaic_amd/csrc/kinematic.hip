@@ -48,6 +48,7 @@ constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
+constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
 constexpr int N_GEOM = 5;            // pixel geometry statistics (mosaic_kin_pixels)
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
@@ -228,7 +229,11 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.frames = c.take<FrameRec>(std::max(1, n_frames));
     const int ppt = pix_per_thread(npix);
     L.pix_blocks = blocks_for(npix, SUM_THREADS * ppt);
-    const int64_t grid_px = static_cast<int64_t>(L.pix_blocks) * std::max(1, n_frames);
+    // the smaller of the pixel grids of the row kernels and of the
+    // matrix-core sum (one wave per 32 x 32 patch, MMA_WAVES per workgroup)
+    const int mma_blocks = std::max(1, npix / (MMA_P * MMA_P * MMA_WAVES));
+    const int64_t grid_px =
+        static_cast<int64_t>(std::min(L.pix_blocks, mma_blocks)) * std::max(1, n_frames);
     int slices = static_cast<int>((TARGET_BLOCKS + grid_px - 1) / grid_px);
     slices = std::max(1, std::min(slices, std::max(1, L.n_tiles)));
     L.tiles_per_slice = (std::max(1, L.n_tiles) + slices - 1) / slices;
@@ -1129,7 +1134,6 @@ struct MmaArgs {
     double K;         // 1 / lambda
 };
 
-constexpr int MMA_WAVES = 4;
 constexpr int MMA_FLUSH = 32;
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));

@@ -10,8 +10,10 @@
 // precedes a free-space step, so the lensed field is never written at its
 // own size.
 //
-// The FFTs are hipFFT plans (cached per size and device, work area supplied by
-// the caller).  The pad, transfer-function and crop passes are hand-written:
+// Two implementations: the pruned three-pass transform below (the default
+// where its size rules hold: power-of-two padded sizes, data lengths up to
+// 8192), and the padded 2-D hipFFT round trip (plans cached per size and
+// device, work area supplied by the caller) with hand-written passes:
 //   k_pad_lens -- reads the small field once, writes the padded grid;
 //   k_tf       -- one read-modify-write of the spectrum; the transfer
 //                 function phase is formed in FP64 and reduced mod 2 pi
@@ -132,6 +134,459 @@ __global__ void k_crop(const float2* __restrict__ F, int Ny2, int y0, int z0, in
     out[static_cast<int64_t>(iz) * Ny + iy] = make_float2(v.x * scale, v.y * scale);
 }
 
+// ---------------------------------------------------------------------------
+// Pruned transforms.  Along one axis the padded line of length M (a power of
+// two) holds the data x_n (n < N) at offset o = (M - N) / 2 and, outside it,
+// the edge values a (left) and b (right) -- x_0 and x_{N-1} in edge mode, the
+// pad constant otherwise.  With L = 2^ceil(log2 N) (at least 64), R = M / L
+// and w = e^{-2 pi i / M}, the forward transform splits as
+//     X_{Rm+r} = w^{o (Rm+r)} FFT_L(x_n w^{n r})[m] + E(Rm+r),
+//     E(k) = [(a - b) - a w^{o k} + b w^{(o+N) k}] / (1 - w^k)   (k != 0),
+//     E(0) = a o + b (M - o - N)
+// (the two constant runs are geometric series), and the inverse transform,
+// needed only on the crop window n < N, splits as
+//     y_n = sum_r w^{-(o+n) r} IFFT_L(X_{R.+r})[(o + n) mod L].
+// So a step never forms the padded grid: one pass transforms the N_z data
+// rows along y, one pass per y-frequency column transforms along z, applies
+// the transfer function and inverts back to the N_z crop rows, and one pass
+// inverts the crop rows along y.  HBM traffic per step is 16 N_y N_z +
+// 32 M_y N_z bytes (the intermediate is M_y x N_z, column-major), against
+// 64 M_y M_z for a padded 2-D FFT round trip.
+//
+// Each line's length-L transforms run in LDS: Stockham autosort, radix 8
+// (then one radix-4 or radix-2 stage), natural order in and out, twiddles
+// from an LDS table e^{-2 pi i t / L}; one float2 of padding per eight
+// spreads the stride-8 stores over the banks.
+
+constexpr int FFT_MIN = 64;
+constexpr int FFT_MAX = 8192;
+
+__host__ __device__ constexpr int fft_threads(int L) { return L / 8 < 64 ? 64 : L / 8; }
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+__device__ __forceinline__ int lp(int e) { return e + (e >> 3); }
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// w^e = e^{-2 pi i e / M} for a power-of-two M, the exponent reduced exactly
+__device__ __forceinline__ float2 wpow(long long e, int M) {
+    const int q = static_cast<int>(e & (M - 1));
+    float s, c;
+    sincospif(-2.0f * static_cast<float>(q) / static_cast<float>(M), &s, &c);
+    return make_float2(c, s);
+}
+
+template <bool INV>
+__device__ __forceinline__ void fft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    const float2 c0 = cadd(v0, v2), c1 = csub(v0, v2), c2 = cadd(v1, v3),
+                 c3 = mul_mi<INV>(csub(v1, v3));
+    v0 = cadd(c0, c2);
+    v2 = csub(c0, c2);
+    v1 = cadd(c1, c3);
+    v3 = csub(c1, c3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void fft8(float2* v) {
+    constexpr float H = 0.70710678118654752f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        a[n] = cadd(v[n], v[n + 4]);
+        b[n] = csub(v[n], v[n + 4]);
+    }
+    // b_n w8^n
+    b[1] = INV ? make_float2(H * (b[1].x - b[1].y), H * (b[1].x + b[1].y))
+               : make_float2(H * (b[1].x + b[1].y), H * (b[1].y - b[1].x));
+    b[2] = mul_mi<INV>(b[2]);
+    b[3] = INV ? make_float2(-H * (b[3].x + b[3].y), H * (b[3].x - b[3].y))
+               : make_float2(H * (b[3].y - b[3].x), -H * (b[3].x + b[3].y));
+    fft4<INV>(a[0], a[1], a[2], a[3]);
+    fft4<INV>(b[0], b[1], b[2], b[3]);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        v[2 * m] = a[m];
+        v[2 * m + 1] = b[m];
+    }
+}
+
+// One Stockham stage of radix RD on butterfly j (j < L / RD), stride Ns.
+template <int L, int RD, bool INV>
+__device__ __forceinline__ void stage_load(const float2* buf, const float2* tw, int Ns, int j,
+                                           float2* v) {
+#pragma unroll
+    for (int r = 0; r < RD; ++r) v[r] = buf[lp(j + r * (L / RD))];
+    const int k = j & (Ns - 1);
+    const int step = k * (L / (Ns * RD));
+#pragma unroll
+    for (int r = 1; r < RD; ++r) {
+        const float2 w = tw[r * step];
+        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+    }
+    if (RD == 8) {
+        fft8<INV>(v);
+    } else if (RD == 4) {
+        fft4<INV>(v[0], v[1], v[2], v[3]);
+    } else {
+        const float2 t = csub(v[0], v[1]);
+        v[0] = cadd(v[0], v[1]);
+        v[1] = t;
+    }
+}
+
+template <int L, int RD>
+__device__ __forceinline__ void stage_store(float2* buf, int Ns, int j, const float2* v) {
+    const int k = j & (Ns - 1);
+    const int d = (j / Ns) * Ns * RD + k;
+#pragma unroll
+    for (int r = 0; r < RD; ++r) buf[lp(d + r * Ns)] = v[r];
+}
+
+// In-place length-L transform of buf (natural order), all threads of the
+// block taking part; begins and ends with the block synchronised.
+template <int L, bool INV>
+__device__ void fft_lds(float2* buf, const float2* tw) {
+    constexpr int T = fft_threads(L);
+    constexpr int LG = ilog2c(L);
+    const int t = threadIdx.x;
+    int Ns = 1;
+#pragma unroll
+    for (int s = 0; s < LG / 3; ++s) {
+        float2 v[8];
+        const bool act = t < L / 8;
+        if (act) stage_load<L, 8, INV>(buf, tw, Ns, t, v);
+        __syncthreads();
+        if (act) stage_store<L, 8>(buf, Ns, t, v);
+        __syncthreads();
+        Ns *= 8;
+    }
+    if (LG % 3 == 2) {
+        constexpr int NB = L / 4, PER = (NB + T - 1) / T;
+        float2 v[PER][4];
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t + q * T < NB) stage_load<L, 4, INV>(buf, tw, Ns, t + q * T, v[q]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t + q * T < NB) stage_store<L, 4>(buf, Ns, t + q * T, v[q]);
+        __syncthreads();
+    } else if (LG % 3 == 1) {
+        constexpr int NB = L / 2, PER = (NB + T - 1) / T;
+        float2 v[PER][2];
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t + q * T < NB) stage_load<L, 2, INV>(buf, tw, Ns, t + q * T, v[q]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t + q * T < NB) stage_store<L, 2>(buf, Ns, t + q * T, v[q]);
+        __syncthreads();
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void fill_twiddles(float2* tw) {
+    for (int i = threadIdx.x; i < L; i += fft_threads(L)) tw[i] = wpow(i, L);
+}
+
+// Contribution of the two constant runs to bin k (see the header above).
+__device__ __forceinline__ float2 edge_term(float2 a, float2 b, int k, int o, int N, int M) {
+    if (k == 0)
+        return make_float2(a.x * o + b.x * (M - o - N), a.y * o + b.y * (M - o - N));
+    const float2 wo = wpow(static_cast<long long>(o) * k, M);
+    const float2 won = wpow(static_cast<long long>(o + N) * k, M);
+    const float2 w1 = wpow(k, M);
+    const float2 num = cadd(csub(csub(a, b), cmul(a, wo)), cmul(b, won));
+    const float2 den = make_float2(1.f - w1.x, -w1.y);
+    const float inv = 1.f / (den.x * den.x + den.y * den.y);
+    const float2 q = cmulc(num, den);
+    return make_float2(q.x * inv, q.y * inv);
+}
+
+// Row r of a pass that streams rows through B[k * Nz + row]: groups of 16
+// adjacent rows run on one XCD (workgroups go round-robin over the 8 XCDs),
+// so the 8-byte column accesses of a group share 128-byte lines in its L2.
+__device__ __forceinline__ int xcd_row(int b, int n_rows) {
+    if (n_rows & 127) return b;
+    const int xcd = b & 7, slot = b >> 3;
+    return ((slot >> 4) << 7) | (xcd << 4) | (slot & 15);
+}
+
+__device__ __forceinline__ float2 lens_phase(float2 v, int iy, int iz, int Ny, int Nz, double dx,
+                                             double dy, const LensArgs& lens) {
+    const double x = (iy - 0.5 * (Ny - 1)) * dx;
+    const double y = (iz - 0.5 * (Nz - 1)) * dy;
+    const double r2 = x * x + y * y;
+    double ph = -lens.half_k_over_f * r2;
+    ph -= TWO_PI * rint(ph / TWO_PI);
+    float s, c;
+    sincosf(static_cast<float>(ph), &s, &c);
+    float a = lens.uniform_amp;
+    if (lens.parab_coef > 0.0) a *= static_cast<float>(exp(-lens.parab_coef * r2));
+    return make_float2(a * (v.x * c - v.y * s), a * (v.x * s + v.y * c));
+}
+
+// Pass 1: forward transform along y of each data row (lens applied first),
+// written column-major: B[ky * Nz + row].
+template <int L>
+__global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_fwd(
+    const float2* __restrict__ in, int Ny, int Nz, int My, int edge, float2 padc, double dx,
+    double dy, LensArgs lens, float2* __restrict__ B) {
+    constexpr int T = fft_threads(L), E = L / T;
+    __shared__ float2 buf[L + L / 8];
+    __shared__ float2 tw[L];
+    fill_twiddles<L>(tw);
+    const int row = xcd_row(blockIdx.x, Nz);
+    const int t = threadIdx.x;
+    const int oy = (My - Ny) / 2;
+    const float2* src = in + static_cast<int64_t>(row) * Ny;
+    float2 x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + e * T;
+        x[e] = make_float2(0.f, 0.f);
+        if (n < Ny) {
+            x[e] = src[n];
+            if (lens.active) x[e] = lens_phase(x[e], n, row, Ny, Nz, dx, dy, lens);
+        }
+    }
+    float2 a = padc, b = padc;
+    if (edge) {
+        a = src[0];
+        b = src[Ny - 1];
+        if (lens.active) {
+            a = lens_phase(a, 0, row, Ny, Nz, dx, dy, lens);
+            b = lens_phase(b, Ny - 1, row, Ny, Nz, dx, dy, lens);
+        }
+    }
+    const int R = My / L;
+    for (int r = 0; r < R; ++r) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + e * T;
+            buf[lp(n)] = n < Ny ? cmul(x[e], wpow(static_cast<long long>(n) * r, My))
+                                : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft_lds<L, false>(buf, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            const int ky = R * m + r;
+            const float2 X = cadd(cmul(buf[lp(m)], wpow(static_cast<long long>(oy) * ky, My)),
+                                  edge_term(a, b, ky, oy, Ny, My));
+            B[static_cast<int64_t>(ky) * Nz + row] = X;
+        }
+    }
+}
+
+// Pass 2: per y-frequency column, forward along z, transfer function,
+// inverse along z onto the Nz crop rows; in place in B.
+template <int L>
+__global__ __launch_bounds__(fft_threads(L)) void k_asp_cols(float2* __restrict__ B, int Nz,
+                                                            int Mz, int My, int edge,
+                                                            float2 padc, double dy, double dz,
+                                                            double kw, double z) {
+    constexpr int T = fft_threads(L), E = L / T;
+    __shared__ float2 buf[L + L / 8];
+    __shared__ float2 tw[L];
+    fill_twiddles<L>(tw);
+    const int ky = blockIdx.x;
+    const int t = threadIdx.x;
+    const int oz = (Mz - Nz) / 2;
+    float2* col = B + static_cast<int64_t>(ky) * Nz;
+    float2 x[E], y[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + e * T;
+        x[e] = n < Nz ? col[n] : make_float2(0.f, 0.f);
+        y[e] = make_float2(0.f, 0.f);
+    }
+    float2 a, b;
+    if (edge) {
+        a = col[0];
+        b = col[Nz - 1];
+    } else {
+        // a constant row transforms to M_y times the constant at ky = 0
+        a = ky == 0 ? make_float2(padc.x * My, padc.y * My) : make_float2(0.f, 0.f);
+        b = a;
+    }
+    const double kyf = kfreq(ky, My, dy);
+    const int R = Mz / L;
+    for (int r = 0; r < R; ++r) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + e * T;
+            buf[lp(n)] = n < Nz ? cmul(x[e], wpow(static_cast<long long>(n) * r, Mz))
+                                : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft_lds<L, false>(buf, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            const int kz = R * m + r;
+            float2 Z = cadd(cmul(buf[lp(m)], wpow(static_cast<long long>(oz) * kz, Mz)),
+                            edge_term(a, b, kz, oz, Nz, Mz));
+            const double kzf = kfreq(kz, Mz, dz);
+            const double kt2 = kyf * kyf + kzf * kzf;
+            const double kx2 = kw * kw - kt2;
+            double ph, amp;
+            if (kx2 >= 0.0) {
+                ph = -z * kt2 / (sqrt(kx2) + kw);
+                amp = 1.0;
+            } else {
+                ph = -z * kw;
+                amp = exp(-fabs(z) * sqrt(-kx2));
+            }
+            ph -= TWO_PI * rint(ph / TWO_PI);
+            float s, c;
+            sincosf(static_cast<float>(ph), &s, &c);
+            const float am = static_cast<float>(amp);
+            Z = cmul(Z, make_float2(am * c, am * s));
+            buf[lp(m)] = Z;
+        }
+        __syncthreads();
+        fft_lds<L, true>(buf, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + e * T;
+            if (n < Nz)
+                y[e] = cadd(y[e], cmulc(buf[lp((oz + n) & (L - 1))],
+                                        wpow(static_cast<long long>(oz + n) * r, Mz)));
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + e * T;
+        if (n < Nz) col[n] = y[e];
+    }
+}
+
+// Pass 3: inverse along y of each crop row, scaled, into the output field.
+template <int L>
+__global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_inv(const float2* __restrict__ B,
+                                                                int Ny, int Nz, int My,
+                                                                float scale,
+                                                                float2* __restrict__ out) {
+    constexpr int T = fft_threads(L), E = L / T;
+    __shared__ float2 buf[L + L / 8];
+    __shared__ float2 tw[L];
+    fill_twiddles<L>(tw);
+    const int row = xcd_row(blockIdx.x, Nz);
+    const int t = threadIdx.x;
+    const int oy = (My - Ny) / 2;
+    float2 acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
+    const int R = My / L;
+    for (int r = 0; r < R; ++r) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int m = t + e * T;
+            buf[lp(m)] = B[static_cast<int64_t>(R * m + r) * Nz + row];
+        }
+        __syncthreads();
+        fft_lds<L, true>(buf, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = t + e * T;
+            if (i < Ny)
+                acc[e] = cadd(acc[e], cmulc(buf[lp((oy + i) & (L - 1))],
+                                            wpow(static_cast<long long>(oy + i) * r, My)));
+        }
+    }
+    float2* dst = out + static_cast<int64_t>(row) * Ny;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = t + e * T;
+        if (i < Ny) dst[i] = make_float2(acc[e].x * scale, acc[e].y * scale);
+    }
+}
+
+int fft_len(int n) {
+    int L = FFT_MIN;
+    while (L < n) L *= 2;
+    return L;
+}
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <int L>
+void launch_rows_fwd(hipStream_t st, const float2* in, int Ny, int Nz, int My, int edge,
+                     float2 padc, double dx, double dy, LensArgs lens, float2* B) {
+    hipLaunchKernelGGL(k_asp_rows_fwd<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, in, Ny, Nz, My,
+                       edge, padc, dx, dy, lens, B);
+}
+template <int L>
+void launch_cols(hipStream_t st, float2* B, int Nz, int Mz, int My, int edge, float2 padc,
+                 double dy, double dz, double kw, double z) {
+    hipLaunchKernelGGL(k_asp_cols<L>, dim3(My), dim3(fft_threads(L)), 0, st, B, Nz, Mz, My, edge,
+                       padc, dy, dz, kw, z);
+}
+template <int L>
+void launch_rows_inv(hipStream_t st, const float2* B, int Ny, int Nz, int My, float scale,
+                     float2* out) {
+    hipLaunchKernelGGL(k_asp_rows_inv<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, B, Ny, Nz, My,
+                       scale, out);
+}
+
+#define MOSAIC_FFT_DISPATCH(L_, CALL) \
+    switch (L_) {                     \
+        case 64: CALL(64); break;     \
+        case 128: CALL(128); break;   \
+        case 256: CALL(256); break;   \
+        case 512: CALL(512); break;   \
+        case 1024: CALL(1024); break; \
+        case 2048: CALL(2048); break; \
+        case 4096: CALL(4096); break; \
+        default: CALL(8192); break;   \
+    }
+
+// The pruned step, or false when its size rules do not hold.
+bool pruned_ok(int Ny, int Nz, int Ny2, int Nz2) {
+    if (!pow2(Ny2) || !pow2(Nz2)) return false;
+    const int Ly = fft_len(Ny), Lz = fft_len(Nz);
+    return Ly <= Ny2 && Lz <= Nz2 && Ly <= FFT_MAX && Lz <= FFT_MAX;
+}
+
+int pruned_step(hipStream_t st, const float2* in, int Ny, int Nz, double dy, double dz, double k,
+                double z, int Ny2, int Nz2, int edge, float2 padc, const LensArgs& lens,
+                float2* B, float2* out) {
+    const int Ly = fft_len(Ny), Lz = fft_len(Nz);
+#define CALL_RF(L) launch_rows_fwd<L>(st, in, Ny, Nz, Ny2, edge, padc, dy, dz, lens, B)
+    MOSAIC_FFT_DISPATCH(Ly, CALL_RF)
+#undef CALL_RF
+    MOSAIC_HIP_CHECK(hipGetLastError());
+#define CALL_C(L) launch_cols<L>(st, B, Nz, Nz2, Ny2, edge, padc, dy, dz, k, z)
+    MOSAIC_FFT_DISPATCH(Lz, CALL_C)
+#undef CALL_C
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    const float scale = static_cast<float>(1.0 / (static_cast<double>(Ny2) * Nz2));
+#define CALL_RI(L) launch_rows_inv<L>(st, B, Ny, Nz, Ny2, scale, out)
+    MOSAIC_FFT_DISPATCH(Ly, CALL_RI)
+#undef CALL_RI
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+
 // hipFFT plans cached per (device, Nz2, Ny2); the caller's workspace is the
 // work area, so no plan allocates.
 struct PlanCache {
@@ -187,11 +642,14 @@ extern "C" int mosaic_asp_workspace(int Ny2, int Nz2, size_t* bytes) {
 extern "C" int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, double dz,
                                     double wavelength, double z, int Ny2, int Nz2, int pad_edge,
                                     float pad_re, float pad_im, const mosaic_lens_t* lens,
-                                    float* field_out, void* workspace, size_t workspace_bytes,
-                                    mosaic_stream_t stream) {
+                                    int method, float* field_out, void* workspace,
+                                    size_t workspace_bytes, mosaic_stream_t stream) {
     if (!field_in || !field_out || Ny < 1 || Nz < 1 || Ny2 < Ny || Nz2 < Nz ||
-        !(dy > 0.0) || !(dz > 0.0) || !(wavelength > 0.0) || !workspace)
+        !(dy > 0.0) || !(dz > 0.0) || !(wavelength > 0.0) || !workspace || method < 0 ||
+        method > 2)
         return MOSAIC_ERR_ARGS;
+    const bool pruned = pruned_ok(Ny, Nz, Ny2, Nz2);
+    if (method == 2 && !pruned) return MOSAIC_ERR_UNSUPPORTED;
     hipfftHandle plan;
     size_t work = 0;
     int rc = get_plan(Nz2, Ny2, &plan, &work);
@@ -211,6 +669,10 @@ extern "C" int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, doubl
         L.parab_coef = lens->parab_coef;
         L.uniform_amp = lens->uniform_amp;
     }
+    if (pruned && method != 1)
+        return pruned_step(st, reinterpret_cast<const float2*>(field_in), Ny, Nz, dy, dz, k, z,
+                           Ny2, Nz2, pad_edge ? 1 : 0, make_float2(pad_re, pad_im), L, F,
+                           reinterpret_cast<float2*>(field_out));
     dim3 blk(256);
     hipLaunchKernelGGL(k_pad_lens, dim3(blocks_for(Ny2, 256), Nz2), blk, 0, st,
                        reinterpret_cast<const float2*>(field_in), Ny, Nz, Ny2, Nz2, y0, z0,

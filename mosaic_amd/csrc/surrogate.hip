@@ -184,17 +184,20 @@ __global__ void k_shuffle2(float* __restrict__ G, float* __restrict__ Y, int nim
 }
 
 // Mean-squared error over each member's outputs; dP = 2 (P - T) / n.
+// Weighted mean squared error of member b: loss[b] += mean_i w(i) (P_b - T)^2,
+// w(i) = row_w[i / row_len] (1 without row weights).
 __global__ void k_mse(const float* __restrict__ P, const float* __restrict__ T,
                       float* __restrict__ dP, float* __restrict__ loss, long long n,
-                      long long sP) {
+                      long long sP, const float* __restrict__ row_w, long long row_len) {
     __shared__ float red[256];
     const int b = blockIdx.y;
     float s = 0.f;
     for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const float w = row_w ? row_w[i / row_len] : 1.f;
         const float d = P[b * sP + i] - T[i];
-        dP[b * sP + i] = 2.f * d / static_cast<float>(n);
-        s += d * d;
+        dP[b * sP + i] = 2.f * w * d / static_cast<float>(n);
+        s += w * d * d;
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -235,7 +238,7 @@ __global__ void k_ens_stats(const float* __restrict__ P, float* __restrict__ mea
         }
         const float mu = a / members;
         mean[static_cast<long long>(f) * npix + p] = mu;
-        s += sqrtf(fmaxf(q / members - mu * mu, 0.f));
+        s += fmaxf(q / members - mu * mu, 0.f);
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -243,7 +246,7 @@ __global__ void k_ens_stats(const float* __restrict__ P, float* __restrict__ mea
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) spread[f] = red[0] / npix;
+    if (threadIdx.x == 0) spread[f] = sqrtf(red[0] / npix);
 }
 
 int grid1(long long n) { return static_cast<int>((n + 255) / 256); }
@@ -336,10 +339,13 @@ extern "C" int mosaic_sg_shuffle2(float* G, float* Y, int nimg, int H, int W, in
 }
 
 extern "C" int mosaic_sg_mse(const float* P, const float* T, float* dP, float* loss, long long n,
-                             long long strideP, int batch, mosaic_stream_t stream) {
+                             long long strideP, int batch, const float* row_weight,
+                             long long row_len, mosaic_stream_t stream) {
     if (!P || !T || !dP || !loss || n < 1) return MOSAIC_ERR_ARGS;
+    if (row_weight && (row_len < 1 || n % row_len)) return MOSAIC_ERR_ARGS;
     hipLaunchKernelGGL(k_mse, dim3(std::min<long long>(1024, grid1(n)), batch), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), P, T, dP, loss, n, strideP);
+                       static_cast<hipStream_t>(stream), P, T, dP, loss, n, strideP, row_weight,
+                       row_len);
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

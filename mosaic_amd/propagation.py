@@ -12,9 +12,10 @@ restated:
   distance is split into ceil(|z| / limit) equal sub-steps, each of which
   recomputes its pad size.
 
-Each step runs in ``libmosaic_amd.so`` (``mosaic_asp_propagate``): pad (with
-an optional fused thin lens), hipFFT forward, transfer function, inverse,
-crop.
+Each step runs in ``libmosaic_amd.so`` (``mosaic_asp_propagate``): by default
+a pruned three-pass transform that never forms the padded grid (thin lens,
+padding, transfer function and crop fused into its passes), else pad,
+hipFFT forward, transfer function, inverse, crop.
 """
 import ctypes
 
@@ -123,9 +124,15 @@ def _ptr(t):
     return ctypes.c_void_p(int(t.data_ptr()))
 
 
+METHODS = {"auto": 0, "fft2d": 1, "pruned": 2}
+
+
 def asp_step(field, dy, dz, wavelength, z, Ny2, Nz2, padding_mode="edge", pad_constant=0.0,
-             lens=None, out=None, stream=None):
-    """One step on a (Nz, Ny) complex64 device tensor; returns the new field."""
+             lens=None, out=None, stream=None, method="auto"):
+    """One step on a (Nz, Ny) complex64 device tensor; returns the new field.
+
+    ``method``: "auto" (the pruned three-pass transform where the sizes allow
+    it), "fft2d" (padded 2-D hipFFT round trip) or "pruned"."""
     if field.dtype != torch.complex64 or field.dim() != 2 or not field.is_cuda:
         raise ValueError("field must be a (Nz, Ny) complex64 device tensor")
     field = field.contiguous()
@@ -139,13 +146,13 @@ def asp_step(field, dy, dz, wavelength, z, Ny2, Nz2, padding_mode="edge", pad_co
         _ptr(torch.view_as_real(field)), Ny, Nz, float(dy), float(dz), float(wavelength),
         float(z), int(Ny2), int(Nz2), int((padding_mode or "edge").lower() != "constant"),
         float(pc.real), float(pc.imag), ctypes.byref(lens) if lens is not None else None,
-        _ptr(torch.view_as_real(out)), _ptr(ws), int(ws.numel()),
+        METHODS[method], _ptr(torch.view_as_real(out)), _ptr(ws), int(ws.numel()),
         ctypes.c_void_p(int(s.cuda_stream))), "mosaic_asp_propagate")
     return out
 
 
 def propagate(field, dy, dz, wavelength, z, step_max=None, pad_factor=1.0, padding_mode="edge",
-              pad_constant=0.0, lens=None):
+              pad_constant=0.0, lens=None, method="auto"):
     """Free-space propagation with the reference's automatic sub-stepping.
 
     ``lens`` (a mosaic_lens_t) is applied to the input field before the
@@ -156,7 +163,7 @@ def propagate(field, dy, dz, wavelength, z, step_max=None, pad_factor=1.0, paddi
     for i, (zs, Ny2, Nz2) in enumerate(plan_steps(Ny, Nz, dy, dz, wavelength, z, step_max,
                                                    pad_factor)):
         E = asp_step(E, dy, dz, wavelength, zs, Ny2, Nz2, padding_mode, pad_constant,
-                     lens=lens if i == 0 else None)
+                     lens=lens if i == 0 else None, method=method)
     return E
 
 

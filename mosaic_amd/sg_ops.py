@@ -72,12 +72,21 @@ def shuffle2(G, Y, nimg, H, W, Co, backward=False):
           "mosaic_sg_shuffle2")
 
 
-def mse(P, T, dP, loss):
+def mse(P, T, dP, loss, row_weight=None):
+    """Per member b of P (members, ...): loss[b] += mean(w (P_b - T)^2) and
+    dP_b = its gradient; ``row_weight`` (rows,) weighs each row of T's last axis."""
     _f32(P, T, dP, loss)
     nb = P.shape[0]
     n = P[0].numel()
-    check(_lib.lib().mosaic_sg_mse(_p(P), _p(T), _p(dP), _p(loss), n, n, nb, _s()),
-          "mosaic_sg_mse")
+    rl = 0
+    if row_weight is not None:
+        _f32(row_weight)
+        rl = n // row_weight.numel()
+        if rl * row_weight.numel() != n:
+            raise ValueError("row_weight must have one entry per row of T")
+    check(_lib.lib().mosaic_sg_mse(_p(P), _p(T), _p(dP), _p(loss), n, n, nb,
+                                   _p(row_weight) if row_weight is not None else None, rl,
+                                   _s()), "mosaic_sg_mse")
 
 
 def adam(w, g, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):

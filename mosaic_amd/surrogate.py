@@ -8,19 +8,26 @@ convolutional decoder of 2x2 stride-2 transposed convolutions,
 (SiLU between layers, none at the output).  Every layer is one batched GEMM
 (``mosaic_sg_gemm``: BF16 MFMA, FP32 accumulation; the ensemble is the GEMM
 batch) plus hand-written elementwise kernels; the backward pass and the Adam
-update run on the same kernels.  The output is the normalised log-intensity
-patch  y = (ln(I / I0 + 1e-4) - mu) / sd.
+update run on the same kernels.  The output is, with ``target="log"`` (the
+default), the normalised log-intensity (ln(I / I0 + 1e-4) - mu) / sd trained
+on its mean squared error, or, with ``target="rel"``, the intensity patch
+y = I / I0 trained on each image's squared relative L2 error
+||y' - y||^2 / ||y||^2.
 
-Inputs (``DefectParams``): dislocation core x, y [nm, +-50], Burgers vector
+Defect parameters (9): dislocation core x, y [nm, +-50], Burgers vector
 polar/azimuth angles, line direction polar/azimuth angles, hydrostatic strain
 (0..1e-3), stacking-fault shift fraction (0..1), rocking angle [deg, +-0.05].
+The network sees them as 11 features (``features``): the two directions as
+unit vectors, so that the azimuth's wrap at 2 pi is no discontinuity.
 
 Training images come from the GPU kinematic sum itself
 (``make_training_set``): the defect field is applied on the device
 (``mosaic_defect_positions``), the crystal staged and summed at its rocking
-angle.  At scan time (``SurrogateScan``) the ensemble predicts every frame;
-frames whose ensemble spread exceeds a threshold calibrated on held-out data
-are recomputed with the plain sum, which also serves as the error judge.
+angle.  At scan time (``SurrogateScan``) the plain sum computes anchor frames (the
+first and last angle), which judge the network and correct each member's
+prediction; the other frames come from the corrected ensemble, and those
+whose ensemble disagreement exceeds a threshold calibrated on held-out scans
+are recomputed with the plain sum.
 """
 import ctypes
 import math
@@ -35,7 +42,8 @@ from . import sg_ops as S
 from . import tables
 from ._lib import check
 
-N_IN = 9
+N_IN = 9            # defect parameters, rocking angle last
+N_FEAT = 11         # network inputs (``features``)
 LOG_EPS = 1e-4
 A_AU = 4.08
 NU_AU = 0.44
@@ -62,6 +70,22 @@ def sample_params(n, rng):
 def normalise(params):
     return (2.0 * (np.asarray(params) - RANGES[:, 0]) / (RANGES[:, 1] - RANGES[:, 0]) - 1.0
             ).astype(np.float32)
+
+
+def features(params):
+    """(B, 9) defect parameters -> (B, 11) float32 network inputs in [-1, 1]:
+    core x, y, Burgers unit vector, line unit vector, strain, fault shift, angle."""
+    P = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    z = normalise(P).astype(np.float64)
+    out = np.empty((P.shape[0], N_FEAT))
+    out[:, 0:2] = z[:, 0:2]
+    for c, (t, f) in ((2, (2, 3)), (5, (4, 5))):
+        th, ph = P[:, t], P[:, f]
+        out[:, c] = np.sin(th) * np.cos(ph)
+        out[:, c + 1] = np.sin(th) * np.sin(ph)
+        out[:, c + 2] = np.cos(th)
+    out[:, 8:11] = z[:, 6:9]
+    return out.astype(np.float32)
 
 
 def _unit(theta, phi):
@@ -165,6 +189,7 @@ class ImageGenerator:
         K.restage(staged, pos)
         out = torch.zeros((F, self.Nz, self.Ny), dtype=torch.complex64, device=self.device)
         K.kin_sum(staged, self.spc, self.opts, rot, tr, self.coords, out)
+        self.path_used = staged.path_used
         return (out.abs() ** 2).reshape(F, -1)
 
 
@@ -199,12 +224,15 @@ class Layer:
 class SurrogateEnsemble:
     """Ensemble MLP + transposed-conv decoder, all on the surrogate kernels."""
 
-    MLP = (N_IN, 256, 512, 8 * 8 * 64)
+    MLP = (N_FEAT, 256, 512, 8 * 8 * 64)
     CHANNELS = (64, 64, 32, 16, 16, 16)
 
-    def __init__(self, members=5, seed=0, device=None, out_hw=128):
+    def __init__(self, members=5, seed=0, device=None, out_hw=128, target="log"):
         if out_hw < 16 or out_hw & (out_hw - 1):
             raise ValueError("out_hw must be a power of two >= 16")
+        if target not in ("rel", "log"):
+            raise ValueError("target is 'rel' or 'log'")
+        self.target = target
         self.out_hw = out_hw
         n_stage = int(round(math.log2(out_hw // 8)))
         # decoder stages (H, Cin, Cout): 8x8x64 doubling up to out_hw, one channel out
@@ -230,10 +258,10 @@ class SurrogateEnsemble:
 
     # ------------------------------------------------------------ forward
     def forward(self, X):
-        """X (B, 9) normalised inputs on the device -> (E, B, 128*128) outputs."""
+        """X (B, 11) ``features`` on the device -> (E, B, 128*128) outputs."""
         B = X.shape[0]
         E = self.E
-        self._X = X.reshape(1, B, N_IN).contiguous()
+        self._X = X.reshape(1, B, N_FEAT).contiguous()
         self._acts = []
         h = self._X
         for L in self.mlp:
@@ -299,24 +327,35 @@ class SurrogateEnsemble:
     # ------------------------------------------------------------- fitting
     def set_normalisation(self, I):
         I0 = float(I.max().item())
+        if self.target == "rel":
+            self.norm = (I0, 0.0, 1.0)
+            return
         y = torch.log(I / I0 + LOG_EPS)
         self.norm = (I0, float(y.mean().item()), float(y.std().item()))
 
     def targets(self, I):
         I0, mu, sd = self.norm
+        if self.target == "rel":
+            return (I / I0).contiguous()
         return ((torch.log(I / I0 + LOG_EPS) - mu) / sd).contiguous()
 
     def intensity(self, y):
         I0, mu, sd = self.norm
+        if self.target == "rel":
+            return I0 * torch.clamp(y, min=0.0)
         return I0 * (torch.exp(y * sd + mu) - LOG_EPS)
 
     def train(self, P, I, epochs, batch=128, lr=2e-3, seed=0, log=print, lr_min=1e-5):
-        """Adam on the mean squared error of the normalised log-intensity."""
+        """Adam on each image's squared relative L2 error (``target="rel"``)
+        or on the mean squared error of the normalised log-intensity."""
         if self.norm is None:
             self.set_normalisation(I)
         n = P.shape[0]
-        X = torch.as_tensor(normalise(P), device=self.device)
+        X = torch.as_tensor(features(P), device=self.device)
         Y = self.targets(I)
+        # per-image weights 1 / mean(y^2): the loss is the squared relative error
+        Wt = (1.0 / (Y * Y).mean(dim=1).clamp_min(1e-30)).contiguous() \
+            if self.target == "rel" else None
         rng = np.random.default_rng(seed)
         loss = torch.zeros(self.E, dtype=torch.float32, device=self.device)
         steps = epochs * (n // batch)
@@ -331,7 +370,8 @@ class SurrogateEnsemble:
                 out = self.forward(X[idx])
                 dP = torch.empty_like(out)
                 # loss and gradient of the mean over the batch's pixels
-                S.mse(out, Y[idx].reshape(-1).contiguous(), dP, loss)
+                S.mse(out, Y[idx].reshape(-1).contiguous(), dP, loss,
+                      row_weight=Wt[idx].contiguous() if Wt is not None else None)
                 self.backward(dP)
                 cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * s / max(1, steps)))
                 self.adam_step(cur)
@@ -340,14 +380,17 @@ class SurrogateEnsemble:
             ep_loss = (loss / nb).cpu().numpy()
             hist.append(ep_loss)
             if log:
-                log(f"[surrogate] epoch {ep + 1}/{epochs} mse per member "
+                log(f"[surrogate] epoch {ep + 1}/{epochs} loss per member "
                     f"{np.array2string(ep_loss, precision=4)}")
         return np.array(hist)
 
     def predict(self, P, chunk=256):
         """Ensemble mean intensity (B, npix) and per-frame spread (B,), in
-        chunks of ``chunk`` frames (the decoder GEMMs have B x H x W rows)."""
-        Xall = torch.as_tensor(normalise(P), device=self.device)
+        chunks of ``chunk`` frames (the decoder GEMMs have B x H x W rows).
+        The spread is the members' RMS deviation over the frame, relative to
+        the RMS of the mean with ``target="rel"`` (comparable with the
+        relative L2 error)."""
+        Xall = torch.as_tensor(features(P), device=self.device)
         means, spreads = [], []
         for i in range(0, Xall.shape[0], chunk):
             out = self.forward(Xall[i:i + chunk].contiguous())
@@ -355,26 +398,110 @@ class SurrogateEnsemble:
             mean = torch.empty((B, out.shape[2]), dtype=torch.float32, device=self.device)
             spread = torch.empty(B, dtype=torch.float32, device=self.device)
             S.ensemble_stats(out.contiguous(), mean, spread)
+            if self.target == "rel":
+                spread = spread / (mean * mean).mean(dim=1).sqrt().clamp_min(1e-30)
             means.append(self.intensity(mean))
             spreads.append(spread)
         return torch.cat(means), torch.cat(spreads)
+
+
+    def member_intensities(self, P, chunk=256):
+        """Each member's intensity (E, B, npix) for (B, 9) defect parameters."""
+        Xall = torch.as_tensor(features(P), device=self.device)
+        outs = []
+        for i in range(0, Xall.shape[0], chunk):
+            outs.append(self.intensity(self.forward(Xall[i:i + chunk].contiguous())))
+        return torch.cat(outs, dim=1)
 
 
 def rel_l2(a, b):
     return (torch.linalg.norm(a - b, dim=-1) / torch.linalg.norm(b, dim=-1))
 
 
-class SurrogateScan:
-    """A rocking scan predicted by the ensemble, gated frames recomputed."""
+def anchor_weights(angles, anchor_angles):
+    """(F, A) weights interpolating anchor values linearly in angle (held
+    constant beyond the outer anchors)."""
+    a = np.asarray(angles, dtype=np.float64)
+    x = np.asarray(anchor_angles, dtype=np.float64)
+    W = np.zeros((a.size, x.size))
+    for f, t in enumerate(a):
+        if x.size == 1 or t <= x[0]:
+            W[f, 0] = 1.0
+        elif t >= x[-1]:
+            W[f, -1] = 1.0
+        else:
+            j = int(np.searchsorted(x, t, side="right")) - 1
+            u = (t - x[j]) / (x[j + 1] - x[j])
+            W[f, j], W[f, j + 1] = 1.0 - u, u
+    return W.astype(np.float32)
 
-    def __init__(self, model, gen, threshold):
-        self.model, self.gen, self.threshold = model, gen, float(threshold)
+
+class SurrogateScan:
+    """A rocking scan from the ensemble, anchored on plain frames.
+
+    The plain sum computes the anchor frames (by default the first and the
+    last angle).  Each member's prediction is corrected by that member's own
+    error at the anchors, interpolated linearly in angle; the corrected
+    members' mean is the frame, and their RMS disagreement relative to the
+    frame's RMS is its spread.  Frames whose spread exceeds ``threshold`` are
+    recomputed with the plain sum.  The anchors also judge the network: when
+    its raw error at an anchor exceeds ``anchor_limit`` the whole scan is
+    computed with the plain sum.
+    """
+
+    def __init__(self, model, gen, threshold, anchors=(0, -1), anchor_limit=0.25):
+        self.model, self.gen = model, gen
+        self.threshold = float(threshold)
+        self.anchors = tuple(anchors)
+        self.anchor_limit = float(anchor_limit)
 
     def run(self, p, angles_deg):
-        P = np.repeat(np.asarray(p, dtype=np.float64)[None], len(angles_deg), axis=0)
-        P[:, 8] = angles_deg
-        I, spread = self.model.predict(P)
+        """(I (F, npix), spread (F,), indices recomputed with the plain sum)."""
+        angles = np.asarray(angles_deg, dtype=np.float64)
+        F = angles.size
+        ia = sorted({a % F for a in self.anchors})
+        P = np.repeat(np.asarray(p, dtype=np.float64)[None], F, axis=0)
+        P[:, 8] = angles
+        Im = self.model.member_intensities(P)                    # (E, F, npix)
+        Ia = self.gen.intensities(p, angles[ia])                 # (A, npix), plain
+        raw = rel_l2(Im[:, ia].mean(dim=0), Ia)
+        if float(raw.max()) > self.anchor_limit:
+            spread = torch.full((F,), float("inf"), device=Ia.device)
+            return self.gen.intensities(p, angles), spread, np.arange(F)
+        W = torch.as_tensor(anchor_weights(angles, angles[ia]), device=Ia.device)
+        C = (Im + torch.matmul(W, Ia[None] - Im[:, ia])).contiguous()
+        mean = torch.empty(C.shape[1:], dtype=torch.float32, device=C.device)
+        spread = torch.empty(F, dtype=torch.float32, device=C.device)
+        S.ensemble_stats(C, mean, spread)
+        spread = spread / (mean * mean).mean(dim=1).sqrt().clamp_min(1e-30)
+        mean = mean.clamp_min(0.0)
+        mean[ia] = Ia
+        spread[ia] = 0.0
         bad = torch.nonzero(spread > self.threshold).flatten().cpu().numpy()
         if bad.size:
-            I[bad] = self.gen.intensities(p, [angles_deg[i] for i in bad])
-        return I, spread, bad
+            mean[bad] = self.gen.intensities(p, angles[bad])
+        return mean, spread, bad
+
+
+def calibrate_threshold(model, gen, angles_deg, n_scans, seed, target=1e-2, anchors=(0, -1)):
+    """Largest spread threshold under which every non-anchor frame of
+    ``n_scans`` held-out scans meets ``target`` (relative L2 against the plain
+    sum); 0 when even the least uncertain frame misses it."""
+    rng = np.random.default_rng(seed)
+    scan = SurrogateScan(model, gen, float("inf"), anchors=anchors, anchor_limit=float("inf"))
+    F = len(angles_deg)
+    ia = {a % F for a in anchors}
+    sp, er = [], []
+    for _ in range(n_scans):
+        p = sample_params(1, rng)[0]
+        I, spread, _ = scan.run(p, angles_deg)
+        e = rel_l2(I, gen.intensities(p, angles_deg)).cpu().numpy()
+        s_ = spread.cpu().numpy()
+        for f in range(F):
+            if f not in ia:
+                sp.append(s_[f])
+                er.append(e[f])
+    sp, er = np.array(sp), np.array(er)
+    order = np.argsort(sp)
+    ok = np.cumprod(er[order] <= target).astype(bool)
+    return float(sp[order][ok].max()) if ok[0] else 0.0

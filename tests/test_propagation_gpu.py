@@ -34,18 +34,59 @@ def smooth_field(ny, nz, seed):
     return (env * np.exp(1j * ph)).astype(np.complex64)
 
 
+@pytest.mark.parametrize("method", ["pruned", "fft2d"])
 @pytest.mark.parametrize("mode", ["edge", "constant"])
-def test_single_step_vs_oracle(device, mode):
+def test_single_step_vs_oracle(device, mode, method):
     E0 = smooth_field(200, 160, 0)
     d, z = 0.1e-6, 0.02
     steps = MP.plan_steps(200, 160, d, d, LAM, z)
     assert len(steps) == 1
     _, Ny2, Nz2 = steps[0]
     got = MP.asp_step(torch.from_numpy(E0).to(device), d, d, LAM, z, Ny2, Nz2,
-                      padding_mode=mode, pad_constant=0.25 - 0.5j).cpu().numpy()
+                      padding_mode=mode, pad_constant=0.25 - 0.5j, method=method).cpu().numpy()
     want = P.asp_step(E0, d, d, LAM, z, Ny2, Nz2, padding_mode=mode, pad_constant=0.25 - 0.5j)
     e = rel(got, want)
-    print(f"[parity] asp step {mode} {Ny2}x{Nz2}: rel={e:.3e}")
+    print(f"[parity] asp step {mode} {method} {Ny2}x{Nz2}: rel={e:.3e}")
+    assert e <= TOL
+
+
+# (Ny, Nz, Ny2, Nz2): odd sizes, no padding (R = 1), one axis unpadded,
+# lengths at the 64 minimum, Nz a multiple of 128 (the XCD row grouping),
+# wide pads (R = 8)
+PRUNED_SHAPES = [(200, 160, 512, 256), (64, 64, 64, 64), (100, 3, 256, 64), (37, 256, 64, 256),
+                 (130, 384, 1024, 1024), (1000, 128, 8192, 128), (256, 96, 2048, 1024)]
+
+
+@pytest.mark.parametrize("shape", PRUNED_SHAPES)
+@pytest.mark.parametrize("lens", [False, True])
+def test_pruned_matches_padded_fft(device, shape, lens):
+    """The pruned three-pass step against the padded 2-D hipFFT step, both on
+    the GPU, edge and constant padding, with and without the fused lens."""
+    Ny, Nz, Ny2, Nz2 = shape
+    E0 = torch.from_numpy(smooth_field(Ny, Nz, 3)).to(device)
+    d, z = 0.1e-6, 0.01
+    L = MP.lens_struct({"focal_length": 500.0, "thickness": 1.0, "number": 1,
+                        "absorption_sigma": 40.0}, LAM) if lens else None
+    for mode in ("edge", "constant"):
+        a = MP.asp_step(E0, d, d, LAM, z, Ny2, Nz2, padding_mode=mode, pad_constant=0.1 + 0.2j,
+                        lens=L, method="pruned").cpu().numpy()
+        b = MP.asp_step(E0, d, d, LAM, z, Ny2, Nz2, padding_mode=mode, pad_constant=0.1 + 0.2j,
+                        lens=L, method="fft2d").cpu().numpy()
+        e = rel(a, b)
+        print(f"[parity] pruned vs padded {shape} {mode} lens={lens}: rel={e:.3e}")
+        assert e <= TOL
+
+
+def test_pruned_bench_size_vs_padded(device):
+    """2048^2 field on 8192^2 pads (the CRL + 2 m workload's step), pruned
+    against the padded hipFFT round trip."""
+    n = 2048
+    E0 = torch.from_numpy(smooth_field(n, n, 5)).to(device)
+    d = 170e-9
+    a = MP.asp_step(E0, d, d, LAM, 1.0, 8192, 8192, method="pruned")
+    b = MP.asp_step(E0, d, d, LAM, 1.0, 8192, 8192, method="fft2d")
+    e = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    print(f"[parity] pruned vs padded 2048^2 on 8192^2: rel={e:.3e}")
     assert e <= TOL
 
 

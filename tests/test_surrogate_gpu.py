@@ -4,8 +4,9 @@
 * the ensemble's hand-written forward/backward against torch autograd on the
   same architecture (float32 with the GEMM operands rounded to bfloat16 as
   the kernels round them);
-* a short training run lowers the loss; the gate sends high-spread frames to
-  the plain sum.
+* a short training run lowers the loss; the anchored scan keeps the plain
+  anchor frames and sends high-spread frames (or, when the anchors judge the
+  network wrong, every frame) to the plain sum.
 """
 import math
 
@@ -71,7 +72,7 @@ def torch_reference(model, X):
 def test_ensemble_forward_backward_vs_torch(device):
     model = SG.SurrogateEnsemble(members=2, seed=3, device=device)
     rng = np.random.default_rng(0)
-    X = torch.as_tensor(rng.uniform(-1, 1, (6, 9)).astype(np.float32), device=device)
+    X = torch.as_tensor(rng.uniform(-1, 1, (6, SG.N_FEAT)).astype(np.float32), device=device)
     T = torch.as_tensor(rng.standard_normal((6, 128 * 128)).astype(np.float32), device=device)
     out = model.forward(X)
     ref, params = torch_reference(model, X)
@@ -101,3 +102,19 @@ def test_short_training_and_gate(device):
     assert hist[-1].mean() < hist[0].mean()
     Ipred, spread = model.predict(P[:8])
     assert Ipred.shape == (8, 32 * 32) and torch.all(spread >= 0)
+
+    # the anchored scan: anchors are the plain frames; with a zero threshold
+    # every other frame falls back to the plain sum; with an infinite one none
+    angles = np.linspace(-0.05, 0.05, 9)
+    p = P[0]
+    plain = gen.intensities(p, angles)
+    I0, sp0, bad0 = SG.SurrogateScan(model, gen, 0.0, anchor_limit=np.inf).run(p, angles)
+    assert sorted(bad0.tolist()) == list(range(1, 8))
+    assert torch.allclose(I0, plain, rtol=1e-6, atol=0)
+    I1, sp1, bad1 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf).run(p, angles)
+    assert bad1.size == 0 and float(sp1[0]) == 0.0 and float(sp1[-1]) == 0.0
+    assert torch.equal(I1[0], plain[0]) and torch.equal(I1[-1], plain[-1])
+    assert torch.all(sp1[1:-1] > 0)
+    # an anchor error above the limit sends the whole scan to the plain sum
+    I2, _, bad2 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=0.0).run(p, angles)
+    assert bad2.size == 9 and torch.allclose(I2, plain, rtol=1e-6, atol=0)

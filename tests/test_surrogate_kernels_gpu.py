@@ -75,6 +75,13 @@ def test_mse_adam_stats(device):
     S.mse(P, T, dP, loss)
     assert torch.allclose(loss, ((P - T) ** 2).mean(dim=1), rtol=1e-5)
     assert torch.allclose(dP, 2 * (P - T) / 50, atol=1e-6)
+    # row weights: T as 5 rows of 10
+    rw = torch.rand(5, device=device) + 0.5
+    loss.zero_()
+    S.mse(P, T, dP, loss, row_weight=rw)
+    wfull = rw.repeat_interleave(10)
+    assert torch.allclose(loss, (wfull * (P - T) ** 2).mean(dim=1), rtol=1e-5)
+    assert torch.allclose(dP, 2 * wfull * (P - T) / 50, atol=1e-6)
     w = torch.randn(100, device=device)
     g = torch.randn(100, device=device)
     m = torch.zeros(100, device=device)
@@ -91,4 +98,4 @@ def test_mse_adam_stats(device):
     spread = torch.empty(4, device=device)
     S.ensemble_stats(Pm, mean, spread)
     assert torch.allclose(mean, Pm.mean(0), atol=1e-6)
-    assert torch.allclose(spread, Pm.std(0, unbiased=False).mean(1), atol=1e-5)
+    assert torch.allclose(spread, Pm.var(0, unbiased=False).mean(1).sqrt(), atol=1e-5)
