@@ -203,7 +203,7 @@ typedef struct {
 } mosaic_lens_t;
 
 /* Workspace bytes for one angular-spectrum step on an (Nz2, Ny2) padded grid
- * (the padded field plus the FFT work area). */
+ * (the padded field, the FFT work area and the pruned transform's tables). */
 int mosaic_asp_workspace(int Ny2, int Nz2, size_t* bytes);
 
 /*

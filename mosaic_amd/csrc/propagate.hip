@@ -224,17 +224,20 @@ __device__ __forceinline__ void fft8(float2* v) {
 }
 
 // One Stockham stage of radix RD on butterfly j (j < L / RD), stride Ns.
+// Twiddles of a stage (stride Ns, radix RD) sit at tw[off + (r - 1) Ns + k],
+// k = j mod Ns: consecutive lanes read consecutive entries.
 template <int L, int RD, bool INV>
-__device__ __forceinline__ void stage_load(const float2* buf, const float2* tw, int Ns, int j,
-                                           float2* v) {
+__device__ __forceinline__ void stage_load(const float2* buf, const float2* tw, int off, int Ns,
+                                           int j, float2* v) {
 #pragma unroll
     for (int r = 0; r < RD; ++r) v[r] = buf[lp(j + r * (L / RD))];
     const int k = j & (Ns - 1);
-    const int step = k * (L / (Ns * RD));
+    if (Ns > 1) {
 #pragma unroll
-    for (int r = 1; r < RD; ++r) {
-        const float2 w = tw[r * step];
-        v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+        for (int r = 1; r < RD; ++r) {
+            const float2 w = tw[off + (r - 1) * Ns + k];
+            v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+        }
     }
     if (RD == 8) {
         fft8<INV>(v);
@@ -262,15 +265,16 @@ __device__ void fft_lds(float2* buf, const float2* tw) {
     constexpr int T = fft_threads(L);
     constexpr int LG = ilog2c(L);
     const int t = threadIdx.x;
-    int Ns = 1;
+    int Ns = 1, off = 0;
 #pragma unroll
     for (int s = 0; s < LG / 3; ++s) {
         float2 v[8];
         const bool act = t < L / 8;
-        if (act) stage_load<L, 8, INV>(buf, tw, Ns, t, v);
+        if (act) stage_load<L, 8, INV>(buf, tw, off, Ns, t, v);
         __syncthreads();
         if (act) stage_store<L, 8>(buf, Ns, t, v);
         __syncthreads();
+        off += 7 * Ns;
         Ns *= 8;
     }
     if (LG % 3 == 2) {
@@ -278,7 +282,7 @@ __device__ void fft_lds(float2* buf, const float2* tw) {
         float2 v[PER][4];
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            if (t + q * T < NB) stage_load<L, 4, INV>(buf, tw, Ns, t + q * T, v[q]);
+            if (t + q * T < NB) stage_load<L, 4, INV>(buf, tw, off, Ns, t + q * T, v[q]);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q)
@@ -289,7 +293,7 @@ __device__ void fft_lds(float2* buf, const float2* tw) {
         float2 v[PER][2];
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            if (t + q * T < NB) stage_load<L, 2, INV>(buf, tw, Ns, t + q * T, v[q]);
+            if (t + q * T < NB) stage_load<L, 2, INV>(buf, tw, off, Ns, t + q * T, v[q]);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q)
@@ -298,23 +302,83 @@ __device__ void fft_lds(float2* buf, const float2* tw) {
     }
 }
 
-template <int L>
-__device__ __forceinline__ void fill_twiddles(float2* tw) {
-    for (int i = threadIdx.x; i < L; i += fft_threads(L)) tw[i] = wpow(i, L);
+// Per-axis tables of a step, computed once in FP64 (k_axis_tables):
+//   w[e]  = w^e = e^{-2 pi i e / M}            (every twiddle of the axis)
+//   c1[k] = (1 - w^{ok}) / (1 - w^k),  c1[0] = o
+//   c2[k] = (w^{(o+N)k} - 1) / (1 - w^k),  c2[0] = M - o - N
+// so that the constant runs contribute a c1[k] + b c2[k] to bin k.
+//   q[k]  = e^{-i z kf^2 / (2 k0)}, kf the bin's signed frequency (rad/m)
+//           (the paraxial part of the transfer function along this axis)
+struct AxisTab {
+    const float2* w;
+    const float2* c1;
+    const float2* c2;
+    const float2* q;
+    int M, N, o;
+};
+
+__global__ void k_axis_tables(int M, int N, double sfreq, double z, double k0,
+                              float2* __restrict__ w, float2* __restrict__ c1,
+                              float2* __restrict__ c2, float2* __restrict__ q) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    const int o = (M - N) / 2;
+    double s, c;
+    sincospi(-2.0 * k / M, &s, &c);
+    w[k] = make_float2(static_cast<float>(c), static_cast<float>(s));
+    {
+        const double kf = sfreq * (k < (M + 1) / 2 ? k : k - M);
+        double ph = -z * kf * kf / (2.0 * k0);
+        ph -= TWO_PI * rint(ph / TWO_PI);
+        double qs, qc;
+        sincos(ph, &qs, &qc);
+        q[k] = make_float2(static_cast<float>(qc), static_cast<float>(qs));
+    }
+    if (k == 0) {
+        c1[0] = make_float2(static_cast<float>(o), 0.f);
+        c2[0] = make_float2(static_cast<float>(M - o - N), 0.f);
+        return;
+    }
+    double so, co, sn, cn;
+    sincospi(-2.0 * ((static_cast<long long>(o) * k) % M) / M, &so, &co);
+    sincospi(-2.0 * ((static_cast<long long>(o + N) * k) % M) / M, &sn, &cn);
+    // 1 / (1 - w^k)
+    const double dr = 1.0 - c, di = -s, inv = 1.0 / (dr * dr + di * di);
+    const double ir = dr * inv, ii = -di * inv;
+    const double ar = 1.0 - co, ai = -so;          // 1 - w^{ok}
+    const double br = cn - 1.0, bi = sn;           // w^{(o+N)k} - 1
+    c1[k] = make_float2(static_cast<float>(ar * ir - ai * ii), static_cast<float>(ar * ii + ai * ir));
+    c2[k] = make_float2(static_cast<float>(br * ir - bi * ii), static_cast<float>(br * ii + bi * ir));
 }
 
-// Contribution of the two constant runs to bin k (see the header above).
-__device__ __forceinline__ float2 edge_term(float2 a, float2 b, int k, int o, int N, int M) {
-    if (k == 0)
-        return make_float2(a.x * o + b.x * (M - o - N), a.y * o + b.y * (M - o - N));
-    const float2 wo = wpow(static_cast<long long>(o) * k, M);
-    const float2 won = wpow(static_cast<long long>(o + N) * k, M);
-    const float2 w1 = wpow(k, M);
-    const float2 num = cadd(csub(csub(a, b), cmul(a, wo)), cmul(b, won));
-    const float2 den = make_float2(1.f - w1.x, -w1.y);
-    const float inv = 1.f / (den.x * den.x + den.y * den.y);
-    const float2 q = cmulc(num, den);
-    return make_float2(q.x * inv, q.y * inv);
+// The stage-major twiddle table of fft_lds (L - 1 entries): for each stage
+// (Ns, RD) and r = 1..RD-1, k < Ns, w_L^{k r L / (Ns RD)} = w^{k r L R / (Ns RD)}.
+template <int L>
+__device__ __forceinline__ void fill_twiddles(float2* tw, const AxisTab& X) {
+    constexpr int LG = ilog2c(L), T = fft_threads(L);
+    const int R = X.M / L;
+    int Ns = 1, off = 0;
+#pragma unroll
+    for (int s = 0; s < LG / 3 + (LG % 3 ? 1 : 0); ++s) {
+        const int RD = s < LG / 3 ? 8 : (LG % 3 == 2 ? 4 : 2);
+        for (int i = threadIdx.x; i < (RD - 1) * Ns; i += T) {
+            const int r = 1 + i / Ns, k = i - (r - 1) * Ns;
+            tw[off + i] = X.w[k * r * (L / (Ns * RD)) * R];
+        }
+        off += (RD - 1) * Ns;
+        Ns *= RD;
+    }
+}
+
+__device__ __forceinline__ float2 wtab(const AxisTab& X, long long e) {
+    return X.w[static_cast<int>(e & (X.M - 1))];
+}
+
+// Bin k of the padded line's transform from FFT_L output value f of branch
+// r = k mod R: w^{ok} f + a c1[k] + b c2[k].
+__device__ __forceinline__ float2 bin_value(const AxisTab& X, float2 f, int k, float2 a, float2 b) {
+    return cadd(cadd(cmul(f, wtab(X, static_cast<long long>(X.o) * k)), cmul(a, X.c1[k])),
+                cmul(b, X.c2[k]));
 }
 
 // Row r of a pass that streams rows through B[k * Nz + row]: groups of 16
@@ -344,15 +408,15 @@ __device__ __forceinline__ float2 lens_phase(float2 v, int iy, int iz, int Ny, i
 // written column-major: B[ky * Nz + row].
 template <int L>
 __global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_fwd(
-    const float2* __restrict__ in, int Ny, int Nz, int My, int edge, float2 padc, double dx,
+    const float2* __restrict__ in, int Nz, const AxisTab Y, int edge, float2 padc, double dx,
     double dy, LensArgs lens, float2* __restrict__ B) {
     constexpr int T = fft_threads(L), E = L / T;
     __shared__ float2 buf[L + L / 8];
     __shared__ float2 tw[L];
-    fill_twiddles<L>(tw);
+    fill_twiddles<L>(tw, Y);
+    const int Ny = Y.N, My = Y.M;
     const int row = xcd_row(blockIdx.x, Nz);
     const int t = threadIdx.x;
-    const int oy = (My - Ny) / 2;
     const float2* src = in + static_cast<int64_t>(row) * Ny;
     float2 x[E];
 #pragma unroll
@@ -379,7 +443,7 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_fwd(
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int n = t + e * T;
-            buf[lp(n)] = n < Ny ? cmul(x[e], wpow(static_cast<long long>(n) * r, My))
+            buf[lp(n)] = n < Ny ? cmul(x[e], wtab(Y, static_cast<long long>(n) * r))
                                 : make_float2(0.f, 0.f);
         }
         __syncthreads();
@@ -388,27 +452,32 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_fwd(
         for (int e = 0; e < E; ++e) {
             const int m = t + e * T;
             const int ky = R * m + r;
-            const float2 X = cadd(cmul(buf[lp(m)], wpow(static_cast<long long>(oy) * ky, My)),
-                                  edge_term(a, b, ky, oy, Ny, My));
-            B[static_cast<int64_t>(ky) * Nz + row] = X;
+            B[static_cast<int64_t>(ky) * Nz + row] = bin_value(Y, buf[lp(m)], ky, a, b);
         }
     }
 }
 
 // Pass 2: per y-frequency column, forward along z, transfer function,
-// inverse along z onto the Nz crop rows; in place in B.
-template <int L>
-__global__ __launch_bounds__(fft_threads(L)) void k_asp_cols(float2* __restrict__ B, int Nz,
-                                                            int Mz, int My, int edge,
-                                                            float2 padc, double dy, double dz,
-                                                            double kw, double z) {
+// inverse along z onto the Nz crop rows; in place in B.  sy, sz convert a
+// signed bin index to rad/m (2 pi / (M d)).
+//
+// Transfer function.  With u = kt^2 / k0^2 the phase -z kt^2 / (k0 + sqrt(k0^2 - kt^2))
+// is -z ky^2/(2 k0) - z kz^2/(2 k0) - z kt^2/(2 k0) u / (1 + sqrt(1 - u))^2:
+// two paraxial terms (per column, and the z table q, both in FP64) and a
+// remainder that the host bounds by 1e-3 rad for EXACT = false, formed in
+// FP32 and exponentiated to third order.  EXACT = true (high numerical
+// aperture, evanescent bins) forms the whole phase in FP64.
+template <int L, bool EXACT>
+__global__ __launch_bounds__(fft_threads(L)) __attribute__((amdgpu_waves_per_eu(4)))
+void k_asp_cols(float2* __restrict__ B, const AxisTab Z, int My, int edge, float2 padc, double sy,
+                double sz, double kw, double z) {
     constexpr int T = fft_threads(L), E = L / T;
     __shared__ float2 buf[L + L / 8];
     __shared__ float2 tw[L];
-    fill_twiddles<L>(tw);
+    fill_twiddles<L>(tw, Z);
+    const int Nz = Z.N, Mz = Z.M, oz = Z.o;
     const int ky = blockIdx.x;
     const int t = threadIdx.x;
-    const int oz = (Mz - Nz) / 2;
     float2* col = B + static_cast<int64_t>(ky) * Nz;
     float2 x[E], y[E];
 #pragma unroll
@@ -426,41 +495,62 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_cols(float2* __restrict_
         a = ky == 0 ? make_float2(padc.x * My, padc.y * My) : make_float2(0.f, 0.f);
         b = a;
     }
-    const double kyf = kfreq(ky, My, dy);
+    const double kyf = sy * (ky < (My + 1) / 2 ? ky : ky - My);
+    const double kw2 = kw * kw - kyf * kyf;
+    float2 cy;
+    {
+        double ph = -z * kyf * kyf / (2.0 * kw);
+        ph -= TWO_PI * rint(ph / TWO_PI);
+        float sn, cs;
+        sincosf(static_cast<float>(ph), &sn, &cs);
+        cy = make_float2(cs, sn);
+    }
+    const float zk = static_cast<float>(z / (2.0 * kw)), ik2 = static_cast<float>(1.0 / (kw * kw));
+    const float kyf2 = static_cast<float>(kyf * kyf), szf = static_cast<float>(sz);
     const int R = Mz / L;
     for (int r = 0; r < R; ++r) {
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int n = t + e * T;
-            buf[lp(n)] = n < Nz ? cmul(x[e], wpow(static_cast<long long>(n) * r, Mz))
+            buf[lp(n)] = n < Nz ? cmul(x[e], wtab(Z, static_cast<long long>(n) * r))
                                 : make_float2(0.f, 0.f);
         }
         __syncthreads();
         fft_lds<L, false>(buf, tw);
-#pragma unroll
+#pragma unroll 2
         for (int e = 0; e < E; ++e) {
             const int m = t + e * T;
             const int kz = R * m + r;
-            float2 Z = cadd(cmul(buf[lp(m)], wpow(static_cast<long long>(oz) * kz, Mz)),
-                            edge_term(a, b, kz, oz, Nz, Mz));
-            const double kzf = kfreq(kz, Mz, dz);
-            const double kt2 = kyf * kyf + kzf * kzf;
-            const double kx2 = kw * kw - kt2;
-            double ph, amp;
-            if (kx2 >= 0.0) {
-                ph = -z * kt2 / (sqrt(kx2) + kw);
-                amp = 1.0;
+            const float2 V = bin_value(Z, buf[lp(m)], kz, a, b);
+            const int kzs = kz < (Mz + 1) / 2 ? kz : kz - Mz;
+            float2 H;
+            if (EXACT) {
+                const double kzf = sz * kzs;
+                const double kx2 = kw2 - kzf * kzf;
+                double ph, amp = 1.0;
+                if (kx2 >= 0.0) {
+                    ph = -z * (kyf * kyf + kzf * kzf) / (sqrt(kx2) + kw);
+                } else {
+                    ph = -z * kw;
+                    amp = exp(-fabs(z) * sqrt(-kx2));
+                }
+                ph -= TWO_PI * rint(ph / TWO_PI);
+                float sn, cs;
+                sincosf(static_cast<float>(ph), &sn, &cs);
+                const float am = static_cast<float>(amp);
+                H = make_float2(am * cs, am * sn);
             } else {
-                ph = -z * kw;
-                amp = exp(-fabs(z) * sqrt(-kx2));
+                const float kzf = szf * static_cast<float>(kzs);
+                const float kt2 = kyf2 + kzf * kzf;
+                const float u = kt2 * ik2;
+                const float sq = 1.f + sqrtf(1.f - u);
+                const float rem = -zk * kt2 * u / (sq * sq);
+                const float r2 = rem * rem;
+                H = cmul(cmul(cy, Z.q[kz]),
+                         make_float2(1.f - 0.5f * r2, rem * (1.f - r2 * (1.f / 6.f))));
             }
-            ph -= TWO_PI * rint(ph / TWO_PI);
-            float s, c;
-            sincosf(static_cast<float>(ph), &s, &c);
-            const float am = static_cast<float>(amp);
-            Z = cmul(Z, make_float2(am * c, am * s));
-            buf[lp(m)] = Z;
+            buf[lp(m)] = cmul(V, H);
         }
         __syncthreads();
         fft_lds<L, true>(buf, tw);
@@ -469,7 +559,7 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_cols(float2* __restrict_
             const int n = t + e * T;
             if (n < Nz)
                 y[e] = cadd(y[e], cmulc(buf[lp((oz + n) & (L - 1))],
-                                        wpow(static_cast<long long>(oz + n) * r, Mz)));
+                                        wtab(Z, static_cast<long long>(oz + n) * r)));
         }
     }
 #pragma unroll
@@ -482,16 +572,16 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_cols(float2* __restrict_
 // Pass 3: inverse along y of each crop row, scaled, into the output field.
 template <int L>
 __global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_inv(const float2* __restrict__ B,
-                                                                int Ny, int Nz, int My,
+                                                                int Nz, const AxisTab Y,
                                                                 float scale,
                                                                 float2* __restrict__ out) {
     constexpr int T = fft_threads(L), E = L / T;
     __shared__ float2 buf[L + L / 8];
     __shared__ float2 tw[L];
-    fill_twiddles<L>(tw);
+    fill_twiddles<L>(tw, Y);
+    const int Ny = Y.N, My = Y.M, oy = Y.o;
     const int row = xcd_row(blockIdx.x, Nz);
     const int t = threadIdx.x;
-    const int oy = (My - Ny) / 2;
     float2 acc[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
@@ -510,7 +600,7 @@ __global__ __launch_bounds__(fft_threads(L)) void k_asp_rows_inv(const float2* _
             const int i = t + e * T;
             if (i < Ny)
                 acc[e] = cadd(acc[e], cmulc(buf[lp((oy + i) & (L - 1))],
-                                            wpow(static_cast<long long>(oy + i) * r, My)));
+                                            wtab(Y, static_cast<long long>(oy + i) * r)));
         }
     }
     float2* dst = out + static_cast<int64_t>(row) * Ny;
@@ -530,22 +620,26 @@ int fft_len(int n) {
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 template <int L>
-void launch_rows_fwd(hipStream_t st, const float2* in, int Ny, int Nz, int My, int edge,
+void launch_rows_fwd(hipStream_t st, const float2* in, int Nz, const AxisTab& Y, int edge,
                      float2 padc, double dx, double dy, LensArgs lens, float2* B) {
-    hipLaunchKernelGGL(k_asp_rows_fwd<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, in, Ny, Nz, My,
-                       edge, padc, dx, dy, lens, B);
+    hipLaunchKernelGGL(k_asp_rows_fwd<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, in, Nz, Y, edge,
+                       padc, dx, dy, lens, B);
 }
 template <int L>
-void launch_cols(hipStream_t st, float2* B, int Nz, int Mz, int My, int edge, float2 padc,
-                 double dy, double dz, double kw, double z) {
-    hipLaunchKernelGGL(k_asp_cols<L>, dim3(My), dim3(fft_threads(L)), 0, st, B, Nz, Mz, My, edge,
-                       padc, dy, dz, kw, z);
+void launch_cols(hipStream_t st, bool exact, float2* B, const AxisTab& Z, int My, int edge,
+                 float2 padc, double sy, double sz, double kw, double z) {
+    if (exact)
+        hipLaunchKernelGGL((k_asp_cols<L, true>), dim3(My), dim3(fft_threads(L)), 0, st, B, Z, My,
+                           edge, padc, sy, sz, kw, z);
+    else
+        hipLaunchKernelGGL((k_asp_cols<L, false>), dim3(My), dim3(fft_threads(L)), 0, st, B, Z,
+                           My, edge, padc, sy, sz, kw, z);
 }
 template <int L>
-void launch_rows_inv(hipStream_t st, const float2* B, int Ny, int Nz, int My, float scale,
+void launch_rows_inv(hipStream_t st, const float2* B, int Nz, const AxisTab& Y, float scale,
                      float2* out) {
-    hipLaunchKernelGGL(k_asp_rows_inv<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, B, Ny, Nz, My,
-                       scale, out);
+    hipLaunchKernelGGL(k_asp_rows_inv<L>, dim3(Nz), dim3(fft_threads(L)), 0, st, B, Nz, Y, scale,
+                       out);
 }
 
 #define MOSAIC_FFT_DISPATCH(L_, CALL) \
@@ -560,27 +654,49 @@ void launch_rows_inv(hipStream_t st, const float2* B, int Ny, int Nz, int My, fl
         default: CALL(8192); break;   \
     }
 
-// The pruned step, or false when its size rules do not hold.
+// The pruned step's size rules.
 bool pruned_ok(int Ny, int Nz, int Ny2, int Nz2) {
     if (!pow2(Ny2) || !pow2(Nz2)) return false;
     const int Ly = fft_len(Ny), Lz = fft_len(Nz);
     return Ly <= Ny2 && Lz <= Nz2 && Ly <= FFT_MAX && Lz <= FFT_MAX;
 }
 
+// Table space of a pruned step: w, c1, c2, q per axis.
+size_t pruned_table_count(int Ny2, int Nz2) { return 4 * (static_cast<size_t>(Ny2) + Nz2); }
+
 int pruned_step(hipStream_t st, const float2* in, int Ny, int Nz, double dy, double dz, double k,
                 double z, int Ny2, int Nz2, int edge, float2 padc, const LensArgs& lens,
-                float2* B, float2* out) {
+                float2* B, float2* tabs, float2* out) {
     const int Ly = fft_len(Ny), Lz = fft_len(Nz);
-#define CALL_RF(L) launch_rows_fwd<L>(st, in, Ny, Nz, Ny2, edge, padc, dy, dz, lens, B)
+    const double sy = TWO_PI / (static_cast<double>(Ny2) * dy);
+    const double sz = TWO_PI / (static_cast<double>(Nz2) * dz);
+    AxisTab Y{tabs, tabs + Ny2, tabs + 2 * Ny2, tabs + 3 * Ny2, Ny2, Ny, (Ny2 - Ny) / 2};
+    float2* tz = tabs + 4 * static_cast<size_t>(Ny2);
+    AxisTab Zt{tz, tz + Nz2, tz + 2 * Nz2, tz + 3 * Nz2, Nz2, Nz, (Nz2 - Nz) / 2};
+    hipLaunchKernelGGL(k_axis_tables, dim3(blocks_for(Ny2, 256)), dim3(256), 0, st, Ny2, Ny, sy,
+                       z, k, tabs, tabs + Ny2, tabs + 2 * Ny2, tabs + 3 * Ny2);
+    hipLaunchKernelGGL(k_axis_tables, dim3(blocks_for(Nz2, 256)), dim3(256), 0, st, Nz2, Nz, sz,
+                       z, k, tz, tz + Nz2, tz + 2 * Nz2, tz + 3 * Nz2);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    // the split transfer function applies where its remainder stays below
+    // 1e-3 rad on the whole grid (no evanescent bins)
+    const double kt2max = (0.5 * Ny2 * sy) * (0.5 * Ny2 * sy) + (0.5 * Nz2 * sz) * (0.5 * Nz2 * sz);
+    const double umax = kt2max / (k * k);
+    bool exact = umax >= 0.5;
+    if (!exact) {
+        const double sq = 1.0 + std::sqrt(1.0 - umax);
+        exact = std::fabs(z) * kt2max / (2.0 * k) * umax / (sq * sq) > 1e-3;
+    }
+#define CALL_RF(L) launch_rows_fwd<L>(st, in, Nz, Y, edge, padc, dy, dz, lens, B)
     MOSAIC_FFT_DISPATCH(Ly, CALL_RF)
 #undef CALL_RF
     MOSAIC_HIP_CHECK(hipGetLastError());
-#define CALL_C(L) launch_cols<L>(st, B, Nz, Nz2, Ny2, edge, padc, dy, dz, k, z)
+#define CALL_C(L) launch_cols<L>(st, exact, B, Zt, Ny2, edge, padc, sy, sz, k, z)
     MOSAIC_FFT_DISPATCH(Lz, CALL_C)
 #undef CALL_C
     MOSAIC_HIP_CHECK(hipGetLastError());
     const float scale = static_cast<float>(1.0 / (static_cast<double>(Ny2) * Nz2));
-#define CALL_RI(L) launch_rows_inv<L>(st, B, Ny, Nz, Ny2, scale, out)
+#define CALL_RI(L) launch_rows_inv<L>(st, B, Nz, Y, scale, out)
     MOSAIC_FFT_DISPATCH(Ly, CALL_RI)
 #undef CALL_RI
     MOSAIC_HIP_CHECK(hipGetLastError());
@@ -635,6 +751,7 @@ extern "C" int mosaic_asp_workspace(int Ny2, int Nz2, size_t* bytes) {
     Carver c(nullptr);
     c.take<float2>(static_cast<size_t>(Ny2) * Nz2);
     c.take<char>(work);
+    c.take<float2>(pruned_table_count(Ny2, Nz2));
     *bytes = c.off + 256;
     return MOSAIC_OK;
 }
@@ -657,6 +774,7 @@ extern "C" int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, doubl
     Carver c(workspace);
     float2* F = c.take<float2>(static_cast<size_t>(Ny2) * Nz2);
     void* wa = c.take<char>(work);
+    float2* tabs = c.take<float2>(pruned_table_count(Ny2, Nz2));
     if (c.off > workspace_bytes) return MOSAIC_ERR_WORKSPACE;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int y0 = (Ny2 - Ny) / 2, z0 = (Nz2 - Nz) / 2;
@@ -672,7 +790,7 @@ extern "C" int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, doubl
     if (pruned && method != 1)
         return pruned_step(st, reinterpret_cast<const float2*>(field_in), Ny, Nz, dy, dz, k, z,
                            Ny2, Nz2, pad_edge ? 1 : 0, make_float2(pad_re, pad_im), L, F,
-                           reinterpret_cast<float2*>(field_out));
+                           tabs, reinterpret_cast<float2*>(field_out));
     dim3 blk(256);
     hipLaunchKernelGGL(k_pad_lens, dim3(blocks_for(Ny2, 256), Nz2), blk, 0, st,
                        reinterpret_cast<const float2*>(field_in), Ny, Nz, Ny2, Nz2, y0, z0,

@@ -159,14 +159,22 @@ class ImageGenerator:
                                       device=self.device)
         self.stage = w["stage"]
         self._staged = {}
+        self._poses = {}
         self._pos = torch.empty_like(self.base)
 
     def pose(self, angle_deg):
-        self.stage.set_single_motor_value_absolute("phi", float(angle_deg))
-        R = np.asarray(self.stage.rotation, np.float32)
-        T = np.asarray(self.stage.translation, np.float32)
-        self.stage.set_single_motor_value_absolute("phi", 0.0)
-        return R, T
+        """(R, T) of the stage at a rocking angle, cached per angle (the stage
+        model costs milliseconds per pose on the host)."""
+        key = float(angle_deg)
+        got = self._poses.get(key)
+        if got is None:
+            self.stage.set_single_motor_value_absolute("phi", key)
+            got = (np.asarray(self.stage.rotation, np.float32),
+                   np.asarray(self.stage.translation, np.float32))
+            self.stage.set_single_motor_value_absolute("phi", 0.0)
+            if len(self._poses) < 65536:
+                self._poses[key] = got
+        return got
 
     def stage_crystal(self, p):
         defect_positions(self.base, p, self.orient, out=self._pos)
