@@ -56,7 +56,8 @@ constexpr double PI = 3.14159265358979323846;
 struct AtomRec {      // 32 B, read whole by one scalar load
     float sx, sy, sz, s2;       // offset from the tile origin [wavelengths], |offset|^2
     float mag, ph;              // incident amplitude |w| and arg(w) / 2 pi
-    float pad0, pad1;           // pad0 carries the original atom index (bits)
+    float pad0;                 // the original atom index (bits)
+    float rmag;                 // sqrt(mag), shared by the two matrix-core operands
 };
 
 struct TileRec {      // 32 B
@@ -739,6 +740,7 @@ __global__ __launch_bounds__(TILE) void k_tiles(
         } else {
             rec.mag = 1.f;
         }
+        rec.rmag = sqrtf(rec.mag);
         rad = sqrtf(rec.s2);
     }
     atoms[static_cast<int64_t>(t) * TILE + i] = rec;
@@ -1096,16 +1098,16 @@ hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t 
 //     e^{2 pi i v.sigma} = A(a) B(b),  A(a) = w e^{2 pi i va(a).sigma},
 //     B(b) = e^{2 pi i wb(b).sigma},
 // and the tile sum S(a, b) = sum_j A_j(a) B_j(b) is a complex matrix product
-// over the atoms.  With A = Ar + i Ai and B = Br + i Bi, two two-block
-// products give it: [Ar; Ai] x [Br; Bi] (blocks Ar.Br and Ai.Bi) and
-// [Ar; Ai] x [Bi; Br] (blocks Ar.Bi and Ai.Br), so S = (C1_0 - C1_1) +
-// i (C2_0 + C2_1), register by register in the same lane.  The instruction is
-// v_mfma_f32_32x32x4_2b_f16: both 32 x 32 blocks take the same four atoms
-// (wave-uniform records, scalar loads), and each operand value is split into
-// two FP16 halves (22 significant bits), so three products per sum keep FP32
-// accuracy: hi.hi + hi.lo + lo.hi.  Each lane forms, per atom, one A value
-// and two B values with one v_cos each (an Im part is the cos of the phase
-// less a quarter turn).
+// over the atoms.  With A = Ar + i Ai and B = Br + i Bi, four real products
+// give it, S = (Ar.Br - Ai.Bi) + i (Ar.Bi + Ai.Br), accumulated register by
+// register in the same lane.  The instruction is v_mfma_f32_32x32x16_f16 (16
+// atoms per step; lane l holds row / column l mod 32 and atoms
+// 8 (l / 32) .. + 7, staged in LDS per tile), and each operand value is split
+// into two FP16 halves (22 significant bits), so three products per real
+// product keep FP32 accuracy: hi.hi + hi.lo + lo.hi, 12 MFMAs per step.
+// Each lane forms, per atom, the A value of its row and the B value of its
+// column (one v_sin and one v_cos each); |w| enters as sqrt|w| on both, so
+// every operand is a product that the FP16 split handles in one mixed FMA.
 //
 // The per-(tile, pixel) origin phase is Theta0(p) + K (X_t - u_p.O +
 // (|O|^2 - (u_p.O)^2) / (2 rho)).  Its pixel-only part, Theta0(p) -
@@ -1268,13 +1270,16 @@ __global__ __launch_bounds__(64 * MMA_WAVES) void k_sum_mma(const MmaArgs A) {
                         const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
                         pa = fmaf(hh, fmaf(-w, w, a.s2), pa);
                     }
-                    const float fa = __builtin_amdgcn_fractf(pa + a.ph);
-                    const float fb = __builtin_amdgcn_fractf(
-                        fmaf(wb[2], a.sz, fmaf(wb[1], a.sy, wb[0] * a.sx)));
-                    split16(a.mag * __builtin_amdgcn_cosf(fa), arh, arl, u);
-                    split16(a.mag * __builtin_amdgcn_sinf(fa), aih, ail, u);
-                    split16(__builtin_amdgcn_cosf(fb), brh, brl, u);
-                    split16(__builtin_amdgcn_sinf(fb), bih, bil, u);
+                    // phases in revolutions; v_sin / v_cos reduce |x| <= 256
+                    // themselves (the host bounds |pa| + 1 by the tile radius)
+                    const float fa = pa + a.ph;
+                    const float fb = fmaf(wb[2], a.sz, fmaf(wb[1], a.sy, wb[0] * a.sx));
+                    // |w| split as sqrt|w| on each operand: every value is a
+                    // product, which the FP16 split turns into one mixed FMA
+                    split16(a.rmag * __builtin_amdgcn_cosf(fa), arh, arl, u);
+                    split16(a.rmag * __builtin_amdgcn_sinf(fa), aih, ail, u);
+                    split16(a.rmag * __builtin_amdgcn_cosf(fb), brh, brl, u);
+                    split16(a.rmag * __builtin_amdgcn_sinf(fb), bih, bil, u);
                 }
                 crr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh, brh, crr, 0, 0, 0);
                 cii = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih, bih, cii, 0, 0, 0);
@@ -2227,8 +2232,11 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // times the largest offset, below 1e-6 rev
     // and the patch-constant second-order origin term: its change across a
     // patch, K |O|^2 |du| / rho, below 1e-7 rev
+    // (and the atom phase |v.sigma| + 1 <= 2 radius + 1 inside the +-256
+    // revolutions that v_sin / v_cos reduce exactly)
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
-                        K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7;
+                        K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
+                        2.0 * radius_wl + 2.0 <= 256.0;
     int path = opts->path == 0 ? (mma_ok ? 3 : (kp ? 2 : 1)) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok)) return MOSAIC_ERR_UNSUPPORTED;
 
