@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="perovskite_screw_5M_512px")
-    ap.add_argument("--cpu-pixels", type=int, default=48)
+    ap.add_argument("--cpu-pixels", type=int, default=768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-pixels", type=int, default=48)
     ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows", "mma"])
