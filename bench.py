@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-pixels", type=int, default=48)
-    ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows", "mma"])
+    ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows", "mma", "poly"])
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -272,7 +272,8 @@ def roofline(path_used, pairs, ksum_ms, workload):
 def kernel_name(path_used):
     """Sum kernel behind a mosaic_kin_sum path code."""
     base = path_used % 100
-    return {32: "k_sum_mma", 24: "k_sum_rec", 28: "k_sum_rec"}.get(base, "k_sum")
+    return {32: "k_sum_mma", 24: "k_sum_rec", 28: "k_sum_rec", 45: "k_sum_poly",
+            46: "k_sum_poly"}.get(base, "k_sum")
 
 
 def bench_adjoint(steps):

@@ -108,11 +108,11 @@ int mosaic_kin_restage(int64_t atom_count, const float* pos, const float* amp,
  * rho0 = |T_p|, frac(rho0 / lambda) in double-double, the forward-removal
  * Q cut (Beam.py:718-746), the unit vector and 1/(2 rho0), and per species
  * the factor (f0 + f' + i f'') r_e sqrt(P) in the pixel's direction.
- * Returns pixel_geom (host out [5]): the smallest pixel distance [m], the
+ * Returns pixel_geom (host out [6]): the smallest pixel distance [m], the
  * largest first and second differences of the pixel direction along a row,
- * the largest deviation of the pixel direction from the matrix-core
- * sum's per-patch quadratic model, and the largest second difference along
- * a column; the call synchronizes the stream for them.  Must precede mosaic_kin_sum
+ * the largest deviation of the pixel direction from the per-patch quadratic
+ * model of the far-field sums, and the largest second and first differences
+ * along a column; the call synchronizes the stream for them.  Must precede mosaic_kin_sum
  * on the same workspace.  `coords` is device (3, Ny*Nz) float64 metres.
  */
 int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host */,

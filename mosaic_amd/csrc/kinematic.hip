@@ -49,7 +49,7 @@ constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
-constexpr int N_GEOM = 5;            // pixel geometry statistics (mosaic_kin_pixels)
+constexpr int N_GEOM = 6;            // pixel geometry statistics (mosaic_kin_pixels)
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
 
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const int npix = Ny * Nz;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const double k = O.k;
-    double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0, du2z = 0.0;
+    double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0, du2z = 0.0, du1z = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
         double f, r, r_lo;
@@ -443,13 +443,17 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
             }
         }
         // and along the column
-        if (iy + 1 < Nz && iy > 0) {
-            double vx, vy, vz, wx, wy, wz;
+        if (iy + 1 < Nz) {
+            double vx, vy, vz;
             pixel_dir(cx, cy, cz, p + Ny, vx, vy, vz);
-            pixel_dir(cx, cy, cz, p - Ny, wx, wy, wz);
-            const double ax = vx - 2.0 * ux + wx, ay = vy - 2.0 * uy + wy,
-                         az = vz - 2.0 * uz + wz;
-            du2z = sqrt(ax * ax + ay * ay + az * az);
+            du1z = sqrt((vx - ux) * (vx - ux) + (vy - uy) * (vy - uy) + (vz - uz) * (vz - uz));
+            if (iy > 0) {
+                double wx, wy, wz;
+                pixel_dir(cx, cy, cz, p - Ny, wx, wy, wz);
+                const double ax = vx - 2.0 * ux + wx, ay = vy - 2.0 * uy + wy,
+                             az = vz - 2.0 * uz + wz;
+                du2z = sqrt(ax * ax + ay * ay + az * az);
+            }
         }
 
         // deviation from the patch model
@@ -475,18 +479,20 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     du2 = block_max(du2, lds);
     dpatch = block_max(dpatch, lds);
     du2z = block_max(du2z, lds);
+    du1z = block_max(du1z, lds);
     if (threadIdx.x == 0) {
         T.stat_blocks[N_GEOM * blockIdx.x] = rmin;
         T.stat_blocks[N_GEOM * blockIdx.x + 1] = du1;
         T.stat_blocks[N_GEOM * blockIdx.x + 2] = du2;
         T.stat_blocks[N_GEOM * blockIdx.x + 3] = dpatch;
         T.stat_blocks[N_GEOM * blockIdx.x + 4] = du2z;
+        T.stat_blocks[N_GEOM * blockIdx.x + 5] = du1z;
     }
 }
 
 __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double lds[16];
-    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0, 0.0};
+    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         v[0] = fmin(v[0], in[N_GEOM * i]);
         for (int q = 1; q < N_GEOM; ++q) v[q] = fmax(v[q], in[N_GEOM * i + q]);
@@ -1363,6 +1369,284 @@ hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------- moment sum
+// Far-field sum by moments over the same 32 x 32 patches and direction model
+// as the matrix-core sum.  With x = a / 16 and y = b / 16 (|x|, |y| <= 1) the
+// phase of atom j at pixel (a, b), relative to the patch's middle pixel, is
+//     2 pi theta_j + P_j x + Q_j x^2 + R_j y + S_j y^2   [rad],
+//     theta_j = va(0).sigma_j + ph_j,  P_j = 2 pi 16 gy.sigma_j,  Q_j = 2 pi 256 hy.sigma_j,
+//     R_j = 2 pi 16 gz.sigma_j,  S_j = 2 pi 256 hz.sigma_j.
+// The host bounds phi = max |P x + R y| (a few tenths of a radian: the tile
+// radius times the patch's angular size) and |Q|, |S| (~1e-4 rad), so
+//     e^{i(P x + R y)} (1 + i Q x^2 + i S y^2)
+//         = sum_{m+n<=D} i^{m+n} [T_mn - i (Q T_{m-2,n} + S T_{m,n-2})] x^m y^n,
+//     T_mn = P^m R^n / (m! n!),
+// to phi^{D+1} / (D+1)! (the Q, S terms kept where the rest of their monomial
+// has degree <= 3).  The tile sum over the patch is then a polynomial,
+//     S_t(x, y) = sum_{m+n<=D} C_mn x^m y^n,  C_mn = sum_j |w_j| e^{2 pi i theta_j} (...)_mn,
+// whose (D+1)(D+2)/2 complex moments cost one sincos and ~50 FMAs per atom
+// and patch (the matrix-core form builds 64 phasors per atom and patch).
+// Each lane accumulates the moments of 4 of the tile's atoms; the wave sums
+// them through LDS; every lane then evaluates its 16 pixels by Horner's
+// rule and applies the per-(tile, pixel) origin phase as the matrix-core
+// sum does.
+constexpr int POLY_WAVES = 4;
+
+template <int D>
+struct PolyN {
+    static constexpr int NC = (D + 1) * (D + 2) / 2;   // moments m + n <= D
+    static constexpr int NV = 2 * NC;                   // floats
+    static constexpr int NVP = (NV + 3) / 4 * 4;        // padded row in LDS
+};
+
+template <int D, bool SECOND>
+__global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
+    constexpr int NC = PolyN<D>::NC, NVP = PolyN<D>::NVP;
+    constexpr float TWO_PI_F = 6.283185307179586f;
+    __shared__ AtomRec tile_buf[2][TILE];
+    __shared__ float red[POLY_WAVES][64 * NVP];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int patch = blockIdx.x * POLY_WAVES + wave;
+    const bool live = patch < A.n_patches;       // idle waves compute on patch 0, write nothing
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+    const int pc = live ? patch : 0;
+    const PatchGeom P = patch_of(pc % A.pby, pc / A.pby, A.Ny, A.Nz);
+    const int Ny = A.Ny;
+    const int g = lane >> 5;
+    const int q = lane & 31;
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+
+    // the patch's direction model (sample frame), as in the matrix-core sum
+    double vc0[3];
+    float gyk[3], hyk[3], gzk[3], hzk[3];
+    {
+        const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                             pmid + (P.vz - 1 - P.mz) * Ny};
+        double vs[5][3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double4 u = A.u0h[samp[k]];
+            const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
+            vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
+            vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
+            vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double gq, hq;
+            vc0[d] = vs[0][d];
+            quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gq, hq);
+            gyk[d] = static_cast<float>(2.0 * PI * 16.0 * gq);
+            hyk[d] = static_cast<float>(2.0 * PI * 256.0 * hq);
+            quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gq, hq);
+            gzk[d] = static_cast<float>(2.0 * PI * 16.0 * gq);
+            hzk[d] = static_cast<float>(2.0 * PI * 256.0 * hq);
+        }
+    }
+    const double4 um = A.u0h[pmid];
+    const double th_mid = A.theta0[pmid];
+    const float xc0 = static_cast<float>(F.xs[0] - vc0[0]);
+    const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
+    const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
+    const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
+
+    // this lane's pixels: rows a(r) = 8 (r / 4) + 4 g + r % 4 of column q
+    const int prow = (P.z0 + min(q, P.vz - 1)) * Ny + P.y0;
+    auto arow = [&](int r) { return 8 * (r >> 2) + 4 * g + (r & 3); };
+    const float yq = static_cast<float>(q - P.mz) * (1.f / 16.f);
+    float acc_re[16], acc_im[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { acc_re[r] = 0.f; acc_im[r] = 0.f; }
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+    bool first = true;
+    int since = 0;
+    float* rw = red[wave];
+
+    if (t0 < t1) tile_buf[0][tid] = A.atoms[static_cast<int64_t>(t0) * TILE + tid];
+    __syncthreads();
+    for (int t = t0; t < t1; ++t) {
+        const int buf = (t - t0) & 1;
+        const bool has_next = t + 1 < t1;
+        const float4* nsrc = reinterpret_cast<const float4*>(
+            A.atoms + static_cast<int64_t>(has_next ? t + 1 : t) * TILE + tid);
+        const float4 n0 = nsrc[0], n1 = nsrc[1];
+        const TileRec tr = A.tiles[t];
+        const int tie = tr.count >> 16;
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+        double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+        ph_t -= floor(ph_t);
+        const double h2 = 2.0 * um.w;
+        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                     czl = (Oz - uo * um.z) * h2;
+        const float vx = static_cast<float>(vc0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+        const float vy = static_cast<float>(vc0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+        const float vz = static_cast<float>(vc0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                    koz = static_cast<float>(A.K * Oz);
+        const float pht = static_cast<float>(ph_t);
+
+        // moments of this lane's 4 atoms (records past the count are zero)
+        float cr[NC], ci[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) { cr[k] = 0.f; ci[k] = 0.f; }
+        const AtomRec* at = tile_buf[buf];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const AtomRec a = at[lane + 64 * i];
+            float th = fmaf(vz, a.sz, fmaf(vy, a.sy, vx * a.sx)) + a.ph;
+            if (SECOND) {
+                const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
+                th = fmaf(hh, fmaf(-w, w, a.s2), th);
+            }
+            const float wr = a.mag * __builtin_amdgcn_cosf(th);
+            const float wi = a.mag * __builtin_amdgcn_sinf(th);
+            const float Pj = fmaf(gyk[2], a.sz, fmaf(gyk[1], a.sy, gyk[0] * a.sx));
+            const float Rj = fmaf(gzk[2], a.sz, fmaf(gzk[1], a.sy, gzk[0] * a.sx));
+            const float Qj = fmaf(hyk[2], a.sz, fmaf(hyk[1], a.sy, hyk[0] * a.sx));
+            const float Sj = fmaf(hzk[2], a.sz, fmaf(hzk[1], a.sy, hzk[0] * a.sx));
+            float am[D + 1], bn[D + 1];
+            am[0] = 1.f;
+            bn[0] = 1.f;
+#pragma unroll
+            for (int m = 1; m <= D; ++m) {
+                am[m] = am[m - 1] * Pj * (1.f / m);
+                bn[m] = bn[m - 1] * Rj * (1.f / m);
+            }
+            int k = 0;
+#pragma unroll
+            for (int m = 0; m <= D; ++m) {
+#pragma unroll
+                for (int n = 0; n <= D - m; ++n, ++k) {
+                    const float T = am[m] * bn[n];
+                    float U = 0.f;
+                    if (m >= 2 && m - 2 + n <= 3) U = fmaf(Qj, am[m - 2] * bn[n], U);
+                    if (n >= 2 && m + n - 2 <= 3) U = fmaf(Sj, am[m] * bn[n - 2], U);
+                    cr[k] = fmaf(wr, T, fmaf(wi, U, cr[k]));
+                    ci[k] = fmaf(wi, T, fmaf(-wr, U, ci[k]));
+                }
+            }
+        }
+        // sum over the wave's lanes: row per lane in LDS, column sums per lane
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            rw[lane * NVP + 2 * k] = cr[k];
+            rw[lane * NVP + 2 * k + 1] = ci[k];
+        }
+        __syncthreads();
+        const int col = min(lane, NVP - 1);
+        float tot = 0.f;
+#pragma unroll 8
+        for (int l = 0; l < 64; ++l) tot += rw[l * NVP + col];
+        // (rw is rewritten only after the barrier that ends this tile)
+        // every lane gets the moments (wave-uniform), with the i^{m+n} factor
+        float Cr[NC], Ci[NC];
+        {
+            int k = 0;
+#pragma unroll
+            for (int m = 0; m <= D; ++m) {
+#pragma unroll
+                for (int n = 0; n <= D - m; ++n, ++k) {
+                    const float re = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(tot), 2 * k));
+                    const float im = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(tot), 2 * k + 1));
+                    switch ((m + n) & 3) {
+                        case 0: Cr[k] = re; Ci[k] = im; break;
+                        case 1: Cr[k] = -im; Ci[k] = re; break;
+                        case 2: Cr[k] = -re; Ci[k] = -im; break;
+                        default: Cr[k] = im; Ci[k] = -re; break;
+                    }
+                }
+            }
+        }
+        // G_m(y) = sum_n C_mn y^n for this lane's column, then each pixel row
+        float Gr[D + 1], Gi[D + 1];
+        {
+            int base = 0;
+#pragma unroll
+            for (int m = 0; m <= D; ++m) {
+                float gr = 0.f, gi = 0.f;
+#pragma unroll
+                for (int n = D - m; n >= 0; --n) {
+                    gr = fmaf(gr, yq, Cr[base + n]);
+                    gi = fmaf(gi, yq, Ci[base + n]);
+                }
+                Gr[m] = gr;
+                Gi[m] = gi;
+                base += D - m + 1;
+            }
+        }
+        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+            const float x = static_cast<float>(arow(r) - P.my) * (1.f / 16.f);
+            float sr = Gr[D], si = Gi[D];
+#pragma unroll
+            for (int m = D - 1; m >= 0; --m) {
+                sr = fmaf(sr, x, Gr[m]);
+                si = fmaf(si, x, Gi[m]);
+            }
+            const int p = prow + min(arow(r), P.vy - 1) + tie;
+            const float4 du = A.dup[p];
+            const float phs = __builtin_amdgcn_fractf(
+                pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+            const float sn = __builtin_amdgcn_sinf(phs);
+            const float cs = __builtin_amdgcn_cosf(phs);
+            const float2 gg = Gm[p];
+            const float zr = gg.x * cs - gg.y * sn, zi = gg.x * sn + gg.y * cs;
+            acc_re[r] = fmaf(zr, sr, fmaf(-zi, si, acc_re[r]));
+            acc_im[r] = fmaf(zr, si, fmaf(zi, sr, acc_im[r]));
+        }
+        if (++since == MMA_FLUSH || t + 1 == t1) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (live && arow(r) < P.vy && q < P.vz) {
+                    double2* o = out + prow + arow(r);
+                    double2 v = first ? make_double2(0.0, 0.0) : *o;
+                    v.x += static_cast<double>(acc_re[r]);
+                    v.y += static_cast<double>(acc_im[r]);
+                    *o = v;
+                }
+                acc_re[r] = 0.f;
+                acc_im[r] = 0.f;
+            }
+            first = false;
+            since = 0;
+        }
+        if (has_next) {
+            float4* dst = reinterpret_cast<float4*>(&tile_buf[buf ^ 1][tid]);
+            dst[0] = n0;
+            dst[1] = n1;
+        }
+        __syncthreads();
+    }
+    if (live && first) {                          // a slice without tiles
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            if (arow(r) < P.vy && q < P.vz) out[prow + arow(r)] = make_double2(0.0, 0.0);
+    }
+}
+
+template <int D, bool SECOND>
+hipError_t launch_poly(const MmaArgs& A, int n_slices, int n_frames, hipStream_t st) {
+    dim3 grid((A.n_patches + POLY_WAVES - 1) / POLY_WAVES, n_slices, n_frames);
+    hipLaunchKernelGGL((k_sum_poly<D, SECOND>), grid, dim3(64 * POLY_WAVES), 0, st, A);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ adjoint
 // Gradient of a real loss L(A) with respect to the sample-frame displacement
 // of every atom, given g = dL/dRe(A) + i dL/dIm(A) per frame and pixel.  With
@@ -2184,7 +2468,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
         !pixel_geom)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 3)
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 4)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
@@ -2237,13 +2521,34 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
                         K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
                         2.0 * radius_wl + 2.0 <= 256.0;
-    int path = opts->path == 0 ? (mma_ok ? 3 : (kp ? 2 : 1)) : opts->path;
-    if ((path == 2 && kp == 0) || (path == 3 && !mma_ok)) return MOSAIC_ERR_UNSUPPORTED;
+    // moment sum: the matrix-core sum's model bounds, plus the truncation of
+    // the moment expansion.  phi bounds |P x + R y| over the patch from the
+    // largest first differences along a row and a column (16 pixels each
+    // way, with the curvature), |Q|, |S| the quadratic model terms.
+    const double du1z = pixel_geom[5], du2z = pixel_geom[4];
+    const double phi = 2.0 * 3.14159265358979 * 16.0 *
+                       ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
+    const double qmax = 2.0 * 3.14159265358979 * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
+    auto trunc = [&](int D) {
+        double t = 1.0;
+        for (int i = 1; i <= D + 1; ++i) t *= phi / i;
+        return t;
+    };
+    // (both truncations are per-atom bounds at the patch corner for the
+    // largest tile; each within 3e-7 of the atom's term)
+    int poly_d = 0;
+    if (mma_ok && qmax <= 3e-4 && qmax * phi * phi * phi * phi / 24.0 <= 3e-7) {
+        if (trunc(5) <= 3e-7) poly_d = 5;
+        else if (trunc(6) <= 3e-7) poly_d = 6;
+    }
+    int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
+    if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
+        return MOSAIC_ERR_UNSUPPORTED;
 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
     hipError_t e;
-    if (path == 3) {
+    if (path == 3 || path == 4) {
         MmaArgs A;
         A.atoms = L.atoms;
         A.tiles = L.tiles;
@@ -2262,8 +2567,15 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         A.pby = (Ny + MMA_P - 1) / MMA_P;
         A.n_patches = A.pby * ((Nz + MMA_P - 1) / MMA_P);
         A.K = K;
-        e = use_second ? launch_mma<true>(A, L.n_slices, n_frames, st)
-                       : launch_mma<false>(A, L.n_slices, n_frames, st);
+        if (path == 3)
+            e = use_second ? launch_mma<true>(A, L.n_slices, n_frames, st)
+                           : launch_mma<false>(A, L.n_slices, n_frames, st);
+        else if (poly_d == 5)
+            e = use_second ? launch_poly<5, true>(A, L.n_slices, n_frames, st)
+                           : launch_poly<5, false>(A, L.n_slices, n_frames, st);
+        else
+            e = use_second ? launch_poly<6, true>(A, L.n_slices, n_frames, st)
+                           : launch_poly<6, false>(A, L.n_slices, n_frames, st);
     } else if (path == 2) {
         RecArgs A;
         A.atoms = L.atoms;
@@ -2330,7 +2642,8 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
                            L.partial, L.n_slices, npix, n_frames, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
     if (path_used)
-        *path_used = path == 3   ? 32 + (use_second ? 100 : 0)
+        *path_used = path == 4   ? 40 + poly_d + (use_second ? 100 : 0)
+                     : path == 3 ? 32 + (use_second ? 100 : 0)
                      : path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
                                  : (use_second ? 12 : 11);
     return MOSAIC_OK;

@@ -36,14 +36,15 @@ def species_table(consts):
     return arr
 
 
-PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3}
+PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4}
 
 
 def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
              apply_decay=False, path="auto"):
     """mosaic_kin_opts_t; ``path`` is "auto", "tile" (per-tile factors),
-    "rows" (row recurrence) or "mma" (matrix-core patches); the last two
-    decline when their bounds do not hold."""
+    "rows" (row recurrence), "mma" (matrix-core patches) or "poly" (moment
+    expansion over the same patches); the last three decline when their
+    bounds do not hold."""
     o = KinOpts()
     o.k = float(k)
     o.remove_forward = int(bool(remove_forward))
@@ -158,7 +159,7 @@ def kin_pixels(staged, species_tab, opts, coords, stream=None):
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
         raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
     coords = coords.contiguous()
-    geom = (ctypes.c_double * 5)()
+    geom = (ctypes.c_double * 6)()
     check(_lib.lib().mosaic_kin_pixels(len(species_tab), ctypes.cast(species_tab, ctypes.c_void_p),
                                        ctypes.byref(opts), staged.Ny, staged.Nz, _ptr(coords),
                                        _ptr(staged.workspace), int(staged.workspace.numel()),

@@ -57,7 +57,7 @@ def test_adjoint_vs_fp64_reference_and_fd(device, path):
     curv = 0.125 * 32 * 32 * m.pixel_geom[4]
     print(f"[adjoint] {path}: forward path {m.path_used}, patch column curvature {curv:.2e}")
     if path != "rows":
-        assert m.path_used % 100 == 32
+        assert m.path_used % 100 in (32, 45, 46)
         assert (curv > 1e-6) == curved
     s = float(It.max())
     uu = torch.from_numpy(u).to(device).requires_grad_(True)

@@ -52,7 +52,7 @@ def report(name, A, R):
     return e, ei
 
 
-@pytest.mark.parametrize("path", ["tile", "rows", "mma"])
+@pytest.mark.parametrize("path", ["tile", "rows", "mma", "poly"])
 @pytest.mark.parametrize("flags", [
     dict(apply_pol=True, decay=False, remove_forward=False),
     dict(apply_pol=False, decay=True, remove_forward=False),
@@ -68,7 +68,7 @@ def test_fcc_small_vs_oracle(device, flags, path):
     assert e80 <= TOL and e64 <= TOL
 
 
-@pytest.mark.parametrize("path", ["tile", "rows", "mma"])
+@pytest.mark.parametrize("path", ["tile", "rows", "mma", "poly"])
 def test_perovskite_multispecies_chunks(device, path):
     # 24 pixels per row: the 8-pixel row blocks tile it exactly; 21 leaves a
     # ragged block at the row end
@@ -111,7 +111,7 @@ def test_ein_bilinear_bitexact(device):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("path", ["tile", "rows", "mma"])
+@pytest.mark.parametrize("path", ["tile", "rows", "mma", "poly"])
 def test_scan_frames_match_single_poses(device, path):
     w = W.fcc_fault_scan111(cells=(6, 6, 8), npix=24, n_angles=5, half_range_deg=0.5)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
@@ -126,18 +126,20 @@ def test_scan_frames_match_single_poses(device, path):
     st.set_single_motor_value_absolute("phi", 0.0)
 
 
-@pytest.mark.parametrize("shape", [(1, 37), (37, 1), (3, 2), (17, 33)])
-def test_mma_ragged_patches(device, shape):
-    """Detectors whose sides are not multiples of the 16-pixel patch, down to
-    single rows and columns (one- and two-point patch models)."""
+@pytest.mark.parametrize("path", ["mma", "poly"])
+@pytest.mark.parametrize("shape", [(1, 37), (37, 1), (3, 2), (17, 33), (70, 45)])
+def test_patch_kernels_ragged(device, shape, path):
+    """Detectors whose sides are not multiples of the 32-pixel patch, down to
+    single rows and columns (one- and two-point patch models), for the
+    matrix-core and the moment sums."""
     w = W.fcc_au_bragg111(n=6, npix=8, seed=4)
     b, d = w["beam"], w["detector"]
     d.create_detector(shape, (7.5e5, 7.5e5))
     d.position_detector_absolute(1.0e10, np.rad2deg(2 * w["theta_b"]), 0.0)
-    A = gpu_field(w, path="mma")
-    assert b._last_path == 32
+    A = gpu_field(w, path=path)
+    assert b._last_path % 100 in ((32,) if path == "mma" else (45, 46))
     R = oracle_field(w, "f80")
-    e, _ = report(f"mma detector {shape}", A, R)
+    e, _ = report(f"{path} detector {shape}", A, R)
     assert e <= TOL
 
 
@@ -226,14 +228,14 @@ def test_empty_and_ragged(device):
 
 def test_c1_full_vs_fp64_oracle(device):
     """32,000 atoms x 128^2 pixels (5.2e8 pairs) against the FP64 restatement,
-    all three kernels; the automatic choice is the matrix-core form here."""
+    all four kernels; the automatic choice is the moment sum here."""
     w = W.fcc_au_bragg111(n=20, npix=128, seed=0)
     R = oracle_field(w, "f64")
-    for path in ("tile", "rows", "mma", "auto"):
+    for path in ("tile", "rows", "mma", "poly", "auto"):
         A = gpu_field(w, path=path)
         e, ei = report(f"C1 full {path} vs f64", A, R)
         assert e <= TOL and ei <= TOL
-    assert w["beam"]._last_path % 100 == 32
+    assert w["beam"]._last_path % 100 in (45, 46)
 
 
 def test_rows_second_order_close_detector(device):
@@ -256,15 +258,15 @@ def test_rows_second_order_close_detector(device):
 
 
 def test_auto_path_choice(device):
-    """Far field picks the matrix-core form; a 5 cm detector with 2 mm pixels
-    bends the rows and patches too much for it and for the recurrence, and
-    falls back to per-tile factors."""
+    """Far field picks the moment sum; a 5 cm detector with 2 mm pixels
+    bends the rows and patches too much for it, the matrix-core form and the
+    recurrence, and falls back to per-tile factors."""
     w = W.fcc_au_bragg111(n=4, npix=16)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
     b._kin_path = "auto"
     gpu_field(w)
     staged_path = b._last_path
-    assert staged_path == 32
+    assert staged_path in (45, 46)
     d.create_detector((16, 16), (2.0e7, 2.0e7))
     d.position_detector_absolute(5.0e8, np.rad2deg(2 * w["theta_b"]), 0.0)
     A = gpu_field(w)
