@@ -48,7 +48,7 @@ typedef struct {
     int apply_pol;          /* scale by sqrt(P) */
     int apply_decay;        /* scale by R0 / r */
     int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence,
-                               3 matrix-core patches */
+                               3 matrix-core patches, 4 moment expansion */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
 } mosaic_kin_opts_t;
@@ -129,19 +129,22 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * mosaic_fast launch (Beam.py:8246-8632, 9336-9391): same per-pair
  * semantics, with form factor, polarization and decay evaluated once per
  * (tile, pixel) and the phase split into an FP64 (tile, pixel) origin term
- * and an FP32 per-atom offset.  Three kernels: per-tile factors (exact
+ * and an FP32 per-atom offset.  Four kernels: per-tile factors (exact
  * distance per tile and pixel, optional second-order offset term) and, in
  * the far field, a row recurrence (per-pixel factors, far-field origin phase,
- * one sincos per atom per KP pixels) and a matrix-core form over 32 x 32
- * pixel patches (the tile sum as a complex product of per-row and
- * per-column phasors, FP16-split MFMA with FP32 accumulation);
- * opts->path = 0 picks the matrix-core form, else the recurrence, whenever
- * their error bounds hold.
+ * one sincos per atom per KP pixels), a matrix-core form over 32 x 32 pixel
+ * patches (the tile sum as a complex product of per-row and per-column
+ * phasors, FP16-split MFMA with FP32 accumulation) and a moment expansion
+ * over the same patches (the tile sum as a polynomial of degree D = 5 or 6 in
+ * the pixel offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle
+ * phasors); opts->path = 0 picks the moment expansion, else the matrix-core
+ * form, else the recurrence, whenever their error bounds hold.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
- * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels) or
- * 32 (matrix-core), plus 100 for the recurrence and matrix-core forms with
- * the atoms' second-order phase.  Stream-ordered, no host synchronization.
+ * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels),
+ * 32 (matrix-core) or 45/46 (moment expansion of degree 5/6), plus 100 for
+ * the far-field forms with the atoms' second-order phase.  Stream-ordered,
+ * no host synchronization.
  * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
  * close for the split's error bound -- the reference's "decline"
  * (Beam.py:9466-9468).
@@ -154,7 +157,7 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const float* trans /* host */, int Ny, int Nz,
                    const double* coords, void* out, int out_is_c128,
                    const double* stage_stats /* host [2] */,
-                   const double* pixel_geom /* host [4] */, void* workspace,
+                   const double* pixel_geom /* host [6] */, void* workspace,
                    size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
 
 /*
@@ -175,7 +178,7 @@ int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
                        const float* rot /* host */, const float* trans /* host */, int Ny,
                        int Nz, const double* stage_stats /* host [2] */,
-                       const double* pixel_geom /* host [4] */, const float* grad_field,
+                       const double* pixel_geom /* host [6] */, const float* grad_field,
                        float* grad_pos, void* workspace, size_t workspace_bytes, int path,
                        mosaic_stream_t stream);
 

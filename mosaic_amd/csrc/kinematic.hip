@@ -1396,7 +1396,8 @@ template <int D>
 struct PolyN {
     static constexpr int NC = (D + 1) * (D + 2) / 2;   // moments m + n <= D
     static constexpr int NV = 2 * NC;                   // floats
-    static constexpr int NVP = (NV + 3) / 4 * 4;        // padded row in LDS
+    static constexpr int NVP = NV | 1;                  // odd row stride in LDS: one
+                                                        // bank per lane for the stores
 };
 
 template <int D, bool SECOND>
