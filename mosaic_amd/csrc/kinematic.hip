@@ -2176,6 +2176,294 @@ void k_adjoint_mma(const AdjMmaArgs A) {
     }
 }
 
+// Adjoint by moments, the dual of k_sum_poly.  Per (frame, patch) and tile,
+// with the patch's direction model v(x, y) = vc + x Gy + x^2 Hy + y Gz + y^2 Hz
+// (x = a / 16, y = b / 16; Gy = 16 gy, Hy = 256 hy, ...) and the atom phase
+// expansion of the moment sum, e^{2 pi i v.sigma_j} = e^{2 pi i e0_j} sum_mn c_mn(j) x^m y^n,
+//     Z_j = sum_ab v(a, b) W(a, b) e^{2 pi i v.sigma_j}
+//         = e^{2 pi i e0_j} sum_mn c_mn(j) (vc M_mn + Gy M_{m+1,n} + Hy M_{m+2,n}
+//                                          + Gz M_{m,n+1} + Hz M_{m,n+2}),
+//     M_mn = sum_ab W(a, b) x^m y^n   (m + n <= D + 2),
+// so a wave forms the patch's pixel moments once (each lane the 16 pixels of
+// one row a = q, summed over the wave through LDS), and every atom contracts
+// its expansion coefficients with five shifted moment sets.  Z accumulates
+// per lane in FP32 over up to 16 (frame, patch) pairs, then in FP64 in LDS.
+template <int D, bool SECOND>
+__global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaArgs A) {
+    constexpr int DM = D + 2;
+    constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
+    constexpr int HALF = (NM + 1) / 2;              // reduced in two rounds
+    constexpr int NVP = (2 * HALF) | 1;
+    constexpr int NC = (D + 1) * (D + 2) / 2;
+    constexpr float TWO_PI_F = 6.283185307179586f;
+    __shared__ AtomRec tile[TILE];
+    __shared__ double zacc[TILE][6];
+    __shared__ float red[POLY_WAVES][64 * NVP];
+    __shared__ double rmx[POLY_WAVES];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int t = blockIdx.x;
+    tile[tid] = A.atoms[static_cast<int64_t>(t) * TILE + tid];
+    for (int i = tid; i < TILE * 6; i += 64 * POLY_WAVES) (&zacc[0][0])[i] = 0.0;
+    float mg = 0.f, mG = 0.f;
+    for (int i = tid; i < AMAX_BLOCKS; i += 64 * POLY_WAVES) {
+        mg = fmaxf(mg, A.amax[i]);
+        mG = fmaxf(mG, A.amax[AMAX_BLOCKS + i]);
+    }
+    const float sg = pow2_scale(static_cast<float>(block_max(static_cast<double>(mg), rmx)));
+    const float sG = pow2_scale(static_cast<float>(block_max(static_cast<double>(mG), rmx)));
+    __syncthreads();
+
+    const TileRec tr = A.tiles[t];
+    const int cnt = tr.count;
+    const float2* __restrict__ Gt = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+    const int h = lane >> 5, q = lane & 31;
+    const int Ny = A.Ny;
+    const int npairs = A.n_frames * A.n_patches;
+    float* rw = red[wave];
+    // this lane's atoms lane + 64 i and their FP32 partial Z
+    float zr[4][3], zi[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { zr[i][d] = 0.f; zi[i][d] = 0.f; }
+    int since = 0;
+    // every wave runs the same number of pair iterations (barriers inside)
+    const int iters = (npairs + POLY_WAVES - 1) / POLY_WAVES;
+
+    for (int it = 0; it < iters; ++it) {
+        const int pr = it * POLY_WAVES + wave;
+        const bool live = pr < npairs;
+        const int prc = live ? pr : 0;
+        const int f = prc / A.n_patches, patch = prc % A.n_patches;
+        const FrameRec& F = A.frames[f];
+        const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
+        const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+        const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+        const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+        const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+
+        float vc[3], Gy[3], Hy[3], Gz[3], Hz[3];
+        double vc0[3];
+        {
+            const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                                 pmid + (P.vz - 1 - P.mz) * Ny};
+            double vs[5][3];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const double4 u = A.u0h[samp[k]];
+                const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
+                vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
+                vs[k][1] = R01 * wx + R11 * wy + R21 * wz;
+                vs[k][2] = R02 * wx + R12 * wy + R22 * wz;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double gq, hq;
+                quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gq, hq);
+                Gy[d] = static_cast<float>(16.0 * gq);
+                Hy[d] = static_cast<float>(256.0 * hq);
+                quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gq, hq);
+                Gz[d] = static_cast<float>(16.0 * gq);
+                Hz[d] = static_cast<float>(256.0 * hq);
+                vc0[d] = vs[0][d];
+            }
+        }
+        const double4 um = A.u0h[pmid];
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+        double ph_t = A.theta0[pmid] + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+        ph_t -= floor(ph_t);
+        const double h2 = 2.0 * um.w;
+        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                     czl = (Oz - uo * um.z) * h2;
+        vc[0] = static_cast<float>(vc0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+        vc[1] = static_cast<float>(vc0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+        vc[2] = static_cast<float>(vc0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+        const float xc0 = static_cast<float>(F.xs[0] - vc0[0]);
+        const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
+        const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
+        const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
+        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                    koz = static_cast<float>(A.K * Oz);
+        const float pht = static_cast<float>(ph_t);
+
+        // W on this lane's row a = q, columns b = 16 s + 8 h + u, folded into
+        // H_n = sum_b W y_b^n
+        float Hr[DM + 1], Hi[DM + 1];
+#pragma unroll
+        for (int n = 0; n <= DM; ++n) { Hr[n] = 0.f; Hi[n] = 0.f; }
+        const float2* __restrict__ gf = A.gin + static_cast<int64_t>(f) * A.npix;
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if ((u & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                const int b = 16 * st + 8 * h + u;
+                const bool okp = live && q < P.vy && b < P.vz;
+                const int p = (P.z0 + min(b, P.vz - 1)) * Ny + P.y0 + min(q, P.vy - 1);
+                float wr = 0.f, wi = 0.f;
+                if (okp) {
+                    const float4 du = A.dup[p];
+                    const float phs = __builtin_amdgcn_fractf(
+                        pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+                    const float sn = __builtin_amdgcn_sinf(phs), cs = __builtin_amdgcn_cosf(phs);
+                    const float2 g0 = Gt[p], q0 = gf[p];
+                    const float gr = g0.x * sG, gi = g0.y * sG, qr = q0.x * sg, qi = q0.y * sg;
+                    const float zr_ = gr * cs - gi * sn, zi_ = gr * sn + gi * cs;
+                    wr = qr * zr_ + qi * zi_;          // conj(g) z
+                    wi = qr * zi_ - qi * zr_;
+                }
+                const float y = static_cast<float>(b - P.mz) * (1.f / 16.f);
+                float yn = 1.f;
+#pragma unroll
+                for (int n = 0; n <= DM; ++n) {
+                    Hr[n] = fmaf(wr, yn, Hr[n]);
+                    Hi[n] = fmaf(wi, yn, Hi[n]);
+                    yn *= y;
+                }
+            }
+        }
+        // M_mn = sum over lanes of x_q^m H_n, in two rounds through LDS
+        const float xq = static_cast<float>(q - P.my) * (1.f / 16.f);
+        float xm[DM + 1];
+        xm[0] = 1.f;
+#pragma unroll
+        for (int m = 1; m <= DM; ++m) xm[m] = xm[m - 1] * xq;
+        float Mr[NM], Mi[NM];
+#pragma unroll
+        for (int round = 0; round < 2; ++round) {
+            int k = 0;
+#pragma unroll
+            for (int m = 0; m <= DM; ++m) {
+#pragma unroll
+                for (int n = 0; n <= DM - m; ++n, ++k) {
+                    if (k / HALF == round) {
+                        const int c = k - round * HALF;
+                        rw[lane * NVP + 2 * c] = xm[m] * Hr[n];
+                        rw[lane * NVP + 2 * c + 1] = xm[m] * Hi[n];
+                    }
+                }
+            }
+            __syncthreads();
+            const int col = min(lane, NVP - 1);
+            float tot = 0.f;
+#pragma unroll 8
+            for (int l = 0; l < 64; ++l) tot += rw[l * NVP + col];
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < NM; ++kk) {
+                if (kk / HALF == round) {
+                    const int c = kk - round * HALF;
+                    Mr[kk] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), 2 * c));
+                    Mi[kk] = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(tot), 2 * c + 1));
+                }
+            }
+        }
+        // index of moment (m, n), m + n <= DM, in the order above
+        auto mk = [](int m, int n) { return m * (DM + 1) - m * (m - 1) / 2 + n; };
+
+        // per atom: expansion coefficients, five contractions, Z
+        const float Pk0 = TWO_PI_F * Gy[0], Pk1 = TWO_PI_F * Gy[1], Pk2 = TWO_PI_F * Gy[2];
+        const float Rk0 = TWO_PI_F * Gz[0], Rk1 = TWO_PI_F * Gz[1], Rk2 = TWO_PI_F * Gz[2];
+        const float Qk0 = TWO_PI_F * Hy[0], Qk1 = TWO_PI_F * Hy[1], Qk2 = TWO_PI_F * Hy[2];
+        const float Sk0 = TWO_PI_F * Hz[0], Sk1 = TWO_PI_F * Hz[1], Sk2 = TWO_PI_F * Hz[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const AtomRec a = tile[lane + 64 * i];
+            float e0 = fmaf(vc[2], a.sz, fmaf(vc[1], a.sy, vc[0] * a.sx));
+            if (SECOND) {
+                const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx)) - e0;
+                e0 = fmaf(hh, fmaf(-w, w, a.s2), e0);
+            }
+            const float Pj = fmaf(Pk2, a.sz, fmaf(Pk1, a.sy, Pk0 * a.sx));
+            const float Rj = fmaf(Rk2, a.sz, fmaf(Rk1, a.sy, Rk0 * a.sx));
+            const float Qj = fmaf(Qk2, a.sz, fmaf(Qk1, a.sy, Qk0 * a.sx));
+            const float Sj = fmaf(Sk2, a.sz, fmaf(Sk1, a.sy, Sk0 * a.sx));
+            float am[D + 1], bn[D + 1];
+            am[0] = 1.f;
+            bn[0] = 1.f;
+#pragma unroll
+            for (int m = 1; m <= D; ++m) {
+                am[m] = am[m - 1] * Pj * (1.f / m);
+                bn[m] = bn[m - 1] * Rj * (1.f / m);
+            }
+            // S_k = sum_mn c_mn M_{(m,n) + shift_k}, shifts (0,0) (1,0) (2,0) (0,1) (0,2)
+            float sr[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, si[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int m = 0; m <= D; ++m) {
+#pragma unroll
+                for (int n = 0; n <= D - m; ++n) {
+                    const float T = am[m] * bn[n];
+                    float U = 0.f;
+                    if (m >= 2 && m - 2 + n <= 3) U = fmaf(Qj, am[m - 2] * bn[n], U);
+                    if (n >= 2 && m + n - 2 <= 3) U = fmaf(Sj, am[m] * bn[n - 2], U);
+                    // c = i^{m+n} (T - i U)
+                    float cr_, ci_;
+                    switch ((m + n) & 3) {
+                        case 0: cr_ = T; ci_ = -U; break;
+                        case 1: cr_ = U; ci_ = T; break;
+                        case 2: cr_ = -T; ci_ = U; break;
+                        default: cr_ = -U; ci_ = -T; break;
+                    }
+                    const int k0 = mk(m, n), k1 = mk(m + 1, n), k2 = mk(m + 2, n),
+                              k3 = mk(m, n + 1), k4 = mk(m, n + 2);
+                    const int ks[5] = {k0, k1, k2, k3, k4};
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        sr[c] = fmaf(cr_, Mr[ks[c]], fmaf(-ci_, Mi[ks[c]], sr[c]));
+                        si[c] = fmaf(cr_, Mi[ks[c]], fmaf(ci_, Mr[ks[c]], si[c]));
+                    }
+                }
+            }
+            const float ce = __builtin_amdgcn_cosf(e0), se = __builtin_amdgcn_sinf(e0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float tr_ = vc[d] * sr[0] + Gy[d] * sr[1] + Hy[d] * sr[2] + Gz[d] * sr[3] +
+                                  Hz[d] * sr[4];
+                const float ti_ = vc[d] * si[0] + Gy[d] * si[1] + Hy[d] * si[2] + Gz[d] * si[3] +
+                                  Hz[d] * si[4];
+                zr[i][d] += ce * tr_ - se * ti_;
+                zi[i][d] += ce * ti_ + se * tr_;
+            }
+        }
+        if (++since == 16 || it + 1 == iters) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = lane + 64 * i;
+                if (j < cnt) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        atomicAdd(&zacc[j][2 * d], static_cast<double>(zr[i][d]));
+                        atomicAdd(&zacc[j][2 * d + 1], static_cast<double>(zi[i][d]));
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { zr[i][d] = 0.f; zi[i][d] = 0.f; }
+            }
+            since = 0;
+        }
+    }
+    __syncthreads();
+    if (tid < cnt) {
+        const AtomRec a = tile[tid];
+        const double scale = -2.0 * PI * A.K * 1e-10 /
+                             (static_cast<double>(sg) * static_cast<double>(sG));
+        const uint32_t idx = __float_as_uint(a.pad0);
+        double wi, wr;
+        sincospi(2.0 * static_cast<double>(a.ph), &wi, &wr);
+        wr *= a.mag;
+        wi *= a.mag;
+        for (int c = 0; c < 3; ++c) {
+            const double g = scale * (wr * zacc[tid][2 * c + 1] + wi * zacc[tid][2 * c]);
+            A.grad[3 * static_cast<int64_t>(idx) + c] += static_cast<float>(g);
+        }
+    }
+}
+
 // Sum slice partials in slice order and add into the output field.
 template <bool C128>
 __global__ void k_reduce(const double2* __restrict__ partial, int n_slices, int npix,
@@ -2455,6 +2743,30 @@ double rec_curv_bound(int kp, double d2u, double radius_wl) {
 }
 }  // namespace
 
+// Degree of the moment expansion (k_sum_poly, k_adjoint_poly), 0 where it
+// does not apply: the matrix-core form's model bounds, plus the truncation.
+// phi bounds |P x + R y| over a patch from the largest first differences
+// along a row and a column (16 pixels each way, with the curvature); qmax
+// bounds the quadratic model terms |Q|, |S|.  Both truncations are per-atom
+// bounds at the patch corner for the largest tile, each within 3e-7 of the
+// atom's term.
+static int poly_degree(bool mma_ok, double radius_wl, const double* pixel_geom) {
+    if (!mma_ok) return 0;
+    const double du1 = pixel_geom[1], du2 = pixel_geom[2];
+    const double du2z = pixel_geom[4], du1z = pixel_geom[5];
+    const double phi = 2.0 * PI * 16.0 * ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
+    const double qmax = 2.0 * PI * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
+    if (qmax > 3e-4 || qmax * phi * phi * phi * phi / 24.0 > 3e-7) return 0;
+    auto trunc = [&](int D) {
+        double t = 1.0;
+        for (int i = 1; i <= D + 1; ++i) t *= phi / i;
+        return t;
+    };
+    if (trunc(5) <= 3e-7) return 5;
+    if (trunc(6) <= 3e-7) return 6;
+    return 0;
+}
+
 extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* counts,
                               const mosaic_species_t* species_tab, const mosaic_kin_opts_t* opts,
                               int n_frames, const float* rot, const float* trans, int Ny, int Nz,
@@ -2522,26 +2834,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
                         K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
                         2.0 * radius_wl + 2.0 <= 256.0;
-    // moment sum: the matrix-core sum's model bounds, plus the truncation of
-    // the moment expansion.  phi bounds |P x + R y| over the patch from the
-    // largest first differences along a row and a column (16 pixels each
-    // way, with the curvature), |Q|, |S| the quadratic model terms.
-    const double du1z = pixel_geom[5], du2z = pixel_geom[4];
-    const double phi = 2.0 * 3.14159265358979 * 16.0 *
-                       ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
-    const double qmax = 2.0 * 3.14159265358979 * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
-    auto trunc = [&](int D) {
-        double t = 1.0;
-        for (int i = 1; i <= D + 1; ++i) t *= phi / i;
-        return t;
-    };
-    // (both truncations are per-atom bounds at the patch corner for the
-    // largest tile; each within 3e-7 of the atom's term)
-    int poly_d = 0;
-    if (mma_ok && qmax <= 3e-4 && qmax * phi * phi * phi * phi / 24.0 <= 3e-7) {
-        if (trunc(5) <= 3e-7) poly_d = 5;
-        else if (trunc(6) <= 3e-7) poly_d = 6;
-    }
+    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom);
     int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
         return MOSAIC_ERR_UNSUPPORTED;
@@ -2679,7 +2972,7 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
     if (!rot || !trans || !stage_stats || !pixel_geom || !grad_field || !grad_pos)
         return MOSAIC_ERR_ARGS;
-    if (path != 0 && path != 2 && path != 3) return MOSAIC_ERR_ARGS;
+    if (path != 0 && path != 2 && path != 3 && path != 4) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
     rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
@@ -2717,13 +3010,17 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     int kp = 0;
     if (ok && rec_curv_bound(16, du2, radius_wl) <= 1e-6) kp = 16;
     else if (ok && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
-    const bool use_mma = path == 3 || (path == 0 && mma_ok);
-    if ((use_mma && !mma_ok) || (!use_mma && kp == 0)) return MOSAIC_ERR_UNSUPPORTED;
+    // the moment form under the forward's truncation bounds
+    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom);
+    const bool use_poly = path == 4 || (path == 0 && poly_d > 0);
+    const bool use_mma = !use_poly && (path == 3 || (path == 0 && mma_ok));
+    if ((use_poly && poly_d == 0) || (use_mma && !mma_ok) || (!use_poly && !use_mma && kp == 0))
+        return MOSAIC_ERR_UNSUPPORTED;
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
     const int64_t n_gin0 = 2 * static_cast<int64_t>(n_frames) * Ny * Nz;
     const int64_t n_gtab0 = 2 * static_cast<int64_t>(n_species) * Ny * Nz;
-    if (use_mma) {
+    if (use_mma || use_poly) {
         AdjMmaArgs M;
         M.atoms = L.atoms;
         M.tiles = L.tiles;
@@ -2748,7 +3045,17 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
         // hz ~ du2z / 2 and |b| <= MMA_P / 2, may be left out below 1e-6
         const bool flat = 0.125 * MMA_P * MMA_P * pixel_geom[4] <= 1e-6;
         const dim3 grid(L.n_tiles), blk(64 * MMA_WAVES);
-        if (use_second) {
+        const dim3 pblk(64 * POLY_WAVES);
+        if (use_poly) {
+            if (poly_d == 5 && use_second)
+                hipLaunchKernelGGL((k_adjoint_poly<5, true>), grid, pblk, 0, st, M);
+            else if (poly_d == 5)
+                hipLaunchKernelGGL((k_adjoint_poly<5, false>), grid, pblk, 0, st, M);
+            else if (use_second)
+                hipLaunchKernelGGL((k_adjoint_poly<6, true>), grid, pblk, 0, st, M);
+            else
+                hipLaunchKernelGGL((k_adjoint_poly<6, false>), grid, pblk, 0, st, M);
+        } else if (use_second) {
             if (flat) hipLaunchKernelGGL((k_adjoint_mma<true, 2>), grid, blk, 0, st, M);
             else hipLaunchKernelGGL((k_adjoint_mma<true, 3>), grid, blk, 0, st, M);
         } else {

@@ -231,8 +231,8 @@ def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream
 
     ``grad_field`` is dL/dRe(A) + i dL/dIm(A) for the field the last
     ``kin_sum`` on ``staged`` produced, (F, Nz, Ny) complex (cast to
-    complex64).  ``path`` is "auto", "rows" or "mma"; raises MosaicDeclined
-    outside the chosen form's bounds.
+    complex64).  ``path`` is "auto", "rows", "mma" or "poly" (moment
+    expansion); raises MosaicDeclined outside the chosen form's bounds.
     """
     F = staged.n_frames
     if staged.pixel_geom is None:

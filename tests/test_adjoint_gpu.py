@@ -44,7 +44,7 @@ def make_problem(cells=(8, 8, 8), npix=32, n_angles=3, seed=0, adjoint_path="aut
 # "mma-curved": a detector at 0.375 m, where the column curvature of the
 # pixel direction over a 32-pixel patch (|hz| b^2 ~ 4e-6) is kept in the
 # matrix-core adjoint's gradient direction
-@pytest.mark.parametrize("path", ["rows", "mma", "mma-curved"])
+@pytest.mark.parametrize("path", ["rows", "mma", "mma-curved", "poly"])
 def test_adjoint_vs_fp64_reference_and_fd(device, path):
     curved = path == "mma-curved"
     w, m, pos, u, u_t, rots, trs = make_problem(
@@ -106,7 +106,7 @@ def test_adjoint_vs_fp64_reference_and_fd(device, path):
     assert np.all(np.isfinite(grad))
 
 
-@pytest.mark.parametrize("path", ["rows", "mma"])
+@pytest.mark.parametrize("path", ["rows", "mma", "poly"])
 def test_adjoint_gradient_descends(device, path):
     """A few gradient steps on u reduce the loss towards the target field."""
     w, m, pos, u, u_t, rots, trs = make_problem(cells=(6, 6, 6), npix=24, adjoint_path=path)
