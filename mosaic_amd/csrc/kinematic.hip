@@ -213,11 +213,15 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.keys_b = c.take<uint64_t>(n);
     L.vals_a = c.take<uint32_t>(n);
     L.vals_b = c.take<uint32_t>(n);
-    size_t sort_bytes = 0;
+    size_t sort_bytes = 0, sort_bytes32 = 0;
     if (rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                   (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  static_cast<unsigned int>(n), 0, 64) != hipSuccess)
+                                  static_cast<unsigned int>(n), 0, 64) != hipSuccess ||
+        rocprim::radix_sort_pairs(nullptr, sort_bytes32, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  static_cast<unsigned int>(n), 0, 32) != hipSuccess)
         return MOSAIC_ERR_HIP;
+    sort_bytes = std::max(sort_bytes, sort_bytes32);
     L.sort_bytes = sort_bytes;
     L.sort_tmp = c.take<char>(sort_bytes);
     L.seg = c.take<uint32_t>(n);
@@ -517,7 +521,8 @@ __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __r
 // of the floats), a key (node, coordinate along the node's longest side), one
 // stable radix sort over all nodes at once, and the nodes of the next level.
 
-constexpr int KD_QBITS = 20;       // coordinate bits of a k-d sort key
+constexpr int KD_QBITS = 20;       // coordinate bits of a 64-bit k-d sort key
+constexpr int KD_QBITS32 = 12;     // and of a 32-bit one (node bits + 12 <= 32)
 
 __device__ __forceinline__ unsigned int f2ord(float f) {
     const unsigned int b = __float_as_uint(f);
@@ -540,8 +545,9 @@ __global__ void k_stage_init(unsigned int* __restrict__ radius_bits) {
 }
 
 // Group by species (stable): key = species code.
+template <typename KeyT>
 __global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
-                               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                               KeyT* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     keys[i] = spc[i] & 0xF;
@@ -653,10 +659,11 @@ __global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
 
 // Key (node, coordinate along the node's longest side); nodes that do not
 // split keep their order (coordinate 0, stable sort).
+template <typename KeyT, int QB>
 __global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restrict__ order,
                           const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split,
                           const unsigned int* __restrict__ box, int64_t n,
-                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                          KeyT* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t s = seg[i];
@@ -674,10 +681,10 @@ __global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restr
         if (ext[2] > ext[ax]) ax = 2;
         if (ext[ax] > 0.f) {
             const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) / ext[ax];
-            q = static_cast<uint32_t>(fminf(fmaxf(t, 0.f), 1.f) * static_cast<float>((1u << KD_QBITS) - 1u));
+            q = static_cast<uint32_t>(fminf(fmaxf(t, 0.f), 1.f) * static_cast<float>((1u << QB) - 1u));
         }
     }
-    keys[i] = (static_cast<uint64_t>(s) << KD_QBITS) | q;
+    keys[i] = (static_cast<KeyT>(s) << QB) | q;
     vals[i] = a;
 }
 
@@ -2594,21 +2601,33 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     const unsigned int n32 = static_cast<unsigned int>(atom_count);
     const int nb = blocks_for(atom_count, KD_THREADS);
     hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
+    int sbits = 0;
+    while ((1 << sbits) < n_species) ++sbits;
+    // 32-bit sort keys (node, 12-bit coordinate) while the node ids fit
+    const bool k32 = KD_QBITS32 + sbits + L.kd_levels <= 32;
+    uint32_t* keys32_a = reinterpret_cast<uint32_t*>(L.keys_a);
+    uint32_t* keys32_b = reinterpret_cast<uint32_t*>(L.keys_b);
     // group by species (stable), then bisect each species level by level
-    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
-                       L.keys_a, L.vals_a);
-    MOSAIC_HIP_CHECK(hipGetLastError());
     size_t sb = L.sort_bytes;
-    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
-                                               L.vals_b, n32, 0, 4, st));
+    if (k32) {
+        hipLaunchKernelGGL(k_species_keys<uint32_t>, dim3(nb), dim3(KD_THREADS), 0, st, species,
+                           atom_count, keys32_a, L.vals_a);
+        MOSAIC_HIP_CHECK(hipGetLastError());
+        MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, keys32_a, keys32_b, L.vals_a,
+                                                   L.vals_b, n32, 0, 4, st));
+    } else {
+        hipLaunchKernelGGL(k_species_keys<uint64_t>, dim3(nb), dim3(KD_THREADS), 0, st, species,
+                           atom_count, L.keys_a, L.vals_a);
+        MOSAIC_HIP_CHECK(hipGetLastError());
+        MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
+                                                   L.vals_b, n32, 0, 4, st));
+    }
     std::vector<uint32_t> plan;       // alive until the synchronisation below
     if (L.kd_levels > 0) {
         kd_plan(n_species, counts, L.kd_levels, plan);
         MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_split, plan.data(), plan.size() * sizeof(uint32_t),
                                         hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_kd_root, dim3(nb), dim3(KD_THREADS), 0, st, atom_count, split, L.seg);
-        int sbits = 0;
-        while ((1 << sbits) < n_species) ++sbits;
         for (int l = 0; l < L.kd_levels; ++l) {
             const int64_t nodes = static_cast<int64_t>(n_species) << l;
             const uint32_t* cut = L.kd_split + kd_split_count(n_species, l);
@@ -2617,13 +2636,24 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
             hipLaunchKernelGGL(k_kd_box, dim3(blocks_for(atom_count, KD_THREADS * KD_CHUNKS)),
                                dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg, cut, atom_count,
                                L.kd_box);
-            hipLaunchKernelGGL(k_kd_keys, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b,
-                               L.seg, cut, L.kd_box, atom_count, L.keys_a, L.vals_a);
-            MOSAIC_HIP_CHECK(hipGetLastError());
             sb = L.sort_bytes;
-            MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b,
-                                                       L.vals_a, L.vals_b, n32, 0,
-                                                       KD_QBITS + l + sbits, st));
+            if (k32) {
+                hipLaunchKernelGGL((k_kd_keys<uint32_t, KD_QBITS32>), dim3(nb), dim3(KD_THREADS), 0,
+                                   st, pos, L.vals_b, L.seg, cut, L.kd_box, atom_count, keys32_a,
+                                   L.vals_a);
+                MOSAIC_HIP_CHECK(hipGetLastError());
+                MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, keys32_a, keys32_b,
+                                                           L.vals_a, L.vals_b, n32, 0,
+                                                           KD_QBITS32 + l + sbits, st));
+            } else {
+                hipLaunchKernelGGL((k_kd_keys<uint64_t, KD_QBITS>), dim3(nb), dim3(KD_THREADS), 0,
+                                   st, pos, L.vals_b, L.seg, cut, L.kd_box, atom_count, L.keys_a,
+                                   L.vals_a);
+                MOSAIC_HIP_CHECK(hipGetLastError());
+                MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b,
+                                                           L.vals_a, L.vals_b, n32, 0,
+                                                           KD_QBITS + l + sbits, st));
+            }
             if (l + 1 < L.kd_levels)
                 hipLaunchKernelGGL(k_kd_next, dim3(nb), dim3(KD_THREADS), 0, st, cut, atom_count,
                                    L.seg);
