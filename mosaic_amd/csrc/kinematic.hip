@@ -1396,7 +1396,8 @@ hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t 
 // Each lane accumulates the moments of 4 of the tile's atoms; the wave sums
 // them through LDS; every lane then evaluates its 16 pixels by Horner's
 // rule and applies the per-(tile, pixel) origin phase as the matrix-core
-// sum does.
+// sum does; the pixel's table factor (a function of species and pixel only)
+// multiplies the running sums when they are flushed.
 constexpr int POLY_WAVES = 4;
 
 template <int D>
@@ -1408,7 +1409,8 @@ struct PolyN {
 };
 
 template <int D, bool SECOND>
-__global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
+__global__ __launch_bounds__(64 * POLY_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
+void k_sum_poly(const MmaArgs A) {
     constexpr int NC = PolyN<D>::NC, NVP = PolyN<D>::NVP;
     constexpr float TWO_PI_F = 6.283185307179586f;
     __shared__ AtomRec tile_buf[2][TILE];
@@ -1476,7 +1478,29 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
     double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
     bool first = true;
     int since = 0;
+    int acc_species = 0;
     float* rw = red[wave];
+    // running sums times the pixel table of their (single) species, into the
+    // FP64 slice partial
+    auto flush = [&]() {
+        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(acc_species) * A.npix;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if (live && arow(r) < P.vy && q < P.vz) {
+                const int p = prow + arow(r);
+                const float2 gg = Gm[p];
+                double2* o = out + p;
+                double2 v = first ? make_double2(0.0, 0.0) : *o;
+                v.x += static_cast<double>(gg.x * acc_re[r] - gg.y * acc_im[r]);
+                v.y += static_cast<double>(gg.x * acc_im[r] + gg.y * acc_re[r]);
+                *o = v;
+            }
+            acc_re[r] = 0.f;
+            acc_im[r] = 0.f;
+        }
+        first = false;
+        since = 0;
+    };
 
     if (t0 < t1) tile_buf[0][tid] = A.atoms[static_cast<int64_t>(t0) * TILE + tid];
     __syncthreads();
@@ -1488,6 +1512,9 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
         const float4 n0 = nsrc[0], n1 = nsrc[1];
         const TileRec tr = A.tiles[t];
         const int tie = tr.count >> 16;
+        // tiles are grouped by species; the sums carry one species' table
+        if (since > 0 && tr.species != acc_species) flush();
+        acc_species = tr.species;
         const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
         const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
         const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
@@ -1596,7 +1623,6 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
                 base += D - m + 1;
             }
         }
-        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             if ((r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
@@ -1607,33 +1633,16 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_sum_poly(const MmaArgs A) {
                 sr = fmaf(sr, x, Gr[m]);
                 si = fmaf(si, x, Gi[m]);
             }
+            // origin phase in revolutions, |phs| < 2: within v_sin's own range
             const int p = prow + min(arow(r), P.vy - 1) + tie;
             const float4 du = A.dup[p];
-            const float phs = __builtin_amdgcn_fractf(
-                pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+            const float phs = pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x));
             const float sn = __builtin_amdgcn_sinf(phs);
             const float cs = __builtin_amdgcn_cosf(phs);
-            const float2 gg = Gm[p];
-            const float zr = gg.x * cs - gg.y * sn, zi = gg.x * sn + gg.y * cs;
-            acc_re[r] = fmaf(zr, sr, fmaf(-zi, si, acc_re[r]));
-            acc_im[r] = fmaf(zr, si, fmaf(zi, sr, acc_im[r]));
+            acc_re[r] = fmaf(cs, sr, fmaf(-sn, si, acc_re[r]));
+            acc_im[r] = fmaf(cs, si, fmaf(sn, sr, acc_im[r]));
         }
-        if (++since == MMA_FLUSH || t + 1 == t1) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                if (live && arow(r) < P.vy && q < P.vz) {
-                    double2* o = out + prow + arow(r);
-                    double2 v = first ? make_double2(0.0, 0.0) : *o;
-                    v.x += static_cast<double>(acc_re[r]);
-                    v.y += static_cast<double>(acc_im[r]);
-                    *o = v;
-                }
-                acc_re[r] = 0.f;
-                acc_im[r] = 0.f;
-            }
-            first = false;
-            since = 0;
-        }
+        if (++since == MMA_FLUSH || t + 1 == t1) flush();
         if (has_next) {
             float4* dst = reinterpret_cast<float4*>(&tile_buf[buf ^ 1][tid]);
             dst[0] = n0;
@@ -2778,22 +2787,27 @@ double rec_curv_bound(int kp, double d2u, double radius_wl) {
 // phi bounds |P x + R y| over a patch from the largest first differences
 // along a row and a column (16 pixels each way, with the curvature); qmax
 // bounds the quadratic model terms |Q|, |S|.  Both truncations are per-atom
-// bounds at the patch corner for the largest tile, each within 3e-7 of the
-// atom's term.
-static int poly_degree(bool mma_ok, double radius_wl, const double* pixel_geom) {
+// bounds at the patch corner for the largest tile, each within 1e-6 of the
+// atom's term (the FP32 offset phase already costs ~3e-6 of the field).
+// The moment sum also takes the per-(tile, pixel) origin phase,
+// K |O| |u_p - u_mid| < K omax 16 sqrt(2) (du_row + du_col) revolutions,
+// into v_sin / v_cos without reduction, so that must stay within their range.
+static int poly_degree(bool mma_ok, double radius_wl, const double* pixel_geom, double k_omax) {
     if (!mma_ok) return 0;
     const double du1 = pixel_geom[1], du2 = pixel_geom[2];
     const double du2z = pixel_geom[4], du1z = pixel_geom[5];
+    if (k_omax * 16.0 * 1.4142135623730951 * (du1 + du1z + 16.0 * (du2 + du2z)) + 1.0 > 200.0)
+        return 0;
     const double phi = 2.0 * PI * 16.0 * ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
     const double qmax = 2.0 * PI * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
-    if (qmax > 3e-4 || qmax * phi * phi * phi * phi / 24.0 > 3e-7) return 0;
+    if (qmax > 3e-4 || qmax * phi * phi * phi * phi / 24.0 > 1e-6) return 0;
     auto trunc = [&](int D) {
         double t = 1.0;
         for (int i = 1; i <= D + 1; ++i) t *= phi / i;
         return t;
     };
-    if (trunc(5) <= 3e-7) return 5;
-    if (trunc(6) <= 3e-7) return 6;
+    if (trunc(5) <= 1e-6) return 5;
+    if (trunc(6) <= 1e-6) return 6;
     return 0;
 }
 
@@ -2864,7 +2878,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
                         K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
                         2.0 * radius_wl + 2.0 <= 256.0;
-    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom);
+    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom, K * omax);
     int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
         return MOSAIC_ERR_UNSUPPORTED;
@@ -3041,7 +3055,7 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     if (ok && rec_curv_bound(16, du2, radius_wl) <= 1e-6) kp = 16;
     else if (ok && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;
     // the moment form under the forward's truncation bounds
-    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom);
+    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom, Kk * omax);
     const bool use_poly = path == 4 || (path == 0 && poly_d > 0);
     const bool use_mma = !use_poly && (path == 3 || (path == 0 && mma_ok));
     if ((use_poly && poly_d == 0) || (use_mma && !mma_ok) || (!use_poly && !use_mma && kp == 0))
