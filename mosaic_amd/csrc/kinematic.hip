@@ -2428,8 +2428,13 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
                     const int k0 = mk(m, n), k1 = mk(m + 1, n), k2 = mk(m + 2, n),
                               k3 = mk(m, n + 1), k4 = mk(m, n + 2);
                     const int ks[5] = {k0, k1, k2, k3, k4};
+                    // the gradient-direction terms scale as |Gy|, |Gz| ~ 1e-2
+                    // and |Hy|, |Hz| ~ 1e-5 of |vc|: their expansions stop at
+                    // degree 4 and 2 (below 1e-7 of Z)
+                    const int deg[5] = {D, 4, 2, 4, 2};
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
+                        if (m + n > deg[c]) continue;
                         sr[c] = fmaf(cr_, Mr[ks[c]], fmaf(-ci_, Mi[ks[c]], sr[c]));
                         si[c] = fmaf(cr_, Mi[ks[c]], fmaf(ci_, Mr[ks[c]], si[c]));
                     }
