@@ -36,11 +36,15 @@ MFMA_FLOP_PER_PAIR = 24.0
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (the copy-kernel calibration
 # reads FETCH_SIZE x 2 and WRITE_SIZE x 1; profiles/k_sum_mma_perovskite5M_512px.md).
 MMA_TRAFFIC_BYTES = (2 * 1831133.0 + 2542443.8) * 1024
-# k_sum_poly<6> on the same workload: FETCH_SIZE 1.797e6 KiB (x2, the copy
-# calibration), WRITE_SIZE 2.632e6 KiB; 7.45e9 VALU wave-instructions per launch
-# for 1.31e12 pairs (SQ_INSTS_VALU), i.e. 0.36 lane-operations per pair
-POLY_TRAFFIC_BYTES = (2 * 1797000.0 + 2632400.0) * 1024
-POLY_VALU_LANE_OPS_PER_PAIR = 7.4467e9 * 64 / (5.0e6 * 512 * 512)
+# k_sum_poly<6> on the same workload: FETCH_SIZE 1.835e6 KiB (x2, the copy
+# calibration), WRITE_SIZE 2.688e6 KiB; 6.97e9 VALU wave-instructions per launch
+# for 1.31e12 pairs (SQ_INSTS_VALU), i.e. 0.34 lane-operations per pair
+# (profiles/k_sum_poly_perovskite5M_512px.md)
+POLY_TRAFFIC_BYTES = (2 * 1835000.0 + 2688200.0) * 1024
+POLY_VALU_LANE_OPS_PER_PAIR = 6.9685e9 * 64 / (5.0e6 * 512 * 512)
+# VALU issue peak in lane-operations per second (an FMA counts once): the FP32
+# peak halved
+PEAK_VALU_LANE_OPS = PEAK_FP32 / 2
 
 
 def parse():
@@ -273,6 +277,8 @@ def roofline(path_used, pairs, ksum_ms, workload):
         # expansion executes far fewer operations per pair, so frac exceeds 1
         r.update({"bound": "valu", "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32,
                   "executed_valu_lane_ops_per_pair": POLY_VALU_LANE_OPS_PER_PAIR,
+                  "executed_valu_frac": pairs * POLY_VALU_LANE_OPS_PER_PAIR / t
+                  / PEAK_VALU_LANE_OPS,
                   "traffic": POLY_TRAFFIC_BYTES if workload == "perovskite_screw_5M_512px"
                   else None})
     else:

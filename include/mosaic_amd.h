@@ -167,12 +167,14 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
  * mosaic_kin_sum produced from the same workspace, adds dL/du (device (N, 3)
  * float32, per Angstrom, original atom order) into grad_pos.  Phases are
  * recomputed, not stored.  No counterpart in the reference, which has no
- * gradient path; it serves displacement-field fitting.  Two forms, chosen
- * like the forward's (`path` 0 automatic, 2 row recurrence, 3 matrix-core):
- * the matrix-core form contracts per-patch weight matrices with the atoms'
- * separable phasors on the FP16-split MFMA; the row form walks row blocks
- * on the VALU.  Declines (MOSAIC_ERR_UNSUPPORTED) outside the far-field
- * bounds of the chosen form.
+ * gradient path; it serves displacement-field fitting.  Three forms, chosen
+ * like the forward's (`path` 0 automatic, 2 row recurrence, 3 matrix-core,
+ * 4 moment expansion): the moment form contracts each atom's expansion
+ * coefficients with per-patch pixel moments of the weights; the matrix-core
+ * form contracts per-patch weight matrices with the atoms' separable
+ * phasors on the FP16-split MFMA; the row form walks row blocks on the VALU.
+ * Declines (MOSAIC_ERR_UNSUPPORTED) outside the far-field bounds of the
+ * chosen form.
  */
 int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
