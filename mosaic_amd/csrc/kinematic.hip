@@ -50,6 +50,13 @@ constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
 constexpr int N_GEOM = 6;            // pixel geometry statistics (mosaic_kin_pixels)
+// The last k-d levels run inside one workgroup per node: a node of at most
+// KDL_MAX atoms sits in LDS and is bisected KDL_LEVELS times there.
+constexpr int KDL_LEVELS = 5;
+constexpr int KDL_MAX = TILE << KDL_LEVELS;          // 8192 atoms per node
+constexpr int KDL_THREADS = 1024;
+constexpr int KDL_PER_THREAD = KDL_MAX / KDL_THREADS;
+constexpr int KDL_SUB = (1 << KDL_LEVELS) - 1;       // sub-nodes over the local levels
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
 
@@ -64,6 +71,13 @@ struct TileRec {      // 32 B
     double ox, oy, oz;   // sample-frame origin [m]
     int species;
     int count;
+};
+
+struct KdNode {       // a k-d node finished in LDS: positions [begin, begin + count)
+    uint32_t begin;
+    uint32_t count;
+    uint32_t tile0;   // index of its first tile
+    uint32_t species;
 };
 
 struct FrameRec {     // 128 B
@@ -112,8 +126,10 @@ struct Layout {
     void* sort_tmp;
     size_t sort_bytes;
     uint32_t* seg;          // k-d node of every position
-    unsigned int* kd_box;   // [node][6] bounding boxes of one level
+    unsigned int* kd_box;   // [node][6] bounding boxes of level 0 (order-preserving integers)
+    float* kd_fbox[2];      // [node][6] float boxes (lo, hi) of two successive levels
     uint32_t* kd_split;     // cut positions of every level
+    KdNode* kd_nodes;       // nodes handed to the in-LDS finish
     int kd_levels;
     unsigned int* radius_bits;
     float* amax_blocks;     // adjoint: per-block max |dL/dA| and max |G|
@@ -168,8 +184,10 @@ size_t kd_split_count(int n_species, int levels) {
 // Cut positions of the k-d bisection, level by level, in heap numbering
 // (node h of a level has children 2h and 2h + 1 on the next): a node of n > T
 // atoms keeps ceil(n / 2T) T on the left; NO_SPLIT where a node holds one
-// tile or less (it passes whole to its left child).
-void kd_plan(int n_species, const int64_t* counts, int levels, std::vector<uint32_t>& split) {
+// tile or less (it passes whole to its left child).  The position ranges of
+// the nodes below the last level are left in [begin, end).
+void kd_plan(int n_species, const int64_t* counts, int levels, std::vector<uint32_t>& split,
+             std::vector<int64_t>& begin, std::vector<int64_t>& end) {
     std::vector<int64_t> b(n_species), e(n_species);
     int64_t a = 0;
     for (int s = 0; s < n_species; ++s) {
@@ -199,6 +217,8 @@ void kd_plan(int n_species, const int64_t* counts, int levels, std::vector<uint3
         b.swap(nb);
         e.swap(ne);
     }
+    begin.swap(b);
+    end.swap(e);
 }
 
 int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
@@ -228,7 +248,10 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.kd_levels = kd_levels(n_species, counts);
     const size_t top = static_cast<size_t>(n_species) << std::max(0, L.kd_levels - 1);
     L.kd_box = c.take<unsigned int>(6 * top);
+    L.kd_fbox[0] = c.take<float>(12 * top);
+    L.kd_fbox[1] = c.take<float>(12 * top);
     L.kd_split = c.take<uint32_t>(kd_split_count(n_species, L.kd_levels) + 1);
+    L.kd_nodes = c.take<KdNode>(std::max(1, L.n_tiles));
     L.radius_bits = c.take<unsigned int>(2);
     L.amax_blocks = c.take<float>(2 * AMAX_BLOCKS);
     L.frames = c.take<FrameRec>(std::max(1, n_frames));
@@ -522,7 +545,24 @@ __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __r
 // stable radix sort over all nodes at once, and the nodes of the next level.
 
 constexpr int KD_QBITS = 20;       // coordinate bits of a 64-bit k-d sort key
-constexpr int KD_QBITS32 = 12;     // and of a 32-bit one (node bits + 12 <= 32)
+constexpr int KD_QBITS32_MIN = 6;  // least coordinate bits of a 32-bit one
+// Global levels whose boxes are measured over the atoms: the first and the
+// last KD_TIGHT_LEVELS; the others cut their parents' boxes at the sorted
+// keys.  Loose boxes near the leaves let an elongated node through (on the
+// 5M-atom perovskite the largest tile grows from 21.7 to 28 wavelengths),
+// tight ones there keep it at 21.8.
+constexpr int KD_TIGHT_LEVELS = 2;
+
+// Coordinate bits of a 32-bit key whose node id has nbits bits: the key is
+// filled up to the next multiple of the sort's 8-bit digit, keeping at least
+// KD_QBITS32_MIN bits.  The count of every cut is exact whatever the bits;
+// they only set how far (one bin of the node's longest side) the two halves
+// may overlap.
+int kd_qbits32(int nbits) {
+    int total = 8;
+    while (total - nbits < KD_QBITS32_MIN) total += 8;
+    return total - nbits;
+}
 
 __device__ __forceinline__ unsigned int f2ord(float f) {
     const unsigned int b = __float_as_uint(f);
@@ -594,9 +634,15 @@ __device__ __forceinline__ void kd_box_flush(uint32_t node, unsigned int (&lo)[3
 
 __global__ __launch_bounds__(KD_THREADS) void k_kd_box(
     const float* __restrict__ pos, const uint32_t* __restrict__ order,
-    const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split, int64_t n,
-    unsigned int* __restrict__ box) {
+    const uint32_t* __restrict__ seg, const uint32_t* __restrict__ prev_split,
+    const uint32_t* __restrict__ split, int64_t n, unsigned int* __restrict__ box) {
     __shared__ unsigned int red[KD_THREADS / 64][6];
+    // node of position j: seg holds the previous level's ids when prev_split
+    // is given (children 2s left of the previous cut, 2s + 1 right of it)
+    auto node_at = [&](int64_t j) {
+        const uint32_t s = seg[j];
+        return prev_split ? 2u * s + (static_cast<uint64_t>(j) >= prev_split[s] ? 1u : 0u) : s;
+    };
     const int lane = threadIdx.x & 63;
     const int64_t base = blockIdx.x * static_cast<int64_t>(KD_THREADS) * KD_CHUNKS;
     uint32_t cur = NO_SPLIT;                      // node of the running box (block-uniform)
@@ -606,7 +652,7 @@ __global__ __launch_bounds__(KD_THREADS) void k_kd_box(
         if (i0 >= n) break;
         const int64_t i = i0 + threadIdx.x;
         const bool in = i < n;
-        const uint32_t s = in ? seg[i] : NO_SPLIT;
+        const uint32_t s = in ? node_at(i) : NO_SPLIT;
         const bool active = in && split[s] != NO_SPLIT;
         unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
         if (active) {
@@ -614,7 +660,7 @@ __global__ __launch_bounds__(KD_THREADS) void k_kd_box(
             for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
         }
         const int64_t i_last = min(i0 + KD_THREADS, n) - 1;
-        const uint32_t sb0 = seg[i0], sb1 = seg[i_last];
+        const uint32_t sb0 = node_at(i0), sb1 = node_at(i_last);
         if (sb0 == sb1 && i0 + KD_THREADS <= n) {
             if (sb0 != cur) {
                 if (cur != NO_SPLIT) kd_box_flush(cur, rlo, rhi, red, split, box);
@@ -657,28 +703,40 @@ __global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
     if (i < words) box[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
 }
 
-// Key (node, coordinate along the node's longest side); nodes that do not
-// split keep their order (coordinate 0, stable sort).
-template <typename KeyT, int QB>
+// Longest side of a float box (lo[3], hi[3]); ties go to the lower axis.
+__device__ __forceinline__ int kd_axis(const float* b, float (&lo)[3], float (&ext)[3]) {
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = b[d];
+        ext[d] = b[3 + d] - lo[d];
+    }
+    int ax = 0;
+    if (ext[1] > ext[ax]) ax = 1;
+    if (ext[2] > ext[ax]) ax = 2;
+    return ax;
+}
+
+// Key (node, coordinate along the node's longest side in QB bits); nodes
+// that do not split keep their order (coordinate 0, stable sort).  With
+// prev_split the node ids first step down from the previous level: children
+// 2s (left of the previous cut) and 2s + 1.
+template <typename KeyT>
 __global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restrict__ order,
-                          const uint32_t* __restrict__ seg, const uint32_t* __restrict__ split,
-                          const unsigned int* __restrict__ box, int64_t n,
-                          KeyT* __restrict__ keys, uint32_t* __restrict__ vals) {
+                          uint32_t* __restrict__ seg, const uint32_t* __restrict__ prev_split,
+                          const uint32_t* __restrict__ split, const float* __restrict__ fbox,
+                          int64_t n, int QB, KeyT* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = seg[i];
+    uint32_t s = seg[i];
+    if (prev_split) {
+        s = 2u * s + (static_cast<uint64_t>(i) >= prev_split[s] ? 1u : 0u);
+        seg[i] = s;
+    }
     const uint32_t a = order[i];
     uint32_t q = 0;
     if (split[s] != NO_SPLIT) {
-        const unsigned int* b = box + 6 * static_cast<int64_t>(s);
         float lo[3], ext[3];
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = ord2f(b[d]);
-            ext[d] = ord2f(b[3 + d]) - lo[d];
-        }
-        int ax = 0;
-        if (ext[1] > ext[ax]) ax = 1;
-        if (ext[2] > ext[ax]) ax = 2;
+        const int ax = kd_axis(fbox + 6 * static_cast<int64_t>(s), lo, ext);
         if (ext[ax] > 0.f) {
             const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) / ext[ax];
             q = static_cast<uint32_t>(fminf(fmaxf(t, 0.f), 1.f) * static_cast<float>((1u << QB) - 1u));
@@ -688,13 +746,39 @@ __global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restr
     vals[i] = a;
 }
 
-// Node ids of the next level: children 2s (left of the cut) and 2s + 1.
-__global__ void k_kd_next(const uint32_t* __restrict__ split, int64_t n,
-                          uint32_t* __restrict__ seg) {
+// Level-0 float boxes from the order-preserving integer boxes.
+__global__ void k_kd_box_float(const unsigned int* __restrict__ box, int64_t words,
+                               float* __restrict__ fbox) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = seg[i];
-    seg[i] = 2u * s + (static_cast<uint64_t>(i) >= split[s] ? 1u : 0u);
+    if (i < words) fbox[i] = ord2f(box[i]);
+}
+
+// Boxes of the next level without a pass over the atoms: a splitting node's
+// children take its box cut along its longest side at the bin of the first
+// key right of the cut (left of it every coordinate bin is at most that
+// bin, right of it at least), a node that does not split hands its box to
+// its left child.  Atoms of a child may stand out of its box by rounding;
+// their keys clamp, which only loosens the next split.
+template <typename KeyT>
+__global__ void k_kd_child_boxes(const float* __restrict__ fbox, const uint32_t* __restrict__ split,
+                                 const KeyT* __restrict__ sorted_keys, int64_t nodes, int QB,
+                                 float* __restrict__ fbox_next) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nodes) return;
+    float b[6];
+    for (int d = 0; d < 6; ++d) b[d] = fbox[6 * s + d];
+    float* left = fbox_next + 12 * s;
+    float* right = left + 6;
+    for (int d = 0; d < 6; ++d) left[d] = right[d] = b[d];
+    const uint32_t c = split[s];
+    if (c == NO_SPLIT) return;
+    float lo[3], ext[3];
+    const int ax = kd_axis(b, lo, ext);
+    if (!(ext[ax] > 0.f)) return;
+    const uint32_t qc = static_cast<uint32_t>(sorted_keys[c] & ((static_cast<KeyT>(1) << QB) - 1));
+    const float qs = ext[ax] / static_cast<float>((1u << QB) - 1u);
+    left[3 + ax] = fminf(b[3 + ax], lo[ax] + static_cast<float>(qc + 1u) * qs);
+    right[ax] = fmaxf(b[ax], lo[ax] + static_cast<float>(qc) * qs);
 }
 
 struct SpeciesSplit {
@@ -710,6 +794,30 @@ __global__ void k_kd_root(int64_t n, SpeciesSplit split, uint32_t* __restrict__ 
     uint32_t s = 0;
     while (static_cast<int>(s) + 1 < split.n_species && i >= split.atom_start[s + 1]) ++s;
     seg[i] = s;
+}
+
+// Record of one atom of a tile: FP32 offset from the tile origin o [m] in
+// wavelengths, amplitude as magnitude and phase in turns.  r is the position
+// in metres (the float position in angstrom over 1e10).
+__device__ __forceinline__ AtomRec atom_record(const double (&r)[3], const double (&o)[3],
+                                               double K, uint32_t idx,
+                                               const float2* __restrict__ amp) {
+    AtomRec rec;
+    rec.sx = static_cast<float>((r[0] - o[0]) * K);
+    rec.sy = static_cast<float>((r[1] - o[1]) * K);
+    rec.sz = static_cast<float>((r[2] - o[2]) * K);
+    rec.s2 = rec.sx * rec.sx + rec.sy * rec.sy + rec.sz * rec.sz;
+    rec.pad0 = __uint_as_float(idx);     // original atom index (adjoint scatter)
+    if (amp) {
+        const float2 w = amp[idx];
+        rec.mag = sqrtf(w.x * w.x + w.y * w.y);
+        rec.ph = atan2f(w.y, w.x) * 0.15915494309189535f;
+    } else {
+        rec.mag = 1.f;
+        rec.ph = 0.f;
+    }
+    rec.rmag = sqrtf(rec.mag);
+    return rec;
 }
 
 // One workgroup per tile: FP64 origin at the tile's bounding-box centre, FP32
@@ -741,19 +849,7 @@ __global__ __launch_bounds__(TILE) void k_tiles(
     AtomRec rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float rad = 0.f;
     if (live) {
-        rec.sx = static_cast<float>((r[0] - o[0]) * K);
-        rec.sy = static_cast<float>((r[1] - o[1]) * K);
-        rec.sz = static_cast<float>((r[2] - o[2]) * K);
-        rec.s2 = rec.sx * rec.sx + rec.sy * rec.sy + rec.sz * rec.sz;
-        rec.pad0 = __uint_as_float(idx);     // original atom index (adjoint scatter)
-        if (amp) {
-            const float2 w = amp[idx];
-            rec.mag = sqrtf(w.x * w.x + w.y * w.y);
-            rec.ph = atan2f(w.y, w.x) * 0.15915494309189535f;
-        } else {
-            rec.mag = 1.f;
-        }
-        rec.rmag = sqrtf(rec.mag);
+        rec = atom_record(r, o, K, idx, amp);
         rad = sqrtf(rec.s2);
     }
     atoms[static_cast<int64_t>(t) * TILE + i] = rec;
@@ -769,6 +865,310 @@ __global__ __launch_bounds__(TILE) void k_tiles(
         atomicMax(radius_bits, __float_as_uint(static_cast<float>(rmax)));
         const float on = static_cast<float>(sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
         atomicMax(radius_bits + 1, __float_as_uint(on * 1.0000002f));
+    }
+}
+
+// LDS index of the slot of position p in k_kd_local: one pad word per 8, so
+// that a wave touching the 8 consecutive positions of each lane (lane stride
+// 9 words) meets 64 different banks.
+__device__ __forceinline__ int kidx(int p) { return p + (p >> 3); }
+
+constexpr int KDL_BINS = 256;      // coordinate bins of an in-LDS split
+constexpr int KDL_SUBMAX = 1 << (KDL_LEVELS - 1);   // sub-nodes of the deepest split level
+
+// The last k-d levels and the tile records of one node, in one workgroup.
+// The node's positions [begin, begin + count) of the order (at most KDL_MAX
+// atoms, a whole number of tiles except at a species end) sit in LDS as
+// coordinates by load slot and a slot per position.  Each of KDL_LEVELS
+// levels splits every sub-node of more than one tile by the count rule of the
+// global levels (the left child keeps ceil(n / 2T) T positions): a bounding
+// box, the longest side, a histogram of the coordinate over that side in
+// KDL_BINS bins, the bin holding the cut, and a stable partition (a packed
+// block-wide scan of "below the cut bin" and "in the cut bin" flags) in which
+// the first positions of the cut bin go left until the count is met.  The
+// halves overlap by at most one bin.  Then the node's order is written back
+// and its tiles are emitted as k_tiles emits them.
+__global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
+    const float* __restrict__ pos, const float2* __restrict__ amp, uint32_t* __restrict__ order,
+    const KdNode* __restrict__ nodes, double K, TileRec* __restrict__ tiles,
+    AtomRec* __restrict__ atoms, unsigned int* __restrict__ radius_bits) {
+    __shared__ float xyz[3][KDL_MAX];                 // coordinates by load slot
+    __shared__ uint32_t slot[KDL_MAX + KDL_MAX / 8];  // load slot at kidx(position)
+    __shared__ unsigned int hist[KDL_SUBMAX][KDL_BINS];
+    __shared__ int sub_b[KDL_SUB], sub_n[KDL_SUB];    // sub-node ranges, level-major heap
+    __shared__ int sub_any[KDL_LEVELS];               // some sub-node of the level splits
+    __shared__ unsigned int box[KDL_SUBMAX][6];
+    __shared__ int sel_bin[KDL_SUBMAX], sel_tie[KDL_SUBMAX];
+    __shared__ uint32_t wsum[KDL_THREADS / 64];
+    __shared__ uint32_t spref[KDL_SUBMAX];
+    __shared__ float red[KDL_THREADS / 64][6];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const KdNode nd = nodes[blockIdx.x];
+    const int n = static_cast<int>(nd.count);
+    const int64_t begin = nd.begin;
+
+    {
+        uint32_t a[KDL_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < KDL_PER_THREAD; ++i) {
+            const int p = tid + i * KDL_THREADS;
+            a[i] = p < n ? order[begin + p] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < KDL_PER_THREAD; ++i) {
+            const int p = tid + i * KDL_THREADS;
+            if (p < n)
+                for (int d = 0; d < 3; ++d) xyz[d][p] = pos[3 * static_cast<int64_t>(a[i]) + d];
+            slot[kidx(p)] = static_cast<uint32_t>(p);
+        }
+    }
+    // sub-node (f, j) is heap entry 2^f - 1 + j; its range follows the cuts
+    // along the path from the root
+    if (tid < KDL_SUB) {
+        int f = 0;
+        while ((2 << f) <= tid + 1) ++f;
+        const int j = tid + 1 - (1 << f);
+        int b = 0, m = n;
+        for (int g = 0; g < f; ++g) {
+            const int cut = m > TILE ? (m + 2 * TILE - 1) / (2 * TILE) * TILE : m;
+            if ((j >> (f - 1 - g)) & 1) {
+                b += cut;
+                m -= cut;
+            } else {
+                m = cut;
+            }
+        }
+        sub_b[tid] = b;
+        sub_n[tid] = m;
+    }
+    if (tid < KDL_LEVELS) sub_any[tid] = 0;
+    __syncthreads();
+    if (tid < KDL_SUB && sub_n[tid] > TILE) {
+        int f = 0;
+        while ((2 << f) <= tid + 1) ++f;
+        atomicOr(&sub_any[f], 1);
+    }
+    __syncthreads();
+
+    for (int f = 0; f < KDL_LEVELS; ++f) {
+        if (!sub_any[f]) break;                       // no sub-node splits any more
+        const int base = (1 << f) - 1, nsub = 1 << f;
+        if (tid < 6 * nsub) box[tid / 6][tid % 6] = (tid % 6 < 3) ? 0xFFFFFFFFu : 0u;
+        for (int i = tid; i < nsub * KDL_BINS; i += KDL_THREADS) hist[i / KDL_BINS][i % KDL_BINS] = 0u;
+        __syncthreads();
+        // bounding boxes: a 64-position chunk never straddles two sub-nodes
+        // (their bounds are multiples of TILE or the node end)
+        for (int c0 = wave * 64; c0 < n; c0 += KDL_THREADS) {
+            int j = 0;
+            while (j + 1 < nsub && c0 >= sub_b[base + j + 1]) ++j;
+            if (sub_n[base + j] <= TILE) continue;    // wave-uniform
+            const int p = c0 + lane;
+            unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+            if (p < n) {
+                const uint32_t sl = slot[kidx(p)];
+                for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(xyz[d][sl]);
+            }
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = wave_umin(lo[d]);
+                hi[d] = wave_umax(hi[d]);
+            }
+            if (lane == 0)
+                for (int d = 0; d < 3; ++d) {
+                    atomicMin(&box[j][d], lo[d]);
+                    atomicMax(&box[j][3 + d], hi[d]);
+                }
+        }
+        __syncthreads();
+        // this thread's positions p0 .. p0 + 7 lie in one sub-node j
+        const int p0 = KDL_PER_THREAD * tid;
+        int j = 0;
+        while (j + 1 < nsub && p0 >= sub_b[base + j + 1]) ++j;
+        const int bj = sub_b[base + j], nj = sub_n[base + j];
+        const bool splits = p0 < n && nj > TILE;
+        uint32_t s[KDL_PER_THREAD];
+        int dg[KDL_PER_THREAD];
+#pragma unroll
+        for (int r = 0; r < KDL_PER_THREAD; ++r) {
+            s[r] = slot[kidx(p0 + r)];
+            dg[r] = 0;
+        }
+        if (splits) {
+            float l3[3], e3[3];
+            for (int d = 0; d < 3; ++d) {
+                l3[d] = ord2f(box[j][d]);
+                e3[d] = ord2f(box[j][3 + d]) - l3[d];
+            }
+            int ax = 0;
+            if (e3[1] > e3[ax]) ax = 1;
+            if (e3[2] > e3[ax]) ax = 2;
+            const float sc = e3[ax] > 0.f ? static_cast<float>(KDL_BINS) / e3[ax] : 0.f;
+#pragma unroll
+            for (int r = 0; r < KDL_PER_THREAD; ++r)
+                if (p0 + r < n) {
+                    const float t = (xyz[ax][s[r]] - l3[ax]) * sc;
+                    dg[r] = min(KDL_BINS - 1, max(0, static_cast<int>(t)));
+                    atomicAdd(&hist[j][dg[r]], 1u);
+                }
+        }
+        __syncthreads();
+        // the bin holding the cut, one wave per sub-node
+        if (wave < nsub && sub_n[base + wave] > TILE) {
+            const int mm = sub_n[base + wave];
+            const uint32_t need = static_cast<uint32_t>((mm + 2 * TILE - 1) / (2 * TILE) * TILE);
+            uint32_t c4[4], own = 0;
+            for (int q = 0; q < 4; ++q) {
+                c4[q] = hist[wave][4 * lane + q];
+                own += c4[q];
+            }
+            uint32_t inc = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+                if (lane >= o) inc += y;
+            }
+            const unsigned long long ball = __ballot(inc >= need);
+            if (lane == __ffsll(ball) - 1) {
+                uint32_t below = inc - own;
+                int q = 0;
+                while (q < 3 && below + c4[q] < need) below += c4[q++];
+                sel_bin[wave] = 4 * lane + q;
+                sel_tie[wave] = static_cast<int>(need - below);
+            }
+        }
+        __syncthreads();
+        // packed counts (below the cut bin) << 16 | (in the cut bin), scanned
+        // over the positions
+        const int cb = splits ? sel_bin[j] : -1;
+        uint32_t fl[KDL_PER_THREAD], own = 0;
+#pragma unroll
+        for (int r = 0; r < KDL_PER_THREAD; ++r) {
+            fl[r] = 0u;
+            if (splits && p0 + r < n) fl[r] = dg[r] < cb ? 0x10000u : (dg[r] == cb ? 1u : 0u);
+            own += fl[r];
+        }
+        uint32_t inc = own;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t excl = inc - own;
+        for (int w = 0; w < wave; ++w) excl += wsum[w];
+        if (splits && p0 == bj) spref[j] = excl;
+        __syncthreads();
+        // stable partition; every slot of the level was read before the
+        // barriers above
+        if (splits) {
+            const int tie = sel_tie[j];
+            const int need = (nj + 2 * TILE - 1) / (2 * TILE) * TILE;
+            uint32_t run = excl - spref[j];
+#pragma unroll
+            for (int r = 0; r < KDL_PER_THREAD; ++r) {
+                const int p = p0 + r;
+                if (p < n) {
+                    const int lb = static_cast<int>(run >> 16), tb = static_cast<int>(run & 0xFFFFu);
+                    const int before = lb + min(tb, tie);
+                    const bool left = fl[r] == 0x10000u || (fl[r] == 1u && tb < tie);
+                    const int dst = left ? bj + before : bj + need + (p - bj - before);
+                    slot[kidx(dst)] = s[r];
+                }
+                run += fl[r];
+            }
+        }
+        __syncthreads();
+    }
+
+    // the node's order; every read of the old order has returned (its value
+    // is stored to LDS) before the barrier that precedes the first write
+    uint32_t sl[KDL_PER_THREAD], at[KDL_PER_THREAD];
+#pragma unroll
+    for (int i = 0; i < KDL_PER_THREAD; ++i) {
+        const int p = tid + i * KDL_THREADS;
+        sl[i] = 0u;
+        at[i] = 0u;
+        if (p < n) {
+            sl[i] = slot[kidx(p)];
+            at[i] = order[begin + sl[i]];
+            slot[kidx(p)] = at[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KDL_PER_THREAD; ++i) {
+        const int p = tid + i * KDL_THREADS;
+        if (p < n) order[begin + p] = slot[kidx(p)];
+    }
+
+    // tiles: thread tid handles position tid + i KDL_THREADS, i.e. lane
+    // tid % TILE of tile 4 i + tid / TILE
+    const int ntl = (n + TILE - 1) / TILE;
+    const int g = tid / TILE, li = tid % TILE;
+    constexpr int WPT = TILE / 64;                    // waves per tile
+    float rmax_run = 0.f, on_run = 0.f;
+    bool emitted = false;
+#pragma unroll
+    for (int i = 0; i < KDL_PER_THREAD; ++i) {
+        if (4 * i >= ntl) break;                      // block-uniform
+        const int t = 4 * i + g;
+        const int p = tid + i * KDL_THREADS;
+        const bool live = p < n;                      // false for every lane of a tile past ntl
+        float lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            const float v = live ? xyz[d][sl[i]] : 0.f;
+            lo[d] = live ? v : INFINITY;
+            hi[d] = live ? v : -INFINITY;
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[d] = fminf(lo[d], __shfl_xor(lo[d], o, 64));
+                hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o, 64));
+            }
+        }
+        if (lane == 0)
+            for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
+        __syncthreads();
+        double o[3];
+        for (int d = 0; d < 3; ++d) {
+            float a = red[g * WPT][d], b = red[g * WPT][3 + d];
+            for (int w = 1; w < WPT; ++w) {
+                a = fminf(a, red[g * WPT + w][d]);
+                b = fmaxf(b, red[g * WPT + w][3 + d]);
+            }
+            // the minimum of float / 1e10 in FP64 is the minimum float / 1e10
+            o[d] = 0.5 * (static_cast<double>(a) / 1e10 + static_cast<double>(b) / 1e10);
+        }
+        AtomRec rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float rad = 0.f;
+        if (live) {
+            double r[3];
+            for (int d = 0; d < 3; ++d) r[d] = static_cast<double>(xyz[d][sl[i]]) / 1e10;
+            rec = atom_record(r, o, K, at[i], amp);
+            rad = sqrtf(rec.s2);
+        }
+        if (t < ntl) atoms[(static_cast<int64_t>(nd.tile0) + t) * TILE + li] = rec;
+        for (int s = 32; s > 0; s >>= 1) rad = fmaxf(rad, __shfl_xor(rad, s, 64));
+        __syncthreads();                              // red reads above are done
+        if (lane == 0) red[wave][0] = rad;
+        __syncthreads();
+        if (li == 0 && t < ntl) {
+            float rm = red[g * WPT][0];
+            for (int w = 1; w < WPT; ++w) rm = fmaxf(rm, red[g * WPT + w][0]);
+            TileRec tr;
+            tr.ox = o[0];
+            tr.oy = o[1];
+            tr.oz = o[2];
+            tr.species = static_cast<int>(nd.species);
+            tr.count = min(TILE, n - t * TILE);
+            tiles[nd.tile0 + t] = tr;
+            rmax_run = fmaxf(rmax_run, rm);
+            emitted = true;
+            const float on = static_cast<float>(sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+            on_run = fmaxf(on_run, on * 1.0000002f);
+        }
+        __syncthreads();                              // red is rewritten next tile
+    }
+    if (emitted) {
+        atomicMax(radius_bits, __float_as_uint(rmax_run));
+        atomicMax(radius_bits + 1, __float_as_uint(on_run));
     }
 }
 
@@ -2617,8 +3017,10 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
     int sbits = 0;
     while ((1 << sbits) < n_species) ++sbits;
-    // 32-bit sort keys (node, 12-bit coordinate) while the node ids fit
-    const bool k32 = KD_QBITS32 + sbits + L.kd_levels <= 32;
+    // levels by global radix sorts; the last (at most KDL_LEVELS) run in LDS
+    const int glevels = std::max(0, L.kd_levels - KDL_LEVELS);
+    // 32-bit sort keys (node, coordinate) while the node ids fit
+    const bool k32 = sbits + std::max(0, glevels - 1) + KD_QBITS32_MIN <= 32;
     uint32_t* keys32_a = reinterpret_cast<uint32_t*>(L.keys_a);
     uint32_t* keys32_b = reinterpret_cast<uint32_t*>(L.keys_b);
     // group by species (stable), then bisect each species level by level
@@ -2636,48 +3038,82 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
                                                    L.vals_b, n32, 0, 4, st));
     }
-    std::vector<uint32_t> plan;       // alive until the synchronisation below
-    if (L.kd_levels > 0) {
-        kd_plan(n_species, counts, L.kd_levels, plan);
+    // host buffers alive until the synchronisation below
+    std::vector<uint32_t> plan;
+    std::vector<int64_t> nbeg, nend;
+    kd_plan(n_species, counts, glevels, plan, nbeg, nend);
+    std::vector<KdNode> lnodes;
+    for (size_t h = 0; h < nbeg.size(); ++h) {
+        if (nend[h] <= nbeg[h]) continue;
+        const int s = static_cast<int>(h >> glevels);
+        if (nend[h] - nbeg[h] > KDL_MAX) return MOSAIC_ERR_ARGS;   // not reached by the plan
+        KdNode nd;
+        nd.begin = static_cast<uint32_t>(nbeg[h]);
+        nd.count = static_cast<uint32_t>(nend[h] - nbeg[h]);
+        nd.tile0 = static_cast<uint32_t>(split.tile_start[s] +
+                                         (nbeg[h] - split.atom_start[s]) / TILE);
+        nd.species = static_cast<uint32_t>(s);
+        lnodes.push_back(nd);
+    }
+    if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
+    if (glevels > 0) {
         MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_split, plan.data(), plan.size() * sizeof(uint32_t),
                                         hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_kd_root, dim3(nb), dim3(KD_THREADS), 0, st, atom_count, split, L.seg);
-        for (int l = 0; l < L.kd_levels; ++l) {
+        for (int l = 0; l < glevels; ++l) {
             const int64_t nodes = static_cast<int64_t>(n_species) << l;
             const uint32_t* cut = L.kd_split + kd_split_count(n_species, l);
-            hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0, st,
-                               L.kd_box, 6 * nodes);
-            hipLaunchKernelGGL(k_kd_box, dim3(blocks_for(atom_count, KD_THREADS * KD_CHUNKS)),
-                               dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg, cut, atom_count,
-                               L.kd_box);
+            const uint32_t* prev = l > 0 ? L.kd_split + kd_split_count(n_species, l - 1) : nullptr;
+            const float* fbox = L.kd_fbox[l & 1];
+            float* fnext = L.kd_fbox[(l + 1) & 1];
+            auto tight = [&](int lv) { return lv == 0 || lv >= glevels - KD_TIGHT_LEVELS; };
+            if (tight(l)) {
+                hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
+                                   st, L.kd_box, 6 * nodes);
+                hipLaunchKernelGGL(k_kd_box, dim3(blocks_for(atom_count, KD_THREADS * KD_CHUNKS)),
+                                   dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg, prev, cut,
+                                   atom_count, L.kd_box);
+                hipLaunchKernelGGL(k_kd_box_float, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
+                                   st, L.kd_box, 6 * nodes, L.kd_fbox[l & 1]);
+            }
+            const bool derive = l + 1 < glevels && !tight(l + 1);
             sb = L.sort_bytes;
             if (k32) {
-                hipLaunchKernelGGL((k_kd_keys<uint32_t, KD_QBITS32>), dim3(nb), dim3(KD_THREADS), 0,
-                                   st, pos, L.vals_b, L.seg, cut, L.kd_box, atom_count, keys32_a,
+                const int qb = kd_qbits32(l + sbits);
+                hipLaunchKernelGGL(k_kd_keys<uint32_t>, dim3(nb), dim3(KD_THREADS), 0, st, pos,
+                                   L.vals_b, L.seg, prev, cut, fbox, atom_count, qb, keys32_a,
                                    L.vals_a);
                 MOSAIC_HIP_CHECK(hipGetLastError());
                 MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, keys32_a, keys32_b,
                                                            L.vals_a, L.vals_b, n32, 0,
-                                                           KD_QBITS32 + l + sbits, st));
+                                                           qb + l + sbits, st));
+                if (derive)
+                    hipLaunchKernelGGL(k_kd_child_boxes<uint32_t>, dim3(blocks_for(nodes, 256)),
+                                       dim3(256), 0, st, fbox, cut, keys32_b, nodes, qb, fnext);
             } else {
-                hipLaunchKernelGGL((k_kd_keys<uint64_t, KD_QBITS>), dim3(nb), dim3(KD_THREADS), 0,
-                                   st, pos, L.vals_b, L.seg, cut, L.kd_box, atom_count, L.keys_a,
-                                   L.vals_a);
+                hipLaunchKernelGGL(k_kd_keys<uint64_t>, dim3(nb), dim3(KD_THREADS), 0, st, pos,
+                                   L.vals_b, L.seg, prev, cut, fbox, atom_count, KD_QBITS,
+                                   L.keys_a, L.vals_a);
                 MOSAIC_HIP_CHECK(hipGetLastError());
                 MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b,
                                                            L.vals_a, L.vals_b, n32, 0,
                                                            KD_QBITS + l + sbits, st));
+                if (derive)
+                    hipLaunchKernelGGL(k_kd_child_boxes<uint64_t>, dim3(blocks_for(nodes, 256)),
+                                       dim3(256), 0, st, fbox, cut, L.keys_b, nodes, KD_QBITS,
+                                       fnext);
             }
-            if (l + 1 < L.kd_levels)
-                hipLaunchKernelGGL(k_kd_next, dim3(nb), dim3(KD_THREADS), 0, st, cut, atom_count,
-                                   L.seg);
+            MOSAIC_HIP_CHECK(hipGetLastError());
         }
     }
     const double K = k * INV_2PI_HI;
-    if (L.n_tiles > 0)
-        hipLaunchKernelGGL(k_tiles, dim3(L.n_tiles), dim3(TILE), 0, st, pos,
-                           reinterpret_cast<const float2*>(amp), L.vals_b, split, K, L.tiles,
-                           L.atoms, L.radius_bits);
+    if (!lnodes.empty()) {
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_nodes, lnodes.data(), lnodes.size() * sizeof(KdNode),
+                                        hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_kd_local, dim3(static_cast<unsigned int>(lnodes.size())),
+                           dim3(KDL_THREADS), 0, st, pos, reinterpret_cast<const float2*>(amp),
+                           L.vals_b, L.kd_nodes, K, L.tiles, L.atoms, L.radius_bits);
+    }
     MOSAIC_HIP_CHECK(hipGetLastError());
     unsigned int bits[2] = {0u, 0u};
     MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
