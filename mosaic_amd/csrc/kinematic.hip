@@ -51,12 +51,10 @@ constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
 constexpr int N_GEOM = 6;            // pixel geometry statistics (mosaic_kin_pixels)
 // The last k-d levels run inside one workgroup per node: a node of at most
-// KDL_MAX atoms sits in LDS and is bisected KDL_LEVELS times there.
+// KDL_MAX atoms sits in LDS and is bisected up to KDL_LEVELS times there
+// (k_kd_local<LV>, 32 << LV threads, for nodes of up to TILE << LV atoms).
 constexpr int KDL_LEVELS = 5;
 constexpr int KDL_MAX = TILE << KDL_LEVELS;          // 8192 atoms per node
-constexpr int KDL_THREADS = 1024;
-constexpr int KDL_PER_THREAD = KDL_MAX / KDL_THREADS;
-constexpr int KDL_SUB = (1 << KDL_LEVELS) - 1;       // sub-nodes over the local levels
 constexpr double R_E = 2.81794092e-15;
 constexpr double PI = 3.14159265358979323846;
 
@@ -119,6 +117,7 @@ struct Layout {
     PixelTables px;
     TileRec* tiles;
     AtomRec* atoms;
+    float* grp_r;           // [tile][2] radius of the pair / quad headed by the tile, else +inf
     uint64_t* keys_a;
     uint64_t* keys_b;
     uint32_t* vals_a;
@@ -228,6 +227,7 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.n_tiles = count_tiles(n_species, counts);
     L.tiles = c.take<TileRec>(std::max(1, L.n_tiles));
     L.atoms = c.take<AtomRec>(static_cast<size_t>(std::max(1, L.n_tiles)) * TILE);
+    L.grp_r = c.take<float>(2 * static_cast<size_t>(std::max(1, L.n_tiles)));
     size_t n = static_cast<size_t>(std::max<int64_t>(1, n_atoms));
     L.keys_a = c.take<uint64_t>(n);
     L.keys_b = c.take<uint64_t>(n);
@@ -796,6 +796,13 @@ __global__ void k_kd_root(int64_t n, SpeciesSplit split, uint32_t* __restrict__ 
     seg[i] = s;
 }
 
+// Origin of a unit of 2 or 4 consecutive tiles: the mean of their origins,
+// taken pairwise (the moment sum and the staging both use this form).
+__device__ __forceinline__ double unit_origin(int n, double a, double b, double c, double d) {
+    const double ab = 0.5 * (a + b);
+    return n == 2 ? ab : 0.5 * (ab + 0.5 * (c + d));
+}
+
 // Record of one atom of a tile: FP32 offset from the tile origin o [m] in
 // wavelengths, amplitude as magnitude and phase in turns.  r is the position
 // in metres (the float position in angstrom over 1e10).
@@ -825,7 +832,7 @@ __device__ __forceinline__ AtomRec atom_record(const double (&r)[3], const doubl
 __global__ __launch_bounds__(TILE) void k_tiles(
     const float* __restrict__ pos, const float2* __restrict__ amp,
     const uint32_t* __restrict__ order, SpeciesSplit split, double K,
-    TileRec* __restrict__ tiles, AtomRec* __restrict__ atoms,
+    TileRec* __restrict__ tiles, AtomRec* __restrict__ atoms, float* __restrict__ grp_r,
     unsigned int* __restrict__ radius_bits) {
     __shared__ double lds[TILE / 64];
     const int t = blockIdx.x;
@@ -855,6 +862,8 @@ __global__ __launch_bounds__(TILE) void k_tiles(
     atoms[static_cast<int64_t>(t) * TILE + i] = rec;
     const double rmax = block_max(static_cast<double>(rad), lds);
     if (i == 0) {
+        grp_r[2 * t] = INFINITY;
+        grp_r[2 * t + 1] = INFINITY;
         TileRec tr;
         tr.ox = o[0];
         tr.oy = o[1];
@@ -876,48 +885,58 @@ __device__ __forceinline__ int kidx(int p) { return p + (p >> 3); }
 constexpr int KDL_BINS = 256;      // coordinate bins of an in-LDS split
 constexpr int KDL_SUBMAX = 1 << (KDL_LEVELS - 1);   // sub-nodes of the deepest split level
 
-// The last k-d levels and the tile records of one node, in one workgroup.
-// The node's positions [begin, begin + count) of the order (at most KDL_MAX
-// atoms, a whole number of tiles except at a species end) sit in LDS as
-// coordinates by load slot and a slot per position.  Each of KDL_LEVELS
-// levels splits every sub-node of more than one tile by the count rule of the
-// global levels (the left child keeps ceil(n / 2T) T positions): a bounding
-// box, the longest side, a histogram of the coordinate over that side in
-// KDL_BINS bins, the bin holding the cut, and a stable partition (a packed
-// block-wide scan of "below the cut bin" and "in the cut bin" flags) in which
-// the first positions of the cut bin go left until the count is met.  The
-// halves overlap by at most one bin.  Then the node's order is written back
-// and its tiles are emitted as k_tiles emits them.
-__global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
+// The last k-d levels and the tile records of one node, in one workgroup of
+// 32 << LV threads (8 positions each).  The node's positions
+// [begin, begin + count) of the order (at most TILE << LV atoms, a whole
+// number of tiles except at a species end) sit in LDS as coordinates by load
+// slot and a slot per position.  Each of LV levels splits every sub-node of
+// more than one tile by the count rule of the global levels (the left child
+// keeps ceil(n / 2T) T positions): a bounding box, the longest side, a
+// histogram of the coordinate over that side in KDL_BINS bins, the bin
+// holding the cut, and a stable partition (a packed block-wide scan of "below
+// the cut bin" and "in the cut bin" flags) in which the first positions of
+// the cut bin go left until the count is met.  The halves overlap by at most
+// one bin.  Then the node's order is written back and its tiles are emitted
+// as k_tiles emits them.  Thread tid holds positions 8 tid .. 8 tid + 7, so
+// a half-wave covers one tile, which never straddles two sub-nodes.
+template <int LV>
+__global__ __launch_bounds__(32 << LV) void k_kd_local(
     const float* __restrict__ pos, const float2* __restrict__ amp, uint32_t* __restrict__ order,
     const KdNode* __restrict__ nodes, double K, TileRec* __restrict__ tiles,
-    AtomRec* __restrict__ atoms, unsigned int* __restrict__ radius_bits) {
-    __shared__ float xyz[3][KDL_MAX];                 // coordinates by load slot
-    __shared__ uint32_t slot[KDL_MAX + KDL_MAX / 8];  // load slot at kidx(position)
-    __shared__ unsigned int hist[KDL_SUBMAX][KDL_BINS];
-    __shared__ int sub_b[KDL_SUB], sub_n[KDL_SUB];    // sub-node ranges, level-major heap
-    __shared__ int sub_any[KDL_LEVELS];               // some sub-node of the level splits
-    __shared__ unsigned int box[KDL_SUBMAX][6];
-    __shared__ int sel_bin[KDL_SUBMAX], sel_tie[KDL_SUBMAX];
-    __shared__ uint32_t wsum[KDL_THREADS / 64];
-    __shared__ uint32_t spref[KDL_SUBMAX];
-    __shared__ float red[KDL_THREADS / 64][6];
+    AtomRec* __restrict__ atoms, float* __restrict__ grp_r, unsigned int* __restrict__ radius_bits) {
+    constexpr int THREADS = 32 << LV, MAXN = TILE << LV, PT = MAXN / THREADS;
+    constexpr int SUB = (1 << LV) - 1, SUBMAX = 1 << (LV - 1), WAVES = THREADS / 64;
+    constexpr int NTILE = 1 << LV;
+    static_assert(PT == 8 && WAVES >= SUBMAX, "one wave per deepest sub-node, 8 positions a lane");
+    __shared__ float xyz[3][MAXN];                    // coordinates by load slot
+    __shared__ uint32_t slot[MAXN + MAXN / 8];        // load slot at kidx(position)
+    __shared__ unsigned int hist[SUBMAX][KDL_BINS];
+    __shared__ int sub_b[SUB], sub_n[SUB];            // sub-node ranges, level-major heap
+    __shared__ int sub_any[LV];                       // some sub-node of the level splits
+    __shared__ unsigned int box[SUBMAX][6];
+    __shared__ int sel_bin[SUBMAX], sel_tie[SUBMAX];
+    __shared__ uint32_t wsum[WAVES];
+    __shared__ uint32_t spref[SUBMAX];
+    __shared__ double tile_o[NTILE][3];
+    __shared__ float tile_r[NTILE], tile_on[NTILE];
+    __shared__ float quad_r[WAVES];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const KdNode nd = nodes[blockIdx.x];
     const int n = static_cast<int>(nd.count);
     const int64_t begin = nd.begin;
+    const int p0 = PT * tid;                          // this thread's first position
 
     {
-        uint32_t a[KDL_PER_THREAD];
+        uint32_t a[PT];
 #pragma unroll
-        for (int i = 0; i < KDL_PER_THREAD; ++i) {
-            const int p = tid + i * KDL_THREADS;
+        for (int i = 0; i < PT; ++i) {
+            const int p = tid + i * THREADS;
             a[i] = p < n ? order[begin + p] : 0u;
         }
 #pragma unroll
-        for (int i = 0; i < KDL_PER_THREAD; ++i) {
-            const int p = tid + i * KDL_THREADS;
+        for (int i = 0; i < PT; ++i) {
+            const int p = tid + i * THREADS;
             if (p < n)
                 for (int d = 0; d < 3; ++d) xyz[d][p] = pos[3 * static_cast<int64_t>(a[i]) + d];
             slot[kidx(p)] = static_cast<uint32_t>(p);
@@ -925,7 +944,7 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
     }
     // sub-node (f, j) is heap entry 2^f - 1 + j; its range follows the cuts
     // along the path from the root
-    if (tid < KDL_SUB) {
+    if (tid < SUB) {
         int f = 0;
         while ((2 << f) <= tid + 1) ++f;
         const int j = tid + 1 - (1 << f);
@@ -942,57 +961,51 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
         sub_b[tid] = b;
         sub_n[tid] = m;
     }
-    if (tid < KDL_LEVELS) sub_any[tid] = 0;
+    if (tid < LV) sub_any[tid] = 0;
     __syncthreads();
-    if (tid < KDL_SUB && sub_n[tid] > TILE) {
+    if (tid < SUB && sub_n[tid] > TILE) {
         int f = 0;
         while ((2 << f) <= tid + 1) ++f;
         atomicOr(&sub_any[f], 1);
     }
     __syncthreads();
 
-    for (int f = 0; f < KDL_LEVELS; ++f) {
+    for (int f = 0; f < LV; ++f) {
         if (!sub_any[f]) break;                       // no sub-node splits any more
         const int base = (1 << f) - 1, nsub = 1 << f;
         if (tid < 6 * nsub) box[tid / 6][tid % 6] = (tid % 6 < 3) ? 0xFFFFFFFFu : 0u;
-        for (int i = tid; i < nsub * KDL_BINS; i += KDL_THREADS) hist[i / KDL_BINS][i % KDL_BINS] = 0u;
-        __syncthreads();
-        // bounding boxes: a 64-position chunk never straddles two sub-nodes
-        // (their bounds are multiples of TILE or the node end)
-        for (int c0 = wave * 64; c0 < n; c0 += KDL_THREADS) {
-            int j = 0;
-            while (j + 1 < nsub && c0 >= sub_b[base + j + 1]) ++j;
-            if (sub_n[base + j] <= TILE) continue;    // wave-uniform
-            const int p = c0 + lane;
-            unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-            if (p < n) {
-                const uint32_t sl = slot[kidx(p)];
-                for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(xyz[d][sl]);
-            }
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = wave_umin(lo[d]);
-                hi[d] = wave_umax(hi[d]);
-            }
-            if (lane == 0)
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(&box[j][d], lo[d]);
-                    atomicMax(&box[j][3 + d], hi[d]);
-                }
-        }
-        __syncthreads();
-        // this thread's positions p0 .. p0 + 7 lie in one sub-node j
-        const int p0 = KDL_PER_THREAD * tid;
+        for (int i = tid; i < nsub * KDL_BINS; i += THREADS) hist[i / KDL_BINS][i % KDL_BINS] = 0u;
         int j = 0;
         while (j + 1 < nsub && p0 >= sub_b[base + j + 1]) ++j;
         const int bj = sub_b[base + j], nj = sub_n[base + j];
         const bool splits = p0 < n && nj > TILE;
-        uint32_t s[KDL_PER_THREAD];
-        int dg[KDL_PER_THREAD];
+        uint32_t s[PT];
+        int dg[PT];
+        unsigned int lo3[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi3[3] = {0u, 0u, 0u};
 #pragma unroll
-        for (int r = 0; r < KDL_PER_THREAD; ++r) {
+        for (int r = 0; r < PT; ++r) {
             s[r] = slot[kidx(p0 + r)];
             dg[r] = 0;
+            if (splits && p0 + r < n)
+                for (int d = 0; d < 3; ++d) {
+                    const unsigned int o = f2ord(xyz[d][s[r]]);
+                    lo3[d] = min(lo3[d], o);
+                    hi3[d] = max(hi3[d], o);
+                }
         }
+        // bounding boxes over the half-wave (one tile of one sub-node)
+        for (int d = 0; d < 3; ++d)
+            for (int o = 16; o > 0; o >>= 1) {
+                lo3[d] = min(lo3[d], static_cast<unsigned int>(__shfl_xor(static_cast<int>(lo3[d]), o, 64)));
+                hi3[d] = max(hi3[d], static_cast<unsigned int>(__shfl_xor(static_cast<int>(hi3[d]), o, 64)));
+            }
+        __syncthreads();                              // box and hist cleared
+        if ((lane & 31) == 0 && splits)
+            for (int d = 0; d < 3; ++d) {
+                atomicMin(&box[j][d], lo3[d]);
+                atomicMax(&box[j][3 + d], hi3[d]);
+            }
+        __syncthreads();
         if (splits) {
             float l3[3], e3[3];
             for (int d = 0; d < 3; ++d) {
@@ -1004,7 +1017,7 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
             if (e3[2] > e3[ax]) ax = 2;
             const float sc = e3[ax] > 0.f ? static_cast<float>(KDL_BINS) / e3[ax] : 0.f;
 #pragma unroll
-            for (int r = 0; r < KDL_PER_THREAD; ++r)
+            for (int r = 0; r < PT; ++r)
                 if (p0 + r < n) {
                     const float t = (xyz[ax][s[r]] - l3[ax]) * sc;
                     dg[r] = min(KDL_BINS - 1, max(0, static_cast<int>(t)));
@@ -1039,9 +1052,9 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
         // packed counts (below the cut bin) << 16 | (in the cut bin), scanned
         // over the positions
         const int cb = splits ? sel_bin[j] : -1;
-        uint32_t fl[KDL_PER_THREAD], own = 0;
+        uint32_t fl[PT], own = 0;
 #pragma unroll
-        for (int r = 0; r < KDL_PER_THREAD; ++r) {
+        for (int r = 0; r < PT; ++r) {
             fl[r] = 0u;
             if (splits && p0 + r < n) fl[r] = dg[r] < cb ? 0x10000u : (dg[r] == cb ? 1u : 0u);
             own += fl[r];
@@ -1064,7 +1077,7 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
             const int need = (nj + 2 * TILE - 1) / (2 * TILE) * TILE;
             uint32_t run = excl - spref[j];
 #pragma unroll
-            for (int r = 0; r < KDL_PER_THREAD; ++r) {
+            for (int r = 0; r < PT; ++r) {
                 const int p = p0 + r;
                 if (p < n) {
                     const int lb = static_cast<int>(run >> 16), tb = static_cast<int>(run & 0xFFFFu);
@@ -1079,79 +1092,43 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
         __syncthreads();
     }
 
-    // the node's order; every read of the old order has returned (its value
-    // is stored to LDS) before the barrier that precedes the first write
-    uint32_t sl[KDL_PER_THREAD], at[KDL_PER_THREAD];
-#pragma unroll
-    for (int i = 0; i < KDL_PER_THREAD; ++i) {
-        const int p = tid + i * KDL_THREADS;
-        sl[i] = 0u;
-        at[i] = 0u;
-        if (p < n) {
-            sl[i] = slot[kidx(p)];
-            at[i] = order[begin + sl[i]];
-            slot[kidx(p)] = at[i];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < KDL_PER_THREAD; ++i) {
-        const int p = tid + i * KDL_THREADS;
-        if (p < n) order[begin + p] = slot[kidx(p)];
-    }
-
-    // tiles: thread tid handles position tid + i KDL_THREADS, i.e. lane
-    // tid % TILE of tile 4 i + tid / TILE
+    // tile origins (bounding-box centres) and radii, one half-wave per tile;
+    // the minimum of float / 1e10 in FP64 is the minimum float / 1e10
     const int ntl = (n + TILE - 1) / TILE;
-    const int g = tid / TILE, li = tid % TILE;
-    constexpr int WPT = TILE / 64;                    // waves per tile
-    float rmax_run = 0.f, on_run = 0.f;
-    bool emitted = false;
+    {
+        uint32_t s[PT];
+        float lo3[3] = {INFINITY, INFINITY, INFINITY}, hi3[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-    for (int i = 0; i < KDL_PER_THREAD; ++i) {
-        if (4 * i >= ntl) break;                      // block-uniform
-        const int t = 4 * i + g;
-        const int p = tid + i * KDL_THREADS;
-        const bool live = p < n;                      // false for every lane of a tile past ntl
-        float lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            const float v = live ? xyz[d][sl[i]] : 0.f;
-            lo[d] = live ? v : INFINITY;
-            hi[d] = live ? v : -INFINITY;
-            for (int o = 32; o > 0; o >>= 1) {
-                lo[d] = fminf(lo[d], __shfl_xor(lo[d], o, 64));
-                hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o, 64));
-            }
+        for (int r = 0; r < PT; ++r) {
+            s[r] = slot[kidx(p0 + r)];
+            if (p0 + r < n)
+                for (int d = 0; d < 3; ++d) {
+                    lo3[d] = fminf(lo3[d], xyz[d][s[r]]);
+                    hi3[d] = fmaxf(hi3[d], xyz[d][s[r]]);
+                }
         }
-        if (lane == 0)
-            for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
-        __syncthreads();
+        for (int d = 0; d < 3; ++d)
+            for (int o = 16; o > 0; o >>= 1) {
+                lo3[d] = fminf(lo3[d], __shfl_xor(lo3[d], o, 64));
+                hi3[d] = fmaxf(hi3[d], __shfl_xor(hi3[d], o, 64));
+            }
         double o[3];
-        for (int d = 0; d < 3; ++d) {
-            float a = red[g * WPT][d], b = red[g * WPT][3 + d];
-            for (int w = 1; w < WPT; ++w) {
-                a = fminf(a, red[g * WPT + w][d]);
-                b = fmaxf(b, red[g * WPT + w][3 + d]);
-            }
-            // the minimum of float / 1e10 in FP64 is the minimum float / 1e10
-            o[d] = 0.5 * (static_cast<double>(a) / 1e10 + static_cast<double>(b) / 1e10);
-        }
-        AtomRec rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int d = 0; d < 3; ++d)
+            o[d] = 0.5 * (static_cast<double>(lo3[d]) / 1e10 + static_cast<double>(hi3[d]) / 1e10);
         float rad = 0.f;
-        if (live) {
-            double r[3];
-            for (int d = 0; d < 3; ++d) r[d] = static_cast<double>(xyz[d][sl[i]]) / 1e10;
-            rec = atom_record(r, o, K, at[i], amp);
-            rad = sqrtf(rec.s2);
-        }
-        if (t < ntl) atoms[(static_cast<int64_t>(nd.tile0) + t) * TILE + li] = rec;
-        for (int s = 32; s > 0; s >>= 1) rad = fmaxf(rad, __shfl_xor(rad, s, 64));
-        __syncthreads();                              // red reads above are done
-        if (lane == 0) red[wave][0] = rad;
-        __syncthreads();
-        if (li == 0 && t < ntl) {
-            float rm = red[g * WPT][0];
-            for (int w = 1; w < WPT; ++w) rm = fmaxf(rm, red[g * WPT + w][0]);
+#pragma unroll
+        for (int r = 0; r < PT; ++r)
+            if (p0 + r < n) {
+                double rr[3];
+                for (int d = 0; d < 3; ++d) rr[d] = static_cast<double>(xyz[d][s[r]]) / 1e10;
+                rad = fmaxf(rad, sqrtf(atom_record(rr, o, K, 0u, nullptr).s2));
+            }
+        for (int q = 16; q > 0; q >>= 1) rad = fmaxf(rad, __shfl_xor(rad, q, 64));
+        const int t = p0 / TILE;
+        if ((lane & 31) == 0 && p0 < n) {
+            for (int d = 0; d < 3; ++d) tile_o[t][d] = o[d];
+            tile_r[t] = rad;
+            tile_on[t] = static_cast<float>(sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2])) * 1.0000002f;
             TileRec tr;
             tr.ox = o[0];
             tr.oy = o[1];
@@ -1159,16 +1136,95 @@ __global__ __launch_bounds__(KDL_THREADS) void k_kd_local(
             tr.species = static_cast<int>(nd.species);
             tr.count = min(TILE, n - t * TILE);
             tiles[nd.tile0 + t] = tr;
-            rmax_run = fmaxf(rmax_run, rm);
-            emitted = true;
-            const float on = static_cast<float>(sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
-            on_run = fmaxf(on_run, on * 1.0000002f);
         }
-        __syncthreads();                              // red is rewritten next tile
     }
-    if (emitted) {
-        atomicMax(radius_bits, __float_as_uint(rmax_run));
-        atomicMax(radius_bits + 1, __float_as_uint(on_run));
+    // radii of the pairs (one wave) and quads (two waves) of tiles about their
+    // unit origins; units stay inside the node
+    __syncthreads();                                  // tile_o complete
+    {
+        const int t = p0 / TILE;
+        float rp = 0.f, rq = 0.f;
+        const int hp = t & ~1, hq = t & ~3;
+        const bool pair = hp + 1 < ntl, quad = hq + 3 < ntl;
+        double op[3], oq[3];
+        for (int d = 0; d < 3; ++d) {
+            op[d] = unit_origin(2, tile_o[hp][d], tile_o[min(hp + 1, NTILE - 1)][d], 0.0, 0.0);
+            oq[d] = unit_origin(4, tile_o[hq][d], tile_o[min(hq + 1, NTILE - 1)][d],
+                                tile_o[min(hq + 2, NTILE - 1)][d], tile_o[min(hq + 3, NTILE - 1)][d]);
+        }
+        const uint32_t* sp = slot;
+#pragma unroll
+        for (int r = 0; r < PT; ++r)
+            if (p0 + r < n) {
+                const uint32_t sl = sp[kidx(p0 + r)];
+                double rr[3];
+                for (int d = 0; d < 3; ++d) rr[d] = static_cast<double>(xyz[d][sl]) / 1e10;
+                if (pair) rp = fmaxf(rp, sqrtf(atom_record(rr, op, K, 0u, nullptr).s2));
+                if (quad) rq = fmaxf(rq, sqrtf(atom_record(rr, oq, K, 0u, nullptr).s2));
+            }
+        for (int q = 32; q > 0; q >>= 1) {
+            rp = fmaxf(rp, __shfl_xor(rp, q, 64));
+            rq = fmaxf(rq, __shfl_xor(rq, q, 64));
+        }
+        if (lane == 0) quad_r[wave] = rq;
+        __syncthreads();
+        if ((lane & 31) == 0 && p0 < n) {
+            const bool ph = pair && t == hp, qh = quad && t == hq;
+            grp_r[2 * (static_cast<int64_t>(nd.tile0) + t)] = ph ? rp : INFINITY;
+            grp_r[2 * (static_cast<int64_t>(nd.tile0) + t) + 1] =
+                qh ? fmaxf(quad_r[wave], quad_r[min(wave + 1, WAVES - 1)]) : INFINITY;
+        }
+    }
+
+    // the node's order; every read of the old order has returned (its value
+    // is stored to LDS) before the barrier that precedes the first write
+    uint32_t sl[PT], at[PT];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+        const int p = tid + i * THREADS;
+        sl[i] = 0u;
+        at[i] = 0u;
+        if (p < n) {
+            sl[i] = slot[kidx(p)];
+            at[i] = order[begin + sl[i]];
+        }
+    }
+    __syncthreads();                                  // slot reads above are done
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+        const int p = tid + i * THREADS;
+        if (p < n) slot[kidx(p)] = at[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+        const int p = tid + i * THREADS;
+        if (p < n) order[begin + p] = slot[kidx(p)];
+    }
+    if (tid == 0) {
+        float rm = 0.f, om = 0.f;
+        for (int t = 0; t < ntl; ++t) {
+            rm = fmaxf(rm, tile_r[t]);
+            om = fmaxf(om, tile_on[t]);
+        }
+        atomicMax(radius_bits, __float_as_uint(rm));
+        atomicMax(radius_bits + 1, __float_as_uint(om));
+    }
+    // atom records, coalesced: position tid + i THREADS
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+        const int p = tid + i * THREADS;
+        const int t = p / TILE;
+        if (t >= ntl) break;
+        AtomRec rec = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (p < n) {
+            double rr[3];
+            for (int d = 0; d < 3; ++d) rr[d] = static_cast<double>(xyz[d][sl[i]]) / 1e10;
+            const double o[3] = {tile_o[t][0], tile_o[t][1], tile_o[t][2]};
+            rec = atom_record(rr, o, K, at[i], amp);
+        }
+        atoms[(static_cast<int64_t>(nd.tile0) + t) * TILE + (p % TILE)] = rec;
     }
 }
 
@@ -1547,6 +1603,8 @@ struct MmaArgs {
     int pby;          // patches along a row
     int n_patches;
     double K;         // 1 / lambda
+    const float* grp_r;   // moment sum: radii of the pairs / quads of tiles ([tile][2])
+    float grp_rmax;       // largest unit radius inside the error bounds (0: tiles alone)
 };
 
 constexpr int MMA_FLUSH = 32;
@@ -1902,6 +1960,12 @@ void k_sum_poly(const MmaArgs A) {
         since = 0;
     };
 
+    // the running unit: 4, 2 or 1 consecutive tiles of one species expanded
+    // about one origin (the staging measured the units' radii)
+    float cr[NC], ci[NC];
+    int left = 0, tie = 0;
+    double uox = 0.0, uoy = 0.0, uoz = 0.0;
+    float vx = 0.f, vy = 0.f, vz = 0.f, kox = 0.f, koy = 0.f, koz = 0.f, pht = 0.f;
     if (t0 < t1) tile_buf[0][tid] = A.atoms[static_cast<int64_t>(t0) * TILE + tid];
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
@@ -1911,46 +1975,71 @@ void k_sum_poly(const MmaArgs A) {
             A.atoms + static_cast<int64_t>(has_next ? t + 1 : t) * TILE + tid);
         const float4 n0 = nsrc[0], n1 = nsrc[1];
         const TileRec tr = A.tiles[t];
-        const int tie = tr.count >> 16;
-        // tiles are grouped by species; the sums carry one species' table
-        if (since > 0 && tr.species != acc_species) flush();
-        acc_species = tr.species;
-        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
-        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
-        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
-        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
-        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
-        double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
-        ph_t -= floor(ph_t);
-        const double h2 = 2.0 * um.w;
-        const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
-                     czl = (Oz - uo * um.z) * h2;
-        const float vx = static_cast<float>(vc0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
-        const float vy = static_cast<float>(vc0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
-        const float vz = static_cast<float>(vc0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
-        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
-                    koz = static_cast<float>(A.K * Oz);
-        const float pht = static_cast<float>(ph_t);
+        if (left == 0) {
+            int gsz = 1;
+            if (A.grp_rmax > 0.f) {
+                if (t + 3 < t1 && A.grp_r[2 * static_cast<int64_t>(t) + 1] <= A.grp_rmax) gsz = 4;
+                else if (t + 1 < t1 && A.grp_r[2 * static_cast<int64_t>(t)] <= A.grp_rmax) gsz = 2;
+            }
+            tie = tr.count >> 16;
+            // tiles are grouped by species; the sums carry one species' table
+            if (since > 0 && tr.species != acc_species) flush();
+            acc_species = tr.species;
+            uox = tr.ox;
+            uoy = tr.oy;
+            uoz = tr.oz;
+            if (gsz > 1) {
+                const TileRec b1 = A.tiles[t + 1];
+                const TileRec b2 = A.tiles[gsz == 4 ? t + 2 : t];
+                const TileRec b3 = A.tiles[gsz == 4 ? t + 3 : t];
+                uox = unit_origin(gsz, tr.ox, b1.ox, b2.ox, b3.ox);
+                uoy = unit_origin(gsz, tr.oy, b1.oy, b2.oy, b3.oy);
+                uoz = unit_origin(gsz, tr.oz, b1.oz, b2.oz, b3.oz);
+            }
+            const double Ox = fma(R00, uox, fma(R01, uoy, fma(R02, uoz, F.T[0])));
+            const double Oy = fma(R10, uox, fma(R11, uoy, fma(R12, uoz, F.T[1])));
+            const double Oz = fma(R20, uox, fma(R21, uoy, fma(R22, uoz, F.T[2])));
+            const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+            const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+            double ph_t = th_mid + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+            ph_t -= floor(ph_t);
+            const double h2 = 2.0 * um.w;
+            const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                         czl = (Oz - uo * um.z) * h2;
+            vx = static_cast<float>(vc0[0] + (R00 * cxl + R10 * cyl + R20 * czl));
+            vy = static_cast<float>(vc0[1] + (R01 * cxl + R11 * cyl + R21 * czl));
+            vz = static_cast<float>(vc0[2] + (R02 * cxl + R12 * cyl + R22 * czl));
+            kox = static_cast<float>(A.K * Ox);
+            koy = static_cast<float>(A.K * Oy);
+            koz = static_cast<float>(A.K * Oz);
+            pht = static_cast<float>(ph_t);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) { cr[k] = 0.f; ci[k] = 0.f; }
+            left = gsz;
+        }
+        // this tile's offset from the unit origin [wavelengths, sample frame]
+        const float dx = static_cast<float>((tr.ox - uox) * A.K);
+        const float dy = static_cast<float>((tr.oy - uoy) * A.K);
+        const float dz = static_cast<float>((tr.oz - uoz) * A.K);
 
         // moments of this lane's 4 atoms (records past the count are zero)
-        float cr[NC], ci[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) { cr[k] = 0.f; ci[k] = 0.f; }
         const AtomRec* at = tile_buf[buf];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const AtomRec a = at[lane + 64 * i];
-            float th = fmaf(vz, a.sz, fmaf(vy, a.sy, vx * a.sx)) + a.ph;
+            const float sx = a.sx + dx, sy = a.sy + dy, sz = a.sz + dz;
+            float th = fmaf(vz, sz, fmaf(vy, sy, vx * sx)) + a.ph;
             if (SECOND) {
-                const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx));
-                th = fmaf(hh, fmaf(-w, w, a.s2), th);
+                const float w = fmaf(xc2, sz, fmaf(xc1, sy, xc0 * sx));
+                const float s2 = fmaf(sz, sz, fmaf(sy, sy, sx * sx));
+                th = fmaf(hh, fmaf(-w, w, s2), th);
             }
             const float wr = a.mag * __builtin_amdgcn_cosf(th);
             const float wi = a.mag * __builtin_amdgcn_sinf(th);
-            const float Pj = fmaf(gyk[2], a.sz, fmaf(gyk[1], a.sy, gyk[0] * a.sx));
-            const float Rj = fmaf(gzk[2], a.sz, fmaf(gzk[1], a.sy, gzk[0] * a.sx));
-            const float Qj = fmaf(hyk[2], a.sz, fmaf(hyk[1], a.sy, hyk[0] * a.sx));
-            const float Sj = fmaf(hzk[2], a.sz, fmaf(hzk[1], a.sy, hzk[0] * a.sx));
+            const float Pj = fmaf(gyk[2], sz, fmaf(gyk[1], sy, gyk[0] * sx));
+            const float Rj = fmaf(gzk[2], sz, fmaf(gzk[1], sy, gzk[0] * sx));
+            const float Qj = fmaf(hyk[2], sz, fmaf(hyk[1], sy, hyk[0] * sx));
+            const float Sj = fmaf(hzk[2], sz, fmaf(hzk[1], sy, hzk[0] * sx));
             float am[D + 1], bn[D + 1];
             am[0] = 1.f;
             bn[0] = 1.f;
@@ -1972,6 +2061,15 @@ void k_sum_poly(const MmaArgs A) {
                     ci[k] = fmaf(wi, T, fmaf(-wr, U, ci[k]));
                 }
             }
+        }
+        if (--left > 0) {                          // the unit goes on in the next tile
+            if (has_next) {
+                float4* dst = reinterpret_cast<float4*>(&tile_buf[buf ^ 1][tid]);
+                dst[0] = n0;
+                dst[1] = n1;
+            }
+            __syncthreads();
+            continue;
         }
         // sum over the wave's lanes: row per lane in LDS, column sums per lane
 #pragma unroll
@@ -3042,18 +3140,25 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     std::vector<uint32_t> plan;
     std::vector<int64_t> nbeg, nend;
     kd_plan(n_species, counts, glevels, plan, nbeg, nend);
+    // nodes of the in-LDS finish, grouped by size: up to 8 tiles, 16, 32
     std::vector<KdNode> lnodes;
-    for (size_t h = 0; h < nbeg.size(); ++h) {
-        if (nend[h] <= nbeg[h]) continue;
-        const int s = static_cast<int>(h >> glevels);
-        if (nend[h] - nbeg[h] > KDL_MAX) return MOSAIC_ERR_ARGS;   // not reached by the plan
-        KdNode nd;
-        nd.begin = static_cast<uint32_t>(nbeg[h]);
-        nd.count = static_cast<uint32_t>(nend[h] - nbeg[h]);
-        nd.tile0 = static_cast<uint32_t>(split.tile_start[s] +
-                                         (nbeg[h] - split.atom_start[s]) / TILE);
-        nd.species = static_cast<uint32_t>(s);
-        lnodes.push_back(nd);
+    int lcount[3] = {0, 0, 0};
+    for (int cls = 0; cls < 3; ++cls) {
+        for (size_t h = 0; h < nbeg.size(); ++h) {
+            const int64_t cnt = nend[h] - nbeg[h];
+            if (cnt <= 0) continue;
+            if (cnt > KDL_MAX) return MOSAIC_ERR_ARGS;   // not reached by the plan
+            if ((cnt <= (TILE << 3) ? 0 : cnt <= (TILE << 4) ? 1 : 2) != cls) continue;
+            const int s = static_cast<int>(h >> glevels);
+            KdNode nd;
+            nd.begin = static_cast<uint32_t>(nbeg[h]);
+            nd.count = static_cast<uint32_t>(cnt);
+            nd.tile0 = static_cast<uint32_t>(split.tile_start[s] +
+                                             (nbeg[h] - split.atom_start[s]) / TILE);
+            nd.species = static_cast<uint32_t>(s);
+            lnodes.push_back(nd);
+            ++lcount[cls];
+        }
     }
     if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
     if (glevels > 0) {
@@ -3110,9 +3215,19 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     if (!lnodes.empty()) {
         MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_nodes, lnodes.data(), lnodes.size() * sizeof(KdNode),
                                         hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_kd_local, dim3(static_cast<unsigned int>(lnodes.size())),
-                           dim3(KDL_THREADS), 0, st, pos, reinterpret_cast<const float2*>(amp),
-                           L.vals_b, L.kd_nodes, K, L.tiles, L.atoms, L.radius_bits);
+        const float2* a2 = reinterpret_cast<const float2*>(amp);
+        const KdNode* nds = L.kd_nodes;
+        if (lcount[0] > 0)
+            hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, pos, a2,
+                               L.vals_b, nds, K, L.tiles, L.atoms, L.grp_r, L.radius_bits);
+        if (lcount[1] > 0)
+            hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, pos, a2,
+                               L.vals_b, nds + lcount[0], K, L.tiles, L.atoms, L.grp_r,
+                               L.radius_bits);
+        if (lcount[2] > 0)
+            hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, pos, a2,
+                               L.vals_b, nds + lcount[0] + lcount[1], K, L.tiles, L.atoms,
+                               L.grp_r, L.radius_bits);
     }
     MOSAIC_HIP_CHECK(hipGetLastError());
     unsigned int bits[2] = {0u, 0u};
@@ -3155,7 +3270,7 @@ extern "C" int mosaic_kin_restage(int64_t atom_count, const float* pos, const fl
     // the order of the last mosaic_kin_stage on this workspace is in vals_b
     hipLaunchKernelGGL(k_tiles, dim3(L.n_tiles), dim3(TILE), 0, st, pos,
                        reinterpret_cast<const float2*>(amp), L.vals_b, split, k * INV_2PI_HI,
-                       L.tiles, L.atoms, L.radius_bits);
+                       L.tiles, L.atoms, L.grp_r, L.radius_bits);
     MOSAIC_HIP_CHECK(hipGetLastError());
     unsigned int bits[2] = {0u, 0u};
     MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
@@ -3323,6 +3438,30 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
         return MOSAIC_ERR_UNSUPPORTED;
+    // Moment sum over units of 2 or 4 tiles: the largest unit radius for
+    // which every radius-dependent bound above still holds at the chosen
+    // degree and second-order setting.
+    float grp_rmax = 0.f;
+    if (path == 4) {
+        auto holds = [&](double r) {
+            const double rm = r / K;
+            if (rm / rho_min > 1e-6 || K * rm * rm * rm / (rho_min * rho_min) > 1e-7) return false;
+            if (!use_second && r * r / (2.0 * K * rho_min) > 1e-7) return false;
+            const bool ok = far && dpatch * r <= 1e-6 &&
+                            K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 && 2.0 * r + 2.0 <= 256.0;
+            const int d = poly_degree(ok, r, pixel_geom, K * omax);
+            return d != 0 && d <= poly_d;
+        };
+        double lo = radius_wl, hi = 256.0;
+        if (holds(lo)) {
+            for (int it = 0; it < 60; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                if (holds(mid)) lo = mid;
+                else hi = mid;
+            }
+            grp_rmax = static_cast<float>(lo * (1.0 - 1e-6));
+        }
+    }
 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
@@ -3346,6 +3485,8 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         A.pby = (Ny + MMA_P - 1) / MMA_P;
         A.n_patches = A.pby * ((Nz + MMA_P - 1) / MMA_P);
         A.K = K;
+        A.grp_r = L.grp_r;
+        A.grp_rmax = grp_rmax;
         if (path == 3)
             e = use_second ? launch_mma<true>(A, L.n_slices, n_frames, st)
                            : launch_mma<false>(A, L.n_slices, n_frames, st);
