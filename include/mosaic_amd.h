@@ -108,11 +108,13 @@ int mosaic_kin_restage(int64_t atom_count, const float* pos, const float* amp,
  * rho0 = |T_p|, frac(rho0 / lambda) in double-double, the forward-removal
  * Q cut (Beam.py:718-746), the unit vector and 1/(2 rho0), and per species
  * the factor (f0 + f' + i f'') r_e sqrt(P) in the pixel's direction.
- * Returns pixel_geom (host out [6]): the smallest pixel distance [m], the
+ * Returns pixel_geom (host out [8]): the smallest pixel distance [m], the
  * largest first and second differences of the pixel direction along a row,
- * the largest deviation of the pixel direction from the per-patch quadratic
- * model of the far-field sums, and the largest second and first differences
- * along a column; the call synchronizes the stream for them.  Must precede mosaic_kin_sum
+ * the largest deviation of the pixel direction from the separable per-patch
+ * quadratic model of the far-field sums, the largest second and first
+ * differences along a column, the largest deviation from the model with the
+ * cross term a b g_yz (the moment expansion's), and the largest |g_yz|; the
+ * call synchronizes the stream for them.  Must precede mosaic_kin_sum
  * on the same workspace.  `coords` is device (3, Ny*Nz) float64 metres.
  */
 int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host */,
@@ -135,9 +137,11 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * one sincos per atom per KP pixels), a matrix-core form over 32 x 32 pixel
  * patches (the tile sum as a complex product of per-row and per-column
  * phasors, FP16-split MFMA with FP32 accumulation) and a moment expansion
- * over the same patches (the tile sum as a polynomial of degree D = 5 or 6 in
- * the pixel offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle
- * phasors); opts->path = 0 picks the moment expansion, else the matrix-core
+ * over the same patches (the sum over a unit of 1, 2 or 4 consecutive tiles as
+ * a polynomial of degree D = 5 or 6 in the pixel offsets, from (D+1)(D+2)/2
+ * moments of the atoms' small-angle phasors; units of more than one tile are
+ * used where their radius, measured by mosaic_kin_stage, keeps every error
+ * bound); opts->path = 0 picks the moment expansion, else the matrix-core
  * form, else the recurrence, whenever their error bounds hold.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
@@ -157,7 +161,7 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const float* trans /* host */, int Ny, int Nz,
                    const double* coords, void* out, int out_is_c128,
                    const double* stage_stats /* host [2] */,
-                   const double* pixel_geom /* host [6] */, void* workspace,
+                   const double* pixel_geom /* host [8] */, void* workspace,
                    size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
 
 /*
@@ -180,7 +184,7 @@ int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
                        const float* rot /* host */, const float* trans /* host */, int Ny,
                        int Nz, const double* stage_stats /* host [2] */,
-                       const double* pixel_geom /* host [6] */, const float* grad_field,
+                       const double* pixel_geom /* host [8] */, const float* grad_field,
                        float* grad_pos, void* workspace, size_t workspace_bytes, int path,
                        mosaic_stream_t stream);
 

@@ -239,6 +239,7 @@ class beam:
             self._workspace = staged.workspace
             K.kin_sum(staged, spc_tab, opts, rot, trans, coords, out)
             self._last_path = staged.path_used
+            self._last_stats = (staged.radius_wl, tuple(staged.pixel_geom))
             staged_prev = staged
             del hold
         del staged_prev

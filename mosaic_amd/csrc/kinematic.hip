@@ -49,7 +49,7 @@ constexpr int MAX_SPECIES = 16;
 constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
-constexpr int N_GEOM = 6;            // pixel geometry statistics (mosaic_kin_pixels)
+constexpr int N_GEOM = 8;            // pixel geometry statistics (mosaic_kin_pixels)
 // The last k-d levels run inside one workgroup per node: a node of at most
 // KDL_MAX atoms sits in LDS and is bisected up to KDL_LEVELS times there
 // (k_kd_local<LV>, 32 << LV threads, for nodes of up to TILE << LV atoms).
@@ -382,8 +382,10 @@ __device__ __forceinline__ void origin_phase(double tx, double ty, double tz, do
 //  * per species, G = (f0(Q) + f' + i f'' [- forward term]) r_e sqrt(P) in
 //    the direction u0 (Beam.py:748-807 with the atom at the lab origin);
 //  * block statistics: nearest pixel distance, the largest first and second
-//    differences of u0 between neighbours along a row, and the largest
-//    deviation of u0 from the patch model of the matrix-core sum.
+//    differences of u0 between neighbours along a row and a column, the
+//    largest deviation of u0 from the separable patch model of the
+//    matrix-core sum and from the moment sum's model with the cross term
+//    a b g_yz, and the largest |g_yz|.
 __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ cz, int Ny, int Nz, const PixOpts O, PixelTables T) {
@@ -392,6 +394,7 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const double k = O.k;
     double rmin = 1e300, du1 = 0.0, du2 = 0.0, dpatch = 0.0, du2z = 0.0, du1z = 0.0;
+    double dpatchx = 0.0, gyzn = 0.0;
     if (p < npix) {
         const double tx = cx[p], ty = cy[p], tz = cz[p];
         double f, r, r_lo;
@@ -491,15 +494,30 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
         pixel_dir(cx, cy, cz, pm + (G.vz - 1 - G.mz) * Ny, e4[0], e4[1], e4[2]);
         const int oa = ix - G.y0 - G.my, ob = iy - G.z0 - G.mz;
         const double u0v[3] = {ux, uy, uz};
-        double dev2 = 0.0;
+        // and from the model with the cross term a b g_yz of the moment sum,
+        // g_yz the mixed difference of the patch's corners
+        double c00[3], c01[3], c10[3], c11[3];
+        const int pc = pm - G.my - G.mz * Ny;
+        pixel_dir(cx, cy, cz, pc, c00[0], c00[1], c00[2]);
+        pixel_dir(cx, cy, cz, pc + G.vy - 1, c01[0], c01[1], c01[2]);
+        pixel_dir(cx, cy, cz, pc + (G.vz - 1) * Ny, c10[0], c10[1], c10[2]);
+        pixel_dir(cx, cy, cz, pc + G.vy - 1 + (G.vz - 1) * Ny, c11[0], c11[1], c11[2]);
+        const double area = static_cast<double>(G.vy - 1) * static_cast<double>(G.vz - 1);
+        double dev2 = 0.0, devx2 = 0.0, g2 = 0.0;
         for (int d = 0; d < 3; ++d) {
             double gy, hy, gz, hz;
             quad_fit(e1[d] - mc[d], e2[d] - mc[d], -G.my, G.vy - 1 - G.my, gy, hy);
             quad_fit(e3[d] - mc[d], e4[d] - mc[d], -G.mz, G.vz - 1 - G.mz, gz, hz);
             const double model = mc[d] + oa * (gy + oa * hy) + ob * (gz + ob * hz);
             dev2 += (u0v[d] - model) * (u0v[d] - model);
+            const double gyz = area > 0.0 ? (c11[d] - c10[d] - c01[d] + c00[d]) / area : 0.0;
+            const double mx = model + static_cast<double>(oa) * ob * gyz;
+            devx2 += (u0v[d] - mx) * (u0v[d] - mx);
+            g2 += gyz * gyz;
         }
         dpatch = sqrt(dev2);
+        dpatchx = sqrt(devx2);
+        gyzn = sqrt(g2);
     }
     rmin = block_min(rmin, lds);
     du1 = block_max(du1, lds);
@@ -507,6 +525,8 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
     dpatch = block_max(dpatch, lds);
     du2z = block_max(du2z, lds);
     du1z = block_max(du1z, lds);
+    dpatchx = block_max(dpatchx, lds);
+    gyzn = block_max(gyzn, lds);
     if (threadIdx.x == 0) {
         T.stat_blocks[N_GEOM * blockIdx.x] = rmin;
         T.stat_blocks[N_GEOM * blockIdx.x + 1] = du1;
@@ -514,12 +534,14 @@ __global__ __launch_bounds__(PRO_THREADS) void k_pixels(
         T.stat_blocks[N_GEOM * blockIdx.x + 3] = dpatch;
         T.stat_blocks[N_GEOM * blockIdx.x + 4] = du2z;
         T.stat_blocks[N_GEOM * blockIdx.x + 5] = du1z;
+        T.stat_blocks[N_GEOM * blockIdx.x + 6] = dpatchx;
+        T.stat_blocks[N_GEOM * blockIdx.x + 7] = gyzn;
     }
 }
 
 __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
     __shared__ double lds[16];
-    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double v[N_GEOM] = {1e300, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         v[0] = fmin(v[0], in[N_GEOM * i]);
         for (int q = 1; q < N_GEOM; ++q) v[q] = fmax(v[q], in[N_GEOM * i + q]);
@@ -1836,25 +1858,31 @@ hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t 
 
 // ------------------------------------------------------------- moment sum
 // Far-field sum by moments over the same 32 x 32 patches and direction model
-// as the matrix-core sum.  With x = a / 16 and y = b / 16 (|x|, |y| <= 1) the
-// phase of atom j at pixel (a, b), relative to the patch's middle pixel, is
-//     2 pi theta_j + P_j x + Q_j x^2 + R_j y + S_j y^2   [rad],
+// as the matrix-core sum, with the model's cross term added.  With x = a / 16
+// and y = b / 16 (|x|, |y| <= 1) the phase of atom j at pixel (a, b), relative
+// to the patch's middle pixel, is
+//     2 pi theta_j + P_j x + Q_j x^2 + R_j y + S_j y^2 + U_j x y   [rad],
 //     theta_j = va(0).sigma_j + ph_j,  P_j = 2 pi 16 gy.sigma_j,  Q_j = 2 pi 256 hy.sigma_j,
-//     R_j = 2 pi 16 gz.sigma_j,  S_j = 2 pi 256 hz.sigma_j.
-// The host bounds phi = max |P x + R y| (a few tenths of a radian: the tile
-// radius times the patch's angular size) and |Q|, |S| (~1e-4 rad), so
-//     e^{i(P x + R y)} (1 + i Q x^2 + i S y^2)
-//         = sum_{m+n<=D} i^{m+n} [T_mn - i (Q T_{m-2,n} + S T_{m,n-2})] x^m y^n,
+//     R_j = 2 pi 16 gz.sigma_j,  S_j = 2 pi 256 hz.sigma_j,  U_j = 2 pi 256 gyz.sigma_j,
+// gyz the mixed difference of the patch's corner directions (the separable
+// model leaves a b gyz out; with it the model's deviation drops about tenfold
+// on a flat detector).  The host bounds phi = max |P x + R y| (a few tenths of
+// a radian: the radius times the patch's angular size), |Q|, |S| (~1e-4 rad)
+// and |U| (~1e-5 rad), so
+//     e^{i(P x + R y)} (1 + i Q x^2 + i S y^2 + i U x y)
+//         = sum_{m+n<=D} i^{m+n} [T_mn - i (Q T_{m-2,n} + S T_{m,n-2} + U T_{m-1,n-1})] x^m y^n,
 //     T_mn = P^m R^n / (m! n!),
 // to phi^{D+1} / (D+1)! (the Q, S terms kept where the rest of their monomial
-// has degree <= 3).  The tile sum over the patch is then a polynomial,
+// has degree <= 3, the U terms where it has degree <= 2).  The sum over a unit
+// of 1, 2 or 4 tiles (their atoms' offsets taken from the unit's origin) over
+// the patch is then a polynomial,
 //     S_t(x, y) = sum_{m+n<=D} C_mn x^m y^n,  C_mn = sum_j |w_j| e^{2 pi i theta_j} (...)_mn,
 // whose (D+1)(D+2)/2 complex moments cost one sincos and ~50 FMAs per atom
 // and patch (the matrix-core form builds 64 phasors per atom and patch).
-// Each lane accumulates the moments of 4 of the tile's atoms; the wave sums
-// them through LDS; every lane then evaluates its 16 pixels by Horner's
-// rule and applies the per-(tile, pixel) origin phase as the matrix-core
-// sum does; the pixel's table factor (a function of species and pixel only)
+// Each lane accumulates the moments of 4 atoms of each tile of the unit; at
+// the unit's last tile the wave sums them through LDS; every lane then
+// evaluates its 16 pixels by Horner's rule and applies the per-(unit, pixel)
+// origin phase as the matrix-core sum does per tile; the pixel's table factor (a function of species and pixel only)
 // multiplies the running sums when they are flushed.
 constexpr int POLY_WAVES = 4;
 
@@ -1894,13 +1922,15 @@ void k_sum_poly(const MmaArgs A) {
 
     // the patch's direction model (sample frame), as in the matrix-core sum
     double vc0[3];
-    float gyk[3], hyk[3], gzk[3], hzk[3];
+    float gyk[3], hyk[3], gzk[3], hzk[3], uyz[3];
     {
-        const int samp[5] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
-                             pmid + (P.vz - 1 - P.mz) * Ny};
-        double vs[5][3];
+        const int pc = pmid - P.my - P.mz * Ny;
+        const int samp[9] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                             pmid + (P.vz - 1 - P.mz) * Ny, pc, pc + P.vy - 1,
+                             pc + (P.vz - 1) * Ny, pc + P.vy - 1 + (P.vz - 1) * Ny};
+        double vs[9][3];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < 9; ++k) {
             const double4 u = A.u0h[samp[k]];
             const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
             vs[k][0] = R00 * wx + R10 * wy + R20 * wz;
@@ -1917,6 +1947,10 @@ void k_sum_poly(const MmaArgs A) {
             quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gq, hq);
             gzk[d] = static_cast<float>(2.0 * PI * 16.0 * gq);
             hzk[d] = static_cast<float>(2.0 * PI * 256.0 * hq);
+            // cross term a b g_yz from the mixed difference of the corners
+            const double area = static_cast<double>(P.vy - 1) * static_cast<double>(P.vz - 1);
+            const double gyz = area > 0.0 ? (vs[8][d] - vs[7][d] - vs[6][d] + vs[5][d]) / area : 0.0;
+            uyz[d] = static_cast<float>(2.0 * PI * 256.0 * gyz);
         }
     }
     const double4 um = A.u0h[pmid];
@@ -2040,6 +2074,7 @@ void k_sum_poly(const MmaArgs A) {
             const float Rj = fmaf(gzk[2], sz, fmaf(gzk[1], sy, gzk[0] * sx));
             const float Qj = fmaf(hyk[2], sz, fmaf(hyk[1], sy, hyk[0] * sx));
             const float Sj = fmaf(hzk[2], sz, fmaf(hzk[1], sy, hzk[0] * sx));
+            const float Uj = fmaf(uyz[2], sz, fmaf(uyz[1], sy, uyz[0] * sx));
             float am[D + 1], bn[D + 1];
             am[0] = 1.f;
             bn[0] = 1.f;
@@ -2057,6 +2092,7 @@ void k_sum_poly(const MmaArgs A) {
                     float U = 0.f;
                     if (m >= 2 && m - 2 + n <= 3) U = fmaf(Qj, am[m - 2] * bn[n], U);
                     if (n >= 2 && m + n - 2 <= 3) U = fmaf(Sj, am[m] * bn[n - 2], U);
+                    if (m >= 1 && n >= 1 && m + n - 2 <= 2) U = fmaf(Uj, am[m - 1] * bn[n - 1], U);
                     cr[k] = fmaf(wr, T, fmaf(wi, U, cr[k]));
                     ci[k] = fmaf(wi, T, fmaf(-wr, U, ci[k]));
                 }
@@ -3339,7 +3375,10 @@ double rec_curv_bound(int kp, double d2u, double radius_wl) {
 }  // namespace
 
 // Degree of the moment expansion (k_sum_poly, k_adjoint_poly), 0 where it
-// does not apply: the matrix-core form's model bounds, plus the truncation.
+// does not apply: patch_ok carries the direction model's bounds (the
+// separable model's for the adjoint, the model with the cross term for the
+// sum, which also passes cross to bound the cross term's truncation), plus
+// the truncation.
 // phi bounds |P x + R y| over a patch from the largest first differences
 // along a row and a column (16 pixels each way, with the curvature); qmax
 // bounds the quadratic model terms |Q|, |S|.  Both truncations are per-atom
@@ -3348,8 +3387,9 @@ double rec_curv_bound(int kp, double d2u, double radius_wl) {
 // The moment sum also takes the per-(tile, pixel) origin phase,
 // K |O| |u_p - u_mid| < K omax 16 sqrt(2) (du_row + du_col) revolutions,
 // into v_sin / v_cos without reduction, so that must stay within their range.
-static int poly_degree(bool mma_ok, double radius_wl, const double* pixel_geom, double k_omax) {
-    if (!mma_ok) return 0;
+static int poly_degree(bool patch_ok, double radius_wl, const double* pixel_geom, double k_omax,
+                       bool cross = false) {
+    if (!patch_ok) return 0;
     const double du1 = pixel_geom[1], du2 = pixel_geom[2];
     const double du2z = pixel_geom[4], du1z = pixel_geom[5];
     if (k_omax * 16.0 * 1.4142135623730951 * (du1 + du1z + 16.0 * (du2 + du2z)) + 1.0 > 200.0)
@@ -3357,6 +3397,10 @@ static int poly_degree(bool mma_ok, double radius_wl, const double* pixel_geom, 
     const double phi = 2.0 * PI * 16.0 * ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
     const double qmax = 2.0 * PI * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
     if (qmax > 3e-4 || qmax * phi * phi * phi * phi / 24.0 > 1e-6) return 0;
+    if (cross) {   // the cross term, kept where the rest of its monomial has degree <= 2
+        const double umax = 2.0 * PI * 256.0 * pixel_geom[7] * radius_wl;
+        if (umax > 3e-4 || umax * phi * phi * phi / 6.0 > 1e-6) return 0;
+    }
     auto trunc = [&](int D) {
         double t = 1.0;
         for (int i = 1; i <= D + 1; ++i) t *= phi / i;
@@ -3434,7 +3478,13 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     const bool mma_ok = far && dpatch * radius_wl <= 1e-6 &&
                         K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
                         2.0 * radius_wl + 2.0 <= 256.0;
-    const int poly_d = poly_degree(mma_ok, radius_wl, pixel_geom, K * omax);
+    // moment sum: the same bounds with the deviation from the model that
+    // includes the cross term
+    auto poly_patch_ok = [&](double r) {
+        return far && pixel_geom[6] * r <= 1e-6 &&
+               K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 && 2.0 * r + 2.0 <= 256.0;
+    };
+    const int poly_d = poly_degree(poly_patch_ok(radius_wl), radius_wl, pixel_geom, K * omax, true);
     int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
     if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
         return MOSAIC_ERR_UNSUPPORTED;
@@ -3447,9 +3497,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
             const double rm = r / K;
             if (rm / rho_min > 1e-6 || K * rm * rm * rm / (rho_min * rho_min) > 1e-7) return false;
             if (!use_second && r * r / (2.0 * K * rho_min) > 1e-7) return false;
-            const bool ok = far && dpatch * r <= 1e-6 &&
-                            K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 && 2.0 * r + 2.0 <= 256.0;
-            const int d = poly_degree(ok, r, pixel_geom, K * omax);
+            const int d = poly_degree(poly_patch_ok(r), r, pixel_geom, K * omax, true);
             return d != 0 && d <= poly_d;
         };
         double lo = radius_wl, hi = 256.0;

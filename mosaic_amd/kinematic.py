@@ -159,7 +159,7 @@ def kin_pixels(staged, species_tab, opts, coords, stream=None):
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
         raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
     coords = coords.contiguous()
-    geom = (ctypes.c_double * 6)()
+    geom = (ctypes.c_double * 8)()
     check(_lib.lib().mosaic_kin_pixels(len(species_tab), ctypes.cast(species_tab, ctypes.c_void_p),
                                        ctypes.byref(opts), staged.Ny, staged.Nz, _ptr(coords),
                                        _ptr(staged.workspace), int(staged.workspace.numel()),

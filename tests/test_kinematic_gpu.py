@@ -274,3 +274,44 @@ def test_auto_path_choice(device):
     R = oracle_field(w, "f80")
     e, _ = report("near detector auto", A, R)
     assert e <= TOL
+
+
+class _PixelBlock:
+    """A rectangular block of another detector's pixels, at the same positions."""
+
+    def __init__(self, det, y0, z0, ny, nz):
+        Ny, Nz = det.shape
+        c = np.asarray(det.pixel_coordinates).reshape(3, Nz, Ny)
+        self.pixel_coordinates = np.ascontiguousarray(c[:, z0:z0 + nz, y0:y0 + ny].reshape(3, -1))
+        self.shape = (ny, nz)
+
+
+def test_moment_sum_cross_term_regime(device):
+    """Corner pixels of a 512^2, 75 um detector at 1 m and an FCC crystal
+    stretched 1.5 times with 0.5 A random displacements (largest tile ~28
+    wavelengths; the disorder keeps the field diffuse, where a perfect
+    stretched lattice would cancel to ~1e-3 of sqrt(N) f and leave the
+    relative error ill-conditioned): the separable patch model's deviation
+    times the radius exceeds 1e-6 (the matrix-core form declines), the model
+    with the cross term a b g_yz stays below it, and the moment sum matches
+    the long-double oracle."""
+    from mosaic_amd import sample as S
+    w = W.fcc_au_bragg111(n=12, npix=8, sigma=0.0)
+    rng = np.random.default_rng(3)
+    base = w["sample"].load_chunk_positions(1)
+    pos = (base * 1.5 + rng.normal(0.0, 0.5, base.shape)).astype(np.float32)
+    w["sample"] = S.sample([pos], [np.zeros(pos.shape[0], np.uint8)], ["Au"])
+    w["detector"] = _PixelBlock(W._detector(512, 2 * w["theta_b"]), 0, 0, 64, 64)
+    A = gpu_field(w)
+    r, geom = w["beam"]._last_stats
+    print(f"[cross] radius {r:.2f} wl, separable {geom[3] * r:.2e}, with cross term "
+          f"{geom[6] * r:.2e}, path {w['beam']._last_path}", flush=True)
+    assert geom[3] * r > 1e-6 >= geom[6] * r
+    assert w["beam"]._last_path % 100 in (45, 46)
+    # the stretched crystal samples the beam's amplitude map where it is not flat
+    bm = w["beam"]
+    amp = O.ein_bilinear_numpy(pos, bm._beam_E0_map, bm._beam_du, bm._beam_dv, bm._beam_uc,
+                               bm._beam_vc)
+    R = oracle_field(w, "f80", amp=amp)
+    e, _ = report("cross-term regime (moment sum)", A, R)
+    assert e <= TOL
