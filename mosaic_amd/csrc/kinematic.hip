@@ -561,10 +561,11 @@ __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __r
 // boxes of near-equal count and bounded aspect, so the tile radius stays near
 // that of a cube of 256 atoms whatever the crystal's shape or orientation (a
 // space-filling-curve order leaves runs that wander through the empty corners
-// of the bounding box: radius up to ~120 wavelengths instead of ~20).  One
-// level = per-node bounding boxes (integer atomics on order-preserving images
-// of the floats), a key (node, coordinate along the node's longest side), one
-// stable radix sort over all nodes at once, and the nodes of the next level.
+// of the bounding box: radius up to ~120 wavelengths instead of ~20).  The
+// global levels are one key pass (node, coordinate along the node's longest
+// side) and one stable radix sort over all nodes at once; the last KDL_LEVELS
+// levels and the tile records run in LDS, one workgroup per node
+// (k_kd_local).
 
 constexpr int KD_QBITS = 20;       // coordinate bits of a 64-bit k-d sort key
 constexpr int KD_QBITS32_MIN = 6;  // least coordinate bits of a 32-bit one
