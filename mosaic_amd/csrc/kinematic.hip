@@ -2076,26 +2076,49 @@ void k_sum_poly(const MmaArgs A) {
             const float Qj = fmaf(hyk[2], sz, fmaf(hyk[1], sy, hyk[0] * sx));
             const float Sj = fmaf(hzk[2], sz, fmaf(hzk[1], sy, hzk[0] * sx));
             const float Uj = fmaf(uyz[2], sz, fmaf(uyz[1], sy, uyz[0] * sx));
-            float am[D + 1], bn[D + 1];
-            am[0] = 1.f;
+            // w P^m / m! (ar, ai) against R^n / n! (bn): two FMAs per moment;
+            // the corrections -i w (Q T_{m-2,n} + S T_{m,n-2} + U T_{m-1,n-1})
+            // against Q R^n / n!, S R^n / n!, U R^n / n!
+            float bn[D + 1], ar[D + 1], ai[D + 1];
             bn[0] = 1.f;
+            ar[0] = wr;
+            ai[0] = wi;
+            float pm = 1.f;
 #pragma unroll
             for (int m = 1; m <= D; ++m) {
-                am[m] = am[m - 1] * Pj * (1.f / m);
+                pm = pm * Pj * (1.f / m);
+                ar[m] = wr * pm;
+                ai[m] = wi * pm;
                 bn[m] = bn[m - 1] * Rj * (1.f / m);
+            }
+            float qb[4], sb[4], ub[3];
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                qb[n] = Qj * bn[n];
+                sb[n] = Sj * bn[n];
+                if (n < 3) ub[n] = Uj * bn[n];
             }
             int k = 0;
 #pragma unroll
             for (int m = 0; m <= D; ++m) {
 #pragma unroll
                 for (int n = 0; n <= D - m; ++n, ++k) {
-                    const float T = am[m] * bn[n];
-                    float U = 0.f;
-                    if (m >= 2 && m - 2 + n <= 3) U = fmaf(Qj, am[m - 2] * bn[n], U);
-                    if (n >= 2 && m + n - 2 <= 3) U = fmaf(Sj, am[m] * bn[n - 2], U);
-                    if (m >= 1 && n >= 1 && m + n - 2 <= 2) U = fmaf(Uj, am[m - 1] * bn[n - 1], U);
-                    cr[k] = fmaf(wr, T, fmaf(wi, U, cr[k]));
-                    ci[k] = fmaf(wi, T, fmaf(-wr, U, ci[k]));
+                    float xr = fmaf(ar[m], bn[n], cr[k]);
+                    float xi = fmaf(ai[m], bn[n], ci[k]);
+                    if (m >= 2 && m - 2 + n <= 3) {
+                        xr = fmaf(ai[m - 2], qb[n], xr);
+                        xi = fmaf(-ar[m - 2], qb[n], xi);
+                    }
+                    if (n >= 2 && m + n - 2 <= 3) {
+                        xr = fmaf(ai[m], sb[n - 2], xr);
+                        xi = fmaf(-ar[m], sb[n - 2], xi);
+                    }
+                    if (m >= 1 && n >= 1 && m + n - 2 <= 2) {
+                        xr = fmaf(ai[m - 1], ub[n - 1], xr);
+                        xi = fmaf(-ar[m - 1], ub[n - 1], xi);
+                    }
+                    cr[k] = xr;
+                    ci[k] = xi;
                 }
             }
         }
