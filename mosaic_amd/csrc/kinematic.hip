@@ -1874,7 +1874,7 @@ hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t 
 //         = sum_{m+n<=D} i^{m+n} [T_mn - i (Q T_{m-2,n} + S T_{m,n-2} + U T_{m-1,n-1})] x^m y^n,
 //     T_mn = P^m R^n / (m! n!),
 // to phi^{D+1} / (D+1)! (the Q, S terms kept where the rest of their monomial
-// has degree <= 3, the U terms where it has degree <= 2).  The sum over a unit
+// has degree <= 3, the U terms where it has degree <= 1).  The sum over a unit
 // of 1, 2 or 4 tiles (their atoms' offsets taken from the unit's origin) over
 // the patch is then a polynomial,
 //     S_t(x, y) = sum_{m+n<=D} C_mn x^m y^n,  C_mn = sum_j |w_j| e^{2 pi i theta_j} (...)_mn,
@@ -2083,20 +2083,19 @@ void k_sum_poly(const MmaArgs A) {
             bn[0] = 1.f;
             ar[0] = wr;
             ai[0] = wi;
-            float pm = 1.f;
 #pragma unroll
             for (int m = 1; m <= D; ++m) {
-                pm = pm * Pj * (1.f / m);
-                ar[m] = wr * pm;
-                ai[m] = wi * pm;
+                const float pq = Pj * (1.f / m);
+                ar[m] = ar[m - 1] * pq;
+                ai[m] = ai[m - 1] * pq;
                 bn[m] = bn[m - 1] * Rj * (1.f / m);
             }
-            float qb[4], sb[4], ub[3];
+            float qb[4], sb[4], ub[2];
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
                 qb[n] = Qj * bn[n];
                 sb[n] = Sj * bn[n];
-                if (n < 3) ub[n] = Uj * bn[n];
+                if (n < 2) ub[n] = Uj * bn[n];
             }
             int k = 0;
 #pragma unroll
@@ -2113,7 +2112,7 @@ void k_sum_poly(const MmaArgs A) {
                         xr = fmaf(ai[m], sb[n - 2], xr);
                         xi = fmaf(-ar[m], sb[n - 2], xi);
                     }
-                    if (m >= 1 && n >= 1 && m + n - 2 <= 2) {
+                    if (m >= 1 && n >= 1 && m + n - 2 <= 1) {
                         xr = fmaf(ai[m - 1], ub[n - 1], xr);
                         xi = fmaf(-ar[m - 1], ub[n - 1], xi);
                     }
@@ -3421,9 +3420,9 @@ static int poly_degree(bool patch_ok, double radius_wl, const double* pixel_geom
     const double phi = 2.0 * PI * 16.0 * ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * radius_wl;
     const double qmax = 2.0 * PI * 256.0 * 0.5 * std::max(du2, du2z) * radius_wl;
     if (qmax > 3e-4 || qmax * phi * phi * phi * phi / 24.0 > 1e-6) return 0;
-    if (cross) {   // the cross term, kept where the rest of its monomial has degree <= 2
+    if (cross) {   // the cross term, kept where the rest of its monomial has degree <= 1
         const double umax = 2.0 * PI * 256.0 * pixel_geom[7] * radius_wl;
-        if (umax > 3e-4 || umax * phi * phi * phi / 6.0 > 1e-6) return 0;
+        if (umax > 3e-4 || umax * phi * phi / 2.0 > 1e-6) return 0;
     }
     auto trunc = [&](int D) {
         double t = 1.0;
