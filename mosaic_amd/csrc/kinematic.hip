@@ -71,6 +71,35 @@ struct TileRec {      // 32 B
     int count;
 };
 
+// A global level works on chunks of KDG_CHUNK positions of one node each,
+// listed by the host from the count-only plan: k_kdg_bin bins every atom's
+// coordinate along its node's longest side into KDG_BINS bins and writes the
+// chunk's histogram, k_kdg_select finds per node the bin holding the cut and
+// each chunk's counts before it, and k_kdg_scatter writes the stable
+// partition (the cut bin's first positions go left until the count is met).
+// The halves overlap by at most one bin of the parent's side.
+constexpr int KDG_THREADS = 256;
+constexpr int KDG_PT = 8;
+constexpr int KDG_CHUNK = KDG_THREADS * KDG_PT;
+constexpr int KDG_BINS = 256;
+static_assert(KDG_BINS == KDG_THREADS, "one bin per thread in k_kdg_select");
+
+struct KdgNode {      // a node of a global level (heap numbering)
+    uint32_t begin;   // positions [begin, begin + count)
+    uint32_t count;
+    uint32_t need;    // positions of the left child
+    uint32_t blk0;    // its chunks [blk0, blk0 + nblk) in the level's list
+    uint32_t nblk;
+    uint32_t pad[3];
+};
+
+struct KdgChunk {
+    uint32_t node;
+    uint32_t begin;
+    uint32_t end;
+    uint32_t pad;
+};
+
 struct KdNode {       // a k-d node finished in LDS: positions [begin, begin + count)
     uint32_t begin;
     uint32_t count;
@@ -118,16 +147,22 @@ struct Layout {
     TileRec* tiles;
     AtomRec* atoms;
     float* grp_r;           // [tile][2] radius of the pair / quad headed by the tile, else +inf
-    uint64_t* keys_a;
-    uint64_t* keys_b;
-    uint32_t* vals_a;
+    uint32_t* keys_a;       // species sort
+    uint32_t* keys_b;
+    uint32_t* vals_a;       // orders (atom index per position)
     uint32_t* vals_b;
     void* sort_tmp;
     size_t sort_bytes;
-    uint32_t* seg;          // k-d node of every position
-    unsigned int* kd_box;   // [node][6] bounding boxes of level 0 (order-preserving integers)
+    unsigned int* kd_box;   // [node][6] tight bounding boxes of a level (order-preserving integers)
     float* kd_fbox[2];      // [node][6] float boxes (lo, hi) of two successive levels
-    uint32_t* kd_split;     // cut positions of every level
+    KdgNode* kdg_nodes;     // nodes of every global level
+    KdgChunk* kdg_chunks;   // chunks of every global level
+    uint8_t* kdg_q8;        // bin of every position
+    uint32_t* kdg_bh;       // [chunk][KDG_BINS] histograms of one level
+    uint2* kdg_cnt;         // per chunk of one level: its counts below / in the cut bin
+    uint2* kdg_boff;        // per chunk of one level: the same counts before it in its node
+    uint32_t* kdg_part;     // [node][slice][KDG_BINS] slice sums of the node histograms
+    int2* kdg_sel;          // per node of one level: cut bin, how many of it go left
     KdNode* kd_nodes;       // nodes handed to the in-LDS finish
     int kd_levels;
     unsigned int* radius_bits;
@@ -175,49 +210,107 @@ int kd_levels(int n_species, const int64_t* counts) {
     return levels;
 }
 
-// Nodes over all levels (level l holds n_species << l of them).
-size_t kd_split_count(int n_species, int levels) {
-    return static_cast<size_t>(n_species) * ((size_t(1) << levels) - 1);
-}
-
-// Cut positions of the k-d bisection, level by level, in heap numbering
+// Node ranges of the k-d bisection for levels 0..levels, in heap numbering
 // (node h of a level has children 2h and 2h + 1 on the next): a node of n > T
-// atoms keeps ceil(n / 2T) T on the left; NO_SPLIT where a node holds one
-// tile or less (it passes whole to its left child).  The position ranges of
-// the nodes below the last level are left in [begin, end).
-void kd_plan(int n_species, const int64_t* counts, int levels, std::vector<uint32_t>& split,
-             std::vector<int64_t>& begin, std::vector<int64_t>& end) {
-    std::vector<int64_t> b(n_species), e(n_species);
+// atoms keeps ceil(n / 2T) T on the left; a node of one tile or less passes
+// whole to its left child.
+void kd_plan_levels(int n_species, const int64_t* counts, int levels,
+                    std::vector<std::vector<int64_t>>& B, std::vector<std::vector<int64_t>>& E) {
+    B.assign(levels + 1, {});
+    E.assign(levels + 1, {});
     int64_t a = 0;
     for (int s = 0; s < n_species; ++s) {
-        b[s] = a;
+        B[0].push_back(a);
         a += counts[s];
-        e[s] = a;
+        E[0].push_back(a);
     }
-    split.clear();
-    split.reserve(kd_split_count(n_species, levels));
     for (int l = 0; l < levels; ++l) {
-        const size_t nodes = b.size();
-        std::vector<int64_t> nb(2 * nodes), ne(2 * nodes);
-        for (size_t h = 0; h < nodes; ++h) {
-            const int64_t n = e[h] - b[h];
-            int64_t cut = e[h];
-            if (n > TILE) {
-                cut = b[h] + (n + 2 * TILE - 1) / (2 * TILE) * TILE;
-                split.push_back(static_cast<uint32_t>(cut));
-            } else {
-                split.push_back(NO_SPLIT);
-            }
-            nb[2 * h] = b[h];
-            ne[2 * h] = cut;
-            nb[2 * h + 1] = cut;
-            ne[2 * h + 1] = e[h];
+        for (size_t h = 0; h < B[l].size(); ++h) {
+            const int64_t b = B[l][h], e = E[l][h], n = e - b;
+            const int64_t cut = n > TILE ? b + (n + 2 * TILE - 1) / (2 * TILE) * TILE : e;
+            B[l + 1].push_back(b);
+            E[l + 1].push_back(cut);
+            B[l + 1].push_back(cut);
+            E[l + 1].push_back(e);
         }
-        b.swap(nb);
-        e.swap(ne);
     }
-    begin.swap(b);
-    end.swap(e);
+}
+
+// Global levels: the k-d levels above the last KDL_LEVELS.
+int kd_global_levels(int n_species, const int64_t* counts) {
+    return std::max(0, kd_levels(n_species, counts) - KDL_LEVELS);
+}
+
+// Slices over which k_kdg_total sums a level's node histograms: about 16
+// chunks each for the level's largest node.
+int kdg_slices(const std::vector<KdgNode>& nodes, size_t n0, size_t n1) {
+    uint32_t mx = 1;
+    for (size_t i = n0; i < n1; ++i) mx = std::max(mx, nodes[i].nblk);
+    return static_cast<int>(std::min<uint32_t>(64u, (mx + 15u) / 16u));
+}
+
+// Workspace sizes of the global levels' tables from the plan alone: nodes
+// and chunks over all levels, and per level the largest chunk count, node
+// count and node x slice count (kdg_slices).
+struct KdgSizes {
+    size_t nodes = 0, chunks = 0, max_chunks = 1, max_nodes = 1, max_part = 1;
+};
+KdgSizes kdg_sizes(int n_species, const int64_t* counts) {
+    const int G = kd_global_levels(n_species, counts);
+    std::vector<std::vector<int64_t>> B, E;
+    kd_plan_levels(n_species, counts, G, B, E);
+    KdgSizes z;
+    for (int l = 0; l < G; ++l) {
+        size_t ch = 0;
+        uint32_t mx = 1;
+        for (size_t h = 0; h < B[l].size(); ++h) {
+            const uint32_t nb = static_cast<uint32_t>((E[l][h] - B[l][h] + KDG_CHUNK - 1) / KDG_CHUNK);
+            ch += nb;
+            mx = std::max(mx, nb);
+        }
+        const size_t S = std::min<uint32_t>(64u, (mx + 15u) / 16u);
+        z.nodes += B[l].size();
+        z.chunks += ch;
+        z.max_chunks = std::max(z.max_chunks, ch);
+        z.max_nodes = std::max(z.max_nodes, B[l].size());
+        z.max_part = std::max(z.max_part, B[l].size() * S);
+    }
+    return z;
+}
+
+// Node and chunk tables of the global levels; per level the offsets of its
+// nodes and chunks in them.
+void kdg_tables(int n_species, const int64_t* counts, std::vector<KdgNode>& nodes,
+                std::vector<KdgChunk>& chunks, std::vector<size_t>& node_off,
+                std::vector<size_t>& chunk_off) {
+    const int G = kd_global_levels(n_species, counts);
+    std::vector<std::vector<int64_t>> B, E;
+    kd_plan_levels(n_species, counts, G, B, E);
+    nodes.clear();
+    chunks.clear();
+    node_off.assign(G + 1, 0);
+    chunk_off.assign(G + 1, 0);
+    for (int l = 0; l < G; ++l) {
+        node_off[l] = nodes.size();
+        chunk_off[l] = chunks.size();
+        for (size_t h = 0; h < B[l].size(); ++h) {
+            const int64_t b = B[l][h], n = E[l][h] - b;
+            KdgNode nd = {};
+            nd.begin = static_cast<uint32_t>(b);
+            nd.count = static_cast<uint32_t>(n);
+            nd.need = static_cast<uint32_t>(n > TILE ? (n + 2 * TILE - 1) / (2 * TILE) * TILE : n);
+            nd.blk0 = static_cast<uint32_t>(chunks.size() - chunk_off[l]);
+            for (int64_t c0 = b; c0 < b + n; c0 += KDG_CHUNK) {
+                KdgChunk c = {static_cast<uint32_t>(h), static_cast<uint32_t>(c0),
+                              static_cast<uint32_t>(std::min<int64_t>(c0 + KDG_CHUNK, b + n)), 0u};
+                chunks.push_back(c);
+            }
+            nd.nblk = static_cast<uint32_t>(chunks.size() - chunk_off[l]) - nd.blk0;
+            nodes.push_back(nd);
+        }
+    }
+    node_off[G] = nodes.size();
+    chunk_off[G] = chunks.size();
 }
 
 int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
@@ -229,28 +322,33 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.atoms = c.take<AtomRec>(static_cast<size_t>(std::max(1, L.n_tiles)) * TILE);
     L.grp_r = c.take<float>(2 * static_cast<size_t>(std::max(1, L.n_tiles)));
     size_t n = static_cast<size_t>(std::max<int64_t>(1, n_atoms));
-    L.keys_a = c.take<uint64_t>(n);
-    L.keys_b = c.take<uint64_t>(n);
+    L.keys_a = c.take<uint32_t>(n);
+    L.keys_b = c.take<uint32_t>(n);
     L.vals_a = c.take<uint32_t>(n);
     L.vals_b = c.take<uint32_t>(n);
-    size_t sort_bytes = 0, sort_bytes32 = 0;
-    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+    size_t sort_bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                   (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  static_cast<unsigned int>(n), 0, 64) != hipSuccess ||
-        rocprim::radix_sort_pairs(nullptr, sort_bytes32, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  static_cast<unsigned int>(n), 0, 32) != hipSuccess)
+                                  static_cast<unsigned int>(n), 0, 4) != hipSuccess)
         return MOSAIC_ERR_HIP;
-    sort_bytes = std::max(sort_bytes, sort_bytes32);
     L.sort_bytes = sort_bytes;
     L.sort_tmp = c.take<char>(sort_bytes);
-    L.seg = c.take<uint32_t>(n);
     L.kd_levels = kd_levels(n_species, counts);
     const size_t top = static_cast<size_t>(n_species) << std::max(0, L.kd_levels - 1);
     L.kd_box = c.take<unsigned int>(6 * top);
     L.kd_fbox[0] = c.take<float>(12 * top);
     L.kd_fbox[1] = c.take<float>(12 * top);
-    L.kd_split = c.take<uint32_t>(kd_split_count(n_species, L.kd_levels) + 1);
+    {
+        const KdgSizes z = kdg_sizes(n_species, counts);
+        L.kdg_nodes = c.take<KdgNode>(std::max<size_t>(1, z.nodes));
+        L.kdg_chunks = c.take<KdgChunk>(std::max<size_t>(1, z.chunks));
+        L.kdg_q8 = c.take<uint8_t>(n);
+        L.kdg_bh = c.take<uint32_t>(z.max_chunks * KDG_BINS);
+        L.kdg_cnt = c.take<uint2>(z.max_chunks);
+        L.kdg_boff = c.take<uint2>(z.max_chunks);
+        L.kdg_part = c.take<uint32_t>(z.max_part * KDG_BINS);
+        L.kdg_sel = c.take<int2>(z.max_nodes);
+    }
     L.kd_nodes = c.take<KdNode>(std::max(1, L.n_tiles));
     L.radius_bits = c.take<unsigned int>(2);
     L.amax_blocks = c.take<float>(2 * AMAX_BLOCKS);
@@ -567,25 +665,13 @@ __global__ void k_stats_reduce(const double* __restrict__ in, int n, double* __r
 // levels and the tile records run in LDS, one workgroup per node
 // (k_kd_local).
 
-constexpr int KD_QBITS = 20;       // coordinate bits of a 64-bit k-d sort key
-constexpr int KD_QBITS32_MIN = 6;  // least coordinate bits of a 32-bit one
 // Global levels whose boxes are measured over the atoms: the first and the
-// last KD_TIGHT_LEVELS; the others cut their parents' boxes at the sorted
-// keys.  Loose boxes near the leaves let an elongated node through (on the
-// 5M-atom perovskite the largest tile grows from 21.7 to 28 wavelengths),
-// tight ones there keep it at 21.8.
+// last KD_TIGHT_LEVELS; the others cut their parents' boxes at the cut bin.
+// Loose boxes near the leaves let an elongated node through (on the 5M-atom
+// perovskite the largest tile grows from 22 to 28 wavelengths), tight ones
+// there keep it near 22.
 constexpr int KD_TIGHT_LEVELS = 2;
 
-// Coordinate bits of a 32-bit key whose node id has nbits bits: the key is
-// filled up to the next multiple of the sort's 8-bit digit, keeping at least
-// KD_QBITS32_MIN bits.  The count of every cut is exact whatever the bits;
-// they only set how far (one bin of the node's longest side) the two halves
-// may overlap.
-int kd_qbits32(int nbits) {
-    int total = 8;
-    while (total - nbits < KD_QBITS32_MIN) total += 8;
-    return total - nbits;
-}
 
 __device__ __forceinline__ unsigned int f2ord(float f) {
     const unsigned int b = __float_as_uint(f);
@@ -608,9 +694,8 @@ __global__ void k_stage_init(unsigned int* __restrict__ radius_bits) {
 }
 
 // Group by species (stable): key = species code.
-template <typename KeyT>
 __global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
-                               KeyT* __restrict__ keys, uint32_t* __restrict__ vals) {
+                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     keys[i] = spc[i] & 0xF;
@@ -618,108 +703,6 @@ __global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
 }
 
 constexpr int KD_THREADS = 256;
-
-// Bounding boxes of the nodes that split at this level.  Node ids do not
-// decrease along the order, so a 256-atom chunk whose first and last
-// positions share a node reduces locally: a workgroup walks KD_CHUNKS chunks
-// and keeps a running box while the node stays the same, issuing 6 atomics
-// per node it leaves (the top levels have a handful of nodes, so atomics per
-// chunk would serialise on them); chunks that straddle nodes fall back to
-// per-wave or per-lane atomics.
-constexpr int KD_CHUNKS = 16;
-
-__device__ __forceinline__ void kd_box_flush(uint32_t node, unsigned int (&lo)[3],
-                                             unsigned int (&hi)[3], unsigned int (*red)[6],
-                                             const uint32_t* __restrict__ split,
-                                             unsigned int* __restrict__ box) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = wave_umin(lo[d]);
-        hi[d] = wave_umax(hi[d]);
-    }
-    if (lane == 0)
-        for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
-    __syncthreads();
-    if (threadIdx.x == 0 && split[node] != NO_SPLIT) {
-        for (int d = 0; d < 3; ++d) {
-            unsigned int a = red[0][d], b = red[0][3 + d];
-            for (int w = 1; w < KD_THREADS / 64; ++w) {
-                a = min(a, red[w][d]);
-                b = max(b, red[w][3 + d]);
-            }
-            atomicMin(box + 6 * static_cast<int64_t>(node) + d, a);
-            atomicMax(box + 6 * static_cast<int64_t>(node) + 3 + d, b);
-        }
-    }
-    __syncthreads();
-    for (int d = 0; d < 3; ++d) { lo[d] = 0xFFFFFFFFu; hi[d] = 0u; }
-}
-
-__global__ __launch_bounds__(KD_THREADS) void k_kd_box(
-    const float* __restrict__ pos, const uint32_t* __restrict__ order,
-    const uint32_t* __restrict__ seg, const uint32_t* __restrict__ prev_split,
-    const uint32_t* __restrict__ split, int64_t n, unsigned int* __restrict__ box) {
-    __shared__ unsigned int red[KD_THREADS / 64][6];
-    // node of position j: seg holds the previous level's ids when prev_split
-    // is given (children 2s left of the previous cut, 2s + 1 right of it)
-    auto node_at = [&](int64_t j) {
-        const uint32_t s = seg[j];
-        return prev_split ? 2u * s + (static_cast<uint64_t>(j) >= prev_split[s] ? 1u : 0u) : s;
-    };
-    const int lane = threadIdx.x & 63;
-    const int64_t base = blockIdx.x * static_cast<int64_t>(KD_THREADS) * KD_CHUNKS;
-    uint32_t cur = NO_SPLIT;                      // node of the running box (block-uniform)
-    unsigned int rlo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, rhi[3] = {0u, 0u, 0u};
-    for (int c = 0; c < KD_CHUNKS; ++c) {
-        const int64_t i0 = base + static_cast<int64_t>(c) * KD_THREADS;
-        if (i0 >= n) break;
-        const int64_t i = i0 + threadIdx.x;
-        const bool in = i < n;
-        const uint32_t s = in ? node_at(i) : NO_SPLIT;
-        const bool active = in && split[s] != NO_SPLIT;
-        unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-        if (active) {
-            const uint32_t a = order[i];
-            for (int d = 0; d < 3; ++d) lo[d] = hi[d] = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
-        }
-        const int64_t i_last = min(i0 + KD_THREADS, n) - 1;
-        const uint32_t sb0 = node_at(i0), sb1 = node_at(i_last);
-        if (sb0 == sb1 && i0 + KD_THREADS <= n) {
-            if (sb0 != cur) {
-                if (cur != NO_SPLIT) kd_box_flush(cur, rlo, rhi, red, split, box);
-                cur = sb0;
-            }
-            for (int d = 0; d < 3; ++d) {
-                rlo[d] = min(rlo[d], lo[d]);
-                rhi[d] = max(rhi[d], hi[d]);
-            }
-            continue;
-        }
-        if (cur != NO_SPLIT) {
-            kd_box_flush(cur, rlo, rhi, red, split, box);
-            cur = NO_SPLIT;
-        }
-        const uint32_t w0 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 0, 64));
-        const uint32_t w1 = static_cast<uint32_t>(__shfl(static_cast<int>(s), 63, 64));
-        if (w0 == w1) {
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = wave_umin(lo[d]);
-                hi[d] = wave_umax(hi[d]);
-            }
-            if (lane == 0 && active)
-                for (int d = 0; d < 3; ++d) {
-                    atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
-                    atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
-                }
-        } else if (active) {
-            for (int d = 0; d < 3; ++d) {
-                atomicMin(box + 6 * static_cast<int64_t>(s) + d, lo[d]);
-                atomicMax(box + 6 * static_cast<int64_t>(s) + 3 + d, hi[d]);
-            }
-        }
-    }
-    if (cur != NO_SPLIT) kd_box_flush(cur, rlo, rhi, red, split, box);
-}
 
 __global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -738,70 +721,260 @@ __device__ __forceinline__ int kd_axis(const float* b, float (&lo)[3], float (&e
     return ax;
 }
 
-// Key (node, coordinate along the node's longest side in QB bits); nodes
-// that do not split keep their order (coordinate 0, stable sort).  With
-// prev_split the node ids first step down from the previous level: children
-// 2s (left of the previous cut) and 2s + 1.
-template <typename KeyT>
-__global__ void k_kd_keys(const float* __restrict__ pos, const uint32_t* __restrict__ order,
-                          uint32_t* __restrict__ seg, const uint32_t* __restrict__ prev_split,
-                          const uint32_t* __restrict__ split, const float* __restrict__ fbox,
-                          int64_t n, int QB, KeyT* __restrict__ keys,
-                          uint32_t* __restrict__ vals) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = seg[i];
-    if (prev_split) {
-        s = 2u * s + (static_cast<uint64_t>(i) >= prev_split[s] ? 1u : 0u);
-        seg[i] = s;
-    }
-    const uint32_t a = order[i];
-    uint32_t q = 0;
-    if (split[s] != NO_SPLIT) {
-        float lo[3], ext[3];
-        const int ax = kd_axis(fbox + 6 * static_cast<int64_t>(s), lo, ext);
-        if (ext[ax] > 0.f) {
-            const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) / ext[ax];
-            q = static_cast<uint32_t>(fminf(fmaxf(t, 0.f), 1.f) * static_cast<float>((1u << QB) - 1u));
-        }
-    }
-    keys[i] = (static_cast<KeyT>(s) << QB) | q;
-    vals[i] = a;
-}
-
-// Level-0 float boxes from the order-preserving integer boxes.
+// Float boxes from the order-preserving integer boxes.
 __global__ void k_kd_box_float(const unsigned int* __restrict__ box, int64_t words,
                                float* __restrict__ fbox) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < words) fbox[i] = ord2f(box[i]);
 }
 
-// Boxes of the next level without a pass over the atoms: a splitting node's
-// children take its box cut along its longest side at the bin of the first
-// key right of the cut (left of it every coordinate bin is at most that
-// bin, right of it at least), a node that does not split hands its box to
-// its left child.  Atoms of a child may stand out of its box by rounding;
-// their keys clamp, which only loosens the next split.
-template <typename KeyT>
-__global__ void k_kd_child_boxes(const float* __restrict__ fbox, const uint32_t* __restrict__ split,
-                                 const KeyT* __restrict__ sorted_keys, int64_t nodes, int QB,
-                                 float* __restrict__ fbox_next) {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= nodes) return;
-    float b[6];
-    for (int d = 0; d < 6; ++d) b[d] = fbox[6 * s + d];
-    float* left = fbox_next + 12 * s;
-    float* right = left + 6;
-    for (int d = 0; d < 6; ++d) left[d] = right[d] = b[d];
-    const uint32_t c = split[s];
-    if (c == NO_SPLIT) return;
-    float lo[3], ext[3];
-    const int ax = kd_axis(b, lo, ext);
-    if (!(ext[ax] > 0.f)) return;
-    const uint32_t qc = static_cast<uint32_t>(sorted_keys[c] & ((static_cast<KeyT>(1) << QB) - 1));
-    const float qs = ext[ax] / static_cast<float>((1u << QB) - 1u);
-    left[3 + ax] = fminf(b[3 + ax], lo[ax] + static_cast<float>(qc + 1u) * qs);
-    right[ax] = fmaxf(b[ax], lo[ax] + static_cast<float>(qc) * qs);
+// Global level: tight boxes of the nodes, one chunk per workgroup (6
+// atomics per chunk on order-preserving images of the floats).
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
+    const float* __restrict__ pos, const uint32_t* __restrict__ order,
+    const KdgChunk* __restrict__ chunks, unsigned int* __restrict__ box) {
+    __shared__ unsigned int red[KDG_THREADS / 64][6];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const KdgChunk C = chunks[blockIdx.x];
+    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        const uint32_t i = C.begin + tid + r * KDG_THREADS;
+        if (i < C.end) {
+            const uint32_t a = order[i];
+            for (int d = 0; d < 3; ++d) {
+                const unsigned int o = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
+                lo[d] = min(lo[d], o);
+                hi[d] = max(hi[d], o);
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_umin(lo[d]);
+        hi[d] = wave_umax(hi[d]);
+    }
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) { red[wave][d] = lo[d]; red[wave][3 + d] = hi[d]; }
+    __syncthreads();
+    if (tid < 6) {
+        unsigned int v = red[0][tid];
+        for (int w = 1; w < KDG_THREADS / 64; ++w)
+            v = tid < 3 ? min(v, red[w][tid]) : max(v, red[w][tid]);
+        unsigned int* dst = box + 6 * static_cast<int64_t>(C.node) + tid;
+        if (tid < 3) atomicMin(dst, v);
+        else atomicMax(dst, v);
+    }
+}
+
+// Global level: each atom's bin (KDG_BINS over its node's longest side) and
+// the chunk's histogram of them.
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_bin(
+    const float* __restrict__ pos, const uint32_t* __restrict__ order,
+    const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
+    const float* __restrict__ fbox, uint8_t* __restrict__ q8, uint32_t* __restrict__ bh) {
+    __shared__ unsigned int h[KDG_BINS];
+    const int tid = threadIdx.x;
+    const KdgChunk C = chunks[blockIdx.x];
+    const uint32_t cnt = nodes[C.node].count;
+    h[tid] = 0u;
+    __syncthreads();
+    if (cnt > TILE) {
+        float lo[3], ext[3];
+        const int ax = kd_axis(fbox + 6 * static_cast<int64_t>(C.node), lo, ext);
+        const float sc = ext[ax] > 0.f ? static_cast<float>(KDG_BINS) / ext[ax] : 0.f;
+#pragma unroll
+        for (int r = 0; r < KDG_PT; ++r) {
+            const uint32_t i = C.begin + tid + r * KDG_THREADS;
+            if (i < C.end) {
+                const uint32_t a = order[i];
+                const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) * sc;
+                const int q = min(KDG_BINS - 1, max(0, static_cast<int>(t)));
+                q8[i] = static_cast<uint8_t>(q);
+                atomicAdd(&h[q], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    bh[static_cast<int64_t>(blockIdx.x) * KDG_BINS + tid] = h[tid];
+}
+
+// Global level: the node histograms, summed over their chunks in slices
+// (slice s of a node takes its chunks s, s + S, ...).
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_total(
+    const KdgNode* __restrict__ nodes, const uint32_t* __restrict__ bh, int S,
+    uint32_t* __restrict__ part) {
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const int64_t h = blockIdx.y;
+    const KdgNode N = nodes[h];
+    uint32_t tot = 0;
+    if (N.count > static_cast<uint32_t>(TILE))
+        for (uint32_t b = s; b < N.nblk; b += S) tot += bh[static_cast<int64_t>(N.blk0 + b) * KDG_BINS + tid];
+    part[(h * S + s) * KDG_BINS + tid] = tot;
+}
+
+// Global level, one workgroup per node: the bin holding the cut (from the
+// S slice sums).  With fbox_next the children's boxes follow from the
+// parent's: cut at the cut bin's edges along the split side (a node that does
+// not split hands its box to both).
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_select(
+    const KdgNode* __restrict__ nodes, const uint32_t* __restrict__ part, int S,
+    const float* __restrict__ fbox, float* __restrict__ fbox_next, int2* __restrict__ sel) {
+    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    __shared__ int s_bin, s_tie;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t h = blockIdx.x;
+    const KdgNode N = nodes[h];
+    float b6[6];
+    for (int d = 0; d < 6; ++d) b6[d] = fbox[6 * h + d];
+    if (N.count <= static_cast<uint32_t>(TILE)) {
+        if (fbox_next && tid < 12) fbox_next[12 * h + tid] = b6[tid % 6];
+        return;
+    }
+    uint32_t tot = 0;
+    for (int s = 0; s < S; ++s) tot += part[(h * S + s) * KDG_BINS + tid];
+    uint32_t inc = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) inc += wsum[w];
+    const uint32_t exc = inc - tot;
+    if (exc < N.need && inc >= N.need) {             // exactly one bin
+        s_bin = tid;
+        s_tie = static_cast<int>(N.need - exc);
+    }
+    __syncthreads();
+    const int bs = s_bin;
+    if (tid == 0) sel[h] = make_int2(bs, s_tie);
+    if (fbox_next && tid < 2) {
+        float lo[3], ext[3];
+        const int ax = kd_axis(b6, lo, ext);
+        const float w = ext[ax] / static_cast<float>(KDG_BINS);
+        float c6[6];
+        for (int d = 0; d < 6; ++d) c6[d] = b6[d];
+        if (ext[ax] > 0.f) {
+            if (tid == 0) c6[3 + ax] = fminf(b6[3 + ax], lo[ax] + static_cast<float>(bs + 1) * w);
+            else c6[ax] = fmaxf(b6[ax], lo[ax] + static_cast<float>(bs) * w);
+        }
+        for (int d = 0; d < 6; ++d) fbox_next[12 * h + 6 * tid + d] = c6[d];
+    }
+}
+
+// Global level, one workgroup per chunk: its counts below the cut bin and
+// in it.
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_chunk_counts(
+    const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
+    const uint32_t* __restrict__ bh, const int2* __restrict__ sel, uint2* __restrict__ cnt) {
+    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const KdgChunk C = chunks[blockIdx.x];
+    if (nodes[C.node].count <= static_cast<uint32_t>(TILE)) return;
+    const int bs = sel[C.node].x;
+    const uint32_t v = bh[static_cast<int64_t>(blockIdx.x) * KDG_BINS + tid];
+    uint32_t below = tid < bs ? v : 0u;
+    for (int o = 32; o > 0; o >>= 1)
+        below += static_cast<uint32_t>(__shfl_xor(static_cast<int>(below), o, 64));
+    if (lane == 0) wsum[wave] = below;
+    __syncthreads();
+    if (tid == bs) {
+        uint32_t b = 0;
+        for (int w = 0; w < KDG_THREADS / 64; ++w) b += wsum[w];
+        cnt[blockIdx.x] = make_uint2(b, v);
+    }
+}
+
+// Global level, one workgroup per node: exclusive prefix of the chunk counts
+// over the node's chunks.
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_chunk_scan(
+    const KdgNode* __restrict__ nodes, const uint2* __restrict__ cnt, uint2* __restrict__ boff) {
+    __shared__ unsigned long long wsum[KDG_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const KdgNode N = nodes[blockIdx.x];
+    if (N.count <= static_cast<uint32_t>(TILE)) return;
+    unsigned long long carry = 0ull;
+    for (uint32_t base = 0; base < N.nblk; base += KDG_THREADS) {
+        const uint32_t b = base + tid;
+        unsigned long long v = 0ull;
+        if (b < N.nblk) {
+            const uint2 c = cnt[N.blk0 + b];
+            v = (static_cast<unsigned long long>(c.x) << 32) | c.y;
+        }
+        unsigned long long inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        unsigned long long before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        const unsigned long long ex = before + inc - v;
+        if (b < N.nblk)
+            boff[N.blk0 + b] = make_uint2(static_cast<uint32_t>(ex >> 32), static_cast<uint32_t>(ex));
+        for (int w = 0; w < KDG_THREADS / 64; ++w) carry += wsum[w];
+        __syncthreads();                              // wsum is rewritten next round
+    }
+}
+
+// Global level: the stable partition of each chunk into its node's halves
+// (a node that does not split is copied).
+__global__ __launch_bounds__(KDG_THREADS) void k_kdg_scatter(
+    const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
+    const uint8_t* __restrict__ q8, const uint32_t* __restrict__ order_in,
+    const int2* __restrict__ sel, const uint2* __restrict__ boff,
+    uint32_t* __restrict__ order_out) {
+    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const KdgChunk C = chunks[blockIdx.x];
+    const KdgNode N = nodes[C.node];
+    const uint32_t i0 = C.begin + KDG_PT * tid;
+    if (N.count <= static_cast<uint32_t>(TILE)) {
+#pragma unroll
+        for (int r = 0; r < KDG_PT; ++r)
+            if (i0 + r < C.end) order_out[i0 + r] = order_in[i0 + r];
+        return;
+    }
+    const int2 sl = sel[C.node];
+    const uint2 bo = boff[blockIdx.x];
+    uint32_t fl[KDG_PT], own = 0;
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        fl[r] = 0u;
+        if (i0 + r < C.end) {
+            const int q = q8[i0 + r];
+            fl[r] = q < sl.x ? 0x10000u : (q == sl.x ? 1u : 0u);
+        }
+        own += fl[r];
+    }
+    uint32_t inc = own;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t run = inc - own;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        const uint32_t i = i0 + r;
+        if (i < C.end) {
+            const int lb = static_cast<int>(run >> 16) + static_cast<int>(bo.x);
+            const int tb = static_cast<int>(run & 0xFFFFu) + static_cast<int>(bo.y);
+            const int before = lb + min(tb, sl.y);
+            const bool left = fl[r] == 0x10000u || (fl[r] == 1u && tb < sl.y);
+            const int64_t dst = left ? static_cast<int64_t>(N.begin) + before
+                                     : static_cast<int64_t>(N.begin) + N.need +
+                                           (static_cast<int64_t>(i) - N.begin - before);
+            // inside the node by construction; the guard keeps a logic error
+            // from writing elsewhere
+            if (dst >= N.begin && dst < static_cast<int64_t>(N.begin) + N.count)
+                order_out[dst] = order_in[i];
+        }
+        run += fl[r];
+    }
 }
 
 struct SpeciesSplit {
@@ -809,15 +982,6 @@ struct SpeciesSplit {
     int64_t atom_start[MAX_SPECIES + 1];
     int tile_start[MAX_SPECIES + 1];
 };
-
-// Level-0 node of every position of the species-grouped order: its species.
-__global__ void k_kd_root(int64_t n, SpeciesSplit split, uint32_t* __restrict__ seg) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = 0;
-    while (static_cast<int>(s) + 1 < split.n_species && i >= split.atom_start[s + 1]) ++s;
-    seg[i] = s;
-}
 
 // Origin of a unit of 2 or 4 consecutive tiles: the mean of their origins,
 // taken pairwise (the moment sum and the staging both use this form).
@@ -920,11 +1084,13 @@ constexpr int KDL_SUBMAX = 1 << (KDL_LEVELS - 1);   // sub-nodes of the deepest 
 // the cut bin" and "in the cut bin" flags) in which the first positions of
 // the cut bin go left until the count is met.  The halves overlap by at most
 // one bin.  Then the node's order is written back and its tiles are emitted
-// as k_tiles emits them.  Thread tid holds positions 8 tid .. 8 tid + 7, so
+// as k_tiles emits them (read from order_in, written to order, which may be
+// the same buffer).  Thread tid holds positions 8 tid .. 8 tid + 7, so
 // a half-wave covers one tile, which never straddles two sub-nodes.
 template <int LV>
 __global__ __launch_bounds__(32 << LV) void k_kd_local(
-    const float* __restrict__ pos, const float2* __restrict__ amp, uint32_t* __restrict__ order,
+    const float* __restrict__ pos, const float2* __restrict__ amp, const uint32_t* order_in,
+    uint32_t* order,
     const KdNode* __restrict__ nodes, double K, TileRec* __restrict__ tiles,
     AtomRec* __restrict__ atoms, float* __restrict__ grp_r, unsigned int* __restrict__ radius_bits) {
     constexpr int THREADS = 32 << LV, MAXN = TILE << LV, PT = MAXN / THREADS;
@@ -955,7 +1121,7 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
 #pragma unroll
         for (int i = 0; i < PT; ++i) {
             const int p = tid + i * THREADS;
-            a[i] = p < n ? order[begin + p] : 0u;
+            a[i] = p < n ? order_in[begin + p] : 0u;
         }
 #pragma unroll
         for (int i = 0; i < PT; ++i) {
@@ -1210,7 +1376,7 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
         at[i] = 0u;
         if (p < n) {
             sl[i] = slot[kidx(p)];
-            at[i] = order[begin + sl[i]];
+            at[i] = order_in[begin + sl[i]];
         }
     }
     __syncthreads();                                  // slot reads above are done
@@ -3172,33 +3338,24 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     const unsigned int n32 = static_cast<unsigned int>(atom_count);
     const int nb = blocks_for(atom_count, KD_THREADS);
     hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
-    int sbits = 0;
-    while ((1 << sbits) < n_species) ++sbits;
-    // levels by global radix sorts; the last (at most KDL_LEVELS) run in LDS
-    const int glevels = std::max(0, L.kd_levels - KDL_LEVELS);
-    // 32-bit sort keys (node, coordinate) while the node ids fit
-    const bool k32 = sbits + std::max(0, glevels - 1) + KD_QBITS32_MIN <= 32;
-    uint32_t* keys32_a = reinterpret_cast<uint32_t*>(L.keys_a);
-    uint32_t* keys32_b = reinterpret_cast<uint32_t*>(L.keys_b);
-    // group by species (stable), then bisect each species level by level
+    // group by species (stable), then bisect each species level by level:
+    // the global levels by chunks, the last (at most KDL_LEVELS) in LDS
     size_t sb = L.sort_bytes;
-    if (k32) {
-        hipLaunchKernelGGL(k_species_keys<uint32_t>, dim3(nb), dim3(KD_THREADS), 0, st, species,
-                           atom_count, keys32_a, L.vals_a);
-        MOSAIC_HIP_CHECK(hipGetLastError());
-        MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, keys32_a, keys32_b, L.vals_a,
-                                                   L.vals_b, n32, 0, 4, st));
-    } else {
-        hipLaunchKernelGGL(k_species_keys<uint64_t>, dim3(nb), dim3(KD_THREADS), 0, st, species,
-                           atom_count, L.keys_a, L.vals_a);
-        MOSAIC_HIP_CHECK(hipGetLastError());
-        MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
-                                                   L.vals_b, n32, 0, 4, st));
-    }
-    // host buffers alive until the synchronisation below
-    std::vector<uint32_t> plan;
-    std::vector<int64_t> nbeg, nend;
-    kd_plan(n_species, counts, glevels, plan, nbeg, nend);
+    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
+                       L.keys_a, L.vals_a);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
+                                               L.vals_b, n32, 0, 4, st));
+    const int glevels = kd_global_levels(n_species, counts);
+    // host tables alive until the synchronisation below
+    std::vector<KdgNode> gnodes;
+    std::vector<KdgChunk> gchunks;
+    std::vector<size_t> node_off, chunk_off;
+    kdg_tables(n_species, counts, gnodes, gchunks, node_off, chunk_off);
+    std::vector<std::vector<int64_t>> RB, RE;
+    kd_plan_levels(n_species, counts, glevels, RB, RE);
+    const std::vector<int64_t>& nbeg = RB[glevels];
+    const std::vector<int64_t>& nend = RE[glevels];
     // nodes of the in-LDS finish, grouped by size: up to 8 tiles, 16, 32
     std::vector<KdNode> lnodes;
     int lcount[3] = {0, 0, 0};
@@ -3220,54 +3377,45 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         }
     }
     if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
+    uint32_t* cur = L.vals_b;                     // the order after the species sort
+    uint32_t* other = L.vals_a;
     if (glevels > 0) {
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_split, plan.data(), plan.size() * sizeof(uint32_t),
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, gnodes.data(), gnodes.size() * sizeof(KdgNode),
                                         hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_kd_root, dim3(nb), dim3(KD_THREADS), 0, st, atom_count, split, L.seg);
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_chunks, gchunks.data(),
+                                        gchunks.size() * sizeof(KdgChunk), hipMemcpyHostToDevice, st));
+        auto tight = [&](int lv) { return lv == 0 || lv >= glevels - KD_TIGHT_LEVELS; };
         for (int l = 0; l < glevels; ++l) {
-            const int64_t nodes = static_cast<int64_t>(n_species) << l;
-            const uint32_t* cut = L.kd_split + kd_split_count(n_species, l);
-            const uint32_t* prev = l > 0 ? L.kd_split + kd_split_count(n_species, l - 1) : nullptr;
-            const float* fbox = L.kd_fbox[l & 1];
-            float* fnext = L.kd_fbox[(l + 1) & 1];
-            auto tight = [&](int lv) { return lv == 0 || lv >= glevels - KD_TIGHT_LEVELS; };
+            const int64_t nodes = static_cast<int64_t>(node_off[l + 1] - node_off[l]);
+            const unsigned int nchunks = static_cast<unsigned int>(chunk_off[l + 1] - chunk_off[l]);
+            const KdgNode* nd = L.kdg_nodes + node_off[l];
+            const KdgChunk* ch = L.kdg_chunks + chunk_off[l];
+            float* fbox = L.kd_fbox[l & 1];
+            float* fnext = (l + 1 < glevels && !tight(l + 1)) ? L.kd_fbox[(l + 1) & 1] : nullptr;
+            if (nchunks == 0) continue;
             if (tight(l)) {
                 hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
                                    st, L.kd_box, 6 * nodes);
-                hipLaunchKernelGGL(k_kd_box, dim3(blocks_for(atom_count, KD_THREADS * KD_CHUNKS)),
-                                   dim3(KD_THREADS), 0, st, pos, L.vals_b, L.seg, prev, cut,
-                                   atom_count, L.kd_box);
+                hipLaunchKernelGGL(k_kdg_box, dim3(nchunks), dim3(KDG_THREADS), 0, st, pos, cur, ch,
+                                   L.kd_box);
                 hipLaunchKernelGGL(k_kd_box_float, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
-                                   st, L.kd_box, 6 * nodes, L.kd_fbox[l & 1]);
+                                   st, L.kd_box, 6 * nodes, fbox);
             }
-            const bool derive = l + 1 < glevels && !tight(l + 1);
-            sb = L.sort_bytes;
-            if (k32) {
-                const int qb = kd_qbits32(l + sbits);
-                hipLaunchKernelGGL(k_kd_keys<uint32_t>, dim3(nb), dim3(KD_THREADS), 0, st, pos,
-                                   L.vals_b, L.seg, prev, cut, fbox, atom_count, qb, keys32_a,
-                                   L.vals_a);
-                MOSAIC_HIP_CHECK(hipGetLastError());
-                MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, keys32_a, keys32_b,
-                                                           L.vals_a, L.vals_b, n32, 0,
-                                                           qb + l + sbits, st));
-                if (derive)
-                    hipLaunchKernelGGL(k_kd_child_boxes<uint32_t>, dim3(blocks_for(nodes, 256)),
-                                       dim3(256), 0, st, fbox, cut, keys32_b, nodes, qb, fnext);
-            } else {
-                hipLaunchKernelGGL(k_kd_keys<uint64_t>, dim3(nb), dim3(KD_THREADS), 0, st, pos,
-                                   L.vals_b, L.seg, prev, cut, fbox, atom_count, KD_QBITS,
-                                   L.keys_a, L.vals_a);
-                MOSAIC_HIP_CHECK(hipGetLastError());
-                MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b,
-                                                           L.vals_a, L.vals_b, n32, 0,
-                                                           KD_QBITS + l + sbits, st));
-                if (derive)
-                    hipLaunchKernelGGL(k_kd_child_boxes<uint64_t>, dim3(blocks_for(nodes, 256)),
-                                       dim3(256), 0, st, fbox, cut, L.keys_b, nodes, KD_QBITS,
-                                       fnext);
-            }
+            hipLaunchKernelGGL(k_kdg_bin, dim3(nchunks), dim3(KDG_THREADS), 0, st, pos, cur, ch, nd,
+                               fbox, L.kdg_q8, L.kdg_bh);
+            const int S = kdg_slices(gnodes, node_off[l], node_off[l + 1]);
+            hipLaunchKernelGGL(k_kdg_total, dim3(S, static_cast<unsigned int>(nodes)),
+                               dim3(KDG_THREADS), 0, st, nd, L.kdg_bh, S, L.kdg_part);
+            hipLaunchKernelGGL(k_kdg_select, dim3(static_cast<unsigned int>(nodes)),
+                               dim3(KDG_THREADS), 0, st, nd, L.kdg_part, S, fbox, fnext, L.kdg_sel);
+            hipLaunchKernelGGL(k_kdg_chunk_counts, dim3(nchunks), dim3(KDG_THREADS), 0, st, ch, nd,
+                               L.kdg_bh, L.kdg_sel, L.kdg_cnt);
+            hipLaunchKernelGGL(k_kdg_chunk_scan, dim3(static_cast<unsigned int>(nodes)),
+                               dim3(KDG_THREADS), 0, st, nd, L.kdg_cnt, L.kdg_boff);
+            hipLaunchKernelGGL(k_kdg_scatter, dim3(nchunks), dim3(KDG_THREADS), 0, st, ch, nd,
+                               L.kdg_q8, cur, L.kdg_sel, L.kdg_boff, other);
             MOSAIC_HIP_CHECK(hipGetLastError());
+            std::swap(cur, other);
         }
     }
     const double K = k * INV_2PI_HI;
@@ -3276,15 +3424,16 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
                                         hipMemcpyHostToDevice, st));
         const float2* a2 = reinterpret_cast<const float2*>(amp);
         const KdNode* nds = L.kd_nodes;
+        // the final order goes to vals_b (mosaic_kin_restage reads it there)
         if (lcount[0] > 0)
-            hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, pos, a2,
+            hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, pos, a2, cur,
                                L.vals_b, nds, K, L.tiles, L.atoms, L.grp_r, L.radius_bits);
         if (lcount[1] > 0)
-            hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, pos, a2,
+            hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, pos, a2, cur,
                                L.vals_b, nds + lcount[0], K, L.tiles, L.atoms, L.grp_r,
                                L.radius_bits);
         if (lcount[2] > 0)
-            hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, pos, a2,
+            hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, pos, a2, cur,
                                L.vals_b, nds + lcount[0] + lcount[1], K, L.tiles, L.atoms,
                                L.grp_r, L.radius_bits);
     }
