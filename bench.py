@@ -36,12 +36,12 @@ MFMA_FLOP_PER_PAIR = 24.0
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (the copy-kernel calibration
 # reads FETCH_SIZE x 2 and WRITE_SIZE x 1; profiles/k_sum_mma_perovskite5M_512px.md).
 MMA_TRAFFIC_BYTES = (2 * 1831133.0 + 2542443.8) * 1024
-# k_sum_poly<6> on the same workload: FETCH_SIZE 1.007e6 KiB (x2, the copy
-# calibration), WRITE_SIZE 0.839e6 KiB; 5.50e9 VALU wave-instructions per launch
-# for 1.31e12 pairs (SQ_INSTS_VALU), i.e. 0.27 lane-operations per pair
+# k_sum_poly<6> on the same workload: FETCH_SIZE 1.006e6 KiB (x2, the copy
+# calibration), WRITE_SIZE 0.838e6 KiB; 4.72e9 VALU wave-instructions per launch
+# for 1.31e12 pairs (SQ_INSTS_VALU), i.e. 0.23 lane-operations per pair
 # (profiles/k_sum_poly_perovskite5M_512px.md)
-POLY_TRAFFIC_BYTES = (2 * 1007200.0 + 838680.0) * 1024
-POLY_VALU_LANE_OPS_PER_PAIR = 5.499e9 * 64 / (5.0e6 * 512 * 512)
+POLY_TRAFFIC_BYTES = (2 * 1006000.0 + 838200.0) * 1024
+POLY_VALU_LANE_OPS_PER_PAIR = 4.7196e9 * 64 / (5.0e6 * 512 * 512)
 # VALU issue peak in lane-operations per second (an FMA counts once): the FP32
 # peak halved
 PEAK_VALU_LANE_OPS = PEAK_FP32 / 2
