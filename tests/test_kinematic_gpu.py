@@ -315,3 +315,27 @@ def test_moment_sum_cross_term_regime(device):
     R = oracle_field(w, "f80", amp=amp)
     e, _ = report("cross-term regime (moment sum)", A, R)
     assert e <= TOL
+
+
+def test_staging_many_species_ragged(device):
+    """Random atoms of 12 species, interleaved in the input, with counts on
+    and around the staging's boundaries: none, one atom, one tile +-1, the
+    2,048-position chunk +-1, several global k-d levels for the largest
+    species; the field against the long-double oracle."""
+    from mosaic_amd.sample import sample
+    w = W.fcc_au_bragg111(n=4, npix=16, sigma=0.0)
+    labels = ["Au", "Cu", "Ag", "Pt", "Pd", "Ni", "Fe", "Co", "Al", "Si", "Ti", "Sr"]
+    counts = [40000, 0, 1, 255, 256, 257, 2047, 2048, 2049, 4097, 513, 6000]
+    n = sum(counts)
+    rng = np.random.default_rng(11)
+    side = (n * 17.0) ** (1.0 / 3.0)
+    pos = ((rng.random((n, 3)) - 0.5) * side).astype(np.float32)
+    codes = np.repeat(np.arange(len(counts), dtype=np.uint8), counts)
+    perm = rng.permutation(n)
+    pos, codes = np.ascontiguousarray(pos[perm]), np.ascontiguousarray(codes[perm])
+    w["sample"] = sample([pos], [codes], labels)
+    A = gpu_field(w)
+    R = oracle_field(w, "f80")
+    e, _ = report(f"{n} random atoms, 12 species, ragged counts (path {w['beam']._last_path})",
+                  A, R)
+    assert e <= TOL
