@@ -178,15 +178,27 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
  * form contracts per-patch weight matrices with the atoms' separable
  * phasors on the FP16-split MFMA; the row form walks row blocks on the VALU.
  * Declines (MOSAIC_ERR_UNSUPPORTED) outside the far-field bounds of the
- * chosen form.
+ * chosen form.  With `scratch` (device, at least mosaic_kin_adjoint_scratch
+ * bytes; NULL or smaller: none) the moment form shares each unit's pixel
+ * moments among its 1, 2 or 4 tiles, as the forward's moment sum does.
  */
 int mosaic_kin_adjoint(int64_t atom_count, int n_species,
                        const int64_t* species_counts /* host */, double k, int n_frames,
                        const float* rot /* host */, const float* trans /* host */, int Ny,
                        int Nz, const double* stage_stats /* host [2] */,
                        const double* pixel_geom /* host [8] */, const float* grad_field,
-                       float* grad_pos, void* workspace, size_t workspace_bytes, int path,
+                       float* grad_pos, void* workspace, size_t workspace_bytes,
+                       void* scratch, size_t scratch_bytes, int path,
                        mosaic_stream_t stream);
+
+/*
+ * Bytes of the scratch that lets mosaic_kin_adjoint's moment form share pixel
+ * moments across units of tiles: 360 B per tile, frame and 32 x 32 patch.  No
+ * counterpart in the reference (an adjoint of this build's own).
+ */
+int mosaic_kin_adjoint_scratch(int64_t atom_count, int n_species,
+                               const int64_t* species_counts /* host */, int n_frames, int Ny,
+                               int Nz, size_t* bytes);
 
 /*
  * Incident amplitude per atom by bilinear sampling of the beam's entrance

@@ -30,6 +30,7 @@ EXPORTS = (
     "mosaic_kin_pixels",
     "mosaic_kin_sum",
     "mosaic_kin_adjoint",
+    "mosaic_kin_adjoint_scratch",
     "mosaic_ein_bilinear",
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
@@ -84,7 +85,8 @@ _SIGS = {
     "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _vp, _vp,
                        _vp, _sz, _vp, _vp],
     "mosaic_kin_adjoint": [_i64, _i, _vp, _d, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
-                           _sz, _i, _vp],
+                           _sz, _vp, _sz, _i, _vp],
+    "mosaic_kin_adjoint_scratch": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,

@@ -2662,7 +2662,35 @@ struct AdjMmaArgs {
     int pby;
     int n_patches;
     double K;
+    // moment adjoint over units of tiles (k_adjoint_moments, k_adjoint_poly<..., true>)
+    const float* grp_r;    // staged pair / quad radii ([tile][2])
+    float grp_rmax;        // largest unit radius inside the error bounds
+    int n_tiles;
+    float* mom;            // [tile][frame x patch][2 NM] pixel moments of unit heads
 };
+
+// Unit of tile t for the moment adjoint: the quad, else the pair, whose
+// staged radius is within rmax and which covers t, else the tile alone.
+// Heads sit at aligned places inside a k-d node and only they carry finite
+// radii, so this is the unit the moment sum's walk forms from a node start.
+__device__ __forceinline__ void unit_of(int t, int n_tiles, const float* __restrict__ grp_r,
+                                        float rmax, int& head, int& size) {
+    head = t;
+    size = 1;
+    if (!(rmax > 0.f)) return;
+    for (int h = max(0, t - 3); h <= t; ++h)
+        if (h + 3 < n_tiles && grp_r[2 * static_cast<int64_t>(h) + 1] <= rmax) {
+            head = h;
+            size = 4;
+            return;
+        }
+    for (int h = max(0, t - 1); h <= t; ++h)
+        if (h + 1 < n_tiles && grp_r[2 * static_cast<int64_t>(h)] <= rmax) {
+            head = h;
+            size = 2;
+            return;
+        }
+}
 
 template <bool SECOND, int NV>
 __global__ __launch_bounds__(64 * MMA_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -2915,6 +2943,140 @@ void k_adjoint_mma(const AdjMmaArgs A) {
     }
 }
 
+// Pixel moments of the moment adjoint over units of tiles: per unit head and
+// (frame, patch) pair, M_mn = sum_ab W(a, b) x^m y^n (m + n <= D + 2), W the
+// weighted residual times the unit's origin phase, formed as k_adjoint_poly
+// forms them per tile; the unit's tiles then share them.
+constexpr int ADJ_GROUP = 64;     // (frame, patch) pairs per k_adjoint_moments workgroup
+
+template <int D>
+__global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMmaArgs A) {
+    constexpr int DM = D + 2;
+    constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
+    constexpr int HALF = (NM + 1) / 2;              // reduced in two rounds
+    constexpr int NVP = (2 * HALF) | 1;
+    __shared__ float red[POLY_WAVES][64 * NVP];
+    __shared__ double rmx[POLY_WAVES];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int t = blockIdx.x;
+    int head, size;
+    unit_of(t, A.n_tiles, A.grp_r, A.grp_rmax, head, size);
+    if (head != t) return;                           // the whole workgroup
+    float mg = 0.f, mG = 0.f;
+    for (int i = tid; i < AMAX_BLOCKS; i += 64 * POLY_WAVES) {
+        mg = fmaxf(mg, A.amax[i]);
+        mG = fmaxf(mG, A.amax[AMAX_BLOCKS + i]);
+    }
+    const float sg = pow2_scale(static_cast<float>(block_max(static_cast<double>(mg), rmx)));
+    const float sG = pow2_scale(static_cast<float>(block_max(static_cast<double>(mG), rmx)));
+    const TileRec tr = A.tiles[t];
+    double uox = tr.ox, uoy = tr.oy, uoz = tr.oz;
+    if (size > 1) {
+        const TileRec b1 = A.tiles[t + 1];
+        const TileRec b2 = A.tiles[size == 4 ? t + 2 : t];
+        const TileRec b3 = A.tiles[size == 4 ? t + 3 : t];
+        uox = unit_origin(size, tr.ox, b1.ox, b2.ox, b3.ox);
+        uoy = unit_origin(size, tr.oy, b1.oy, b2.oy, b3.oy);
+        uoz = unit_origin(size, tr.oz, b1.oz, b2.oz, b3.oz);
+    }
+    const float2* __restrict__ Gt = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
+    const int h = lane >> 5, q = lane & 31;
+    const int Ny = A.Ny;
+    const int npairs = A.n_frames * A.n_patches;
+    const int p0 = blockIdx.y * ADJ_GROUP, p1 = min(npairs, p0 + ADJ_GROUP);
+    float* rw = red[wave];
+    // every wave runs the same number of pair iterations (barriers inside)
+    const int iters = (p1 - p0 + POLY_WAVES - 1) / POLY_WAVES;
+    for (int it = 0; it < iters; ++it) {
+        const int pr = p0 + it * POLY_WAVES + wave;
+        const bool live = pr < p1;
+        const int prc = live ? pr : p0;
+        const int f = prc / A.n_patches, patch = prc % A.n_patches;
+        const FrameRec& F = A.frames[f];
+        const PatchGeom P = patch_of(patch % A.pby, patch / A.pby, A.Ny, A.Nz);
+        const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+        const double4 um = A.u0h[pmid];
+        const double Ox = fma(F.R[0], uox, fma(F.R[1], uoy, fma(F.R[2], uoz, F.T[0])));
+        const double Oy = fma(F.R[3], uox, fma(F.R[4], uoy, fma(F.R[5], uoz, F.T[1])));
+        const double Oz = fma(F.R[6], uox, fma(F.R[7], uoy, fma(F.R[8], uoz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+        double ph_t = A.theta0[pmid] + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
+        ph_t -= floor(ph_t);
+        const float kox = static_cast<float>(A.K * Ox), koy = static_cast<float>(A.K * Oy),
+                    koz = static_cast<float>(A.K * Oz);
+        const float pht = static_cast<float>(ph_t);
+
+        // W on this lane's row a = q, columns b = 16 s + 8 h + u, folded into
+        // H_n = sum_b W y_b^n
+        float Hr[DM + 1], Hi[DM + 1];
+#pragma unroll
+        for (int n = 0; n <= DM; ++n) { Hr[n] = 0.f; Hi[n] = 0.f; }
+        const float2* __restrict__ gf = A.gin + static_cast<int64_t>(f) * A.npix;
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if ((u & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+                const int b = 16 * st + 8 * h + u;
+                const bool okp = live && q < P.vy && b < P.vz;
+                const int p = (P.z0 + min(b, P.vz - 1)) * Ny + P.y0 + min(q, P.vy - 1);
+                float wr = 0.f, wi = 0.f;
+                if (okp) {
+                    const float4 du = A.dup[p];
+                    const float phs = __builtin_amdgcn_fractf(
+                        pht - fmaf(koz, du.z, fmaf(koy, du.y, kox * du.x)));
+                    const float sn = __builtin_amdgcn_sinf(phs), cs = __builtin_amdgcn_cosf(phs);
+                    const float2 g0 = Gt[p], q0 = gf[p];
+                    const float gr = g0.x * sG, gi = g0.y * sG, qr = q0.x * sg, qi = q0.y * sg;
+                    const float zr_ = gr * cs - gi * sn, zi_ = gr * sn + gi * cs;
+                    wr = qr * zr_ + qi * zi_;          // conj(g) z
+                    wi = qr * zi_ - qi * zr_;
+                }
+                const float y = static_cast<float>(b - P.mz) * (1.f / 16.f);
+                float yn = 1.f;
+#pragma unroll
+                for (int n = 0; n <= DM; ++n) {
+                    Hr[n] = fmaf(wr, yn, Hr[n]);
+                    Hi[n] = fmaf(wi, yn, Hi[n]);
+                    yn *= y;
+                }
+            }
+        }
+        // M_mn = sum over lanes of x_q^m H_n, in two rounds through LDS; the
+        // lane holding a column's sum writes it
+        const float xq = static_cast<float>(q - P.my) * (1.f / 16.f);
+        float xm[DM + 1];
+        xm[0] = 1.f;
+#pragma unroll
+        for (int m = 1; m <= DM; ++m) xm[m] = xm[m - 1] * xq;
+        float* out = A.mom + (static_cast<int64_t>(t) * npairs + prc) * (2 * NM);
+#pragma unroll
+        for (int round = 0; round < 2; ++round) {
+            int k = 0;
+#pragma unroll
+            for (int m = 0; m <= DM; ++m) {
+#pragma unroll
+                for (int n = 0; n <= DM - m; ++n, ++k) {
+                    if (k / HALF == round) {
+                        const int c = k - round * HALF;
+                        rw[lane * NVP + 2 * c] = xm[m] * Hr[n];
+                        rw[lane * NVP + 2 * c + 1] = xm[m] * Hi[n];
+                    }
+                }
+            }
+            __syncthreads();
+            const int col = min(lane, NVP - 1);
+            float tot = 0.f;
+#pragma unroll 8
+            for (int l = 0; l < 64; ++l) tot += rw[l * NVP + col];
+            __syncthreads();
+            const int idx = round * 2 * HALF + lane;
+            if (live && lane < 2 * HALF && idx < 2 * NM) out[idx] = tot;
+        }
+    }
+}
+
 // Adjoint by moments, the dual of k_sum_poly.  Per (frame, patch) and tile,
 // with the patch's direction model v(x, y) = vc + x Gy + x^2 Hy + y Gz + y^2 Hz
 // (x = a / 16, y = b / 16; Gy = 16 gy, Hy = 256 hy, ...) and the atom phase
@@ -2927,7 +3089,7 @@ void k_adjoint_mma(const AdjMmaArgs A) {
 // one row a = q, summed over the wave through LDS), and every atom contracts
 // its expansion coefficients with five shifted moment sets.  Z accumulates
 // per lane in FP32 over up to 16 (frame, patch) pairs, then in FP64 in LDS.
-template <int D, bool SECOND>
+template <int D, bool SECOND, bool UNITS>
 __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaArgs A) {
     constexpr int DM = D + 2;
     constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
@@ -2937,7 +3099,7 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
     constexpr float TWO_PI_F = 6.283185307179586f;
     __shared__ AtomRec tile[TILE];
     __shared__ double zacc[TILE][6];
-    __shared__ float red[POLY_WAVES][64 * NVP];
+    __shared__ float red[UNITS ? 1 : POLY_WAVES][UNITS ? 1 : 64 * NVP];
     __shared__ double rmx[POLY_WAVES];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int t = blockIdx.x;
@@ -2954,11 +3116,30 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
 
     const TileRec tr = A.tiles[t];
     const int cnt = tr.count;
+    // with UNITS the pixel moments are the unit's (k_adjoint_moments), taken
+    // about the unit's origin; this tile's atoms shift to it by dsh
+    int uhead = t, usize = 1;
+    double uox = tr.ox, uoy = tr.oy, uoz = tr.oz;
+    if (UNITS) {
+        unit_of(t, A.n_tiles, A.grp_r, A.grp_rmax, uhead, usize);
+        if (usize > 1) {
+            const TileRec b0 = A.tiles[uhead];
+            const TileRec b1 = A.tiles[uhead + 1];
+            const TileRec b2 = A.tiles[usize == 4 ? uhead + 2 : uhead];
+            const TileRec b3 = A.tiles[usize == 4 ? uhead + 3 : uhead];
+            uox = unit_origin(usize, b0.ox, b1.ox, b2.ox, b3.ox);
+            uoy = unit_origin(usize, b0.oy, b1.oy, b2.oy, b3.oy);
+            uoz = unit_origin(usize, b0.oz, b1.oz, b2.oz, b3.oz);
+        }
+    }
+    const float dshx = static_cast<float>((tr.ox - uox) * A.K);
+    const float dshy = static_cast<float>((tr.oy - uoy) * A.K);
+    const float dshz = static_cast<float>((tr.oz - uoz) * A.K);
     const float2* __restrict__ Gt = A.gmma + static_cast<int64_t>(tr.species) * A.npix;
     const int h = lane >> 5, q = lane & 31;
     const int Ny = A.Ny;
     const int npairs = A.n_frames * A.n_patches;
-    float* rw = red[wave];
+    float* rw = &red[UNITS ? 0 : wave][0];
     // this lane's atoms lane + 64 i and their FP32 partial Z
     float zr[4][3], zi[4][3];
 #pragma unroll
@@ -3008,9 +3189,9 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
             }
         }
         const double4 um = A.u0h[pmid];
-        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
-        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
-        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double Ox = fma(R00, uox, fma(R01, uoy, fma(R02, uoz, F.T[0])));
+        const double Oy = fma(R10, uox, fma(R11, uoy, fma(R12, uoz, F.T[1])));
+        const double Oz = fma(R20, uox, fma(R21, uoy, fma(R22, uoz, F.T[2])));
         const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
         const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
         double ph_t = A.theta0[pmid] + A.K * (Ox - uo + (O2 - uo * uo) * um.w);
@@ -3029,6 +3210,15 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
                     koz = static_cast<float>(A.K * Oz);
         const float pht = static_cast<float>(ph_t);
 
+        float Mr[NM], Mi[NM];
+        if constexpr (UNITS) {
+            const float* mo = A.mom + (static_cast<int64_t>(uhead) * npairs + prc) * (2 * NM);
+#pragma unroll
+            for (int kk = 0; kk < NM; ++kk) {
+                Mr[kk] = live ? mo[2 * kk] : 0.f;
+                Mi[kk] = live ? mo[2 * kk + 1] : 0.f;
+            }
+        } else {
         // W on this lane's row a = q, columns b = 16 s + 8 h + u, folded into
         // H_n = sum_b W y_b^n
         float Hr[DM + 1], Hi[DM + 1];
@@ -3071,7 +3261,6 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
         xm[0] = 1.f;
 #pragma unroll
         for (int m = 1; m <= DM; ++m) xm[m] = xm[m - 1] * xq;
-        float Mr[NM], Mi[NM];
 #pragma unroll
         for (int round = 0; round < 2; ++round) {
             int k = 0;
@@ -3102,6 +3291,7 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
                 }
             }
         }
+        }
         // index of moment (m, n), m + n <= DM, in the order above
         auto mk = [](int m, int n) { return m * (DM + 1) - m * (m - 1) / 2 + n; };
 
@@ -3113,15 +3303,17 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const AtomRec a = tile[lane + 64 * i];
-            float e0 = fmaf(vc[2], a.sz, fmaf(vc[1], a.sy, vc[0] * a.sx));
+            const float sx = a.sx + dshx, sy = a.sy + dshy, sz = a.sz + dshz;
+            float e0 = fmaf(vc[2], sz, fmaf(vc[1], sy, vc[0] * sx));
             if (SECOND) {
-                const float w = fmaf(xc2, a.sz, fmaf(xc1, a.sy, xc0 * a.sx)) - e0;
-                e0 = fmaf(hh, fmaf(-w, w, a.s2), e0);
+                const float w = fmaf(xc2, sz, fmaf(xc1, sy, xc0 * sx)) - e0;
+                const float s2 = UNITS ? fmaf(sz, sz, fmaf(sy, sy, sx * sx)) : a.s2;
+                e0 = fmaf(hh, fmaf(-w, w, s2), e0);
             }
-            const float Pj = fmaf(Pk2, a.sz, fmaf(Pk1, a.sy, Pk0 * a.sx));
-            const float Rj = fmaf(Rk2, a.sz, fmaf(Rk1, a.sy, Rk0 * a.sx));
-            const float Qj = fmaf(Qk2, a.sz, fmaf(Qk1, a.sy, Qk0 * a.sx));
-            const float Sj = fmaf(Sk2, a.sz, fmaf(Sk1, a.sy, Sk0 * a.sx));
+            const float Pj = fmaf(Pk2, sz, fmaf(Pk1, sy, Pk0 * sx));
+            const float Rj = fmaf(Rk2, sz, fmaf(Rk1, sy, Rk0 * sx));
+            const float Qj = fmaf(Qk2, sz, fmaf(Qk1, sy, Qk0 * sx));
+            const float Sj = fmaf(Sk2, sz, fmaf(Sk1, sy, Sk0 * sx));
             float am[D + 1], bn[D + 1];
             am[0] = 1.f;
             bn[0] = 1.f;
@@ -3806,12 +3998,30 @@ extern "C" int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const f
     return MOSAIC_OK;
 }
 
+// Pixel moments of the moment adjoint's units: per tile (unit heads use
+// theirs) and (frame, patch) pair, 2 (D + 3)(D + 4) / 2 floats at D = 6.
+static size_t adjoint_scratch_bytes(int n_tiles, int n_frames, int Ny, int Nz) {
+    const size_t patches = static_cast<size_t>((Ny + MMA_P - 1) / MMA_P) * ((Nz + MMA_P - 1) / MMA_P);
+    constexpr size_t NMF = 2 * (6 + 3) * (6 + 4) / 2;
+    return static_cast<size_t>(std::max(1, n_tiles)) * n_frames * patches * NMF * sizeof(float);
+}
+
+extern "C" int mosaic_kin_adjoint_scratch(int64_t atom_count, int n_species, const int64_t* counts,
+                                          int n_frames, int Ny, int Nz, size_t* bytes) {
+    size_t need = 0;
+    const int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    *bytes = adjoint_scratch_bytes(count_tiles(n_species, counts), n_frames, Ny, Nz);
+    return MOSAIC_OK;
+}
+
 extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64_t* counts,
                                   double k, int n_frames, const float* rot, const float* trans,
                                   int Ny, int Nz, const double* stage_stats,
                                   const double* pixel_geom,
                                   const float* grad_field, float* grad_pos, void* workspace,
-                                  size_t workspace_bytes, int path, mosaic_stream_t stream) {
+                                  size_t workspace_bytes, void* scratch, size_t scratch_bytes,
+                                  int path, mosaic_stream_t stream) {
     size_t need = 0;
     int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
     if (rc != MOSAIC_OK) return rc;
@@ -3892,15 +4102,59 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
         const bool flat = 0.125 * MMA_P * MMA_P * pixel_geom[4] <= 1e-6;
         const dim3 grid(L.n_tiles), blk(64 * MMA_WAVES);
         const dim3 pblk(64 * POLY_WAVES);
-        if (use_poly) {
+        // units of tiles with a scratch for their pixel moments: the largest
+        // unit radius for which the bounds above hold at the chosen degree
+        float grp_rmax = 0.f;
+        if (use_poly && scratch &&
+            scratch_bytes >= adjoint_scratch_bytes(L.n_tiles, n_frames, Ny, Nz)) {
+            auto holds = [&](double r) {
+                const double rm = r / Kk;
+                if (rm / rho_min > 1e-6 || Kk * rm * rm * rm / (rho_min * rho_min) > 1e-7)
+                    return false;
+                if (!use_second && r * r / (2.0 * Kk * rho_min) > 1e-7) return false;
+                const bool ok_r = far && pixel_geom[3] * r <= 1e-6 &&
+                                  Kk * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
+                                  2.0 * r + 2.0 <= 256.0;
+                const int d = poly_degree(ok_r, r, pixel_geom, Kk * omax);
+                return d != 0 && d <= poly_d;
+            };
+            double lo = radius_wl, hi = 256.0;
+            if (holds(lo)) {
+                for (int it = 0; it < 60; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    if (holds(mid)) lo = mid;
+                    else hi = mid;
+                }
+                grp_rmax = static_cast<float>(lo * (1.0 - 1e-6));
+            }
+        }
+        M.grp_r = L.grp_r;
+        M.grp_rmax = grp_rmax;
+        M.n_tiles = L.n_tiles;
+        M.mom = static_cast<float*>(scratch);
+        if (use_poly && grp_rmax > 0.f) {
+            const int npairs = n_frames * M.n_patches;
+            const dim3 mgrid(L.n_tiles, (npairs + ADJ_GROUP - 1) / ADJ_GROUP);
+            if (poly_d == 5) hipLaunchKernelGGL((k_adjoint_moments<5>), mgrid, pblk, 0, st, M);
+            else hipLaunchKernelGGL((k_adjoint_moments<6>), mgrid, pblk, 0, st, M);
+            MOSAIC_HIP_CHECK(hipGetLastError());
             if (poly_d == 5 && use_second)
-                hipLaunchKernelGGL((k_adjoint_poly<5, true>), grid, pblk, 0, st, M);
+                hipLaunchKernelGGL((k_adjoint_poly<5, true, true>), grid, pblk, 0, st, M);
             else if (poly_d == 5)
-                hipLaunchKernelGGL((k_adjoint_poly<5, false>), grid, pblk, 0, st, M);
+                hipLaunchKernelGGL((k_adjoint_poly<5, false, true>), grid, pblk, 0, st, M);
             else if (use_second)
-                hipLaunchKernelGGL((k_adjoint_poly<6, true>), grid, pblk, 0, st, M);
+                hipLaunchKernelGGL((k_adjoint_poly<6, true, true>), grid, pblk, 0, st, M);
             else
-                hipLaunchKernelGGL((k_adjoint_poly<6, false>), grid, pblk, 0, st, M);
+                hipLaunchKernelGGL((k_adjoint_poly<6, false, true>), grid, pblk, 0, st, M);
+        } else if (use_poly) {
+            if (poly_d == 5 && use_second)
+                hipLaunchKernelGGL((k_adjoint_poly<5, true, false>), grid, pblk, 0, st, M);
+            else if (poly_d == 5)
+                hipLaunchKernelGGL((k_adjoint_poly<5, false, false>), grid, pblk, 0, st, M);
+            else if (use_second)
+                hipLaunchKernelGGL((k_adjoint_poly<6, true, false>), grid, pblk, 0, st, M);
+            else
+                hipLaunchKernelGGL((k_adjoint_poly<6, false, false>), grid, pblk, 0, st, M);
         } else if (use_second) {
             if (flat) hipLaunchKernelGGL((k_adjoint_mma<true, 2>), grid, blk, 0, st, M);
             else hipLaunchKernelGGL((k_adjoint_mma<true, 3>), grid, blk, 0, st, M);

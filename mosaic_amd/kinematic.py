@@ -247,10 +247,19 @@ def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream
     if not grad_pos.is_contiguous():
         raise ValueError("grad_pos must be contiguous")
     c, cp_ = _counts_arr(staged.counts)
+    # scratch for the moment form's per-unit pixel moments, kept with the sample
+    nbytes = ctypes.c_size_t(0)
+    check(_lib.lib().mosaic_kin_adjoint_scratch(staged.n_atoms, int(c.shape[0]), cp_, F,
+                                                staged.Ny, staged.Nz, ctypes.byref(nbytes)),
+          "mosaic_kin_adjoint_scratch")
+    scratch = getattr(staged, "adj_scratch", None)
+    if scratch is None or scratch.numel() < nbytes.value:
+        scratch = torch.empty(int(nbytes.value), dtype=torch.uint8, device=staged.workspace.device)
+        staged.adj_scratch = scratch
     check(_lib.lib().mosaic_kin_adjoint(
         staged.n_atoms, int(c.shape[0]), cp_, float(k), F, rot.ctypes.data_as(ctypes.c_void_p),
         trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, staged.stage_stats,
         staged.pixel_geom, _ptr(torch.view_as_real(g)), _ptr(grad_pos),
-        _ptr(staged.workspace), int(staged.workspace.numel()), PATHS[path],
-        _stream_handle(stream)), "mosaic_kin_adjoint")
+        _ptr(staged.workspace), int(staged.workspace.numel()), _ptr(scratch),
+        int(scratch.numel()), PATHS[path], _stream_handle(stream)), "mosaic_kin_adjoint")
     return grad_pos
