@@ -226,13 +226,16 @@ def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
     return out
 
 
-def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream=None):
+def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream=None,
+                share_units=True):
     """Add dL/du (per Angstrom, original atom order) into grad_pos (N, 3) float32.
 
     ``grad_field`` is dL/dRe(A) + i dL/dIm(A) for the field the last
     ``kin_sum`` on ``staged`` produced, (F, Nz, Ny) complex (cast to
     complex64).  ``path`` is "auto", "rows", "mma" or "poly" (moment
     expansion); raises MosaicDeclined outside the chosen form's bounds.
+    ``share_units`` lets the moment form compute each unit's pixel moments
+    once (a scratch kept with ``staged``); False runs it tile by tile.
     """
     F = staged.n_frames
     if staged.pixel_geom is None:
@@ -253,7 +256,9 @@ def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream
                                                 staged.Ny, staged.Nz, ctypes.byref(nbytes)),
           "mosaic_kin_adjoint_scratch")
     scratch = getattr(staged, "adj_scratch", None)
-    if scratch is None or scratch.numel() < nbytes.value:
+    if not share_units:
+        scratch = torch.empty(0, dtype=torch.uint8, device=staged.workspace.device)
+    elif scratch is None or scratch.numel() < nbytes.value:
         scratch = torch.empty(int(nbytes.value), dtype=torch.uint8, device=staged.workspace.device)
         staged.adj_scratch = scratch
     check(_lib.lib().mosaic_kin_adjoint(
