@@ -147,6 +147,7 @@ struct Layout {
     TileRec* tiles;
     AtomRec* atoms;
     float* grp_r;           // [tile][2] radius of the pair / quad headed by the tile, else +inf
+    uint32_t* tile_pos;     // [tile] index in its k-d node | node's tile count << 8
     uint32_t* keys_a;       // species sort
     uint32_t* keys_b;
     uint32_t* vals_a;       // orders (atom index per position)
@@ -321,6 +322,7 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.tiles = c.take<TileRec>(std::max(1, L.n_tiles));
     L.atoms = c.take<AtomRec>(static_cast<size_t>(std::max(1, L.n_tiles)) * TILE);
     L.grp_r = c.take<float>(2 * static_cast<size_t>(std::max(1, L.n_tiles)));
+    L.tile_pos = c.take<uint32_t>(static_cast<size_t>(std::max(1, L.n_tiles)));
     size_t n = static_cast<size_t>(std::max<int64_t>(1, n_atoms));
     L.keys_a = c.take<uint32_t>(n);
     L.keys_b = c.take<uint32_t>(n);
@@ -1019,7 +1021,7 @@ __device__ __forceinline__ AtomRec atom_record(const double (&r)[3], const doubl
 __global__ __launch_bounds__(TILE) void k_tiles(
     const float* __restrict__ pos, const float2* __restrict__ amp,
     const uint32_t* __restrict__ order, SpeciesSplit split, double K,
-    TileRec* __restrict__ tiles, AtomRec* __restrict__ atoms, float* __restrict__ grp_r,
+    TileRec* __restrict__ tiles, AtomRec* __restrict__ atoms,
     unsigned int* __restrict__ radius_bits) {
     __shared__ double lds[TILE / 64];
     const int t = blockIdx.x;
@@ -1049,8 +1051,6 @@ __global__ __launch_bounds__(TILE) void k_tiles(
     atoms[static_cast<int64_t>(t) * TILE + i] = rec;
     const double rmax = block_max(static_cast<double>(rad), lds);
     if (i == 0) {
-        grp_r[2 * t] = INFINITY;
-        grp_r[2 * t + 1] = INFINITY;
         TileRec tr;
         tr.ox = o[0];
         tr.oy = o[1];
@@ -1072,6 +1072,70 @@ __device__ __forceinline__ int kidx(int p) { return p + (p >> 3); }
 constexpr int KDL_BINS = 256;      // coordinate bins of an in-LDS split
 constexpr int KDL_SUBMAX = 1 << (KDL_LEVELS - 1);   // sub-nodes of the deepest split level
 
+// After a restage (new positions, same k-d order): the radii of the pairs and
+// quads of tiles about their unit origins, from the new records.  k_unit_init
+// clears the heads (0) and marks the rest (+inf); k_unit_radii, one
+// workgroup per tile, takes the largest |offset + tile shift| of its atoms
+// into its pair and quad heads (atomic max on the float bits, non-negative).
+__global__ void k_unit_init(const uint32_t* __restrict__ tile_pos, int n_tiles,
+                            float* __restrict__ grp_r) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const int j = tile_pos[t] & 0xFF, ntl = tile_pos[t] >> 8;
+    grp_r[2 * static_cast<int64_t>(t)] = ((j & 1) == 0 && j + 1 < ntl) ? 0.f : INFINITY;
+    grp_r[2 * static_cast<int64_t>(t) + 1] = ((j & 3) == 0 && j + 3 < ntl) ? 0.f : INFINITY;
+}
+
+__global__ __launch_bounds__(TILE) void k_unit_radii(const TileRec* __restrict__ tiles,
+                                                     const AtomRec* __restrict__ atoms,
+                                                     const uint32_t* __restrict__ tile_pos,
+                                                     double K, float* __restrict__ grp_r) {
+    __shared__ float red[TILE / 64][2];
+    const int t = blockIdx.x, i = threadIdx.x, lane = i & 63, wave = i >> 6;
+    const int j = tile_pos[t] & 0xFF, ntl = tile_pos[t] >> 8;
+    const int hp = t - (j & 1), hq = t - (j & 3);
+    const bool pair = (j & ~1) + 1 < ntl, quad = (j & ~3) + 3 < ntl;
+    if (!pair && !quad) return;                      // the whole workgroup
+    const TileRec tr = tiles[t];
+    float dp[3] = {0.f, 0.f, 0.f}, dq[3] = {0.f, 0.f, 0.f};
+    if (pair) {
+        const TileRec a = tiles[hp], b = tiles[hp + 1];
+        dp[0] = static_cast<float>((tr.ox - unit_origin(2, a.ox, b.ox, 0.0, 0.0)) * K);
+        dp[1] = static_cast<float>((tr.oy - unit_origin(2, a.oy, b.oy, 0.0, 0.0)) * K);
+        dp[2] = static_cast<float>((tr.oz - unit_origin(2, a.oz, b.oz, 0.0, 0.0)) * K);
+    }
+    if (quad) {
+        const TileRec a = tiles[hq], b = tiles[hq + 1], c = tiles[hq + 2], d = tiles[hq + 3];
+        dq[0] = static_cast<float>((tr.ox - unit_origin(4, a.ox, b.ox, c.ox, d.ox)) * K);
+        dq[1] = static_cast<float>((tr.oy - unit_origin(4, a.oy, b.oy, c.oy, d.oy)) * K);
+        dq[2] = static_cast<float>((tr.oz - unit_origin(4, a.oz, b.oz, c.oz, d.oz)) * K);
+    }
+    float rp = 0.f, rq = 0.f;
+    if (i < tr.count) {
+        const AtomRec r = atoms[static_cast<int64_t>(t) * TILE + i];
+        const float px = r.sx + dp[0], py = r.sy + dp[1], pz = r.sz + dp[2];
+        const float qx = r.sx + dq[0], qy = r.sy + dq[1], qz = r.sz + dq[2];
+        rp = sqrtf(px * px + py * py + pz * pz);
+        rq = sqrtf(qx * qx + qy * qy + qz * qz);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rp = fmaxf(rp, __shfl_xor(rp, o, 64));
+        rq = fmaxf(rq, __shfl_xor(rq, o, 64));
+    }
+    if (lane == 0) { red[wave][0] = rp; red[wave][1] = rq; }
+    __syncthreads();
+    if (i == 0) {
+        for (int w = 1; w < TILE / 64; ++w) {
+            rp = fmaxf(rp, red[w][0]);
+            rq = fmaxf(rq, red[w][1]);
+        }
+        if (pair) atomicMax(reinterpret_cast<unsigned int*>(grp_r) + 2 * static_cast<int64_t>(hp),
+                            __float_as_uint(rp));
+        if (quad) atomicMax(reinterpret_cast<unsigned int*>(grp_r) + 2 * static_cast<int64_t>(hq) + 1,
+                            __float_as_uint(rq));
+    }
+}
+
 // The last k-d levels and the tile records of one node, in one workgroup of
 // 32 << LV threads (8 positions each).  The node's positions
 // [begin, begin + count) of the order (at most TILE << LV atoms, a whole
@@ -1092,7 +1156,8 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
     const float* __restrict__ pos, const float2* __restrict__ amp, const uint32_t* order_in,
     uint32_t* order,
     const KdNode* __restrict__ nodes, double K, TileRec* __restrict__ tiles,
-    AtomRec* __restrict__ atoms, float* __restrict__ grp_r, unsigned int* __restrict__ radius_bits) {
+    AtomRec* __restrict__ atoms, float* __restrict__ grp_r, uint32_t* __restrict__ tile_pos,
+    unsigned int* __restrict__ radius_bits) {
     constexpr int THREADS = 32 << LV, MAXN = TILE << LV, PT = MAXN / THREADS;
     constexpr int SUB = (1 << LV) - 1, SUBMAX = 1 << (LV - 1), WAVES = THREADS / 64;
     constexpr int NTILE = 1 << LV;
@@ -1325,6 +1390,7 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
             tr.species = static_cast<int>(nd.species);
             tr.count = min(TILE, n - t * TILE);
             tiles[nd.tile0 + t] = tr;
+            tile_pos[nd.tile0 + t] = static_cast<uint32_t>(t) | (static_cast<uint32_t>(ntl) << 8);
         }
     }
     // radii of the pairs (one wave) and quads (two waves) of tiles about their
@@ -3619,15 +3685,16 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         // the final order goes to vals_b (mosaic_kin_restage reads it there)
         if (lcount[0] > 0)
             hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, pos, a2, cur,
-                               L.vals_b, nds, K, L.tiles, L.atoms, L.grp_r, L.radius_bits);
+                               L.vals_b, nds, K, L.tiles, L.atoms, L.grp_r, L.tile_pos,
+                               L.radius_bits);
         if (lcount[1] > 0)
             hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, pos, a2, cur,
                                L.vals_b, nds + lcount[0], K, L.tiles, L.atoms, L.grp_r,
-                               L.radius_bits);
+                               L.tile_pos, L.radius_bits);
         if (lcount[2] > 0)
             hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, pos, a2, cur,
                                L.vals_b, nds + lcount[0] + lcount[1], K, L.tiles, L.atoms,
-                               L.grp_r, L.radius_bits);
+                               L.grp_r, L.tile_pos, L.radius_bits);
     }
     MOSAIC_HIP_CHECK(hipGetLastError());
     unsigned int bits[2] = {0u, 0u};
@@ -3670,7 +3737,13 @@ extern "C" int mosaic_kin_restage(int64_t atom_count, const float* pos, const fl
     // the order of the last mosaic_kin_stage on this workspace is in vals_b
     hipLaunchKernelGGL(k_tiles, dim3(L.n_tiles), dim3(TILE), 0, st, pos,
                        reinterpret_cast<const float2*>(amp), L.vals_b, split, k * INV_2PI_HI,
-                       L.tiles, L.atoms, L.grp_r, L.radius_bits);
+                       L.tiles, L.atoms, L.radius_bits);
+    // the units' radii from the new records (the node places come from the
+    // k-d staging of this workspace)
+    hipLaunchKernelGGL(k_unit_init, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
+                       L.tile_pos, L.n_tiles, L.grp_r);
+    hipLaunchKernelGGL(k_unit_radii, dim3(L.n_tiles), dim3(TILE), 0, st, L.tiles, L.atoms,
+                       L.tile_pos, k * INV_2PI_HI, L.grp_r);
     MOSAIC_HIP_CHECK(hipGetLastError());
     unsigned int bits[2] = {0u, 0u};
     MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));

@@ -200,6 +200,38 @@ def test_restage_moved_atoms(device):
     assert e <= TOL
 
 
+def test_restage_keeps_units(device):
+    """Restaging the staged positions rebuilds the same tiles and re-measures
+    the radii of the pairs and quads of tiles, so the moment sum forms the
+    same units and the image is unchanged; summing each tile alone instead
+    differs by the expansion error (about 1e-6)."""
+    from mosaic_amd import tables
+    w = W.fcc_au_bragg111(n=10, npix=24, seed=5)
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    pos = torch.as_tensor(s.load_chunk_positions(1).astype(np.float32), device=device)
+    k = 2 * np.pi / b.wavelength
+    counts = np.array([pos.shape[0]], np.int64)
+    codes = torch.zeros(pos.shape[0], dtype=torch.uint8, device=device)
+    Ny, Nz = d.shape
+    spc = K.species_table(tables.species_constants(["Au"], b.energy))
+    opts = K.kin_opts(k, False, True, b._polarization_state(), False)
+    coords = torch.as_tensor(np.ascontiguousarray(d.pixel_coordinates / 1e10), device=device)
+    rot = np.asarray(st.rotation, np.float32)[None]
+    tr = np.asarray(st.translation, np.float32)[None]
+    staged = K.stage(pos, codes, None, counts, k, 1, Ny, Nz)
+    a = torch.zeros((1, Nz, Ny), dtype=torch.complex128, device=device)
+    K.kin_sum(staged, spc, opts, rot, tr, coords, a)
+    path_a = staged.path_used
+    K.restage(staged, pos)
+    b2 = torch.zeros_like(a)
+    K.kin_sum(staged, spc, opts, rot, tr, coords, b2)
+    path_b = staged.path_used
+    diff = float((a - b2).abs().max() / a.abs().max())
+    print(f"[restage] same positions: path {path_a} -> {path_b}, max diff {diff:.3e}")
+    assert path_a == path_b and path_a // 10 % 10 == 4
+    assert diff <= 1e-9
+
+
 def test_declines_near_field(device):
     w = W.fcc_au_bragg111(n=4, npix=8)
     d = w["detector"]
