@@ -3388,24 +3388,30 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
                 am[m] = am[m - 1] * Pj * (1.f / m);
                 bn[m] = bn[m - 1] * Rj * (1.f / m);
             }
-            // S_k = sum_mn c_mn M_{(m,n) + shift_k}, shifts (0,0) (1,0) (2,0) (0,1) (0,2)
+            // S_k = sum_mn c_mn M_{(m,n) + shift_k}, shifts (0,0) (1,0) (2,0) (0,1) (0,2),
+            // with c_mn = i^{m+n} (T - i U) = i^{m+n} T + i^{m+n+3} U and T, U
+            // real: each power of i swaps or negates the moment's parts, so a
+            // term costs two FMAs for T and two more only where U is not zero
             float sr[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, si[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+            // (ar, ai) += w i^e (Mr[kk], Mi[kk])
+            auto mac = [&](float w, int e, int kk, float& ar, float& ai) {
+                switch (e & 3) {
+                    case 0: ar = fmaf(w, Mr[kk], ar); ai = fmaf(w, Mi[kk], ai); break;
+                    case 1: ar = fmaf(-w, Mi[kk], ar); ai = fmaf(w, Mr[kk], ai); break;
+                    case 2: ar = fmaf(-w, Mr[kk], ar); ai = fmaf(-w, Mi[kk], ai); break;
+                    default: ar = fmaf(w, Mi[kk], ar); ai = fmaf(-w, Mr[kk], ai); break;
+                }
+            };
 #pragma unroll
             for (int m = 0; m <= D; ++m) {
 #pragma unroll
                 for (int n = 0; n <= D - m; ++n) {
                     const float T = am[m] * bn[n];
+                    const bool qterm = m >= 2 && m - 2 + n <= 3;
+                    const bool sterm = n >= 2 && m + n - 2 <= 3;
                     float U = 0.f;
-                    if (m >= 2 && m - 2 + n <= 3) U = fmaf(Qj, am[m - 2] * bn[n], U);
-                    if (n >= 2 && m + n - 2 <= 3) U = fmaf(Sj, am[m] * bn[n - 2], U);
-                    // c = i^{m+n} (T - i U)
-                    float cr_, ci_;
-                    switch ((m + n) & 3) {
-                        case 0: cr_ = T; ci_ = -U; break;
-                        case 1: cr_ = U; ci_ = T; break;
-                        case 2: cr_ = -T; ci_ = U; break;
-                        default: cr_ = -U; ci_ = -T; break;
-                    }
+                    if (qterm) U = fmaf(Qj, am[m - 2] * bn[n], U);
+                    if (sterm) U = fmaf(Sj, am[m] * bn[n - 2], U);
                     const int k0 = mk(m, n), k1 = mk(m + 1, n), k2 = mk(m + 2, n),
                               k3 = mk(m, n + 1), k4 = mk(m, n + 2);
                     const int ks[5] = {k0, k1, k2, k3, k4};
@@ -3416,8 +3422,8 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaAr
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
                         if (m + n > deg[c]) continue;
-                        sr[c] = fmaf(cr_, Mr[ks[c]], fmaf(-ci_, Mi[ks[c]], sr[c]));
-                        si[c] = fmaf(cr_, Mi[ks[c]], fmaf(ci_, Mr[ks[c]], si[c]));
+                        mac(T, m + n, ks[c], sr[c], si[c]);
+                        if (qterm || sterm) mac(U, m + n + 3, ks[c], sr[c], si[c]);
                     }
                 }
             }
