@@ -91,7 +91,9 @@ int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* specie
  * order), from new positions and amplitudes, keeping the tile assignment.
  * For displacements small against a tile (strain fields, defects, thermal
  * draws, fitting steps) the tiles stay compact; stage_stats reports their
- * radius, which the sum's bounds check as usual.  The counterpart of the
+ * radius, which the sum's bounds check as usual.  The radii of the pairs and
+ * quads of tiles that the moment sum groups into units are measured again
+ * from the new positions.  The counterpart of the
  * reference's per-step re-staging (reload, Morton order and origins per scan
  * step, Experiment.py:305-310; Beam.py:1205-1300), minus the sort.
  * Synchronizes the stream to return stage_stats.
