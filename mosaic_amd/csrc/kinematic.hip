@@ -3019,7 +3019,8 @@ template <int D>
 __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMmaArgs A) {
     constexpr int DM = D + 2;
     constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
-    constexpr int HALF = (NM + 1) / 2;              // reduced in two rounds
+    constexpr int ROUNDS = 3;                       // reduced in three rounds: a
+    constexpr int HALF = (NM + ROUNDS - 1) / ROUNDS; // 32 KB buffer fits 5 workgroups per CU
     constexpr int NVP = (2 * HALF) | 1;
     __shared__ float red[POLY_WAVES][64 * NVP];
     __shared__ double rmx[POLY_WAVES];
@@ -3109,7 +3110,7 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMm
                 }
             }
         }
-        // M_mn = sum over lanes of x_q^m H_n, in two rounds through LDS; the
+        // M_mn = sum over lanes of x_q^m H_n, in ROUNDS rounds through LDS; the
         // lane holding a column's sum writes it
         const float xq = static_cast<float>(q - P.my) * (1.f / 16.f);
         float xm[DM + 1];
@@ -3118,7 +3119,7 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMm
         for (int m = 1; m <= DM; ++m) xm[m] = xm[m - 1] * xq;
         float* out = A.mom + (static_cast<int64_t>(t) * npairs + prc) * (2 * NM);
 #pragma unroll
-        for (int round = 0; round < 2; ++round) {
+        for (int round = 0; round < ROUNDS; ++round) {
             int k = 0;
 #pragma unroll
             for (int m = 0; m <= DM; ++m) {
