@@ -48,9 +48,17 @@ typedef struct {
     int apply_pol;          /* scale by sqrt(P) */
     int apply_decay;        /* scale by R0 / r */
     int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence,
-                               3 matrix-core patches, 4 moment expansion */
+                               3 matrix-core patches, 4 moment expansion,
+                               5 exact per pair */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
+    int analyser_kind;      /* 0 off, 1 top-hat, 2 Darwin roll-off (Beam.py:2350-2378);
+                               any other than 0 takes the exact per-pair path */
+    int pad_;
+    double analyser_axis[3];       /* analyser axis (lab); the reference uses the
+                                      detector's centre pixel (Beam.py:6128-6132) */
+    double analyser_accept_rad;    /* top-hat half-angle */
+    double analyser_darwin_hw_rad; /* Darwin half-width; <= 0: weight 1 */
 } mosaic_kin_opts_t;
 
 /*
@@ -129,31 +137,40 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  *   lab = rot[f] @ sample + trans[f]   (rot row-major float32 3x3, trans Angstrom)
  * onto pixels `coords` (device (3, Ny*Nz) float64 metres), accumulated into
  * `out` (device (n_frames, Nz, Ny) complex64, or complex128 when
- * out_is_c128).  Replaces compute_scattering_cffi (Beam.py:629-816) and the
- * mosaic_fast launch (Beam.py:8246-8632, 9336-9391): same per-pair
- * semantics, with form factor, polarization and decay evaluated once per
- * (tile, pixel) and the phase split into an FP64 (tile, pixel) origin term
- * and an FP32 per-atom offset.  Four kernels: per-tile factors (exact
- * distance per tile and pixel, optional second-order offset term) and, in
- * the far field, a row recurrence (per-pixel factors, far-field origin phase,
- * one sincos per atom per KP pixels), a matrix-core form over 32 x 32 pixel
- * patches (the tile sum as a complex product of per-row and per-column
- * phasors, FP16-split MFMA with FP32 accumulation) and a moment expansion
- * over the same patches (the sum over a unit of 1, 2 or 4 consecutive tiles as
- * a polynomial of degree D = 5 or 6 in the pixel offsets, from (D+1)(D+2)/2
- * moments of the atoms' small-angle phasors; units of more than one tile are
- * used where their radius, measured by mosaic_kin_stage, keeps every error
- * bound); opts->path = 0 picks the moment expansion, else the matrix-core
- * form, else the recurrence, whenever their error bounds hold.
+ * out_is_c128).  Replaces the per-chunk kernel launches of interact_beam_gpu
+ * (Beam.py:6279-6377): the fast path behind dispatch (mosaic_fast,
+ * Beam.py:8246-8632, 9336-9496) and its fallback, the general kernel
+ * (interaction_kernal, Beam.py:1912-2412), with compute_scattering_cffi's
+ * per-pair semantics (Beam.py:663-816).  It works on a staged sample (sample
+ * frame, species codes, incident amplitudes), not on the reference's lab
+ * positions in metres and per-atom table rows: mosaic_scatter below takes
+ * those.  Five kernels:
+ *   - exact per pair (path 5): no expansion; the distance, form factor,
+ *     polarization, decay and analyser weight of every atom-pixel pair, with
+ *     an FP64 (tile, pixel) origin phase and FP32 offsets;
+ *   - per-tile factors (1): exact distance per tile and pixel, form factor,
+ *     polarization and decay once per (tile, pixel), optional second-order
+ *     offset term; it needs the tile radius below 1e-6 of the pixel distance;
+ *   - and, in the far field, a row recurrence (2: per-pixel factors, far-field
+ *     origin phase, one sincos per atom per KP pixels), a matrix-core form over
+ *     32 x 32 pixel patches (3: the tile sum as a complex product of per-row and
+ *     per-column phasors, FP16-split MFMA with FP32 accumulation) and a moment
+ *     expansion over the same patches (4: the sum over a unit of 1, 2 or 4
+ *     consecutive tiles as a polynomial of degree D = 5 or 6 in the pixel
+ *     offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle phasors;
+ *     units of more than one tile are used where their radius, measured by
+ *     mosaic_kin_stage, keeps every error bound).
+ * opts->path = 0 picks the moment expansion, else the matrix-core form, else
+ * the recurrence, else the per-tile form, whenever their error bounds hold,
+ * and the exact kernel otherwise; an analyser always takes the exact kernel.
+ * A forced path (1-4) returns MOSAIC_ERR_UNSUPPORTED (nothing written) when
+ * its bounds fail.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels),
- * 32 (matrix-core) or 45/46 (moment expansion of degree 5/6), plus 100 for
- * the far-field forms with the atoms' second-order phase.  Stream-ordered,
- * no host synchronization.
- * Returns MOSAIC_ERR_UNSUPPORTED (nothing written) when the pixels are too
- * close for the split's error bound -- the reference's "decline"
- * (Beam.py:9466-9468).
+ * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), plus 100 for
+ * the far-field forms with the atoms' second-order phase, or 50 + the
+ * analyser kind (exact).  Stream-ordered, no host synchronization.
  */
 int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const int64_t* species_counts /* host */,
@@ -201,6 +218,33 @@ int mosaic_kin_adjoint(int64_t atom_count, int n_species,
 int mosaic_kin_adjoint_scratch(int64_t atom_count, int n_species,
                                const int64_t* species_counts /* host */, int n_frames, int Ny,
                                int Nz, size_t* bytes);
+
+/*
+ * compute_scattering_cffi on the GPU (Beam.py:629-816): the same arguments
+ * with the same meaning, each array a device pointer -- positions (N, 3)
+ * float64 lab metres, per-atom WK rows (N, 11) float32, f0(0), f', f'' and the
+ * incident amplitude (N,) float32 each, pixel coordinates (Ny*Nz,) float64
+ * metres each -- accumulating the sum into the float32 device outputs
+ * out_r / out_i (+=), ordered on `stream`.  Exact per pair, like the
+ * reference: no far-field expansion, pairs with r == 0 skipped.  The pair
+ * geometry and phase are FP64 (the phase as Theta0 + K (x + r - R0) with
+ * r - R0 in cancellation-free form, so it is free of the reference's
+ * ~1e-5 rad fmod(k (x + r), 2 pi) rounding), the form factor, polarization
+ * and phasor FP32, the accumulation FP64.  `workspace` is device memory of
+ * at least mosaic_scatter_workspace bytes (slice partials).  Returns
+ * MOSAIC_OK, MOSAIC_ERR_ARGS or MOSAIC_ERR_WORKSPACE where the reference
+ * returns 0 or 1 (allocation failure, Beam.py:5768-5769).
+ */
+int mosaic_scatter_workspace(int atom_count, int Ny, int Nz, size_t* bytes);
+
+int mosaic_scatter(int atom_count, const double* positions, const float* f0_params,
+                   const float* f0_zero, int remove_forward, const float* s_anom_real,
+                   const float* s_anom_imag, const float* initial_amp_r,
+                   const float* initial_amp_i, int Ny, int Nz, const double* coords_x,
+                   const double* coords_y, const double* coords_z, double k_val,
+                   int apply_pol, double pol_deg, double pol_ex, double pol_ey,
+                   double pol_ez, int apply_spherical_decay, float* out_r, float* out_i,
+                   void* workspace, size_t workspace_bytes, mosaic_stream_t stream);
 
 /*
  * Incident amplitude per atom by bilinear sampling of the beam's entrance

@@ -31,6 +31,8 @@ EXPORTS = (
     "mosaic_kin_sum",
     "mosaic_kin_adjoint",
     "mosaic_kin_adjoint_scratch",
+    "mosaic_scatter_workspace",
+    "mosaic_scatter",
     "mosaic_ein_bilinear",
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
@@ -64,7 +66,10 @@ class KinOpts(ctypes.Structure):
     _fields_ = [("k", ctypes.c_double), ("remove_forward", ctypes.c_int),
                 ("apply_pol", ctypes.c_int), ("apply_decay", ctypes.c_int),
                 ("path", ctypes.c_int), ("pol_deg", ctypes.c_double),
-                ("pol_e", ctypes.c_double * 3)]
+                ("pol_e", ctypes.c_double * 3), ("analyser_kind", ctypes.c_int),
+                ("pad_", ctypes.c_int), ("analyser_axis", ctypes.c_double * 3),
+                ("analyser_accept_rad", ctypes.c_double),
+                ("analyser_darwin_hw_rad", ctypes.c_double)]
 
 
 class Lens(ctypes.Structure):
@@ -87,6 +92,9 @@ _SIGS = {
     "mosaic_kin_adjoint": [_i64, _i, _vp, _d, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
                            _sz, _vp, _sz, _i, _vp],
     "mosaic_kin_adjoint_scratch": [_i64, _i, _vp, _i, _i, _i, ctypes.POINTER(_sz)],
+    "mosaic_scatter_workspace": [_i, _i, _i, ctypes.POINTER(_sz)],
+    "mosaic_scatter": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _d, _i,
+                       _d, _d, _d, _d, _i, _vp, _vp, _vp, _sz, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,

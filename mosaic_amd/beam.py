@@ -17,19 +17,18 @@ this build raises: the CPU kernels are not part of it.
 Deliberate differences, each bounded by the tests:
 * the sample-to-lab transform ``pos @ R.T + T`` is applied exactly (FP64)
   inside the kernels instead of rounding lab positions to float32 first;
-* chunk fields are accumulated in complex128 and cast once at the end;
-* form factor, polarization and 1/r decay are evaluated once per
-  (256-atom tile, pixel) instead of per atom -- exact to first order in
-  tile radius / distance, with a decline beyond 1e-6.
+* all chunks are staged together and summed in one launch (all poses of a
+  scan too), accumulated in complex128 and cast once at the end;
+* the kernel is chosen by error bounds: the far-field forms hoist form
+  factor, polarization and 1/r decay per (tile, pixel) or per pixel where
+  that is exact to ~1e-7; elsewhere, and with an analyser, the exact
+  per-pair kernel runs.
 """
 import numpy as np
 import torch
 
 from . import kinematic as K
 from . import tables
-
-_ANALYSER_OFF = ("off", "none", "disabled")
-
 
 def _require_gpu():
     if not torch.cuda.is_available():
@@ -150,32 +149,78 @@ class beam:
         return self._energy
 
     # ------------------------------------------------------------ helpers
-    def _species_setup(self, sample):
-        """Global label table and per-chunk code remaps."""
-        labels = []
-        index = {}
-        remaps = []
-        for cid in range(1, int(sample.chunk_total or 0) + 1):
+    # Atoms staged together at most: the staging indexes atoms with 32 bits;
+    # larger samples are summed in groups of whole chunks.
+    MAX_GROUP_ATOMS = 1 << 30
+
+    def _pinned(self, name, n, dtype):
+        """A cached pinned host buffer of at least n elements."""
+        bufs = self.__dict__.setdefault("_pin_bufs", {})
+        b = bufs.get(name)
+        if b is None or b.numel() < n or b.dtype != dtype:
+            b = torch.empty(max(int(n), 1), dtype=dtype, pin_memory=True)
+            bufs[name] = b
+        return b
+
+    def _gather_chunks(self, sample, chunk_ids, labels, index, dev, copy):
+        """Load the chunks ``chunk_ids`` (positions with the sample's thermal
+        displacements, species codes remapped onto the global label table)
+        into one pinned host buffer and copy each chunk to the device on the
+        ``copy`` stream as soon as it is loaded, so that loading the next chunk
+        overlaps the transfer.  Returns device positions (N, 3) float32,
+        codes (N,) uint8, the per-species counts and the copies' event."""
+        sizes = [int(sample.chunk_atom_count(c)) for c in chunk_ids]
+        total = sum(sizes)
+        ev_prev = getattr(self, "_pin_event", None)
+        if ev_prev is not None:
+            ev_prev.synchronize()        # the buffers' previous copies are done
+        hp = self._pinned("pos", 3 * total, torch.float32)
+        hc = self._pinned("codes", total, torch.uint8)
+        d_pos = torch.empty((total, 3), dtype=torch.float32, device=dev)
+        d_cod = torch.empty(total, dtype=torch.uint8, device=dev)
+        copy.wait_stream(torch.cuda.current_stream(dev))
+        hp_np, hc_np = hp.numpy(), hc.numpy()
+        off = 0
+        for cid, n in zip(chunk_ids, sizes):
+            pos = sample.load_chunk_positions(cid, use_gpu=False)
             codes, lab = sample.load_chunk_species(cid, use_gpu=False, codes=True)
+            if n == 0:
+                continue
+            pos = np.asarray(pos, dtype=np.float32).reshape(n, 3)
             rm = np.zeros(max(1, len(lab)), dtype=np.uint8)
             for i, el in enumerate(lab):
                 el = str(el)
                 if el not in index:
+                    if len(labels) >= 16:
+                        raise ValueError("at most 16 species per sample are supported")
                     index[el] = len(labels)
                     labels.append(el)
                 rm[i] = index[el]
-            remaps.append(rm)
-        if len(labels) > 16:
-            raise ValueError("at most 16 species per sample are supported")
-        return labels, remaps
+            hp_np[3 * off:3 * (off + n)] = pos.ravel()
+            hc_np[off:off + n] = rm[np.asarray(codes, dtype=np.int64)]
+            with torch.cuda.stream(copy):
+                d_pos[off:off + n].copy_(hp[3 * off:3 * (off + n)].view(n, 3), non_blocking=True)
+                d_cod[off:off + n].copy_(hc[off:off + n], non_blocking=True)
+            off += n
+        ev = torch.cuda.Event()
+        ev.record(copy)
+        self._pin_event = ev
+        counts = np.bincount(hc_np[:total], minlength=max(1, len(labels))).astype(np.int64)
+        return d_pos, d_cod, counts, ev
 
     def _kin_field(self, sample, coords_A, shape, stage, frames, remove_forward,
-                   apply_polarization, spherical_decay, out_dtype=torch.complex128):
+                   apply_polarization, spherical_decay, out_dtype=torch.complex128,
+                   analyser=None):
         """Sum every chunk of ``sample`` for the poses ``frames`` = (rot, trans).
 
-        Chunks are uploaded from pinned host buffers on a copy stream,
-        overlapping the previous chunk's sum.  Returns a (F, Nz, Ny) device
-        tensor.
+        All chunks go into one device array (pinned host buffer, copies on a
+        side stream overlapping the loading of the next chunk), are staged
+        once and summed in one launch for all poses.  The incident amplitude
+        is sampled at every pose (Beam.py:6279-6290); where it differs
+        between poses (a beam profile the sample moves through), each pose is
+        re-staged with its own amplitudes (no re-sort) and summed alone.
+        ``analyser`` = (kind, axis_m, accept_rad, darwin_hw_rad).  Returns a
+        (F, Nz, Ny) device tensor.
         """
         _require_gpu()
         if self._wavelength is None:
@@ -191,58 +236,62 @@ class beam:
         n_chunks = int(sample.chunk_total or 0)
         if n_chunks == 0:
             return out
-        labels, remaps = self._species_setup(sample)
-        consts = tables.species_constants(labels, self._energy)
-        spc_tab = K.species_table(consts)
-        opts = K.kin_opts(k_val, remove_forward, apply_polarization, self._polarization_state(),
-                          spherical_decay, path=self._kin_path)
+        ana = (0, (1.0, 0.0, 0.0), 0.0, 0.0) if analyser is None else analyser
         E0 = torch.from_numpy(self._beam_E0_map).to(dev)
-        compute = torch.cuda.current_stream()
-        copy = torch.cuda.Stream()
-        staged_prev = None
-
-        def upload(cid):
-            pos = np.ascontiguousarray(sample.load_chunk_positions(cid, use_gpu=False),
-                                       dtype=np.float32)
-            codes, _ = sample.load_chunk_species(cid, use_gpu=False, codes=True)
-            codes = remaps[cid - 1][np.asarray(codes, dtype=np.int64)] if len(codes) else \
-                np.zeros(0, dtype=np.uint8)
-            counts = np.bincount(codes, minlength=len(labels)).astype(np.int64)
-            h_pos = torch.from_numpy(pos).pin_memory()
-            h_cod = torch.from_numpy(np.ascontiguousarray(codes, dtype=np.uint8)).pin_memory()
-            with torch.cuda.stream(copy):
-                d_pos = h_pos.to(dev, non_blocking=True)
-                d_cod = h_cod.to(dev, non_blocking=True)
-                ev = torch.cuda.Event()
-                ev.record(copy)
-            return d_pos, d_cod, counts, ev, (h_pos, h_cod)
-
-        pending = upload(1)
+        compute = torch.cuda.current_stream(dev)
+        copy = self.__dict__.setdefault("_copy_stream", torch.cuda.Stream(dev))
+        # chunk groups of at most MAX_GROUP_ATOMS atoms (one group normally)
+        groups, cur, cur_n = [], [], 0
         for cid in range(1, n_chunks + 1):
-            d_pos, d_cod, counts, ev, hold = pending
+            n = int(sample.chunk_atom_count(cid))
+            if cur and cur_n + n > self.MAX_GROUP_ATOMS:
+                groups.append(cur)
+                cur, cur_n = [], 0
+            cur.append(cid)
+            cur_n += n
+        groups.append(cur)
+        labels, index = [], {}
+        staged_prev = None
+        for g in groups:
+            d_pos, d_cod, counts, ev = self._gather_chunks(sample, g, labels, index, dev, copy)
             compute.wait_event(ev)
-            d_pos.record_stream(compute)
-            d_cod.record_stream(compute)
-            if cid < n_chunks:
-                pending = upload(cid + 1)
-            n = int(d_pos.shape[0])
-            if n == 0:
+            if d_pos.shape[0] == 0:
                 continue
-            amp = K.ein_bilinear(d_pos, rot[0], trans[0], E0, self._beam_du, self._beam_dv,
-                                 self._beam_uc, self._beam_vc)
-            staged = K.stage(d_pos, d_cod, amp, counts, k_val, F, Ny, Nz,
+            consts = tables.species_constants(labels, self._energy)
+            spc_tab = K.species_table(consts)
+            counts = np.concatenate([counts, np.zeros(len(labels) - counts.shape[0],
+                                                      dtype=np.int64)])[:len(labels)]
+            opts = K.kin_opts(k_val, remove_forward, apply_polarization,
+                              self._polarization_state(), spherical_decay, path=self._kin_path,
+                              analyser=ana[0], analyser_axis=ana[1], accept_rad=ana[2],
+                              darwin_hw_rad=ana[3])
+            def ein(f):
+                return K.ein_bilinear(d_pos, rot[f], trans[f], E0, self._beam_du,
+                                      self._beam_dv, self._beam_uc, self._beam_vc)
+
+            amp0 = ein(0)
+            same = all(torch.equal(amp0, ein(f)) for f in range(1, F))
+            staged = K.stage(d_pos, d_cod, amp0, counts, k_val, F, Ny, Nz,
                              workspace=self._workspace)
-            if staged_prev is not None and staged.workspace is staged_prev.workspace:
+            if (staged_prev is not None and staged.workspace is staged_prev.workspace
+                    and staged_prev.pixels_key is not None
+                    and staged_prev.counts.shape == staged.counts.shape):
                 # pixel tables sit at the head of the workspace and survive staging
                 staged.pixels_key = staged_prev.pixels_key
                 staged.pixel_geom = staged_prev.pixel_geom
             self._workspace = staged.workspace
-            K.kin_sum(staged, spc_tab, opts, rot, trans, coords, out)
+            if same:
+                K.kin_sum(staged, spc_tab, opts, rot, trans, coords, out)
+            else:
+                for f in range(F):
+                    if f > 0:
+                        K.restage(staged, d_pos, ein(f))
+                    K.kin_sum(staged, spc_tab, opts, rot[f:f + 1], trans[f:f + 1], coords,
+                              out[f:f + 1])
             self._last_path = staged.path_used
             self._last_stats = (staged.radius_wl, tuple(staged.pixel_geom))
             staged_prev = staged
-            del hold
-        del staged_prev
+            del d_pos, d_cod, amp0
         return out
 
     # ------------------------------------------------------- public calls
@@ -253,20 +302,31 @@ class beam:
                           analyser_darwin_halfwidth_rad=0.0):
         if use_depth_ein:
             raise NotImplementedError("depth-dependent Ein is not part of this build")
-        off = (analyser_mode is None
-               or (isinstance(analyser_mode, str) and analyser_mode.strip().lower() in _ANALYSER_OFF)
-               or (not isinstance(analyser_mode, str) and analyser_mode == 0))
-        if not off:
-            raise NotImplementedError("analyser acceptance is not part of this build")
         Ny, Nz = measurement_shape
         rot = np.asarray(stage.rotation, dtype=np.float32).reshape(1, 3, 3)
         trans = np.asarray(stage.translation, dtype=np.float32).reshape(1, 3)
         mp = measurement_positions
         if isinstance(mp, torch.Tensor):
             mp = mp.detach().cpu().numpy()
+        mp = np.asarray(mp, dtype=np.float64)
         out = self._kin_field(sample, mp, (Ny, Nz), stage, (rot, trans), remove_forward,
-                              apply_polarization, spherical_decay)
+                              apply_polarization, spherical_decay,
+                              analyser=self._analyser(mp, Ny, Nz, analyser_mode,
+                                                      analyser_acceptance_angle_rad,
+                                                      analyser_darwin_halfwidth_rad))
         return out[0].to(torch.complex64).cpu().numpy()
+
+    @staticmethod
+    def _analyser(mp, Ny, Nz, mode, accept_rad, darwin_hw_rad):
+        """(kind, axis [m], accept, hw) of the reference's analyser: its axis
+        is the direction of the detector's centre pixel (Ny // 2, Nz // 2)
+        (Beam.py:6128-6143)."""
+        kind = K.analyser_kind(mode)
+        if kind == 0:
+            return None
+        c = (Nz // 2) * Ny + Ny // 2
+        axis = tuple(float(v) / 1e10 for v in np.asarray(mp)[:, c])
+        return (kind, axis, float(accept_rad), float(darwin_hw_rad))
 
     def atomic_scattering_kinematic(self, sample, detector, stage, offset=None, use_gpu=True,
                                     remove_forward=False, use_depth_ein=False,
@@ -307,7 +367,8 @@ class beam:
         absolutely and the stage rotation/translation recorded, as one
         ``Experiment.scan_nD`` step would (Experiment.py:285-320).  Returns a
         (n_angles, Nz, Ny) device tensor.  The incident amplitude is sampled
-        at the first pose.
+        at every pose; where it is the same for all of them, the poses run in
+        one launch.
         """
         start = stage.get_single_motor_value(motor, degrees=True)
         rots, trs = [], []

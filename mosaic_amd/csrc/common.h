@@ -31,6 +31,33 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
     e = (a - (s - bb)) + (b - bb);
 }
 
+// frac(K |T|) in double-double (K = k / 2 pi), with |T| = r + r_lo.
+__device__ __forceinline__ void origin_phase(double tx, double ty, double tz, double k,
+                                             double& f, double& r, double& r_lo) {
+    // |T|^2 in double-double
+    double a, ae, b, be, c, ce, s, se, s2, se2;
+    two_prod(tx, tx, a, ae);
+    two_prod(ty, ty, b, be);
+    two_prod(tz, tz, c, ce);
+    two_sum(a, b, s, se);
+    two_sum(s, c, s2, se2);
+    const double sq_hi = s2, sq_lo = (se + se2) + (ae + be + ce);
+    r = sqrt(sq_hi);
+    double rr, rre;
+    two_prod(r, r, rr, rre);
+    const double resid = ((sq_hi - rr) - rre) + sq_lo;
+    r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
+    double K_hi, K_lo;
+    two_prod(k, INV_2PI_HI, K_hi, K_lo);
+    K_lo = fma(k, INV_2PI_LO, K_lo);
+    double th, the;
+    two_prod(K_hi, r, th, the);
+    the += K_hi * r_lo + K_lo * r;
+    const double n = floor(th);
+    f = (th - n) + the;
+    f -= floor(f);
+}
+
 // Carves aligned sub-buffers out of one caller-provided workspace.  With a
 // null base it only measures.
 struct Carver {

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pair_physics.h"
 
 namespace mosaic {
 namespace {
@@ -444,33 +445,6 @@ __device__ __forceinline__ PatchGeom patch_of(int py, int pz, int Ny, int Nz) {
     g.my = g.vy / 2;
     g.mz = g.vz / 2;
     return g;
-}
-
-// frac(K |T|) in double-double (K = k / 2 pi), with |T| = r + r_lo.
-__device__ __forceinline__ void origin_phase(double tx, double ty, double tz, double k,
-                                             double& f, double& r, double& r_lo) {
-    // |T|^2 in double-double
-    double a, ae, b, be, c, ce, s, se, s2, se2;
-    two_prod(tx, tx, a, ae);
-    two_prod(ty, ty, b, be);
-    two_prod(tz, tz, c, ce);
-    two_sum(a, b, s, se);
-    two_sum(s, c, s2, se2);
-    const double sq_hi = s2, sq_lo = (se + se2) + (ae + be + ce);
-    r = sqrt(sq_hi);
-    double rr, rre;
-    two_prod(r, r, rr, rre);
-    const double resid = ((sq_hi - rr) - rre) + sq_lo;
-    r_lo = (r > 0.0) ? resid / (2.0 * r) : 0.0;
-    double K_hi, K_lo;
-    two_prod(k, INV_2PI_HI, K_hi, K_lo);
-    K_lo = fma(k, INV_2PI_LO, K_lo);
-    double th, the;
-    two_prod(K_hi, r, th, the);
-    the += K_hi * r_lo + K_lo * r;
-    const double n = floor(th);
-    f = (th - n) + the;
-    f -= floor(f);
 }
 
 // Per pixel:
@@ -1618,6 +1592,221 @@ __global__ __launch_bounds__(SUM_THREADS) void k_sum(const SumArgs A) {
                 const float cs = __builtin_amdgcn_cosf(r);
                 S_r[i] = fmaf(a.mag, cs, S_r[i]);
                 S_i[i] = fmaf(a.mag, sn, S_i[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) {
+            const double g_r = gr[i], g_i = gi[i], s_r = S_r[i], s_i = S_i[i];
+            acc_r[i] = fma(g_r, s_r, fma(-g_i, s_i, acc_r[i]));
+            acc_i[i] = fma(g_r, s_i, fma(g_i, s_r, acc_i[i]));
+        }
+    }
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+#pragma unroll
+    for (int i = 0; i < PIX; ++i)
+        if (valid[i]) out[pix[i]] = make_double2(acc_r[i], acc_i[i]);
+}
+
+// Exact per-pair form of the sum, with no expansion in tile radius / distance:
+// the form every other kernel declines to (detectors within ~1e6 tile radii,
+// nanometres to millimetres away) and the one that applies the analyser, a
+// per-ray acceptance (Beam.py:2350-2378).  The reference's general kernel
+// (interaction_kernal, Beam.py:1912-2412) in its exact mode.
+//
+// Per (tile, pixel), FP64, sample frame: the vector from the tile origin to
+// the pixel, D = rho u, and the origin phase
+//     Theta = frac(Theta0 + K (X_t + rho - rho0)),
+//     rho - rho0 = (|O|^2 - 2 T.O) / (rho + rho0)          (no cancellation).
+// Per pair, FP32 and exact, with sigma the atom's offset in wavelengths,
+// h = 1 / (K rho), s = u . sigma, x = R^T x_hat and v = x - u:
+//     r / rho = q = sqrt(1 + t),  t = h (|sigma|^2 h - 2 s),
+//     psi = x . sigma + K (r - rho) = v . sigma + (s t / (1 + q) + |sigma|^2 h) / (1 + q),
+//     1 - cos 2theta = (m0 + h psi) / q,  m0 = 1 - x . u   (m0 from FP64),
+//     e x r_hat = (e x u - h e x sigma) / q,  R0 / r = (R0 / rho) / q,
+// so nothing of size rho enters the FP32 arithmetic and every difference
+// that could cancel is formed from small terms.  The form factor (five
+// v_exp_f32), sqrt(P), the decay and the analyser weight are evaluated per
+// pair (pair_physics.h); the tile sum multiplies r_e e^{2 pi i Theta} into
+// the FP64 accumulators.
+struct ExactArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double* cx;
+    const double* cy;
+    const double* cz;
+    const double* theta0;
+    const double* rho0;
+    const double* qcut;
+    double2* partial;
+    int n_tiles;
+    int tiles_per_slice;
+    int n_slices;
+    int npix;
+    double K;         // 1 / lambda
+    double k;         // 2 pi / lambda
+    double pol_e[3];  // polarization vector (lab)
+    double ana_c[3];  // analyser axis (lab, unit)
+    float pol_deg;
+    float ana_accept; // top-hat half-angle [rad]
+    float ana_inv_hw; // 1 / Darwin half-width [1/rad], 0 for none
+    int remove_forward;
+    int apply_decay;
+    FormRow form[MAX_SPECIES];
+};
+
+template <int PIX, bool POL, int ANA>
+__global__ __launch_bounds__(SUM_THREADS) void k_sum_exact(const ExactArgs A) {
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+
+    int pix[PIX];
+    bool valid[PIX];
+    float ocut[PIX];
+#pragma unroll
+    for (int i = 0; i < PIX; ++i) {
+        const int p = blockIdx.x * (SUM_THREADS * PIX) + i * SUM_THREADS + threadIdx.x;
+        valid[i] = p < A.npix;
+        pix[i] = valid[i] ? p : A.npix - 1;
+        // Q < Q_cut  <=>  1 - cos 2theta < (Q_cut / k)^2 / 2
+        const double qc = A.qcut[pix[i]] / A.k;
+        ocut[i] = A.remove_forward ? static_cast<float>(0.5 * qc * qc) : -1.0f;
+    }
+    double acc_r[PIX], acc_i[PIX];
+#pragma unroll
+    for (int i = 0; i < PIX; ++i) { acc_r[i] = 0.0; acc_i[i] = 0.0; }
+
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    // lab vectors in the sample frame: R^T x_hat, R^T e, R^T c
+    const double xs0 = F.xs[0], xs1 = F.xs[1], xs2 = F.xs[2];
+    const double es0 = R00 * A.pol_e[0] + R10 * A.pol_e[1] + R20 * A.pol_e[2];
+    const double es1 = R01 * A.pol_e[0] + R11 * A.pol_e[1] + R21 * A.pol_e[2];
+    const double es2 = R02 * A.pol_e[0] + R12 * A.pol_e[1] + R22 * A.pol_e[2];
+    const double cs0 = R00 * A.ana_c[0] + R10 * A.ana_c[1] + R20 * A.ana_c[2];
+    const double cs1 = R01 * A.ana_c[0] + R11 * A.ana_c[1] + R21 * A.ana_c[2];
+    const double cs2 = R02 * A.ana_c[0] + R12 * A.ana_c[1] + R22 * A.ana_c[2];
+    const float fes0 = static_cast<float>(es0), fes1 = static_cast<float>(es1),
+                fes2 = static_cast<float>(es2);
+    const float fcs0 = static_cast<float>(cs0), fcs1 = static_cast<float>(cs1),
+                fcs2 = static_cast<float>(cs2);
+    const float pdeg = A.pol_deg;
+
+    for (int t = t0; t < t1; ++t) {
+        const TileRec tr = A.tiles[t];
+        const FormRow& fr = A.form[tr.species];
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+
+        float ux[PIX], uy[PIX], uz[PIX], vx[PIX], vy[PIX], vz[PIX], hh[PIX], m0[PIX];
+        float dmul[PIX], dadd[PIX], gr[PIX], gi[PIX];
+        float eux[PIX], euy[PIX], euz[PIX], cux[PIX], cuy[PIX], cuz[PIX], c0[PIX];
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) {
+            const int p = pix[i];
+            const double tx = A.cx[p], ty = A.cy[p], tz = A.cz[p];
+            const double Dx = tx - Ox, Dy = ty - Oy, Dz = tz - Oz;
+            const double rho = sqrt(Dx * Dx + Dy * Dy + Dz * Dz);
+            const double r0 = A.rho0[p];
+            const double dR = (O2 - 2.0 * (tx * Ox + ty * Oy + tz * Oz)) / (rho + r0);
+            double th = A.theta0[p] + A.K * (Ox + dR);
+            th -= rint(th);
+            const double inv = 1.0 / rho;
+            const double lx = Dx * inv, ly = Dy * inv, lz = Dz * inv;   // lab unit vector
+            const double sx = R00 * lx + R10 * ly + R20 * lz;           // sample frame
+            const double sy = R01 * lx + R11 * ly + R21 * lz;
+            const double sz = R02 * lx + R12 * ly + R22 * lz;
+            ux[i] = static_cast<float>(sx);
+            uy[i] = static_cast<float>(sy);
+            uz[i] = static_cast<float>(sz);
+            vx[i] = static_cast<float>(xs0 - sx);
+            vy[i] = static_cast<float>(xs1 - sy);
+            vz[i] = static_cast<float>(xs2 - sz);
+            hh[i] = static_cast<float>(inv / A.K);
+            m0[i] = static_cast<float>((Dx > 0.0) ? (Dy * Dy + Dz * Dz) * inv / (rho + Dx) : 1.0 - lx);
+            dmul[i] = (A.apply_decay && r0 > 0.0) ? static_cast<float>(r0 * inv) : 0.0f;
+            dadd[i] = (A.apply_decay && r0 > 0.0) ? 0.0f : 1.0f;
+            if (POL) {
+                eux[i] = static_cast<float>(es1 * sz - es2 * sy);
+                euy[i] = static_cast<float>(es2 * sx - es0 * sz);
+                euz[i] = static_cast<float>(es0 * sy - es1 * sx);
+            }
+            if (ANA) {
+                cux[i] = static_cast<float>(cs1 * sz - cs2 * sy);
+                cuy[i] = static_cast<float>(cs2 * sx - cs0 * sz);
+                cuz[i] = static_cast<float>(cs0 * sy - cs1 * sx);
+                c0[i] = static_cast<float>(cs0 * sx + cs1 * sy + cs2 * sz);
+            }
+            const float thf = static_cast<float>(th);
+            gr[i] = static_cast<float>(R_E) * __builtin_amdgcn_cosf(thf);
+            gi[i] = static_cast<float>(R_E) * __builtin_amdgcn_sinf(thf);
+        }
+
+        float S_r[PIX], S_i[PIX];
+#pragma unroll
+        for (int i = 0; i < PIX; ++i) { S_r[i] = 0.f; S_i[i] = 0.f; }
+        const AtomRec* __restrict__ at = A.atoms + static_cast<int64_t>(t) * TILE;
+        const int cnt = tr.count;
+        for (int j = 0; j < cnt; ++j) {
+            const AtomRec a = at[j];
+            // per-atom cross products with the polarization and analyser vectors
+            float ex_ = 0.f, ey_ = 0.f, ez_ = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f, cd_ = 0.f;
+            if (POL) {
+                ex_ = fmaf(fes1, a.sz, -fes2 * a.sy);
+                ey_ = fmaf(fes2, a.sx, -fes0 * a.sz);
+                ez_ = fmaf(fes0, a.sy, -fes1 * a.sx);
+            }
+            if (ANA) {
+                cx_ = fmaf(fcs1, a.sz, -fcs2 * a.sy);
+                cy_ = fmaf(fcs2, a.sx, -fcs0 * a.sz);
+                cz_ = fmaf(fcs0, a.sy, -fcs1 * a.sx);
+                cd_ = fmaf(fcs2, a.sz, fmaf(fcs1, a.sy, fcs0 * a.sx));
+            }
+#pragma unroll
+            for (int i = 0; i < PIX; ++i) {
+                const float h = hh[i];
+                const float s = fmaf(uz[i], a.sz, fmaf(uy[i], a.sy, ux[i] * a.sx));
+                const float s2h = a.s2 * h;
+                const float tt = (s2h - 2.0f * s) * h;
+                const float q2 = fmaxf(1.0f + tt, 1e-30f);
+                const float invq = __builtin_amdgcn_rsqf(q2);
+                const float q = q2 * invq;
+                const float ip = __builtin_amdgcn_rcpf(1.0f + q);
+                const float corr = fmaf(s * tt, ip, s2h) * ip;
+                const float psi = fmaf(vz[i], a.sz, fmaf(vy[i], a.sy, fmaf(vx[i], a.sx, corr)));
+                const float omc = fmaf(h, psi, m0[i]) * invq;
+                float sr = form_f0(fr.a, fr.nb, fr.c, omc) + fr.fp, si = fr.fpp;
+                if (omc < ocut[i]) {
+                    sr -= fr.f0z + fr.fp;
+                    si -= fr.fpp;
+                }
+                float wt = a.mag * fmaf(dmul[i], invq, dadd[i]);
+                if (POL) {
+                    const float wx = fmaf(-h, ex_, eux[i]), wy = fmaf(-h, ey_, euy[i]),
+                                wz = fmaf(-h, ez_, euz[i]);
+                    const float ec2 = fmaf(wz, wz, fmaf(wy, wy, wx * wx)) * (invq * invq);
+                    wt *= pol_factor(1.0f - omc, ec2, pdeg);
+                }
+                if (ANA) {
+                    const float wx = fmaf(-h, cx_, cux[i]), wy = fmaf(-h, cy_, cuy[i]),
+                                wz = fmaf(-h, cz_, cuz[i]);
+                    const float sd = __builtin_sqrtf(fmaf(wz, wz, fmaf(wy, wy, wx * wx)));
+                    wt *= analyser_weight<ANA>(sd, fmaf(-h, cd_, c0[i]), A.ana_accept,
+                                               A.ana_inv_hw);
+                }
+                // skip an atom sitting on the pixel (r == 0, Beam.py:765)
+                wt = (1.0f + tt) > 0.0f ? wt : 0.0f;
+                const float arg = __builtin_amdgcn_fractf(psi + a.ph);
+                const float cr = wt * __builtin_amdgcn_cosf(arg);
+                const float ci = wt * __builtin_amdgcn_sinf(arg);
+                S_r[i] = fmaf(sr, cr, fmaf(-si, ci, S_r[i]));
+                S_i[i] = fmaf(sr, ci, fmaf(si, cr, S_i[i]));
             }
         }
 #pragma unroll
@@ -3549,6 +3738,14 @@ hipError_t launch_sum(const SumArgs& A, int pix_blocks, int n_slices, int n_fram
     return hipGetLastError();
 }
 
+template <bool POL, int ANA>
+hipError_t launch_exact(const ExactArgs& A, int n_slices, int n_frames, hipStream_t st) {
+    constexpr int PIX = 4;
+    dim3 grid(blocks_for(A.npix, SUM_THREADS * PIX), n_slices, n_frames);
+    hipLaunchKernelGGL((k_sum_exact<PIX, POL, ANA>), grid, dim3(SUM_THREADS), 0, st, A);
+    return hipGetLastError();
+}
+
 }  // namespace
 }  // namespace mosaic
 
@@ -3869,7 +4066,8 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
         !pixel_geom)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 4)
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 5 ||
+        opts->analyser_kind < 0 || opts->analyser_kind > 2)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
@@ -3881,15 +4079,16 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
 
     // Error bounds.  Per-tile path: tile radius against the nearest pixel
-    // distance (tiles move rigidly with the sample pose).
+    // distance (tiles move rigidly with the sample pose).  Where it fails,
+    // the automatic choice takes the exact per-pair kernel, which has no
+    // such bound.
     const double radius_wl = stage_stats[0];
     const double rho_min = pixel_geom[0], du1 = pixel_geom[1], du2 = pixel_geom[2];
     const double dpatch = pixel_geom[3];
     const double rad_m = radius_wl / K;
-    if (!(rho_min > 0.0) || rad_m / rho_min > 1e-6) return MOSAIC_ERR_UNSUPPORTED;
     const double third = K * rad_m * rad_m * rad_m / (rho_min * rho_min);
-    if (third > 1e-7) return MOSAIC_ERR_UNSUPPORTED;
-    const double second = radius_wl * radius_wl / (2.0 * K * rho_min);
+    const bool tile_ok = rho_min > 0.0 && rad_m / rho_min <= 1e-6 && third <= 1e-7;
+    const double second = tile_ok ? radius_wl * radius_wl / (2.0 * K * rho_min) : 0.0;
     const bool use_second = second > 1e-7;
 
     std::vector<FrameRec> fr(n_frames);
@@ -3907,7 +4106,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // order of the origin phase below 1e-8 rev, the row line fit below 1e-6
     // rev and the step phasor inside its Taylor range.
     const double omax = stage_stats[1] + tmax;
-    const bool far = omax / rho_min <= 1e-6 &&
+    const bool far = tile_ok && omax / rho_min <= 1e-6 &&
                      K * omax * omax * omax / (rho_min * rho_min) <= 1e-8;
     const bool rec_geom = far && 6.2831853 * du1 * radius_wl <= 0.05;
     int kp = 0;
@@ -3929,8 +4128,14 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
                K * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 && 2.0 * r + 2.0 <= 256.0;
     };
     const int poly_d = poly_degree(poly_patch_ok(radius_wl), radius_wl, pixel_geom, K * omax, true);
-    int path = opts->path == 0 ? (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : 1))) : opts->path;
-    if ((path == 2 && kp == 0) || (path == 3 && !mma_ok) || (path == 4 && poly_d == 0))
+    // The analyser weighs each ray (atom -> pixel) by its own direction: only
+    // the exact kernel applies it.
+    const int ana = opts->analyser_kind;
+    int path = opts->path;
+    if (path == 0)
+        path = ana ? 5 : (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : (tile_ok ? 1 : 5))));
+    if ((path == 1 && !tile_ok) || (path == 2 && kp == 0) || (path == 3 && !mma_ok) ||
+        (path == 4 && poly_d == 0) || (ana && path != 5))
         return MOSAIC_ERR_UNSUPPORTED;
     // Moment sum over units of 2 or 4 tiles: the largest unit radius for
     // which every radius-dependent bound above still holds at the chosen
@@ -3958,7 +4163,52 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     MOSAIC_HIP_CHECK(hipMemcpyAsync(L.frames, fr.data(), sizeof(FrameRec) * n_frames,
                                     hipMemcpyHostToDevice, st));
     hipError_t e;
-    if (path == 3 || path == 4) {
+    if (path == 5) {
+        ExactArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.cx = coords;
+        A.cy = coords + npix;
+        A.cz = coords + 2 * (int64_t)npix;
+        A.theta0 = L.px.theta0;
+        A.rho0 = L.px.rho0;
+        A.qcut = L.px.qcut;
+        A.partial = L.partial;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.npix = npix;
+        A.K = K;
+        A.k = opts->k;
+        double an = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            A.pol_e[d] = opts->pol_e[d];
+            an += opts->analyser_axis[d] * opts->analyser_axis[d];
+        }
+        an = an > 0.0 ? 1.0 / std::sqrt(an) : 0.0;
+        for (int d = 0; d < 3; ++d) A.ana_c[d] = opts->analyser_axis[d] * an;
+        A.pol_deg = static_cast<float>(opts->pol_deg);
+        A.ana_accept = static_cast<float>(opts->analyser_accept_rad);
+        A.ana_inv_hw = opts->analyser_darwin_hw_rad > 0.0
+                           ? static_cast<float>(1.0 / opts->analyser_darwin_hw_rad)
+                           : 0.0f;
+        A.remove_forward = opts->remove_forward ? 1 : 0;
+        A.apply_decay = opts->apply_decay ? 1 : 0;
+        for (int sp = 0; sp < n_species; ++sp)
+            A.form[sp] = make_form_row(species_tab[sp].wk, species_tab[sp].f0_zero,
+                                       species_tab[sp].fp, species_tab[sp].fpp, opts->k);
+        const bool pol = opts->apply_pol != 0;
+        if (ana == 1)
+            e = pol ? launch_exact<true, 1>(A, L.n_slices, n_frames, st)
+                    : launch_exact<false, 1>(A, L.n_slices, n_frames, st);
+        else if (ana == 2)
+            e = pol ? launch_exact<true, 2>(A, L.n_slices, n_frames, st)
+                    : launch_exact<false, 2>(A, L.n_slices, n_frames, st);
+        else
+            e = pol ? launch_exact<true, 0>(A, L.n_slices, n_frames, st)
+                    : launch_exact<false, 0>(A, L.n_slices, n_frames, st);
+    } else if (path == 3 || path == 4) {
         MmaArgs A;
         A.atoms = L.atoms;
         A.tiles = L.tiles;
@@ -4054,7 +4304,8 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
                            L.partial, L.n_slices, npix, n_frames, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
     if (path_used)
-        *path_used = path == 4   ? 40 + poly_d + (use_second ? 100 : 0)
+        *path_used = path == 5   ? 50 + ana
+                     : path == 4 ? 40 + poly_d + (use_second ? 100 : 0)
                      : path == 3 ? 32 + (use_second ? 100 : 0)
                      : path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
                                  : (use_second ? 12 : 11);
@@ -4141,8 +4392,12 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
                      Kk * omax * omax * omax / (rho_min * rho_min) <= 1e-8;
     const bool ok = far && 6.2831853 * du1 * radius_wl <= 0.05;
     // the matrix-core form under the forward's bounds, else the recurrence
+    // (the atom phases, |v.sigma| + 1 <= 2 radius + 1 revolutions, go to
+    // v_sin / v_cos unreduced, so they must stay inside their +-256 range; the
+    // moment form takes the same gate)
     const bool mma_ok = far && pixel_geom[3] * radius_wl <= 1e-6 &&
-                        Kk * omax * omax * (MMA_P * du1) / rho_min <= 1e-7;
+                        Kk * omax * omax * (MMA_P * du1) / rho_min <= 1e-7 &&
+                        2.0 * radius_wl + 2.0 <= 256.0;
     int kp = 0;
     if (ok && rec_curv_bound(16, du2, radius_wl) <= 1e-6) kp = 16;
     else if (ok && rec_curv_bound(8, du2, radius_wl) <= 1e-6) kp = 8;

@@ -36,15 +36,34 @@ def species_table(consts):
     return arr
 
 
-PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4}
+PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4, "exact": 5}
+
+# analyser_mode names of the reference's GPU driver (Beam.py:6134-6143)
+_ANALYSER = {"off": 0, "none": 0, "disabled": 0, "top_hat": 1, "tophat": 1, "top-hat": 1,
+             "top": 1, "darwin": 2, "rolloff": 2, "roll-off": 2}
+
+
+def analyser_kind(mode):
+    """0 (off), 1 (top-hat) or 2 (Darwin) from the reference's analyser_mode
+    argument: a name, an int, or None; unknown values mean off
+    (Beam.py:6134-6143)."""
+    if mode is None:
+        return 0
+    if isinstance(mode, str):
+        return _ANALYSER.get(mode.strip().lower(), 0)
+    k = int(mode)
+    return k if k in (0, 1, 2) else 0
 
 
 def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
-             apply_decay=False, path="auto"):
-    """mosaic_kin_opts_t; ``path`` is "auto", "tile" (per-tile factors),
-    "rows" (row recurrence), "mma" (matrix-core patches) or "poly" (moment
-    expansion over the same patches); the last three decline when their
-    bounds do not hold."""
+             apply_decay=False, path="auto", analyser=0, analyser_axis=(1.0, 0.0, 0.0),
+             accept_rad=0.0, darwin_hw_rad=0.0):
+    """mosaic_kin_opts_t; ``path`` is "auto", "exact" (per pair), "tile"
+    (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches)
+    or "poly" (moment expansion over the same patches); the last four decline
+    when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
+    top-hat, 2 Darwin) with its axis (lab), acceptance half-angle and Darwin
+    half-width; it takes the exact kernel."""
     o = KinOpts()
     o.k = float(k)
     o.remove_forward = int(bool(remove_forward))
@@ -54,6 +73,11 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
     o.pol_deg = float(pol[0])
     for d in range(3):
         o.pol_e[d] = float(pol[1 + d])
+    o.analyser_kind = int(analyser)
+    for d in range(3):
+        o.analyser_axis[d] = float(analyser_axis[d])
+    o.analyser_accept_rad = float(accept_rad)
+    o.analyser_darwin_hw_rad = float(darwin_hw_rad)
     return o
 
 
@@ -178,13 +202,16 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
     """Accumulate the kinematic field of a staged sample into ``out``.
 
     rot (F,3,3) and trans (F,3) are float32 host arrays (sample -> lab,
-    Angstrom); coords a (3, Ny*Nz) float64 device tensor in metres; out a
-    (F, Nz, Ny) complex64 or complex128 device tensor.  Runs the pixel
+    Angstrom), F at most the poses ``staged`` was staged for; coords a
+    (3, Ny*Nz) float64 device tensor in metres; out a (F, Nz, Ny) complex64
+    or complex128 device tensor.  Runs the pixel
     prologue first unless it already ran for these pixels, species and flags
     on this workspace.
     """
-    F = staged.n_frames
-    rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(F, 3, 3))
+    rot = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(-1, 3, 3))
+    F = rot.shape[0]
+    if F > staged.n_frames:
+        raise ValueError(f"{F} poses for a workspace staged for {staged.n_frames}")
     trans = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(F, 3))
     npix = staged.Ny * staged.Nz
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
@@ -268,3 +295,40 @@ def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream
         _ptr(staged.workspace), int(staged.workspace.numel()), _ptr(scratch),
         int(scratch.numel()), PATHS[path], _stream_handle(stream)), "mosaic_kin_adjoint")
     return grad_pos
+
+
+def scatter_workspace_bytes(n_atoms, Ny, Nz):
+    out = ctypes.c_size_t(0)
+    check(_lib.lib().mosaic_scatter_workspace(int(n_atoms), int(Ny), int(Nz), ctypes.byref(out)),
+          "mosaic_scatter_workspace")
+    return int(out.value)
+
+
+def scatter(pos_m, f0p, f0z, sanr, sani, amp_r, amp_i, cx, cy, cz, k_val, remove_forward,
+            Ny, Nz, apply_pol, pol_deg, pol_e, apply_decay, out_r, out_i, workspace=None,
+            stream=None):
+    """compute_scattering_cffi on the device (mosaic_scatter): the
+    reference's arguments as device tensors -- positions (N, 3) float64 lab
+    metres, WK rows (N, 11) float32, f0(0), f', f'', amplitude re/im (N,)
+    float32, pixel coordinates (Ny*Nz,) float64 metres -- accumulating into
+    the float32 device tensors out_r / out_i (Ny*Nz,).  Returns the workspace
+    (reusable)."""
+    n = int(pos_m.shape[0])
+    arrs = [(pos_m, torch.float64, (n, 3)), (f0p, torch.float32, (n, 11)),
+            (f0z, torch.float32, (n,)), (sanr, torch.float32, (n,)), (sani, torch.float32, (n,)),
+            (amp_r, torch.float32, (n,)), (amp_i, torch.float32, (n,)),
+            (cx, torch.float64, (Ny * Nz,)), (cy, torch.float64, (Ny * Nz,)),
+            (cz, torch.float64, (Ny * Nz,)), (out_r, torch.float32, (Ny * Nz,)),
+            (out_i, torch.float32, (Ny * Nz,))]
+    for t, dt, shp in arrs:
+        if not t.is_cuda or t.dtype != dt or tuple(t.shape) != shp or not t.is_contiguous():
+            raise ValueError(f"mosaic_scatter: expected a contiguous {dt} device tensor {shp}")
+    nbytes = scatter_workspace_bytes(n, Ny, Nz)
+    ws = ensure_workspace(workspace, nbytes, out_r.device)
+    check(_lib.lib().mosaic_scatter(
+        n, _ptr(pos_m), _ptr(f0p), _ptr(f0z), int(bool(remove_forward)), _ptr(sanr), _ptr(sani),
+        _ptr(amp_r), _ptr(amp_i), int(Ny), int(Nz), _ptr(cx), _ptr(cy), _ptr(cz), float(k_val),
+        int(bool(apply_pol)), float(pol_deg), float(pol_e[0]), float(pol_e[1]), float(pol_e[2]),
+        int(bool(apply_decay)), _ptr(out_r), _ptr(out_i), _ptr(ws), int(ws.numel()),
+        _stream_handle(stream)), "mosaic_scatter")
+    return ws

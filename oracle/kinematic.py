@@ -49,10 +49,32 @@ def q_cut(cx, cy, cz, Ny, Nz, k_val, dtype=np.float64):
     return 0.5 * np.sqrt(qx * qx + qy * qy)
 
 
+def analyser_weight(kind, d_unit, axis, accept_rad, darwin_hw_rad):
+    """Analyser acceptance of the reference's general kernel (Beam.py:2350-2378)
+    for unit rays ``d_unit`` (..., 3) from atom to pixel: 1 inside the cone of
+    half-angle ``accept_rad`` about ``axis`` (top-hat, kind 1), or the Darwin
+    roll-off 1 / (1 + (delta / hw)^2) (kind 2); delta the ray's angle to the
+    axis, here atan2(|c x d|, c . d), the intended rule (the reference compares
+    float32 cosines, which rounds cos(accept) to 1 below ~2.4e-4 rad)."""
+    c = np.asarray(axis, dtype=np.float64)
+    c = c / np.linalg.norm(c)
+    cr = np.cross(np.broadcast_to(c, d_unit.shape), d_unit)
+    delta = np.arctan2(np.sqrt((cr * cr).sum(axis=-1)), (d_unit * c).sum(axis=-1))
+    if kind == 1:
+        return (delta <= accept_rad).astype(d_unit.dtype)
+    if kind == 2 and darwin_hw_rad > 0:
+        r = delta / darwin_hw_rad
+        return 1.0 / (1.0 + r * r)
+    return np.ones(delta.shape, dtype=d_unit.dtype)
+
+
 def kin_sum_numpy(pos_m, f0p, f0z, sanr, sani, amp_r, amp_i, cx, cy, cz, k_val,
                   remove_forward, Ny, Nz, apply_pol, pol_deg, pol_e, apply_decay,
-                  dtype=np.float64, tile=256):
-    """Vectorised restatement of the reference kernel; returns complex sums."""
+                  dtype=np.float64, tile=256, analyser=None):
+    """Vectorised restatement of the reference kernel; returns complex sums.
+
+    ``analyser`` = (kind, axis, accept_rad, darwin_hw_rad) adds the general
+    kernel's analyser weight per pair (see ``analyser_weight``)."""
     dt = dtype
     two_pi = dt(2) * (dt(np.pi) if dt is np.float64 else np.arccos(dt(-1)))
     pi_r = two_pi / 2
@@ -115,6 +137,10 @@ def kin_sum_numpy(pos_m, f0p, f0z, sanr, sani, amp_r, amp_i, cx, cy, cz, k_val,
             er = (ex * dx + ey * dy + ez * dz) / rs
             P = np.clip(dt(0.5) * (1 - pdeg) * (1 + dotv * dotv) + pdeg * (1 - er * er), 0, 1)
             scale = scale * np.sqrt(P)
+        if analyser is not None and analyser[0]:
+            du = np.stack([dx, dy, dz], axis=-1) / rs[..., None]
+            scale = scale * analyser_weight(analyser[0], du.astype(np.float64), analyser[1],
+                                            analyser[2], analyser[3]).astype(dt)
         acc_r += np.where(ok, v_re * scale, 0).sum(axis=0)
         acc_i += np.where(ok, v_im * scale, 0).sum(axis=0)
     return acc_r + 1j * acc_i

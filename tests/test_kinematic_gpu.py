@@ -232,12 +232,21 @@ def test_restage_keeps_units(device):
     assert diff <= 1e-9
 
 
-def test_declines_near_field(device):
+def test_near_field_forced_forms_decline_auto_is_exact(device):
+    """At 1 mm the far-field and per-tile forms decline when forced (nothing
+    written); the automatic choice runs the exact per-pair kernel, which
+    matches the long-double oracle."""
     w = W.fcc_au_bragg111(n=4, npix=8)
     d = w["detector"]
+    d.create_detector((8, 8), (750.0, 750.0))
     d.position_detector_absolute(1e7, np.rad2deg(2 * w["theta_b"]), 0.0)  # 1 mm
-    with pytest.raises(MosaicDeclined):
-        gpu_field(w)
+    for path in ("tile", "rows", "mma", "poly"):
+        with pytest.raises(MosaicDeclined):
+            gpu_field(w, path=path)
+    A = gpu_field(w, path="auto")
+    assert w["beam"]._last_path == 50
+    e, _ = report("near field 1 mm auto", A, oracle_field(w, "f80"))
+    assert e <= TOL
 
 
 def test_empty_and_ragged(device):
