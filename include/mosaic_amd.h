@@ -29,6 +29,7 @@ extern "C" {
 #define MOSAIC_ERR_HIP 2           /* a HIP runtime call failed */
 #define MOSAIC_ERR_UNSUPPORTED 3   /* declined: outside the fast path's error bound */
 #define MOSAIC_ERR_WORKSPACE 4     /* workspace smaller than the query returned */
+#define MOSAIC_ERR_TOO_LARGE 5     /* beyond the fixed scratch of an allocation-free call */
 
 /* Opaque stream handle (a hipStream_t). */
 typedef void* mosaic_stream_t;
@@ -245,6 +246,27 @@ int mosaic_scatter(int atom_count, const double* positions, const float* f0_para
                    int apply_pol, double pol_deg, double pol_ex, double pol_ey,
                    double pol_ez, int apply_spherical_decay, float* out_r, float* out_i,
                    void* workspace, size_t workspace_bytes, mosaic_stream_t stream);
+
+/*
+ * mosaic_scatter without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation).  No counterpart in the reference.  The slice
+ * partials live in a fixed per-device scratch of 64 MiB inside the library,
+ * which each GPU it runs on holds until the process ends; the call allocates
+ * nothing.  Detectors of more than 4,194,304 pixels (2048 x 2048) return
+ * MOSAIC_ERR_TOO_LARGE; fewer atom slices are used where the scratch holds
+ * fewer, so the result equals mosaic_scatter's to FP64 summation order.
+ * Calls that overlap on one GPU (different streams or threads) share the
+ * scratch and race.
+ */
+int mosaic_scatter_noalloc(int atom_count, const double* positions, const float* f0_params,
+                           const float* f0_zero, int remove_forward, const float* s_anom_real,
+                           const float* s_anom_imag, const float* initial_amp_r,
+                           const float* initial_amp_i, int Ny, int Nz, const double* coords_x,
+                           const double* coords_y, const double* coords_z, double k_val,
+                           int apply_pol, double pol_deg, double pol_ex, double pol_ey,
+                           double pol_ez, int apply_spherical_decay, float* out_r,
+                           float* out_i, mosaic_stream_t stream);
 
 /*
  * Incident amplitude per atom by bilinear sampling of the beam's entrance

@@ -14,12 +14,14 @@ MOSAIC_ERR_ARGS = 1
 MOSAIC_ERR_HIP = 2
 MOSAIC_ERR_UNSUPPORTED = 3
 MOSAIC_ERR_WORKSPACE = 4
+MOSAIC_ERR_TOO_LARGE = 5
 
 _ERRORS = {
     MOSAIC_ERR_ARGS: "invalid argument",
     MOSAIC_ERR_HIP: "HIP runtime error",
     MOSAIC_ERR_UNSUPPORTED: "declined: geometry outside the fast path's error bound",
     MOSAIC_ERR_WORKSPACE: "workspace too small",
+    MOSAIC_ERR_TOO_LARGE: "beyond the allocation-free call's fixed scratch",
 }
 
 # Every symbol include/mosaic_amd.h declares.
@@ -33,6 +35,7 @@ EXPORTS = (
     "mosaic_kin_adjoint_scratch",
     "mosaic_scatter_workspace",
     "mosaic_scatter",
+    "mosaic_scatter_noalloc",
     "mosaic_ein_bilinear",
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
@@ -95,6 +98,8 @@ _SIGS = {
     "mosaic_scatter_workspace": [_i, _i, _i, ctypes.POINTER(_sz)],
     "mosaic_scatter": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _d, _i,
                        _d, _d, _d, _d, _i, _vp, _vp, _vp, _sz, _vp],
+    "mosaic_scatter_noalloc": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp,
+                               _d, _i, _d, _d, _d, _d, _i, _vp, _vp, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,

@@ -24,6 +24,7 @@
 // slice order and added, as float32, into out_r / out_i, as the reference
 // adds its FP64 accumulators once per call.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "pair_physics.h"
@@ -285,6 +286,34 @@ int sc_launch(int atom_count, const double* positions, const float* f0_params,
     return MOSAIC_OK;
 }
 
+// Scratch of the allocation-free entry point: its slice partials live in
+// this file-scope device array instead of a caller workspace.  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the name is only a stand-in; the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's scratch address, looked up on the device's first call.
+int scratch_address(void** out) {
+    int dev = -1;
+    MOSAIC_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return MOSAIC_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MOSAIC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return MOSAIC_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MOSAIC_OK;
+}
+
 }  // namespace
 }  // namespace mosaic
 
@@ -320,5 +349,40 @@ extern "C" int mosaic_scatter(int atom_count, const double* positions, const flo
                      s_anom_imag, initial_amp_r, initial_amp_i, Ny, Nz, coords_x, coords_y,
                      coords_z, k_val, apply_pol, pol_deg, pol_ex, pol_ey, pol_ez,
                      apply_spherical_decay, out_r, out_i, static_cast<double2*>(workspace), P,
+                     static_cast<hipStream_t>(stream));
+}
+
+// mosaic_scatter without a workspace: the slice partials (16 B per pixel per
+// atom slice) go into the library's per-device scratch, so the call
+// allocates nothing.  Every partial the reduction reads is written first by
+// the same call, so nothing needs clearing; calls that overlap on one GPU
+// share the scratch and race.
+extern "C" int mosaic_scatter_noalloc(int atom_count, const double* positions,
+                                      const float* f0_params, const float* f0_zero,
+                                      int remove_forward, const float* s_anom_real,
+                                      const float* s_anom_imag, const float* initial_amp_r,
+                                      const float* initial_amp_i, int Ny, int Nz,
+                                      const double* coords_x, const double* coords_y,
+                                      const double* coords_z, double k_val, int apply_pol,
+                                      double pol_deg, double pol_ex, double pol_ey,
+                                      double pol_ez, int apply_spherical_decay, float* out_r,
+                                      float* out_i, mosaic_stream_t stream) {
+    if (!sc_args_ok(atom_count, Ny, Nz, k_val)) return MOSAIC_ERR_ARGS;
+    if (!coords_x || !coords_y || !coords_z || !out_r || !out_i) return MOSAIC_ERR_ARGS;
+    if (atom_count > 0 && (!positions || !f0_params || !f0_zero || !s_anom_real ||
+                           !s_anom_imag || !initial_amp_r || !initial_amp_i))
+        return MOSAIC_ERR_ARGS;
+    const size_t per_slice = static_cast<size_t>(Ny) * Nz * sizeof(double2);
+    if (per_slice > NOALLOC_SCRATCH_BYTES) return MOSAIC_ERR_TOO_LARGE;
+    if (atom_count == 0) return MOSAIC_OK;
+    void* base = nullptr;
+    const int rc = scratch_address(&base);
+    if (rc != MOSAIC_OK) return rc;
+    const int max_slices = static_cast<int>(NOALLOC_SCRATCH_BYTES / per_slice);
+    const ScPlan P = sc_plan(atom_count, Ny * Nz, max_slices);
+    return sc_launch(atom_count, positions, f0_params, f0_zero, remove_forward, s_anom_real,
+                     s_anom_imag, initial_amp_r, initial_amp_i, Ny, Nz, coords_x, coords_y,
+                     coords_z, k_val, apply_pol, pol_deg, pol_ex, pol_ey, pol_ez,
+                     apply_spherical_decay, out_r, out_i, static_cast<double2*>(base), P,
                      static_cast<hipStream_t>(stream));
 }
