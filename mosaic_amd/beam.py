@@ -270,7 +270,7 @@ class beam:
                                       self._beam_dv, self._beam_uc, self._beam_vc)
 
             amp0 = ein(0)
-            same = all(torch.equal(amp0, ein(f)) for f in range(1, F))
+            same = all(K.same_amplitudes(amp0, ein(f)) for f in range(1, F))
             staged = K.stage(d_pos, d_cod, amp0, counts, k_val, F, Ny, Nz,
                              workspace=self._workspace)
             if (staged_prev is not None and staged.workspace is staged_prev.workspace

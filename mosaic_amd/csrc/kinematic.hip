@@ -3203,11 +3203,17 @@ void k_adjoint_mma(const AdjMmaArgs A) {
 // weighted residual times the unit's origin phase, formed as k_adjoint_poly
 // forms them per tile; the unit's tiles then share them.
 constexpr int ADJ_GROUP = 64;     // (frame, patch) pairs per k_adjoint_moments workgroup
+// Pixel moments M_mn (m + n <= D + 2) per unit head and (frame, patch) pair
+// at expansion degree D; the scratch is sized for the largest degree the
+// host chooses (poly_degree returns 5 or 6).
+constexpr int adj_moments(int D) { return (D + 3) * (D + 4) / 2; }
+constexpr int ADJ_MAX_DEGREE = 6;
 
 template <int D>
 __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMmaArgs A) {
+    static_assert(D <= ADJ_MAX_DEGREE, "the moment scratch is sized for ADJ_MAX_DEGREE");
     constexpr int DM = D + 2;
-    constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
+    constexpr int NM = adj_moments(D);              // pixel moments
     constexpr int ROUNDS = 3;                       // reduced in three rounds: a
     constexpr int HALF = (NM + ROUNDS - 1) / ROUNDS; // 32 KB buffer fits 5 workgroups per CU
     constexpr int NVP = (2 * HALF) | 1;
@@ -3347,8 +3353,9 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_moments(const AdjMm
 // per lane in FP32 over up to 16 (frame, patch) pairs, then in FP64 in LDS.
 template <int D, bool SECOND, bool UNITS>
 __global__ __launch_bounds__(64 * POLY_WAVES) void k_adjoint_poly(const AdjMmaArgs A) {
+    static_assert(D <= ADJ_MAX_DEGREE, "the moment scratch is sized for ADJ_MAX_DEGREE");
     constexpr int DM = D + 2;
-    constexpr int NM = (DM + 1) * (DM + 2) / 2;     // pixel moments
+    constexpr int NM = adj_moments(D);              // pixel moments
     constexpr int HALF = (NM + 1) / 2;              // reduced in two rounds
     constexpr int NVP = (2 * HALF) | 1;
     constexpr int NC = (D + 1) * (D + 2) / 2;
@@ -4330,10 +4337,10 @@ extern "C" int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const f
 }
 
 // Pixel moments of the moment adjoint's units: per tile (unit heads use
-// theirs) and (frame, patch) pair, 2 (D + 3)(D + 4) / 2 floats at D = 6.
+// theirs) and (frame, patch) pair, 2 adj_moments(D) floats at the largest D.
 static size_t adjoint_scratch_bytes(int n_tiles, int n_frames, int Ny, int Nz) {
     const size_t patches = static_cast<size_t>((Ny + MMA_P - 1) / MMA_P) * ((Nz + MMA_P - 1) / MMA_P);
-    constexpr size_t NMF = 2 * (6 + 3) * (6 + 4) / 2;
+    constexpr size_t NMF = 2 * adj_moments(ADJ_MAX_DEGREE);
     return static_cast<size_t>(std::max(1, n_tiles)) * n_frames * patches * NMF * sizeof(float);
 }
 

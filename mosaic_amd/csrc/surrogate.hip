@@ -263,6 +263,14 @@ extern "C" int mosaic_sg_gemm(int transA, int transB, int M, int N, int K, const
     if (M < 0 || N < 0 || K < 0 || batch < 1 || !A || !B || !C) return MOSAIC_ERR_ARGS;
     if (M == 0 || N == 0) return MOSAIC_OK;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (K == 0) {   // an empty product: C = beta C
+        const long long mn = static_cast<long long>(M) * N;
+        if (mn == 0) return MOSAIC_OK;
+        hipLaunchKernelGGL(k_scale, dim3(static_cast<unsigned>((mn + 255) / 256), batch), dim3(256),
+                           0, st, C, M, N, ldc, strideC, beta);
+        MOSAIC_HIP_CHECK(hipGetLastError());
+        return MOSAIC_OK;
+    }
     // Few output tiles over a long K (weight gradients: K = batch x pixels):
     // split K across workgroups so the chip fills, adding partials atomically.
     const long long tiles = static_cast<long long>((N + GBN - 1) / GBN) * ((M + GBM - 1) / GBM);

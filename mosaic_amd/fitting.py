@@ -8,7 +8,8 @@ is mosaic_kin_sum on the displaced positions, the backward the hand-written
 adjoint kernel (mosaic_kin_adjoint), which recomputes phases instead of
 storing them.  Form factors, polarization and the incident amplitudes are
 held fixed with respect to u (their dependence on the position is of order
-|u| / distance).
+|u| / distance).  The atoms are sorted into tiles once; each forward
+re-stages the displaced atoms into the same tiles (mosaic_kin_restage).
 """
 import numpy as np
 import torch
@@ -20,17 +21,23 @@ from . import tables
 class _KinField(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u, model):
-        staged = model._forward(u)
+        model._forward(u)
         ctx.model = model
-        ctx.staged = staged
+        ctx.version = model._version
+        ctx.pos = model._pos
         ctx.n = u.shape[0]
         return model._out.clone()
 
     @staticmethod
     def backward(ctx, grad_A):
         m = ctx.model
+        if m._version != ctx.version:
+            # a later forward re-staged the workspace: restore this one's tiles
+            K.restage(m._staged, ctx.pos, m.amp)
+            m._version = ctx.version
+            m._pos = ctx.pos
         grad = torch.zeros((ctx.n, 3), dtype=torch.float32, device=grad_A.device)
-        K.kin_adjoint(ctx.staged, m.k, m.rot, m.trans, grad_A, grad, path=m.adjoint_path)
+        K.kin_adjoint(m._staged, m.k, m.rot, m.trans, grad_A, grad, path=m.adjoint_path)
         return grad, None
 
 
@@ -57,16 +64,35 @@ class KinematicModel:
         self.coords = torch.as_tensor(np.ascontiguousarray(detector.pixel_coordinates / 1e10),
                                       device=self.device)
         E0 = torch.as_tensor(beam._beam_E0_map, device=self.device)
-        self.amp = K.ein_bilinear(self.pos0, self.rot[0], self.trans[0], E0, beam._beam_du,
-                                  beam._beam_dv, beam._beam_uc, beam._beam_vc)
+        amps = [K.ein_bilinear(self.pos0, self.rot[f], self.trans[f], E0, beam._beam_du,
+                               beam._beam_dv, beam._beam_uc, beam._beam_vc)
+                for f in range(self.F)]
+        if not all(K.same_amplitudes(amps[0], a) for a in amps[1:]):
+            raise NotImplementedError(
+                "KinematicModel holds one incident amplitude per atom: the beam profile must "
+                "give the same amplitudes at every pose (e.g. a uniform beam covering the "
+                "sample)")
+        self.amp = amps[0]
         self._out = torch.zeros((self.F, self.Nz, self.Ny), dtype=torch.complex128,
                                 device=self.device)
+        # one staged sample for the model's lifetime: the first forward sorts
+        # the atoms into tiles, later forwards re-stage the moved atoms into the
+        # same tiles (no sort, no allocation); the adjoint's scratch stays with it
+        self._staged = None
+        self._pos = None
+        self._version = 0
 
     def _forward(self, u):
-        pos = self.pos0 + u.detach().to(torch.float32) if u is not None else self.pos0
-        # a fresh workspace per forward: the backward of this call reads it
-        staged = K.stage(pos.contiguous(), self.codes, self.amp, self.counts, self.k, self.F,
-                         self.Ny, self.Nz)
+        pos = (self.pos0 + u.detach().to(torch.float32) if u is not None
+               else self.pos0).contiguous()
+        if self._staged is None:
+            self._staged = K.stage(pos, self.codes, self.amp, self.counts, self.k, self.F,
+                                   self.Ny, self.Nz)
+        else:
+            K.restage(self._staged, pos, self.amp)
+        self._version += 1
+        self._pos = pos
+        staged = self._staged
         self._out.zero_()
         K.kin_sum(staged, self.spc, self.opts, self.rot, self.trans, self.coords, self._out)
         self.path_used = staged.path_used

@@ -253,6 +253,17 @@ def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
     return out
 
 
+def same_amplitudes(a, b, rtol=1e-6):
+    """Whether two poses' incident amplitudes agree to ``rtol`` of the largest.
+
+    Inside a uniform beam the bilinear sampler returns 1 up to one float32
+    rounding of its weights, which differs between poses; a shared amplitude
+    then changes no atom's term by more than ~1e-7."""
+    if a.numel() == 0:
+        return True
+    return bool(torch.max(torch.abs(a - b)) <= rtol * torch.max(torch.abs(a)))
+
+
 def kin_adjoint(staged, k, rot, trans, grad_field, grad_pos, path="auto", stream=None,
                 share_units=True):
     """Add dL/du (per Angstrom, original atom order) into grad_pos (N, 3) float32.

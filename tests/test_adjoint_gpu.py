@@ -101,7 +101,7 @@ def test_adjoint_vs_fp64_reference_and_fd(device, path):
     print(f"[parity] adjoint ({path}) vs long-double reference: rel={e_ref:.3e}; "
           f"reference vs FD: rel={e_fd:.3e}")
     assert e_fd < 1e-4          # the analytic formula is the derivative
-    assert e_ref < 1e-3         # the GPU adjoint matches it
+    assert e_ref < 1e-5         # the GPU adjoint matches it (measured ~2e-6)
     # untouched atoms beyond the sample do not exist; every gradient is finite
     assert np.all(np.isfinite(grad))
 
@@ -152,4 +152,6 @@ def test_adjoint_units_match_tiles(device):
     a, b = g_units.cpu().numpy(), g_tiles.cpu().numpy()
     e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     print(f"[adjoint] units vs tiles: rel={e:.3e} (forward path {staged.path_used})", flush=True)
-    assert e <= 1e-5
+    # units formed: sharing a unit's moments changes the summation (e > 0;
+    # without units both calls run the same tile-by-tile kernel and e == 0)
+    assert 0.0 < e <= 1e-5
