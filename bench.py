@@ -6,15 +6,19 @@ Workload (default): the precision-form sample -- SrTiO3-type perovskite,
 100^3 cells = 5,000,000 atoms (3 species, 64 spatial chunks) with a screw
 dislocation, 10 keV, 512 x 512 pixels at 1 m on the (200) reflection:
 1.31e12 atom-pixel pairs per step.  The sample is resident in HBM before
-the timed region; one step stages it (Morton sort into tiles), runs the
+the timed region; one step stages it (k-d cut into tiles), runs the
 pixel prologue and the sum.
 
-Prints one JSON line.  ``roofline`` prices the dominant kernel (k_sum) at the
-algorithmic 20 FP32 flop per pair against the 157.3 TFLOP/s FP32 vector peak;
-its duration comes from HIP events around the mosaic_kin_sum call on the
-stream it runs on.  ``cpu_baseline`` times the reference's own C kernel
+Prints one JSON line.  ``roofline`` prices the dominant kernel (the sum) by
+the FP32 flops it executes per launch (the profile's counters of the same
+workload) over its launch time, against the 157.3 TFLOP/s FP32 peak; the
+time comes from HIP events around the mosaic_kin_sum call on the stream it
+runs on.  ``cpu_baseline`` times the reference's own C kernel
 (compute_scattering_cffi, built into oracle/_ref) on a bounded pixel sample
-of the same workload, on the host's cores.
+of the same workload, on the host's cores.  ``end_to_end`` times the drop-in
+call on the host-resident chunks (H2D included); ``scan``, ``adjoint`` and
+``propagation`` cover the other workloads, each with its error against the
+oracle.
 """
 import argparse
 import json
@@ -28,23 +32,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 20.0          # algorithmic FP32 work per atom-pixel pair (20 flop + 2 trans.)
-PEAK_FP32 = 157.3e12          # MI355X FP32 (vector and dense matrix)
+PEAK_FP32 = 157.3e12          # MI355X FP32 vector (packed FMA), the published peak
 PEAK_FP16_MFMA = 2.5e15       # MI355X dense FP16 matrix
+# VALU issue peak in lane-operations per second (an FMA counts once): 256 CUs x
+# 4 SIMDs x 16 lanes x 2.4 GHz, the FP32 peak halved
+PEAK_VALU_LANE_OPS = PEAK_FP32 / 2
+# v_sin_f32 / v_cos_f32 measured on the box (a probe kernel, 8 chains per lane):
+# 1.81e13 lane-operations/s, a quarter of the FMA issue rate
+PEAK_TRANS_LANE_OPS = 1.81e13
 # The matrix-core sum executes 3 split products x 4 real multiply-adds per pair.
 MFMA_FLOP_PER_PAIR = 24.0
-# HBM bytes per launch of the matrix-core sum on the 5M-atom, 512^2 workload,
-# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (the copy-kernel calibration
-# reads FETCH_SIZE x 2 and WRITE_SIZE x 1; profiles/k_sum_mma_perovskite5M_512px.md).
-MMA_TRAFFIC_BYTES = (2 * 1831133.0 + 2542443.8) * 1024
-# k_sum_poly<6> on the same workload: FETCH_SIZE 1.006e6 KiB (x2, the copy
-# calibration), WRITE_SIZE 0.838e6 KiB; 4.72e9 VALU wave-instructions per launch
-# for 1.31e12 pairs (SQ_INSTS_VALU), i.e. 0.23 lane-operations per pair
-# (profiles/k_sum_poly_perovskite5M_512px.md)
-POLY_TRAFFIC_BYTES = (2 * 1006000.0 + 838200.0) * 1024
-POLY_VALU_LANE_OPS_PER_PAIR = 4.7196e9 * 64 / (5.0e6 * 512 * 512)
-# VALU issue peak in lane-operations per second (an FMA counts once): the FP32
-# peak halved
-PEAK_VALU_LANE_OPS = PEAK_FP32 / 2
+
+# Counters of one launch of the sum kernels on the 5M-atom, 512^2 workload
+# (rocprofv3 --pmc, same bench command; the profile files hold the commands):
+# FETCH_SIZE / WRITE_SIZE in KiB (scaled by the copy calibration, FETCH x 2,
+# WRITE x 1) and the FP32 VALU instruction counts by kind (wave-instructions).
+COUNTERS = {
+    "k_sum_poly": dict(
+        source="profiles/k_sum_poly_perovskite5M_512px.md",
+        fetch_kib=1.01095e6, write_kib=842670.0, valu=4.71956e9,
+        add_f32=2.06103e8, mul_f32=8.27142e8, fma_f32=2.82412e9, trans_f32=9.04499e7),
+    "k_sum_mma": dict(
+        source="profiles/k_sum_mma_perovskite5M_512px.md",
+        fetch_kib=1831133.0, write_kib=2542443.8, valu=None,
+        add_f32=None, mul_f32=None, fma_f32=None, trans_f32=None),
+}
+WORKLOAD_OF_COUNTERS = "perovskite_screw_5M_512px"
+# The pruned propagation's three passes on the bench's 2048^2-on-8192^2 step:
+# FETCH_SIZE, WRITE_SIZE (KiB per launch, same scaling) from
+# profiles/propagation_crl2m_2048_on_8192.md.
+PROP_COUNTERS = {"k_asp_rows_fwd": (18263.6, 593124.0), "k_asp_cols": (86624.1, 178921.0),
+                 "k_asp_rows_inv": (77330.7, 32768.0)}
 
 
 def parse():
@@ -56,10 +74,12 @@ def parse():
     ap.add_argument("--cpu-pixels", type=int, default=768)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-pixels", type=int, default=48)
-    ap.add_argument("--path", default="auto", choices=["auto", "tile", "rows", "mma", "poly"])
+    ap.add_argument("--path", default="auto",
+                    choices=["auto", "tile", "rows", "mma", "poly", "exact"])
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--surrogate", action="store_true",
                     help="run the surrogate workload (data generation, training, gated scan)")
     ap.add_argument("--sg-images", type=int, default=20000)
@@ -100,6 +120,31 @@ def gather(w):
     return np.concatenate(pos), np.concatenate(codes), labels
 
 
+def host_cpu():
+    """Threads the CPU baseline uses and the host's identity.
+
+    The reference's CPU driver uses every logical core (Beam.py:5873-5955,
+    hardware.py:790-803); on a shared host the cores this process may run on
+    (its affinity, and OMP_NUM_THREADS where the host sets it) are that set."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity_cpus": affinity,
+                     "omp_num_threads": omp, "model": model}
+
+
 def cpu_baseline(w, n_pix, seed=0):
     from mosaic_amd import workloads as W
     from oracle import kinematic as O
@@ -108,7 +153,7 @@ def cpu_baseline(w, n_pix, seed=0):
     a = W.oracle_inputs(w)
     rng = np.random.default_rng(seed)
     sel = rng.choice(a["Ny"] * a["Nz"], size=n_pix, replace=False)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, host = host_cpu()
     kw = dict(pos_m=a["pos_m"], f0p=a["f0p"], f0z=a["f0z"], sanr=a["sanr"], sani=a["sani"],
               amp_r=a["amp_r"], amp_i=a["amp_i"], cx=a["cx"][sel], cy=a["cy"][sel],
               cz=a["cz"][sel], k_val=a["k_val"], remove_forward=False, Ny=n_pix, Nz=1,
@@ -122,8 +167,8 @@ def cpu_baseline(w, n_pix, seed=0):
     dt = time.perf_counter() - t
     pairs = a["pos_m"].shape[0] * n_pix
     return {"value": pairs / dt, "unit": "atom*pixel pairs/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"compute_scattering_cffi (oracle/_ref, gcc -O3) on all "
+            "kind": "reference", "host": host,
+            "sample": f"compute_scattering_cffi (oracle/_ref, gcc -O3 -march=x86-64-v3) on all "
                       f"{a['pos_m'].shape[0]} atoms x {n_pix} random pixels, "
                       f"{threads} threads over >=1000-atom slices, {dt:.1f} s"}
 
@@ -138,7 +183,7 @@ def accuracy(w, field, n_pix, seed=1):
     R = O.kin_sum_c(a["pos_m"], a["f0p"], a["f0z"], a["sanr"], a["sani"], a["amp_r"],
                     a["amp_i"], a["cx"][sel], a["cy"][sel], a["cz"][sel], a["k_val"], False,
                     n_pix, 1, True, a["pol_deg"], a["pol_e"], False, precision="f64",
-                    threads=min(16, os.cpu_count() or 1))
+                    threads=host_cpu()[0])
     A = field.ravel()[sel]
     return float(np.linalg.norm(A - R) / np.linalg.norm(R))
 
@@ -197,6 +242,9 @@ def bench_propagation(steps, warmup, wavelength):
             "substeps": [[float(z), a, b] for z, a, b in plan], "ms": ms,
             "algorithmic_bytes": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9,
             "peak_GBps": HBM_PEAK / 1e9, "frac": alg / (ms * 1e-3) / HBM_PEAK,
+            "traffic_per_step_bytes": sum((2.0 * f + w_) * 1024.0
+                                          for f, w_ in PROP_COUNTERS.values()),
+            "traffic_source": "profiles/propagation_crl2m_2048_on_8192.md",
             "padded_fft2d_ms": ms2d,
             "padded_fft2d_frac": alg2d / (ms2d * 1e-3) / HBM_PEAK,
             "rel_diff_vs_padded": float(torch.linalg.norm(out - out2d) / torch.linalg.norm(out2d)),
@@ -252,38 +300,94 @@ def bench_scan(steps, warmup):
         ts.append(ev0.elapsed_time(ev1))
     ms = float(np.mean(ts))
     pairs = float(n) * Ny * Nz * F
+    # error: 3 frames x 32 random pixels against the FP64 oracle over all atoms
+    from oracle import kinematic as O
+    field = out.cpu().numpy().reshape(F, -1)
+    rng = np.random.default_rng(21)
+    num = den = 0.0
+    for f in (0, F // 2, F - 1):
+        a = W.oracle_inputs(w, frame_rot=rot[f], frame_trans=tr[f])
+        sel = rng.choice(Ny * Nz, size=32, replace=False)
+        R = O.kin_sum_c(a["pos_m"], a["f0p"], a["f0z"], a["sanr"], a["sani"], a["amp_r"],
+                        a["amp_i"], a["cx"][sel], a["cy"][sel], a["cz"][sel], a["k_val"], False,
+                        32, 1, True, a["pol_deg"], a["pol_e"], False, precision="f64",
+                        threads=host_cpu()[0])
+        num += float(np.sum(np.abs(field[f][sel] - R) ** 2))
+        den += float(np.sum(np.abs(R) ** 2))
     return {"workload": w["name"], "atoms": n, "frames": F, "pixels": [Ny, Nz],
             "ms_per_scan": ms, "frames_per_s": F / (ms * 1e-3),
-            "pairs_per_s": pairs / (ms * 1e-3), "kernel_path": sg.path_used}
+            "pairs_per_s": pairs / (ms * 1e-3), "kernel_path": sg.path_used,
+            "rel_err_vs_fp64_oracle": float(np.sqrt(num / den)),
+            "rel_err_sample": "frames 0, 31, 63 x 32 random pixels, all atoms"}
+
+
+def bench_end_to_end(w, steps):
+    """The drop-in as a MOSAIC caller uses it: beam.atomic_direct_interaction
+    on the host-resident chunked sample (64 chunks of the 5M-atom workload):
+    chunks loaded and copied from pinned host memory, incident amplitudes,
+    k-d staging, pixel prologue, sum, the field copied back and written to
+    the detector.  Host clock around each call, device synchronized."""
+    import torch
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    kw = dict(apply_polarization=True)
+    b.atomic_direct_interaction(s, d, st, sc_kwargs=kw)      # warm-up (pinned buffers)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        b.atomic_direct_interaction(s, d, st, sc_kwargs=kw)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    n = s.atom_count
+    Ny, Nz = d.shape
+    return {"call": "beam.atomic_direct_interaction(sample, detector, stage, "
+                    "sc_kwargs={'apply_polarization': True})",
+            "chunks": s.chunk_total, "ms": ms,
+            "pairs_per_s": float(n) * Ny * Nz / (ms * 1e-3),
+            "h2d_bytes": 13 * n, "d2h_bytes": 8 * Ny * Nz,
+            "kernel_path": b._last_path}
 
 
 def roofline(path_used, pairs, ksum_ms, workload):
-    """Roofline object of the dominant (sum) kernel: algorithmic flops at 20 per
-    pair against the peak of the unit the kernel runs on."""
+    """Roofline object of the dominant (sum) kernel.
+
+    ``achieved`` is the work the kernel's own algorithm does per launch over
+    its live launch time: for the VALU kernels the FP32 flops it executes
+    (add, mul, 2 per fma, one per transcendental; 64 lanes per counted
+    wave-instruction, from the profile's SQ_INSTS_VALU_*_F32 counters of the
+    same workload -- the counts are fixed by the workload), against the
+    157.3 TFLOP/s FP32 peak; for the matrix-core kernel the FP16 matrix
+    flops it issues, against 2.5 PFLOP/s.  ``pair_equivalent_tflops`` is a
+    throughput figure, not a fraction of a roofline: 20 flop per pair (the
+    per-pair algorithm) times the pairs the kernel's result stands for."""
     t = ksum_ms * 1e-3
-    achieved = pairs * FLOP_PER_PAIR / t
     name = kernel_name(path_used)
+    c = COUNTERS.get(name) if workload == WORKLOAD_OF_COUNTERS else None
     r = {"kernel": name, "kernel_path": path_used, "kernel_ms": ksum_ms,
-         "kernel_pairs_per_s": pairs / t, "unit": "TFLOP/s", "achieved": achieved / 1e12}
+         "kernel_pairs_per_s": pairs / t, "unit": "TFLOP/s",
+         "pair_equivalent_tflops": pairs * FLOP_PER_PAIR / t / 1e12,
+         "traffic": None}
+    if c is not None:
+        r["traffic"] = (2.0 * c["fetch_kib"] + c["write_kib"]) * 1024.0
+        r["counters_source"] = c["source"]
     if name == "k_sum_mma":
-        r.update({"bound": "mfma", "peak": PEAK_FP16_MFMA / 1e12,
-                  "frac": achieved / PEAK_FP16_MFMA,
-                  "mfma_executed_tflops": pairs * MFMA_FLOP_PER_PAIR / t / 1e12,
-                  "fp32_equivalent_frac": achieved / PEAK_FP32,
-                  "traffic": MMA_TRAFFIC_BYTES if workload == "perovskite_screw_5M_512px"
-                  else None})
-    elif name == "k_sum_poly":
-        # achieved stays the 20-flop-per-pair algorithmic figure; the moment
-        # expansion executes far fewer operations per pair, so frac exceeds 1
-        r.update({"bound": "valu", "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32,
-                  "executed_valu_lane_ops_per_pair": POLY_VALU_LANE_OPS_PER_PAIR,
-                  "executed_valu_frac": pairs * POLY_VALU_LANE_OPS_PER_PAIR / t
-                  / PEAK_VALU_LANE_OPS,
-                  "traffic": POLY_TRAFFIC_BYTES if workload == "perovskite_screw_5M_512px"
-                  else None})
+        flop = pairs * MFMA_FLOP_PER_PAIR
+        r.update({"bound": "mfma", "achieved": flop / t / 1e12, "peak": PEAK_FP16_MFMA / 1e12,
+                  "frac": flop / t / PEAK_FP16_MFMA})
+        return r
+    r.update({"bound": "valu", "peak": PEAK_FP32 / 1e12})
+    if c is not None and c.get("fma_f32") is not None:
+        flop = 64.0 * (c["add_f32"] + c["mul_f32"] + 2.0 * c["fma_f32"] + c["trans_f32"])
+        r.update({"achieved": flop / t / 1e12, "frac": flop / t / PEAK_FP32,
+                  "fp32_flop_per_launch": flop, "fp32_flop_per_pair": flop / pairs,
+                  "trans_lane_ops_per_s": 64.0 * c["trans_f32"] / t,
+                  "trans_frac": 64.0 * c["trans_f32"] / t / PEAK_TRANS_LANE_OPS,
+                  "valu_lane_ops_per_pair": 64.0 * c["valu"] / pairs,
+                  "valu_issue_frac": 64.0 * c["valu"] / t / PEAK_VALU_LANE_OPS})
     else:
-        r.update({"bound": "valu", "peak": PEAK_FP32 / 1e12, "frac": achieved / PEAK_FP32,
-                  "traffic": None})
+        r.update({"achieved": None, "frac": None,
+                  "note": "no counters of this kernel on this workload"})
     return r
 
 
@@ -291,7 +395,8 @@ def kernel_name(path_used):
     """Sum kernel behind a mosaic_kin_sum path code."""
     base = path_used % 100
     return {32: "k_sum_mma", 24: "k_sum_rec", 28: "k_sum_rec", 45: "k_sum_poly",
-            46: "k_sum_poly"}.get(base, "k_sum")
+            46: "k_sum_poly", 50: "k_sum_exact", 51: "k_sum_exact",
+            52: "k_sum_exact"}.get(base, "k_sum")
 
 
 def bench_adjoint(steps):
@@ -331,10 +436,36 @@ def bench_adjoint(steps):
             tf.append(ev[0].elapsed_time(ev[1]))
             tb.append(ev[1].elapsed_time(ev[2]))
     pairs = float(pos.shape[0]) * 16 * 256 * 256
+    # error: dL/du of 20 random atoms against the oracle's reference gradient
+    # (exact per-pair phase gradient, long-double atom terms), given the
+    # GPU field's upstream gradient
+    from mosaic_amd import tables
+    from oracle import adjoint as AO
+    grad = u.grad.cpu().numpy()
+    A_np = A.detach().cpu().numpy().reshape(16, -1)
+    It_np = It.cpu().numpy().reshape(16, -1)
+    g = 4 * (np.abs(A_np) ** 2 - It_np) * A_np / scale ** 2
+    c = tables.species_constants(["Au"], b.energy)
+    k = 2 * np.pi / b.wavelength
+    pc = d.pixel_coordinates / 1e10
+    rot_a, tr_a = np.stack(rots), np.stack(trs)
+    p_all = (pos + w["u"]).astype(np.float32)
+    atoms = np.random.default_rng(31).choice(pos.shape[0], size=20, replace=False)
+    ref = []
+    for j in atoms:
+        cj = AO.atom_terms(p_all[j], c["wk"][0], float(c["f0z"][0]), float(c["fp"][0]),
+                           float(c["fpp"][0]), np.complex64(1.0), rot_a, tr_a, pc, k,
+                           b._polarization_state(), host_cpu()[0])
+        ref.append(AO.reference_gradient(cj, AO.phase_gradient(p_all[j], rot_a, tr_a, pc, k),
+                                         g))
+    ref = np.array(ref)
+    e = float(np.linalg.norm(grad[atoms] - ref) / np.linalg.norm(ref))
     return {"workload": w["name"], "atoms": int(pos.shape[0]), "poses": 16,
             "forward_ms": float(np.mean(tf)), "backward_ms": float(np.mean(tb)),
             "backward_pairs_per_s": pairs / (np.mean(tb) * 1e-3),
-            "grad_norm": float(torch.linalg.norm(u.grad).item())}
+            "grad_norm": float(torch.linalg.norm(u.grad).item()),
+            "rel_err_vs_reference_gradient": e,
+            "rel_err_sample": "20 random atoms, long-double reference gradient"}
 
 
 def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
@@ -521,6 +652,8 @@ def main():
                    "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
         "roofline": roofline(path_used[0], pairs, ksum_ms, args.workload),
     }
+    if not args.no_end_to_end:
+        res["end_to_end"] = bench_end_to_end(w, args.steps)
     if not args.no_propagation:
         res["propagation"] = bench_propagation(args.steps, args.warmup, b.wavelength)
     if not args.no_scan:
