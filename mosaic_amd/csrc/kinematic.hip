@@ -6,8 +6,8 @@
 //           * [R0/r] * [sqrt(P)]
 // accumulated over atoms.  The MI355X formulation:
 //
-//  * Staging (mosaic_kin_stage): atoms sorted by (species, Hilbert index) and
-//    cut into 256-atom single-species tiles.  A tile stores its origin O_t
+//  * Staging (mosaic_kin_stage): atoms grouped by species and cut by a k-d
+//    bisection into 256-atom single-species tiles.  A tile stores its origin O_t
 //    in FP64 and each atom as an FP32 offset sigma_j = (r_j - O_t) / lambda
 //    (in wavelengths) with its complex amplitude, 32 B per atom.
 //  * Pixel prologue: per pixel, rho0 = |T_p| and Theta0 = frac(rho0/lambda)
@@ -26,9 +26,11 @@
 //    detectors; slice partials (FP64) are summed in slice order by a second
 //    kernel, so results do not depend on scheduling.
 //
-// The hoist is exact to first order in (tile radius / distance); the host
-// declines (MOSAIC_ERR_UNSUPPORTED) when that ratio exceeds 1e-6 or the
-// dropped third-order phase term exceeds 1e-7 revolutions.
+// The hoist is exact to first order in (tile radius / distance); where that
+// ratio exceeds 1e-6 or the dropped third-order phase term exceeds 1e-7
+// revolutions, the exact per-pair kernel (k_sum_exact) runs instead.  The
+// far-field forms (row recurrence, matrix-core, moment expansion) are
+// described at their kernels.
 
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>

@@ -12,5 +12,10 @@ Contents:
   propagation.py -- NumPy restatement of the angular-spectrum propagator,
                     the pad rule and the thin lens (Beam.py:4622-4900,
                     7945-8022; Optics.py:49-130, 513-639).
-  closed_forms.py -- analytic results used to pin the restatements.
+  adjoint.py     -- the displacement gradient from long-double atom terms
+                    and its central finite differences.
+  defects.py     -- NumPy twin of the device defect field.
+The closed forms that pin these restatements live in the tests
+(tests/test_oracle.py, tests/test_oracle_pins.py,
+tests/test_propagation_oracle.py, tests/test_adjoint_oracle.py).
 """
