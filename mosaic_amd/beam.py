@@ -181,8 +181,15 @@ class beam:
         copy.wait_stream(torch.cuda.current_stream(dev))
         hp_np, hc_np = hp.numpy(), hc.numpy()
         off = 0
+        path_of = getattr(sample, "chunk_position_path", None)
         for cid, n in zip(chunk_ids, sizes):
-            pos = sample.load_chunk_positions(cid, use_gpu=False)
+            path = path_of(cid) if path_of is not None else None
+            if path is not None and not getattr(sample, "enable_temp", False):
+                # a chunk file with nothing to apply on load: copied once, from
+                # the page cache straight into the pinned buffer
+                pos = np.load(path, mmap_mode="r")
+            else:
+                pos = sample.load_chunk_positions(cid, use_gpu=False)
             codes, lab = sample.load_chunk_species(cid, use_gpu=False, codes=True)
             if n == 0:
                 continue
