@@ -280,6 +280,49 @@ int mosaic_ein_bilinear(int64_t atom_count, const float* pos, const float* rot /
                         int NyB, int NzB, double du, double dv, double uc,
                         double vc, float* amp_out, mosaic_stream_t stream);
 
+/*
+ * Depth-dependent entrance field, for a +x beam on the beam grid (NyB x NzB,
+ * pitches du, dv [A], centre indices uc, vc).  Replaces the reference's
+ * precompute_depth_ein_all_chunks (Beam.py:4963-5238) with its beam-column map
+ * (_compute_beam_column_A_map_cpu/_gpu, Beam.py:7118-7282), depth window
+ * (_compute_global_depth_bounds, Beam.py:5583-5620) and E0 * tau * phi
+ * sampler (ein_bilinear_kernel / _ein_bilinear_cpu, Beam.py:956-1045,
+ * 2439-2578).  Positions are float32 sample-frame Angstrom with the pose
+ * lab = rot @ pos + trans (host float32); every call works in one caller
+ * workspace of mosaic_ein_depth_workspace bytes, and mosaic_ein_depth reads
+ * the maps that the last mosaic_ein_column_map on the same workspace made.
+ */
+int mosaic_ein_depth_workspace(int NyB, int NzB, size_t* bytes);
+
+/* (s_min, s_max) = min and max of the atoms' lab x [A] (host out [2]);
+ * (0, 1) when degenerate.  Synchronizes the stream. */
+int mosaic_ein_depth_bounds(int64_t atom_count, const float* pos, const float* rot /* host */,
+                            const float* trans /* host */, int NyB, int NzB, double* bounds,
+                            void* workspace, size_t workspace_bytes, mosaic_stream_t stream);
+
+/* Beam-column map: triangular-shaped-cloud deposit of each in-grid atom's
+ * forward factors fwd[2 * species] = (f0(0) + f', f'') (host float32), then
+ * phi = -k C sum_re, tau = max(k C sum_im, 0), C = r_e lambda^2 / (2 pi du dv),
+ * an optional Gaussian blur of kernel_radius pixels (<= 16; circular, kernel
+ * at the grid corner), A = exp(-tau + i phi) (written to A_out, device
+ * NyB x NzB complex64, when not NULL) and the sampled maps
+ * tau' = -ln(|A| + 1e-20), phi' = arg A kept in the workspace. */
+int mosaic_ein_column_map(int64_t atom_count, const float* pos, const uint8_t* species,
+                          int n_species, const float* fwd /* host */, const float* rot /* host */,
+                          const float* trans /* host */, int NyB, int NzB, double du, double dv,
+                          double uc, double vc, double wavelength_A, int kernel_radius,
+                          float* A_out, void* workspace, size_t workspace_bytes,
+                          mosaic_stream_t stream);
+
+/* Per-atom Ein = E0 exp(-f tau') exp(i f phi') (E0, tau', phi' bilinear at
+ * the atom's (u, v); f = clip((x - s_min) / (s_max - s_min), 0, 1)) into
+ * amp_out (device (N,) complex64); zero outside the grid. */
+int mosaic_ein_depth(int64_t atom_count, const float* pos, const float* rot /* host */,
+                     const float* trans /* host */, const float* E0 /* device */, int NyB,
+                     int NzB, double du, double dv, double uc, double vc, double s_min,
+                     double s_max, float* amp_out, const void* workspace,
+                     size_t workspace_bytes, mosaic_stream_t stream);
+
 /* ----------------------------------------------------------- propagation */
 
 /* Thin lens exp(-i k r^2 / 2f) with optional CRL absorption

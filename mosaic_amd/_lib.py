@@ -37,6 +37,10 @@ EXPORTS = (
     "mosaic_scatter",
     "mosaic_scatter_noalloc",
     "mosaic_ein_bilinear",
+    "mosaic_ein_depth_workspace",
+    "mosaic_ein_depth_bounds",
+    "mosaic_ein_column_map",
+    "mosaic_ein_depth",
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
     "mosaic_lens_apply",
@@ -101,6 +105,12 @@ _SIGS = {
     "mosaic_scatter_noalloc": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp,
                                _d, _i, _d, _d, _d, _d, _i, _vp, _vp, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
+    "mosaic_ein_depth_workspace": [_i, _i, ctypes.POINTER(_sz)],
+    "mosaic_ein_depth_bounds": [_i64, _vp, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp],
+    "mosaic_ein_column_map": [_i64, _vp, _vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _i,
+                              _vp, _vp, _sz, _vp],
+    "mosaic_ein_depth": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _d, _vp, _vp,
+                         _sz, _vp],
     "mosaic_asp_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
                              ctypes.c_float, _vp, _i, _vp, _vp, _sz, _vp],

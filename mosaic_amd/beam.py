@@ -24,6 +24,10 @@ Deliberate differences, each bounded by the tests:
   that is exact to ~1e-7; elsewhere, and with an analyser, the exact
   per-pair kernel runs.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import torch
 
@@ -49,8 +53,12 @@ class beam:
         self._q = tables.Q_E
         self._hq = self._h / self._q
         self._workspace = None
-        # kinematic kernel selection: "auto", "tile" or "rows" (tests pin it)
+        # kinematic kernel selection: "auto", "exact", "tile", "rows", "mma" or
+        # "poly" (tests pin it)
         self._kin_path = "auto"
+        # Gaussian blur radius [beam pixels] of the depth-Ein column map; the
+        # reference's driver uses 0 (Beam.py:6240-6247)
+        self._ein_kernel_radius = 0
 
     # -------------------------------------------------------------- tables
     parse_f0_db_all = staticmethod(lambda database_name="f0_WaasKirf.dat": tables.parse_f0_db_all())
@@ -215,9 +223,91 @@ class beam:
         counts = np.bincount(hc_np[:total], minlength=max(1, len(labels))).astype(np.int64)
         return d_pos, d_cod, counts, ev
 
+    # --------------------------------------------------- depth-dependent Ein
+    def _ein_cache_key(self, sample, rot, trans, s_min, s_max, kernel_radius=0):
+        """SHA1 naming one pose's cached depth-dependent Ein files: the beam,
+        the pose, the depth window, the sample's chunk files (size and
+        modification time) and thermal settings (the reference's key,
+        Beam.py:5520-5564)."""
+        files = []
+        sdir = str(getattr(sample, "directory", "") or "")
+        for cid in range(1, int(getattr(sample, "chunk_total", 0) or 0) + 1):
+            p = os.path.join(sdir, f"atomic_positions_{cid}.npy")
+            try:
+                st = os.stat(p)
+                files.append([cid, int(st.st_size), int(st.st_mtime_ns)])
+            except OSError:
+                files.append([cid, None, None])
+        temp = getattr(sample, "temp_params", None)
+        gw = self._gauss_waist
+        key = dict(
+            E_eV=float(self._energy), lam=float(self._wavelength),
+            direction=[float(x) for x in self._direction],
+            stage_R=np.asarray(rot, dtype=float).round(7).tolist(),
+            stage_T=[float(x) for x in np.asarray(trans, dtype=float)],
+            beam_size=[float(x) for x in self._beam_size],
+            beam_samples=[int(self._beam_Ny), int(self._beam_Nz)],
+            beam_profile=self._beam_profile,
+            gauss_waist=[None if gw is None else float(gw[0]), None if gw is None else float(gw[1])],
+            s_min=float(s_min), s_max=float(s_max),
+            sample_dir=os.path.abspath(sdir) if sdir else "",
+            chunk_files=files, enable_temp=bool(getattr(sample, "enable_temp", False)),
+            temp_params=None if temp is None else [str(x) for x in temp],
+            thermal_expansion=bool(getattr(sample, "_thermal_expansion_enabled", False)),
+            kernel_radius=int(kernel_radius))
+        return hashlib.sha1(json.dumps(key, sort_keys=True).encode("utf-8")).hexdigest()
+
+    @staticmethod
+    def _prune_ein_cache(cache_dir, key_hash):
+        """Delete cached Ein files of other keys (Beam.py:5566-5581)."""
+        if not os.path.isdir(cache_dir):
+            return
+        for name in os.listdir(cache_dir):
+            if name.startswith("ein_chunk_") and name.endswith(".npz") and key_hash not in name:
+                try:
+                    os.remove(os.path.join(cache_dir, name))
+                except OSError:
+                    pass
+
+    def _depth_ein(self, dein, sample, chunk_ids, sizes, d_pos, d_cod, fwd, rot, trans, E0,
+                   cache_dir, recompute):
+        """Per-atom depth-dependent Ein for one pose, through the Ein cache
+        when ``cache_dir`` is set (one ``ein_chunk_<cid>_<hash>.npz`` per
+        chunk, as the reference writes them, Beam.py:5021-5044)."""
+        s_min, s_max = dein.bounds(d_pos, rot, trans)
+        key = paths = None
+        if cache_dir:
+            key = self._ein_cache_key(sample, rot, trans, s_min, s_max, self._ein_kernel_radius)
+            self._prune_ein_cache(cache_dir, key)
+            os.makedirs(cache_dir, exist_ok=True)
+            paths = [os.path.join(cache_dir, f"ein_chunk_{cid}_{key}.npz") for cid in chunk_ids]
+            if not recompute and all(os.path.isfile(p) for p in paths):
+                parts = []
+                for p in paths:
+                    with np.load(p) as z:
+                        parts.append(np.asarray(z["ein"], dtype=np.complex64))
+                host = np.concatenate(parts) if parts else np.zeros(0, np.complex64)
+                if host.shape[0] == d_pos.shape[0]:
+                    self._last_ein_source = "cache"
+                    return torch.from_numpy(host).to(d_pos.device)
+        dein.column_map(d_pos, d_cod, fwd, rot, trans, self._beam_du, self._beam_dv,
+                        self._beam_uc, self._beam_vc, self._wavelength * 1e10,
+                        self._ein_kernel_radius)
+        amp = dein.sample(d_pos, rot, trans, E0, self._beam_du, self._beam_dv, self._beam_uc,
+                          self._beam_vc, s_min, s_max)
+        self._last_ein_source = "computed"
+        if paths is not None:
+            host = amp.cpu().numpy()
+            off = 0
+            for p, n in zip(paths, sizes):
+                np.savez_compressed(p, ein=host[off:off + n])
+                off += n
+        return amp
+
     def _kin_field(self, sample, coords_A, shape, stage, frames, remove_forward,
                    apply_polarization, spherical_decay, out_dtype=torch.complex128,
-                   analyser=None):
+                   analyser=None, use_depth_ein=False, ein_cache_dir=None,
+                   recompute_cache=False):
         """Sum every chunk of ``sample`` for the poses ``frames`` = (rot, trans).
 
         All chunks go into one device array (pinned host buffer, copies on a
@@ -259,6 +349,13 @@ class beam:
         groups.append(cur)
         labels, index = [], {}
         staged_prev = None
+        if use_depth_ein and len(groups) > 1:
+            raise NotImplementedError("depth-dependent Ein over more than "
+                                      f"{self.MAX_GROUP_ATOMS} atoms")
+        cache_dir = None
+        if use_depth_ein:
+            cache_dir = ein_cache_dir or (os.path.join(self.directory, "ein_cache")
+                                          if self.directory else None)
         for g in groups:
             d_pos, d_cod, counts, ev = self._gather_chunks(sample, g, labels, index, dev, copy)
             compute.wait_event(ev)
@@ -272,7 +369,16 @@ class beam:
                               self._polarization_state(), spherical_decay, path=self._kin_path,
                               analyser=ana[0], analyser_axis=ana[1], accept_rad=ana[2],
                               darwin_hw_rad=ana[3])
+            if use_depth_ein:
+                dein = K.DepthEin(self._beam_Ny, self._beam_Nz, dev)
+                fwd = np.stack([(consts["f0z"] + consts["fp"]).astype(np.float32),
+                                consts["fpp"]], axis=1)
+                sizes = [int(sample.chunk_atom_count(c)) for c in g]
+
             def ein(f):
+                if use_depth_ein:
+                    return self._depth_ein(dein, sample, g, sizes, d_pos, d_cod, fwd, rot[f],
+                                           trans[f], E0, cache_dir, recompute_cache)
                 return K.ein_bilinear(d_pos, rot[f], trans[f], E0, self._beam_du,
                                       self._beam_dv, self._beam_uc, self._beam_vc)
 
@@ -307,8 +413,6 @@ class beam:
                           recompute_cache=False, apply_polarization=False, spherical_decay=True,
                           analyser_mode="off", analyser_acceptance_angle_rad=0.0,
                           analyser_darwin_halfwidth_rad=0.0):
-        if use_depth_ein:
-            raise NotImplementedError("depth-dependent Ein is not part of this build")
         Ny, Nz = measurement_shape
         rot = np.asarray(stage.rotation, dtype=np.float32).reshape(1, 3, 3)
         trans = np.asarray(stage.translation, dtype=np.float32).reshape(1, 3)
@@ -320,7 +424,9 @@ class beam:
                               apply_polarization, spherical_decay,
                               analyser=self._analyser(mp, Ny, Nz, analyser_mode,
                                                       analyser_acceptance_angle_rad,
-                                                      analyser_darwin_halfwidth_rad))
+                                                      analyser_darwin_halfwidth_rad),
+                              use_depth_ein=use_depth_ein, ein_cache_dir=ein_cache_dir,
+                              recompute_cache=recompute_cache)
         return out[0].to(torch.complex64).cpu().numpy()
 
     @staticmethod

@@ -253,6 +253,64 @@ def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
     return out
 
 
+class DepthEin:
+    """The depth-dependent entrance field of one sample on one beam grid
+    (mosaic_ein_depth_*): the depth window, the beam-column map and the
+    per-atom sampler, with their workspace."""
+
+    def __init__(self, NyB, NzB, device):
+        self.NyB, self.NzB = int(NyB), int(NzB)
+        n = ctypes.c_size_t(0)
+        check(_lib.lib().mosaic_ein_depth_workspace(self.NyB, self.NzB, ctypes.byref(n)),
+              "mosaic_ein_depth_workspace")
+        self.ws = torch.empty(max(1, n.value), dtype=torch.uint8, device=device)
+
+    @staticmethod
+    def _pose(rot, trans):
+        r = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(3, 3))
+        t = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(3))
+        return r, t
+
+    def bounds(self, pos, rot, trans, stream=None):
+        """(s_min, s_max) along the beam [Angstrom]; synchronizes."""
+        r, t = self._pose(rot, trans)
+        out = (ctypes.c_double * 2)()
+        check(_lib.lib().mosaic_ein_depth_bounds(
+            int(pos.shape[0]), _ptr(pos.contiguous()), r.ctypes.data_as(ctypes.c_void_p),
+            t.ctypes.data_as(ctypes.c_void_p), self.NyB, self.NzB, out, _ptr(self.ws),
+            int(self.ws.numel()), _stream_handle(stream)), "mosaic_ein_depth_bounds")
+        return float(out[0]), float(out[1])
+
+    def column_map(self, pos, codes, fwd, rot, trans, du, dv, uc, vc, wavelength_A,
+                   kernel_radius=0, A_out=None, stream=None):
+        """Deposit the atoms' forward factors (fwd: (S, 2) float32 host,
+        f0(0) + f' and f'') and form the sampled maps; optionally A (NyB, NzB)
+        complex64 into ``A_out``."""
+        r, t = self._pose(rot, trans)
+        f = np.ascontiguousarray(np.asarray(fwd, dtype=np.float32).reshape(-1, 2))
+        check(_lib.lib().mosaic_ein_column_map(
+            int(pos.shape[0]), _ptr(pos.contiguous()), _ptr(codes.contiguous()), int(f.shape[0]),
+            f.ctypes.data_as(ctypes.c_void_p), r.ctypes.data_as(ctypes.c_void_p),
+            t.ctypes.data_as(ctypes.c_void_p), self.NyB, self.NzB, float(du), float(dv),
+            float(uc), float(vc), float(wavelength_A), int(kernel_radius),
+            _ptr(torch.view_as_real(A_out) if A_out is not None else None), _ptr(self.ws),
+            int(self.ws.numel()), _stream_handle(stream)), "mosaic_ein_column_map")
+
+    def sample(self, pos, rot, trans, E0, du, dv, uc, vc, s_min, s_max, out=None, stream=None):
+        """Per-atom Ein (N,) complex64 from the maps of the last column_map."""
+        n = int(pos.shape[0])
+        if out is None:
+            out = torch.empty(n, dtype=torch.complex64, device=pos.device)
+        r, t = self._pose(rot, trans)
+        check(_lib.lib().mosaic_ein_depth(
+            n, _ptr(pos.contiguous()), r.ctypes.data_as(ctypes.c_void_p),
+            t.ctypes.data_as(ctypes.c_void_p), _ptr(torch.view_as_real(E0.contiguous())),
+            self.NyB, self.NzB, float(du), float(dv), float(uc), float(vc), float(s_min),
+            float(s_max), _ptr(torch.view_as_real(out)), _ptr(self.ws), int(self.ws.numel()),
+            _stream_handle(stream)), "mosaic_ein_depth")
+        return out
+
+
 def same_amplitudes(a, b, rtol=1e-6):
     """Whether two poses' incident amplitudes agree to ``rtol`` of the largest.
 
