@@ -367,6 +367,72 @@ int mosaic_asp_propagate(const float* field_in, int Ny, int Nz, double dy, doubl
 int mosaic_lens_apply(float* field, int Ny, int Nz, double dx, double dy, double wavelength,
                       const mosaic_lens_t* lens /* host */, mosaic_stream_t stream);
 
+/* -------------------------------------------- optics elements, detector */
+
+/* Two-bounce Bragg magnifier, a geometric resampling (_apply_bragg_magnifier_2b,
+ * Optics.py:132-209): output pixel (x, y) samples the input at
+ * (cx + (x - cx) / Mx, cy + (y - cy) / My), nearest (order 0) or bilinear,
+ * zeros outside or edge-clamped (pad_edge), times sqrt(reflectivity)
+ * [/ sqrt |Mx My| when conserve_energy] e^{i phase_shift}.  (Ny, Nx)
+ * complex64 device fields; field_in and field_out must differ. */
+int mosaic_bragg_magnifier(const float* field_in, float* field_out, int Ny, int Nx, double Mx,
+                           double My, double reflectivity, double phase_shift, int order,
+                           int pad_edge, int conserve_energy, mosaic_stream_t stream);
+
+/* Angular filter on the field's 2-D spectrum (_apply_angular_filter_kspace,
+ * Optics.py:211-336): angles theta = k_perp / k from fftfreq, shifted by the
+ * centre, rolled by roll_deg; mask 1 / (1 + r^2) (Darwin), r <= 1 (top-hat)
+ * or 1 / sqrt(1 + r^(2 order)) (Butterworth), r the normalised deviation of
+ * the 1-D slit, the ellipse or the circle; then sqrt(transmission)
+ * e^{i phase_shift}.  In place on the (Ny, Nx) complex64 device field, pitches
+ * dx (columns), dy (rows) [m]; workspace: the FFT's work area. */
+typedef struct {
+    double center_x_mrad, center_y_mrad;
+    double half_angle_x_mrad, half_angle_y_mrad;
+    double transmission, phase_shift, roll_deg;
+    int mode_1d;       /* 1: slit along the rolled pass axis */
+    int elliptical;    /* 2-D shape: 1 elliptical, 0 circular */
+    int rolloff;       /* 0 top-hat, 1 Darwin, 2 Butterworth */
+    int order;         /* Butterworth order */
+} mosaic_angular_filter_t;
+
+int mosaic_angular_filter_workspace(int Ny, int Nx, size_t* bytes);
+int mosaic_angular_filter(float* field, int Ny, int Nx, double dx, double dy, double wavelength,
+                          const mosaic_angular_filter_t* filter /* host */, void* workspace,
+                          size_t workspace_bytes, mosaic_stream_t stream);
+
+/* Fresnel zone plate as a real-space transmission (_apply_fresnel_zone_plate,
+ * Optics.py:338-425): zones r_n = sqrt(n lambda f1), f1 = D dr_N / lambda;
+ * amplitude (even zones open), phase (odd zones -1) or ideal (thin-lens phase
+ * at f1 / order) inside the diameter, minus a central stop, with an optional
+ * cosine edge taper; times sqrt(efficiency).  In place. */
+typedef struct {
+    double outermost_zone_width_nm, diameter_um, efficiency;
+    double central_stop_fraction, edge_taper;
+    int order;
+    int type;          /* 0 amplitude, 1 phase, 2 ideal */
+} mosaic_zone_plate_t;
+
+int mosaic_zone_plate(float* field, int Ny, int Nx, double dx, double dy, double wavelength,
+                      const mosaic_zone_plate_t* zp /* host */, mosaic_stream_t stream);
+
+/* Aperture of width (or diameter) width_mm centred on the field, square or
+ * circular (_apply_aperture, Optics.py:427-471).  In place. */
+int mosaic_aperture(float* field, int Ny, int Nx, double dx, double dy, double width_mm,
+                    int circular, mosaic_stream_t stream);
+
+/* Detector point-spread function (apply_psf, Detector.py:873-1087): the
+ * intensity |E|^2 (or the amplitude |E| when `amplitude`) of the (rows, cols)
+ * complex64 device field convolved in FP64 with `kernel` (host, kernel_size^2,
+ * odd size, normalised by the caller), boundary modes per axis as
+ * scipy.ndimage: 0 constant, 1 reflect, 2 nearest, 3 mirror, 4 wrap; clipped
+ * at 0; the field becomes sqrt(result) (or result) with its phase kept.  In
+ * place; synchronizes the stream. */
+int mosaic_psf_workspace(int rows, int cols, int kernel_size, size_t* bytes);
+int mosaic_psf_apply(float* field, int rows, int cols, const double* kernel /* host */,
+                     int kernel_size, int mode_rows, int mode_cols, int amplitude,
+                     void* workspace, size_t workspace_bytes, mosaic_stream_t stream);
+
 /* Defected crystal for the surrogate's training images: base (N, 3) float32
  * crystal-frame positions [A] plus one dislocation (isotropic elasticity),
  * a hydrostatic strain and one stacking fault, rotated into the sample frame,

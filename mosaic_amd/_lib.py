@@ -44,6 +44,13 @@ EXPORTS = (
     "mosaic_asp_workspace",
     "mosaic_asp_propagate",
     "mosaic_lens_apply",
+    "mosaic_bragg_magnifier",
+    "mosaic_angular_filter_workspace",
+    "mosaic_angular_filter",
+    "mosaic_zone_plate",
+    "mosaic_aperture",
+    "mosaic_psf_workspace",
+    "mosaic_psf_apply",
     "mosaic_defect_positions",
     "mosaic_sg_gemm",
     "mosaic_sg_bias_act",
@@ -84,6 +91,20 @@ class Lens(ctypes.Structure):
                 ("uniform_amp", ctypes.c_float), ("pad_", ctypes.c_float)]
 
 
+class AngularFilter(ctypes.Structure):
+    _fields_ = [("center_x_mrad", ctypes.c_double), ("center_y_mrad", ctypes.c_double),
+                ("half_angle_x_mrad", ctypes.c_double), ("half_angle_y_mrad", ctypes.c_double),
+                ("transmission", ctypes.c_double), ("phase_shift", ctypes.c_double),
+                ("roll_deg", ctypes.c_double), ("mode_1d", ctypes.c_int),
+                ("elliptical", ctypes.c_int), ("rolloff", ctypes.c_int), ("order", ctypes.c_int)]
+
+
+class ZonePlate(ctypes.Structure):
+    _fields_ = [("outermost_zone_width_nm", ctypes.c_double), ("diameter_um", ctypes.c_double),
+                ("efficiency", ctypes.c_double), ("central_stop_fraction", ctypes.c_double),
+                ("edge_taper", ctypes.c_double), ("order", ctypes.c_int), ("type", ctypes.c_int)]
+
+
 _LIB = None
 
 _vp, _i, _i64, _d, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double,
@@ -115,6 +136,13 @@ _SIGS = {
     "mosaic_asp_propagate": [_vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, ctypes.c_float,
                              ctypes.c_float, _vp, _i, _vp, _vp, _sz, _vp],
     "mosaic_lens_apply": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
+    "mosaic_bragg_magnifier": [_vp, _vp, _i, _i, _d, _d, _d, _d, _i, _i, _i, _vp],
+    "mosaic_angular_filter_workspace": [_i, _i, ctypes.POINTER(_sz)],
+    "mosaic_angular_filter": [_vp, _i, _i, _d, _d, _d, _vp, _vp, _sz, _vp],
+    "mosaic_zone_plate": [_vp, _i, _i, _d, _d, _d, _vp, _vp],
+    "mosaic_aperture": [_vp, _i, _i, _d, _d, _d, _i, _vp],
+    "mosaic_psf_workspace": [_i, _i, _i, ctypes.POINTER(_sz)],
+    "mosaic_psf_apply": [_vp, _i, _i, _vp, _i, _i, _i, _i, _vp, _sz, _vp],
     "mosaic_defect_positions": [_i64, _vp, _vp, _vp, _vp],
     "mosaic_sg_gemm": [_i, _i, _i, _i, _i, _vp, _i, _ll, _vp, _i, _ll, _vp, _i, _ll, _i, _f, _f,
                        _vp],

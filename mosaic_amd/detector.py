@@ -13,7 +13,11 @@ consumes (Detector.py:73-231, 569-770):
   (Detector.py:613-741);
 * ``pixel_coordinates`` (3, Ny*Nz) float64 Angstrom, ``shape`` (Ny, Nz),
   ``pixel_size`` (dy, dz) Angstrom, ``input_pixel_values`` and the
-  ``pixel_values`` / ``pixel_intensity`` views.
+  ``pixel_values`` / ``pixel_intensity`` views;
+* ``add_noise`` (Poisson, Gaussian, uniform; Detector.py:772-846) and
+  ``apply_psf`` (Gaussian, Lorentzian or pseudo-Voigt kernel, intensity or
+  amplitude, scipy.ndimage boundary modes; Detector.py:848-1087), the
+  convolution on the GPU.
 
 Only rectangular spatial detectors in plane or shell construction are
 implemented; ring and angular layouts raise ``NotImplementedError``.
@@ -142,6 +146,140 @@ class detector:
     def pixel_phase(self):
         v = self._pixel_values
         return None if v is None else np.angle(v)
+
+    # ------------------------------------------------ noise and resolution
+    def add_noise(self, noise_type, photon_flux=None, sigma=None, seed=None):
+        """Poisson, Gaussian or uniform noise on the stored field (add_noise,
+        Detector.py:772-846).  The draws come from numpy's default_rng(seed)
+        in the reference's order, so a seed gives the reference's realisation;
+        the arithmetic around them is elementwise float32 (host)."""
+        if self._pixel_values is None:
+            raise ValueError("No pixel values to add noise to. Run a simulation first.")
+        rng = np.random.default_rng(seed)
+        E = np.array(self._pixel_values, dtype=np.complex64, copy=True)
+        noise_type = str(noise_type).lower()
+        if noise_type == "poisson":
+            if photon_flux is None:
+                raise ValueError("photon_flux is required for Poisson noise")
+            photon_flux = float(photon_flux)
+            intensity = np.abs(E) ** 2
+            I_max = intensity.max()
+            if I_max <= 0.0:
+                return
+            counts = rng.poisson(intensity * (photon_flux / I_max)).astype(np.float32)
+            amp = np.sqrt(counts * (I_max / photon_flux))
+            phase = np.angle(E)
+            E = (amp * (np.cos(phase) + 1j * np.sin(phase))).astype(np.complex64)
+        elif noise_type == "gaussian":
+            if sigma is None:
+                raise ValueError("sigma is required for Gaussian noise")
+            scale = float(sigma) * np.abs(E).mean() / np.sqrt(2.0)
+            noise = scale * (rng.standard_normal(E.shape).astype(np.float32)
+                             + 1j * rng.standard_normal(E.shape).astype(np.float32))
+            E = (E + noise).astype(np.complex64)
+        elif noise_type == "uniform":
+            if sigma is None:
+                raise ValueError("sigma is required for uniform noise")
+            scale = float(sigma) * np.abs(E).mean()
+            noise = scale * (rng.uniform(-1, 1, E.shape).astype(np.float32)
+                             + 1j * rng.uniform(-1, 1, E.shape).astype(np.float32))
+            E = (E + noise).astype(np.complex64)
+        else:
+            raise ValueError(f"Unknown noise type '{noise_type}'. "
+                             f"Use 'poisson', 'gaussian', or 'uniform'.")
+        self.input_pixel_values(E)
+
+    @staticmethod
+    def _build_psf_kernel(sigma_ax0, sigma_ax1, profile="gaussian", rotation=0.0, eta_pv=0.5,
+                          truncate=4.0):
+        """Normalised PSF kernel on the pixel grid (float64), Gaussian,
+        Lorentzian or pseudo-Voigt, rotated ellipse (Detector.py:848-871)."""
+        max_sigma = max(sigma_ax0, sigma_ax1)
+        if max_sigma <= 0:
+            raise ValueError("PSF width must be positive")
+        hw = max(1, int(np.ceil(truncate * max_sigma)))
+        ax = np.arange(2 * hw + 1, dtype=np.float64) - hw
+        g0, g1 = np.meshgrid(ax, ax, indexing="ij")
+        rot = np.deg2rad(rotation)
+        c, s = np.cos(rot), np.sin(rot)
+        u = (c * g0 + s * g1) / sigma_ax0
+        v = (-s * g0 + c * g1) / sigma_ax1
+        r2 = u ** 2 + v ** 2
+        profile = profile.lower()
+        if profile == "gaussian":
+            k = np.exp(-0.5 * r2)
+        elif profile == "lorentzian":
+            k = 1.0 / (1.0 + r2)
+        elif profile == "pseudo-voigt":
+            e = float(np.clip(eta_pv, 0.0, 1.0))
+            k = (1.0 - e) * np.exp(-0.5 * r2) + e / (1.0 + r2)
+        else:
+            raise ValueError(f"Unknown profile '{profile}'. "
+                             f"Use 'gaussian', 'lorentzian', or 'pseudo-voigt'.")
+        tot = k.sum()
+        if tot > 0:
+            k /= tot
+        return k
+
+    _BOUNDARY = {"constant": 0, "reflect": 1, "nearest": 2, "mirror": 3, "wrap": 4}
+
+    def apply_psf(self, width, profile="gaussian", rotation=0.0, units="pixels",
+                  field="intensity", eta_pv=0.5, boundary="auto", truncate=4.0):
+        """Convolve the detector intensity (or amplitude) with a PSF kernel,
+        keeping the phase (apply_psf, Detector.py:873-1087); the convolution
+        runs on the GPU (mosaic_psf_apply)."""
+        import ctypes
+
+        import torch
+
+        from . import _lib
+        if self._pixel_values is None:
+            raise ValueError("No pixel values to apply PSF to. Run a simulation first.")
+        if self._construction_mode == "shell" and self._input_mode != "angular":
+            raise NotImplementedError(
+                "PSF convolution is not supported for shell+spatial detectors because the pixel "
+                "grid is non-regular after spherical projection. Use shell+angular mode instead.")
+        if isinstance(width, (int, float, np.integer, np.floating)):
+            s0 = s1 = float(width)
+        elif hasattr(width, "__len__") and len(width) == 2:
+            s0, s1 = float(width[0]), float(width[1])
+        else:
+            raise ValueError("width must be a scalar (circular) or 2-tuple (elliptical).")
+        units = str(units).lower()
+        if units == "physical":
+            s0 /= float(self._pixel_size[0])
+            s1 /= float(self._pixel_size[1])
+        elif units == "angular":
+            raise ValueError("units='angular' requires an angular-mode detector. "
+                             "Use 'physical' or 'pixels'.")
+        elif units != "pixels":
+            raise ValueError(f"Unknown units '{units}'. Use 'pixels', 'physical', or 'angular'.")
+        if s0 <= 0 or s1 <= 0:
+            raise ValueError("PSF width must be positive after unit conversion.")
+        kernel = np.ascontiguousarray(
+            self._build_psf_kernel(s1, s0, profile=profile, rotation=rotation, eta_pv=eta_pv,
+                                   truncate=truncate), dtype=np.float64)
+        field = str(field).lower()
+        if field not in ("intensity", "amplitude"):
+            raise ValueError(f"Unknown field '{field}'. Use 'intensity' or 'amplitude'.")
+        boundary = str(boundary).lower()
+        mode = 0 if boundary == "auto" else self._BOUNDARY.get(boundary)
+        if mode is None:
+            raise ValueError(f"Unknown boundary '{boundary}'")
+        rows, cols = self._shape[1], self._shape[0]
+        dev = torch.device("cuda", torch.cuda.current_device())
+        E = torch.as_tensor(np.ascontiguousarray(
+            np.asarray(self._pixel_values, dtype=np.complex64).reshape(rows, cols)), device=dev)
+        n = ctypes.c_size_t(0)
+        _lib.check(_lib.lib().mosaic_psf_workspace(rows, cols, kernel.shape[0], ctypes.byref(n)),
+                   "mosaic_psf_workspace")
+        ws = torch.empty(max(1, n.value), dtype=torch.uint8, device=dev)
+        _lib.check(_lib.lib().mosaic_psf_apply(
+            ctypes.c_void_p(int(torch.view_as_real(E).data_ptr())), rows, cols,
+            kernel.ctypes.data_as(ctypes.c_void_p), int(kernel.shape[0]), mode, mode,
+            int(field == "amplitude"), ctypes.c_void_p(int(ws.data_ptr())), int(ws.numel()),
+            ctypes.c_void_p(int(torch.cuda.current_stream().cuda_stream))), "mosaic_psf_apply")
+        self.input_pixel_values(E.cpu().numpy().ravel())
 
     @property
     def pixel_coordinates(self):
