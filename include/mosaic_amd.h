@@ -7,8 +7,9 @@
  *     memory, "host" pointers are ordinary host memory;
  *   - every call is ordered on the HIP stream it is given (0 = the null
  *     stream) and returns an int status (MOSAIC_OK or an error below);
- *   - the library keeps no state between calls: scratch memory comes from
- *     the caller as a workspace whose size a *_workspace query returns;
+ *   - scratch memory comes from the caller as a workspace whose size a
+ *     *_workspace query returns; the library keeps only hipFFT plans (per
+ *     device and size) and the allocation-free entry point's fixed scratch;
  *   - complex arrays are interleaved (re, im) pairs.
  *
  * Pixel layout follows the reference: p = iz * Ny + iy, fields (Nz, Ny)
@@ -322,6 +323,23 @@ int mosaic_ein_depth(int64_t atom_count, const float* pos, const float* rot /* h
                      int NzB, double du, double dv, double uc, double vc, double s_min,
                      double s_max, float* amp_out, const void* workspace,
                      size_t workspace_bytes, mosaic_stream_t stream);
+
+/* M-channel incident amplitude of the general kernel (Beam.py:2283-2309):
+ * amp[j] = sum_m E[j * M + m] e^{2 pi i g_m . r_j} with r_j = rot @ pos_j +
+ * trans in metres (pos float32 sample-frame Angstrom, rot/trans host float32,
+ * trans in Angstrom), g (host float64, M x 3) in cycles per metre, M <= 64;
+ * E (N, M) and amp_out (N,) complex64 device.  The phase is formed and
+ * reduced in FP64.  The result feeds any sum as the atoms' amplitudes. */
+int mosaic_mbeam_amplitude(int64_t atom_count, const float* pos, const float* rot /* host */,
+                           const float* trans /* host */, const float* E, int M,
+                           const double* g /* host */, float* amp_out, mosaic_stream_t stream);
+
+/* Hardware sine and cosine of 2 pi x, x in revolutions (v_sin_f32 /
+ * v_cos_f32), as the sum kernels use them, on n device floats; for
+ * characterising their accuracy over the argument ranges the kernels feed.
+ * No counterpart in the reference. */
+int mosaic_sincos_rev(int64_t n, const float* x, float* sin_out, float* cos_out,
+                      mosaic_stream_t stream);
 
 /* ----------------------------------------------------------- propagation */
 

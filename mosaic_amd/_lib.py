@@ -36,6 +36,8 @@ EXPORTS = (
     "mosaic_scatter_workspace",
     "mosaic_scatter",
     "mosaic_scatter_noalloc",
+    "mosaic_sincos_rev",
+    "mosaic_mbeam_amplitude",
     "mosaic_ein_bilinear",
     "mosaic_ein_depth_workspace",
     "mosaic_ein_depth_bounds",
@@ -126,6 +128,8 @@ _SIGS = {
     "mosaic_scatter_noalloc": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp,
                                _d, _i, _d, _d, _d, _d, _i, _vp, _vp, _vp],
     "mosaic_ein_bilinear": [_i64, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _vp, _vp],
+    "mosaic_sincos_rev": [_i64, _vp, _vp, _vp, _vp],
+    "mosaic_mbeam_amplitude": [_i64, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "mosaic_ein_depth_workspace": [_i, _i, ctypes.POINTER(_sz)],
     "mosaic_ein_depth_bounds": [_i64, _vp, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp],
     "mosaic_ein_column_map": [_i64, _vp, _vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _d, _i,

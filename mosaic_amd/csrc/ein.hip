@@ -272,6 +272,40 @@ __global__ __launch_bounds__(EIN_THREADS) void k_ein_depth(const EinDepthArgs A)
     A.out[i] = res;
 }
 
+// M-channel incident amplitude (the general kernel's lattice-phase sum,
+// Beam.py:2283-2309): amp_j = sum_m E[j][m] e^{2 pi i g_m . r_j}, r_j the lab
+// position R s_j + T in metres (FP64), the phase reduced modulo one cycle in
+// FP64 before the float32 sine and cosine.
+constexpr int MBEAM_MAX = 64;
+struct MbeamArgs {
+    double RT[12];              // R row-major, then T [A]
+    double g[3 * MBEAM_MAX];    // reciprocal vectors [cycles per metre]
+    int M;
+};
+
+__global__ __launch_bounds__(EIN_THREADS) void k_mbeam(const float* __restrict__ pos, int64_t n,
+                                                       const float2* __restrict__ E,
+                                                       const MbeamArgs A,
+                                                       float2* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const double lx = (fma(A.RT[0], x, fma(A.RT[1], y, A.RT[2] * z)) + A.RT[9]) * 1e-10;
+    const double ly = (fma(A.RT[3], x, fma(A.RT[4], y, A.RT[5] * z)) + A.RT[10]) * 1e-10;
+    const double lz = (fma(A.RT[6], x, fma(A.RT[7], y, A.RT[8] * z)) + A.RT[11]) * 1e-10;
+    double ar = 0.0, ai = 0.0;
+    for (int m = 0; m < A.M; ++m) {
+        double t = fma(A.g[3 * m], lx, fma(A.g[3 * m + 1], ly, A.g[3 * m + 2] * lz));
+        t -= rint(t);
+        const float tf = static_cast<float>(t);
+        const float cs = __builtin_amdgcn_cosf(tf), sn = __builtin_amdgcn_sinf(tf);
+        const float2 e = E[i * A.M + m];
+        ar += static_cast<double>(e.x * cs - e.y * sn);
+        ai += static_cast<double>(e.x * sn + e.y * cs);
+    }
+    out[i] = make_float2(static_cast<float>(ar), static_cast<float>(ai));
+}
+
 Pose pose_of(const float* rot, const float* trans) {
     Pose P;
     for (int q = 0; q < 9; ++q) P.R[q] = rot[q];
@@ -433,6 +467,24 @@ extern "C" int mosaic_ein_depth(int64_t atom_count, const float* pos, const floa
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_ein_depth, dim3(blocks_for(atom_count, EIN_THREADS)), dim3(EIN_THREADS),
                        0, st, A);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_mbeam_amplitude(int64_t atom_count, const float* pos, const float* rot,
+                                      const float* trans, const float* E, int M,
+                                      const double* g, float* amp_out, mosaic_stream_t stream) {
+    if (atom_count < 0 || !rot || !trans || M < 1 || M > MBEAM_MAX || !g) return MOSAIC_ERR_ARGS;
+    if (atom_count == 0) return MOSAIC_OK;
+    if (!pos || !E || !amp_out) return MOSAIC_ERR_ARGS;
+    MbeamArgs A;
+    for (int q = 0; q < 9; ++q) A.RT[q] = rot[q];
+    for (int d = 0; d < 3; ++d) A.RT[9 + d] = trans[d];
+    for (int q = 0; q < 3 * MBEAM_MAX; ++q) A.g[q] = q < 3 * M ? g[q] : 0.0;
+    A.M = M;
+    hipLaunchKernelGGL(k_mbeam, dim3(blocks_for(atom_count, EIN_THREADS)), dim3(EIN_THREADS), 0,
+                       static_cast<hipStream_t>(stream), pos, atom_count,
+                       reinterpret_cast<const float2*>(E), A, reinterpret_cast<float2*>(amp_out));
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

@@ -216,6 +216,14 @@ __global__ void k_scatter_reduce(const double2* __restrict__ partial, int n_slic
     out_i[p] += static_cast<float>(im * SC_RE);
 }
 
+__global__ void k_sincos_rev(const float* __restrict__ x, int64_t n, float* __restrict__ s,
+                             float* __restrict__ c) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = __builtin_amdgcn_sinf(x[i]);
+    c[i] = __builtin_amdgcn_cosf(x[i]);
+}
+
 struct ScPlan {
     int pix_blocks;
     int n_slices;
@@ -385,4 +393,15 @@ extern "C" int mosaic_scatter_noalloc(int atom_count, const double* positions,
                      coords_z, k_val, apply_pol, pol_deg, pol_ex, pol_ey, pol_ez,
                      apply_spherical_decay, out_r, out_i, static_cast<double2*>(base), P,
                      static_cast<hipStream_t>(stream));
+}
+
+extern "C" int mosaic_sincos_rev(int64_t n, const float* x, float* sin_out, float* cos_out,
+                                 mosaic_stream_t stream) {
+    if (n < 0) return MOSAIC_ERR_ARGS;
+    if (n == 0) return MOSAIC_OK;
+    if (!x || !sin_out || !cos_out) return MOSAIC_ERR_ARGS;
+    hipLaunchKernelGGL(k_sincos_rev, dim3(blocks_for(n, 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), x, n, sin_out, cos_out);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
 }

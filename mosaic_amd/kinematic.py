@@ -311,6 +311,26 @@ class DepthEin:
         return out
 
 
+def mbeam_amplitude(pos, rot, trans, E, g, out=None, stream=None):
+    """M-channel incident amplitude sum_m E[:, m] e^{2 pi i g_m . r} (N,)
+    complex64; E (N, M) complex64 device, g (M, 3) host cycles per metre,
+    r the lab position in metres (Beam.py:2283-2309)."""
+    n = int(pos.shape[0])
+    E = E.to(torch.complex64).contiguous()
+    M = int(E.shape[1])
+    gg = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(M, 3))
+    if out is None:
+        out = torch.empty(n, dtype=torch.complex64, device=pos.device)
+    r = np.ascontiguousarray(np.asarray(rot, dtype=np.float32).reshape(3, 3))
+    t = np.ascontiguousarray(np.asarray(trans, dtype=np.float32).reshape(3))
+    check(_lib.lib().mosaic_mbeam_amplitude(
+        n, _ptr(pos.contiguous()), r.ctypes.data_as(ctypes.c_void_p),
+        t.ctypes.data_as(ctypes.c_void_p), _ptr(torch.view_as_real(E)), M,
+        gg.ctypes.data_as(ctypes.c_void_p), _ptr(torch.view_as_real(out)),
+        _stream_handle(stream)), "mosaic_mbeam_amplitude")
+    return out
+
+
 def same_amplitudes(a, b, rtol=1e-6):
     """Whether two poses' incident amplitudes agree to ``rtol`` of the largest.
 

@@ -11,7 +11,9 @@ NumPy restatement, in float32 and the reference's operation order, of
 * the sampler: tau' = -ln(|A| + 1e-20), phi' = arg A, E0, tau', phi'
   bilinear at (u, v), f = clip((x - s_min) / (s_max - s_min), 0, 1),
   Ein = E0 exp(-f tau') exp(i f phi') (_ein_bilinear_cpu, Beam.py:956-1045;
-  precompute_depth_ein_all_chunks, Beam.py:5044-5060).
+  precompute_depth_ein_all_chunks, Beam.py:5044-5060);
+* the M-channel incident amplitude sum_m E_m e^{2 pi i g_m . r}
+  (Beam.py:2283-2309).
 """
 import numpy as np
 
@@ -136,3 +138,16 @@ def ein_depth(pos_lab, tau, phi, E0, du, dv, uc, vc, s_min, s_max):
     out[inb] = ((es.real * c - es.imag * s) * amp + 1j * (es.real * s + es.imag * c) * amp
                 ).astype(np.complex64)
     return out
+
+
+def mbeam_amplitude(pos_lab_m, E, g):
+    """M-channel incident amplitude sum_m E[:, m] exp(2 pi i g_m . r) in
+    float64, r the lab position in metres and g (M, 3) in cycles per metre
+    (the lattice-phase sum of the general kernel, Beam.py:2283-2309; the
+    reference forms g . r with an error-free split in float32 and reduces it
+    modulo one cycle, which float64 does directly at these magnitudes)."""
+    r = np.asarray(pos_lab_m, np.float64)
+    G = np.asarray(g, np.float64).reshape(-1, 3)
+    t = r @ G.T
+    t -= np.rint(t)
+    return np.sum(np.asarray(E, np.complex128) * np.exp(2j * np.pi * t), axis=1)
