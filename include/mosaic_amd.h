@@ -513,6 +513,45 @@ int mosaic_sg_adam(float* w, const float* g, float* m, float* v, long long n, fl
 int mosaic_sg_ensemble_stats(const float* P, float* mean, float* spread, int members,
                              int frames, int npix, long long strideP, mosaic_stream_t stream);
 
+/* Weights of one ensemble packed for inference: W (members, K, N) float32
+ * -> per member [ceil(N/32)][ceil(K/16)][64][8] bf16 (2 B each, K and N
+ * zero-padded to multiples of 16 and 32), the matrix cores' operand order;
+ * `packed` holds members x that many elements. */
+int mosaic_sg_pack(int members, int K, int N, const float* W, long long strideW, void* packed,
+                   mosaic_stream_t stream);
+
+/* The surrogate network 11 -> 256 -> 512 -> 8x8x64 -> 16x16x64 -> 32x32x32
+ * -> 64x64x16 -> 128x128x1 (SiLU after every layer but the last; each
+ * decoder stage a 2x2 stride-2 transposed convolution), packed weights
+ * (mosaic_sg_pack of the (K, N) matrices 11x256, 256x512, 512x4096, 64x256,
+ * 64x128, 32x64, 16x4) and float32 biases (members, N or Cout). */
+typedef struct {
+    const void* W1;
+    const float* b1;
+    const void* W2;
+    const float* b2;
+    const void* W3;
+    const float* b3;
+    const void* D1;
+    const float* d1;
+    const void* D2;
+    const float* d2;
+    const void* D3;
+    const float* d3;
+    const void* D4;
+    const float* d4;
+} mosaic_sg_net;
+
+/* Bytes of device workspace mosaic_sg_infer needs (the second layer's
+ * activations in operand order). */
+int mosaic_sg_infer_workspace(int members, int batch, size_t* bytes);
+
+/* Forward pass of every member on batch inputs X (batch, 11) in two launches:
+ * out (members, batch, 128 * 128) float32, the same function as the
+ * layer-by-layer GEMMs (BF16 operands, FP32 accumulation). */
+int mosaic_sg_infer(int members, int batch, const float* X, const mosaic_sg_net* net,
+                    void* workspace, size_t workspace_bytes, float* out, mosaic_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,9 @@ EXPORTS = (
     "mosaic_sg_mse",
     "mosaic_sg_adam",
     "mosaic_sg_ensemble_stats",
+    "mosaic_sg_pack",
+    "mosaic_sg_infer_workspace",
+    "mosaic_sg_infer",
 )
 
 
@@ -99,6 +102,11 @@ class AngularFilter(ctypes.Structure):
                 ("transmission", ctypes.c_double), ("phase_shift", ctypes.c_double),
                 ("roll_deg", ctypes.c_double), ("mode_1d", ctypes.c_int),
                 ("elliptical", ctypes.c_int), ("rolloff", ctypes.c_int), ("order", ctypes.c_int)]
+
+
+class SgNet(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("W1", "b1", "W2", "b2", "W3", "b3", "D1", "d1",
+                                                "D2", "d2", "D3", "d3", "D4", "d4")]
 
 
 class ZonePlate(ctypes.Structure):
@@ -157,6 +165,9 @@ _SIGS = {
     "mosaic_sg_mse": [_vp, _vp, _vp, _vp, _ll, _ll, _i, _vp, _ll, _vp],
     "mosaic_sg_adam": [_vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _f, _vp],
     "mosaic_sg_ensemble_stats": [_vp, _vp, _vp, _i, _i, _i, _ll, _vp],
+    "mosaic_sg_pack": [_i, _i, _i, _vp, _ll, _vp, _vp],
+    "mosaic_sg_infer_workspace": [_i, _i, ctypes.POINTER(_sz)],
+    "mosaic_sg_infer": [_i, _i, _vp, ctypes.POINTER(SgNet), _vp, _sz, _vp, _vp],
 }
 
 

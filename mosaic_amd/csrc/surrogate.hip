@@ -10,16 +10,27 @@
 //   op(A)[m][k] = A[m*lda + k] (TA = 0) or A[k*lda + m] (TA = 1)
 //   op(B)[k][n] = B[k*ldb + n] (TB = 0) or B[n*ldb + k] (TB = 1)
 // A 64x64 output tile per 256-thread workgroup; each wave owns a 32x32
-// quadrant held in one v_mfma_f32_32x32x8_bf16 accumulator (16 FP32 per
-// lane); K advances 32 per LDS stage (4 MFMAs).  grid.z indexes the batch
+// quadrant held in one v_mfma_f32_32x32x16_bf16 accumulator (16 FP32 per
+// lane); K advances 32 per LDS stage (2 MFMAs).  grid.z indexes the batch
 // (ensemble member) with element strides sA, sB, sC.
+//
+// Inference (mosaic_sg_infer) runs the whole ensemble forward in two
+// launches on weights packed once into the matrix cores' operand order
+// (mosaic_sg_pack): the MLP's first two layers, then one workgroup per
+// (member, 8x8-level pixel, 32 frames) for the 512 -> 4096 layer's 64
+// columns of that pixel and the whole transposed-convolution tree below it.
+// Each 2x2 stride-2 transposed convolution maps one input pixel to its own
+// 2x2 block, so an 8x8-level pixel's 16x16 block of the 128x128 output
+// depends on that pixel's 64 channels alone, and every intermediate stays in
+// LDS.
 
 #include "common.h"
 
 namespace mosaic {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int GBM = 64, GBN = 64, GBK = 32;
@@ -40,8 +51,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
                                                    float beta, int splits, int kchunk) {
     // LDS tiles as bf16: As[m][k], Bs[n][k] (k contiguous, padded against
     // bank conflicts)
-    __shared__ unsigned short As[GBM][GBK + 4];
-    __shared__ unsigned short Bs[GBN][GBK + 4];
+    __shared__ __attribute__((aligned(16))) unsigned short As[GBM][GBK + 8];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[GBN][GBK + 8];
     // grid.z = batch x K-splits; a split adds its partial product atomically
     const int z = blockIdx.z / splits, split = blockIdx.z % splits;
     const int kb = split * kchunk, ke = min(K, kb + kchunk);
@@ -77,16 +88,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict__ A,
         }
         __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < GBK; ks += 8) {
-            // lane l: A row wm + l%32, k = ks + 4*(l/32) .. +3; B column wn + l%32
-            const int r = lane & 31, kq = ks + 4 * (lane >> 5);
-            bf16x4 a, b;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                a[i] = static_cast<short>(As[wm + r][kq + i]);
-                b[i] = static_cast<short>(Bs[wn + r][kq + i]);
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        for (int ks = 0; ks < GBK; ks += 16) {
+            // lane l: A row wm + l%32, k = ks + 8*(l/32) .. +7; B column wn + l%32
+            const int r = lane & 31, kq = ks + 8 * (lane >> 5);
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + r][kq]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + r][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -249,6 +256,251 @@ __global__ void k_ens_stats(const float* __restrict__ P, float* __restrict__ mea
     if (threadIdx.x == 0) spread[f] = sqrtf(red[0] / npix);
 }
 
+// ------------------------------------------------------------ inference
+// Operand order of v_mfma_f32_32x32x16_bf16: lane l holds A row l%32 (B
+// column l%32) at k = 8*(l/32) .. +7; accumulator element j of lane l is
+// row 8*(j/4) + 4*(l/32) + j%4, column l%32.  A packed weight matrix W
+// (K x N) is stored per member as [N/32][K/16][64 lanes][8] bf16, K and N
+// zero-padded to multiples of 16 and 32, so that a wave's B operand for one
+// instruction is one contiguous 1 KB load.
+constexpr int SG_FB = 32;           // frames per inference block
+constexpr int SG_IN = 11;           // network inputs
+constexpr int SG_H1 = 256, SG_H2 = 512, SG_H3 = 4096;
+
+__device__ __forceinline__ bf16x8 ld_frag(const unsigned short* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int acc_row(int j, int lane) {
+    return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+}
+
+__global__ void k_sg_pack(const float* __restrict__ W, int K, int N, int Kp, int Np,
+                          long long sW, unsigned short* __restrict__ out) {
+    const long long per = static_cast<long long>(Kp) * Np;
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    const int e = blockIdx.y;
+    if (i >= per) return;
+    const int ii = static_cast<int>(i & 7), lane = static_cast<int>((i >> 3) & 63);
+    const long long t = i >> 9;
+    const int nks = Kp / 16;
+    const int ks = static_cast<int>(t % nks), nb = static_cast<int>(t / nks);
+    const int k = ks * 16 + 8 * (lane >> 5) + ii, n = nb * 32 + (lane & 31);
+    const float v = (k < K && n < N) ? W[e * sW + static_cast<long long>(k) * N + n] : 0.f;
+    out[e * per + i] = to_bf16(v);
+}
+
+struct SgNet {
+    const unsigned short *W1, *W2, *W3, *D1, *D2, *D3, *D4;   // packed, per member
+    const float *b1, *b2, *b3, *d1, *d2, *d3, *d4;            // biases (E, n)
+};
+
+// Layers 1-2 (11 -> 256 -> 512, SiLU) of members blockIdx.z for frames
+// 32*blockIdx.x .. +31; blockIdx.y picks 128 of layer 2's 512 columns.  The
+// output goes to h2 in A-operand order, [E][blocks][32 k-steps][64][8].
+__global__ __launch_bounds__(256) void k_sg_mlp12(const float* __restrict__ X, int B, SgNet net,
+                                                  unsigned short* __restrict__ h2, int nfb) {
+    __shared__ __attribute__((aligned(16))) unsigned short X1s[SG_FB][SG_H1 + 8];
+    const int fb = blockIdx.x, split = blockIdx.y, e = blockIdx.z;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    u16x8 xa;
+    {
+        const int f = fb * SG_FB + (lane & 31);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = 8 * (lane >> 5) + i;
+            xa[i] = to_bf16((f < B && k < SG_IN) ? X[static_cast<long long>(f) * SG_IN + k] : 0.f);
+        }
+    }
+    const bf16x8 xf = __builtin_bit_cast(bf16x8, xa);
+    const unsigned short* W1 = net.W1 + static_cast<long long>(e) * 16 * SG_H1;
+#pragma unroll
+    for (int c2 = 0; c2 < 2; ++c2) {
+        const int nb = wave * 2 + c2;
+        f32x16 acc = {};
+        acc = mfma16(xf, ld_frag(W1 + nb * 512 + lane * 8), acc);
+        const int n = nb * 32 + (lane & 31);
+        const float bias = net.b1[e * SG_H1 + n];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) X1s[acc_row(j, lane)][n] = to_bf16(silu(acc[j] + bias));
+    }
+    __syncthreads();
+    const int nb = split * 4 + wave;
+    const unsigned short* W2 = net.W2 + static_cast<long long>(e) * SG_H1 * SG_H2 + nb * 16 * 512;
+    f32x16 acc = {};
+#pragma unroll
+    for (int ks = 0; ks < SG_H1 / 16; ++ks) {
+        const bf16x8 a = ld_frag(&X1s[lane & 31][ks * 16 + 8 * (lane >> 5)]);
+        acc = mfma16(a, ld_frag(W2 + ks * 512 + lane * 8), acc);
+    }
+    const int n = nb * 32 + (lane & 31);
+    const float bias = net.b2[e * SG_H2 + n];
+    unsigned short* dst = h2 + static_cast<long long>(e * nfb + fb) * (SG_H2 / 16) * 512;
+    const int ks2 = n >> 4, half = (n >> 3) & 1, ii = n & 7;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        dst[(ks2 * 64 + acc_row(j, lane) + 32 * half) * 8 + ii] = to_bf16(silu(acc[j] + bias));
+}
+
+// Layer 3 for 8x8-level pixel q = blockIdx.x (its 64 channels) and the
+// decoder tree under it, for frames 32*blockIdx.y .. +31 of member
+// blockIdx.z.  Stage rows: stage 1 (f), 2 (f, s1), 3 (f, s1, s2), 4 (f, s1,
+// s2, s3), with s = 2 dy + dx the sub-pixel a transposed convolution adds;
+// the output pixel is y = 16 (q / 8) + 8 dy1 + 4 dy2 + 2 dy3 + dy4, x alike.
+constexpr int SG_X3_LD = 40, SG_X12_LD = 72, SG_X4_LD = 24;
+constexpr int SG_X3_BYTES = 512 * SG_X3_LD * 2;                       // 40960
+constexpr int SG_X4_BYTES = 128 * SG_X4_LD * 2;                       // 6144 per wave
+constexpr int SG_UNION_BYTES = 4 * SG_X4_BYTES;                       // >= X1s + X2s
+constexpr int SG_OUT_BYTES = 2 * 256 * 4;                             // 2 frames x 16 x 16
+static_assert((32 + 128) * SG_X12_LD * 2 <= SG_UNION_BYTES, "X1s + X2s fit the union");
+
+__global__ __launch_bounds__(256) void k_sg_decode(const unsigned short* __restrict__ h2, int nfb,
+                                                   int B, SgNet net, float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16)))
+        unsigned char smem[SG_X3_BYTES + SG_UNION_BYTES + 4 * SG_OUT_BYTES];
+    const int q = blockIdx.x, fb = blockIdx.y, e = blockIdx.z;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto X3 = reinterpret_cast<unsigned short(*)[SG_X3_LD]>(smem);
+    auto X1 = reinterpret_cast<unsigned short(*)[SG_X12_LD]>(smem + SG_X3_BYTES);
+    auto X2 = reinterpret_cast<unsigned short(*)[SG_X12_LD]>(smem + SG_X3_BYTES +
+                                                            32 * SG_X12_LD * 2);
+    auto X4 = reinterpret_cast<unsigned short(*)[SG_X4_LD]>(smem + SG_X3_BYTES +
+                                                           wave * SG_X4_BYTES);
+    float* Ow = reinterpret_cast<float*>(smem + SG_X3_BYTES + SG_UNION_BYTES +
+                                         wave * SG_OUT_BYTES);
+    float* red = reinterpret_cast<float*>(smem);   // aliases X3 (not yet written)
+
+    // layer 3: columns 64 q .. +63 (two blocks of 32), K = 512 split over wave pairs
+    {
+        const int cb = wave & 1, kh = wave >> 1;
+        const unsigned short* A = h2 + static_cast<long long>(e * nfb + fb) * 32 * 512;
+        const unsigned short* Bw = net.W3 + static_cast<long long>(e) * SG_H2 * SG_H3 +
+                                   static_cast<long long>(2 * q + cb) * 32 * 512;
+        f32x16 acc = {};
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int ks = kh * 16 + s;
+            acc = mfma16(ld_frag(A + ks * 512 + lane * 8), ld_frag(Bw + ks * 512 + lane * 8), acc);
+        }
+        if (kh == 1)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) red[(cb * 16 + j) * 64 + lane] = acc[j];
+        __syncthreads();
+        if (kh == 0) {
+            const int c = cb * 32 + (lane & 31);
+            const float bias = net.b3[e * SG_H3 + 64 * q + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                X1[acc_row(j, lane)][c] =
+                    to_bf16(silu(acc[j] + red[(cb * 16 + j) * 64 + lane] + bias));
+        }
+        __syncthreads();
+    }
+    // stage 1: 64 -> 4 x 64 (rows f)
+    {
+        bf16x8 a[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) a[ks] = ld_frag(&X1[lane & 31][ks * 16 + 8 * (lane >> 5)]);
+        const unsigned short* Bw = net.D1 + static_cast<long long>(e) * 64 * 256;
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2) {
+            const int nb = wave * 2 + c2;
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc = mfma16(a[ks], ld_frag(Bw + (nb * 4 + ks) * 512 + lane * 8), acc);
+            const int n = nb * 32 + (lane & 31), s1 = n >> 6, c = n & 63;
+            const float bias = net.d1[e * 64 + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) X2[acc_row(j, lane) * 4 + s1][c] = to_bf16(silu(acc[j] + bias));
+        }
+        __syncthreads();
+    }
+    // stage 2: 64 -> 4 x 32 (rows (f, s1)); wave = row block
+    {
+        bf16x8 a[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            a[ks] = ld_frag(&X2[32 * wave + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        const unsigned short* Bw = net.D2 + static_cast<long long>(e) * 64 * 128;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc = mfma16(a[ks], ld_frag(Bw + (nb * 4 + ks) * 512 + lane * 8), acc);
+            const int n = nb * 32 + (lane & 31), s2 = n >> 5, c = n & 31;
+            const float bias = net.d2[e * 32 + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                X3[(32 * wave + acc_row(j, lane)) * 4 + s2][c] = to_bf16(silu(acc[j] + bias));
+        }
+        __syncthreads();
+    }
+    // stages 3 and 4, one block of 32 stage-3 rows (two frames) at a time
+    const unsigned short* B3 = net.D3 + static_cast<long long>(e) * 32 * 64;
+    bf16x8 w3[2][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) w3[cb][ks] = ld_frag(B3 + (cb * 2 + ks) * 512 + lane * 8);
+    const bf16x8 w4 = ld_frag(net.D4 + static_cast<long long>(e) * 16 * 32 + lane * 8);
+    const float b4 = net.d4[e];
+    const int Y0 = 16 * (q >> 3), X0 = 16 * (q & 7);
+    for (int t = 0; t < 4; ++t) {
+        const int rb = wave * 4 + t;
+        bf16x8 a[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+            a[ks] = ld_frag(&X3[32 * rb + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) acc = mfma16(a[ks], w3[cb][ks], acc);
+            const int n = cb * 32 + (lane & 31), s3 = n >> 4, c = n & 15;
+            const float bias = net.d3[e * 16 + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) X4[acc_row(j, lane) * 4 + s3][c] = to_bf16(silu(acc[j] + bias));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+            f32x16 acc = {};
+            acc = mfma16(ld_frag(&X4[32 * r4 + (lane & 31)][8 * (lane >> 5)]), w4, acc);
+            const int s4 = lane & 31;
+            if (s4 < 4) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int row4 = 32 * r4 + acc_row(j, lane);   // (r3, s3) in this block
+                    const int r3 = row4 >> 2, s3 = row4 & 3;
+                    const int R3 = 32 * rb + r3;                   // (f, s1, s2)
+                    const int s2 = R3 & 3, s1 = (R3 >> 2) & 3, fl = r3 >> 4;
+                    const int yl = 8 * (s1 >> 1) + 4 * (s2 >> 1) + 2 * (s3 >> 1) + (s4 >> 1);
+                    const int xl = 8 * (s1 & 1) + 4 * (s2 & 1) + 2 * (s3 & 1) + (s4 & 1);
+                    Ow[(fl * 16 + yl) * 16 + xl] = acc[j] + b4;
+                }
+            }
+        }
+        __syncthreads();
+        // two frames' 16 x 16 blocks: lane -> (frame, row, 8-float half row)
+        {
+            const int fl = lane >> 5, yl = (lane >> 1) & 15, xh = (lane & 1) * 8;
+            const int f = fb * SG_FB + 2 * rb + fl;
+            if (f < B) {
+                const float4* src = reinterpret_cast<const float4*>(Ow + (fl * 16 + yl) * 16 + xh);
+                float4* dst = reinterpret_cast<float4*>(
+                    out + (static_cast<long long>(e) * B + f) * 16384 + (Y0 + yl) * 128 + X0 + xh);
+                dst[0] = src[0];
+                dst[1] = src[1];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 int grid1(long long n) { return static_cast<int>((n + 255) / 256); }
 
 }  // namespace
@@ -376,6 +628,67 @@ extern "C" int mosaic_sg_ensemble_stats(const float* P, float* mean, float* spre
     if (!P || !mean || !spread || members < 1 || frames < 1 || npix < 1) return MOSAIC_ERR_ARGS;
     hipLaunchKernelGGL(k_ens_stats, dim3(frames), dim3(256), 0, static_cast<hipStream_t>(stream),
                        P, mean, spread, members, frames, npix, strideP);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_sg_pack(int members, int K, int N, const float* W, long long strideW,
+                              void* packed, mosaic_stream_t stream) {
+    if (members < 1 || K < 1 || N < 1 || !W || !packed) return MOSAIC_ERR_ARGS;
+    const int Kp = (K + 15) / 16 * 16, Np = (N + 31) / 32 * 32;
+    const long long per = static_cast<long long>(Kp) * Np;
+    hipLaunchKernelGGL(k_sg_pack, dim3(static_cast<unsigned>((per + 255) / 256), members),
+                       dim3(256), 0, static_cast<hipStream_t>(stream), W, K, N, Kp, Np, strideW,
+                       static_cast<unsigned short*>(packed));
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    return MOSAIC_OK;
+}
+
+static size_t sg_infer_bytes(int members, int batch) {
+    const size_t nfb = (static_cast<size_t>(batch) + SG_FB - 1) / SG_FB;
+    return static_cast<size_t>(members) * nfb * SG_FB * SG_H2 * sizeof(unsigned short);
+}
+
+extern "C" int mosaic_sg_infer_workspace(int members, int batch, size_t* bytes) {
+    if (members < 1 || batch < 0 || !bytes) return MOSAIC_ERR_ARGS;
+    *bytes = sg_infer_bytes(members, batch);
+    return MOSAIC_OK;
+}
+
+extern "C" int mosaic_sg_infer(int members, int batch, const float* X, const mosaic_sg_net* net,
+                               void* workspace, size_t workspace_bytes, float* out,
+                               mosaic_stream_t stream) {
+    if (members < 1 || batch < 0 || !net) return MOSAIC_ERR_ARGS;
+    if (batch == 0) return MOSAIC_OK;
+    if (!X || !out || !workspace) return MOSAIC_ERR_ARGS;
+    if (workspace_bytes < sg_infer_bytes(members, batch)) return MOSAIC_ERR_WORKSPACE;
+    const void* const ptrs[] = {net->W1, net->b1, net->W2, net->b2, net->W3, net->b3, net->D1,
+                                net->d1, net->D2, net->d2, net->D3, net->d3, net->D4, net->d4};
+    for (const void* p : ptrs)
+        if (!p) return MOSAIC_ERR_ARGS;
+    SgNet n;
+    n.W1 = static_cast<const unsigned short*>(net->W1);
+    n.W2 = static_cast<const unsigned short*>(net->W2);
+    n.W3 = static_cast<const unsigned short*>(net->W3);
+    n.D1 = static_cast<const unsigned short*>(net->D1);
+    n.D2 = static_cast<const unsigned short*>(net->D2);
+    n.D3 = static_cast<const unsigned short*>(net->D3);
+    n.D4 = static_cast<const unsigned short*>(net->D4);
+    n.b1 = net->b1;
+    n.b2 = net->b2;
+    n.b3 = net->b3;
+    n.d1 = net->d1;
+    n.d2 = net->d2;
+    n.d3 = net->d3;
+    n.d4 = net->d4;
+    const int nfb = (batch + SG_FB - 1) / SG_FB;
+    if (nfb > 65535 || members > 65535) return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned short* h2 = static_cast<unsigned short*>(workspace);
+    hipLaunchKernelGGL(k_sg_mlp12, dim3(nfb, SG_H2 / 128, members), dim3(256), 0, st, X, batch, n,
+                       h2, nfb);
+    hipLaunchKernelGGL(k_sg_decode, dim3(64, nfb, members), dim3(256), 0, st, h2, nfb, batch, n,
+                       out);
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

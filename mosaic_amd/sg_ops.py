@@ -102,3 +102,40 @@ def ensemble_stats(P, mean, spread):
     members, frames, npix = P.shape
     check(_lib.lib().mosaic_sg_ensemble_stats(_p(P), _p(mean), _p(spread), members, frames, npix,
                                               frames * npix, _s()), "mosaic_sg_ensemble_stats")
+
+
+def pack(W):
+    """(E, K, N) float32 weights -> packed bf16 operand blocks (int16 tensor of
+    E x ceil(K/16)*16 x ceil(N/32)*32 elements) for ``infer``."""
+    _f32(W)
+    E, K, N = W.shape
+    Kp, Np = (K + 15) // 16 * 16, (N + 31) // 32 * 32
+    out = torch.empty(E * Kp * Np, dtype=torch.int16, device=W.device)
+    check(_lib.lib().mosaic_sg_pack(E, K, N, _p(W), K * N, _p(out), _s()), "mosaic_sg_pack")
+    return out
+
+
+def infer_workspace_bytes(members, batch):
+    n = ctypes.c_size_t(0)
+    check(_lib.lib().mosaic_sg_infer_workspace(members, batch, ctypes.byref(n)),
+          "mosaic_sg_infer_workspace")
+    return int(n.value)
+
+
+def infer(X, packed, biases, out, workspace):
+    """Fused forward of every member: X (B, 11), ``packed`` the 7 packed
+    weight tensors and ``biases`` the 7 (E, n) bias tensors in layer order,
+    out (E, B, 128*128)."""
+    _f32(X, out, *biases)
+    E, B = out.shape[0], out.shape[1]
+    if X.shape != (B, 11) or out.shape[2] != 128 * 128:
+        raise ValueError(f"infer shapes {tuple(X.shape)} {tuple(out.shape)}")
+    net = _lib.SgNet()
+    for name, t in zip(("W1", "W2", "W3", "D1", "D2", "D3", "D4"), packed):
+        setattr(net, name, int(t.data_ptr()))
+    for name, t in zip(("b1", "b2", "b3", "d1", "d2", "d3", "d4"), biases):
+        setattr(net, name, int(t.data_ptr()))
+    check(_lib.lib().mosaic_sg_infer(E, B, _p(X), ctypes.byref(net), _p(workspace),
+                                     workspace.numel() * workspace.element_size(), _p(out), _s()),
+          "mosaic_sg_infer")
+    return out

@@ -255,6 +255,31 @@ class SurrogateEnsemble:
                     for i, (h, ci, co) in enumerate(self.DEC)]
         self.step_count = 0
         self.norm = None     # (I0, mu, sd)
+        self._packed = None  # (step_count, packed weights) for the fused inference
+        self._infer_ws = None
+
+    FUSED_DEC = ((8, 64, 64), (16, 64, 32), (32, 32, 16), (64, 16, 1))
+
+    def fused(self):
+        """Whether the fused inference kernels cover this architecture."""
+        return self.MLP == (N_FEAT, 256, 512, 4096) and self.DEC == self.FUSED_DEC
+
+    def infer(self, X):
+        """X (B, 11) ``features`` on the device -> (E, B, 128*128) outputs in
+        two launches (mosaic_sg_infer); the weights are packed into the
+        matrix cores' operand order once per weight update."""
+        if not self.fused():
+            raise ValueError("the fused inference covers the 128 x 128 network only")
+        if self._packed is None or self._packed[0] != self.step_count:
+            self._packed = (self.step_count, [S.pack(L.W) for L in self.layers])
+        B = int(X.shape[0])
+        nb = S.infer_workspace_bytes(self.E, B)
+        if self._infer_ws is None or self._infer_ws.numel() < nb:
+            self._infer_ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        out = torch.empty((self.E, B, self.out_hw * self.out_hw), dtype=torch.float32,
+                          device=self.device)
+        return S.infer(X.contiguous(), self._packed[1], [L.b for L in self.layers], out,
+                       self._infer_ws)
 
     @property
     def layers(self):
@@ -400,8 +425,9 @@ class SurrogateEnsemble:
         relative L2 error)."""
         Xall = torch.as_tensor(features(P), device=self.device)
         means, spreads = [], []
+        run = self.infer if self.fused() else self.forward
         for i in range(0, Xall.shape[0], chunk):
-            out = self.forward(Xall[i:i + chunk].contiguous())
+            out = run(Xall[i:i + chunk].contiguous())
             B = out.shape[1]
             mean = torch.empty((B, out.shape[2]), dtype=torch.float32, device=self.device)
             spread = torch.empty(B, dtype=torch.float32, device=self.device)
@@ -417,8 +443,9 @@ class SurrogateEnsemble:
         """Each member's intensity (E, B, npix) for (B, 9) defect parameters."""
         Xall = torch.as_tensor(features(P), device=self.device)
         outs = []
+        run = self.infer if self.fused() else self.forward
         for i in range(0, Xall.shape[0], chunk):
-            outs.append(self.intensity(self.forward(Xall[i:i + chunk].contiguous())))
+            outs.append(self.intensity(run(Xall[i:i + chunk].contiguous())))
         return torch.cat(outs, dim=1)
 
 

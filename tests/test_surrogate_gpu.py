@@ -38,15 +38,38 @@ def test_defect_field_vs_numpy(device):
         assert bad.mean() < 1e-3
 
 
-def torch_reference(model, X):
-    """Same network in torch autograd (float32, GEMM operands rounded to bf16)."""
-    def r(t):
-        return t.to(torch.bfloat16).to(torch.float32)
+def _r(t):
+    return t.to(torch.bfloat16).to(t.dtype)
+
+
+class _Bf16MatMul(torch.autograd.Function):
+    """A @ B with every GEMM operand rounded to bfloat16, forward and
+    backward, as the kernels stage them (products accumulate in float32)."""
+
+    @staticmethod
+    def forward(ctx, A, B):
+        ctx.save_for_backward(A, B)
+        return torch.matmul(_r(A), _r(B))
+
+    @staticmethod
+    def backward(ctx, g):
+        A, B = ctx.saved_tensors
+        rg = _r(g)
+        return (torch.matmul(rg, _r(B).transpose(-1, -2).contiguous()),
+                torch.matmul(_r(A).transpose(-1, -2).contiguous(), rg))
+
+
+def torch_reference(model, X, dtype=torch.float32):
+    """Same network in torch autograd, with the operands of every GEMM
+    (forward and both backward products) rounded to bf16 and the rest in
+    ``dtype``."""
+    mm = _Bf16MatMul.apply
+    X = X.to(dtype)
 
     params = []
     for L in model.layers:
-        W = L.W.clone().requires_grad_(True)
-        b = L.b.clone().requires_grad_(True)
+        W = L.W.to(dtype).clone().requires_grad_(True)
+        b = L.b.to(dtype).clone().requires_grad_(True)
         params += [W, b]
     E = model.E
     B = X.shape[0]
@@ -55,13 +78,13 @@ def torch_reference(model, X):
     for L in model.mlp:
         W, b = params[i], params[i + 1]
         i += 2
-        z = torch.matmul(r(h), r(W)) + b[:, None, :]
+        z = mm(h.contiguous(), W) + b[:, None, :]
         h = torch.nn.functional.silu(z)
     img = h.reshape(E, B, 8, 8, 64)
     for k, (L, (H, ci, co)) in enumerate(zip(model.dec, model.DEC)):
         W, b = params[i], params[i + 1]
         i += 2
-        g = torch.matmul(r(img.reshape(E, -1, ci)), r(W)) + b.repeat(1, 4)[:, None, :]
+        g = mm(img.reshape(E, -1, ci).contiguous(), W) + b.repeat(1, 4)[:, None, :]
         if L.act:
             g = torch.nn.functional.silu(g)
         g = g.reshape(E, B, H, H, 2, 2, co).permute(0, 1, 2, 4, 3, 5, 6)
@@ -70,27 +93,73 @@ def torch_reference(model, X):
 
 
 def test_ensemble_forward_backward_vs_torch(device):
+    """The kernels' forward and parameter gradients against torch autograd
+    with the same bf16 operand rounding.  Two float32 evaluations of this
+    network differ by more than float32 rounding: a summation-order change
+    of ~1e-6 moves ~3e-4 of the activations to the other bf16 neighbour
+    (2^-8 off each), and that compounds through the layers (most for the
+    first layer's gradient).  So both the kernels and torch's float32
+    evaluation are compared with a float64 evaluation of the same rounded
+    network, and the kernels must sit within 3x torch's own float32 spread
+    (plus 2e-5), and within 5e-4 (forward) and 3e-3 (gradients) outright."""
     model = SG.SurrogateEnsemble(members=2, seed=3, device=device)
     rng = np.random.default_rng(0)
     X = torch.as_tensor(rng.uniform(-1, 1, (6, SG.N_FEAT)).astype(np.float32), device=device)
     T = torch.as_tensor(rng.standard_normal((6, 128 * 128)).astype(np.float32), device=device)
     out = model.forward(X)
-    ref, params = torch_reference(model, X)
-    e_fwd = ((out - ref).abs().max() / ref.abs().max()).item()
     dP = torch.empty_like(out)
     loss = torch.zeros(2, device=device)
     from mosaic_amd import sg_ops as S
     S.mse(out, T.reshape(-1).contiguous(), dP, loss)
     model.backward(dP)
-    lref = ((ref - T[None]) ** 2).mean(dim=(1, 2)).sum()
-    lref.backward()
-    errs = []
-    for L, (W, b) in zip(model.layers, zip(params[0::2], params[1::2])):
-        for mine, theirs in ((L.grads[0], W.grad), (L.grads[1], b.grad)):
-            errs.append(((mine - theirs).norm() / theirs.norm()).item())
-    print(f"[parity] surrogate forward rel={e_fwd:.2e}, grad rel max={max(errs):.2e}")
-    assert e_fwd < 2e-2
-    assert max(errs) < 5e-2
+    refs = {}
+    for dt in (torch.float32, torch.float64):
+        ref, params = torch_reference(model, X, dt)
+        ((ref - T[None].to(dt)) ** 2).mean(dim=(1, 2)).sum().backward()
+        refs[dt] = (ref.detach(), [p.grad for p in params])
+
+    def rel(a, b):
+        return ((a.double() - b.double()).norm() / b.double().norm()).item()
+
+    o64, g64 = refs[torch.float64]
+    o32, g32 = refs[torch.float32]
+    mine = [g for L in model.layers for g in L.grads]
+    e_fwd, s_fwd = rel(out, o64), rel(o32, o64)
+    e_g = np.array([rel(m, t) for m, t in zip(mine, g64)])
+    s_g = np.array([rel(m, t) for m, t in zip(g32, g64)])
+    print(f"[parity] surrogate forward: kernels {e_fwd:.2e}, torch float32 {s_fwd:.2e} "
+          f"(vs float64)", flush=True)
+    print(f"[parity] gradients (W, b by layer) kernels {np.array2string(e_g, precision=2)}",
+          flush=True)
+    print(f"[parity] gradients torch float32    {np.array2string(s_g, precision=2)}",
+          flush=True)
+    assert e_fwd <= 3 * s_fwd + 2e-5 and e_fwd < 5e-4
+    assert np.all(e_g <= 3 * s_g + 2e-5) and e_g.max() < 3e-3
+
+
+@pytest.mark.parametrize("members,batch", [(5, 64), (2, 45), (1, 1)])
+def test_fused_inference_matches_layers(device, members, batch):
+    """mosaic_sg_infer (two launches, packed weights) against the
+    layer-by-layer GEMM forward, on trained-looking weights (random) and
+    after an optimizer step (the packed weights are refreshed)."""
+    model = SG.SurrogateEnsemble(members=members, seed=5, device=device)
+    rng = np.random.default_rng(batch)
+    X = torch.as_tensor(rng.uniform(-1, 1, (batch, SG.N_FEAT)).astype(np.float32),
+                        device=device)
+    for L in model.layers:                       # non-zero biases
+        L.b.copy_(torch.as_tensor(rng.standard_normal(tuple(L.b.shape)).astype(np.float32)
+                                  * 0.1, device=device))
+    ref = model.forward(X)
+    got = model.infer(X)
+    e = ((got - ref).norm() / ref.norm()).item()
+    print(f"[parity] fused inference E={members} B={batch}: rel {e:.2e}", flush=True)
+    assert got.shape == ref.shape and e < 5e-4
+    for L in model.layers:                       # one Adam step with fake gradients
+        for g in L.grads:
+            g.copy_(torch.randn_like(g) * 1e-2)
+    model.adam_step(1e-3)
+    e2 = ((model.infer(X) - model.forward(X)).norm() / model.forward(X).norm()).item()
+    assert e2 < 5e-4
 
 
 def test_short_training_and_gate(device):
