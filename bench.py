@@ -468,13 +468,16 @@ def bench_adjoint(steps):
             "rel_err_sample": "20 random atoms, long-double reference gradient"}
 
 
-def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
+def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, judge_pixels=64,
+                    threshold=1e-2):
     """Learned rocking scan: ensemble trained on GPU-generated images of a
     200k-atom FCC Au crystal (37^3 cells) with a random dislocation, strain
     and stacking fault; 64-angle scans anchored on plain frames and judged
-    by the plain sum.  Beside it: the network alone, and the anchors alone
-    (linear interpolation between the same two plain frames; quadratic
-    interpolation through three), i.e. the method without the network."""
+    by exact probes of every other frame.  Beside it: the network alone, the
+    anchors alone (linear interpolation between the same two plain frames;
+    quadratic interpolation through three), i.e. the method without the
+    network, the fused inference's time, and the floor that the anchors and
+    the probe set on any judged scan."""
     import torch
     from mosaic_amd import surrogate as SG
     t0 = time.perf_counter()
@@ -493,12 +496,11 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
     Ipv, _ = model.predict(Pv)
     errv = SG.rel_l2(Ipv, Iv).cpu().numpy()
     angles = np.linspace(-0.05, 0.05, 64)
-    tau = SG.calibrate_threshold(model, gen, angles, n_calib, seed=seed + 3)
-    print(f"[surrogate] gate threshold {tau:.4g}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed + 7)
     scans = []
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    scan = SG.SurrogateScan(model, gen, tau)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(10)]
+    scan = SG.SurrogateScan(model, gen, threshold, judge_pixels=judge_pixels, seed=seed)
+    inner = np.arange(1, 63)
     for sidx in range(n_scans):
         p = SG.sample_params(1, rng)[0]
         gen.intensities(p, angles[:2])          # warm
@@ -507,13 +509,33 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
         Ip = gen.intensities(p, angles)
         ev[1].record()
         ev[2].record()
-        Is, spread, bad = scan.run(p, angles)
+        Is, est, bad = scan.run(p, angles)
         ev[3].record()
-        ev[3].synchronize()
+        # the parts: fused inference, anchors, probes
+        P64 = np.column_stack([np.repeat(p[None, :8], 64, 0), angles])
+        ev[4].record()
+        model.member_intensities(P64)
+        ev[5].record()
+        gen.intensities(p, angles[[0, 63]])
+        ev[6].record()
+        pix = torch.randint(0, Ip.shape[1], (judge_pixels,), device=Ip.device)
+        ev[7].record()
+        gen.probe(p, angles[inner], pix)
+        ev[8].record()
+        ev[8].synchronize()
         t_plain = ev[0].elapsed_time(ev[1])
         t_sg = ev[2].elapsed_time(ev[3])
         err = SG.rel_l2(Is, Ip).cpu().numpy()
-        P64 = np.column_stack([np.repeat(p[None, :8], 64, 0), angles])
+        est_c = est.cpu().numpy()
+        acc_mask = np.ones(64, bool)
+        acc_mask[list(bad)] = False
+        acc_mask[[0, 63]] = False
+        # the judge's own quality on this scan: the unjudged frames' true error
+        # against its estimate (before any fallback)
+        W = torch.as_tensor(SG.anchor_weights(angles, angles[[0, 63]]), device=Ip.device)
+        Im = model.member_intensities(P64)
+        pre = (Im + torch.matmul(W, Ip[[0, 63]][None] - Im[:, [0, 63]])).mean(0).clamp_min(0)
+        pre_err = SG.rel_l2(pre, Ip).cpu().numpy()[inner]
         raw = SG.rel_l2(model.predict(P64)[0], Ip).cpu().numpy()
         # the anchors without the network
         W = torch.as_tensor(SG.anchor_weights(angles, angles[[0, 63]]), device=Ip.device)
@@ -525,7 +547,14 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
                          Ip).cpu().numpy()
         scans.append({"plain_ms": t_plain, "surrogate_ms": t_sg,
                       "speedup": t_plain / t_sg, "fallback_frames": int(len(bad)),
+                      "inference_ms": ev[4].elapsed_time(ev[5]),
+                      "anchors_ms": ev[5].elapsed_time(ev[6]),
+                      "probe_ms": ev[7].elapsed_time(ev[8]),
                       "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err)),
+                      "accepted_max_rel_err": float(err[acc_mask].max()) if acc_mask.any()
+                      else 0.0,
+                      "judge_est_vs_true_corr": float(np.corrcoef(est_c[inner], pre_err)[0, 1]),
+                      "judge_est_over_true_median": float(np.median(est_c[inner] / pre_err)),
                       "network_alone_median_rel_err": float(np.median(raw)),
                       "network_alone_max_rel_err": float(raw.max()),
                       "anchors_linear_median_rel_err": float(np.median(lin)),
@@ -539,7 +568,8 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
             "images": n_images, "members": members, "epochs": epochs,
             "params_per_member": model.n_params(), "target": model.target,
             "gen_s": t_gen, "train_s": t_train,
-            "final_loss": [float(x) for x in hist[-1]], "gate_threshold": tau,
+            "final_loss": [float(x) for x in hist[-1]], "judge_threshold": threshold,
+            "judge_pixels": judge_pixels,
             "val_median_rel_err": float(np.median(errv)),
             "val_frac_below_1e-2": float(np.mean(errv <= 1e-2)),
             "plain_frames_per_s": 64 / (np.mean([x["plain_ms"] for x in scans]) * 1e-3),
@@ -547,6 +577,12 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, n_calib=8):
             "speedup": float(np.mean([x["plain_ms"] for x in scans]) /
                              np.mean([x["surrogate_ms"] for x in scans])),
             "scan_max_rel_err": agg("max_rel_err", np.max),
+            "accepted_max_rel_err": agg("accepted_max_rel_err", np.max),
+            "inference_ms": agg("inference_ms"), "anchors_ms": agg("anchors_ms"),
+            "probe_ms": agg("probe_ms"),
+            "floor_speedup": float(np.mean([x["plain_ms"] for x in scans]) /
+                                   (agg("anchors_ms") + agg("probe_ms") + agg("inference_ms"))),
+            "judge_est_vs_true_corr": agg("judge_est_vs_true_corr"),
             "scan_median_rel_err": agg("median_rel_err"),
             "fallback_frames_total": int(sum(x["fallback_frames"] for x in scans)),
             "network_alone_median_rel_err": agg("network_alone_median_rel_err"),

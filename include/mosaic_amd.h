@@ -51,7 +51,7 @@ typedef struct {
     int apply_decay;        /* scale by R0 / r */
     int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence,
                                3 matrix-core patches, 4 moment expansion,
-                               5 exact per pair */
+                               5 exact per pair, 6 probe (few pixels) */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
     int analyser_kind;      /* 0 off, 1 top-hat, 2 Darwin roll-off (Beam.py:2350-2378);
@@ -161,16 +161,19 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  *     consecutive tiles as a polynomial of degree D = 5 or 6 in the pixel
  *     offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle phasors;
  *     units of more than one tile are used where their radius, measured by
- *     mosaic_kin_stage, keeps every error bound).
+ *     mosaic_kin_stage, keeps every error bound); and a probe form for a
+ *     few scattered pixels (6, requested only: one lane per (frame, pixel),
+ *     the recurrence's far-field model with the exact direction at each
+ *     pixel, one sincos per pair).
  * opts->path = 0 picks the moment expansion, else the matrix-core form, else
  * the recurrence, else the per-tile form, whenever their error bounds hold,
  * and the exact kernel otherwise; an analyser always takes the exact kernel.
- * A forced path (1-4) returns MOSAIC_ERR_UNSUPPORTED (nothing written) when
- * its bounds fail.
+ * A forced path (1-4, 6) returns MOSAIC_ERR_UNSUPPORTED (nothing written)
+ * when its bounds fail.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels),
- * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), plus 100 for
+ * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), 60 (probe), plus 100 for
  * the far-field forms with the atoms' second-order phase, or 50 + the
  * analyser kind (exact).  Stream-ordered, no host synchronization.
  */
@@ -548,9 +551,13 @@ int mosaic_sg_infer_workspace(int members, int batch, size_t* bytes);
 
 /* Forward pass of every member on batch inputs X (batch, 11) in two launches:
  * out (members, batch, 128 * 128) float32, the same function as the
- * layer-by-layer GEMMs (BF16 operands, FP32 accumulation). */
+ * layer-by-layer GEMMs (BF16 operands, FP32 accumulation).  `post` (host, 4
+ * floats, or NULL for the raw output) maps each output y to
+ * post[2] exp(post[0] y + post[1]) + post[3]: the log-intensity target's
+ * inverse, fused into the store. */
 int mosaic_sg_infer(int members, int batch, const float* X, const mosaic_sg_net* net,
-                    void* workspace, size_t workspace_bytes, float* out, mosaic_stream_t stream);
+                    const float* post /* host [4] or NULL */, void* workspace,
+                    size_t workspace_bytes, float* out, mosaic_stream_t stream);
 
 #ifdef __cplusplus
 }

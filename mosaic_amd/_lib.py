@@ -167,7 +167,7 @@ _SIGS = {
     "mosaic_sg_ensemble_stats": [_vp, _vp, _vp, _i, _i, _i, _ll, _vp],
     "mosaic_sg_pack": [_i, _i, _i, _vp, _ll, _vp, _vp],
     "mosaic_sg_infer_workspace": [_i, _i, ctypes.POINTER(_sz)],
-    "mosaic_sg_infer": [_i, _i, _vp, ctypes.POINTER(SgNet), _vp, _sz, _vp, _vp],
+    "mosaic_sg_infer": [_i, _i, _vp, ctypes.POINTER(SgNet), _vp, _vp, _sz, _vp, _vp],
 }
 
 

@@ -2002,6 +2002,136 @@ hipError_t launch_rec(const RecArgs& A, int n_slices, int n_frames, hipStream_t 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ probe sum
+// Far-field sum for a few pixels (sparse probes of a frame): one lane per
+// (frame, pixel) output, workgroups over those outputs and atom slices.  The
+// row kernel gives each lane a run of KP pixels and walks the slice's atoms
+// once per lane; with a few dozen pixels its grid is a handful of lanes per
+// workgroup.  Here every lane of a workgroup is a different output: the
+// workgroup loads each tile's atom records into LDS once (coalesced, the next
+// tile's load in flight during the current one), and every lane reads them
+// as broadcasts.  Per pair a lane does psi = v . sigma (v the exact direction
+// at its own pixel, no line model), one sincos and the amplitude accumulate.
+// Per (tile, output) the origin phase and factor are the row kernel's
+// (far-field expansion, per-pixel table); the bounds are the row kernel's
+// without its row line fit.
+struct ProbeArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float2* gtab;
+    double2* partial;
+    int n_tiles;
+    int tiles_per_slice;
+    int n_slices;
+    int npix;
+    int n_frames;
+    double K;         // 1 / lambda
+};
+
+static_assert(TILE == SUM_THREADS, "one atom record per thread when a tile is loaded");
+
+template <bool SECOND>
+__global__ __launch_bounds__(SUM_THREADS) void k_probe(const ProbeArgs A) {
+    __shared__ float4 s_pos[2][TILE];     // sx, sy, sz, |s|^2
+    __shared__ float2 s_amp[2][TILE];     // |w|, arg(w) / 2 pi
+    const int slice = blockIdx.y;
+    const int t0 = slice * A.tiles_per_slice;
+    const int t1 = min(A.n_tiles, t0 + A.tiles_per_slice);
+    const int64_t o = blockIdx.x * static_cast<int64_t>(SUM_THREADS) + threadIdx.x;
+    const bool ok = o < static_cast<int64_t>(A.npix) * A.n_frames;
+    const int frame = ok ? static_cast<int>(o / A.npix) : 0;
+    const int p = ok ? static_cast<int>(o % A.npix) : 0;
+    const FrameRec& F = A.frames[frame];
+    const double R00 = F.R[0], R01 = F.R[1], R02 = F.R[2];
+    const double R10 = F.R[3], R11 = F.R[4], R12 = F.R[5];
+    const double R20 = F.R[6], R21 = F.R[7], R22 = F.R[8];
+    const float xs0 = static_cast<float>(F.xs[0]);
+    const float xs1 = static_cast<float>(F.xs[1]);
+    const float xs2 = static_cast<float>(F.xs[2]);
+    const double4 u = A.u0h[p];
+    const double th0 = A.theta0[p];
+    const double h2 = 2.0 * u.w;                                  // 1 / rho0
+    const float hh = SECOND ? static_cast<float>(u.w / A.K) : 0.f;  // 1 / (2 K rho0)
+    double acc_r = 0.0, acc_i = 0.0;
+    if (t0 >= t1) {
+        if (ok) A.partial[(static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix + p] =
+                    make_double2(0.0, 0.0);
+        return;
+    }
+    // records past a tile's count stay zero (|w| = 0), so every lane can
+    // walk the full tile without a count-dependent loop bound
+    auto fetch = [&](int t, float4& ps, float2& am) {
+        const AtomRec r = A.atoms[static_cast<int64_t>(t) * TILE + threadIdx.x];
+        ps = make_float4(r.sx, r.sy, r.sz, r.s2);
+        am = make_float2(r.mag, r.ph);
+    };
+    float4 nps;
+    float2 nam;
+    fetch(t0, nps, nam);
+    s_pos[0][threadIdx.x] = nps;
+    s_amp[0][threadIdx.x] = nam;
+    __syncthreads();
+    for (int t = t0; t < t1; ++t) {
+        const int cur = (t - t0) & 1;
+        if (t + 1 < t1) fetch(t + 1, nps, nam);          // in flight through this tile
+        const TileRec tr = A.tiles[t];
+        const double Ox = fma(R00, tr.ox, fma(R01, tr.oy, fma(R02, tr.oz, F.T[0])));
+        const double Oy = fma(R10, tr.ox, fma(R11, tr.oy, fma(R12, tr.oz, F.T[1])));
+        const double Oz = fma(R20, tr.ox, fma(R21, tr.oy, fma(R22, tr.oz, F.T[2])));
+        const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+        const double uo = u.x * Ox + u.y * Oy + u.z * Oz;
+        // x_hat minus the direction from the tile origin to the pixel, to
+        // first order in |O| / rho0, in the sample frame
+        const double wx = 1.0 - u.x + (Ox - uo * u.x) * h2;
+        const double wy = -u.y + (Oy - uo * u.y) * h2;
+        const double wz = -u.z + (Oz - uo * u.z) * h2;
+        const float vx = static_cast<float>(R00 * wx + R10 * wy + R20 * wz);
+        const float vy = static_cast<float>(R01 * wx + R11 * wy + R21 * wz);
+        const float vz = static_cast<float>(R02 * wx + R12 * wy + R22 * wz);
+        float S_r = 0.f, S_i = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < TILE; ++j) {
+            const float4 a = s_pos[cur][j];
+            const float2 w = s_amp[cur][j];
+            float psi = fmaf(vz, a.z, fmaf(vy, a.y, vx * a.x));
+            if (SECOND) {
+                const float us = fmaf(xs2, a.z, fmaf(xs1, a.y, xs0 * a.x)) - psi;
+                psi = fmaf(hh, fmaf(-us, us, a.w), psi);
+            }
+            const float r = __builtin_amdgcn_fractf(psi + w.y);
+            S_r = fmaf(w.x, __builtin_amdgcn_cosf(r), S_r);
+            S_i = fmaf(w.x, __builtin_amdgcn_sinf(r), S_i);
+        }
+        double th = th0 + A.K * (Ox - uo + (O2 - uo * uo) * u.w);
+        th -= rint(th);
+        const float thf = __builtin_amdgcn_fractf(static_cast<float>(th));
+        const float sn = __builtin_amdgcn_sinf(thf), cs = __builtin_amdgcn_cosf(thf);
+        const float2 g = A.gtab[static_cast<int64_t>(tr.species) * A.npix + p];
+        const float zr = g.x * cs - g.y * sn, zi = g.x * sn + g.y * cs;
+        acc_r += static_cast<double>(zr * S_r - zi * S_i);
+        acc_i += static_cast<double>(zr * S_i + zi * S_r);
+        if (t + 1 < t1) {
+            s_pos[cur ^ 1][threadIdx.x] = nps;           // the other buffer: last read a
+            s_amp[cur ^ 1][threadIdx.x] = nam;           // tile ago, before the barrier
+        }
+        __syncthreads();
+    }
+    if (ok)
+        A.partial[(static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix + p] =
+            make_double2(acc_r, acc_i);
+}
+
+template <bool SECOND>
+hipError_t launch_probe(const ProbeArgs& A, int n_slices, hipStream_t st) {
+    const int64_t outs = static_cast<int64_t>(A.npix) * A.n_frames;
+    dim3 grid(blocks_for(outs, SUM_THREADS), n_slices);
+    hipLaunchKernelGGL((k_probe<SECOND>), grid, dim3(SUM_THREADS), 0, st, A);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- matrix-core sum
 // Far-field sum on the matrix cores.  One wave owns a patch of MMA_P x MMA_P
 // pixels (a along the row, b down the columns, both offsets from the patch's
@@ -4075,7 +4205,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
         !pixel_geom)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 5 ||
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 6 ||
         opts->analyser_kind < 0 || opts->analyser_kind > 2)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -4144,7 +4274,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (path == 0)
         path = ana ? 5 : (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : (tile_ok ? 1 : 5))));
     if ((path == 1 && !tile_ok) || (path == 2 && kp == 0) || (path == 3 && !mma_ok) ||
-        (path == 4 && poly_d == 0) || (ana && path != 5))
+        (path == 4 && poly_d == 0) || (path == 6 && !far) || (ana && path != 5))
         return MOSAIC_ERR_UNSUPPORTED;
     // Moment sum over units of 2 or 4 tiles: the largest unit radius for
     // which every radius-dependent bound above still holds at the chosen
@@ -4247,6 +4377,23 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         else
             e = use_second ? launch_poly<6, true>(A, L.n_slices, n_frames, st)
                            : launch_poly<6, false>(A, L.n_slices, n_frames, st);
+    } else if (path == 6) {
+        ProbeArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.u0h = L.px.u0h;
+        A.theta0 = L.px.theta0;
+        A.gtab = L.px.gtab;
+        A.partial = L.partial;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.npix = npix;
+        A.n_frames = n_frames;
+        A.K = K;
+        e = use_second ? launch_probe<true>(A, L.n_slices, st)
+                       : launch_probe<false>(A, L.n_slices, st);
     } else if (path == 2) {
         RecArgs A;
         A.atoms = L.atoms;
@@ -4314,6 +4461,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     MOSAIC_HIP_CHECK(hipGetLastError());
     if (path_used)
         *path_used = path == 5   ? 50 + ana
+                     : path == 6 ? 60 + (use_second ? 100 : 0)
                      : path == 4 ? 40 + poly_d + (use_second ? 100 : 0)
                      : path == 3 ? 32 + (use_second ? 100 : 0)
                      : path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)

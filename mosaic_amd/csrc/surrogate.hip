@@ -261,8 +261,15 @@ __global__ void k_ens_stats(const float* __restrict__ P, float* __restrict__ mea
 // column l%32) at k = 8*(l/32) .. +7; accumulator element j of lane l is
 // row 8*(j/4) + 4*(l/32) + j%4, column l%32.  A packed weight matrix W
 // (K x N) is stored per member as [N/32][K/16][64 lanes][8] bf16, K and N
-// zero-padded to multiples of 16 and 32, so that a wave's B operand for one
+// zero-padded to multiples of 16 and 32, so that a wave's operand for one
 // instruction is one contiguous 1 KB load.
+//
+// The inference kernels form each layer transposed, Z^T = W^T X^T: the
+// packed weights are the A operand (row n = output channel) and the
+// activations, rows r with channels along k, the B operand.  A lane then
+// holds, for activation row l%32, the output channels 8a + 4(l/32) + 0..3
+// (a = 0..3): four runs of four consecutive channels, each stored to LDS as
+// one 8-byte write of four bf16.
 constexpr int SG_FB = 32;           // frames per inference block
 constexpr int SG_IN = 11;           // network inputs
 constexpr int SG_H1 = 256, SG_H2 = 512, SG_H3 = 4096;
@@ -273,8 +280,23 @@ __device__ __forceinline__ bf16x8 ld_frag(const unsigned short* p) {
 __device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ int acc_row(int j, int lane) {
-    return 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+// four floats -> four bf16 (round to nearest even), in memory order
+__device__ __forceinline__ uint2 bf16x4_bits(float a, float b, float c, float d) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 lo = {static_cast<__bf16>(a), static_cast<__bf16>(b)};
+    const bf16x2 hi = {static_cast<__bf16>(c), static_cast<__bf16>(d)};
+    return make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
+}
+// SiLU with the hardware reciprocal (1 ulp), whose result is rounded to bf16
+__device__ __forceinline__ float silu_rcp(float x) {
+    return x * __builtin_amdgcn_rcpf(1.f + __expf(-x));
+}
+// bias + SiLU of accumulator run a (elements 4a .. 4a+3), channels c0 .. c0+3
+// (bias in LDS)
+__device__ __forceinline__ uint2 act_run(const f32x16& acc, int a, const float* bias) {
+    const float4 b = *reinterpret_cast<const float4*>(bias);
+    return bf16x4_bits(silu_rcp(acc[4 * a] + b.x), silu_rcp(acc[4 * a + 1] + b.y),
+                       silu_rcp(acc[4 * a + 2] + b.z), silu_rcp(acc[4 * a + 3] + b.w));
 }
 
 __global__ void k_sg_pack(const float* __restrict__ W, int K, int N, int Kp, int Np,
@@ -297,207 +319,249 @@ struct SgNet {
     const float *b1, *b2, *b3, *d1, *d2, *d3, *d4;            // biases (E, n)
 };
 
-// Layers 1-2 (11 -> 256 -> 512, SiLU) of members blockIdx.z for frames
+// optional output map y -> c exp(a y + b) + d (the log-intensity target's inverse)
+struct SgPost {
+    float a, b, c, d;
+    int on;
+};
+
+// Layers 1-2 (11 -> 256 -> 512, SiLU) of member blockIdx.z for frames
 // 32*blockIdx.x .. +31; blockIdx.y picks 128 of layer 2's 512 columns.  The
-// output goes to h2 in A-operand order, [E][blocks][32 k-steps][64][8].
+// output goes to h2 in B-operand order, [E][blocks][32 k-steps][64][8].
 __global__ __launch_bounds__(256) void k_sg_mlp12(const float* __restrict__ X, int B, SgNet net,
                                                   unsigned short* __restrict__ h2, int nfb) {
     __shared__ __attribute__((aligned(16))) unsigned short X1s[SG_FB][SG_H1 + 8];
+    __shared__ __attribute__((aligned(16))) float bs1[SG_H1], bs2[128];
     const int fb = blockIdx.x, split = blockIdx.y, e = blockIdx.z;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    bs1[threadIdx.x] = net.b1[e * SG_H1 + threadIdx.x];
+    if (threadIdx.x < 128) bs2[threadIdx.x] = net.b2[e * SG_H2 + split * 128 + threadIdx.x];
+    __syncthreads();
+    // layer 2's weights first, in flight while layer 1 runs
+    const int nb2 = split * 4 + wave;
+    const unsigned short* W2 = net.W2 + static_cast<long long>(e) * SG_H1 * SG_H2 + nb2 * 16 * 512;
+    bf16x8 w2[SG_H1 / 16];
+#pragma unroll
+    for (int ks = 0; ks < SG_H1 / 16; ++ks) w2[ks] = ld_frag(W2 + ks * 512 + lane * 8);
+    const unsigned short* W1 = net.W1 + static_cast<long long>(e) * 16 * SG_H1;
+    bf16x8 w1[2];
+#pragma unroll
+    for (int c2 = 0; c2 < 2; ++c2) w1[c2] = ld_frag(W1 + (wave * 2 + c2) * 512 + lane * 8);
     u16x8 xa;
     {
-        const int f = fb * SG_FB + (lane & 31);
+        const int f = fb * SG_FB + r;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int k = 8 * (lane >> 5) + i;
+            const int k = 8 * h + i;
             xa[i] = to_bf16((f < B && k < SG_IN) ? X[static_cast<long long>(f) * SG_IN + k] : 0.f);
         }
     }
     const bf16x8 xf = __builtin_bit_cast(bf16x8, xa);
-    const unsigned short* W1 = net.W1 + static_cast<long long>(e) * 16 * SG_H1;
 #pragma unroll
     for (int c2 = 0; c2 < 2; ++c2) {
         const int nb = wave * 2 + c2;
         f32x16 acc = {};
-        acc = mfma16(xf, ld_frag(W1 + nb * 512 + lane * 8), acc);
-        const int n = nb * 32 + (lane & 31);
-        const float bias = net.b1[e * SG_H1 + n];
+        acc = mfma16(w1[c2], xf, acc);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) X1s[acc_row(j, lane)][n] = to_bf16(silu(acc[j] + bias));
+        for (int a = 0; a < 4; ++a) {
+            const int c0 = nb * 32 + 8 * a + 4 * h;
+            *reinterpret_cast<uint2*>(&X1s[r][c0]) = act_run(acc, a, bs1 + c0);
+        }
     }
     __syncthreads();
-    const int nb = split * 4 + wave;
-    const unsigned short* W2 = net.W2 + static_cast<long long>(e) * SG_H1 * SG_H2 + nb * 16 * 512;
     f32x16 acc = {};
 #pragma unroll
-    for (int ks = 0; ks < SG_H1 / 16; ++ks) {
-        const bf16x8 a = ld_frag(&X1s[lane & 31][ks * 16 + 8 * (lane >> 5)]);
-        acc = mfma16(a, ld_frag(W2 + ks * 512 + lane * 8), acc);
-    }
-    const int n = nb * 32 + (lane & 31);
-    const float bias = net.b2[e * SG_H2 + n];
+    for (int ks = 0; ks < SG_H1 / 16; ++ks)
+        acc = mfma16(w2[ks], ld_frag(&X1s[r][ks * 16 + 8 * h]), acc);
     unsigned short* dst = h2 + static_cast<long long>(e * nfb + fb) * (SG_H2 / 16) * 512;
-    const int ks2 = n >> 4, half = (n >> 3) & 1, ii = n & 7;
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-        dst[(ks2 * 64 + acc_row(j, lane) + 32 * half) * 8 + ii] = to_bf16(silu(acc[j] + bias));
+    for (int a = 0; a < 4; ++a) {
+        const int n = nb2 * 32 + 8 * a + 4 * h;
+        const int ks2 = n >> 4, half = (n >> 3) & 1, i0 = n & 7;
+        *reinterpret_cast<uint2*>(dst + (ks2 * 64 + r + 32 * half) * 8 + i0) =
+            act_run(acc, a, bs2 + (n - split * 128));
+    }
 }
 
 // Layer 3 for 8x8-level pixel q = blockIdx.x (its 64 channels) and the
 // decoder tree under it, for frames 32*blockIdx.y .. +31 of member
-// blockIdx.z.  Stage rows: stage 1 (f), 2 (f, s1), 3 (f, s1, s2), 4 (f, s1,
-// s2, s3), with s = 2 dy + dx the sub-pixel a transposed convolution adds;
+// blockIdx.z.  s = 2 dy + dx is the sub-pixel a transposed convolution adds;
 // the output pixel is y = 16 (q / 8) + 8 dy1 + 4 dy2 + 2 dy3 + dy4, x alike.
-constexpr int SG_X3_LD = 40, SG_X12_LD = 72, SG_X4_LD = 24;
-constexpr int SG_X3_BYTES = 512 * SG_X3_LD * 2;                       // 40960
-constexpr int SG_X4_BYTES = 128 * SG_X4_LD * 2;                       // 6144 per wave
-constexpr int SG_UNION_BYTES = 4 * SG_X4_BYTES;                       // >= X1s + X2s
-constexpr int SG_OUT_BYTES = 2 * 256 * 4;                             // 2 frames x 16 x 16
-static_assert((32 + 128) * SG_X12_LD * 2 <= SG_UNION_BYTES, "X1s + X2s fit the union");
+// Wave w computes stage 1's channels of s1 = w for all 32 frames, and from
+// there on works alone on its s1: per s2 one 32-row block of stage 2, its
+// stage 3 (two s3 at a time) and stage 4.  Stage rows are frames (row = lane
+// % 32), so neighbouring lanes store to neighbouring rows (2-way LDS bank
+// conflicts at most).  The per-wave buffers keep the workgroup at 46 KB of
+// LDS: three workgroups per CU.
+constexpr int SG_LD72 = 72, SG_LD40 = 40, SG_LD24 = 24;
+constexpr int SG_X1_BYTES = 32 * SG_LD72 * 2;                                 // 4608
+constexpr int SG_X2W = 32 * SG_LD72 * 2, SG_X3W = 32 * SG_LD40 * 2;           // 4608, 2560
+constexpr int SG_X4W = 64 * SG_LD24 * 2;                                      // 3072
+constexpr int SG_WAVE_BYTES = SG_X2W + SG_X3W + SG_X4W;                       // 10240
+static_assert(2 * 16 * 64 * 4 <= 4 * SG_WAVE_BYTES, "the layer-3 reduction fits");
 
-__global__ __launch_bounds__(256) void k_sg_decode(const unsigned short* __restrict__ h2, int nfb,
-                                                   int B, SgNet net, float* __restrict__ out) {
-    __shared__ __attribute__((aligned(16)))
-        unsigned char smem[SG_X3_BYTES + SG_UNION_BYTES + 4 * SG_OUT_BYTES];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_sg_decode(const unsigned short* __restrict__ h2, int nfb,
+                                                   int B, SgNet net, SgPost post,
+                                                   float* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SG_X1_BYTES + 4 * SG_WAVE_BYTES];
+    // biases of this pixel's layer-3 channels and of the decoder stages
+    __shared__ __attribute__((aligned(16))) float bs3[64], bsd1[64], bsd2[32], bsd3[16];
     const int q = blockIdx.x, fb = blockIdx.y, e = blockIdx.z;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    auto X3 = reinterpret_cast<unsigned short(*)[SG_X3_LD]>(smem);
-    auto X1 = reinterpret_cast<unsigned short(*)[SG_X12_LD]>(smem + SG_X3_BYTES);
-    auto X2 = reinterpret_cast<unsigned short(*)[SG_X12_LD]>(smem + SG_X3_BYTES +
-                                                            32 * SG_X12_LD * 2);
-    auto X4 = reinterpret_cast<unsigned short(*)[SG_X4_LD]>(smem + SG_X3_BYTES +
-                                                           wave * SG_X4_BYTES);
-    float* Ow = reinterpret_cast<float*>(smem + SG_X3_BYTES + SG_UNION_BYTES +
-                                         wave * SG_OUT_BYTES);
-    float* red = reinterpret_cast<float*>(smem);   // aliases X3 (not yet written)
-
-    // layer 3: columns 64 q .. +63 (two blocks of 32), K = 512 split over wave pairs
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     {
-        const int cb = wave & 1, kh = wave >> 1;
-        const unsigned short* A = h2 + static_cast<long long>(e * nfb + fb) * 32 * 512;
+        const int t = threadIdx.x;
+        if (t < 64) bs3[t] = net.b3[e * SG_H3 + 64 * q + t];
+        else if (t < 128) bsd1[t - 64] = net.d1[e * 64 + t - 64];
+        else if (t < 160) bsd2[t - 128] = net.d2[e * 32 + t - 128];
+        else if (t < 176) bsd3[t - 160] = net.d3[e * 16 + t - 160];
+    }
+    auto X1 = reinterpret_cast<unsigned short(*)[SG_LD72]>(smem);
+    unsigned char* wbuf = smem + SG_X1_BYTES + wave * SG_WAVE_BYTES;
+    auto X2 = reinterpret_cast<unsigned short(*)[SG_LD72]>(wbuf);
+    auto X3 = reinterpret_cast<unsigned short(*)[SG_LD40]>(wbuf + SG_X2W);
+    auto X4 = reinterpret_cast<unsigned short(*)[SG_LD24]>(wbuf + SG_X2W + SG_X3W);
+    float* red = reinterpret_cast<float*>(smem + SG_X1_BYTES);   // before stage 1 only
+
+    // layer 3: columns 64 q .. +63 (two blocks of 32), K = 512 split over wave
+    // pairs; every operand load issued before the first product
+    const int cb3 = wave & 1, kh = wave >> 1;
+    {
+        bf16x8 la[16], lw[16];
+        const unsigned short* A = h2 + static_cast<long long>(e * nfb + fb) * 32 * 512 +
+                                  kh * 16 * 512 + lane * 8;
         const unsigned short* Bw = net.W3 + static_cast<long long>(e) * SG_H2 * SG_H3 +
-                                   static_cast<long long>(2 * q + cb) * 32 * 512;
-        f32x16 acc = {};
+                                   static_cast<long long>(2 * q + cb3) * 32 * 512 +
+                                   kh * 16 * 512 + lane * 8;
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            const int ks = kh * 16 + s;
-            acc = mfma16(ld_frag(A + ks * 512 + lane * 8), ld_frag(Bw + ks * 512 + lane * 8), acc);
+            lw[s] = ld_frag(Bw + s * 512);
+            la[s] = ld_frag(A + s * 512);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        f32x16 acc = {};
+#pragma unroll
+        for (int s = 0; s < 16; ++s) acc = mfma16(lw[s], la[s], acc);
         if (kh == 1)
 #pragma unroll
-            for (int j = 0; j < 16; ++j) red[(cb * 16 + j) * 64 + lane] = acc[j];
+            for (int j = 0; j < 16; ++j) red[(cb3 * 16 + j) * 64 + lane] = acc[j];
         __syncthreads();
         if (kh == 0) {
-            const int c = cb * 32 + (lane & 31);
-            const float bias = net.b3[e * SG_H3 + 64 * q + c];
 #pragma unroll
-            for (int j = 0; j < 16; ++j)
-                X1[acc_row(j, lane)][c] =
-                    to_bf16(silu(acc[j] + red[(cb * 16 + j) * 64 + lane] + bias));
+            for (int j = 0; j < 16; ++j) acc[j] += red[(cb3 * 16 + j) * 64 + lane];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int c0 = cb3 * 32 + 8 * a + 4 * h;
+                *reinterpret_cast<uint2*>(&X1[r][c0]) = act_run(acc, a, bs3 + c0);
+            }
         }
-        __syncthreads();
     }
-    // stage 1: 64 -> 4 x 64 (rows f)
-    {
-        bf16x8 a[4];
+    // the decoder's weights, in flight through the layer-3 epilogue
+    bf16x8 w1[2][4], w2[4][4], w3[2][2];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) a[ks] = ld_frag(&X1[lane & 31][ks * 16 + 8 * (lane >> 5)]);
-        const unsigned short* Bw = net.D1 + static_cast<long long>(e) * 64 * 256;
-#pragma unroll
-        for (int c2 = 0; c2 < 2; ++c2) {
-            const int nb = wave * 2 + c2;
-            f32x16 acc = {};
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                acc = mfma16(a[ks], ld_frag(Bw + (nb * 4 + ks) * 512 + lane * 8), acc);
-            const int n = nb * 32 + (lane & 31), s1 = n >> 6, c = n & 63;
-            const float bias = net.d1[e * 64 + c];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) X2[acc_row(j, lane) * 4 + s1][c] = to_bf16(silu(acc[j] + bias));
-        }
-        __syncthreads();
-    }
-    // stage 2: 64 -> 4 x 32 (rows (f, s1)); wave = row block
-    {
-        bf16x8 a[4];
+    for (int c2 = 0; c2 < 2; ++c2)
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks)
-            a[ks] = ld_frag(&X2[32 * wave + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
-        const unsigned short* Bw = net.D2 + static_cast<long long>(e) * 64 * 128;
+            w1[c2][ks] = ld_frag(net.D1 + static_cast<long long>(e) * 64 * 256 +
+                                 ((wave * 2 + c2) * 4 + ks) * 512 + lane * 8);
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            f32x16 acc = {};
+    for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                acc = mfma16(a[ks], ld_frag(Bw + (nb * 4 + ks) * 512 + lane * 8), acc);
-            const int n = nb * 32 + (lane & 31), s2 = n >> 5, c = n & 31;
-            const float bias = net.d2[e * 32 + c];
-#pragma unroll
-            for (int j = 0; j < 16; ++j)
-                X3[(32 * wave + acc_row(j, lane)) * 4 + s2][c] = to_bf16(silu(acc[j] + bias));
-        }
-        __syncthreads();
-    }
-    // stages 3 and 4, one block of 32 stage-3 rows (two frames) at a time
-    const unsigned short* B3 = net.D3 + static_cast<long long>(e) * 32 * 64;
-    bf16x8 w3[2][2];
+        for (int ks = 0; ks < 4; ++ks)
+            w2[nb][ks] = ld_frag(net.D2 + static_cast<long long>(e) * 64 * 128 +
+                                 (nb * 4 + ks) * 512 + lane * 8);
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) w3[cb][ks] = ld_frag(B3 + (cb * 2 + ks) * 512 + lane * 8);
+        for (int ks = 0; ks < 2; ++ks)
+            w3[cb][ks] = ld_frag(net.D3 + static_cast<long long>(e) * 32 * 64 +
+                                 (cb * 2 + ks) * 512 + lane * 8);
     const bf16x8 w4 = ld_frag(net.D4 + static_cast<long long>(e) * 16 * 32 + lane * 8);
     const float b4 = net.d4[e];
-    const int Y0 = 16 * (q >> 3), X0 = 16 * (q & 7);
-    for (int t = 0; t < 4; ++t) {
-        const int rb = wave * 4 + t;
-        bf16x8 a[2];
+    __syncthreads();
+    // stage 1: 64 -> 4 x 64; wave w computes channels 64 w .. +63, i.e. s1 = w
+    {
+        bf16x8 x[4];
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-            a[ks] = ld_frag(&X3[32 * rb + (lane & 31)][ks * 16 + 8 * (lane >> 5)]);
+        for (int ks = 0; ks < 4; ++ks) x[ks] = ld_frag(&X1[r][ks * 16 + 8 * h]);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2) {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = mfma16(w1[c2][ks], x[ks], acc);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int c0 = c2 * 32 + 8 * a + 4 * h;
+                *reinterpret_cast<uint2*>(&X2[r][c0]) = act_run(acc, a, bsd1 + c0);
+            }
+        }
+    }
+    __syncthreads();
+    bf16x8 x2[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) x2[ks] = ld_frag(&X2[r][ks * 16 + 8 * h]);
+    const int f = fb * SG_FB + r;
+    float* orow = out + (static_cast<long long>(e) * B + (f < B ? f : 0)) * 16384;
+    const int s1 = wave;
+    for (int s2 = 0; s2 < 4; ++s2) {
+        // stage 2 block s2: 64 -> 32 channels
+        {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = mfma16(w2[s2][ks], x2[ks], acc);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int c0 = 8 * a + 4 * h;
+                *reinterpret_cast<uint2*>(&X3[r][c0]) = act_run(acc, a, bsd2 + c0);
+            }
+        }
+        __syncthreads();
+        bf16x8 x3[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) x3[ks] = ld_frag(&X3[r][ks * 16 + 8 * h]);
+        float4 o[4];
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
+            // stage 3 block cb: s3 = 2 cb + a / 2, channel 8 (a % 2) + 4 h + 0..3
             f32x16 acc = {};
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) acc = mfma16(a[ks], w3[cb][ks], acc);
-            const int n = cb * 32 + (lane & 31), s3 = n >> 4, c = n & 15;
-            const float bias = net.d3[e * 16 + c];
+            for (int ks = 0; ks < 2; ++ks) acc = mfma16(w3[cb][ks], x3[ks], acc);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) X4[acc_row(j, lane) * 4 + s3][c] = to_bf16(silu(acc[j] + bias));
+            for (int a = 0; a < 4; ++a) {
+                const int c0 = 8 * (a & 1) + 4 * h;
+                *reinterpret_cast<uint2*>(&X4[(a >> 1) * 32 + r][c0]) =
+                    act_run(acc, a, bsd3 + c0);
+            }
+            __syncthreads();
+            // stage 4: rows n = s4 = 0..3 are elements 0..3 of lanes 0..31
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int s3 = 2 * cb + sl;
+                f32x16 acc4 = {};
+                acc4 = mfma16(w4, ld_frag(&X4[sl * 32 + r][8 * h]), acc4);
+                const int yi = 2 * (s3 >> 1), xi = 2 * (s3 & 1);
+                reinterpret_cast<float*>(&o[yi])[xi] = acc4[0] + b4;
+                reinterpret_cast<float*>(&o[yi])[xi + 1] = acc4[1] + b4;
+                reinterpret_cast<float*>(&o[yi + 1])[xi] = acc4[2] + b4;
+                reinterpret_cast<float*>(&o[yi + 1])[xi + 1] = acc4[3] + b4;
+            }
+            __syncthreads();      // X4 is rewritten by the next block
         }
-        __syncthreads();
+        // the frame's 4 x 4 block under (s1, s2)
+        if (h == 0 && f < B) {
+            const int yb = 16 * (q >> 3) + 8 * (s1 >> 1) + 4 * (s2 >> 1);
+            const int xb = 16 * (q & 7) + 8 * (s1 & 1) + 4 * (s2 & 1);
 #pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4) {
-            f32x16 acc = {};
-            acc = mfma16(ld_frag(&X4[32 * r4 + (lane & 31)][8 * (lane >> 5)]), w4, acc);
-            const int s4 = lane & 31;
-            if (s4 < 4) {
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int row4 = 32 * r4 + acc_row(j, lane);   // (r3, s3) in this block
-                    const int r3 = row4 >> 2, s3 = row4 & 3;
-                    const int R3 = 32 * rb + r3;                   // (f, s1, s2)
-                    const int s2 = R3 & 3, s1 = (R3 >> 2) & 3, fl = r3 >> 4;
-                    const int yl = 8 * (s1 >> 1) + 4 * (s2 >> 1) + 2 * (s3 >> 1) + (s4 >> 1);
-                    const int xl = 8 * (s1 & 1) + 4 * (s2 & 1) + 2 * (s3 & 1) + (s4 & 1);
-                    Ow[(fl * 16 + yl) * 16 + xl] = acc[j] + b4;
+            for (int yi = 0; yi < 4; ++yi) {
+                float4 v = o[yi];
+                if (post.on) {
+                    v.x = fmaf(post.c, __expf(fmaf(post.a, v.x, post.b)), post.d);
+                    v.y = fmaf(post.c, __expf(fmaf(post.a, v.y, post.b)), post.d);
+                    v.z = fmaf(post.c, __expf(fmaf(post.a, v.z, post.b)), post.d);
+                    v.w = fmaf(post.c, __expf(fmaf(post.a, v.w, post.b)), post.d);
                 }
+                *reinterpret_cast<float4*>(orow + (yb + yi) * 128 + xb) = v;
             }
         }
-        __syncthreads();
-        // two frames' 16 x 16 blocks: lane -> (frame, row, 8-float half row)
-        {
-            const int fl = lane >> 5, yl = (lane >> 1) & 15, xh = (lane & 1) * 8;
-            const int f = fb * SG_FB + 2 * rb + fl;
-            if (f < B) {
-                const float4* src = reinterpret_cast<const float4*>(Ow + (fl * 16 + yl) * 16 + xh);
-                float4* dst = reinterpret_cast<float4*>(
-                    out + (static_cast<long long>(e) * B + f) * 16384 + (Y0 + yl) * 128 + X0 + xh);
-                dst[0] = src[0];
-                dst[1] = src[1];
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -656,8 +720,8 @@ extern "C" int mosaic_sg_infer_workspace(int members, int batch, size_t* bytes) 
 }
 
 extern "C" int mosaic_sg_infer(int members, int batch, const float* X, const mosaic_sg_net* net,
-                               void* workspace, size_t workspace_bytes, float* out,
-                               mosaic_stream_t stream) {
+                               const float* post, void* workspace, size_t workspace_bytes,
+                               float* out, mosaic_stream_t stream) {
     if (members < 1 || batch < 0 || !net) return MOSAIC_ERR_ARGS;
     if (batch == 0) return MOSAIC_OK;
     if (!X || !out || !workspace) return MOSAIC_ERR_ARGS;
@@ -687,8 +751,10 @@ extern "C" int mosaic_sg_infer(int members, int batch, const float* X, const mos
     unsigned short* h2 = static_cast<unsigned short*>(workspace);
     hipLaunchKernelGGL(k_sg_mlp12, dim3(nfb, SG_H2 / 128, members), dim3(256), 0, st, X, batch, n,
                        h2, nfb);
+    SgPost pm = {0.f, 0.f, 0.f, 0.f, 0};
+    if (post) pm = {post[0], post[1], post[2], post[3], 1};
     hipLaunchKernelGGL(k_sg_decode, dim3(64, nfb, members), dim3(256), 0, st, h2, nfb, batch, n,
-                       out);
+                       pm, out);
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

@@ -6,6 +6,7 @@ caller owns every buffer, the output is accumulated into, and a decline
 PyTorch; all arithmetic runs in ``libmosaic_amd.so``.
 """
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -36,7 +37,7 @@ def species_table(consts):
     return arr
 
 
-PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4, "exact": 5}
+PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4, "exact": 5, "probe": 6}
 
 # analyser_mode names of the reference's GPU driver (Beam.py:6134-6143)
 _ANALYSER = {"off": 0, "none": 0, "disabled": 0, "top_hat": 1, "tophat": 1, "top-hat": 1,
@@ -59,9 +60,10 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
              apply_decay=False, path="auto", analyser=0, analyser_axis=(1.0, 0.0, 0.0),
              accept_rad=0.0, darwin_hw_rad=0.0):
     """mosaic_kin_opts_t; ``path`` is "auto", "exact" (per pair), "tile"
-    (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches)
-    or "poly" (moment expansion over the same patches); the last four decline
-    when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
+    (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches),
+    "poly" (moment expansion over the same patches) or "probe" (a few
+    scattered pixels, one lane per frame and pixel); all but "auto" and
+    "exact" decline when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
     top-hat, 2 Darwin) with its axis (lab), acceptance half-angle and Darwin
     half-width; it takes the exact kernel."""
     o = KinOpts()
@@ -195,7 +197,19 @@ def kin_pixels(staged, species_tab, opts, coords, stream=None):
 
 
 def _pixels_key(species_tab, opts, coords):
-    return (int(coords.data_ptr()), bytes(species_tab), bytes(opts))
+    """What the pixel prologue's tables depend on: the coordinate tensor
+    itself (held weakly, with its address and version counter, since a new
+    tensor can be allocated at a freed one's address), species and flags."""
+    return (weakref.ref(coords), int(coords.data_ptr()), int(coords._version),
+            bytes(species_tab), bytes(opts))
+
+
+def _same_pixels(key, species_tab, opts, coords):
+    if key is None:
+        return False
+    ref, ptr, ver, sp, op = key
+    return (ref() is coords and ptr == int(coords.data_ptr()) and ver == int(coords._version)
+            and sp == bytes(species_tab) and op == bytes(opts))
 
 
 def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
@@ -221,7 +235,7 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
         raise ValueError("out must hold (frames, Nz, Ny) complex64/complex128")
     if not out.is_contiguous():
         raise ValueError("out must be contiguous")
-    if staged.pixels_key != _pixels_key(species_tab, opts, coords):
+    if not _same_pixels(staged.pixels_key, species_tab, opts, coords):
         kin_pixels(staged, species_tab, opts, coords, stream)
     c, cp_ = _counts_arr(staged.counts)
     used = ctypes.c_int(0)

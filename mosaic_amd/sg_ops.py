@@ -122,10 +122,11 @@ def infer_workspace_bytes(members, batch):
     return int(n.value)
 
 
-def infer(X, packed, biases, out, workspace):
+def infer(X, packed, biases, out, workspace, post=None):
     """Fused forward of every member: X (B, 11), ``packed`` the 7 packed
     weight tensors and ``biases`` the 7 (E, n) bias tensors in layer order,
-    out (E, B, 128*128)."""
+    out (E, B, 128*128); ``post`` = (a, b, c, d) stores c exp(a y + b) + d
+    in place of each output y."""
     _f32(X, out, *biases)
     E, B = out.shape[0], out.shape[1]
     if X.shape != (B, 11) or out.shape[2] != 128 * 128:
@@ -135,7 +136,12 @@ def infer(X, packed, biases, out, workspace):
         setattr(net, name, int(t.data_ptr()))
     for name, t in zip(("b1", "b2", "b3", "d1", "d2", "d3", "d4"), biases):
         setattr(net, name, int(t.data_ptr()))
-    check(_lib.lib().mosaic_sg_infer(E, B, _p(X), ctypes.byref(net), _p(workspace),
-                                     workspace.numel() * workspace.element_size(), _p(out), _s()),
+    pm = None
+    if post is not None:
+        pm = (ctypes.c_float * 4)(*[float(x) for x in post])
+    check(_lib.lib().mosaic_sg_infer(E, B, _p(X), ctypes.byref(net),
+                                     ctypes.cast(pm, ctypes.c_void_p) if pm is not None else None,
+                                     _p(workspace), workspace.numel() * workspace.element_size(),
+                                     _p(out), _s()),
           "mosaic_sg_infer")
     return out

@@ -25,9 +25,11 @@ Training images come from the GPU kinematic sum itself
 (``mosaic_defect_positions``), the crystal staged and summed at its rocking
 angle.  At scan time (``SurrogateScan``) the plain sum computes anchor frames (the
 first and last angle), which judge the network and correct each member's
-prediction; the other frames come from the corrected ensemble, and those
-whose ensemble disagreement exceeds a threshold calibrated on held-out scans
-are recomputed with the plain sum.
+prediction; the other frames come from the corrected ensemble.  An exact
+probe of each such frame at a few dozen importance-sampled pixels estimates
+its relative error, and frames whose estimate exceeds the threshold are
+recomputed with the plain sum.  Inference runs fused (``infer``: two
+launches on the 32x32x16 BF16 matrix cores).
 """
 import ctypes
 import math
@@ -200,6 +202,29 @@ class ImageGenerator:
         self.path_used = staged.path_used
         return (out.abs() ** 2).reshape(F, -1)
 
+    def probe(self, p, angles_deg, pix):
+        """Exact |A|^2 (F, P) at detector pixels ``pix`` (P flat indices,
+        device) for defect parameters p at the given angles: the plain sum's
+        probe form, one lane per (angle, pixel)."""
+        pos = self.stage_crystal(p)
+        F, P = len(angles_deg), int(pix.numel())
+        poses = [self.pose(a) for a in angles_deg]
+        rot = np.stack([q[0] for q in poses])
+        tr = np.stack([q[1] for q in poses])
+        key = ("probe", F, P)
+        staged = self._staged.get(key)
+        if staged is None:
+            staged = K.stage(self.base, self.codes, None, self.counts, self.k, F, P, 1)
+            self._staged[key] = staged
+            self._probe_opts = K.kin_opts(self.k, False, True,
+                                          self.w["beam"]._polarization_state(), False,
+                                          path="probe")
+        K.restage(staged, pos)
+        coords = self.coords[:, pix].contiguous()
+        out = torch.zeros((F, 1, P), dtype=torch.complex64, device=self.device)
+        K.kin_sum(staged, self.spc, self._probe_opts, rot, tr, coords, out)
+        return (out.abs() ** 2).reshape(F, P)
+
 
 def make_training_set(gen, n, seed, report_every=2000):
     """n (params, intensity) pairs from the GPU kinematic sum."""
@@ -264,10 +289,12 @@ class SurrogateEnsemble:
         """Whether the fused inference kernels cover this architecture."""
         return self.MLP == (N_FEAT, 256, 512, 4096) and self.DEC == self.FUSED_DEC
 
-    def infer(self, X):
+    def infer(self, X, intensity=False):
         """X (B, 11) ``features`` on the device -> (E, B, 128*128) outputs in
-        two launches (mosaic_sg_infer); the weights are packed into the
-        matrix cores' operand order once per weight update."""
+        two launches (mosaic_sg_infer), or with ``intensity`` (log target)
+        the intensities, the target's inverse fused into the store; the
+        weights are packed into the matrix cores' operand order once per
+        weight update."""
         if not self.fused():
             raise ValueError("the fused inference covers the 128 x 128 network only")
         if self._packed is None or self._packed[0] != self.step_count:
@@ -278,8 +305,12 @@ class SurrogateEnsemble:
             self._infer_ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
         out = torch.empty((self.E, B, self.out_hw * self.out_hw), dtype=torch.float32,
                           device=self.device)
+        post = None
+        if intensity:
+            I0, mu, sd = self.norm
+            post = (sd, mu, I0, -I0 * LOG_EPS)
         return S.infer(X.contiguous(), self._packed[1], [L.b for L in self.layers], out,
-                       self._infer_ws)
+                       self._infer_ws, post)
 
     @property
     def layers(self):
@@ -443,14 +474,19 @@ class SurrogateEnsemble:
         """Each member's intensity (E, B, npix) for (B, 9) defect parameters."""
         Xall = torch.as_tensor(features(P), device=self.device)
         outs = []
-        run = self.infer if self.fused() else self.forward
+        fused = self.fused() and self.target == "log"
         for i in range(0, Xall.shape[0], chunk):
-            outs.append(self.intensity(run(Xall[i:i + chunk].contiguous())))
+            x = Xall[i:i + chunk].contiguous()
+            outs.append(self.infer(x, intensity=True) if fused
+                        else self.intensity(self.infer(x) if self.fused() else self.forward(x)))
         return torch.cat(outs, dim=1)
 
 
 def rel_l2(a, b):
-    return (torch.linalg.norm(a - b, dim=-1) / torch.linalg.norm(b, dim=-1))
+    """Relative L2 error along the last axis, in float64 (intensities in r_e
+    units are ~1e-18: their squares underflow float32)."""
+    a, b = a.double(), b.double()
+    return (torch.linalg.norm(a - b, dim=-1) / torch.linalg.norm(b, dim=-1)).float()
 
 
 def anchor_weights(angles, anchor_angles):
@@ -472,71 +508,88 @@ def anchor_weights(angles, anchor_angles):
 
 
 class SurrogateScan:
-    """A rocking scan from the ensemble, anchored on plain frames.
+    """A rocking scan from the ensemble, anchored on plain frames and judged
+    by exact probes.
 
     The plain sum computes the anchor frames (by default the first and the
     last angle).  Each member's prediction is corrected by that member's own
-    error at the anchors, interpolated linearly in angle; the corrected
-    members' mean is the frame, and their RMS disagreement relative to the
-    frame's RMS is its spread.  Frames whose spread exceeds ``threshold`` are
-    recomputed with the plain sum.  The anchors also judge the network: when
-    its raw error at an anchor exceeds ``anchor_limit`` the whole scan is
-    computed with the plain sum.
+    error at the anchors, interpolated linearly in angle, and the corrected
+    members' mean is the frame.  The judge then evaluates the exact field at
+    ``judge_pixels`` detector pixels of every other frame (the probe form of
+    the plain sum), drawn with replacement with probability q_p proportional
+    to the predicted scan-mean intensity plus 5 % of its mean, and estimates
+    each frame's relative L2 error by the importance-weighted ratio
+
+        e_f^2 = sum_k (I'_fk - I_fk)^2 / q_k / sum_k I_fk^2 / q_k.
+
+    A frame is kept when the estimate's upper bound, e_f^2 plus ``sigmas``
+    standard errors of the importance-sampled ratio (delta method over the
+    draws), stays within ``threshold`` squared; the others are recomputed
+    with the plain sum.  The anchors also judge the network: when its raw error at an anchor
+    exceeds ``anchor_limit`` the whole scan is computed with the plain sum.
     """
 
-    def __init__(self, model, gen, threshold, anchors=(0, -1), anchor_limit=0.25):
+    def __init__(self, model, gen, threshold=1e-2, anchors=(0, -1), anchor_limit=0.25,
+                 judge_pixels=64, seed=0, sigmas=3.0):
         self.model, self.gen = model, gen
         self.threshold = float(threshold)
+        self.sigmas = float(sigmas)
         self.anchors = tuple(anchors)
         self.anchor_limit = float(anchor_limit)
+        self.judge_pixels = int(judge_pixels)
+        self._gen = torch.Generator(device=gen.device)
+        self._gen.manual_seed(seed)
 
     def run(self, p, angles_deg):
-        """(I (F, npix), spread (F,), indices recomputed with the plain sum)."""
+        """(I (F, npix), estimated relative error per frame (F,; 0 for plain
+        frames), indices recomputed with the plain sum).  The host waits on
+        the device once, for the anchors' verdict and the rejected frames."""
         angles = np.asarray(angles_deg, dtype=np.float64)
         F = angles.size
         ia = sorted({a % F for a in self.anchors})
+        inner = np.array([f for f in range(F) if f not in ia], dtype=np.int64)
         P = np.repeat(np.asarray(p, dtype=np.float64)[None], F, axis=0)
         P[:, 8] = angles
         Im = self.model.member_intensities(P)                    # (E, F, npix)
         Ia = self.gen.intensities(p, angles[ia])                 # (A, npix), plain
-        raw = rel_l2(Im[:, ia].mean(dim=0), Ia)
-        if float(raw.max()) > self.anchor_limit:
-            spread = torch.full((F,), float("inf"), device=Ia.device)
-            return self.gen.intensities(p, angles), spread, np.arange(F)
-        W = torch.as_tensor(anchor_weights(angles, angles[ia]), device=Ia.device)
-        C = (Im + torch.matmul(W, Ia[None] - Im[:, ia])).contiguous()
-        mean = torch.empty(C.shape[1:], dtype=torch.float32, device=C.device)
-        spread = torch.empty(F, dtype=torch.float32, device=C.device)
-        S.ensemble_stats(C, mean, spread)
-        spread = spread / (mean * mean).mean(dim=1).sqrt().clamp_min(1e-30)
-        mean = mean.clamp_min(0.0)
-        mean[ia] = Ia
-        spread[ia] = 0.0
-        bad = torch.nonzero(spread > self.threshold).flatten().cpu().numpy()
+        dev = Ia.device
+        ia_t = torch.as_tensor(ia, dtype=torch.int64, device=dev)
+        in_t = torch.as_tensor(inner, dtype=torch.int64, device=dev)
+        raw = rel_l2(Im[:, ia_t].mean(dim=0), Ia)
+        W = torch.as_tensor(anchor_weights(angles, angles[ia]), device=dev)
+        mean = (Im + torch.matmul(W, Ia[None] - Im[:, ia_t])).mean(dim=0).clamp_min(0.0)
+        mean[ia_t] = Ia
+        est = torch.zeros(F, dtype=torch.float32, device=dev)
+        upper = torch.zeros_like(est)
+        if inner.size and self.judge_pixels > 0:
+            mi = mean[in_t]
+            w = mi.mean(dim=0)
+            w = w + 0.05 * w.mean() + 1e-30
+            q = w / w.sum()
+            pix = torch.multinomial(q, self.judge_pixels, replacement=True,
+                                    generator=self._gen)
+            # float64: intensities in r_e units are ~1e-18, their squares
+            # and squared differences underflow float32
+            exact = self.gen.probe(p, angles[inner], pix).double()     # (Fi, P)
+            iw = 1.0 / q[pix].double()
+            d = mi[:, pix].double() - exact
+            a_ = d * d * iw                       # per-draw terms of the ratio
+            b_ = exact * exact * iw
+            ma, mb = a_.mean(dim=1), b_.mean(dim=1)
+            mb = torch.where(mb > 0, mb, torch.ones_like(mb))
+            R = ma / mb
+            # delta-method variance of the ratio of means over the n draws
+            n = a_.shape[1]
+            va = ((a_ - ma[:, None]) - R[:, None] * (b_ - mb[:, None])).pow(2).sum(dim=1)
+            var = va / (max(n - 1, 1) * n) / (mb * mb)
+            est[in_t] = torch.sqrt(R).float()
+            upper[in_t] = torch.sqrt(R + self.sigmas * torch.sqrt(var)).float()
+        flags = torch.cat([(raw.max() > self.anchor_limit).reshape(1),
+                           upper > self.threshold]).cpu().numpy()
+        if flags[0]:
+            est = torch.full((F,), float("inf"), device=dev)
+            return self.gen.intensities(p, angles), est, np.arange(F)
+        bad = np.nonzero(flags[1:])[0]
         if bad.size:
-            mean[bad] = self.gen.intensities(p, angles[bad])
-        return mean, spread, bad
-
-
-def calibrate_threshold(model, gen, angles_deg, n_scans, seed, target=1e-2, anchors=(0, -1)):
-    """Largest spread threshold under which every non-anchor frame of
-    ``n_scans`` held-out scans meets ``target`` (relative L2 against the plain
-    sum); 0 when even the least uncertain frame misses it."""
-    rng = np.random.default_rng(seed)
-    scan = SurrogateScan(model, gen, float("inf"), anchors=anchors, anchor_limit=float("inf"))
-    F = len(angles_deg)
-    ia = {a % F for a in anchors}
-    sp, er = [], []
-    for _ in range(n_scans):
-        p = sample_params(1, rng)[0]
-        I, spread, _ = scan.run(p, angles_deg)
-        e = rel_l2(I, gen.intensities(p, angles_deg)).cpu().numpy()
-        s_ = spread.cpu().numpy()
-        for f in range(F):
-            if f not in ia:
-                sp.append(s_[f])
-                er.append(e[f])
-    sp, er = np.array(sp), np.array(er)
-    order = np.argsort(sp)
-    ok = np.cumprod(er[order] <= target).astype(bool)
-    return float(sp[order][ok].max()) if ok[0] else 0.0
+            mean[torch.as_tensor(bad, device=dev)] = self.gen.intensities(p, angles[bad])
+        return mean, est, bad

@@ -44,3 +44,22 @@ def test_every_export_has_a_signature():
     """ctypes converts Python floats only with declared argument types."""
     from mosaic_amd import _lib
     assert set(_lib._SIGS) == set(_lib.EXPORTS)
+
+
+def test_pixel_tables_cache_key():
+    """The pixel prologue is reused only for the same coordinate tensor,
+    unchanged: another tensor at the same address (here a view), or an
+    in-place change, recomputes it."""
+    import torch
+    from mosaic_amd import kinematic as K
+    spc = K.species_table(__import__("mosaic_amd.tables", fromlist=["x"]).species_constants(
+        ["Au"], 10000.0))
+    opts = K.kin_opts(5e10)
+    c = torch.zeros(3, 16, dtype=torch.float64)
+    key = K._pixels_key(spc, opts, c)
+    assert K._same_pixels(key, spc, opts, c)
+    assert not K._same_pixels(key, spc, opts, c.view(3, 16))     # same address, other tensor
+    assert not K._same_pixels(key, spc, K.kin_opts(5e10, apply_pol=True), c)
+    c.add_(1.0)
+    assert not K._same_pixels(key, spc, opts, c)
+    assert not K._same_pixels(None, spc, opts, c)

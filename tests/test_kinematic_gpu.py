@@ -380,3 +380,40 @@ def test_staging_many_species_ragged(device):
     e, _ = report(f"{n} random atoms, 12 species, ragged counts (path {w['beam']._last_path})",
                   A, R)
     assert e <= TOL
+
+
+def test_probe_scattered_pixels(device):
+    """The probe form (one lane per frame and pixel) on 40 scattered pixels
+    of a 32 x 32 detector at three poses, against the long-double oracle at
+    those pixels, and against the moment sum over the whole detector."""
+    w = W.fcc_au_bragg111(n=8, npix=32, seed=3)
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    pc = np.asarray(d.pixel_coordinates)
+    sel = np.sort(np.random.default_rng(4).choice(32 * 32, 40, replace=False))
+    poses = []
+    for a in (-0.02, 0.0, 0.03):
+        st.set_single_motor_value_absolute("phi", a)
+        poses.append((np.asarray(st.rotation, np.float32), np.asarray(st.translation, np.float32)))
+    st.set_single_motor_value_absolute("phi", 0.0)
+    rot = np.stack([q[0] for q in poses])
+    tr = np.stack([q[1] for q in poses])
+    b._kin_path = "probe"
+    got = b._kin_field(s, np.ascontiguousarray(pc[:, sel]), (40, 1), st, (rot, tr), False,
+                       True, False)
+    torch.cuda.synchronize()
+    assert b._last_path in (60, 160)
+    b._kin_path = "auto"
+    full = b._kin_field(s, pc, d.shape, st, (rot, tr), False, True, False)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy().reshape(3, 40)
+    full = full.cpu().numpy().reshape(3, -1)[:, sel]
+    for i, (R, T) in enumerate(poses):
+        a = W.oracle_inputs(w, frame_rot=R, frame_trans=T)
+        kw = {k: a[k] for k in ("pos_m", "f0p", "f0z", "sanr", "sani", "amp_r", "amp_i",
+                                "k_val", "pol_deg", "pol_e")}
+        ref = O.kin_sum_c(**kw, cx=a["cx"][sel], cy=a["cy"][sel], cz=a["cz"][sel], Ny=40, Nz=1,
+                          remove_forward=False, apply_pol=True, apply_decay=False,
+                          precision="f80")
+        e, _ = report(f"probe pose {i}", got[i], ref)
+        assert e <= TOL
+        assert rel(got[i], full[i]) <= TOL

@@ -4,9 +4,11 @@
 * the ensemble's hand-written forward/backward against torch autograd on the
   same architecture (float32 with the GEMM operands rounded to bfloat16 as
   the kernels round them);
-* a short training run lowers the loss; the anchored scan keeps the plain
-  anchor frames and sends high-spread frames (or, when the anchors judge the
-  network wrong, every frame) to the plain sum.
+* a short training run lowers the loss; the judge's probes equal the plain
+  frames at their pixels and, with many probes, its error estimate equals a
+  frame's true error; the anchored scan keeps the plain anchor frames and
+  sends frames the judge rejects (or, when the anchors judge the network
+  wrong, every frame) to the plain sum.
 """
 import math
 
@@ -160,6 +162,13 @@ def test_fused_inference_matches_layers(device, members, batch):
     model.adam_step(1e-3)
     e2 = ((model.infer(X) - model.forward(X)).norm() / model.forward(X).norm()).item()
     assert e2 < 5e-4
+    # the log target's inverse fused into the store
+    model.norm = (3.0e-18, -4.0, 2.5)
+    want = model.intensity(model.infer(X)).double()
+    got = model.infer(X, intensity=True).double()
+    e3 = ((got - want).norm() / want.norm()).item()
+    print(f"[parity] fused intensity map: rel {e3:.2e}", flush=True)
+    assert e3 < 1e-5
 
 
 def test_short_training_and_gate(device):
@@ -172,11 +181,26 @@ def test_short_training_and_gate(device):
     Ipred, spread = model.predict(P[:8])
     assert Ipred.shape == (8, 32 * 32) and torch.all(spread >= 0)
 
-    # the anchored scan: anchors are the plain frames; with a zero threshold
-    # every other frame falls back to the plain sum; with an infinite one none
+    # the judge's probes are the plain frames' pixels
     angles = np.linspace(-0.05, 0.05, 9)
     p = P[0]
     plain = gen.intensities(p, angles)
+    pix = torch.as_tensor([0, 5, 517, 1023, 300, 301], device=device)
+    pr = gen.probe(p, angles, pix)
+    e_pr = ((pr.double() - plain[:, pix].double()).norm() / plain[:, pix].double().norm()).item()
+    assert e_pr > 0.0                       # two different kernels
+    print(f"[surrogate] probe vs plain frames: rel {e_pr:.2e}")
+    assert e_pr < 1e-5
+    # with many probes the judge's estimate is the frame's true error
+    s_many = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf, judge_pixels=8192)
+    Im, est, _ = s_many.run(p, angles)
+    Ipre = Im.clone()
+    true = SG.rel_l2(Ipre, plain).cpu().numpy()
+    print(f"[surrogate] judge estimate {est.cpu().numpy()[1:-1]} true {true[1:-1]}")
+    assert np.allclose(est.cpu().numpy()[1:-1], true[1:-1], rtol=0.2)
+
+    # the anchored scan: anchors are the plain frames; with a zero threshold
+    # every other frame falls back to the plain sum; with an infinite one none
     I0, sp0, bad0 = SG.SurrogateScan(model, gen, 0.0, anchor_limit=np.inf).run(p, angles)
     assert sorted(bad0.tolist()) == list(range(1, 8))
     assert torch.allclose(I0, plain, rtol=1e-6, atol=0)
