@@ -80,10 +80,12 @@ def parse():
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-end-to-end", action="store_true")
+    ap.add_argument("--no-surrogate-inference", action="store_true")
     ap.add_argument("--surrogate", action="store_true",
                     help="run the surrogate workload (data generation, training, gated scan)")
     ap.add_argument("--sg-images", type=int, default=20000)
     ap.add_argument("--sg-epochs", type=int, default=100)
+    ap.add_argument("--sg-scans", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to HBM bytes)")
@@ -468,6 +470,48 @@ def bench_adjoint(steps):
             "rel_err_sample": "20 random atoms, long-double reference gradient"}
 
 
+# multiply-accumulates of one member on one image: the MLP 11-256-512-4096
+# and the four 2x2 stride-2 transposed convolutions (input pixels x Cin x 4 Cout)
+SG_MACS = (11 * 256 + 256 * 512 + 512 * 4096 + 64 * 64 * 256 + 256 * 64 * 128 +
+           1024 * 32 * 64 + 4096 * 16 * 4)
+
+
+def bench_surrogate_inference(reps, members=5, frames=64):
+    """The surrogate ensemble's fused inference (mosaic_sg_infer, two
+    launches) on one 64-angle scan, against its layer-by-layer form.
+    Random-init weights of the trained architecture: the work does not depend
+    on their values."""
+    import torch
+    from mosaic_amd import surrogate as SG
+    model = SG.SurrogateEnsemble(members=members, seed=0)
+    rng = np.random.default_rng(0)
+    X = torch.as_tensor(rng.uniform(-1, 1, (frames, SG.N_FEAT)).astype(np.float32),
+                        device=model.device)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(3):
+        model.infer(X)
+        model.forward(X)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        model.infer(X)
+    ev[1].record()
+    for _ in range(reps):
+        model.forward(X)
+    ev[2].record()
+    ev[2].synchronize()
+    t_f = ev[0].elapsed_time(ev[1]) / reps
+    t_l = ev[1].elapsed_time(ev[2]) / reps
+    flop = 2.0 * SG_MACS * members * frames
+    return {"members": members, "frames": frames, "fused_us": t_f * 1e3,
+            "layered_us": t_l * 1e3, "flop": flop,
+            "achieved_tflops": flop / (t_f * 1e-3) / 1e12, "peak_tflops_bf16": 2500.0,
+            "frac": flop / (t_f * 1e-3) / 2.5e15,
+            "note": "algorithmic BF16 MFMA flops (15.5 MFLOP per member-image) over the "
+                    "HIP-event time of both launches; the elementwise SiLU work (v_exp, "
+                    "v_rcp per activation) and launch latency bound it, not the matrix cores"}
+
+
 def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, judge_pixels=64,
                     threshold=1e-2):
     """Learned rocking scan: ensemble trained on GPU-generated images of a
@@ -604,7 +648,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.surrogate:
-        print(json.dumps({"surrogate": bench_surrogate(args.sg_images, args.sg_epochs)}),
+        print(json.dumps({"surrogate": bench_surrogate(args.sg_images, args.sg_epochs,
+                                                       n_scans=args.sg_scans)}),
               flush=True)
         return
     if args.copy_calibration:
@@ -696,6 +741,8 @@ def main():
         res["scan"] = bench_scan(max(1, args.steps - 1), 1)
     if not args.no_adjoint:
         res["adjoint"] = bench_adjoint(1)
+    if not args.no_surrogate_inference:
+        res["surrogate_inference"] = bench_surrogate_inference(max(3, args.steps))
     if args.err_pixels > 0:
         res["rel_err_vs_fp64_oracle"] = accuracy(w, field, args.err_pixels)
         res["rel_err_pixels"] = args.err_pixels

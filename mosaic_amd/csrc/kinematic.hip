@@ -366,6 +366,15 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     const int64_t grid_px =
         static_cast<int64_t>(std::min(L.pix_blocks, mma_blocks)) * std::max(1, n_frames);
     int slices = static_cast<int>((TARGET_BLOCKS + grid_px - 1) / grid_px);
+    // Each slice writes FP64 partials for every pixel and pose (16 B) that the
+    // reduction reads back: keep them within four times the atom records the
+    // slice reads (32 B per atom), at least npix * poses / 2048 tiles per
+    // slice, so that a small detector does not spend more on partials than on
+    // the sum (a 128^2 frame of 200k atoms: 512 slices of 1.5 tiles before).
+    const int64_t min_tps = static_cast<int64_t>(npix) * std::max(1, n_frames) / 2048;
+    if (min_tps > 1)
+        slices = static_cast<int>(std::min<int64_t>(
+            slices, (std::max(1, L.n_tiles) + min_tps - 1) / min_tps));
     slices = std::max(1, std::min(slices, std::max(1, L.n_tiles)));
     L.tiles_per_slice = (std::max(1, L.n_tiles) + slices - 1) / slices;
     L.n_slices = (std::max(1, L.n_tiles) + L.tiles_per_slice - 1) / L.tiles_per_slice;

@@ -146,14 +146,18 @@ __global__ void k_act_bwd(const float* __restrict__ Z, const float* __restrict__
     dZ[i] = act ? dY[i] * silu_grad(Z[i]) : dY[i];
 }
 
-// dbias[b][c % period] = sum over rows of dZ[b][row][c] (one block per (c, b))
+// dbias[b][c % period] = sum over rows of dZ[b][row][c]: block (p, b, z) sums
+// its share of the (row, repeat) elements of bias p and adds it atomically
+// (dbias cleared first by k_fill).
 __global__ void k_bias_grad(const float* __restrict__ dZ, float* __restrict__ db, long long rows,
-                            int cols, int period, long long sZ, long long sB) {
+                            int cols, int period, long long sZ, long long sB, long long per_block) {
     __shared__ float red[256];
     const int p = blockIdx.x, b = blockIdx.y;
-    float s = 0.f;
     const int reps = cols / period;
-    for (long long r = threadIdx.x; r < rows * reps; r += blockDim.x) {
+    const long long total = rows * reps;
+    const long long r0 = blockIdx.z * per_block, r1 = min(total, r0 + per_block);
+    float s = 0.f;
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const long long row = r / reps;
         const int c = static_cast<int>(r % reps) * period + p;
         s += dZ[b * sZ + row * cols + c];
@@ -164,7 +168,12 @@ __global__ void k_bias_grad(const float* __restrict__ dZ, float* __restrict__ db
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[b * sB + p] = red[0];
+    if (threadIdx.x == 0) atomicAdd(db + b * sB + p, red[0]);
+}
+
+__global__ void k_fill(float* __restrict__ x, long long n, float v) {
+    const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+    if (i < n) x[i] = v;
 }
 
 // 2x2 stride-2 transposed-convolution shuffle (channels last):
@@ -642,10 +651,26 @@ extern "C" int mosaic_sg_act_bwd(const float* Z, const float* dY, float* dZ, lon
 extern "C" int mosaic_sg_bias_grad(const float* dZ, float* dbias, long long rows, int cols,
                                    int period, long long strideZ, long long strideBias,
                                    int batch, mosaic_stream_t stream) {
-    if (!dZ || !dbias || cols < 1 || period < 1 || cols % period) return MOSAIC_ERR_ARGS;
-    hipLaunchKernelGGL(k_bias_grad, dim3(period, batch), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), dZ, dbias, rows, cols, period, strideZ,
-                       strideBias);
+    if (!dZ || !dbias || cols < 1 || period < 1 || cols % period || rows < 0 || batch < 1)
+        return MOSAIC_ERR_ARGS;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // dbias rows: batch blocks of strideBias, the first `period` of each used
+    const long long nb = static_cast<long long>(batch - 1) * strideBias + period;
+    hipLaunchKernelGGL(k_fill, dim3(grid1(nb)), dim3(256), 0, st, dbias, nb, 0.f);
+    // about 1024 workgroups, each over at least 4096 elements of its bias
+    const long long total = rows * (cols / period);
+    if (total == 0) {
+        MOSAIC_HIP_CHECK(hipGetLastError());
+        return MOSAIC_OK;
+    }
+    long long splits = std::max(1LL, std::min<long long>((total + 4095) / 4096,
+                                                        1024LL / (static_cast<long long>(period) *
+                                                                  batch)));
+    splits = std::min<long long>(splits, 65535);
+    const long long per_block = (total + splits - 1) / splits;
+    splits = std::max(1LL, (total + per_block - 1) / per_block);
+    hipLaunchKernelGGL(k_bias_grad, dim3(period, batch, static_cast<unsigned>(splits)), dim3(256),
+                       0, st, dZ, dbias, rows, cols, period, strideZ, strideBias, per_block);
     MOSAIC_HIP_CHECK(hipGetLastError());
     return MOSAIC_OK;
 }

@@ -99,3 +99,17 @@ def test_mse_adam_stats(device):
     S.ensemble_stats(Pm, mean, spread)
     assert torch.allclose(mean, Pm.mean(0), atol=1e-6)
     assert torch.allclose(spread, Pm.var(0, unbiased=False).mean(1).sqrt(), atol=1e-5)
+
+
+@pytest.mark.parametrize("rows,cols,period", [(300000, 4, 1), (5000, 256, 64), (7, 12, 4)])
+def test_bias_grad_split_rows(device, rows, cols, period):
+    """The bias gradient split over workgroups (atomic partial sums) against
+    torch's float64 sum, after a stale value in the output."""
+    g = torch.Generator(device="cpu").manual_seed(rows)
+    dZ = torch.randn((3, rows, cols), generator=g).to(device)
+    db = torch.full((3, period), 7.0, device=device)
+    S.bias_grad(dZ, db, period)
+    ref = dZ.double().reshape(3, rows, cols // period, period).sum(dim=(1, 2))
+    err = ((db.double() - ref).abs().max() / ref.abs().max()).item()
+    print(f"[parity] bias_grad rows={rows} cols={cols} period={period}: rel={err:.2e}")
+    assert err < 1e-5
