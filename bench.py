@@ -86,6 +86,8 @@ def parse():
     ap.add_argument("--sg-images", type=int, default=20000)
     ap.add_argument("--sg-epochs", type=int, default=100)
     ap.add_argument("--sg-scans", type=int, default=8)
+    ap.add_argument("--sg-infer-only", action="store_true",
+                    help="time only the surrogate's fused inference (5 members x 64 frames)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 3 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to HBM bytes)")
@@ -647,6 +649,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.sg_infer_only:
+        print(json.dumps({"surrogate_inference": bench_surrogate_inference(50)}), flush=True)
+        return
     if args.surrogate:
         print(json.dumps({"surrogate": bench_surrogate(args.sg_images, args.sg_epochs,
                                                        n_scans=args.sg_scans)}),

@@ -122,20 +122,27 @@ def infer_workspace_bytes(members, batch):
     return int(n.value)
 
 
-def infer(X, packed, biases, out, workspace, post=None):
-    """Fused forward of every member: X (B, 11), ``packed`` the 7 packed
-    weight tensors and ``biases`` the 7 (E, n) bias tensors in layer order,
-    out (E, B, 128*128); ``post`` = (a, b, c, d) stores c exp(a y + b) + d
-    in place of each output y."""
-    _f32(X, out, *biases)
-    E, B = out.shape[0], out.shape[1]
-    if X.shape != (B, 11) or out.shape[2] != 128 * 128:
-        raise ValueError(f"infer shapes {tuple(X.shape)} {tuple(out.shape)}")
+def infer_net(packed, biases):
+    """The mosaic_sg_net struct of ``packed`` (the 7 packed weight tensors)
+    and ``biases`` (the 7 (E, n) bias tensors) in layer order; the tensors
+    must outlive it."""
+    _f32(*biases)
     net = _lib.SgNet()
     for name, t in zip(("W1", "W2", "W3", "D1", "D2", "D3", "D4"), packed):
         setattr(net, name, int(t.data_ptr()))
     for name, t in zip(("b1", "b2", "b3", "d1", "d2", "d3", "d4"), biases):
         setattr(net, name, int(t.data_ptr()))
+    return net
+
+
+def infer(X, net, out, workspace, post=None):
+    """Fused forward of every member: X (B, 11), ``net`` from ``infer_net``,
+    out (E, B, 128*128); ``post`` = (a, b, c, d) stores c exp(a y + b) + d
+    in place of each output y."""
+    _f32(X, out)
+    E, B = out.shape[0], out.shape[1]
+    if X.shape != (B, 11) or out.shape[2] != 128 * 128:
+        raise ValueError(f"infer shapes {tuple(X.shape)} {tuple(out.shape)}")
     pm = None
     if post is not None:
         pm = (ctypes.c_float * 4)(*[float(x) for x in post])

@@ -298,7 +298,9 @@ class SurrogateEnsemble:
         if not self.fused():
             raise ValueError("the fused inference covers the 128 x 128 network only")
         if self._packed is None or self._packed[0] != self.step_count:
-            self._packed = (self.step_count, [S.pack(L.W) for L in self.layers])
+            packed = [S.pack(L.W) for L in self.layers]
+            self._packed = (self.step_count, packed,
+                            S.infer_net(packed, [L.b for L in self.layers]))
         B = int(X.shape[0])
         nb = S.infer_workspace_bytes(self.E, B)
         if self._infer_ws is None or self._infer_ws.numel() < nb:
@@ -309,8 +311,7 @@ class SurrogateEnsemble:
         if intensity:
             I0, mu, sd = self.norm
             post = (sd, mu, I0, -I0 * LOG_EPS)
-        return S.infer(X.contiguous(), self._packed[1], [L.b for L in self.layers], out,
-                       self._infer_ws, post)
+        return S.infer(X.contiguous(), self._packed[2], out, self._infer_ws, post)
 
     @property
     def layers(self):
