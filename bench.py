@@ -50,8 +50,8 @@ MFMA_FLOP_PER_PAIR = 24.0
 COUNTERS = {
     "k_sum_poly": dict(
         source="profiles/k_sum_poly_perovskite5M_512px.md",
-        fetch_kib=1.01095e6, write_kib=842670.0, valu=4.71956e9,
-        add_f32=2.06103e8, mul_f32=8.27142e8, fma_f32=2.82412e9, trans_f32=9.04499e7),
+        fetch_kib=719146.0, write_kib=266402.0, valu=4.711e9,
+        add_f32=2.06103e8, mul_f32=8.25971e8, fma_f32=2.82295e9, trans_f32=9.04499e7),
     "k_sum_mma": dict(
         source="profiles/k_sum_mma_perovskite5M_512px.md",
         fetch_kib=1831133.0, write_kib=2542443.8, valu=None,

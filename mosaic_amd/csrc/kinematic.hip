@@ -2448,6 +2448,10 @@ hipError_t launch_mma(const MmaArgs& A, int n_slices, int n_frames, hipStream_t 
 // origin phase as the matrix-core sum does per tile; the pixel's table factor (a function of species and pixel only)
 // multiplies the running sums when they are flushed.
 constexpr int POLY_WAVES = 4;
+// Units summed in FP32 between flushes into the FP64 slice partial (each
+// flush reads and writes 16 B per pixel of the patch).
+constexpr int POLY_FLUSH = 128;
+
 
 template <int D>
 struct PolyN {
@@ -2761,7 +2765,7 @@ void k_sum_poly(const MmaArgs A) {
             acc_re[r] = fmaf(cs, sr, fmaf(-sn, si, acc_re[r]));
             acc_im[r] = fmaf(cs, si, fmaf(sn, sr, acc_im[r]));
         }
-        if (++since == MMA_FLUSH || t + 1 == t1) flush();
+        if (++since == POLY_FLUSH || t + 1 == t1) flush();
         if (has_next) {
             float4* dst = reinterpret_cast<float4*>(&tile_buf[buf ^ 1][tid]);
             dst[0] = n0;
