@@ -25,6 +25,7 @@ Deliberate differences, each bounded by the tests:
   per-pair kernel runs.
 """
 import hashlib
+from concurrent.futures import ThreadPoolExecutor
 import json
 import os
 
@@ -174,9 +175,12 @@ class beam:
         """Load the chunks ``chunk_ids`` (positions with the sample's thermal
         displacements, species codes remapped onto the global label table)
         into one pinned host buffer and copy each chunk to the device on the
-        ``copy`` stream as soon as it is loaded, so that loading the next chunk
-        overlaps the transfer.  Returns device positions (N, 3) float32,
-        codes (N,) uint8, the per-species counts and the copies' event."""
+        ``copy`` stream as soon as it is loaded, so that loading the next
+        chunks overlaps the transfer.  The label table is built first, in
+        chunk order; the chunks are then loaded by a pool of threads (NumPy
+        releases the GIL for the copies) straight into their slices of the
+        pinned buffer.  Returns device positions (N, 3) float32, codes (N,)
+        uint8, the per-species counts and the copies' event."""
         sizes = [int(sample.chunk_atom_count(c)) for c in chunk_ids]
         total = sum(sizes)
         ev_prev = getattr(self, "_pin_event", None)
@@ -188,20 +192,11 @@ class beam:
         d_cod = torch.empty(total, dtype=torch.uint8, device=dev)
         copy.wait_stream(torch.cuda.current_stream(dev))
         hp_np, hc_np = hp.numpy(), hc.numpy()
+        # species: each chunk's codes and its map onto the global labels
+        offs, maps, code_arrs = [], [], []
         off = 0
-        path_of = getattr(sample, "chunk_position_path", None)
         for cid, n in zip(chunk_ids, sizes):
-            path = path_of(cid) if path_of is not None else None
-            if path is not None and not getattr(sample, "enable_temp", False):
-                # a chunk file with nothing to apply on load: copied once, from
-                # the page cache straight into the pinned buffer
-                pos = np.load(path, mmap_mode="r")
-            else:
-                pos = sample.load_chunk_positions(cid, use_gpu=False)
             codes, lab = sample.load_chunk_species(cid, use_gpu=False, codes=True)
-            if n == 0:
-                continue
-            pos = np.asarray(pos, dtype=np.float32).reshape(n, 3)
             rm = np.zeros(max(1, len(lab)), dtype=np.uint8)
             for i, el in enumerate(lab):
                 el = str(el)
@@ -211,16 +206,58 @@ class beam:
                     index[el] = len(labels)
                     labels.append(el)
                 rm[i] = index[el]
-            hp_np[3 * off:3 * (off + n)] = pos.ravel()
-            hc_np[off:off + n] = rm[np.asarray(codes, dtype=np.int64)]
-            with torch.cuda.stream(copy):
-                d_pos[off:off + n].copy_(hp[3 * off:3 * (off + n)].view(n, 3), non_blocking=True)
-                d_cod[off:off + n].copy_(hc[off:off + n], non_blocking=True)
+            offs.append(off)
+            maps.append(None if np.array_equal(rm[:len(lab)], np.arange(len(lab))) else rm)
+            code_arrs.append(codes)
             off += n
+        view_of = getattr(sample, "chunk_positions_view", None)
+
+        def load(k):
+            """Fill chunk k's slices of the pinned buffers; returns its
+            per-species counts on the global label table."""
+            cid, n, o = chunk_ids[k], sizes[k], offs[k]
+            if n == 0:
+                return None
+            pos = view_of(cid) if view_of is not None else None
+            if pos is None:
+                pos = sample.load_chunk_positions(cid, use_gpu=False)
+            np.copyto(hp_np[3 * o:3 * (o + n)].reshape(n, 3),
+                      np.asarray(pos).reshape(n, 3), casting="same_kind")
+            codes, rm = code_arrs[k], maps[k]
+            dst = hc_np[o:o + n]
+            if rm is None:                                      # codes already global
+                np.copyto(dst, codes, casting="unsafe")
+            else:                                               # codes checked on load
+                np.take(rm, codes, out=dst, mode="clip")
+            local = np.bincount(codes.reshape(-1), minlength=1)
+            glob = np.zeros(16, dtype=np.int64)
+            np.add.at(glob, np.arange(local.size) if rm is None else rm[:local.size], local)
+            return glob
+
+        workers = self.__dict__.get("_loader")
+        if workers is None:
+            workers = ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1)))
+            self._loader = workers
+        futures = [workers.submit(load, k) for k in range(len(chunk_ids))]
+        counts = np.zeros(16, dtype=np.int64)
+        # copies of runs of loaded chunks, about an eighth of the atoms each
+        batch = max(1, total // 8)
+        lo = 0
+        hp2 = hp[:3 * total].view(total, 3) if total else None
+        for k, fut in enumerate(futures):
+            c = fut.result()
+            if c is not None:
+                counts += c
+            hi = offs[k] + sizes[k]
+            if hi > lo and (hi - lo >= batch or k + 1 == len(futures)):
+                with torch.cuda.stream(copy):
+                    d_pos[lo:hi].copy_(hp2[lo:hi], non_blocking=True)
+                    d_cod[lo:hi].copy_(hc[lo:hi], non_blocking=True)
+                lo = hi
         ev = torch.cuda.Event()
         ev.record(copy)
         self._pin_event = ev
-        counts = np.bincount(hc_np[:total], minlength=max(1, len(labels))).astype(np.int64)
+        counts = counts[:max(1, len(labels))].copy()
         return d_pos, d_cod, counts, ev
 
     # --------------------------------------------------- depth-dependent Ein

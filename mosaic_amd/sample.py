@@ -344,6 +344,16 @@ class sample:
             return np.load(self.chunk_position_path(chunk_number))
         return np.array(self._pos[chunk_number - 1], copy=True)
 
+    def chunk_positions_view(self, chunk_number):
+        """The chunk's positions without a copy (the stored array, or a
+        memory map of the chunk file) when nothing applies on load (no
+        thermal displacements or expansion); None otherwise."""
+        if self.enable_temp:
+            return None
+        if self._on_disk:
+            return np.load(self.chunk_position_path(chunk_number), mmap_mode="r")
+        return self._pos[chunk_number - 1]
+
     def load_chunk_positions(self, chunk_number, use_gpu=False, raw=False):
         pos = self._raw_positions(chunk_number)
         if self.enable_temp and not raw:
