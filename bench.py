@@ -591,8 +591,11 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, judge_pixels
                                axis=0) for j in range(3)], axis=1).astype(np.float32)
         quad = SG.rel_l2(torch.matmul(torch.as_tensor(Lq, device=Ip.device), Ip[[0, 32, 63]]),
                          Ip).cpu().numpy()
+        sp = scan.last_spread.cpu().numpy()
         scans.append({"plain_ms": t_plain, "surrogate_ms": t_sg,
                       "speedup": t_plain / t_sg, "fallback_frames": int(len(bad)),
+                      "spread_fallbacks": int(np.sum(sp > scan.spread_limit)),
+                      "spread_max": float(sp.max()),
                       "inference_ms": ev[4].elapsed_time(ev[5]),
                       "anchors_ms": ev[5].elapsed_time(ev[6]),
                       "probe_ms": ev[7].elapsed_time(ev[8]),
@@ -615,7 +618,8 @@ def bench_surrogate(n_images, epochs, members=5, seed=0, n_scans=8, judge_pixels
             "params_per_member": model.n_params(), "target": model.target,
             "gen_s": t_gen, "train_s": t_train,
             "final_loss": [float(x) for x in hist[-1]], "judge_threshold": threshold,
-            "judge_pixels": judge_pixels,
+            "judge_pixels": judge_pixels, "spread_limit": scan.spread_limit,
+            "spread_fallbacks_total": int(sum(x["spread_fallbacks"] for x in scans)),
             "val_median_rel_err": float(np.median(errv)),
             "val_frac_below_1e-2": float(np.mean(errv <= 1e-2)),
             "plain_frames_per_s": 64 / (np.mean([x["plain_ms"] for x in scans]) * 1e-3),

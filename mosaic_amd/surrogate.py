@@ -25,10 +25,11 @@ Training images come from the GPU kinematic sum itself
 (``mosaic_defect_positions``), the crystal staged and summed at its rocking
 angle.  At scan time (``SurrogateScan``) the plain sum computes anchor frames (the
 first and last angle), which judge the network and correct each member's
-prediction; the other frames come from the corrected ensemble.  An exact
-probe of each such frame at a few dozen importance-sampled pixels estimates
-its relative error, and frames whose estimate exceeds the threshold are
-recomputed with the plain sum.  Inference runs fused (``infer``: two
+prediction; the other frames come from the corrected ensemble.  Frames on
+which the members disagree beyond a limit fall back to the plain sum, and
+so do frames that the plain sum judges: an exact probe of each frame at a
+few dozen importance-sampled pixels estimates its relative error, and
+frames whose estimate's upper bound exceeds the threshold are recomputed.  Inference runs fused (``infer``: two
 launches on the 32x32x16 BF16 matrix cores).
 """
 import ctypes
@@ -523,17 +524,20 @@ class SurrogateScan:
 
         e_f^2 = sum_k (I'_fk - I_fk)^2 / q_k / sum_k I_fk^2 / q_k.
 
-    A frame is kept when the estimate's upper bound, e_f^2 plus ``sigmas``
-    standard errors of the importance-sampled ratio (delta method over the
-    draws), stays within ``threshold`` squared; the others are recomputed
-    with the plain sum.  The anchors also judge the network: when its raw error at an anchor
+    A frame is kept when the members agree (their RMS spread about the
+    frame, relative to its RMS, at most ``spread_limit``) and the estimate's
+    upper bound, e_f^2 plus ``sigmas`` standard errors of the
+    importance-sampled ratio (delta method over the draws), stays within
+    ``threshold`` squared; the others are recomputed with the plain sum.  The anchors also judge the network: when its raw error at an anchor
     exceeds ``anchor_limit`` the whole scan is computed with the plain sum.
     """
 
     def __init__(self, model, gen, threshold=1e-2, anchors=(0, -1), anchor_limit=0.25,
-                 judge_pixels=64, seed=0, sigmas=3.0):
+                 judge_pixels=64, seed=0, sigmas=3.0, spread_limit=0.05):
         self.model, self.gen = model, gen
         self.threshold = float(threshold)
+        self.spread_limit = float(spread_limit)
+        self.last_spread = None
         self.sigmas = float(sigmas)
         self.anchors = tuple(anchors)
         self.anchor_limit = float(anchor_limit)
@@ -558,7 +562,16 @@ class SurrogateScan:
         in_t = torch.as_tensor(inner, dtype=torch.int64, device=dev)
         raw = rel_l2(Im[:, ia_t].mean(dim=0), Ia)
         W = torch.as_tensor(anchor_weights(angles, angles[ia]), device=dev)
-        mean = (Im + torch.matmul(W, Ia[None] - Im[:, ia_t])).mean(dim=0).clamp_min(0.0)
+        C = Im + torch.matmul(W, Ia[None] - Im[:, ia_t])         # corrected members
+        mean = C.mean(dim=0)
+        # the members' disagreement, relative to the frame (float64: these
+        # intensities are ~1e-18 in r_e units)
+        Cd, md = C.double(), mean.double()
+        spread = torch.sqrt((Cd - md[None]).pow(2).mean(dim=(0, 2)) /
+                            md.pow(2).mean(dim=1).clamp_min(1e-300)).float()
+        spread[ia_t] = 0.0
+        self.last_spread = spread
+        mean = mean.clamp_min(0.0)
         mean[ia_t] = Ia
         est = torch.zeros(F, dtype=torch.float32, device=dev)
         upper = torch.zeros_like(est)
@@ -586,7 +599,7 @@ class SurrogateScan:
             est[in_t] = torch.sqrt(R).float()
             upper[in_t] = torch.sqrt(R + self.sigmas * torch.sqrt(var)).float()
         flags = torch.cat([(raw.max() > self.anchor_limit).reshape(1),
-                           upper > self.threshold]).cpu().numpy()
+                           (upper > self.threshold) | (spread > self.spread_limit)]).cpu().numpy()
         if flags[0]:
             est = torch.full((F,), float("inf"), device=dev)
             return self.gen.intensities(p, angles), est, np.arange(F)

@@ -155,13 +155,18 @@ def test_fused_inference_matches_layers(device, members, batch):
     got = model.infer(X)
     e = ((got - ref).norm() / ref.norm()).item()
     print(f"[parity] fused inference E={members} B={batch}: rel {e:.2e}", flush=True)
-    assert got.shape == ref.shape and e < 5e-4
+    # the two forms differ only in float32 summation order (layer 3's K split
+    # over wave pairs, the layered GEMMs' split K), i.e. by the bf16
+    # rounding-flip floor of the test above (1.5e-4 to 6e-4 measured,
+    # depending on the weights); a dropped term or a wrong operand is O(1)
+    assert got.shape == ref.shape and e < 1e-3
     for L in model.layers:                       # one Adam step with fake gradients
         for g in L.grads:
             g.copy_(torch.randn_like(g) * 1e-2)
     model.adam_step(1e-3)
     e2 = ((model.infer(X) - model.forward(X)).norm() / model.forward(X).norm()).item()
-    assert e2 < 5e-4
+    print(f"[parity] fused inference after an update: rel {e2:.2e}", flush=True)
+    assert e2 < 1e-3
     # the log target's inverse fused into the store
     model.norm = (3.0e-18, -4.0, 2.5)
     want = model.intensity(model.infer(X)).double()
@@ -192,7 +197,8 @@ def test_short_training_and_gate(device):
     print(f"[surrogate] probe vs plain frames: rel {e_pr:.2e}")
     assert e_pr < 1e-5
     # with many probes the judge's estimate is the frame's true error
-    s_many = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf, judge_pixels=8192)
+    s_many = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf, judge_pixels=8192,
+                              spread_limit=np.inf)
     Im, est, _ = s_many.run(p, angles)
     Ipre = Im.clone()
     true = SG.rel_l2(Ipre, plain).cpu().numpy()
@@ -204,10 +210,20 @@ def test_short_training_and_gate(device):
     I0, sp0, bad0 = SG.SurrogateScan(model, gen, 0.0, anchor_limit=np.inf).run(p, angles)
     assert sorted(bad0.tolist()) == list(range(1, 8))
     assert torch.allclose(I0, plain, rtol=1e-6, atol=0)
-    I1, sp1, bad1 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf).run(p, angles)
+    I1, sp1, bad1 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf,
+                                     spread_limit=np.inf).run(p, angles)
     assert bad1.size == 0 and float(sp1[0]) == 0.0 and float(sp1[-1]) == 0.0
-    assert torch.equal(I1[0], plain[0]) and torch.equal(I1[-1], plain[-1])
+    # the anchors are plain frames (summed over another slicing of the atoms
+    # than the 9-frame scan's: equal up to the FP64 partials' order)
+    assert torch.allclose(I1[0], plain[0], rtol=1e-6, atol=0)
+    assert torch.allclose(I1[-1], plain[-1], rtol=1e-6, atol=0)
     assert torch.all(sp1[1:-1] > 0)
+    # members that disagree send their frames to the plain sum
+    s0 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=np.inf, spread_limit=0.0)
+    I3, _, bad3 = s0.run(p, angles)
+    assert sorted(bad3.tolist()) == list(range(1, 8))
+    assert float(s0.last_spread[0]) == 0.0 and torch.all(s0.last_spread[1:-1] > 0)
+    assert torch.allclose(I3, plain, rtol=1e-6, atol=0)
     # an anchor error above the limit sends the whole scan to the plain sum
     I2, _, bad2 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=0.0).run(p, angles)
     assert bad2.size == 9 and torch.allclose(I2, plain, rtol=1e-6, atol=0)
