@@ -564,11 +564,13 @@ class SurrogateScan:
         W = torch.as_tensor(anchor_weights(angles, angles[ia]), device=dev)
         C = Im + torch.matmul(W, Ia[None] - Im[:, ia_t])         # corrected members
         mean = C.mean(dim=0)
-        # the members' disagreement, relative to the frame (float64: these
-        # intensities are ~1e-18 in r_e units)
-        Cd, md = C.double(), mean.double()
-        spread = torch.sqrt((Cd - md[None]).pow(2).mean(dim=(0, 2)) /
-                            md.pow(2).mean(dim=1).clamp_min(1e-300)).float()
+        # the members' disagreement, relative to the frame, on intensities
+        # scaled by the anchors' peak (in r_e units they are ~1e-18, whose
+        # squares float32 cannot hold to full precision)
+        inv = 1.0 / Ia.max().clamp_min(1e-37)
+        Cs, ms = C * inv, mean * inv
+        spread = torch.sqrt((Cs - ms[None]).pow(2).mean(dim=(0, 2)) /
+                            ms.pow(2).mean(dim=1).clamp_min(1e-30))
         spread[ia_t] = 0.0
         self.last_spread = spread
         mean = mean.clamp_min(0.0)
