@@ -227,3 +227,37 @@ def test_short_training_and_gate(device):
     # an anchor error above the limit sends the whole scan to the plain sum
     I2, _, bad2 = SG.SurrogateScan(model, gen, np.inf, anchor_limit=0.0).run(p, angles)
     assert bad2.size == 9 and torch.allclose(I2, plain, rtol=1e-6, atol=0)
+
+
+def test_judged_scan_full_crystal(device):
+    """The surrogate workload's crystal (FCC Au, 37^3 cells, 200k atoms) and
+    128^2 detector, trained briefly (2,000 images, 8 epochs): two judged
+    64-angle scans in +-0.05 deg.  Every frame of the result is within 1e-2
+    (relative L2) of the plain sum, whatever the network's quality, because
+    frames the judge cannot vouch for fall back; and on the frames it
+    estimates, the judge's 64-probe estimate follows the true error."""
+    gen = SG.ImageGenerator(cells=(37, 37, 37), npix=128, device=device)
+    P, I = SG.make_training_set(gen, 2000, seed=11, report_every=0)
+    model = SG.SurrogateEnsemble(members=5, seed=1, device=device)
+    model.train(P, I, epochs=8, batch=128, lr=2e-3, log=None)
+    angles = np.linspace(-0.05, 0.05, 64)
+    scan = SG.SurrogateScan(model, gen, seed=3)
+    rng = np.random.default_rng(12)
+    for k in range(2):
+        p = SG.sample_params(1, rng)[0]
+        Is, est, bad = scan.run(p, angles)
+        plain = gen.intensities(p, angles)
+        err = SG.rel_l2(Is, plain).cpu().numpy()
+        # the judge on the frames it estimated, against their pre-fallback error
+        P64 = np.repeat(np.asarray(p)[None], 64, axis=0)
+        P64[:, 8] = angles
+        Im = model.member_intensities(P64)
+        W = torch.as_tensor(SG.anchor_weights(angles, angles[[0, 63]]), device=device)
+        pre = (Im + torch.matmul(W, plain[[0, 63]][None] - Im[:, [0, 63]])).mean(0).clamp_min(0)
+        true = SG.rel_l2(pre, plain).cpu().numpy()[1:63]
+        ratio = est.cpu().numpy()[1:63] / true
+        print(f"[surrogate] judged scan {k}: {len(bad)} fallbacks, max err {err.max():.2e}, "
+              f"judge estimate / true error {np.median(ratio):.2f} "
+              f"[{ratio.min():.2f}, {ratio.max():.2f}]", flush=True)
+        assert err.max() <= 1e-2
+        assert 0.5 <= np.median(ratio) <= 2.0
