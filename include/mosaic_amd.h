@@ -162,12 +162,13 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  *     offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle phasors;
  *     units of more than one tile are used where their radius, measured by
  *     mosaic_kin_stage, keeps every error bound); and a probe form for a
- *     few scattered pixels (6, requested only: one lane per (frame, pixel),
- *     the recurrence's far-field model with the exact direction at each
- *     pixel, one sincos per pair).
+ *     few scattered pixels (6: one lane per (frame, pixel), the recurrence's
+ *     far-field model with the exact direction at each pixel, one sincos per
+ *     pair).
  * opts->path = 0 picks the moment expansion, else the matrix-core form, else
- * the recurrence, else the per-tile form, whenever their error bounds hold,
- * and the exact kernel otherwise; an analyser always takes the exact kernel.
+ * the recurrence, else (at most 16384 pixel-poses) the probe form, else the
+ * per-tile form, whenever their error bounds hold, and the exact kernel
+ * otherwise; an analyser always takes the exact kernel.
  * A forced path (1-4, 6) returns MOSAIC_ERR_UNSUPPORTED (nothing written)
  * when its bounds fail.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and

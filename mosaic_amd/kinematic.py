@@ -62,8 +62,9 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
     """mosaic_kin_opts_t; ``path`` is "auto", "exact" (per pair), "tile"
     (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches),
     "poly" (moment expansion over the same patches) or "probe" (a few
-    scattered pixels, one lane per frame and pixel); all but "auto" and
-    "exact" decline when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
+    scattered pixels, one lane per frame and pixel; "auto" takes it for at
+    most 16384 pixel-poses); all but "auto" and "exact" decline when their
+    bounds do not hold.  ``analyser`` is the kind (0 off, 1
     top-hat, 2 Darwin) with its axis (lab), acceptance half-angle and Darwin
     half-width; it takes the exact kernel."""
     o = KinOpts()

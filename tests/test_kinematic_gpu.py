@@ -301,7 +301,8 @@ def test_rows_second_order_close_detector(device):
 def test_auto_path_choice(device):
     """Far field picks the moment sum; a 5 cm detector with 2 mm pixels
     bends the rows and patches too much for it, the matrix-core form and the
-    recurrence, and falls back to per-tile factors."""
+    recurrence: its 256 pixels take the probe form (one lane per pixel), and
+    with the probe excluded (forced per-tile path) the per-tile factors."""
     w = W.fcc_au_bragg111(n=4, npix=16)
     b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
     b._kin_path = "auto"
@@ -311,9 +312,13 @@ def test_auto_path_choice(device):
     d.create_detector((16, 16), (2.0e7, 2.0e7))
     d.position_detector_absolute(5.0e8, np.rad2deg(2 * w["theta_b"]), 0.0)
     A = gpu_field(w)
-    assert b._last_path in (11, 12)
+    assert b._last_path in (60, 160)
     R = oracle_field(w, "f80")
     e, _ = report("near detector auto", A, R)
+    assert e <= TOL
+    A = gpu_field(w, path="tile")
+    assert b._last_path in (11, 12)
+    e, _ = report("near detector per-tile", A, R)
     assert e <= TOL
 
 
