@@ -245,7 +245,20 @@ class beam:
         lo = 0
         hp2 = hp[:3 * total].view(total, 3) if total else None
         for k, fut in enumerate(futures):
-            c = fut.result()
+            try:
+                c = fut.result()
+            except BaseException:
+                # no worker may still write into the pinned buffers, which the
+                # next call reuses
+                for f in futures:
+                    f.cancel()
+                for f in futures:
+                    if not f.cancelled():
+                        try:
+                            f.result()
+                        except BaseException:
+                            pass
+                raise
             if c is not None:
                 counts += c
             hi = offs[k] + sizes[k]

@@ -107,3 +107,18 @@ def test_thermal_draws_keyed_and_scaled():
         assert abs(d.std() / sig - 1.0) < 0.05
         c = s.load_chunk_positions(1)
         assert not np.allclose(c - s.load_chunk_positions(1, raw=True), d[:300])
+
+
+def test_chunk_positions_view(tmp_path):
+    """The copy-free view the drop-in gathers from equals the loaded positions,
+    in memory and from chunk files, and is withheld when thermal
+    displacements apply on load."""
+    s = _sample()
+    s.write_chunks(str(tmp_path))
+    d = S.sample.from_directory(str(tmp_path))
+    for smp in (s, d):
+        for cid in range(1, smp.chunk_total + 1):
+            v = smp.chunk_positions_view(cid)
+            assert v is not None and np.array_equal(np.asarray(v), smp.load_chunk_positions(cid))
+    s.set_temperature("gaussian", 0.05, None, 3)
+    assert s.chunk_positions_view(1) is None
