@@ -66,7 +66,9 @@ typedef struct {
 /*
  * Workspace bytes for mosaic_kin_stage + mosaic_kin_sum on a sample of
  * `atom_count` atoms split by species as `species_counts[n_species]`, for
- * `n_frames` poses of an Ny x Nz detector.
+ * `n_frames` poses of an Ny x Nz detector.  Limits (MOSAIC_ERR_ARGS beyond
+ * them, here and in every staged-sum call): fewer than 2^32 atoms, at most
+ * 16 species, 65535 poses per call, 2^30 pixels and 2^31 - 1 pixel-poses.
  */
 int mosaic_kin_workspace(int64_t atom_count, int n_species,
                          const int64_t* species_counts /* host */,

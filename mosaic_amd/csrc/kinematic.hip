@@ -46,6 +46,10 @@ namespace mosaic {
 namespace {
 
 constexpr int TILE = 256;            // atoms per tile
+constexpr int MAX_FRAMES = 65535;    // poses per call: the sum kernels' grid.z
+constexpr int64_t MAX_PIXELS = int64_t(1) << 30;   // pixel indices stay 32-bit
+// pixel-poses per call: the reduction and the probe launch one lane per output
+constexpr int64_t MAX_OUTPUTS = (int64_t(1) << 31) - 1;
 constexpr uint32_t NO_SPLIT = 0xFFFFFFFFu;   // k-d node that does not split
 constexpr int SUM_THREADS = 256;     // threads per sum workgroup
 constexpr int MAX_SPECIES = 16;
@@ -3909,7 +3913,10 @@ using namespace mosaic;
 extern "C" int mosaic_kin_workspace(int64_t atom_count, int n_species, const int64_t* counts,
                                     int n_frames, int Ny, int Nz, size_t* bytes) {
     if (!bytes || atom_count < 0 || n_species < 1 || n_species > MAX_SPECIES || !counts ||
-        n_frames < 1 || Ny < 1 || Nz < 1 || atom_count >= (int64_t(1) << 32))
+        n_frames < 1 || n_frames > MAX_FRAMES || Ny < 1 || Nz < 1 ||
+        static_cast<int64_t>(Ny) * Nz > MAX_PIXELS ||
+        static_cast<int64_t>(Ny) * Nz * n_frames > MAX_OUTPUTS ||
+        atom_count >= (int64_t(1) << 32))
         return MOSAIC_ERR_ARGS;
     int64_t tot = 0;
     for (int s = 0; s < n_species; ++s) {
@@ -4133,8 +4140,9 @@ extern "C" int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_
                                  const mosaic_kin_opts_t* opts, int Ny, int Nz,
                                  const double* coords, void* workspace, size_t workspace_bytes,
                                  double* pixel_geom, mosaic_stream_t stream) {
-    if (Ny < 1 || Nz < 1 || !coords || !opts || !(opts->k > 0.0) || !workspace ||
-        !pixel_geom || n_species < 1 || n_species > MAX_SPECIES || !species_tab)
+    if (Ny < 1 || Nz < 1 || static_cast<int64_t>(Ny) * Nz > MAX_PIXELS || !coords || !opts ||
+        !(opts->k > 0.0) || !workspace || !pixel_geom || n_species < 1 ||
+        n_species > MAX_SPECIES || !species_tab)
         return MOSAIC_ERR_ARGS;
     const int npix = Ny * Nz;
     Carver c(workspace);

@@ -63,3 +63,27 @@ def test_pixel_tables_cache_key():
     c.add_(1.0)
     assert not K._same_pixels(key, spc, opts, c)
     assert not K._same_pixels(None, spc, opts, c)
+
+
+def test_staged_sum_size_limits():
+    """mosaic_kin_workspace (called first by every staged-sum entry) rejects
+    sizes past the kernels' grids: more than 65535 poses, more than 2^30
+    pixels (a 40000^2 detector would overflow 32-bit indices), more than
+    2^31 - 1 pixel-poses, 2^32 atoms."""
+    import ctypes
+    from mosaic_amd import _lib
+    lib = _lib.lib()
+    n = ctypes.c_size_t(0)
+    counts = (ctypes.c_int64 * 1)(1000)
+
+    def ws(atoms, frames, ny, nz):
+        counts[0] = atoms
+        return lib.mosaic_kin_workspace(atoms, 1, counts, frames, ny, nz, ctypes.byref(n))
+
+    assert ws(1000, 64, 256, 256) == 0 and n.value > 0
+    assert ws(1000, 65535, 8, 8) == 0
+    assert ws(1000, 65536, 8, 8) == _lib.MOSAIC_ERR_ARGS
+    assert ws(1000, 1, 32768, 32768) == 0                   # 2^30 pixels
+    assert ws(1000, 1, 40000, 40000) == _lib.MOSAIC_ERR_ARGS
+    assert ws(1 << 32, 1, 8, 8) == _lib.MOSAIC_ERR_ARGS
+    assert ws(1000, 65535, 256, 256) == _lib.MOSAIC_ERR_ARGS    # 4.3e9 pixel-poses
