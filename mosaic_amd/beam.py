@@ -211,6 +211,15 @@ class beam:
             code_arrs.append(codes)
             off += n
         view_of = getattr(sample, "chunk_positions_view", None)
+        pinned_of = getattr(sample, "pinned_chunk", None)
+        # pinned chunks whose codes are already global go by DMA, no host copy
+        direct = {}
+        if pinned_of is not None:
+            for k, cid in enumerate(chunk_ids):
+                if sizes[k] and maps[k] is None:
+                    pc = pinned_of(cid)
+                    if pc is not None:
+                        direct[k] = pc
 
         def load(k):
             """Fill chunk k's slices of the pinned buffers; returns its
@@ -238,22 +247,39 @@ class beam:
         if workers is None:
             workers = ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1)))
             self._loader = workers
-        futures = [workers.submit(load, k) for k in range(len(chunk_ids))]
         counts = np.zeros(16, dtype=np.int64)
+        if direct:
+            with torch.cuda.stream(copy):
+                for k, (tp, tc, cnt) in direct.items():
+                    o, n = offs[k], sizes[k]
+                    d_pos[o:o + n].copy_(tp, non_blocking=True)
+                    d_cod[o:o + n].copy_(tc, non_blocking=True)
+                    counts[:len(cnt)] += cnt
+        futures = [None if k in direct else workers.submit(load, k)
+                   for k in range(len(chunk_ids))]
         # copies of runs of loaded chunks, about an eighth of the atoms each
         batch = max(1, total // 8)
         lo = 0
         hp2 = hp[:3 * total].view(total, 3) if total else None
         for k, fut in enumerate(futures):
+            if fut is None:                     # copied by DMA above
+                hi = offs[k]
+                if hi > lo:                     # the host-loaded run before it
+                    with torch.cuda.stream(copy):
+                        d_pos[lo:hi].copy_(hp2[lo:hi], non_blocking=True)
+                        d_cod[lo:hi].copy_(hc[lo:hi], non_blocking=True)
+                lo = offs[k] + sizes[k]
+                continue
             try:
                 c = fut.result()
             except BaseException:
                 # no worker may still write into the pinned buffers, which the
                 # next call reuses
                 for f in futures:
-                    f.cancel()
+                    if f is not None:
+                        f.cancel()
                 for f in futures:
-                    if not f.cancelled():
+                    if f is not None and not f.cancelled():
                         try:
                             f.result()
                         except BaseException:

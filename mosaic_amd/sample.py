@@ -344,6 +344,43 @@ class sample:
             return np.load(self.chunk_position_path(chunk_number))
         return np.array(self._pos[chunk_number - 1], copy=True)
 
+    def pin_memory(self):
+        """Move the in-memory chunks into page-locked host memory (one block
+        for the positions, one for the species codes) so that the drop-in
+        copies them to the device by DMA, without a host copy.  Chunk files
+        are not affected.  Returns the sample."""
+        if self._on_disk or not self._pos or getattr(self, "_pinned", None) is not None:
+            return self
+        import torch
+        n = sum(int(p.shape[0]) for p in self._pos)
+        pos = torch.empty((max(n, 1), 3), dtype=torch.float32, pin_memory=True)
+        cod = torch.empty(max(n, 1), dtype=torch.uint8, pin_memory=True)
+        pn, cn = pos.numpy(), cod.numpy()
+        views_p, views_c, counts = [], [], []
+        off = 0
+        for p, c in zip(self._pos, self._codes):
+            k = int(p.shape[0])
+            pn[off:off + k] = p
+            cn[off:off + k] = c
+            views_p.append(pn[off:off + k])
+            views_c.append(cn[off:off + k])
+            counts.append(np.bincount(c, minlength=max(1, len(self._species_labels))))
+            off += k
+        self._pos, self._codes = views_p, views_c
+        self._pinned = (pos, cod, counts)
+        return self
+
+    def pinned_chunk(self, chunk_number):
+        """(positions (N, 3) float32, codes (N,) uint8 -- pinned torch
+        tensors, per-species counts of the chunk's codes) when the sample is
+        pinned and nothing applies on load; None otherwise."""
+        pin = getattr(self, "_pinned", None)
+        if pin is None or self.enable_temp:
+            return None
+        p, c = self._pos[chunk_number - 1], self._codes[chunk_number - 1]
+        import torch
+        return torch.from_numpy(p), torch.from_numpy(c), pin[2][chunk_number - 1]
+
     def chunk_positions_view(self, chunk_number):
         """The chunk's positions without a copy (the stored array, or a
         memory map of the chunk file) when nothing applies on load (no

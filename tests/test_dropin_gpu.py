@@ -129,3 +129,26 @@ def test_chunk_files_full_perovskite(device, tmp_path):
     e = rel(A_file.ravel()[sel], R)
     print(f"[dropin] 64 chunk files: 32 pixels vs FP64 oracle rel {e:.2e}", flush=True)
     assert e <= TOL
+
+
+def test_pinned_sample_same_field(device):
+    """A sample moved into page-locked memory (sample.pin_memory(): chunks
+    copied by DMA, no host copy) gives the same field bit for bit; with a
+    thermal draw on load its chunks take the host path again."""
+    w = W.perovskite_screw_bragg200(n=12, npix=32)
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    A0 = b.interact_beam_gpu(s, d.pixel_coordinates, d.shape, st, spherical_decay=False)
+    s.pin_memory()
+    assert s.pinned_chunk(1) is not None
+    A1 = b.interact_beam_gpu(s, d.pixel_coordinates, d.shape, st, spherical_decay=False)
+    assert np.array_equal(A0, A1)
+    s.set_temperature("gaussian", 0.05, None, 3)
+    assert s.pinned_chunk(1) is None
+    A2 = b.interact_beam_gpu(s, d.pixel_coordinates, d.shape, st, spherical_decay=False)
+    assert not np.array_equal(A2, A1)
+    w2 = W.fcc_au_bragg111(n=6, npix=16, seed=0, sigma=0.0)
+    b2, s2, d2, st2 = w2["beam"], w2["sample"], w2["detector"], w2["stage"]
+    B0 = b2.interact_beam_gpu(s2, d2.pixel_coordinates, d2.shape, st2)
+    s2.pin_memory()
+    B1 = b2.interact_beam_gpu(s2, d2.pixel_coordinates, d2.shape, st2)
+    assert np.array_equal(B0, B1)
