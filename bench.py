@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--no-surrogate-inference", action="store_true")
+    ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--surrogate", action="store_true",
                     help="run the surrogate workload (data generation, training, gated scan)")
     ap.add_argument("--sg-images", type=int, default=20000)
@@ -190,6 +191,88 @@ def accuracy(w, field, n_pix, seed=1):
                     threads=host_cpu()[0])
     A = field.ravel()[sel]
     return float(np.linalg.norm(A - R) / np.linalg.norm(R))
+
+
+def bench_exact(reps=3, n_err=24):
+    """The exact per-pair forms on the surrogate workload's crystal (FCC Au,
+    37^3 cells, 202,612 atoms) and a 128^2 detector at its (111) Bragg angle:
+    the staged sum forced onto the exact kernel (the reference's general
+    kernel, interaction_kernal, in its exact mode) and mosaic_scatter
+    (compute_scattering_cffi on the device: lab positions in FP64 metres,
+    per-atom table rows), each with its error against the FP64 oracle on
+    random pixels.  Inputs resident in HBM; HIP events around each call."""
+    import torch
+    from mosaic_amd import kinematic as K
+    from mosaic_amd import workloads as W
+    from oracle import kinematic as O
+    w = W.fcc_fault_scan111(cells=(37, 37, 37), npix=128, n_angles=1, fault=False)
+    b, smp, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    a = W.oracle_inputs(w)
+    n = a["pos_m"].shape[0]
+    Ny, Nz = a["Ny"], a["Nz"]
+    dev = torch.device("cuda", 0)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    # staged sum, exact kernel
+    b._kin_path = "exact"
+    rot = np.asarray(st.rotation, dtype=np.float32)[None]
+    tr = np.asarray(st.translation, dtype=np.float32)[None]
+    f = b._kin_field(smp, d.pixel_coordinates, d.shape, st, (rot, tr), False, True, False)
+    torch.cuda.synchronize()
+    field = f[0].cpu().numpy().ravel()
+    # time the sum alone: the staged sample of the call above
+    pos = torch.as_tensor(np.concatenate([smp.load_chunk_positions(c) for c in
+                                          range(1, smp.chunk_total + 1)]), device=dev)
+    codes = torch.zeros(n, dtype=torch.uint8, device=dev)
+    import mosaic_amd.tables as T
+    spc = K.species_table(T.species_constants(["Au"], b.energy))
+    opts = K.kin_opts(a["k_val"], False, True, b._polarization_state(), False, path="exact")
+    staged = K.stage(pos, codes, None, np.array([n]), a["k_val"], 1, Ny, Nz)
+    coords = torch.as_tensor(np.ascontiguousarray(np.asarray(d.pixel_coordinates) / 1e10),
+                             device=dev)
+    out = torch.zeros((1, Nz, Ny), dtype=torch.complex64, device=dev)
+    K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+    ev[0].record()
+    for _ in range(reps):
+        K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+    ev[1].record()
+    # mosaic_scatter on the oracle's own per-atom arrays
+    t = {k: torch.as_tensor(np.ascontiguousarray(a[k]), device=dev)
+         for k in ("pos_m", "f0p", "f0z", "sanr", "sani", "amp_r", "amp_i", "cx", "cy", "cz")}
+    t["pos_m"] = t["pos_m"].double()
+    for k in ("f0p", "f0z", "sanr", "sani", "amp_r", "amp_i"):
+        t[k] = t[k].float()
+    for k in ("cx", "cy", "cz"):
+        t[k] = t[k].double()
+    o_r = torch.zeros(Ny * Nz, dtype=torch.float32, device=dev)
+    o_i = torch.zeros_like(o_r)
+    args = (t["pos_m"], t["f0p"], t["f0z"], t["sanr"], t["sani"], t["amp_r"], t["amp_i"],
+            t["cx"], t["cy"], t["cz"], a["k_val"], False, Ny, Nz, True, a["pol_deg"],
+            a["pol_e"], False)
+    K.scatter(*args, o_r, o_i)
+    torch.cuda.synchronize()
+    sc_field = o_r.cpu().numpy().astype(np.float64) + 1j * o_i.cpu().numpy()
+    ev[2].record()
+    for _ in range(reps):
+        K.scatter(*args, o_r, o_i)
+    ev[3].record()
+    ev[3].synchronize()
+    t_ex = ev[0].elapsed_time(ev[1]) / reps
+    t_sc = ev[2].elapsed_time(ev[3]) / reps
+    rng = np.random.default_rng(5)
+    sel = rng.choice(Ny * Nz, size=n_err, replace=False)
+    R = O.kin_sum_c(a["pos_m"], a["f0p"], a["f0z"], a["sanr"], a["sani"], a["amp_r"],
+                    a["amp_i"], a["cx"][sel], a["cy"][sel], a["cz"][sel], a["k_val"], False,
+                    n_err, 1, True, a["pol_deg"], a["pol_e"], False, precision="f64",
+                    threads=host_cpu()[0])
+    pairs = float(n) * Ny * Nz
+    return {"workload": "fcc_au_37c_128px (202,612 atoms x 128^2 pixels)",
+            "exact_kernel_ms": t_ex, "exact_kernel_pairs_per_s": pairs / (t_ex * 1e-3),
+            "exact_rel_err_vs_fp64_oracle": float(np.linalg.norm(field[sel] - R) /
+                                                  np.linalg.norm(R)),
+            "scatter_ms": t_sc, "scatter_pairs_per_s": pairs / (t_sc * 1e-3),
+            "scatter_rel_err_vs_fp64_oracle": float(np.linalg.norm(sc_field[sel] - R) /
+                                                    np.linalg.norm(R)),
+            "rel_err_sample": f"{n_err} random pixels, all atoms"}
 
 
 PROP_N = 2048           # exit-wave grid
@@ -766,6 +849,8 @@ def main():
         res["adjoint"] = bench_adjoint(1)
     if not args.no_surrogate_inference:
         res["surrogate_inference"] = bench_surrogate_inference(max(3, args.steps))
+    if not args.no_exact:
+        res["exact"] = bench_exact()
     if args.err_pixels > 0:
         res["rel_err_vs_fp64_oracle"] = accuracy(w, field, args.err_pixels)
         res["rel_err_pixels"] = args.err_pixels
