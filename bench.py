@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-pixels", type=int, default=48)
     ap.add_argument("--path", default="auto",
-                    choices=["auto", "tile", "rows", "mma", "poly", "exact"])
+                    choices=["auto", "tile", "rows", "mma", "poly", "gemm", "exact"])
     ap.add_argument("--no-propagation", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -496,7 +496,7 @@ def kernel_name(path_used):
     """Sum kernel behind a mosaic_kin_sum path code."""
     base = path_used % 100
     return {32: "k_sum_mma", 24: "k_sum_rec", 28: "k_sum_rec", 45: "k_sum_poly",
-            46: "k_sum_poly", 50: "k_sum_exact", 51: "k_sum_exact",
+            46: "k_sum_poly", 76: "k_gm_moments", 50: "k_sum_exact", 51: "k_sum_exact",
             52: "k_sum_exact"}.get(base, "k_sum")
 
 

@@ -57,6 +57,9 @@ constexpr int TARGET_BLOCKS = 2048;  // workgroups the sum grid aims for
 constexpr int MMA_P = 32;            // pixel patch edge of the matrix-core sum
 constexpr int MMA_WAVES = 4;         // waves (patches) per matrix-core workgroup
 constexpr int N_GEOM = 8;            // pixel geometry statistics (mosaic_kin_pixels)
+constexpr int GM_MOM = 28;           // moments per (unit, patch) of the matrix-core moment sum
+constexpr int GM_REC = 32;           // float2 per (unit, pose, patch) record: the moments, phase terms
+constexpr int64_t GM_MAX_BYTES = int64_t(3) << 29;   // its moment buffer: at most 1.5 GiB
 // The last k-d levels run inside one workgroup per node: a node of at most
 // KDL_MAX atoms sits in LDS and is bisected up to KDL_LEVELS times there
 // (k_kd_local<LV>, 32 << LV threads, for nodes of up to TILE << LV atoms).
@@ -177,6 +180,9 @@ struct Layout {
     float* amax_blocks;     // adjoint: per-block max |dL/dA| and max |G|
     FrameRec* frames;
     double2* partial;
+    float2* gm_mom;         // moment buffer of the matrix-core moment sum
+    uint8_t* gm_head;       // [tile] its unit's size if it heads one, else 0
+    size_t gm_count;        // its float2 elements
     size_t total;
     int n_tiles;
     int n_slices;
@@ -322,7 +328,8 @@ void kdg_tables(int n_species, const int64_t* counts, std::vector<KdgNode>& node
 }
 
 int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* counts,
-                int n_frames, int npix, Layout& L) {
+                int n_frames, int Ny, int Nz, Layout& L) {
+    const int npix = Ny * Nz;
     Carver c(base);
     take_pixel_tables(c, npix, n_species, L.px);
     L.n_tiles = count_tiles(n_species, counts);
@@ -383,6 +390,17 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.tiles_per_slice = (std::max(1, L.n_tiles) + slices - 1) / slices;
     L.n_slices = (std::max(1, L.n_tiles) + L.tiles_per_slice - 1) / L.tiles_per_slice;
     L.partial = c.take<double2>(static_cast<size_t>(L.n_slices) * std::max(1, n_frames) * npix);
+    // the matrix-core moment sum's buffer: a GM_REC record per (unit head, pose,
+    // 32 x 32 patch), for every tile if it fits, else for rounds of tiles;
+    // and the unit size of every head tile (0 elsewhere)
+    {
+        const int64_t patches = static_cast<int64_t>((Ny + MMA_P - 1) / MMA_P) * ((Nz + MMA_P - 1) / MMA_P);
+        const int64_t per_tile = patches * std::max(1, n_frames) * GM_REC;
+        const int64_t cap = std::max<int64_t>(per_tile, GM_MAX_BYTES / static_cast<int64_t>(sizeof(float2)));
+        L.gm_count = static_cast<size_t>(std::min<int64_t>(per_tile * std::max(1, L.n_tiles), cap));
+        L.gm_mom = c.take<float2>(L.gm_count);
+        L.gm_head = c.take<uint8_t>(static_cast<size_t>(std::max(1, L.n_tiles)));
+    }
     L.total = c.off + 256;
     return MOSAIC_OK;
 }
@@ -2794,6 +2812,731 @@ hipError_t launch_poly(const MmaArgs& A, int n_slices, int n_frames, hipStream_t
     return hipGetLastError();
 }
 
+// Unit of tile t for the moment sums over units and the moment adjoint: the quad, else the pair, whose
+// staged radius is within rmax and which covers t, else the tile alone.
+// Heads sit at aligned places inside a k-d node and only they carry finite
+// radii, so this is the unit the moment sum's walk forms from a node start.
+__device__ __forceinline__ void unit_of(int t, int n_tiles, const float* __restrict__ grp_r,
+                                        float rmax, int& head, int& size) {
+    head = t;
+    size = 1;
+    if (!(rmax > 0.f)) return;
+    for (int h = max(0, t - 3); h <= t; ++h)
+        if (h + 3 < n_tiles && grp_r[2 * static_cast<int64_t>(h) + 1] <= rmax) {
+            head = h;
+            size = 4;
+            return;
+        }
+    for (int h = max(0, t - 1); h <= t; ++h)
+        if (h + 1 < n_tiles && grp_r[2 * static_cast<int64_t>(h)] <= rmax) {
+            head = h;
+            size = 2;
+            return;
+        }
+}
+
+// ------------------------------------------------- moment sum on the matrix cores
+// The moment expansion of k_sum_poly with its per-atom work moved to the FP16
+// matrix cores.  Per unit of tiles and per patch the moment sum needs
+//     C_mn = sum_j a_j T_mn(j),   a_j = |w_j| e^{2 pi i theta_j(patch)},
+//     T_mn = P_j^m R_j^n / (m! n!),
+// with P_j, R_j the atom offset projected on the patch's row and column
+// gradients.  Over patches that share those gradients the T_mn do not depend
+// on the patch, and the moments of 16 patches become one real matrix
+// product over the unit's atoms: rows (patch, Re or Im of a_j), columns the
+// atom's basis functions.  A wave owns a 4 x 4 group of patches and expands
+// about the group's mean gradients gy_, gz_.  Each patch's own model differs
+// from the group's by
+//     eps_j(x, y) = sum_c sigma_jc E_c(x, y),
+//     E_c = 2 pi [16 (gy - gy_)_c x + 16 (gz - gz_)_c y + 256 hy_c x^2
+//                 + 256 hz_c y^2 + 256 gyz_c x y]
+// (its largest value over a patch is a few 1e-3 rad for the largest units;
+// the host bounds it), which enters to second order:
+//     e^{i (P_ x + R_ y + eps)} = e^{i (P_ x + R_ y)} (1 + i eps - eps^2 / 2) + O(eps^3).
+// So each atom's basis row holds (sigma in wavelengths, P_ and R_ in rad)
+//     P_^m R_^n                    m + n <= 6   (28 columns)
+//     sigma_c P_^m R_^n            m + n <= 4   (45)
+//     sigma_c sigma_d {1, P_, R_}   c <= d       (18)
+// and a fold per (unit, patch) turns the 91 products into the 28
+// coefficients of the patch polynomial S(x, y) = sum M_mn x^m y^n (factorials
+// and powers of i included), written to a moment buffer.  A second kernel
+// (k_gm_eval) evaluates the polynomials at the pixels with the per-(unit,
+// pixel) origin phase, as k_sum_poly does.
+//
+// Operands: every value x is split into FP16 halves x = hi + lo (22
+// significant bits) and each product is hi hi + hi lo + lo hi with FP32
+// accumulation (v_mfma_f32_32x32x16_f16: A row = lane % 32, B column = lane
+// % 32, k = 8 (lane / 32) + 0..7; result row 8 (r / 4) + 4 (lane / 32) + r % 4).
+// The carrier a_j takes one v_sin per operand value: the Re rows add a
+// quarter turn to the phase.  Each wave builds its basis rows for 32 atoms
+// at a time into its own LDS buffer (two lanes per atom), then runs two
+// 16-atom steps of 3 x 3 products; the waves of a workgroup share nothing.
+constexpr int GM_WAVES = 4;                       // patch groups per workgroup
+constexpr int GM_GRP = 4;                         // a wave's group: GM_GRP x GM_GRP patches
+constexpr int GM_REG = 2 * GM_GRP;                // a workgroup's region: 2 x 2 groups
+constexpr int GM_CHUNK = 64;                      // atoms per basis chunk
+constexpr int GM_NT = 28;                         // P^m R^n, m + n <= 6
+constexpr int GM_N4 = 15;                         // m + n <= 4
+constexpr int GM_COL_S = GM_NT;                   // sigma_c P^m R^n: GM_COL_S + 15 c + j
+constexpr int GM_COL_SS = GM_NT + 3 * GM_N4;      // sigma_c sigma_d {1, P, R}: + 3 pair + s
+constexpr int GM_NCOL = GM_COL_SS + 18;           // 91
+constexpr int GM_NB = 3;                          // 32-column blocks
+constexpr int GM_BS = 72;                         // halves per basis column: 64 atoms + pad (144 B)
+constexpr int GM_CS = 92;                         // floats per product row at a unit's end
+static_assert(GM_NCOL <= 32 * GM_NB, "basis columns fit the blocks");
+static_assert(GM_CS >= GM_NCOL, "a product row holds every basis column");
+
+__host__ __device__ constexpr int gm_kT(int m, int n) { return 7 * m - m * (m - 1) / 2 + n; }
+__host__ __device__ constexpr int gm_k4(int m, int n) { return 5 * m - m * (m - 1) / 2 + n; }
+
+struct GmArgs {
+    const AtomRec* atoms;
+    const TileRec* tiles;
+    const FrameRec* frames;
+    const double4* u0h;
+    const double* theta0;
+    const float4* dup;
+    const float2* gmma;
+    double2* partial;
+    float2* mom;              // [head tile - t_begin][frame][patch][GM_REC]
+    const uint8_t* heads;     // [tile] unit size of a head tile, else 0
+    const float* grp_r;
+    float grp_rmax;
+    int n_tiles;
+    int t_begin;              // this round's unit heads: [t_begin, t_end)
+    int t_end;
+    int a_tiles_per_slice;    // k_gm_moments' tile slices
+    int tiles_per_slice;      // k_gm_eval: the slices of the partials
+    int n_slices;
+    int n_frames;
+    int Ny;
+    int Nz;
+    int npix;
+    int pby;                  // patches along a row / a column
+    int pbz;
+    int n_patches;
+    int ngy;                  // regions of GM_REG x GM_REG patches along a row
+    int n_groups;             // regions (one workgroup each)
+    int accumulate;           // k_gm_eval adds to the partials (rounds after the first)
+    double K;
+};
+
+// LDS written by some lanes of a wave and read by others of the same wave:
+// the wave's LDS operations complete in order; wait for them (lgkmcnt(0),
+// vector memory left in flight) and keep the compiler from moving memory
+// operations across.
+__device__ __forceinline__ void gm_wave_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Unit size of every head tile (0 for the others), from unit_of.
+__global__ void k_gm_heads(const float* __restrict__ grp_r, float rmax, int n_tiles,
+                           uint8_t* __restrict__ heads) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    int head, size;
+    unit_of(t, n_tiles, grp_r, rmax, head, size);
+    heads[t] = head == t ? static_cast<uint8_t>(size) : static_cast<uint8_t>(0);
+}
+
+// Walks the unit heads of the tiles [t0, t1) in order, 64 tiles a load: each
+// lane takes one tile's unit size, a ballot marks the heads.
+struct GmHeadWalk {
+    const uint8_t* heads;
+    int base, t1, lane, hs;
+    uint64_t mask;
+    __device__ GmHeadWalk(const uint8_t* h, int t0, int t1_, int lane_)
+        : heads(h), base(t0 - 64), t1(t1_), lane(lane_), hs(0), mask(0) {}
+    // the next head tile and its unit size; false past the end
+    __device__ __forceinline__ bool next(int& t, int& size) {
+        while (mask == 0) {
+            base += 64;
+            if (base >= t1) return false;
+            const int tl = base + lane;
+            hs = tl < t1 ? heads[tl] : 0;
+            mask = __ballot(hs != 0);
+        }
+        const int i = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        t = base + i;
+        size = __builtin_amdgcn_readlane(hs, i);
+        return true;
+    }
+};
+
+typedef __fp16 halfx2 __attribute__((ext_vector_type(2)));
+typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
+
+// Two operand values into their hi and lo words (value 0 in the low halves).
+__device__ __forceinline__ void gm_split_pair(float x0, float x1, unsigned int& hi,
+                                             unsigned int& lo) {
+    const halfx2 h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+    const halfx2 l = __builtin_amdgcn_cvt_pkrtz(x0 - static_cast<float>(h[0]),
+                                                x1 - static_cast<float>(h[1]));
+    hi = __builtin_bit_cast(unsigned int, h);
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
+// (re, im) times i^k
+__device__ __forceinline__ void gm_rot(int k, float& re, float& im) {
+    const float r = re, i = im;
+    switch (k & 3) {
+        case 0: break;
+        case 1: re = -i; im = r; break;
+        case 2: re = -r; im = -i; break;
+        default: re = i; im = -r; break;
+    }
+}
+
+__device__ __forceinline__ float gm_inv_fact(int m) {
+    return m == 0 ? 1.f : m == 1 ? 1.f : m == 2 ? 0.5f : m == 3 ? (1.f / 6.f) :
+           m == 4 ? (1.f / 24.f) : m == 5 ? (1.f / 120.f) : (1.f / 720.f);
+}
+
+// The fold of one patch: the 91 products (Re in row p, Im in row 16 + p of
+// cb) into the 28 polynomial coefficients, written to dst with the unit's
+// phase terms at the patch (pht, K O_x; K O_y, K O_z) and its species.  Ev[5 c .. 5 c + 4]
+// hold the patch's model difference E_c: x, y, x^2, y^2, x y coefficients [rad].
+__device__ __forceinline__ void gm_fold(const float* cb, int p, const float* __restrict__ Ev,
+                                        float2 ph0, float2 ph1, int species,
+                                        float2* __restrict__ dst) {
+    float E[3][5];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int u = 0; u < 5; ++u) E[c][u] = Ev[5 * c + u];
+    const float* cr = cb + p * GM_CS;
+    const float* ci = cb + (16 + p) * GM_CS;
+    float Mr[GM_MOM], Mi[GM_MOM];
+#pragma unroll
+    for (int m = 0; m <= 6; ++m) {
+#pragma unroll
+        for (int n = 0; n <= 6 - m; ++n) {
+            const int k = gm_kT(m, n);
+            const float f = gm_inv_fact(m) * gm_inv_fact(n);
+            float re = cr[k] * f, im = ci[k] * f;
+            gm_rot(m + n, re, im);
+            Mr[k] = re;
+            Mi[k] = im;
+        }
+    }
+    // first order: i E_c(x, y) sum_mn i^{m+n} / (m! n!) C[sigma_c P^m R^n] x^m y^n
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int m = 0; m <= 4; ++m) {
+#pragma unroll
+            for (int n = 0; n <= 4 - m; ++n) {
+                const int col = GM_COL_S + GM_N4 * c + gm_k4(m, n);
+                const float f = gm_inv_fact(m) * gm_inv_fact(n);
+                float vr = cr[col] * f, vi = ci[col] * f;
+                gm_rot(m + n + 1, vr, vi);
+                const int k10 = gm_kT(m + 1, n), k01 = gm_kT(m, n + 1), k20 = gm_kT(m + 2, n),
+                          k02 = gm_kT(m, n + 2), k11 = gm_kT(m + 1, n + 1);
+                Mr[k10] = fmaf(E[c][0], vr, Mr[k10]); Mi[k10] = fmaf(E[c][0], vi, Mi[k10]);
+                Mr[k01] = fmaf(E[c][1], vr, Mr[k01]); Mi[k01] = fmaf(E[c][1], vi, Mi[k01]);
+                Mr[k20] = fmaf(E[c][2], vr, Mr[k20]); Mi[k20] = fmaf(E[c][2], vi, Mi[k20]);
+                Mr[k02] = fmaf(E[c][3], vr, Mr[k02]); Mi[k02] = fmaf(E[c][3], vi, Mi[k02]);
+                Mr[k11] = fmaf(E[c][4], vr, Mr[k11]); Mi[k11] = fmaf(E[c][4], vi, Mi[k11]);
+            }
+        }
+    }
+    // second order: -1/2 E_c E_d (C[s_c s_d] + i C[s_c s_d P] x + i C[s_c s_d R] y)
+    constexpr int DM[5] = {1, 0, 2, 0, 1}, DN[5] = {0, 1, 0, 2, 1};
+    int pair = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int d = c; d < 3; ++d, ++pair) {
+            const float sc = c == d ? -0.5f : -1.f;
+            float e2[GM_MOM];
+#pragma unroll
+            for (int k = 0; k < GM_MOM; ++k) e2[k] = 0.f;
+#pragma unroll
+            for (int u = 0; u < 5; ++u)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    const int k = gm_kT(DM[u] + DM[v], DN[u] + DN[v]);
+                    e2[k] = fmaf(E[c][u] * sc, E[d][v], e2[k]);
+                }
+            const int col = GM_COL_SS + 3 * pair;
+            const float w0r = cr[col], w0i = ci[col];
+            const float w1r = -ci[col + 1], w1i = cr[col + 1];     // i C[.. P]
+            const float w2r = -ci[col + 2], w2i = cr[col + 2];     // i C[.. R]
+#pragma unroll
+            for (int m = 0; m <= 4; ++m) {
+#pragma unroll
+                for (int n = 0; n <= 4 - m; ++n) {
+                    if (m + n < 2) continue;
+                    const float e = e2[gm_kT(m, n)];
+                    const int k0 = gm_kT(m, n), k1 = gm_kT(m + 1, n), k2 = gm_kT(m, n + 1);
+                    Mr[k0] = fmaf(e, w0r, Mr[k0]); Mi[k0] = fmaf(e, w0i, Mi[k0]);
+                    Mr[k1] = fmaf(e, w1r, Mr[k1]); Mi[k1] = fmaf(e, w1i, Mi[k1]);
+                    Mr[k2] = fmaf(e, w2r, Mr[k2]); Mi[k2] = fmaf(e, w2i, Mi[k2]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < GM_MOM; ++k) dst[k] = make_float2(Mr[k], Mi[k]);
+    dst[GM_MOM] = ph0;
+    dst[GM_MOM + 1] = ph1;
+    dst[GM_MOM + 2] = make_float2(__int_as_float(species), 0.f);
+    dst[GM_MOM + 3] = make_float2(0.f, 0.f);
+}
+
+// Phase terms of a unit with origin (uox, uoy, uoz) [m, sample frame] at a
+// patch's middle pixel (the direction um, Theta0 th_mid, direction-model
+// centre vc0): the carrier direction v (sample frame, first order in the
+// origin's offset), K times the lab origin and the origin phase at the
+// middle pixel, as k_sum_poly forms them.
+struct GmUnitPhase {
+    float vx, vy, vz, kox, koy, koz, pht;
+};
+__device__ __forceinline__ GmUnitPhase gm_unit_phase(const FrameRec& F, double uox, double uoy,
+                                                     double uoz, const double4& um,
+                                                     double th_mid, const double (&vc0)[3],
+                                                     double K) {
+    const double Ox = fma(F.R[0], uox, fma(F.R[1], uoy, fma(F.R[2], uoz, F.T[0])));
+    const double Oy = fma(F.R[3], uox, fma(F.R[4], uoy, fma(F.R[5], uoz, F.T[1])));
+    const double Oz = fma(F.R[6], uox, fma(F.R[7], uoy, fma(F.R[8], uoz, F.T[2])));
+    const double O2 = Ox * Ox + Oy * Oy + Oz * Oz;
+    const double uo = um.x * Ox + um.y * Oy + um.z * Oz;
+    double ph_t = th_mid + K * (Ox - uo + (O2 - uo * uo) * um.w);
+    ph_t -= floor(ph_t);
+    const double h2 = 2.0 * um.w;
+    const double cxl = (Ox - uo * um.x) * h2, cyl = (Oy - uo * um.y) * h2,
+                 czl = (Oz - uo * um.z) * h2;
+    GmUnitPhase r;
+    r.vx = static_cast<float>(vc0[0] + (F.R[0] * cxl + F.R[3] * cyl + F.R[6] * czl));
+    r.vy = static_cast<float>(vc0[1] + (F.R[1] * cxl + F.R[4] * cyl + F.R[7] * czl));
+    r.vz = static_cast<float>(vc0[2] + (F.R[2] * cxl + F.R[5] * cyl + F.R[8] * czl));
+    r.kox = static_cast<float>(K * Ox);
+    r.koy = static_cast<float>(K * Oy);
+    r.koz = static_cast<float>(K * Oz);
+    r.pht = static_cast<float>(ph_t);
+    return r;
+}
+
+// Origin of the unit headed by tile t with gsz tiles.
+__device__ __forceinline__ void gm_unit_origin(const TileRec* __restrict__ tiles, int t, int gsz,
+                                               const TileRec& tr, double& ox, double& oy,
+                                               double& oz) {
+    ox = tr.ox;
+    oy = tr.oy;
+    oz = tr.oz;
+    if (gsz > 1) {
+        const TileRec b1 = tiles[t + 1];
+        const TileRec b2 = tiles[gsz == 4 ? t + 2 : t];
+        const TileRec b3 = tiles[gsz == 4 ? t + 3 : t];
+        ox = unit_origin(gsz, tr.ox, b1.ox, b2.ox, b3.ox);
+        oy = unit_origin(gsz, tr.oy, b1.oy, b2.oy, b3.oy);
+        oz = unit_origin(gsz, tr.oz, b1.oz, b2.oz, b3.oz);
+    }
+}
+
+// The patch's direction model (sample frame) as k_sum_poly fits it: centre
+// vc0 and per pixel the row and column slopes g, curvatures h and the mixed
+// term gyz.
+__device__ __forceinline__ void gm_patch_model(const FrameRec& F, const double4* __restrict__ u0h,
+                                               const PatchGeom& P, int Ny, double (&vc0)[3],
+                                               double (&gy)[3], double (&hy)[3], double (&gz)[3],
+                                               double (&hz)[3], double (&gyz)[3]) {
+    const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+    const int pc = pmid - P.my - P.mz * Ny;
+    const int samp[9] = {pmid, pmid - P.my, pmid - P.my + P.vy - 1, pmid - P.mz * Ny,
+                         pmid + (P.vz - 1 - P.mz) * Ny, pc, pc + P.vy - 1,
+                         pc + (P.vz - 1) * Ny, pc + P.vy - 1 + (P.vz - 1) * Ny};
+    double vs[9][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double4 u = u0h[samp[k]];
+        const double wx = 1.0 - u.x, wy = -u.y, wz = -u.z;
+        vs[k][0] = F.R[0] * wx + F.R[3] * wy + F.R[6] * wz;
+        vs[k][1] = F.R[1] * wx + F.R[4] * wy + F.R[7] * wz;
+        vs[k][2] = F.R[2] * wx + F.R[5] * wy + F.R[8] * wz;
+    }
+    const double area = static_cast<double>(P.vy - 1) * static_cast<double>(P.vz - 1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        vc0[d] = vs[0][d];
+        quad_fit(vs[1][d] - vs[0][d], vs[2][d] - vs[0][d], -P.my, P.vy - 1 - P.my, gy[d], hy[d]);
+        quad_fit(vs[3][d] - vs[0][d], vs[4][d] - vs[0][d], -P.mz, P.vz - 1 - P.mz, gz[d], hz[d]);
+        gyz[d] = area > 0.0 ? (vs[8][d] - vs[7][d] - vs[6][d] + vs[5][d]) / area : 0.0;
+    }
+}
+
+template <bool SECOND>
+__global__ __launch_bounds__(64 * GM_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
+void k_gm_moments(const GmArgs A) {
+    __shared__ __align__(16) _Float16 sbh[GM_NB * 32 * GM_BS];   // basis hi halves [column][atom]
+    __shared__ __align__(16) _Float16 sbl[GM_NB * 32 * GM_BS];   // basis lo halves
+    __shared__ float4 sat[GM_CHUNK];                             // offset [wavelengths], phase [turns]
+    __shared__ __align__(16) float scb[GM_WAVES][32 * GM_CS];    // each wave's products at a unit's end
+    __shared__ float sev[GM_WAVES][16][15];                      // each patch's E_c coefficients
+    __shared__ double sgs[GM_WAVES][8];                          // the waves' gradient sums
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int frame = blockIdx.z, slice = blockIdx.y;
+    const FrameRec& F = A.frames[frame];
+    const int g = lane >> 5, q = lane & 31, pl = lane & 15;
+    // this workgroup's region of GM_REG x GM_REG patches, this wave's 4 x 4 group in it
+    const int rgn = blockIdx.x;
+    const int gy0 = (rgn % A.ngy) * GM_REG + GM_GRP * (wave & 1);
+    const int gz0 = (rgn / A.ngy) * GM_REG + GM_GRP * (wave >> 1);
+    const int ppy = gy0 + (pl & 3), ppz = gz0 + (pl >> 2);
+    const bool plive = ppy < A.pby && ppz < A.pbz;
+    const PatchGeom P = patch_of(min(ppy, A.pby - 1), min(ppz, A.pbz - 1), A.Ny, A.Nz);
+    const int pmid = (P.z0 + P.mz) * A.Ny + P.y0 + P.my;
+
+    // this lane's patch model; the region's mean gradients over its live
+    // patches (lanes 0..15 of each wave hold its patches once)
+    double vc0[3], gyv[3], hyv[3], gzv[3], hzv[3], gyzv[3];
+    gm_patch_model(F, A.u0h, P, A.Ny, vc0, gyv, hyv, gzv, hzv, gyzv);
+    {
+        double sg[7];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            sg[d] = plive ? gyv[d] : 0.0;
+            sg[3 + d] = plive ? gzv[d] : 0.0;
+        }
+        sg[6] = plive ? 1.0 : 0.0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) sg[k] += __shfl_xor(sg[k], o, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 7; ++k) sgs[wave][k] = sg[k];
+        // the basis pad columns stay zero
+        for (int i = tid; i < (GM_NB * 32 - GM_NCOL) * GM_BS; i += 64 * GM_WAVES) {
+            sbh[GM_NCOL * GM_BS + i] = static_cast<_Float16>(0.f);
+            sbl[GM_NCOL * GM_BS + i] = static_cast<_Float16>(0.f);
+        }
+    }
+    __syncthreads();
+    double mg[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < GM_WAVES; ++w)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) mg[k] += sgs[w][k];
+    const double inv_n = 1.0 / mg[6];             // the region's first patch is live
+    constexpr double TP16 = 2.0 * PI * 16.0, TP256 = 2.0 * PI * 256.0;
+    float gby[3], gbz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double my = mg[d] * inv_n, mz = mg[3 + d] * inv_n;
+        gby[d] = static_cast<float>(TP16 * my);
+        gbz[d] = static_cast<float>(TP16 * mz);
+        if (lane < 16) {
+            float* ev = sev[wave][lane] + 5 * d;
+            ev[0] = static_cast<float>(TP16 * (gyv[d] - my));
+            ev[1] = static_cast<float>(TP16 * (gzv[d] - mz));
+            ev[2] = static_cast<float>(TP256 * hyv[d]);
+            ev[3] = static_cast<float>(TP256 * hzv[d]);
+            ev[4] = static_cast<float>(TP256 * gyzv[d]);
+        }
+    }
+    const double4 um = A.u0h[pmid];
+    const double th_mid = A.theta0[pmid];
+    const float xc0 = static_cast<float>(F.xs[0] - vc0[0]);
+    const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
+    const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
+    const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
+    const float shift = q < 16 ? 0.25f : 0.f;      // Re rows: cos = sin of a quarter turn more
+    float* const cbuf = scb[wave];
+
+    const int t0 = A.t_begin + slice * A.a_tiles_per_slice;
+    const int t1 = min(A.t_end, t0 + A.a_tiles_per_slice);
+    GmHeadWalk walk(A.heads, t0, t1, lane);      // every wave walks the same units
+    int t, gsz;
+    while (walk.next(t, gsz)) {
+        const TileRec tr = A.tiles[t];
+        double uox, uoy, uoz;
+        gm_unit_origin(A.tiles, t, gsz, tr, uox, uoy, uoz);
+        const GmUnitPhase U = gm_unit_phase(F, uox, uoy, uoz, um, th_mid, vc0, A.K);
+        floatx16 c0, c1, c2;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) { c0[r] = 0.f; c1[r] = 0.f; c2[r] = 0.f; }
+        // the unit's 64-atom chunks in order, each chunk's records loaded one
+        // chunk ahead (every wave loads all 64: a lane builds one atom's
+        // share of the basis)
+        int u = 0, cb0 = 0;
+        TileRec tu = tr;
+        AtomRec cur = A.atoms[static_cast<int64_t>(t) * TILE + lane];
+        while (true) {
+            const float dx = static_cast<float>((tu.ox - uox) * A.K);
+            const float dy = static_cast<float>((tu.oy - uoy) * A.K);
+            const float dz = static_cast<float>((tu.oz - uoz) * A.K);
+            const int cnt = tu.count & 0xFFFF;
+            int nu = u, nc = cb0 + GM_CHUNK;
+            if (nc >= cnt) { nu = u + 1; nc = 0; }
+            const bool more = nu < gsz;
+            const AtomRec nxt = A.atoms[static_cast<int64_t>(t + (more ? nu : u)) * TILE +
+                                        (more ? nc : cb0) + lane];
+            // the chunk's basis rows, each value times the atom's |w| (the
+            // carrier operand is then a plain sine): wave w writes the
+            // columns f_w P^m R^n (m + n <= 4), f = 1, sigma_x, sigma_y,
+            // sigma_z, and eight of the rest (records past the count are
+            // zero: weight 0)
+            {
+                const float sx = cur.sx + dx, sy = cur.sy + dy, sz = cur.sz + dz;
+                if (wave == 0) sat[lane] = make_float4(sx, sy, sz, cur.ph);
+                const float Pb = fmaf(gby[2], sz, fmaf(gby[1], sy, gby[0] * sx));
+                const float Rb = fmaf(gbz[2], sz, fmaf(gbz[1], sy, gbz[0] * sx));
+                float pw[7], rw[7];
+                pw[0] = cur.mag;
+                rw[0] = 1.f;
+#pragma unroll
+                for (int m = 1; m <= 6; ++m) {
+                    pw[m] = pw[m - 1] * Pb;
+                    rw[m] = rw[m - 1] * Rb;
+                }
+                const float f = wave == 0 ? 1.f : (wave == 1 ? sx : (wave == 2 ? sy : sz));
+                auto put = [&](int col, float v) {
+                    const _Float16 h = static_cast<_Float16>(v);
+                    sbh[col * GM_BS + lane] = h;
+                    sbl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
+                };
+#pragma unroll
+                for (int m = 0; m <= 4; ++m)
+#pragma unroll
+                    for (int n = 0; n <= 4 - m; ++n) {
+                        const int col = wave == 0 ? gm_kT(m, n)
+                                                  : GM_COL_S + GM_N4 * (wave - 1) + gm_k4(m, n);
+                        put(col, f * (pw[m] * rw[n]));
+                    }
+                // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
+                // waves 0..3 take eight each (the last one a pad column)
+                const float ms = cur.mag;
+                const float ss[6] = {ms * sx * sx, ms * sx * sy, ms * sx * sz,
+                                     ms * sy * sy, ms * sy * sz, ms * sz * sz};
+                const float s3[3] = {1.f, Pb, Rb};
+                if (wave == 0) {
+                    put(gm_kT(0, 5), pw[0] * rw[5]); put(gm_kT(1, 4), pw[1] * rw[4]);
+                    put(gm_kT(2, 3), pw[2] * rw[3]); put(gm_kT(3, 2), pw[3] * rw[2]);
+                    put(gm_kT(4, 1), pw[4] * rw[1]); put(gm_kT(5, 0), pw[5] * rw[0]);
+                    put(gm_kT(0, 6), pw[0] * rw[6]); put(gm_kT(1, 5), pw[1] * rw[5]);
+                } else if (wave == 1) {
+                    put(gm_kT(2, 4), pw[2] * rw[4]); put(gm_kT(3, 3), pw[3] * rw[3]);
+                    put(gm_kT(4, 2), pw[4] * rw[2]); put(gm_kT(5, 1), pw[5] * rw[1]);
+                    put(gm_kT(6, 0), pw[6] * rw[0]);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                } else if (wave == 2) {
+#pragma unroll
+                    for (int i = 3; i < 11; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                } else {
+#pragma unroll
+                    for (int i = 11; i < 18; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < GM_CHUNK / 16; ++s) {
+                uintx4 hw, lw;
+#pragma unroll
+                for (int u2 = 0; u2 < 4; ++u2) {
+                    float v[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const float4 ar = sat[16 * s + 8 * g + 2 * u2 + e];
+                        float th =
+                            fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w + shift)));
+                        if (SECOND) {
+                            const float w = fmaf(xc2, ar.z, fmaf(xc1, ar.y, xc0 * ar.x));
+                            const float s2 = fmaf(ar.z, ar.z, fmaf(ar.y, ar.y, ar.x * ar.x));
+                            th = fmaf(hh, fmaf(-w, w, s2), th);
+                        }
+                        v[e] = __builtin_amdgcn_sinf(th);
+                    }
+                    unsigned int hv, lv;
+                    gm_split_pair(v[0], v[1], hv, lv);
+                    hw[u2] = hv;
+                    lw[u2] = lv;
+                }
+                const halfx8 ah = __builtin_bit_cast(halfx8, hw);
+                const halfx8 al = __builtin_bit_cast(halfx8, lw);
+                const int boff = 16 * s + 8 * g;
+                const halfx8 bh0 = *reinterpret_cast<const halfx8*>(sbh + q * GM_BS + boff);
+                const halfx8 bl0 = *reinterpret_cast<const halfx8*>(sbl + q * GM_BS + boff);
+                const halfx8 bh1 = *reinterpret_cast<const halfx8*>(sbh + (32 + q) * GM_BS + boff);
+                const halfx8 bl1 = *reinterpret_cast<const halfx8*>(sbl + (32 + q) * GM_BS + boff);
+                const halfx8 bh2 = *reinterpret_cast<const halfx8*>(sbh + (64 + q) * GM_BS + boff);
+                const halfx8 bl2 = *reinterpret_cast<const halfx8*>(sbl + (64 + q) * GM_BS + boff);
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl2, c2, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh2, c2, 0, 0, 0);
+            }
+            __syncthreads();                       // the next chunk rewrites the basis
+            if (!more) break;
+            if (nu != u) tu = A.tiles[t + nu];
+            u = nu;
+            cb0 = nc;
+            cur = nxt;
+        }
+        // the products into this wave's LDS (rows: Re of patches 0..15, then
+        // Im), then the fold
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 8 * (r >> 2) + 4 * g + (r & 3);
+            cbuf[row * GM_CS + q] = c0[r];
+            cbuf[row * GM_CS + 32 + q] = c1[r];
+            if (64 + q < GM_NCOL) cbuf[row * GM_CS + 64 + q] = c2[r];   // not the pad columns
+        }
+        gm_wave_sync();
+        if (lane < 16 && plive) {
+            const int patch = ppz * A.pby + ppy;
+            float2* dst = A.mom + ((static_cast<int64_t>(t - A.t_begin) * A.n_frames + frame) *
+                                       A.n_patches + patch) * GM_REC;
+            gm_fold(cbuf, lane, sev[wave][lane], make_float2(U.pht, U.kox),
+                    make_float2(U.koy, U.koz), tr.species, dst);
+        }
+        gm_wave_sync();                            // before the next unit's products
+    }
+}
+
+template <bool SECOND>
+hipError_t launch_gm_moments(const GmArgs& A, int slices, hipStream_t st) {
+    dim3 grid(A.n_groups, slices, A.n_frames);
+    hipLaunchKernelGGL((k_gm_moments<SECOND>), grid, dim3(64 * GM_WAVES), 0, st, A);
+    return hipGetLastError();
+}
+
+// Evaluation of the moment buffer: one wave per patch walks the unit heads
+// of its slice (within this round's [t_begin, t_end)), evaluates each unit's
+// polynomial at its 16 pixels per lane by Horner's rule, applies the
+// per-(unit, pixel) origin phase and accumulates in FP32; the running sums
+// go, times the pixel table of their species, into the slice's FP64 partial
+// at a species change, every POLY_FLUSH units and at the end (as k_sum_poly).
+// A unit's record (64 floats: moments, phase terms, species) arrives one
+// float per lane, loaded while the previous unit is evaluated, and is
+// broadcast by readlane; the pixels' direction offsets stay in registers.
+__global__ __launch_bounds__(64 * POLY_WAVES) void k_gm_eval(const GmArgs A) {
+    constexpr int D = 6;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int patch = blockIdx.x * POLY_WAVES + wave;
+    const bool live = patch < A.n_patches;
+    const int frame = blockIdx.z;
+    const int slice = blockIdx.y;
+    const int s0 = slice * A.tiles_per_slice;
+    const int t0 = max(A.t_begin, s0);
+    const int t1 = min(min(A.t_end, A.n_tiles), s0 + A.tiles_per_slice);
+    const int pc = live ? patch : 0;
+    const PatchGeom P = patch_of(pc % A.pby, pc / A.pby, A.Ny, A.Nz);
+    const int Ny = A.Ny;
+    const int g = lane >> 5;
+    const int q = lane & 31;
+    const int prow = (P.z0 + min(q, P.vz - 1)) * Ny + P.y0;
+    auto arow = [&](int r) { return 8 * (r >> 2) + 4 * g + (r & 3); };
+    const float yq = static_cast<float>(q - P.mz) * (1.f / 16.f);
+    float dux[16], duy[16], duz[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const float4 du = A.dup[prow + min(arow(r), P.vy - 1)];
+        dux[r] = du.x;
+        duy[r] = du.y;
+        duz[r] = du.z;
+    }
+    float acc_re[16], acc_im[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { acc_re[r] = 0.f; acc_im[r] = 0.f; }
+    double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
+    bool first = A.accumulate == 0;
+    int since = 0;
+    int acc_species = 0;
+    auto flush = [&]() {
+        const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(acc_species) * A.npix;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            if (live && arow(r) < P.vy && q < P.vz) {
+                const int p = prow + arow(r);
+                const float2 gg = Gm[p];
+                double2* o = out + p;
+                double2 v = first ? make_double2(0.0, 0.0) : *o;
+                v.x += static_cast<double>(gg.x * acc_re[r] - gg.y * acc_im[r]);
+                v.y += static_cast<double>(gg.x * acc_im[r] + gg.y * acc_re[r]);
+                *o = v;
+            }
+            acc_re[r] = 0.f;
+            acc_im[r] = 0.f;
+        }
+        first = false;
+        since = 0;
+    };
+    const float* const recs = reinterpret_cast<const float*>(A.mom);
+    auto rec_at = [&](int t) {
+        return recs[((static_cast<int64_t>(t - A.t_begin) * A.n_frames + frame) * A.n_patches + pc) *
+                        (2 * GM_REC) + lane];
+    };
+    GmHeadWalk walk(A.heads, t0, t1, lane);
+    int tc, gc;
+    bool have = walk.next(tc, gc);
+    float rec = have ? rec_at(tc) : 0.f;
+    while (have) {
+        int tn, gn;
+        const bool hn = walk.next(tn, gn);
+        const float rec_next = hn ? rec_at(tn) : 0.f;      // in flight during this unit
+        const int species = __builtin_amdgcn_readlane(__float_as_int(rec), 2 * GM_MOM + 4);
+        if (since > 0 && species != acc_species) flush();
+        acc_species = species;
+        auto mv = [&](int i) {
+            return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec), i));
+        };
+        const float pht = mv(2 * GM_MOM), kox = mv(2 * GM_MOM + 1);
+        const float koy = mv(2 * GM_MOM + 2), koz = mv(2 * GM_MOM + 3);
+        // G_m(y) = sum_n M_mn y^n for this lane's column, then each pixel row
+        float Gr[D + 1], Gi[D + 1];
+        {
+            int base = 0;
+#pragma unroll
+            for (int m = 0; m <= D; ++m) {
+                float gr = 0.f, gi = 0.f;
+#pragma unroll
+                for (int n = D - m; n >= 0; --n) {
+                    gr = fmaf(gr, yq, mv(2 * (base + n)));
+                    gi = fmaf(gi, yq, mv(2 * (base + n) + 1));
+                }
+                Gr[m] = gr;
+                Gi[m] = gi;
+                base += D - m + 1;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float x = static_cast<float>(arow(r) - P.my) * (1.f / 16.f);
+            float sr = Gr[D], si = Gi[D];
+#pragma unroll
+            for (int m = D - 1; m >= 0; --m) {
+                sr = fmaf(sr, x, Gr[m]);
+                si = fmaf(si, x, Gi[m]);
+            }
+            const float phs = pht - fmaf(koz, duz[r], fmaf(koy, duy[r], kox * dux[r]));
+            const float sn = __builtin_amdgcn_sinf(phs);
+            const float cs = __builtin_amdgcn_cosf(phs);
+            acc_re[r] = fmaf(cs, sr, fmaf(-sn, si, acc_re[r]));
+            acc_im[r] = fmaf(cs, si, fmaf(sn, sr, acc_im[r]));
+        }
+        if (++since == POLY_FLUSH) flush();
+        have = hn;
+        tc = tn;
+        rec = rec_next;
+    }
+    if (since > 0) flush();
+    if (live && first) {                          // no unit of this slice in the first round
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            if (arow(r) < P.vy && q < P.vz) out[prow + arow(r)] = make_double2(0.0, 0.0);
+    }
+}
+
 // ------------------------------------------------------------------ adjoint
 // Gradient of a real loss L(A) with respect to the sample-frame displacement
 // of every atom, given g = dL/dRe(A) + i dL/dIm(A) per frame and pixel.  With
@@ -3075,29 +3818,6 @@ struct AdjMmaArgs {
     int n_tiles;
     float* mom;            // [tile][frame x patch][2 NM] pixel moments of unit heads
 };
-
-// Unit of tile t for the moment adjoint: the quad, else the pair, whose
-// staged radius is within rmax and which covers t, else the tile alone.
-// Heads sit at aligned places inside a k-d node and only they carry finite
-// radii, so this is the unit the moment sum's walk forms from a node start.
-__device__ __forceinline__ void unit_of(int t, int n_tiles, const float* __restrict__ grp_r,
-                                        float rmax, int& head, int& size) {
-    head = t;
-    size = 1;
-    if (!(rmax > 0.f)) return;
-    for (int h = max(0, t - 3); h <= t; ++h)
-        if (h + 3 < n_tiles && grp_r[2 * static_cast<int64_t>(h) + 1] <= rmax) {
-            head = h;
-            size = 4;
-            return;
-        }
-    for (int h = max(0, t - 1); h <= t; ++h)
-        if (h + 1 < n_tiles && grp_r[2 * static_cast<int64_t>(h)] <= rmax) {
-            head = h;
-            size = 2;
-            return;
-        }
-}
 
 template <bool SECOND, int NV>
 __global__ __launch_bounds__(64 * MMA_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -3925,7 +4645,7 @@ extern "C" int mosaic_kin_workspace(int64_t atom_count, int n_species, const int
     }
     if (tot != atom_count) return MOSAIC_ERR_ARGS;
     Layout L;
-    const int rc = make_layout(nullptr, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    const int rc = make_layout(nullptr, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
     *bytes = L.total;
     return MOSAIC_OK;
@@ -3944,7 +4664,7 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     if (!(k > 0.0) || !stage_stats) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
-    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
     stage_stats[0] = 0.0;
     stage_stats[1] = 0.0;
@@ -4085,7 +4805,7 @@ extern "C" int mosaic_kin_restage(int64_t atom_count, const float* pos, const fl
     if (!(k > 0.0) || !stage_stats) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
-    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
     stage_stats[0] = 0.0;
     stage_stats[1] = 0.0;
@@ -4215,6 +4935,27 @@ static int poly_degree(bool patch_ok, double radius_wl, const double* pixel_geom
     return 0;
 }
 
+// Bounds of the matrix-core moment sum (k_gm_moments) for a unit of radius
+// r [wavelengths]: the degree-6 expansion of k_sum_poly, with each patch's
+// row and column gradients replaced by the mean over its region of 8 x 8
+// patches.  The region's patch middles lie within 112 pixels (per axis) of
+// their mean, so a patch's row gradient differs from the region's by at most
+// 112 (du2 + |gyz|) per pixel (column: 112 (du2z + |gyz|)).  With the quadratic and cross terms
+// that bounds eps, the largest |eps_j(x, y)| over a patch, which the
+// expansion keeps to second order; the truncations are phi^7 / 7! (degree 6),
+// eps phi^5 / 5! (first order to degree 4), eps^2 / 2 phi^2 / 2 (second
+// order to degree 1) and eps^3 / 6, each within 1e-6 of the atom's term.
+static bool gm_bounds(double r, const double* pixel_geom) {
+    const double du1 = pixel_geom[1], du2 = pixel_geom[2];
+    const double du2z = pixel_geom[4], du1z = pixel_geom[5], gyz = pixel_geom[7];
+    const double phi = 2.0 * PI * 16.0 * ((du1 + 16.0 * du2) + (du1z + 16.0 * du2z)) * r;
+    const double dg = 112.0 * (du2 + du2z + 2.0 * gyz);
+    const double eps = 2.0 * PI * (16.0 * dg + 256.0 * (0.5 * du2 + 0.5 * du2z + gyz)) * r;
+    const double p2 = phi * phi, p5 = p2 * p2 * phi, p7 = p5 * p2;
+    return p7 / 5040.0 <= 1e-6 && eps * p5 / 120.0 <= 1e-6 &&
+           0.25 * eps * eps * p2 <= 1e-6 && eps * eps * eps / 6.0 <= 1e-6;
+}
+
 extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* counts,
                               const mosaic_species_t* species_tab, const mosaic_kin_opts_t* opts,
                               int n_frames, const float* rot, const float* trans, int Ny, int Nz,
@@ -4229,12 +4970,12 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
         !pixel_geom)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 6 ||
+    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 7 ||
         opts->analyser_kind < 0 || opts->analyser_kind > 2)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
-    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
     const int npix = Ny * Nz;
     const double K = opts->k * INV_2PI_HI;
@@ -4299,21 +5040,36 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     // far field ahead of the per-tile form, which gives each lane a run of
     // pixels and starves on them.
     const bool probe_small = far && static_cast<int64_t>(npix) * n_frames <= PROBE_AUTO_OUTPUTS;
+    // matrix-core moment sum: the moment sum's geometry bounds, the range of
+    // the origin phase, and the expansion about each group's mean gradients;
+    // the automatic choice takes it from one full region of 8 x 8 patches up
+    // (with fewer, most of a region's waves would only build the basis)
+    const int n_patches = ((Ny + MMA_P - 1) / MMA_P) * ((Nz + MMA_P - 1) / MMA_P);
+    auto gm_ok = [&](double r) {
+        return poly_patch_ok(r) && poly_degree(true, r, pixel_geom, K * omax, true) != 0 &&
+               gm_bounds(r, pixel_geom);
+    };
+    const bool gm_path = n_patches >= GM_REG * GM_REG && gm_ok(radius_wl);
     if (path == 0)
         path = ana ? 5
-                   : (poly_d ? 4 : (mma_ok ? 3 : (kp ? 2 : (probe_small ? 6 : (tile_ok ? 1 : 5)))));
+                   : (gm_path ? 7
+                              : (poly_d ? 4
+                                        : (mma_ok ? 3
+                                                  : (kp ? 2 : (probe_small ? 6 : (tile_ok ? 1 : 5))))));
     if ((path == 1 && !tile_ok) || (path == 2 && kp == 0) || (path == 3 && !mma_ok) ||
-        (path == 4 && poly_d == 0) || (path == 6 && !far) || (ana && path != 5))
+        (path == 4 && poly_d == 0) || (path == 6 && !far) || (path == 7 && !gm_ok(radius_wl)) ||
+        (ana && path != 5))
         return MOSAIC_ERR_UNSUPPORTED;
     // Moment sum over units of 2 or 4 tiles: the largest unit radius for
     // which every radius-dependent bound above still holds at the chosen
     // degree and second-order setting.
     float grp_rmax = 0.f;
-    if (path == 4) {
+    if (path == 4 || path == 7) {
         auto holds = [&](double r) {
             const double rm = r / K;
             if (rm / rho_min > 1e-6 || K * rm * rm * rm / (rho_min * rho_min) > 1e-7) return false;
             if (!use_second && r * r / (2.0 * K * rho_min) > 1e-7) return false;
+            if (path == 7) return gm_ok(r);
             const int d = poly_degree(poly_patch_ok(r), r, pixel_geom, K * omax, true);
             return d != 0 && d <= poly_d;
         };
@@ -4406,6 +5162,58 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         else
             e = use_second ? launch_poly<6, true>(A, L.n_slices, n_frames, st)
                            : launch_poly<6, false>(A, L.n_slices, n_frames, st);
+    } else if (path == 7) {
+        GmArgs A;
+        A.atoms = L.atoms;
+        A.tiles = L.tiles;
+        A.frames = L.frames;
+        A.u0h = L.px.u0h;
+        A.theta0 = L.px.theta0;
+        A.dup = L.px.dup;
+        A.gmma = L.px.gmma;
+        A.partial = L.partial;
+        A.mom = L.gm_mom;
+        A.heads = L.gm_head;
+        A.grp_r = L.grp_r;
+        A.grp_rmax = grp_rmax;
+        A.n_tiles = L.n_tiles;
+        A.tiles_per_slice = L.tiles_per_slice;
+        A.n_slices = L.n_slices;
+        A.n_frames = n_frames;
+        A.Ny = Ny;
+        A.Nz = Nz;
+        A.npix = npix;
+        A.pby = (Ny + MMA_P - 1) / MMA_P;
+        A.pbz = (Nz + MMA_P - 1) / MMA_P;
+        A.n_patches = n_patches;
+        A.ngy = (A.pby + GM_REG - 1) / GM_REG;
+        A.n_groups = A.ngy * ((A.pbz + GM_REG - 1) / GM_REG);
+        A.K = K;
+        // rounds of unit heads whose moments fit the buffer
+        const int64_t per_tile = static_cast<int64_t>(n_patches) * n_frames * GM_REC;
+        const int tiles_per_round = static_cast<int>(
+            std::max<int64_t>(1, std::min<int64_t>(L.n_tiles, static_cast<int64_t>(L.gm_count) / per_tile)));
+        // moments kernel: about 4096 waves over the round's tiles
+        const int64_t waves = static_cast<int64_t>(A.n_groups) * GM_WAVES * n_frames;
+        hipLaunchKernelGGL(k_gm_heads, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
+                           L.grp_r, grp_rmax, L.n_tiles, L.gm_head);
+        e = hipGetLastError();
+        for (int tb = 0; tb < L.n_tiles && e == hipSuccess; tb += tiles_per_round) {
+            const int te = std::min(L.n_tiles, tb + tiles_per_round);
+            const int rt = te - tb;
+            const int want = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(rt, (4096 + waves - 1) / waves)));
+            A.a_tiles_per_slice = (rt + want - 1) / want;
+            const int slices_a = (rt + A.a_tiles_per_slice - 1) / A.a_tiles_per_slice;
+            A.t_begin = tb;
+            A.t_end = te;
+            A.accumulate = tb > 0 ? 1 : 0;
+            e = use_second ? launch_gm_moments<true>(A, slices_a, st)
+                           : launch_gm_moments<false>(A, slices_a, st);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL(k_gm_eval, dim3((n_patches + POLY_WAVES - 1) / POLY_WAVES, L.n_slices, n_frames),
+                               dim3(64 * POLY_WAVES), 0, st, A);
+            e = hipGetLastError();
+        }
     } else if (path == 6) {
         ProbeArgs A;
         A.atoms = L.atoms;
@@ -4491,6 +5299,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     if (path_used)
         *path_used = path == 5   ? 50 + ana
                      : path == 6 ? 60 + (use_second ? 100 : 0)
+                     : path == 7 ? 76 + (use_second ? 100 : 0)
                      : path == 4 ? 40 + poly_d + (use_second ? 100 : 0)
                      : path == 3 ? 32 + (use_second ? 100 : 0)
                      : path == 2 ? (kp == 8 ? 28 : 24) + (use_second ? 100 : 0)
@@ -4548,7 +5357,7 @@ extern "C" int mosaic_kin_adjoint(int64_t atom_count, int n_species, const int64
     if (path != 0 && path != 2 && path != 3 && path != 4) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
-    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny * Nz, L);
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
     if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
     // The adjoint reads the staged sample and the pixel tables of the forward

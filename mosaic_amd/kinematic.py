@@ -37,7 +37,8 @@ def species_table(consts):
     return arr
 
 
-PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4, "exact": 5, "probe": 6}
+PATHS = {"auto": 0, "tile": 1, "rows": 2, "mma": 3, "poly": 4, "exact": 5, "probe": 6,
+         "gemm": 7}
 
 # analyser_mode names of the reference's GPU driver (Beam.py:6134-6143)
 _ANALYSER = {"off": 0, "none": 0, "disabled": 0, "top_hat": 1, "tophat": 1, "top-hat": 1,
@@ -61,10 +62,11 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
              accept_rad=0.0, darwin_hw_rad=0.0):
     """mosaic_kin_opts_t; ``path`` is "auto", "exact" (per pair), "tile"
     (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches),
-    "poly" (moment expansion over the same patches) or "probe" (a few
-    scattered pixels, one lane per frame and pixel; "auto" takes it for at
-    most 16384 pixel-poses); all but "auto" and "exact" decline when their
-    bounds do not hold.  ``analyser`` is the kind (0 off, 1
+    "poly" (moment expansion over the same patches), "gemm" (the moment
+    expansion with its per-atom work on the matrix cores; "auto" prefers it
+    from 64 patches up) or "probe" (a few scattered pixels, one lane per
+    frame and pixel; "auto" takes it for at most 16384 pixel-poses); all
+    but "auto" and "exact" decline when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
     top-hat, 2 Darwin) with its axis (lab), acceptance half-angle and Darwin
     half-width; it takes the exact kernel."""
     o = KinOpts()

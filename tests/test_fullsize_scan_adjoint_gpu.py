@@ -76,7 +76,7 @@ def test_scan_sampled_pixels_vs_fp64_oracle(scan):
                         threads=THREADS)
         errs.append(rel(scan["frames"][f][sel], R))
         print(f"[full scan] frame {f}: 32 pixels vs FP64 oracle rel={errs[-1]:.3e}", flush=True)
-    assert scan["path"] % 100 in (45, 46)
+    assert scan["path"] % 100 == 76          # the matrix-core moment sum
     assert max(errs) <= 1e-5
 
 
