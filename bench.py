@@ -9,11 +9,15 @@ dislocation, 10 keV, 512 x 512 pixels at 1 m on the (200) reflection:
 the timed region; one step stages it (k-d cut into tiles), runs the
 pixel prologue and the sum.
 
-Prints one JSON line.  ``roofline`` prices the dominant kernel (the sum) by
-the FP32 flops it executes per launch (the profile's counters of the same
-workload) over its launch time, against the 157.3 TFLOP/s FP32 peak; the
-time comes from HIP events around the mosaic_kin_sum call on the stream it
-runs on.  ``cpu_baseline`` times the reference's own C kernel
+Prints one JSON line.  ``roofline`` prices the dominant kernel of the sum:
+on this workload the matrix-core moment sum's moments kernel (k_gm_moments),
+by the FP16 matrix flops it executes per launch (the profile's counters of
+the same workload) over its launch time, against the 2.5 PFLOP/s dense FP16
+peak, with its FP32 VALU flops and issue fraction beside them (the VALU
+kernels are priced by their FP32 flops against the 157.3 TFLOP/s peak).
+The launch time comes from the library's HIP events around that kernel
+(mosaic_kin_timing) on the stream it runs on; ``sum_ms`` is the whole
+mosaic_kin_sum call by HIP events.  ``cpu_baseline`` times the reference's own C kernel
 (compute_scattering_cffi, built into oracle/_ref) on a bounded pixel sample
 of the same workload, on the host's cores.  ``end_to_end`` times the drop-in
 call on the host-resident chunks (H2D included); ``scan``, ``adjoint`` and
@@ -42,6 +46,8 @@ PEAK_VALU_LANE_OPS = PEAK_FP32 / 2
 PEAK_TRANS_LANE_OPS = 1.81e13
 # The matrix-core sum executes 3 split products x 4 real multiply-adds per pair.
 MFMA_FLOP_PER_PAIR = 24.0
+# One v_mfma_f32_32x32x16_f16: 32 x 32 x 16 multiply-adds.
+MFMA_FLOP_PER_INST = 2.0 * 32 * 32 * 16
 
 # Counters of one launch of the sum kernels on the 5M-atom, 512^2 workload
 # (rocprofv3 --pmc, same bench command; the profile files hold the commands):
@@ -52,6 +58,11 @@ COUNTERS = {
         source="profiles/k_sum_poly_perovskite5M_512px.md",
         fetch_kib=719146.0, write_kib=266402.0, valu=4.711e9,
         add_f32=2.06103e8, mul_f32=8.25971e8, fma_f32=2.82295e9, trans_f32=9.04499e7),
+    "k_gm_moments": dict(
+        source="profiles/k_gm_moments_perovskite5M_512px.md",
+        fetch_kib=517181.5, write_kib=723943.4, valu=6.405e8,
+        add_f32=8.4375e7, mul_f32=4.4519e7, fma_f32=2.3917e8, trans_f32=4.0004e7,
+        mfma=4.5e7),
     "k_sum_mma": dict(
         source="profiles/k_sum_mma_perovskite5M_512px.md",
         fetch_kib=1831133.0, write_kib=2542443.8, valu=None,
@@ -450,7 +461,7 @@ def bench_end_to_end(w, steps):
             "kernel_path": b._last_path}
 
 
-def roofline(path_used, pairs, ksum_ms, workload):
+def roofline(path_used, pairs, ksum_ms, workload, kernel_ms=None):
     """Roofline object of the dominant (sum) kernel.
 
     ``achieved`` is the work the kernel's own algorithm does per launch over
@@ -462,9 +473,33 @@ def roofline(path_used, pairs, ksum_ms, workload):
     flops it issues, against 2.5 PFLOP/s.  ``pair_equivalent_tflops`` is a
     throughput figure, not a fraction of a roofline: 20 flop per pair (the
     per-pair algorithm) times the pairs the kernel's result stands for."""
-    t = ksum_ms * 1e-3
     name = kernel_name(path_used)
     c = COUNTERS.get(name) if workload == WORKLOAD_OF_COUNTERS else None
+    if name == "k_gm_moments" and kernel_ms is not None:
+        # the matrix-core moment sum is two kernels: the moments (dominant)
+        # and their evaluation, timed apart by the library's HIP events
+        t = kernel_ms[0] * 1e-3
+        r = {"kernel": name, "kernel_path": path_used, "kernel_ms": kernel_ms[0],
+             "eval_kernel": "k_gm_eval", "eval_ms": kernel_ms[1], "sum_ms": ksum_ms,
+             "sum_pairs_per_s": pairs / (ksum_ms * 1e-3), "unit": "TFLOP/s",
+             "pair_equivalent_tflops": pairs * FLOP_PER_PAIR / (ksum_ms * 1e-3) / 1e12,
+             "traffic": None}
+        if c is not None:
+            r["traffic"] = (2.0 * c["fetch_kib"] + c["write_kib"]) * 1024.0
+            r["counters_source"] = c["source"]
+            flop = c["mfma"] * MFMA_FLOP_PER_INST
+            v32 = 64.0 * (c["add_f32"] + c["mul_f32"] + 2.0 * c["fma_f32"] + c["trans_f32"])
+            r.update({"bound": "mfma", "achieved": flop / t / 1e12,
+                      "peak": PEAK_FP16_MFMA / 1e12, "frac": flop / t / PEAK_FP16_MFMA,
+                      "mfma_flop_per_launch": flop,
+                      "fp32_valu_tflops": v32 / t / 1e12, "fp32_valu_frac": v32 / t / PEAK_FP32,
+                      "valu_issue_frac": 64.0 * c["valu"] / t / PEAK_VALU_LANE_OPS,
+                      "trans_frac": 64.0 * c["trans_f32"] / t / PEAK_TRANS_LANE_OPS})
+        else:
+            r.update({"bound": "mfma", "peak": PEAK_FP16_MFMA / 1e12, "achieved": None,
+                      "frac": None, "note": "no counters of this kernel on this workload"})
+        return r
+    t = ksum_ms * 1e-3
     r = {"kernel": name, "kernel_path": path_used, "kernel_ms": ksum_ms,
          "kernel_pairs_per_s": pairs / t, "unit": "TFLOP/s",
          "pair_equivalent_tflops": pairs * FLOP_PER_PAIR / t / 1e12,
@@ -793,6 +828,8 @@ def main():
     ws = [None]
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     sum_ms = []
+    kern_ms = []
+    K.sum_timing(True)
 
     path_used = [None]
 
@@ -808,6 +845,7 @@ def main():
         if timed:
             ev1.synchronize()
             sum_ms.append(ev0.elapsed_time(ev1))
+            kern_ms.append(K.sum_timing(True))
 
     for _ in range(args.warmup):
         step(False)
@@ -837,7 +875,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "atoms": int(n_atoms), "pixels": [Ny, Nz],
                    "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
-        "roofline": roofline(path_used[0], pairs, ksum_ms, args.workload),
+        "roofline": roofline(path_used[0], pairs, ksum_ms, args.workload,
+                             tuple(np.mean(np.asarray(kern_ms), axis=0))),
     }
     if not args.no_end_to_end:
         res["end_to_end"] = bench_end_to_end(w, args.steps)

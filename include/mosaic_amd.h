@@ -163,20 +163,26 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  *     consecutive tiles as a polynomial of degree D = 5 or 6 in the pixel
  *     offsets, from (D+1)(D+2)/2 moments of the atoms' small-angle phasors;
  *     units of more than one tile are used where their radius, measured by
- *     mosaic_kin_stage, keeps every error bound); and a probe form for a
- *     few scattered pixels (6: one lane per (frame, pixel), the recurrence's
- *     far-field model with the exact direction at each pixel, one sincos per
- *     pair).
- * opts->path = 0 picks the moment expansion, else the matrix-core form, else
- * the recurrence, else (at most 16384 pixel-poses) the probe form, else the
+ *     mosaic_kin_stage, keeps every error bound); the same expansion with its
+ *     per-atom work on the matrix cores (7: the degree-6 moments of 64
+ *     patches as one FP16-split matrix product over each unit's atoms, with
+ *     each patch's gradients expanded to second order about its region's
+ *     mean, then evaluated at the pixels from a moment buffer in the
+ *     workspace); and a probe form for a few scattered pixels (6: one lane
+ *     per (frame, pixel), the recurrence's far-field model with the exact
+ *     direction at each pixel, one sincos per pair).
+ * opts->path = 0 picks the matrix-core moment sum (from 64 patches up),
+ * else the moment expansion, else the matrix-core form, else the
+ * recurrence, else (at most 16384 pixel-poses) the probe form, else the
  * per-tile form, whenever their error bounds hold, and the exact kernel
  * otherwise; an analyser always takes the exact kernel.
- * A forced path (1-4, 6) returns MOSAIC_ERR_UNSUPPORTED (nothing written)
- * when its bounds fail.
+ * A forced path (1-4, 6, 7) returns MOSAIC_ERR_UNSUPPORTED (nothing
+ * written) when its bounds fail.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
  * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels),
- * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), 60 (probe), plus 100 for
+ * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), 76 (the same on
+ * the matrix cores), 60 (probe), plus 100 for
  * the far-field forms with the atoms' second-order phase, or 50 + the
  * analyser kind (exact).  Stream-ordered, no host synchronization.
  */
@@ -190,6 +196,18 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const double* stage_stats /* host [2] */,
                    const double* pixel_geom /* host [8] */, void* workspace,
                    size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
+
+/*
+ * Launch times of the kernels of mosaic_kin_sum calls on the calling
+ * thread's current device, for measurement.  With enable = 1 the
+ * matrix-core moment sum (path 7) records HIP events on its stream around
+ * its moments and its evaluation kernels; ms_out (host [2], may be NULL)
+ * receives the last such call's times in milliseconds (moments,
+ * evaluation), after waiting for its events.  No counterpart in the
+ * reference, which times its kernels from Python with CuPy events
+ * (hardware.py:1944-1980).
+ */
+int mosaic_kin_timing(int enable, float* ms_out);
 
 /*
  * Adjoint of mosaic_kin_sum with respect to the atoms' sample-frame

@@ -4956,6 +4956,56 @@ static bool gm_bounds(double r, const double* pixel_geom) {
            0.25 * eps * eps * p2 <= 1e-6 && eps * eps * eps / 6.0 <= 1e-6;
 }
 
+// Launch timing of the matrix-core moment sum, for measurement (enabled by
+// mosaic_kin_timing): HIP events on the sum's stream around each round's
+// moments and evaluation kernels, per device.
+namespace {
+constexpr int TIMING_DEVICES = 16;
+constexpr int TIMING_ROUNDS = 16;
+struct KinTiming {
+    bool on = false;
+    bool made = false;
+    int rounds = 0;
+    hipEvent_t ev[TIMING_ROUNDS][4];
+};
+KinTiming g_timing[TIMING_DEVICES];
+
+KinTiming* timing_state() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TIMING_DEVICES) return nullptr;
+    return &g_timing[dev];
+}
+
+// event k (0, 1: moments start, stop; 2, 3: evaluation) of round r, if timing is on
+void timing_mark(KinTiming* T, int r, int k, hipStream_t st) {
+    if (T && T->on && T->made && r < TIMING_ROUNDS) (void)hipEventRecord(T->ev[r][k], st);
+}
+}  // namespace
+
+extern "C" int mosaic_kin_timing(int enable, float* ms_out) {
+    KinTiming* T = timing_state();
+    if (!T) return MOSAIC_ERR_HIP;
+    if (enable && !T->made) {
+        for (int r = 0; r < TIMING_ROUNDS; ++r)
+            for (int k = 0; k < 4; ++k) MOSAIC_HIP_CHECK(hipEventCreate(&T->ev[r][k]));
+        T->made = true;
+    }
+    if (ms_out) {
+        ms_out[0] = 0.f;
+        ms_out[1] = 0.f;
+        for (int r = 0; r < T->rounds && r < TIMING_ROUNDS; ++r) {
+            MOSAIC_HIP_CHECK(hipEventSynchronize(T->ev[r][3]));
+            float a = 0.f, b = 0.f;
+            MOSAIC_HIP_CHECK(hipEventElapsedTime(&a, T->ev[r][0], T->ev[r][1]));
+            MOSAIC_HIP_CHECK(hipEventElapsedTime(&b, T->ev[r][2], T->ev[r][3]));
+            ms_out[0] += a;
+            ms_out[1] += b;
+        }
+    }
+    T->on = enable != 0;
+    return MOSAIC_OK;
+}
+
 extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* counts,
                               const mosaic_species_t* species_tab, const mosaic_kin_opts_t* opts,
                               int n_frames, const float* rot, const float* trans, int Ny, int Nz,
@@ -5198,7 +5248,9 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         hipLaunchKernelGGL(k_gm_heads, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
                            L.grp_r, grp_rmax, L.n_tiles, L.gm_head);
         e = hipGetLastError();
-        for (int tb = 0; tb < L.n_tiles && e == hipSuccess; tb += tiles_per_round) {
+        KinTiming* T = timing_state();
+        int round = 0;
+        for (int tb = 0; tb < L.n_tiles && e == hipSuccess; tb += tiles_per_round, ++round) {
             const int te = std::min(L.n_tiles, tb + tiles_per_round);
             const int rt = te - tb;
             const int want = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(rt, (4096 + waves - 1) / waves)));
@@ -5207,13 +5259,18 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
             A.t_begin = tb;
             A.t_end = te;
             A.accumulate = tb > 0 ? 1 : 0;
+            timing_mark(T, round, 0, st);
             e = use_second ? launch_gm_moments<true>(A, slices_a, st)
                            : launch_gm_moments<false>(A, slices_a, st);
+            timing_mark(T, round, 1, st);
             if (e != hipSuccess) break;
+            timing_mark(T, round, 2, st);
             hipLaunchKernelGGL(k_gm_eval, dim3((n_patches + POLY_WAVES - 1) / POLY_WAVES, L.n_slices, n_frames),
                                dim3(64 * POLY_WAVES), 0, st, A);
+            timing_mark(T, round, 3, st);
             e = hipGetLastError();
         }
+        if (T) T->rounds = std::min(round, TIMING_ROUNDS);
     } else if (path == 6) {
         ProbeArgs A;
         A.atoms = L.atoms;

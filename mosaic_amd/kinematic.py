@@ -253,6 +253,16 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
     return out
 
 
+def sum_timing(enable=True):
+    """Per-kernel launch times of the matrix-core moment sum (path "gemm"):
+    enables or disables HIP events around its moments and evaluation kernels
+    on the current device, and returns the last call's (moments, evaluation)
+    times in milliseconds (waits for their events)."""
+    ms = (ctypes.c_float * 2)()
+    check(_lib.lib().mosaic_kin_timing(int(bool(enable)), ms), "mosaic_kin_timing")
+    return float(ms[0]), float(ms[1])
+
+
 def ein_bilinear(pos, rot, trans, E0, du, dv, uc, vc, out=None, stream=None):
     """Incident amplitude per atom sampled from the beam grid E0 (NyB, NzB)."""
     n = int(pos.shape[0])
