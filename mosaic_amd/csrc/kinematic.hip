@@ -3410,16 +3410,36 @@ hipError_t launch_gm_moments(const GmArgs& A, int slices, hipStream_t st) {
 }
 
 // Evaluation of the moment buffer: one wave per patch walks the unit heads
-// of its slice (within this round's [t_begin, t_end)), evaluates each unit's
-// polynomial at its 16 pixels per lane by Horner's rule, applies the
-// per-(unit, pixel) origin phase and accumulates in FP32; the running sums
-// go, times the pixel table of their species, into the slice's FP64 partial
-// at a species change, every POLY_FLUSH units and at the end (as k_sum_poly).
-// A unit's record (64 floats: moments, phase terms, species) arrives one
-// float per lane, loaded while the previous unit is evaluated, and is
-// broadcast by readlane; the pixels' direction offsets stay in registers.
-__global__ __launch_bounds__(64 * POLY_WAVES) void k_gm_eval(const GmArgs A) {
+// of its slice (within this round's [t_begin, t_end)) and accumulates each
+// unit's polynomial P_u(x, y) = sum M_mn x^m y^n times its origin phase
+//     e^{2 pi i (pht_u - K O_u . du(a, b))}
+// at the patch's 32 x 32 pixels (du = u_p - u_mid, a along the row, b down
+// the column).  With a reference origin O_s per slice (its first unit's) the
+// phase splits into e^{-2 pi i K O_s . du}, applied exactly per pixel when
+// the sums are flushed, and the unit's residual K dO . du, dO = O_u - O_s,
+// whose du is modelled as dr(a) + dc(b) + (a - my)(b - mz) h: the exact
+// offsets along the patch's middle row and column and a bilinear cross term
+// (h from the patch corners).  Then, to first order in the cross term,
+//     S_u(a, b) = alpha(a) beta(b) [P_u - 256 i kappa x y P_u],
+//     alpha = e^{-2 pi i K dO . dr(a)},  beta = e^{2 pi i pht_u} e^{-2 pi i K dO . dc(b)},
+//     kappa = 2 pi K dO . h,
+// a sum of 14 products L(a, k) N(k, b) with L = alpha x^m (main) or alpha
+// x^{m+1} (cross), N = beta G_m(y) or -256 i kappa y beta G_m(y), G_m(y) =
+// sum_n M_mn y^n: one complex 32 x 32 x 16 product per unit, which the
+// matrix cores accumulate over the units in the pixels' running sums (FP16
+// split operands, 12 products).  The model's error, K |dO| times the largest
+// deviation of the lane pixels' du from it (measured per patch), must stay
+// within 1e-6 revolutions and the cross term's second order within 1e-6;
+// a unit that fails either is evaluated pixel by pixel with its exact
+// phase (Horner's rule and one sincos per pixel), into the same sums.  The
+// running sums go, times the pixel table of their species and the slice's
+// reference phase, into the slice's FP64 partial at a species change, every
+// POLY_FLUSH units and at the end.
+__global__ __launch_bounds__(64 * POLY_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
+void k_gm_eval(const GmArgs A) {
     constexpr int D = 6;
+    constexpr float TWO_PI_F = 6.283185307179586f;
+    __shared__ __align__(16) float srec[POLY_WAVES][2 * GM_REC];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int patch = blockIdx.x * POLY_WAVES + wave;
@@ -3436,37 +3456,88 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_gm_eval(const GmArgs A) {
     const int q = lane & 31;
     const int prow = (P.z0 + min(q, P.vz - 1)) * Ny + P.y0;
     auto arow = [&](int r) { return 8 * (r >> 2) + 4 * g + (r & 3); };
-    const float yq = static_cast<float>(q - P.mz) * (1.f / 16.f);
-    float dux[16], duy[16], duz[16];
+    auto pix = [&](int r) { return prow + min(arow(r), P.vy - 1); };
+    float* const rw = srec[wave];
+
+    // this lane's operand roles: L row a = q (its x powers and middle-row
+    // offset), N column b = q (its y powers and middle-column offset)
+    const float xa = static_cast<float>(min(q, P.vy - 1) - P.my) * (1.f / 16.f);
+    const float yb = static_cast<float>(min(q, P.vz - 1) - P.mz) * (1.f / 16.f);
+    float lx[8];
+    {
+        float xp[8];
+        xp[0] = 1.f;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const float4 du = A.dup[prow + min(arow(r), P.vy - 1)];
-        dux[r] = du.x;
-        duy[r] = du.y;
-        duz[r] = du.z;
+        for (int m = 1; m < 8; ++m) xp[m] = xp[m - 1] * xa;
+        // k = 8 g + j: main m = k (k < 7), cross m = k - 7 (x^{m+1}), pads
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float v0 = j < 7 ? xp[j] : xp[1];
+            const float v1 = j < 6 ? xp[j + 2] : 0.f;
+            lx[j] = g ? v1 : v0;
+        }
     }
-    float acc_re[16], acc_im[16];
+    const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
+    const float4 dr = A.dup[(P.z0 + P.mz) * Ny + P.y0 + min(q, P.vy - 1)];
+    const float4 dc = A.dup[(P.z0 + min(q, P.vz - 1)) * Ny + P.y0 + P.my];
+    // bilinear cross term from the corners, and the model's largest
+    // deviation at this wave's pixels
+    float hx = 0.f, hy = 0.f, hz = 0.f, dev = 0.f;
+    {
+        const int c00 = P.z0 * Ny + P.y0, c10 = c00 + P.vy - 1, c01 = c00 + (P.vz - 1) * Ny,
+                  c11 = c01 + P.vy - 1;
+        const float area = static_cast<float>(P.vy - 1) * static_cast<float>(P.vz - 1);
+        if (area > 0.f) {
+            const float4 d00 = A.dup[c00], d10 = A.dup[c10], d01 = A.dup[c01], d11 = A.dup[c11];
+            const float ia = 1.f / area;
+            hx = (d11.x - d10.x - d01.x + d00.x) * ia;
+            hy = (d11.y - d10.y - d01.y + d00.y) * ia;
+            hz = (d11.z - d10.z - d01.z + d00.z) * ia;
+        }
+        const float4 dcq = dc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) { acc_re[r] = 0.f; acc_im[r] = 0.f; }
+        for (int r = 0; r < 16; ++r) {
+            const int a = min(arow(r), P.vy - 1);
+            const float4 du = A.dup[pix(r)];
+            const float4 dra = A.dup[(P.z0 + P.mz) * Ny + P.y0 + a];
+            const float w = static_cast<float>(a - P.my) * static_cast<float>(min(q, P.vz - 1) - P.mz);
+            const float ex = du.x - dra.x - dcq.x - w * hx;
+            const float ey = du.y - dra.y - dcq.y - w * hy;
+            const float ez = du.z - dra.z - dcq.z - w * hz;
+            dev = fmaxf(dev, sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez))));
+        }
+        for (int o = 32; o > 0; o >>= 1) dev = fmaxf(dev, __shfl_xor(dev, o, 64));
+    }
+    const float hn = sqrtf(fmaf(hx, hx, fmaf(hy, hy, hz * hz)));
+    (void)pmid;
+
+    floatx16 sr, si;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { sr[r] = 0.f; si[r] = 0.f; }
     double2* out = A.partial + (static_cast<int64_t>(frame) * A.n_slices + slice) * A.npix;
     bool first = A.accumulate == 0;
     int since = 0;
     int acc_species = 0;
+    float kos_x = 0.f, kos_y = 0.f, kos_z = 0.f;   // K O_s, the slice's reference origin
     auto flush = [&]() {
         const float2* __restrict__ Gm = A.gmma + static_cast<int64_t>(acc_species) * A.npix;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             if (live && arow(r) < P.vy && q < P.vz) {
                 const int p = prow + arow(r);
+                const float4 du = A.dup[p];
+                const float ph = -fmaf(kos_z, du.z, fmaf(kos_y, du.y, kos_x * du.x));
+                const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+                const float zr = cs * sr[r] - sn * si[r], zi = cs * si[r] + sn * sr[r];
                 const float2 gg = Gm[p];
                 double2* o = out + p;
                 double2 v = first ? make_double2(0.0, 0.0) : *o;
-                v.x += static_cast<double>(gg.x * acc_re[r] - gg.y * acc_im[r]);
-                v.y += static_cast<double>(gg.x * acc_im[r] + gg.y * acc_re[r]);
+                v.x += static_cast<double>(gg.x * zr - gg.y * zi);
+                v.y += static_cast<double>(gg.x * zi + gg.y * zr);
                 *o = v;
             }
-            acc_re[r] = 0.f;
-            acc_im[r] = 0.f;
+            sr[r] = 0.f;
+            si[r] = 0.f;
         }
         first = false;
         since = 0;
@@ -3480,19 +3551,31 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_gm_eval(const GmArgs A) {
     int tc, gc;
     bool have = walk.next(tc, gc);
     float rec = have ? rec_at(tc) : 0.f;
+    bool ref_set = false;
     while (have) {
         int tn, gn;
-        const bool hn = walk.next(tn, gn);
-        const float rec_next = hn ? rec_at(tn) : 0.f;      // in flight during this unit
-        const int species = __builtin_amdgcn_readlane(__float_as_int(rec), 2 * GM_MOM + 4);
+        const bool hn_ = walk.next(tn, gn);
+        const float rec_next = hn_ ? rec_at(tn) : 0.f;     // in flight during this unit
+        rw[lane] = rec;
+        gm_wave_sync();
+        const int species = __float_as_int(rw[2 * GM_MOM + 4]);
         if (since > 0 && species != acc_species) flush();
         acc_species = species;
-        auto mv = [&](int i) {
-            return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec), i));
-        };
-        const float pht = mv(2 * GM_MOM), kox = mv(2 * GM_MOM + 1);
-        const float koy = mv(2 * GM_MOM + 2), koz = mv(2 * GM_MOM + 3);
-        // G_m(y) = sum_n M_mn y^n for this lane's column, then each pixel row
+        const float pht = rw[2 * GM_MOM], kox = rw[2 * GM_MOM + 1];
+        const float koy = rw[2 * GM_MOM + 2], koz = rw[2 * GM_MOM + 3];
+        if (!ref_set) {
+            kos_x = kox;
+            kos_y = koy;
+            kos_z = koz;
+            ref_set = true;
+        }
+        const float dox = kox - kos_x, doy = koy - kos_y, doz = koz - kos_z;
+        const float kdo = sqrtf(fmaf(dox, dox, fmaf(doy, doy, doz * doz)));
+        // the unit's cross term at the patch corners [rad]
+        const float cmax = TWO_PI_F * kdo * hn * 256.f;
+        const bool sep = kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
+        // G_m(y) = sum_n M_mn y^n at this lane's column (y = yb for the
+        // product's N operand; the fallback evaluates at its own pixels)
         float Gr[D + 1], Gi[D + 1];
         {
             int base = 0;
@@ -3501,31 +3584,94 @@ __global__ __launch_bounds__(64 * POLY_WAVES) void k_gm_eval(const GmArgs A) {
                 float gr = 0.f, gi = 0.f;
 #pragma unroll
                 for (int n = D - m; n >= 0; --n) {
-                    gr = fmaf(gr, yq, mv(2 * (base + n)));
-                    gi = fmaf(gi, yq, mv(2 * (base + n) + 1));
+                    gr = fmaf(gr, yb, rw[2 * (base + n)]);
+                    gi = fmaf(gi, yb, rw[2 * (base + n) + 1]);
                 }
                 Gr[m] = gr;
                 Gi[m] = gi;
                 base += D - m + 1;
             }
         }
+        if (sep) {
+            const float tha = -fmaf(doz, dr.z, fmaf(doy, dr.y, dox * dr.x));
+            const float ar = __builtin_amdgcn_cosf(tha), ai = __builtin_amdgcn_sinf(tha);
+            const float thb = pht - fmaf(doz, dc.z, fmaf(doy, dc.y, dox * dc.x));
+            const float br = __builtin_amdgcn_cosf(thb), bi = __builtin_amdgcn_sinf(thb);
+            const float kap = TWO_PI_F * fmaf(doz, hz, fmaf(doy, hy, dox * hx));
+            // cross factor -256 i kappa y
+            const float cf = -256.f * kap * yb;
+            float Tr[D + 1], Ti[D + 1];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const float x = static_cast<float>(arow(r) - P.my) * (1.f / 16.f);
-            float sr = Gr[D], si = Gi[D];
-#pragma unroll
-            for (int m = D - 1; m >= 0; --m) {
-                sr = fmaf(sr, x, Gr[m]);
-                si = fmaf(si, x, Gi[m]);
+            for (int m = 0; m <= D; ++m) {
+                Tr[m] = br * Gr[m] - bi * Gi[m];
+                Ti[m] = br * Gi[m] + bi * Gr[m];
             }
-            const float phs = pht - fmaf(koz, duz[r], fmaf(koy, duy[r], kox * dux[r]));
-            const float sn = __builtin_amdgcn_sinf(phs);
-            const float cs = __builtin_amdgcn_cosf(phs);
-            acc_re[r] = fmaf(cs, sr, fmaf(-sn, si, acc_re[r]));
-            acc_im[r] = fmaf(cs, si, fmaf(sn, sr, acc_im[r]));
+            // N(k, b): g = 0: T_0..T_6, then the cross T_0; g = 1: the cross
+            // T_1..T_6, two pads.  The cross value is cf i T = (-cf Ti, cf Tr).
+            float nr[8], ni[8], lr[8], li[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c0 = j < 7 ? j : 0, c1 = j < 6 ? j + 1 : 0;
+                const float m0r = j < 7 ? Tr[c0] : -cf * Ti[c0];
+                const float m0i = j < 7 ? Ti[c0] : cf * Tr[c0];
+                const float m1r = j < 6 ? -cf * Ti[c1] : 0.f;
+                const float m1i = j < 6 ? cf * Tr[c1] : 0.f;
+                nr[j] = g ? m1r : m0r;
+                ni[j] = g ? m1i : m0i;
+                lr[j] = ar * lx[j];
+                li[j] = ai * lx[j];
+            }
+            uintx4 lrh, lrl, lih, lil, nrh, nrl, nih, nil;
+#pragma unroll
+            for (int u2 = 0; u2 < 4; ++u2) {
+                unsigned int h, l;
+                gm_split_pair(lr[2 * u2], lr[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
+                gm_split_pair(li[2 * u2], li[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
+                gm_split_pair(nr[2 * u2], nr[2 * u2 + 1], h, l); nrh[u2] = h; nrl[u2] = l;
+                gm_split_pair(ni[2 * u2], ni[2 * u2 + 1], h, l); nih[u2] = h; nil[u2] = l;
+            }
+            const uintx4 sgn = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+            const halfx8 Lrh = __builtin_bit_cast(halfx8, lrh), Lrl = __builtin_bit_cast(halfx8, lrl);
+            const halfx8 Lih = __builtin_bit_cast(halfx8, lih), Lil = __builtin_bit_cast(halfx8, lil);
+            const halfx8 Mih = __builtin_bit_cast(halfx8, lih ^ sgn), Mil = __builtin_bit_cast(halfx8, lil ^ sgn);
+            const halfx8 Nrh = __builtin_bit_cast(halfx8, nrh), Nrl = __builtin_bit_cast(halfx8, nrl);
+            const halfx8 Nih = __builtin_bit_cast(halfx8, nih), Nil = __builtin_bit_cast(halfx8, nil);
+            // Re += Lr Nr - Li Ni, Im += Lr Ni + Li Nr
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrh, Nrh, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrh, Nih, si, 0, 0, 0);
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Mih, Nih, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lih, Nrh, si, 0, 0, 0);
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrh, Nrl, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrh, Nil, si, 0, 0, 0);
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Mih, Nil, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lih, Nrl, si, 0, 0, 0);
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrl, Nrh, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lrl, Nih, si, 0, 0, 0);
+            sr = __builtin_amdgcn_mfma_f32_32x32x16_f16(Mil, Nih, sr, 0, 0, 0);
+            si = __builtin_amdgcn_mfma_f32_32x32x16_f16(Lil, Nrh, si, 0, 0, 0);
+        } else {
+            // exact phase per pixel: G at this lane's column is G_m(yb) (the
+            // lane's pixels share its column), Horner's rule in x per pixel
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float x = static_cast<float>(arow(r) - P.my) * (1.f / 16.f);
+                float pr = Gr[D], pi = Gi[D];
+#pragma unroll
+                for (int m = D - 1; m >= 0; --m) {
+                    pr = fmaf(pr, x, Gr[m]);
+                    pi = fmaf(pi, x, Gi[m]);
+                }
+                const float4 du = A.dup[pix(r)];
+                const float phs = pht - fmaf(doz, du.z, fmaf(doy, du.y, dox * du.x));
+                const float sn = __builtin_amdgcn_sinf(phs);
+                const float cs = __builtin_amdgcn_cosf(phs);
+                sr[r] = fmaf(cs, pr, fmaf(-sn, pi, sr[r]));
+                si[r] = fmaf(cs, pi, fmaf(sn, pr, si[r]));
+            }
         }
+        gm_wave_sync();                            // the record buffer's next write
         if (++since == POLY_FLUSH) flush();
-        have = hn;
+        have = hn_;
         tc = tn;
         rec = rec_next;
     }
