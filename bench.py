@@ -481,6 +481,7 @@ def roofline(path_used, pairs, ksum_ms, workload, kernel_ms=None):
         t = kernel_ms[0] * 1e-3
         r = {"kernel": name, "kernel_path": path_used, "kernel_ms": kernel_ms[0],
              "eval_kernel": "k_gm_eval", "eval_ms": kernel_ms[1], "sum_ms": ksum_ms,
+             "eval_pixel_fallbacks": int(round(kernel_ms[2])) if len(kernel_ms) > 2 else None,
              "sum_pairs_per_s": pairs / (ksum_ms * 1e-3), "unit": "TFLOP/s",
              "pair_equivalent_tflops": pairs * FLOP_PER_PAIR / (ksum_ms * 1e-3) / 1e12,
              "traffic": None}
@@ -845,7 +846,7 @@ def main():
         if timed:
             ev1.synchronize()
             sum_ms.append(ev0.elapsed_time(ev1))
-            kern_ms.append(K.sum_timing(True))
+            kern_ms.append(K.sum_timing(True, fallbacks=True))
 
     for _ in range(args.warmup):
         step(False)

@@ -51,12 +51,15 @@ typedef struct {
     int apply_decay;        /* scale by R0 / r */
     int path;               /* 0 automatic, 1 per-tile factors, 2 row recurrence,
                                3 matrix-core patches, 4 moment expansion,
-                               5 exact per pair, 6 probe (few pixels) */
+                               5 exact per pair, 6 probe (few pixels),
+                               7 moment expansion on the matrix cores */
     double pol_deg;         /* degree of linear polarization p */
     double pol_e[3];        /* its unit vector (lab) */
     int analyser_kind;      /* 0 off, 1 top-hat, 2 Darwin roll-off (Beam.py:2350-2378);
                                any other than 0 takes the exact per-pair path */
-    int pad_;
+    int gm_pixel_eval;      /* path 7: evaluate every (unit, patch) pixel by pixel
+                               instead of by the separable phase model (the
+                               model's own fallback; slower, same bar) */
     double analyser_axis[3];       /* analyser axis (lab); the reference uses the
                                       detector's centre pixel (Beam.py:6128-6132) */
     double analyser_accept_rad;    /* top-hat half-angle */
@@ -201,13 +204,14 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
  * Launch times of the kernels of mosaic_kin_sum calls on the calling
  * thread's current device, for measurement.  With enable = 1 the
  * matrix-core moment sum (path 7) records HIP events on its stream around
- * its moments and its evaluation kernels; ms_out (host [2], may be NULL)
- * receives the last such call's times in milliseconds (moments,
- * evaluation), after waiting for its events.  No counterpart in the
- * reference, which times its kernels from Python with CuPy events
- * (hardware.py:1944-1980).
+ * its moments and its evaluation kernels and counts the (unit, patch)
+ * evaluations that its separable phase model declined (evaluated pixel by
+ * pixel); out (host [3], may be NULL) receives the last such call's times
+ * in milliseconds (moments, evaluation) and that count, after waiting for
+ * its events.  No counterpart in the reference, which times its kernels
+ * from Python with CuPy events (hardware.py:1944-1980).
  */
-int mosaic_kin_timing(int enable, float* ms_out);
+int mosaic_kin_timing(int enable, float* out);
 
 /*
  * Adjoint of mosaic_kin_sum with respect to the atoms' sample-frame

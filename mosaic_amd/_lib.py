@@ -87,7 +87,7 @@ class KinOpts(ctypes.Structure):
                 ("apply_pol", ctypes.c_int), ("apply_decay", ctypes.c_int),
                 ("path", ctypes.c_int), ("pol_deg", ctypes.c_double),
                 ("pol_e", ctypes.c_double * 3), ("analyser_kind", ctypes.c_int),
-                ("pad_", ctypes.c_int), ("analyser_axis", ctypes.c_double * 3),
+                ("gm_pixel_eval", ctypes.c_int), ("analyser_axis", ctypes.c_double * 3),
                 ("analyser_accept_rad", ctypes.c_double),
                 ("analyser_darwin_hw_rad", ctypes.c_double)]
 

@@ -2918,6 +2918,8 @@ struct GmArgs {
     int ngy;                  // regions of GM_REG x GM_REG patches along a row
     int n_groups;             // regions (one workgroup each)
     int accumulate;           // k_gm_eval adds to the partials (rounds after the first)
+    unsigned int* fallbacks;  // units evaluated pixel by pixel (counted when not null)
+    int pixel_eval;           // evaluate every unit pixel by pixel
     double K;
 };
 
@@ -2932,8 +2934,9 @@ __device__ __forceinline__ void gm_wave_sync() {
 
 // Unit size of every head tile (0 for the others), from unit_of.
 __global__ void k_gm_heads(const float* __restrict__ grp_r, float rmax, int n_tiles,
-                           uint8_t* __restrict__ heads) {
+                           uint8_t* __restrict__ heads, unsigned int* __restrict__ fallbacks) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && fallbacks) *fallbacks = 0u;   // the evaluation's counter, when counting
     if (t >= n_tiles) return;
     int head, size;
     unit_of(t, n_tiles, grp_r, rmax, head, size);
@@ -3573,7 +3576,8 @@ void k_gm_eval(const GmArgs A) {
         const float kdo = sqrtf(fmaf(dox, dox, fmaf(doy, doy, doz * doz)));
         // the unit's cross term at the patch corners [rad]
         const float cmax = TWO_PI_F * kdo * hn * 256.f;
-        const bool sep = kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
+        const bool sep = !A.pixel_eval && kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
+        if (!sep && live && lane == 0 && A.fallbacks) atomicAdd(A.fallbacks, 1u);
         // G_m(y) = sum_n M_mn y^n at this lane's column (y = yb for the
         // product's N operand; the fallback evaluates at its own pixels)
         float Gr[D + 1], Gi[D + 1];
@@ -5104,7 +5108,8 @@ static bool gm_bounds(double r, const double* pixel_geom) {
 
 // Launch timing of the matrix-core moment sum, for measurement (enabled by
 // mosaic_kin_timing): HIP events on the sum's stream around each round's
-// moments and evaluation kernels, per device.
+// moments and evaluation kernels, and a count of the units its evaluation
+// took pixel by pixel, per device.
 namespace {
 constexpr int TIMING_DEVICES = 16;
 constexpr int TIMING_ROUNDS = 16;
@@ -5113,6 +5118,7 @@ struct KinTiming {
     bool made = false;
     int rounds = 0;
     hipEvent_t ev[TIMING_ROUNDS][4];
+    unsigned int* fallbacks = nullptr;   // device counter of the last call
 };
 KinTiming g_timing[TIMING_DEVICES];
 
@@ -5128,17 +5134,21 @@ void timing_mark(KinTiming* T, int r, int k, hipStream_t st) {
 }
 }  // namespace
 
-extern "C" int mosaic_kin_timing(int enable, float* ms_out) {
+extern "C" int mosaic_kin_timing(int enable, float* ms_out) {   // ms_out: host [3]
     KinTiming* T = timing_state();
     if (!T) return MOSAIC_ERR_HIP;
     if (enable && !T->made) {
         for (int r = 0; r < TIMING_ROUNDS; ++r)
             for (int k = 0; k < 4; ++k) MOSAIC_HIP_CHECK(hipEventCreate(&T->ev[r][k]));
+        MOSAIC_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&T->fallbacks), sizeof(unsigned int)));
+        MOSAIC_HIP_CHECK(hipMemset(T->fallbacks, 0, sizeof(unsigned int)));
         T->made = true;
     }
+    if (enable && !T->on) T->rounds = 0;   // no call timed since timing was off
     if (ms_out) {
         ms_out[0] = 0.f;
         ms_out[1] = 0.f;
+        ms_out[2] = 0.f;
         for (int r = 0; r < T->rounds && r < TIMING_ROUNDS; ++r) {
             MOSAIC_HIP_CHECK(hipEventSynchronize(T->ev[r][3]));
             float a = 0.f, b = 0.f;
@@ -5146,6 +5156,11 @@ extern "C" int mosaic_kin_timing(int enable, float* ms_out) {
             MOSAIC_HIP_CHECK(hipEventElapsedTime(&b, T->ev[r][2], T->ev[r][3]));
             ms_out[0] += a;
             ms_out[1] += b;
+        }
+        if (T->made && T->rounds > 0) {
+            unsigned int n = 0;
+            MOSAIC_HIP_CHECK(hipMemcpy(&n, T->fallbacks, sizeof(n), hipMemcpyDeviceToHost));
+            ms_out[2] = static_cast<float>(n);
         }
     }
     T->on = enable != 0;
@@ -5391,10 +5406,12 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
             std::max<int64_t>(1, std::min<int64_t>(L.n_tiles, static_cast<int64_t>(L.gm_count) / per_tile)));
         // moments kernel: about 4096 waves over the round's tiles
         const int64_t waves = static_cast<int64_t>(A.n_groups) * GM_WAVES * n_frames;
-        hipLaunchKernelGGL(k_gm_heads, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
-                           L.grp_r, grp_rmax, L.n_tiles, L.gm_head);
-        e = hipGetLastError();
         KinTiming* T = timing_state();
+        A.fallbacks = (T && T->on && T->made) ? T->fallbacks : nullptr;
+        A.pixel_eval = opts->gm_pixel_eval ? 1 : 0;
+        hipLaunchKernelGGL(k_gm_heads, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
+                           L.grp_r, grp_rmax, L.n_tiles, L.gm_head, A.fallbacks);
+        e = hipGetLastError();
         int round = 0;
         for (int tb = 0; tb < L.n_tiles && e == hipSuccess; tb += tiles_per_round, ++round) {
             const int te = std::min(L.n_tiles, tb + tiles_per_round);
@@ -5416,7 +5433,7 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
             timing_mark(T, round, 3, st);
             e = hipGetLastError();
         }
-        if (T) T->rounds = std::min(round, TIMING_ROUNDS);
+        if (T && T->on && T->made) T->rounds = std::min(round, TIMING_ROUNDS);   // events recorded
     } else if (path == 6) {
         ProbeArgs A;
         A.atoms = L.atoms;

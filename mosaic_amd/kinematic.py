@@ -59,7 +59,7 @@ def analyser_kind(mode):
 
 def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
              apply_decay=False, path="auto", analyser=0, analyser_axis=(1.0, 0.0, 0.0),
-             accept_rad=0.0, darwin_hw_rad=0.0):
+             accept_rad=0.0, darwin_hw_rad=0.0, gm_pixel_eval=False):
     """mosaic_kin_opts_t; ``path`` is "auto", "exact" (per pair), "tile"
     (per-tile factors), "rows" (row recurrence), "mma" (matrix-core patches),
     "poly" (moment expansion over the same patches), "gemm" (the moment
@@ -68,7 +68,9 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
     frame and pixel; "auto" takes it for at most 16384 pixel-poses); all
     but "auto" and "exact" decline when their bounds do not hold.  ``analyser`` is the kind (0 off, 1
     top-hat, 2 Darwin) with its axis (lab), acceptance half-angle and Darwin
-    half-width; it takes the exact kernel."""
+    half-width; it takes the exact kernel.  ``gm_pixel_eval`` makes "gemm"
+    evaluate every (unit, patch) pixel by pixel, its evaluation's fallback
+    from the separable phase model."""
     o = KinOpts()
     o.k = float(k)
     o.remove_forward = int(bool(remove_forward))
@@ -83,6 +85,7 @@ def kin_opts(k, remove_forward=False, apply_pol=False, pol=(0.0, 0.0, 1.0, 0.0),
         o.analyser_axis[d] = float(analyser_axis[d])
     o.analyser_accept_rad = float(accept_rad)
     o.analyser_darwin_hw_rad = float(darwin_hw_rad)
+    o.gm_pixel_eval = int(bool(gm_pixel_eval))
     return o
 
 
@@ -253,13 +256,17 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
     return out
 
 
-def sum_timing(enable=True):
+def sum_timing(enable=True, fallbacks=False):
     """Per-kernel launch times of the matrix-core moment sum (path "gemm"):
     enables or disables HIP events around its moments and evaluation kernels
     on the current device, and returns the last call's (moments, evaluation)
-    times in milliseconds (waits for their events)."""
-    ms = (ctypes.c_float * 2)()
+    times in milliseconds (waits for their events); with ``fallbacks`` also
+    the number of (unit, patch) evaluations its separable phase model
+    declined, which it evaluated pixel by pixel."""
+    ms = (ctypes.c_float * 3)()
     check(_lib.lib().mosaic_kin_timing(int(bool(enable)), ms), "mosaic_kin_timing")
+    if fallbacks:
+        return float(ms[0]), float(ms[1]), int(ms[2])
     return float(ms[0]), float(ms[1])
 
 

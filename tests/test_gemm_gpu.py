@@ -156,3 +156,55 @@ def test_gemm_matches_valu_moment_sum(device):
     assert w["beam"]._last_path % 100 in (45, 46)
     e = report("200k atoms x 256^2: gemm vs poly", A, B)
     assert e <= TOL
+
+
+def _staged_field(w, pixel_eval, device):
+    """The field of workload ``w`` through the staged C-ABI calls with path
+    "gemm", and the number of (unit, patch) evaluations taken pixel by pixel."""
+    from mosaic_amd import kinematic as K
+    from mosaic_amd import tables
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    pos = s.load_chunk_positions(1).astype(np.float32)
+    codes = np.zeros(pos.shape[0], np.uint8)
+    k = 2.0 * np.pi / b.wavelength
+    spc = K.species_table(tables.species_constants(["Au"], b.energy))
+    opts = K.kin_opts(k, False, True, b._polarization_state(), False, path="gemm",
+                      gm_pixel_eval=pixel_eval)
+    rot = np.asarray(st.rotation, np.float32)[None]
+    tr = np.asarray(st.translation, np.float32)[None]
+    coords = torch.from_numpy(np.ascontiguousarray(d.pixel_coordinates / 1e10)).to(device)
+    Ny, Nz = d.shape
+    d_pos = torch.from_numpy(pos).to(device)
+    amp = K.ein_bilinear(d_pos, rot[0], tr[0], torch.from_numpy(b._beam_E0_map).to(device),
+                         b._beam_du, b._beam_dv, b._beam_uc, b._beam_vc)
+    staged = K.stage(d_pos, torch.from_numpy(codes).to(device), amp, [pos.shape[0]], k, 1, Ny, Nz)
+    out = torch.zeros((1, Nz, Ny), dtype=torch.complex128, device=device)
+    K.sum_timing(True)
+    try:
+        K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+        torch.cuda.synchronize()
+        _, _, n_fb = K.sum_timing(True, fallbacks=True)
+    finally:
+        K.sum_timing(False)
+    return out[0].cpu().numpy().ravel(), staged.path_used, n_fb
+
+
+def test_gemm_pixel_eval_fallback(device):
+    """The evaluation's per-pixel fallback (taken for a unit whose origin is
+    too far from its slice's reference for the separable phase model), forced
+    for every unit: 32,000 atoms x 128^2 pixels (16 patches) against the FP64
+    restatement and against the separable evaluation of the same moments;
+    the count shows which evaluation ran (every (unit, patch), or none)."""
+    w = W.fcc_au_bragg111(n=20, npix=128, seed=0)
+    n_patches = 16
+    A, path, n_fb = _staged_field(w, True, device)
+    B, path_b, n_sep = _staged_field(w, False, device)
+    print(f"[gemm] pixel evaluation: paths {path} {path_b}, per-pixel (unit, patch) "
+          f"{n_fb} forced, {n_sep} by the model's check", flush=True)
+    assert path in GEMM and path_b in GEMM
+    n_tiles = (32000 + 255) // 256
+    assert n_fb % n_patches == 0 and n_fb >= n_patches * ((n_tiles + 3) // 4)
+    assert n_sep < n_fb
+    e = report("pixel evaluation vs f64", A, oracle_field(w, "f64"))
+    e2 = report("pixel evaluation vs separable", A, B)
+    assert e <= TOL and e2 <= TOL
