@@ -2863,31 +2863,45 @@ __device__ __forceinline__ void unit_of(int t, int n_tiles, const float* __restr
 // (k_gm_eval) evaluates the polynomials at the pixels with the per-(unit,
 // pixel) origin phase, as k_sum_poly does.
 //
-// Operands: every value x is split into FP16 halves x = hi + lo (22
-// significant bits) and each product is hi hi + hi lo + lo hi with FP32
-// accumulation (v_mfma_f32_32x32x16_f16: A row = lane % 32, B column = lane
-// % 32, k = 8 (lane / 32) + 0..7; result row 8 (r / 4) + 4 (lane / 32) + r % 4).
-// The carrier a_j takes one v_sin per operand value: the Re rows add a
-// quarter turn to the phase.  Each wave builds its basis rows for 32 atoms
-// at a time into its own LDS buffer (two lanes per atom), then runs two
-// 16-atom steps of 3 x 3 products; the waves of a workgroup share nothing.
+// Operands: values are split into FP16 halves x = hi + lo (22 significant
+// bits) and a product is hi hi + hi lo + lo hi with FP32 accumulation
+// (v_mfma_f32_32x32x16_f16: A row = lane % 32, B column = lane % 32, k = 8
+// (lane / 32) + 0..7; result row 8 (r / 4) + 4 (lane / 32) + r % 4).  The
+// columns sit in three 32-column blocks by the size of their terms: P^m R^n
+// (m + n <= 4) and sigma_c P^m R^n in blocks 0 and 1 with the split
+// products; P^m R^n (m + n = 5, 6: at most phi^5 / 5! ~ 2e-4 of the sum
+// within the degree-6 bound) and sigma_c sigma_d (at most eps^2 / 2 ~ 2e-4)
+// in block 2 with hi hi alone, whose rounding (2^-10 of those terms) stays
+// below 3e-7 of the sum.  The carrier a_j takes one v_sin per operand value:
+// the Re rows read the chunk's atom records with a quarter turn added to the
+// phase.  The four waves of a workgroup build the chunk's basis rows into
+// shared LDS (a lane per atom, each wave a quarter of the columns), then
+// each runs four 16-atom steps of 3 + 3 + 1 products for its patch group.
 constexpr int GM_WAVES = 4;                       // patch groups per workgroup
 constexpr int GM_GRP = 4;                         // a wave's group: GM_GRP x GM_GRP patches
 constexpr int GM_REG = 2 * GM_GRP;                // a workgroup's region: 2 x 2 groups
 constexpr int GM_CHUNK = 64;                      // atoms per basis chunk
 constexpr int GM_NT = 28;                         // P^m R^n, m + n <= 6
 constexpr int GM_N4 = 15;                         // m + n <= 4
-constexpr int GM_COL_S = GM_NT;                   // sigma_c P^m R^n: GM_COL_S + 15 c + j
-constexpr int GM_COL_SS = GM_NT + 3 * GM_N4;      // sigma_c sigma_d {1, P, R}: + 3 pair + s
+constexpr int GM_COL_S = GM_N4;                   // sigma_c P^m R^n: GM_COL_S + 15 c + j
+constexpr int GM_COL_T56 = GM_N4 + 3 * GM_N4;     // P^m R^n, m + n = 5 (by m), then 6 (by m)
+constexpr int GM_COL_SS = GM_COL_T56 + 13;        // sigma_c sigma_d {1, P, R}: + 3 pair + s
 constexpr int GM_NCOL = GM_COL_SS + 18;           // 91
 constexpr int GM_NB = 3;                          // 32-column blocks
+constexpr int GM_NSPLIT = 64;                     // blocks 0, 1: split products (lo halves kept)
 constexpr int GM_BS = 72;                         // halves per basis column: 64 atoms + pad (144 B)
 constexpr int GM_CS = 92;                         // floats per product row at a unit's end
 static_assert(GM_NCOL <= 32 * GM_NB, "basis columns fit the blocks");
 static_assert(GM_CS >= GM_NCOL, "a product row holds every basis column");
+static_assert(GM_COL_T56 + 4 == GM_NSPLIT && GM_COL_SS >= GM_NSPLIT,
+              "the degree <= 4 and first-order columns fill the split blocks");
 
 __host__ __device__ constexpr int gm_kT(int m, int n) { return 7 * m - m * (m - 1) / 2 + n; }
 __host__ __device__ constexpr int gm_k4(int m, int n) { return 5 * m - m * (m - 1) / 2 + n; }
+// basis column of P^m R^n, m + n <= 6
+__host__ __device__ constexpr int gm_colT(int m, int n) {
+    return m + n <= 4 ? gm_k4(m, n) : GM_COL_T56 + (m + n == 5 ? m : 6 + m);
+}
 
 struct GmArgs {
     const AtomRec* atoms;
@@ -2975,10 +2989,15 @@ typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void gm_split_pair(float x0, float x1, unsigned int& hi,
                                              unsigned int& lo) {
     const halfx2 h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
-    const halfx2 l = __builtin_amdgcn_cvt_pkrtz(x0 - static_cast<float>(h[0]),
-                                                x1 - static_cast<float>(h[1]));
     hi = __builtin_bit_cast(unsigned int, h);
-    lo = __builtin_bit_cast(unsigned int, l);
+    // the residuals x - hi, each one mixed-precision FMA (FP16 hi half of the
+    // packed word times -1 plus the FP32 value, rounded to FP16 into the low
+    // or high half of lo)
+    unsigned int l;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "+v"(l) : "v"(hi), "v"(x1));
+    lo = l;
 }
 
 // (re, im) times i^k
@@ -3016,9 +3035,9 @@ __device__ __forceinline__ void gm_fold(const float* cb, int p, const float* __r
     for (int m = 0; m <= 6; ++m) {
 #pragma unroll
         for (int n = 0; n <= 6 - m; ++n) {
-            const int k = gm_kT(m, n);
+            const int k = gm_kT(m, n), col = gm_colT(m, n);
             const float f = gm_inv_fact(m) * gm_inv_fact(n);
-            float re = cr[k] * f, im = ci[k] * f;
+            float re = cr[col] * f, im = ci[col] * f;
             gm_rot(m + n, re, im);
             Mr[k] = re;
             Mi[k] = im;
@@ -3174,8 +3193,8 @@ template <bool SECOND>
 __global__ __launch_bounds__(64 * GM_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void k_gm_moments(const GmArgs A) {
     __shared__ __align__(16) _Float16 sbh[GM_NB * 32 * GM_BS];   // basis hi halves [column][atom]
-    __shared__ __align__(16) _Float16 sbl[GM_NB * 32 * GM_BS];   // basis lo halves
-    __shared__ float4 sat[GM_CHUNK];                             // offset [wavelengths], phase [turns]
+    __shared__ __align__(16) _Float16 sbl[GM_NSPLIT * GM_BS];    // basis lo halves (blocks 0, 1)
+    __shared__ float4 sat[2][GM_CHUNK];   // offset [wavelengths], phase [turns]; phase + 1/4 (Re rows)
     __shared__ __align__(16) float scb[GM_WAVES][32 * GM_CS];    // each wave's products at a unit's end
     __shared__ float sev[GM_WAVES][16][15];                      // each patch's E_c coefficients
     __shared__ double sgs[GM_WAVES][8];                          // the waves' gradient sums
@@ -3214,7 +3233,6 @@ void k_gm_moments(const GmArgs A) {
         // the basis pad columns stay zero
         for (int i = tid; i < (GM_NB * 32 - GM_NCOL) * GM_BS; i += 64 * GM_WAVES) {
             sbh[GM_NCOL * GM_BS + i] = static_cast<_Float16>(0.f);
-            sbl[GM_NCOL * GM_BS + i] = static_cast<_Float16>(0.f);
         }
     }
     __syncthreads();
@@ -3246,7 +3264,7 @@ void k_gm_moments(const GmArgs A) {
     const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
     const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
     const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
-    const float shift = q < 16 ? 0.25f : 0.f;      // Re rows: cos = sin of a quarter turn more
+    const float4* const sa = sat[q < 16 ? 1 : 0];   // Re rows: cos = sin of a quarter turn more
     float* const cbuf = scb[wave];
 
     const int t0 = A.t_begin + slice * A.a_tiles_per_slice;
@@ -3284,7 +3302,7 @@ void k_gm_moments(const GmArgs A) {
             // zero: weight 0)
             {
                 const float sx = cur.sx + dx, sy = cur.sy + dy, sz = cur.sz + dz;
-                if (wave == 0) sat[lane] = make_float4(sx, sy, sz, cur.ph);
+                if (wave < 2) sat[wave][lane] = make_float4(sx, sy, sz, wave ? cur.ph + 0.25f : cur.ph);
                 const float Pb = fmaf(gby[2], sz, fmaf(gby[1], sy, gby[0] * sx));
                 const float Rb = fmaf(gbz[2], sz, fmaf(gbz[1], sy, gbz[0] * sx));
                 float pw[7], rw[7];
@@ -3299,13 +3317,14 @@ void k_gm_moments(const GmArgs A) {
                 auto put = [&](int col, float v) {
                     const _Float16 h = static_cast<_Float16>(v);
                     sbh[col * GM_BS + lane] = h;
-                    sbl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
+                    if (col < GM_NSPLIT)
+                        sbl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
                 };
 #pragma unroll
                 for (int m = 0; m <= 4; ++m)
 #pragma unroll
                     for (int n = 0; n <= 4 - m; ++n) {
-                        const int col = wave == 0 ? gm_kT(m, n)
+                        const int col = wave == 0 ? gm_k4(m, n)
                                                   : GM_COL_S + GM_N4 * (wave - 1) + gm_k4(m, n);
                         put(col, f * (pw[m] * rw[n]));
                     }
@@ -3316,14 +3335,14 @@ void k_gm_moments(const GmArgs A) {
                                      ms * sy * sy, ms * sy * sz, ms * sz * sz};
                 const float s3[3] = {1.f, Pb, Rb};
                 if (wave == 0) {
-                    put(gm_kT(0, 5), pw[0] * rw[5]); put(gm_kT(1, 4), pw[1] * rw[4]);
-                    put(gm_kT(2, 3), pw[2] * rw[3]); put(gm_kT(3, 2), pw[3] * rw[2]);
-                    put(gm_kT(4, 1), pw[4] * rw[1]); put(gm_kT(5, 0), pw[5] * rw[0]);
-                    put(gm_kT(0, 6), pw[0] * rw[6]); put(gm_kT(1, 5), pw[1] * rw[5]);
+                    put(gm_colT(0, 5), pw[0] * rw[5]); put(gm_colT(1, 4), pw[1] * rw[4]);
+                    put(gm_colT(2, 3), pw[2] * rw[3]); put(gm_colT(3, 2), pw[3] * rw[2]);
+                    put(gm_colT(4, 1), pw[4] * rw[1]); put(gm_colT(5, 0), pw[5] * rw[0]);
+                    put(gm_colT(0, 6), pw[0] * rw[6]); put(gm_colT(1, 5), pw[1] * rw[5]);
                 } else if (wave == 1) {
-                    put(gm_kT(2, 4), pw[2] * rw[4]); put(gm_kT(3, 3), pw[3] * rw[3]);
-                    put(gm_kT(4, 2), pw[4] * rw[2]); put(gm_kT(5, 1), pw[5] * rw[1]);
-                    put(gm_kT(6, 0), pw[6] * rw[0]);
+                    put(gm_colT(2, 4), pw[2] * rw[4]); put(gm_colT(3, 3), pw[3] * rw[3]);
+                    put(gm_colT(4, 2), pw[4] * rw[2]); put(gm_colT(5, 1), pw[5] * rw[1]);
+                    put(gm_colT(6, 0), pw[6] * rw[0]);
 #pragma unroll
                     for (int i = 0; i < 3; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
                 } else if (wave == 2) {
@@ -3343,9 +3362,8 @@ void k_gm_moments(const GmArgs A) {
                     float v[2];
 #pragma unroll
                     for (int e = 0; e < 2; ++e) {
-                        const float4 ar = sat[16 * s + 8 * g + 2 * u2 + e];
-                        float th =
-                            fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w + shift)));
+                        const float4 ar = sa[16 * s + 8 * g + 2 * u2 + e];
+                        float th = fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w)));
                         if (SECOND) {
                             const float w = fmaf(xc2, ar.z, fmaf(xc1, ar.y, xc0 * ar.x));
                             const float s2 = fmaf(ar.z, ar.z, fmaf(ar.y, ar.y, ar.x * ar.x));
@@ -3366,16 +3384,13 @@ void k_gm_moments(const GmArgs A) {
                 const halfx8 bh1 = *reinterpret_cast<const halfx8*>(sbh + (32 + q) * GM_BS + boff);
                 const halfx8 bl1 = *reinterpret_cast<const halfx8*>(sbl + (32 + q) * GM_BS + boff);
                 const halfx8 bh2 = *reinterpret_cast<const halfx8*>(sbh + (64 + q) * GM_BS + boff);
-                const halfx8 bl2 = *reinterpret_cast<const halfx8*>(sbl + (64 + q) * GM_BS + boff);
                 c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
                 c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
                 c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
-                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl2, c2, 0, 0, 0);
                 c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
                 c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
-                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh2, c2, 0, 0, 0);
             }
             __syncthreads();                       // the next chunk rewrites the basis
             if (!more) break;
