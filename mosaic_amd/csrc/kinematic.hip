@@ -36,6 +36,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -3192,10 +3193,16 @@ __device__ __forceinline__ void gm_patch_model(const FrameRec& F, const double4*
 template <bool SECOND>
 __global__ __launch_bounds__(64 * GM_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void k_gm_moments(const GmArgs A) {
-    __shared__ __align__(16) _Float16 sbh[GM_NB * 32 * GM_BS];   // basis hi halves [column][atom]
-    __shared__ __align__(16) _Float16 sbl[GM_NSPLIT * GM_BS];    // basis lo halves (blocks 0, 1)
-    __shared__ float4 sat[2][GM_CHUNK];   // offset [wavelengths], phase [turns]; phase + 1/4 (Re rows)
-    __shared__ __align__(16) float scb[GM_WAVES][32 * GM_CS];    // each wave's products at a unit's end
+    // Two chunk buffers, each: the basis hi halves [column][atom], the lo
+    // halves of blocks 0 and 1, and the chunk's atom records (offset
+    // [wavelengths], phase [turns]; then the same with the Re rows' quarter
+    // turn).  At a unit's end the same bytes hold each wave's products for
+    // the fold.
+    constexpr int SB_H = GM_NB * 32 * GM_BS * 2, SB_L = GM_NSPLIT * GM_BS * 2;
+    constexpr int SB_A = 2 * GM_CHUNK * 16, SB = SB_H + SB_L + SB_A;
+    static_assert(SB % 16 == 0 && 2 * SB >= GM_WAVES * 32 * GM_CS * 4,
+                  "the products at a unit's end fit the chunk buffers");
+    __shared__ __align__(16) unsigned char smem[2 * SB];
     __shared__ float sev[GM_WAVES][16][15];                      // each patch's E_c coefficients
     __shared__ double sgs[GM_WAVES][8];                          // the waves' gradient sums
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -3230,10 +3237,8 @@ void k_gm_moments(const GmArgs A) {
         if (lane == 0)
 #pragma unroll
             for (int k = 0; k < 7; ++k) sgs[wave][k] = sg[k];
-        // the basis pad columns stay zero
-        for (int i = tid; i < (GM_NB * 32 - GM_NCOL) * GM_BS; i += 64 * GM_WAVES) {
-            sbh[GM_NCOL * GM_BS + i] = static_cast<_Float16>(0.f);
-        }
+        // (the basis pad columns 91..95 are not written: their products land
+        // in result columns that are never read)
     }
     __syncthreads();
     double mg[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -3264,140 +3269,230 @@ void k_gm_moments(const GmArgs A) {
     const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
     const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
     const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
-    const float4* const sa = sat[q < 16 ? 1 : 0];   // Re rows: cos = sin of a quarter turn more
-    float* const cbuf = scb[wave];
+    float* const cbuf = reinterpret_cast<float*>(smem) + wave * 32 * GM_CS;
 
     const int t0 = A.t_begin + slice * A.a_tiles_per_slice;
     const int t1 = min(A.t_end, t0 + A.a_tiles_per_slice);
     GmHeadWalk walk(A.heads, t0, t1, lane);      // every wave walks the same units
     int t, gsz;
     while (walk.next(t, gsz)) {
-        const TileRec tr = A.tiles[t];
-        double uox, uoy, uoz;
-        gm_unit_origin(A.tiles, t, gsz, tr, uox, uoy, uoz);
+        // the unit's tiles (a unit of 2 repeats its second tile), its origin
+        // and each tile's origin offset from it [wavelengths]
+        TileRec trk[4];
+        trk[0] = A.tiles[t];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) trk[k] = A.tiles[t + min(k, gsz - 1)];
+        const TileRec& tr = trk[0];
+        double uox = tr.ox, uoy = tr.oy, uoz = tr.oz;
+        if (gsz > 1) {
+            uox = unit_origin(gsz, trk[0].ox, trk[1].ox, trk[2].ox, trk[3].ox);
+            uoy = unit_origin(gsz, trk[0].oy, trk[1].oy, trk[2].oy, trk[3].oy);
+            uoz = unit_origin(gsz, trk[0].oz, trk[1].oz, trk[2].oz, trk[3].oz);
+        }
+        // a chunk's place in the unit (tile ti, offset co) with its tile's
+        // count and origin offset [wavelengths]
+        struct ChunkAt {
+            int ti, co, cnt;
+            float dx, dy, dz;
+        };
+        auto at_tile = [&](int ti, const TileRec& r) {
+            ChunkAt c;
+            c.ti = ti;
+            c.co = 0;
+            c.cnt = r.count & 0xFFFF;
+            c.dx = static_cast<float>((r.ox - uox) * A.K);
+            c.dy = static_cast<float>((r.oy - uoy) * A.K);
+            c.dz = static_cast<float>((r.oz - uoz) * A.K);
+            return c;
+        };
+        // the chunk after c, false past the unit's last
+        auto advance = [&](ChunkAt& c) {
+            c.co += GM_CHUNK;
+            if (c.co >= c.cnt) {
+                if (c.ti + 1 >= gsz) return false;
+                c = at_tile(c.ti + 1, A.tiles[t + c.ti + 1]);
+            }
+            return true;
+        };
         const GmUnitPhase U = gm_unit_phase(F, uox, uoy, uoz, um, th_mid, vc0, A.K);
         floatx16 c0, c1, c2;
 #pragma unroll
         for (int r = 0; r < 16; ++r) { c0[r] = 0.f; c1[r] = 0.f; c2[r] = 0.f; }
-        // the unit's 64-atom chunks in order, each chunk's records loaded one
-        // chunk ahead (every wave loads all 64: a lane builds one atom's
-        // share of the basis)
-        int u = 0, cb0 = 0;
-        TileRec tu = tr;
-        AtomRec cur = A.atoms[static_cast<int64_t>(t) * TILE + lane];
-        while (true) {
-            const float dx = static_cast<float>((tu.ox - uox) * A.K);
-            const float dy = static_cast<float>((tu.oy - uoy) * A.K);
-            const float dz = static_cast<float>((tu.oz - uoz) * A.K);
-            const int cnt = tu.count & 0xFFFF;
-            int nu = u, nc = cb0 + GM_CHUNK;
-            if (nc >= cnt) { nu = u + 1; nc = 0; }
-            const bool more = nu < gsz;
-            const AtomRec nxt = A.atoms[static_cast<int64_t>(t + (more ? nu : u)) * TILE +
-                                        (more ? nc : cb0) + lane];
-            // the chunk's basis rows, each value times the atom's |w| (the
-            // carrier operand is then a plain sine): wave w writes the
-            // columns f_w P^m R^n (m + n <= 4), f = 1, sigma_x, sigma_y,
-            // sigma_z, and eight of the rest (records past the count are
-            // zero: weight 0)
-            {
-                const float sx = cur.sx + dx, sy = cur.sy + dy, sz = cur.sz + dz;
-                if (wave < 2) sat[wave][lane] = make_float4(sx, sy, sz, wave ? cur.ph + 0.25f : cur.ph);
-                const float Pb = fmaf(gby[2], sz, fmaf(gby[1], sy, gby[0] * sx));
-                const float Rb = fmaf(gbz[2], sz, fmaf(gbz[1], sy, gbz[0] * sx));
-                float pw[7], rw[7];
-                pw[0] = cur.mag;
-                rw[0] = 1.f;
+
+        // A chunk's basis rows, each value times the atom's |w| (the carrier
+        // operand is then a plain sine): wave w writes the columns f_w P^m
+        // R^n (m + n <= 4), f = 1, sigma_x, sigma_y, sigma_z, in four parts
+        // placed between the previous chunk's product steps, and then eight of
+        // the rest (records past a tile's count are zero: weight 0).
+        struct Rows {
+            float sx, sy, sz, f, Pb, Rb, ph;
+            float pw[7], rw[7];
+        };
+        auto rows_of = [&](const AtomRec& at, const ChunkAt& ca) {
+            Rows r;
+            r.sx = at.sx + ca.dx;
+            r.sy = at.sy + ca.dy;
+            r.sz = at.sz + ca.dz;
+            r.ph = at.ph;
+            r.Pb = fmaf(gby[2], r.sz, fmaf(gby[1], r.sy, gby[0] * r.sx));
+            r.Rb = fmaf(gbz[2], r.sz, fmaf(gbz[1], r.sy, gbz[0] * r.sx));
+            r.pw[0] = at.mag;
+            r.rw[0] = 1.f;
 #pragma unroll
-                for (int m = 1; m <= 6; ++m) {
-                    pw[m] = pw[m - 1] * Pb;
-                    rw[m] = rw[m - 1] * Rb;
-                }
-                const float f = wave == 0 ? 1.f : (wave == 1 ? sx : (wave == 2 ? sy : sz));
-                auto put = [&](int col, float v) {
-                    const _Float16 h = static_cast<_Float16>(v);
-                    sbh[col * GM_BS + lane] = h;
-                    if (col < GM_NSPLIT)
-                        sbl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
-                };
-#pragma unroll
-                for (int m = 0; m <= 4; ++m)
-#pragma unroll
-                    for (int n = 0; n <= 4 - m; ++n) {
-                        const int col = wave == 0 ? gm_k4(m, n)
-                                                  : GM_COL_S + GM_N4 * (wave - 1) + gm_k4(m, n);
-                        put(col, f * (pw[m] * rw[n]));
-                    }
-                // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
-                // waves 0..3 take eight each (the last one a pad column)
-                const float ms = cur.mag;
-                const float ss[6] = {ms * sx * sx, ms * sx * sy, ms * sx * sz,
-                                     ms * sy * sy, ms * sy * sz, ms * sz * sz};
-                const float s3[3] = {1.f, Pb, Rb};
-                if (wave == 0) {
-                    put(gm_colT(0, 5), pw[0] * rw[5]); put(gm_colT(1, 4), pw[1] * rw[4]);
-                    put(gm_colT(2, 3), pw[2] * rw[3]); put(gm_colT(3, 2), pw[3] * rw[2]);
-                    put(gm_colT(4, 1), pw[4] * rw[1]); put(gm_colT(5, 0), pw[5] * rw[0]);
-                    put(gm_colT(0, 6), pw[0] * rw[6]); put(gm_colT(1, 5), pw[1] * rw[5]);
-                } else if (wave == 1) {
-                    put(gm_colT(2, 4), pw[2] * rw[4]); put(gm_colT(3, 3), pw[3] * rw[3]);
-                    put(gm_colT(4, 2), pw[4] * rw[2]); put(gm_colT(5, 1), pw[5] * rw[1]);
-                    put(gm_colT(6, 0), pw[6] * rw[0]);
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
-                } else if (wave == 2) {
-#pragma unroll
-                    for (int i = 3; i < 11; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
-                } else {
-#pragma unroll
-                    for (int i = 11; i < 18; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
-                }
+            for (int m = 1; m <= 6; ++m) {
+                r.pw[m] = r.pw[m - 1] * r.Pb;
+                r.rw[m] = r.rw[m - 1] * r.Rb;
             }
-            __syncthreads();
+            r.f = wave == 0 ? 1.f : (wave == 1 ? r.sx : (wave == 2 ? r.sy : r.sz));
+            return r;
+        };
+        auto put_split = [&](_Float16* bh, _Float16* bl, int col, float v) {   // col < GM_NSPLIT
+            const _Float16 h = static_cast<_Float16>(v);
+            bh[col * GM_BS + lane] = h;
+            bl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
+        };
+        // part 0..3 of the main columns (m = 0; 1; 2; 3, 4) into buffer B;
+        // part 0 also writes the chunk's atom records
+        auto build_main = [&](auto B, const Rows& r, int part) {
+            constexpr int b = decltype(B)::value;
+            _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
+            _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
+            if (part == 0) {      // waves 2, 3 write the same values as waves 0, 1 (no branch)
+                float4* const ba = reinterpret_cast<float4*>(smem + b * SB + SB_H + SB_L);
+                const int sh = wave & 1;
+                ba[sh * GM_CHUNK + lane] = make_float4(r.sx, r.sy, r.sz, sh ? r.ph + 0.25f : r.ph);
+            }
+            const int col0 = wave == 0 ? 0 : GM_COL_S + GM_N4 * (wave - 1);
+#pragma unroll
+            for (int m = 0; m <= 4; ++m) {
+                if (m != part && !(part == 3 && m == 4)) continue;
+#pragma unroll
+                for (int n = 0; n <= 4 - m; ++n)
+                    put_split(bh, bl, col0 + gm_k4(m, n), r.f * (r.pw[m] * r.rw[n]));
+            }
+        };
+        // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
+        // waves 0..3 take eight each (the last one a pad column)
+        auto build_rest = [&](auto B, const Rows& r) {
+            constexpr int b = decltype(B)::value;
+            _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
+            _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
+            auto put = [&](int col, float v) {                   // col known at compile time
+                if (col < GM_NSPLIT) put_split(bh, bl, col, v);
+                else bh[col * GM_BS + lane] = static_cast<_Float16>(v);
+            };
+            const float ms = r.pw[0];
+            const float ss[6] = {ms * r.sx * r.sx, ms * r.sx * r.sy, ms * r.sx * r.sz,
+                                 ms * r.sy * r.sy, ms * r.sy * r.sz, ms * r.sz * r.sz};
+            const float s3[3] = {1.f, r.Pb, r.Rb};
+            const float* pw = r.pw;
+            const float* rw = r.rw;
+            if (wave == 0) {
+                put(gm_colT(0, 5), pw[0] * rw[5]); put(gm_colT(1, 4), pw[1] * rw[4]);
+                put(gm_colT(2, 3), pw[2] * rw[3]); put(gm_colT(3, 2), pw[3] * rw[2]);
+                put(gm_colT(4, 1), pw[4] * rw[1]); put(gm_colT(5, 0), pw[5] * rw[0]);
+                put(gm_colT(0, 6), pw[0] * rw[6]); put(gm_colT(1, 5), pw[1] * rw[5]);
+            } else if (wave == 1) {
+                put(gm_colT(2, 4), pw[2] * rw[4]); put(gm_colT(3, 3), pw[3] * rw[3]);
+                put(gm_colT(4, 2), pw[4] * rw[2]); put(gm_colT(5, 1), pw[5] * rw[1]);
+                put(gm_colT(6, 0), pw[6] * rw[0]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+            } else if (wave == 2) {
+#pragma unroll
+                for (int i = 3; i < 11; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+            } else {
+#pragma unroll
+                for (int i = 11; i < 18; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+            }
+        };
+        // Product step s (16 atoms) of the chunk in buffer B: the carrier (one
+        // sine per operand value) against the three blocks.
+        auto product_step = [&](auto B, int s) {
+            constexpr int b = decltype(B)::value;
+            const _Float16* const bh = reinterpret_cast<const _Float16*>(smem + b * SB);
+            const _Float16* const bl = reinterpret_cast<const _Float16*>(smem + b * SB + SB_H);
+            const float4* const sa =
+                reinterpret_cast<const float4*>(smem + b * SB + SB_H + SB_L) + (q < 16 ? GM_CHUNK : 0);
+            uintx4 hw, lw;
+#pragma unroll
+            for (int u2 = 0; u2 < 4; ++u2) {
+                float v[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float4 ar = sa[16 * s + 8 * g + 2 * u2 + e];
+                    float th = fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w)));
+                    if (SECOND) {
+                        const float w = fmaf(xc2, ar.z, fmaf(xc1, ar.y, xc0 * ar.x));
+                        const float s2 = fmaf(ar.z, ar.z, fmaf(ar.y, ar.y, ar.x * ar.x));
+                        th = fmaf(hh, fmaf(-w, w, s2), th);
+                    }
+                    v[e] = __builtin_amdgcn_sinf(th);
+                }
+                unsigned int hv, lv;
+                gm_split_pair(v[0], v[1], hv, lv);
+                hw[u2] = hv;
+                lw[u2] = lv;
+            }
+            const halfx8 ah = __builtin_bit_cast(halfx8, hw);
+            const halfx8 al = __builtin_bit_cast(halfx8, lw);
+            const int boff = 16 * s + 8 * g;
+            const halfx8 bh0 = *reinterpret_cast<const halfx8*>(bh + q * GM_BS + boff);
+            const halfx8 bl0 = *reinterpret_cast<const halfx8*>(bl + q * GM_BS + boff);
+            const halfx8 bh1 = *reinterpret_cast<const halfx8*>(bh + (32 + q) * GM_BS + boff);
+            const halfx8 bl1 = *reinterpret_cast<const halfx8*>(bl + (32 + q) * GM_BS + boff);
+            const halfx8 bh2 = *reinterpret_cast<const halfx8*>(bh + (64 + q) * GM_BS + boff);
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
+        };
+        auto rec = [&](const ChunkAt& c) {
+            return A.atoms[static_cast<int64_t>(t + c.ti) * TILE + c.co + lane];
+        };
+        // The unit's 64-atom chunks in order: the next chunk's basis goes into
+        // the other buffer between the current chunk's product steps, one
+        // barrier per chunk; each chunk's records are loaded a chunk ahead of
+        // its basis (every wave loads all 64: a lane builds one atom's share).
+        // After the last chunk the first one is built again into the buffer
+        // that nothing reads any more.
+        using B0 = std::integral_constant<int, 0>;
+        using B1 = std::integral_constant<int, 1>;
+        const ChunkAt first = at_tile(0, tr);
+        ChunkAt cn = first;
+        bool more = advance(cn);                         // cn: the chunk after the first
+        AtomRec nxt = rec(more ? cn : first);
+        {
+            const Rows r0 = rows_of(rec(first), first);
+#pragma unroll
+            for (int part = 0; part < 4; ++part) build_main(B0{}, r0, part);
+            build_rest(B0{}, r0);
+        }
+        __syncthreads();
+        // one chunk: products from buffer B, the next chunk's rows into the other
+        auto chunk = [&](auto B) {
+            using NB = std::integral_constant<int, 1 - decltype(B)::value>;
+            ChunkAt c2 = cn;
+            const bool more2 = more && advance(c2);
+            const AtomRec nxt2 = rec(more2 ? c2 : first);
+            const Rows r = rows_of(nxt, more ? cn : first);
 #pragma unroll
             for (int s = 0; s < GM_CHUNK / 16; ++s) {
-                uintx4 hw, lw;
-#pragma unroll
-                for (int u2 = 0; u2 < 4; ++u2) {
-                    float v[2];
-#pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        const float4 ar = sa[16 * s + 8 * g + 2 * u2 + e];
-                        float th = fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w)));
-                        if (SECOND) {
-                            const float w = fmaf(xc2, ar.z, fmaf(xc1, ar.y, xc0 * ar.x));
-                            const float s2 = fmaf(ar.z, ar.z, fmaf(ar.y, ar.y, ar.x * ar.x));
-                            th = fmaf(hh, fmaf(-w, w, s2), th);
-                        }
-                        v[e] = __builtin_amdgcn_sinf(th);
-                    }
-                    unsigned int hv, lv;
-                    gm_split_pair(v[0], v[1], hv, lv);
-                    hw[u2] = hv;
-                    lw[u2] = lv;
-                }
-                const halfx8 ah = __builtin_bit_cast(halfx8, hw);
-                const halfx8 al = __builtin_bit_cast(halfx8, lw);
-                const int boff = 16 * s + 8 * g;
-                const halfx8 bh0 = *reinterpret_cast<const halfx8*>(sbh + q * GM_BS + boff);
-                const halfx8 bl0 = *reinterpret_cast<const halfx8*>(sbl + q * GM_BS + boff);
-                const halfx8 bh1 = *reinterpret_cast<const halfx8*>(sbh + (32 + q) * GM_BS + boff);
-                const halfx8 bl1 = *reinterpret_cast<const halfx8*>(sbl + (32 + q) * GM_BS + boff);
-                const halfx8 bh2 = *reinterpret_cast<const halfx8*>(sbh + (64 + q) * GM_BS + boff);
-                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
-                c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
-                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
-                c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
-                c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
+                product_step(B, s);
+                build_main(NB{}, r, s);
             }
-            __syncthreads();                       // the next chunk rewrites the basis
-            if (!more) break;
-            if (nu != u) tu = A.tiles[t + nu];
-            u = nu;
-            cb0 = nc;
-            cur = nxt;
+            build_rest(NB{}, r);
+            __syncthreads();                       // buffer B is free, the other complete
+            const bool had = more;
+            cn = c2;
+            more = more2;
+            nxt = nxt2;
+            return had;
+        };
+        while (chunk(B0{}) && chunk(B1{})) {
         }
         // the products into this wave's LDS (rows: Re of patches 0..15, then
         // Im), then the fold
@@ -3416,7 +3511,7 @@ void k_gm_moments(const GmArgs A) {
             gm_fold(cbuf, lane, sev[wave][lane], make_float2(U.pht, U.kox),
                     make_float2(U.koy, U.koz), tr.species, dst);
         }
-        gm_wave_sync();                            // before the next unit's products
+        __syncthreads();                           // the next unit's first basis rewrites the products
     }
 }
 
