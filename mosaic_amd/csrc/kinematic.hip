@@ -3017,27 +3017,45 @@ __device__ __forceinline__ float gm_inv_fact(int m) {
            m == 4 ? (1.f / 24.f) : m == 5 ? (1.f / 120.f) : (1.f / 720.f);
 }
 
-// The fold of one patch: the 91 products (Re in row p, Im in row 16 + p of
-// cb) into the 28 polynomial coefficients, written to dst with the unit's
-// phase terms at the patch (pht, K O_x; K O_y, K O_z) and its species.  Ev[5 c .. 5 c + 4]
-// hold the patch's model difference E_c: x, y, x^2, y^2, x y coefficients [rad].
-__device__ __forceinline__ void gm_fold(const float* cb, int p, const float* __restrict__ Ev,
-                                        float2 ph0, float2 ph1, int species,
-                                        float2* __restrict__ dst) {
-    float E[3][5];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int u = 0; u < 5; ++u) E[c][u] = Ev[5 * c + u];
+// The fold of patch p = lane % 16 over the four lanes p + 16 j (j = 0..3)
+// of a wave: the 91 products (Re in row p, Im in row 16 + p of cb) into the
+// 28 polynomial coefficients, with the unit's phase terms at the patch (pht,
+// K O_x; K O_y, K O_z) and its species, written to dst when dst is not null.
+// Ev[p][5 c .. 5 c + 4] hold the patch's model difference E_c: x, y, x^2,
+// y^2, x y coefficients [rad].  Every lane runs the same instructions on its
+// own share: lanes j = 0..2 the first-order terms of sigma_j and the
+// second-order terms in the form sum_c E_c Y_c, Y_c = -1/2 sum_d E_d W_cd (the
+// sum over c <= d of s_cd E_c E_d W_cd with s_cc = -1/2, s_cd = -1 regrouped),
+// lane j = 0 the main terms; then a reduce-scatter over the four lanes
+// (v_permlane16_swap, v_permlane32_swap) leaves each lane 7 of the 28
+// coefficients.  All 64 lanes must take part.
+__device__ __forceinline__ void gm_fold4(const float* cb, int lane, const float (*Ev)[15],
+                                         float2 ph0, float2 ph1, int species,
+                                         float2* __restrict__ dst) {
+    constexpr int DM[5] = {1, 0, 2, 0, 1}, DN[5] = {0, 1, 0, 2, 1};   // E_c monomials
+    constexpr int YM[9] = {1, 0, 2, 1, 0, 3, 2, 1, 0}, YN[9] = {0, 1, 0, 1, 2, 0, 1, 2, 3};
+    auto yidx = [](int m, int n) { return (m + n == 1 ? 0 : (m + n == 2 ? 2 : 5)) + n; };
+    const int p = lane & 15, j = lane >> 4;
+    const int c = j < 3 ? j : 2;                       // lane j = 3: weight 0
     const float* cr = cb + p * GM_CS;
     const float* ci = cb + (16 + p) * GM_CS;
+    float E[3][5], Ec[5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int u = 0; u < 5; ++u) E[d][u] = Ev[p][5 * d + u];
+    const float w1 = j < 3 ? 1.f : 0.f;
+#pragma unroll
+    for (int u = 0; u < 5; ++u) Ec[u] = Ev[p][5 * c + u] * w1;
     float Mr[GM_MOM], Mi[GM_MOM];
+    // main terms (lane j = 0)
+    const float wm = j == 0 ? 1.f : 0.f;
 #pragma unroll
     for (int m = 0; m <= 6; ++m) {
 #pragma unroll
         for (int n = 0; n <= 6 - m; ++n) {
             const int k = gm_kT(m, n), col = gm_colT(m, n);
-            const float f = gm_inv_fact(m) * gm_inv_fact(n);
+            const float f = gm_inv_fact(m) * gm_inv_fact(n) * wm;
             float re = cr[col] * f, im = ci[col] * f;
             gm_rot(m + n, re, im);
             Mr[k] = re;
@@ -3045,68 +3063,86 @@ __device__ __forceinline__ void gm_fold(const float* cb, int p, const float* __r
         }
     }
     // first order: i E_c(x, y) sum_mn i^{m+n} / (m! n!) C[sigma_c P^m R^n] x^m y^n
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    {
+        const float* crc = cr + GM_COL_S + GM_N4 * c;
+        const float* cic = ci + GM_COL_S + GM_N4 * c;
 #pragma unroll
         for (int m = 0; m <= 4; ++m) {
 #pragma unroll
             for (int n = 0; n <= 4 - m; ++n) {
-                const int col = GM_COL_S + GM_N4 * c + gm_k4(m, n);
                 const float f = gm_inv_fact(m) * gm_inv_fact(n);
-                float vr = cr[col] * f, vi = ci[col] * f;
+                float vr = crc[gm_k4(m, n)] * f, vi = cic[gm_k4(m, n)] * f;
                 gm_rot(m + n + 1, vr, vi);
-                const int k10 = gm_kT(m + 1, n), k01 = gm_kT(m, n + 1), k20 = gm_kT(m + 2, n),
-                          k02 = gm_kT(m, n + 2), k11 = gm_kT(m + 1, n + 1);
-                Mr[k10] = fmaf(E[c][0], vr, Mr[k10]); Mi[k10] = fmaf(E[c][0], vi, Mi[k10]);
-                Mr[k01] = fmaf(E[c][1], vr, Mr[k01]); Mi[k01] = fmaf(E[c][1], vi, Mi[k01]);
-                Mr[k20] = fmaf(E[c][2], vr, Mr[k20]); Mi[k20] = fmaf(E[c][2], vi, Mi[k20]);
-                Mr[k02] = fmaf(E[c][3], vr, Mr[k02]); Mi[k02] = fmaf(E[c][3], vi, Mi[k02]);
-                Mr[k11] = fmaf(E[c][4], vr, Mr[k11]); Mi[k11] = fmaf(E[c][4], vi, Mi[k11]);
+#pragma unroll
+                for (int u = 0; u < 5; ++u) {
+                    const int k = gm_kT(m + DM[u], n + DN[u]);
+                    Mr[k] = fmaf(Ec[u], vr, Mr[k]);
+                    Mi[k] = fmaf(Ec[u], vi, Mi[k]);
+                }
             }
         }
     }
-    // second order: -1/2 E_c E_d (C[s_c s_d] + i C[s_c s_d P] x + i C[s_c s_d R] y)
-    constexpr int DM[5] = {1, 0, 2, 0, 1}, DN[5] = {0, 1, 0, 2, 1};
-    int pair = 0;
+    // second order: E_c Y_c, Y_c = -1/2 sum_d E_d (C[s_c s_d] + i C[s_c s_d P] x + i C[s_c s_d R] y)
+    {
+        float Yr[9], Yi[9];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+        for (int v = 0; v < 9; ++v) { Yr[v] = 0.f; Yi[v] = 0.f; }
 #pragma unroll
-        for (int d = c; d < 3; ++d, ++pair) {
-            const float sc = c == d ? -0.5f : -1.f;
-            float e2[GM_MOM];
-#pragma unroll
-            for (int k = 0; k < GM_MOM; ++k) e2[k] = 0.f;
+        for (int d = 0; d < 3; ++d) {
+            const int a = c < d ? c : d, b = c < d ? d : c;
+            const int pair = a == 0 ? b : (a == 1 ? 2 + b : 5);
+            const int col = GM_COL_SS + 3 * pair;
+            const float wr[3] = {cr[col], -ci[col + 1], -ci[col + 2]};
+            const float wi[3] = {ci[col], cr[col + 1], cr[col + 2]};
 #pragma unroll
             for (int u = 0; u < 5; ++u)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    const int k = gm_kT(DM[u] + DM[v], DN[u] + DN[v]);
-                    e2[k] = fmaf(E[c][u] * sc, E[d][v], e2[k]);
+                for (int t = 0; t < 3; ++t) {
+                    const int y = yidx(DM[u] + (t == 1 ? 1 : 0), DN[u] + (t == 2 ? 1 : 0));
+                    Yr[y] = fmaf(E[d][u], wr[t], Yr[y]);
+                    Yi[y] = fmaf(E[d][u], wi[t], Yi[y]);
                 }
-            const int col = GM_COL_SS + 3 * pair;
-            const float w0r = cr[col], w0i = ci[col];
-            const float w1r = -ci[col + 1], w1i = cr[col + 1];     // i C[.. P]
-            const float w2r = -ci[col + 2], w2i = cr[col + 2];     // i C[.. R]
+        }
 #pragma unroll
-            for (int m = 0; m <= 4; ++m) {
+        for (int u = 0; u < 5; ++u) {
+            const float e = -0.5f * Ec[u];
 #pragma unroll
-                for (int n = 0; n <= 4 - m; ++n) {
-                    if (m + n < 2) continue;
-                    const float e = e2[gm_kT(m, n)];
-                    const int k0 = gm_kT(m, n), k1 = gm_kT(m + 1, n), k2 = gm_kT(m, n + 1);
-                    Mr[k0] = fmaf(e, w0r, Mr[k0]); Mi[k0] = fmaf(e, w0i, Mi[k0]);
-                    Mr[k1] = fmaf(e, w1r, Mr[k1]); Mi[k1] = fmaf(e, w1i, Mi[k1]);
-                    Mr[k2] = fmaf(e, w2r, Mr[k2]); Mi[k2] = fmaf(e, w2i, Mi[k2]);
-                }
+            for (int v = 0; v < 9; ++v) {
+                const int k = gm_kT(DM[u] + YM[v], DN[u] + YN[v]);
+                Mr[k] = fmaf(e, Yr[v], Mr[k]);
+                Mi[k] = fmaf(e, Yi[v], Mi[k]);
             }
         }
     }
+    // reduce-scatter over the patch's four lanes: values v[2 k] = Mr[k],
+    // v[2 k + 1] = Mi[k]; rows 0, 1 (2, 3) of 16 lanes pair up first, then
+    // the two halves; lane j ends with v[14 q .. 14 q + 13], q = (j >> 1) | 2 (j & 1)
+    float h[28];
 #pragma unroll
-    for (int k = 0; k < GM_MOM; ++k) dst[k] = make_float2(Mr[k], Mi[k]);
-    dst[GM_MOM] = ph0;
-    dst[GM_MOM + 1] = ph1;
-    dst[GM_MOM + 2] = make_float2(__int_as_float(species), 0.f);
-    dst[GM_MOM + 3] = make_float2(0.f, 0.f);
+    for (int i = 0; i < 28; ++i) {
+        const float x = (i & 1) ? Mi[i >> 1] : Mr[i >> 1];
+        const float y = ((28 + i) & 1) ? Mi[(28 + i) >> 1] : Mr[(28 + i) >> 1];
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+        h[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    float o[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(h[i]), __float_as_uint(h[14 + i]),
+                                                        false, false);
+        o[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    if (dst) {
+        const int qb = 7 * ((j >> 1) | ((j & 1) << 1));
+#pragma unroll
+        for (int i = 0; i < 7; ++i) dst[qb + i] = make_float2(o[2 * i], o[2 * i + 1]);
+        if (j == 0) {
+            dst[GM_MOM] = ph0;
+            dst[GM_MOM + 1] = ph1;
+            dst[GM_MOM + 2] = make_float2(__int_as_float(species), 0.f);
+            dst[GM_MOM + 3] = make_float2(0.f, 0.f);
+        }
+    }
 }
 
 // Phase terms of a unit with origin (uox, uoy, uoz) [m, sample frame] at a
@@ -3504,12 +3540,13 @@ void k_gm_moments(const GmArgs A) {
             if (64 + q < GM_NCOL) cbuf[row * GM_CS + 64 + q] = c2[r];   // not the pad columns
         }
         gm_wave_sync();
-        if (lane < 16 && plive) {
+        {
             const int patch = ppz * A.pby + ppy;
-            float2* dst = A.mom + ((static_cast<int64_t>(t - A.t_begin) * A.n_frames + frame) *
-                                       A.n_patches + patch) * GM_REC;
-            gm_fold(cbuf, lane, sev[wave][lane], make_float2(U.pht, U.kox),
-                    make_float2(U.koy, U.koz), tr.species, dst);
+            float2* dst = plive ? A.mom + ((static_cast<int64_t>(t - A.t_begin) * A.n_frames + frame) *
+                                               A.n_patches + patch) * GM_REC
+                                : nullptr;
+            gm_fold4(cbuf, lane, sev[wave], make_float2(U.pht, U.kox), make_float2(U.koy, U.koz),
+                     tr.species, dst);
         }
         __syncthreads();                           // the next unit's first basis rewrites the products
     }
