@@ -170,6 +170,7 @@ struct Layout {
     KdgNode* kdg_nodes;     // nodes of every global level
     KdgChunk* kdg_chunks;   // chunks of every global level
     uint8_t* kdg_q8;        // bin of every position
+    float* kdg_xyz[2][3];   // coordinates moved with the order through the global levels
     uint32_t* kdg_bh;       // [chunk][KDG_BINS] histograms of one level
     uint2* kdg_cnt;         // per chunk of one level: its counts below / in the cut bin
     uint2* kdg_boff;        // per chunk of one level: the same counts before it in its node
@@ -360,6 +361,8 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
         L.kdg_nodes = c.take<KdgNode>(std::max<size_t>(1, z.nodes));
         L.kdg_chunks = c.take<KdgChunk>(std::max<size_t>(1, z.chunks));
         L.kdg_q8 = c.take<uint8_t>(n);
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 3; ++d) L.kdg_xyz[b][d] = c.take<float>(n);
         L.kdg_bh = c.take<uint32_t>(z.max_chunks * KDG_BINS);
         L.kdg_cnt = c.take<uint2>(z.max_chunks);
         L.kdg_boff = c.take<uint2>(z.max_chunks);
@@ -738,11 +741,32 @@ __global__ void k_kd_box_float(const unsigned int* __restrict__ box, int64_t wor
     if (i < words) fbox[i] = ord2f(box[i]);
 }
 
+// The global levels move each position's coordinates and atom index
+// together, in four arrays (k_kdg_gather fills the first set from the order
+// of the species sort), so that every level reads and writes them
+// contiguously.
+struct KdgSet {
+    float* x;
+    float* y;
+    float* z;
+    uint32_t* idx;
+};
+
+__global__ void k_kdg_gather(const float* __restrict__ pos, const uint32_t* __restrict__ order,
+                             int64_t n, KdgSet out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t a = order[i];
+    out.x[i] = pos[3 * static_cast<int64_t>(a)];
+    out.y[i] = pos[3 * static_cast<int64_t>(a) + 1];
+    out.z[i] = pos[3 * static_cast<int64_t>(a) + 2];
+    out.idx[i] = a;
+}
+
 // Global level: tight boxes of the nodes, one chunk per workgroup (6
 // atomics per chunk on order-preserving images of the floats).
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
-    const float* __restrict__ pos, const uint32_t* __restrict__ order,
-    const KdgChunk* __restrict__ chunks, unsigned int* __restrict__ box) {
+    const KdgSet in, const KdgChunk* __restrict__ chunks, unsigned int* __restrict__ box) {
     __shared__ unsigned int red[KDG_THREADS / 64][6];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const KdgChunk C = chunks[blockIdx.x];
@@ -751,11 +775,10 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
     for (int r = 0; r < KDG_PT; ++r) {
         const uint32_t i = C.begin + tid + r * KDG_THREADS;
         if (i < C.end) {
-            const uint32_t a = order[i];
+            const unsigned int o[3] = {f2ord(in.x[i]), f2ord(in.y[i]), f2ord(in.z[i])};
             for (int d = 0; d < 3; ++d) {
-                const unsigned int o = f2ord(pos[3 * static_cast<int64_t>(a) + d]);
-                lo[d] = min(lo[d], o);
-                hi[d] = max(hi[d], o);
+                lo[d] = min(lo[d], o[d]);
+                hi[d] = max(hi[d], o[d]);
             }
         }
     }
@@ -776,11 +799,10 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
     }
 }
 
-// Global level: each atom's bin (KDG_BINS over its node's longest side) and
-// the chunk's histogram of them.
+// Global level: each position's bin (KDG_BINS over its node's longest side)
+// and the chunk's histogram of them.
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_bin(
-    const float* __restrict__ pos, const uint32_t* __restrict__ order,
-    const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
+    const KdgSet in, const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
     const float* __restrict__ fbox, uint8_t* __restrict__ q8, uint32_t* __restrict__ bh) {
     __shared__ unsigned int h[KDG_BINS];
     const int tid = threadIdx.x;
@@ -792,12 +814,12 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_bin(
         float lo[3], ext[3];
         const int ax = kd_axis(fbox + 6 * static_cast<int64_t>(C.node), lo, ext);
         const float sc = ext[ax] > 0.f ? static_cast<float>(KDG_BINS) / ext[ax] : 0.f;
+        const float* __restrict__ c = ax == 0 ? in.x : (ax == 1 ? in.y : in.z);
 #pragma unroll
         for (int r = 0; r < KDG_PT; ++r) {
             const uint32_t i = C.begin + tid + r * KDG_THREADS;
             if (i < C.end) {
-                const uint32_t a = order[i];
-                const float t = (pos[3 * static_cast<int64_t>(a) + ax] - lo[ax]) * sc;
+                const float t = (c[i] - lo[ax]) * sc;
                 const int q = min(KDG_BINS - 1, max(0, static_cast<int>(t)));
                 q8[i] = static_cast<uint8_t>(q);
                 atomicAdd(&h[q], 1u);
@@ -929,61 +951,77 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_chunk_scan(
 }
 
 // Global level: the stable partition of each chunk into its node's halves
-// (a node that does not split is copied).
+// (a node that does not split is copied).  Wave w takes the chunk's
+// positions 512 w + 64 r + lane (r = 0..7), so neighbouring lanes hold
+// neighbouring positions and mostly write neighbouring slots; the counts
+// before a position come from ballots (within a step), running sums over the
+// steps and the waves before it (through LDS).
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_scatter(
     const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
-    const uint8_t* __restrict__ q8, const uint32_t* __restrict__ order_in,
-    const int2* __restrict__ sel, const uint2* __restrict__ boff,
-    uint32_t* __restrict__ order_out) {
-    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const uint8_t* __restrict__ q8, const KdgSet in, const int2* __restrict__ sel,
+    const uint2* __restrict__ boff, const KdgSet out) {
+    static_assert(KDG_CHUNK == 4 * 64 * KDG_PT && KDG_THREADS == 256, "four waves of 8 steps");
+    __shared__ uint32_t wb[4], wt[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const KdgChunk C = chunks[blockIdx.x];
     const KdgNode N = nodes[C.node];
-    const uint32_t i0 = C.begin + KDG_PT * tid;
+    const uint32_t e0 = C.begin + 512 * wave + lane;
+    auto move = [&](uint32_t from, int64_t to) {
+        out.x[to] = in.x[from];
+        out.y[to] = in.y[from];
+        out.z[to] = in.z[from];
+        out.idx[to] = in.idx[from];
+    };
     if (N.count <= static_cast<uint32_t>(TILE)) {
 #pragma unroll
         for (int r = 0; r < KDG_PT; ++r)
-            if (i0 + r < C.end) order_out[i0 + r] = order_in[i0 + r];
+            if (e0 + 64 * r < C.end) move(e0 + 64 * r, e0 + 64 * r);
         return;
     }
     const int2 sl = sel[C.node];
     const uint2 bo = boff[blockIdx.x];
-    uint32_t fl[KDG_PT], own = 0;
+    auto below_mask = [](uint64_t m) {             // set lanes below this one
+        return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    };
+    bool lb[KDG_PT], tb[KDG_PT];
+    uint32_t nb = 0, nt = 0;
 #pragma unroll
     for (int r = 0; r < KDG_PT; ++r) {
-        fl[r] = 0u;
-        if (i0 + r < C.end) {
-            const int q = q8[i0 + r];
-            fl[r] = q < sl.x ? 0x10000u : (q == sl.x ? 1u : 0u);
-        }
-        own += fl[r];
+        const uint32_t e = e0 + 64 * r;
+        const int q = e < C.end ? static_cast<int>(q8[e]) : 256;
+        lb[r] = q < sl.x;
+        tb[r] = q == sl.x;
+        nb += __popcll(__ballot(lb[r]));
+        nt += __popcll(__ballot(tb[r]));
     }
-    uint32_t inc = own;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
-        if (lane >= o) inc += y;
+    if (lane == 0) {
+        wb[wave] = nb;
+        wt[wave] = nt;
     }
-    if (lane == 63) wsum[wave] = inc;
     __syncthreads();
-    uint32_t run = inc - own;
-    for (int w = 0; w < wave; ++w) run += wsum[w];
+    uint32_t runb = bo.x, runt = bo.y;
+    for (int w = 0; w < wave; ++w) {
+        runb += wb[w];
+        runt += wt[w];
+    }
 #pragma unroll
     for (int r = 0; r < KDG_PT; ++r) {
-        const uint32_t i = i0 + r;
-        if (i < C.end) {
-            const int lb = static_cast<int>(run >> 16) + static_cast<int>(bo.x);
-            const int tb = static_cast<int>(run & 0xFFFFu) + static_cast<int>(bo.y);
-            const int before = lb + min(tb, sl.y);
-            const bool left = fl[r] == 0x10000u || (fl[r] == 1u && tb < sl.y);
+        const uint32_t e = e0 + 64 * r;
+        const uint64_t mb = __ballot(lb[r]), mt = __ballot(tb[r]);
+        const uint32_t before_b = runb + below_mask(mb), before_t = runt + below_mask(mt);
+        if (e < C.end) {
+            const uint32_t before = before_b + min(before_t, static_cast<uint32_t>(sl.y));
+            const bool left = lb[r] || (tb[r] && before_t < static_cast<uint32_t>(sl.y));
             const int64_t dst = left ? static_cast<int64_t>(N.begin) + before
                                      : static_cast<int64_t>(N.begin) + N.need +
-                                           (static_cast<int64_t>(i) - N.begin - before);
+                                           (static_cast<int64_t>(e) - N.begin - before);
             // inside the node by construction; the guard keeps a logic error
             // from writing elsewhere
-            if (dst >= N.begin && dst < static_cast<int64_t>(N.begin) + N.count)
-                order_out[dst] = order_in[i];
+            if (dst >= N.begin && dst < static_cast<int64_t>(N.begin) + N.count) move(e, dst);
         }
-        run += fl[r];
+        runb += __popcll(mb);
+        runt += __popcll(mt);
     }
 }
 
@@ -1156,13 +1194,13 @@ __global__ __launch_bounds__(TILE) void k_unit_radii(const TileRec* __restrict__
 // the cut bin" and "in the cut bin" flags) in which the first positions of
 // the cut bin go left until the count is met.  The halves overlap by at most
 // one bin.  Then the node's order is written back and its tiles are emitted
-// as k_tiles emits them (read from order_in, written to order, which may be
-// the same buffer).  Thread tid holds positions 8 tid .. 8 tid + 7, so
+// as k_tiles emits them (the coordinates and atom indices read from the
+// global levels' arrays; the order written to order, which may be their
+// index array).  Thread tid holds positions 8 tid .. 8 tid + 7, so
 // a half-wave covers one tile, which never straddles two sub-nodes.
 template <int LV>
 __global__ __launch_bounds__(32 << LV) void k_kd_local(
-    const float* __restrict__ pos, const float2* __restrict__ amp, const uint32_t* order_in,
-    uint32_t* order,
+    const KdgSet in, const float2* __restrict__ amp, uint32_t* order,
     const KdNode* __restrict__ nodes, double K, TileRec* __restrict__ tiles,
     AtomRec* __restrict__ atoms, float* __restrict__ grp_r, uint32_t* __restrict__ tile_pos,
     unsigned int* __restrict__ radius_bits) {
@@ -1189,20 +1227,15 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
     const int64_t begin = nd.begin;
     const int p0 = PT * tid;                          // this thread's first position
 
-    {
-        uint32_t a[PT];
 #pragma unroll
-        for (int i = 0; i < PT; ++i) {
-            const int p = tid + i * THREADS;
-            a[i] = p < n ? order_in[begin + p] : 0u;
+    for (int i = 0; i < PT; ++i) {
+        const int p = tid + i * THREADS;
+        if (p < n) {
+            xyz[0][p] = in.x[begin + p];
+            xyz[1][p] = in.y[begin + p];
+            xyz[2][p] = in.z[begin + p];
         }
-#pragma unroll
-        for (int i = 0; i < PT; ++i) {
-            const int p = tid + i * THREADS;
-            if (p < n)
-                for (int d = 0; d < 3; ++d) xyz[d][p] = pos[3 * static_cast<int64_t>(a[i]) + d];
-            slot[kidx(p)] = static_cast<uint32_t>(p);
-        }
+        slot[kidx(p)] = static_cast<uint32_t>(p);
     }
     // sub-node (f, j) is heap entry 2^f - 1 + j; its range follows the cuts
     // along the path from the root
@@ -1450,7 +1483,7 @@ __global__ __launch_bounds__(32 << LV) void k_kd_local(
         at[i] = 0u;
         if (p < n) {
             sl[i] = slot[kidx(p)];
-            at[i] = order_in[begin + sl[i]];
+            at[i] = in.idx[begin + sl[i]];
         }
     }
     __syncthreads();                                  // slot reads above are done
@@ -5018,8 +5051,11 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         }
     }
     if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
-    uint32_t* cur = L.vals_b;                     // the order after the species sort
-    uint32_t* other = L.vals_a;
+    // the coordinates in the order of the species sort (its order array is the
+    // first set's index array), then the levels move them between two sets
+    KdgSet cur = {L.kdg_xyz[0][0], L.kdg_xyz[0][1], L.kdg_xyz[0][2], L.vals_b};
+    KdgSet other = {L.kdg_xyz[1][0], L.kdg_xyz[1][1], L.kdg_xyz[1][2], L.vals_a};
+    hipLaunchKernelGGL(k_kdg_gather, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b, atom_count, cur);
     if (glevels > 0) {
         MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, gnodes.data(), gnodes.size() * sizeof(KdgNode),
                                         hipMemcpyHostToDevice, st));
@@ -5037,12 +5073,12 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
             if (tight(l)) {
                 hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
                                    st, L.kd_box, 6 * nodes);
-                hipLaunchKernelGGL(k_kdg_box, dim3(nchunks), dim3(KDG_THREADS), 0, st, pos, cur, ch,
+                hipLaunchKernelGGL(k_kdg_box, dim3(nchunks), dim3(KDG_THREADS), 0, st, cur, ch,
                                    L.kd_box);
                 hipLaunchKernelGGL(k_kd_box_float, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
                                    st, L.kd_box, 6 * nodes, fbox);
             }
-            hipLaunchKernelGGL(k_kdg_bin, dim3(nchunks), dim3(KDG_THREADS), 0, st, pos, cur, ch, nd,
+            hipLaunchKernelGGL(k_kdg_bin, dim3(nchunks), dim3(KDG_THREADS), 0, st, cur, ch, nd,
                                fbox, L.kdg_q8, L.kdg_bh);
             const int S = kdg_slices(gnodes, node_off[l], node_off[l + 1]);
             hipLaunchKernelGGL(k_kdg_total, dim3(S, static_cast<unsigned int>(nodes)),
@@ -5067,15 +5103,15 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         const KdNode* nds = L.kd_nodes;
         // the final order goes to vals_b (mosaic_kin_restage reads it there)
         if (lcount[0] > 0)
-            hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, pos, a2, cur,
+            hipLaunchKernelGGL(k_kd_local<3>, dim3(lcount[0]), dim3(32 << 3), 0, st, cur, a2,
                                L.vals_b, nds, K, L.tiles, L.atoms, L.grp_r, L.tile_pos,
                                L.radius_bits);
         if (lcount[1] > 0)
-            hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, pos, a2, cur,
+            hipLaunchKernelGGL(k_kd_local<4>, dim3(lcount[1]), dim3(32 << 4), 0, st, cur, a2,
                                L.vals_b, nds + lcount[0], K, L.tiles, L.atoms, L.grp_r,
                                L.tile_pos, L.radius_bits);
         if (lcount[2] > 0)
-            hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, pos, a2, cur,
+            hipLaunchKernelGGL(k_kd_local<5>, dim3(lcount[2]), dim3(32 << 5), 0, st, cur, a2,
                                L.vals_b, nds + lcount[0] + lcount[1], K, L.tiles, L.atoms,
                                L.grp_r, L.tile_pos, L.radius_bits);
     }
