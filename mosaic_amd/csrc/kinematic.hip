@@ -3610,7 +3610,8 @@ hipError_t launch_gm_moments(const GmArgs& A, int slices, hipStream_t st) {
 // x^{m+1} (cross), N = beta G_m(y) or -256 i kappa y beta G_m(y), G_m(y) =
 // sum_n M_mn y^n: one complex 32 x 32 x 16 product per unit, which the
 // matrix cores accumulate over the units in the pixels' running sums (FP16
-// split operands, 12 products).  The model's error, K |dO| times the largest
+// split operands, 12 products).  Each half of the wave holds the k slots of
+// four powers m (so a lane forms four G_m and four T = beta G_m, not seven).  The model's error, K |dO| times the largest
 // deviation of the lane pixels' du from it (measured per patch), must stay
 // within 1e-6 revolutions and the cross term's second order within 1e-6;
 // a unit that fails either is evaluated pixel by pixel with its exact
@@ -3646,18 +3647,28 @@ void k_gm_eval(const GmArgs A) {
     // offset), N column b = q (its y powers and middle-column offset)
     const float xa = static_cast<float>(min(q, P.vy - 1) - P.my) * (1.f / 16.f);
     const float yb = static_cast<float>(min(q, P.vz - 1) - P.mz) * (1.f / 16.f);
+    // The product's 16 k slots: half g of the wave holds four powers m of
+    // the patch polynomial, m in {0, 3, 5, 6} (g = 0) or {1, 2, 4} and a pad
+    // (g = 1), which balances the Horner sums G_m(y) = sum_{n <= 6 - m} M_mn
+    // y^n between the halves; slot j < 4 is the main term of the j-th power,
+    // slot 4 + j its cross term.  L(a, k) = alpha x^m (main) or alpha x^{m+1}
+    // (cross); each lane's four powers, the record offset of M_m0 and the
+    // largest n of each.
+    constexpr int GM_MA[4] = {0, 3, 5, 6}, GM_MB[4] = {1, 2, 4, 6};
+    constexpr int GM_HL[4] = {7, 5, 3, 1};         // Horner length of slot s (the larger half's)
+    int hbase[4], hlim[4];
     float lx[8];
     {
         float xp[8];
         xp[0] = 1.f;
 #pragma unroll
         for (int m = 1; m < 8; ++m) xp[m] = xp[m - 1] * xa;
-        // k = 8 g + j: main m = k (k < 7), cross m = k - 7 (x^{m+1}), pads
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float v0 = j < 7 ? xp[j] : xp[1];
-            const float v1 = j < 6 ? xp[j + 2] : 0.f;
-            lx[j] = g ? v1 : v0;
+        for (int j = 0; j < 4; ++j) {
+            lx[j] = g ? xp[GM_MB[j]] : xp[GM_MA[j]];
+            lx[4 + j] = g ? xp[GM_MB[j] + 1] : xp[GM_MA[j] + 1];
+            hbase[j] = g ? gm_kT(GM_MB[j], 0) : gm_kT(GM_MA[j], 0);
+            hlim[j] = g ? (j == 3 ? -1 : 6 - GM_MB[j]) : 6 - GM_MA[j];   // the pad: no term
         }
     }
     const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
@@ -3758,25 +3769,26 @@ void k_gm_eval(const GmArgs A) {
         const float cmax = TWO_PI_F * kdo * hn * 256.f;
         const bool sep = !A.pixel_eval && kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
         if (!sep && live && lane == 0 && A.fallbacks) atomicAdd(A.fallbacks, 1u);
-        // G_m(y) = sum_n M_mn y^n at this lane's column (y = yb for the
-        // product's N operand; the fallback evaluates at its own pixels)
-        float Gr[D + 1], Gi[D + 1];
-        {
-            int base = 0;
+        if (sep) {
+            // G of this lane's four powers at its column y = yb; a half's
+            // top term beyond its own degree is zero
+            float Gr[4], Gi[4];
 #pragma unroll
-            for (int m = 0; m <= D; ++m) {
+            for (int j = 0; j < 4; ++j) {
                 float gr = 0.f, gi = 0.f;
 #pragma unroll
-                for (int n = D - m; n >= 0; --n) {
-                    gr = fmaf(gr, yb, rw[2 * (base + n)]);
-                    gi = fmaf(gi, yb, rw[2 * (base + n) + 1]);
+                for (int n = GM_HL[j] - 1; n >= 0; --n) {
+                    float cr = rw[2 * (hbase[j] + n)], ci = rw[2 * (hbase[j] + n) + 1];
+                    if (n == GM_HL[j] - 1) {
+                        cr = n <= hlim[j] ? cr : 0.f;
+                        ci = n <= hlim[j] ? ci : 0.f;
+                    }
+                    gr = fmaf(gr, yb, cr);
+                    gi = fmaf(gi, yb, ci);
                 }
-                Gr[m] = gr;
-                Gi[m] = gi;
-                base += D - m + 1;
+                Gr[j] = gr;
+                Gi[j] = gi;
             }
-        }
-        if (sep) {
             const float tha = -fmaf(doz, dr.z, fmaf(doy, dr.y, dox * dr.x));
             const float ar = __builtin_amdgcn_cosf(tha), ai = __builtin_amdgcn_sinf(tha);
             const float thb = pht - fmaf(doz, dc.z, fmaf(doy, dc.y, dox * dc.x));
@@ -3784,24 +3796,20 @@ void k_gm_eval(const GmArgs A) {
             const float kap = TWO_PI_F * fmaf(doz, hz, fmaf(doy, hy, dox * hx));
             // cross factor -256 i kappa y
             const float cf = -256.f * kap * yb;
-            float Tr[D + 1], Ti[D + 1];
-#pragma unroll
-            for (int m = 0; m <= D; ++m) {
-                Tr[m] = br * Gr[m] - bi * Gi[m];
-                Ti[m] = br * Gi[m] + bi * Gr[m];
-            }
-            // N(k, b): g = 0: T_0..T_6, then the cross T_0; g = 1: the cross
-            // T_1..T_6, two pads.  The cross value is cf i T = (-cf Ti, cf Tr).
+            // N(k, b): slot j < 4: T = beta G of the j-th power; slot 4 + j its
+            // cross value cf i T = (-cf Ti, cf Tr)
             float nr[8], ni[8], lr[8], li[8];
 #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float tr = br * Gr[j] - bi * Gi[j];
+                const float ti = br * Gi[j] + bi * Gr[j];
+                nr[j] = tr;
+                ni[j] = ti;
+                nr[4 + j] = -cf * ti;
+                ni[4 + j] = cf * tr;
+            }
+#pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int c0 = j < 7 ? j : 0, c1 = j < 6 ? j + 1 : 0;
-                const float m0r = j < 7 ? Tr[c0] : -cf * Ti[c0];
-                const float m0i = j < 7 ? Ti[c0] : cf * Tr[c0];
-                const float m1r = j < 6 ? -cf * Ti[c1] : 0.f;
-                const float m1i = j < 6 ? cf * Tr[c1] : 0.f;
-                nr[j] = g ? m1r : m0r;
-                ni[j] = g ? m1i : m0i;
                 lr[j] = ar * lx[j];
                 li[j] = ai * lx[j];
             }
@@ -3836,6 +3844,22 @@ void k_gm_eval(const GmArgs A) {
         } else {
             // exact phase per pixel: G at this lane's column is G_m(yb) (the
             // lane's pixels share its column), Horner's rule in x per pixel
+            float Gr[D + 1], Gi[D + 1];
+            {
+                int base = 0;
+#pragma unroll
+                for (int m = 0; m <= D; ++m) {
+                    float gr = 0.f, gi = 0.f;
+#pragma unroll
+                    for (int n = D - m; n >= 0; --n) {
+                        gr = fmaf(gr, yb, rw[2 * (base + n)]);
+                        gi = fmaf(gi, yb, rw[2 * (base + n) + 1]);
+                    }
+                    Gr[m] = gr;
+                    Gi[m] = gi;
+                    base += D - m + 1;
+                }
+            }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const float x = static_cast<float>(arow(r) - P.my) * (1.f / 16.f);
