@@ -60,9 +60,9 @@ COUNTERS = {
         add_f32=2.06103e8, mul_f32=8.25971e8, fma_f32=2.82295e9, trans_f32=9.04499e7),
     "k_gm_moments": dict(
         source="profiles/k_gm_moments_perovskite5M_512px.md",
-        fetch_kib=517181.5, write_kib=723943.4, valu=6.405e8,
-        add_f32=8.4375e7, mul_f32=4.4519e7, fma_f32=2.3917e8, trans_f32=4.0004e7,
-        mfma=4.5e7),
+        fetch_kib=316290.0, write_kib=440380.0, valu=4.6777e8,
+        add_f32=1.0064e7, mul_f32=4.2031e7, fma_f32=1.6177e8, trans_f32=4.0004e7,
+        mfma=3.5e7),
     "k_sum_mma": dict(
         source="profiles/k_sum_mma_perovskite5M_512px.md",
         fetch_kib=1831133.0, write_kib=2542443.8, valu=None,

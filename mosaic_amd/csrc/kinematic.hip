@@ -5033,19 +5033,9 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         split.tile_start[s + 1] =
             split.tile_start[s] + static_cast<int>((counts[s] + TILE - 1) / TILE);
     }
-    const unsigned int n32 = static_cast<unsigned int>(atom_count);
-    const int nb = blocks_for(atom_count, KD_THREADS);
-    hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
-    // group by species (stable), then bisect each species level by level:
-    // the global levels by chunks, the last (at most KDL_LEVELS) in LDS
-    size_t sb = L.sort_bytes;
-    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
-                       L.keys_a, L.vals_a);
-    MOSAIC_HIP_CHECK(hipGetLastError());
-    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
-                                               L.vals_b, n32, 0, 4, st));
     const int glevels = kd_global_levels(n_species, counts);
-    // host tables alive until the synchronisation below
+    // the host plan first (host tables alive until the synchronisation
+    // below), so that it overlaps the stream's earlier work
     std::vector<KdgNode> gnodes;
     std::vector<KdgChunk> gchunks;
     std::vector<size_t> node_off, chunk_off;
@@ -5075,6 +5065,17 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         }
     }
     if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
+    const unsigned int n32 = static_cast<unsigned int>(atom_count);
+    const int nb = blocks_for(atom_count, KD_THREADS);
+    hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
+    // group by species (stable), then bisect each species level by level:
+    // the global levels by chunks, the last (at most KDL_LEVELS) in LDS
+    size_t sb = L.sort_bytes;
+    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
+                       L.keys_a, L.vals_a);
+    MOSAIC_HIP_CHECK(hipGetLastError());
+    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
+                                               L.vals_b, n32, 0, 4, st));
     // the coordinates in the order of the species sort (its order array is the
     // first set's index array), then the levels move them between two sets
     KdgSet cur = {L.kdg_xyz[0][0], L.kdg_xyz[0][1], L.kdg_xyz[0][2], L.vals_b};
