@@ -172,8 +172,7 @@ struct Layout {
     uint8_t* kdg_q8;        // bin of every position
     float* kdg_xyz[2][3];   // coordinates moved with the order through the global levels
     uint32_t* kdg_bh;       // [chunk][KDG_BINS] histograms of one level
-    uint2* kdg_cnt;         // per chunk of one level: its counts below / in the cut bin
-    uint2* kdg_boff;        // per chunk of one level: the same counts before it in its node
+    uint2* kdg_boff;        // per chunk of one level: its node's counts below / in the cut bin before it
     uint32_t* kdg_part;     // [node][slice][KDG_BINS] slice sums of the node histograms
     int2* kdg_sel;          // per node of one level: cut bin, how many of it go left
     KdNode* kd_nodes;       // nodes handed to the in-LDS finish
@@ -364,7 +363,6 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
         for (int b = 0; b < 2; ++b)
             for (int d = 0; d < 3; ++d) L.kdg_xyz[b][d] = c.take<float>(n);
         L.kdg_bh = c.take<uint32_t>(z.max_chunks * KDG_BINS);
-        L.kdg_cnt = c.take<uint2>(z.max_chunks);
         L.kdg_boff = c.take<uint2>(z.max_chunks);
         L.kdg_part = c.take<uint32_t>(z.max_part * KDG_BINS);
         L.kdg_sel = c.take<int2>(z.max_nodes);
@@ -800,7 +798,8 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
 }
 
 // Global level: each position's bin (KDG_BINS over its node's longest side)
-// and the chunk's histogram of them.
+// and the chunk's cumulative histogram of them (entry b: positions in bins
+// 0..b).
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_bin(
     const KdgSet in, const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
     const float* __restrict__ fbox, uint8_t* __restrict__ q8, uint32_t* __restrict__ bh) {
@@ -827,11 +826,22 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_bin(
         }
     }
     __syncthreads();
-    bh[static_cast<int64_t>(blockIdx.x) * KDG_BINS + tid] = h[tid];
+    // inclusive scan over the bins (one bin per thread)
+    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const int lane = tid & 63, wave = tid >> 6;
+    uint32_t inc = h[tid];
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) inc += wsum[w];
+    bh[static_cast<int64_t>(blockIdx.x) * KDG_BINS + tid] = inc;
 }
 
-// Global level: the node histograms, summed over their chunks in slices
-// (slice s of a node takes its chunks s, s + S, ...).
+// Global level: the nodes' cumulative histograms, summed over their chunks
+// in slices (slice s of a node takes its chunks s, s + S, ...).
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_total(
     const KdgNode* __restrict__ nodes, const uint32_t* __restrict__ bh, int S,
     uint32_t* __restrict__ part) {
@@ -845,13 +855,18 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_total(
 }
 
 // Global level, one workgroup per node: the bin holding the cut (from the
-// S slice sums).  With fbox_next the children's boxes follow from the
-// parent's: cut at the cut bin's edges along the split side (a node that does
-// not split hands its box to both).
+// S slice sums of the cumulative histograms), then each of the node's chunks'
+// counts below the cut bin and in it (two entries of its cumulative
+// histogram) and their exclusive prefix over the node's chunks.  With
+// fbox_next the children's boxes follow from the parent's: cut at the cut
+// bin's edges along the split side (a node that does not split hands its box
+// to both).
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_select(
     const KdgNode* __restrict__ nodes, const uint32_t* __restrict__ part, int S,
-    const float* __restrict__ fbox, float* __restrict__ fbox_next, int2* __restrict__ sel) {
-    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const uint32_t* __restrict__ bh, const float* __restrict__ fbox, float* __restrict__ fbox_next,
+    int2* __restrict__ sel, uint2* __restrict__ boff) {
+    __shared__ uint32_t cum[KDG_BINS];
+    __shared__ unsigned long long wsum[KDG_THREADS / 64];
     __shared__ int s_bin, s_tie;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t h = blockIdx.x;
@@ -864,16 +879,10 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_select(
     }
     uint32_t tot = 0;
     for (int s = 0; s < S; ++s) tot += part[(h * S + s) * KDG_BINS + tid];
-    uint32_t inc = tot;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
-        if (lane >= o) inc += y;
-    }
-    if (lane == 63) wsum[wave] = inc;
+    cum[tid] = tot;
     __syncthreads();
-    for (int w = 0; w < wave; ++w) inc += wsum[w];
-    const uint32_t exc = inc - tot;
-    if (exc < N.need && inc >= N.need) {             // exactly one bin
+    const uint32_t exc = tid > 0 ? cum[tid - 1] : 0u;
+    if (exc < N.need && tot >= N.need) {             // exactly one bin
         s_bin = tid;
         s_tie = static_cast<int>(N.need - exc);
     }
@@ -892,46 +901,15 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_select(
         }
         for (int d = 0; d < 6; ++d) fbox_next[12 * h + 6 * tid + d] = c6[d];
     }
-}
-
-// Global level, one workgroup per chunk: its counts below the cut bin and
-// in it.
-__global__ __launch_bounds__(KDG_THREADS) void k_kdg_chunk_counts(
-    const KdgChunk* __restrict__ chunks, const KdgNode* __restrict__ nodes,
-    const uint32_t* __restrict__ bh, const int2* __restrict__ sel, uint2* __restrict__ cnt) {
-    __shared__ uint32_t wsum[KDG_THREADS / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const KdgChunk C = chunks[blockIdx.x];
-    if (nodes[C.node].count <= static_cast<uint32_t>(TILE)) return;
-    const int bs = sel[C.node].x;
-    const uint32_t v = bh[static_cast<int64_t>(blockIdx.x) * KDG_BINS + tid];
-    uint32_t below = tid < bs ? v : 0u;
-    for (int o = 32; o > 0; o >>= 1)
-        below += static_cast<uint32_t>(__shfl_xor(static_cast<int>(below), o, 64));
-    if (lane == 0) wsum[wave] = below;
-    __syncthreads();
-    if (tid == bs) {
-        uint32_t b = 0;
-        for (int w = 0; w < KDG_THREADS / 64; ++w) b += wsum[w];
-        cnt[blockIdx.x] = make_uint2(b, v);
-    }
-}
-
-// Global level, one workgroup per node: exclusive prefix of the chunk counts
-// over the node's chunks.
-__global__ __launch_bounds__(KDG_THREADS) void k_kdg_chunk_scan(
-    const KdgNode* __restrict__ nodes, const uint2* __restrict__ cnt, uint2* __restrict__ boff) {
-    __shared__ unsigned long long wsum[KDG_THREADS / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const KdgNode N = nodes[blockIdx.x];
-    if (N.count <= static_cast<uint32_t>(TILE)) return;
+    // the chunks' (below, in) counts, packed high / low, scanned over the node
     unsigned long long carry = 0ull;
     for (uint32_t base = 0; base < N.nblk; base += KDG_THREADS) {
         const uint32_t b = base + tid;
         unsigned long long v = 0ull;
         if (b < N.nblk) {
-            const uint2 c = cnt[N.blk0 + b];
-            v = (static_cast<unsigned long long>(c.x) << 32) | c.y;
+            const uint32_t* c = bh + static_cast<int64_t>(N.blk0 + b) * KDG_BINS;
+            const uint32_t below = bs > 0 ? c[bs - 1] : 0u;
+            v = (static_cast<unsigned long long>(below) << 32) | (c[bs] - below);
         }
         unsigned long long inc = v;
         for (int o = 1; o < 64; o <<= 1) {
@@ -5109,11 +5087,8 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
             hipLaunchKernelGGL(k_kdg_total, dim3(S, static_cast<unsigned int>(nodes)),
                                dim3(KDG_THREADS), 0, st, nd, L.kdg_bh, S, L.kdg_part);
             hipLaunchKernelGGL(k_kdg_select, dim3(static_cast<unsigned int>(nodes)),
-                               dim3(KDG_THREADS), 0, st, nd, L.kdg_part, S, fbox, fnext, L.kdg_sel);
-            hipLaunchKernelGGL(k_kdg_chunk_counts, dim3(nchunks), dim3(KDG_THREADS), 0, st, ch, nd,
-                               L.kdg_bh, L.kdg_sel, L.kdg_cnt);
-            hipLaunchKernelGGL(k_kdg_chunk_scan, dim3(static_cast<unsigned int>(nodes)),
-                               dim3(KDG_THREADS), 0, st, nd, L.kdg_cnt, L.kdg_boff);
+                               dim3(KDG_THREADS), 0, st, nd, L.kdg_part, S, L.kdg_bh, fbox, fnext,
+                               L.kdg_sel, L.kdg_boff);
             hipLaunchKernelGGL(k_kdg_scatter, dim3(nchunks), dim3(KDG_THREADS), 0, st, ch, nd,
                                L.kdg_q8, cur, L.kdg_sel, L.kdg_boff, other);
             MOSAIC_HIP_CHECK(hipGetLastError());
