@@ -827,37 +827,38 @@ def main():
                          b._beam_vc)
     out = torch.zeros((1, Nz, Ny), dtype=torch.complex128, device=dev)
     ws = [None]
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    sum_ms = []
-    kern_ms = []
-    K.sum_timing(True)
+    # HIP events around each timed step's sum, read after the timed loop (no
+    # host wait inside it beyond the staging call's own)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(max(1, args.steps))]
 
     path_used = [None]
 
-    def step(timed):
+    def step(i=None):
         out.zero_()
         staged = K.stage(d_pos, d_cod, amp, counts, k_val, 1, Ny, Nz, workspace=ws[0])
         ws[0] = staged.workspace
         K.kin_pixels(staged, spc, opts, coords)
-        ev0.record()
+        if i is not None:
+            evs[i][0].record()
         K.kin_sum(staged, spc, opts, rot, tr, coords, out)
-        ev1.record()
+        if i is not None:
+            evs[i][1].record()
         path_used[0] = staged.path_used
-        if timed:
-            ev1.synchronize()
-            sum_ms.append(ev0.elapsed_time(ev1))
-            kern_ms.append(K.sum_timing(True, fallbacks=True))
 
     for _ in range(args.warmup):
-        step(False)
+        step()
     torch.cuda.synchronize()
+    K.sum_timing(True)          # a fresh record of the timed steps' kernel launches
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(True)
-        print(f"[bench] step {i + 1}/{args.steps} k_sum {sum_ms[-1]:.2f} ms", file=sys.stderr,
-              flush=True)
+        step(i)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    sum_ms = [a.elapsed_time(b) for a, b in evs[:args.steps]]
+    kern_ms = K.sum_timing(False, fallbacks=True)   # means over the timed steps
+    for i, v in enumerate(sum_ms):
+        print(f"[bench] step {i + 1}/{args.steps} k_sum {v:.2f} ms", file=sys.stderr, flush=True)
     ms_step = (t1 - t0) * 1e3 / args.steps
     ksum_ms = float(np.mean(sum_ms))
     field = out[0].cpu().numpy()
@@ -877,7 +878,7 @@ def main():
         "config": {"workload": args.workload, "atoms": int(n_atoms), "pixels": [Ny, Nz],
                    "pairs_per_step": pairs, "species": labels, "energy_eV": b.energy},
         "roofline": roofline(path_used[0], pairs, ksum_ms, args.workload,
-                             tuple(np.mean(np.asarray(kern_ms), axis=0))),
+                             tuple(kern_ms)),
     }
     if not args.no_end_to_end:
         res["end_to_end"] = bench_end_to_end(w, args.steps)

@@ -202,14 +202,16 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
 
 /*
  * Launch times of the kernels of mosaic_kin_sum calls on the calling
- * thread's current device, for measurement.  With enable = 1 the
- * matrix-core moment sum (path 7) records HIP events on its stream around
- * its moments and its evaluation kernels and counts the (unit, patch)
- * evaluations that its separable phase model declined (evaluated pixel by
- * pixel); out (host [3], may be NULL) receives the last such call's times
- * in milliseconds (moments, evaluation) and that count, after waiting for
- * its events.  No counterpart in the reference, which times its kernels
- * from Python with CuPy events (hardware.py:1944-1980).
+ * thread's current device, for measurement.  Switching timing on (enable = 1
+ * after 0) starts a fresh record: the matrix-core moment sum (path 7) then
+ * records HIP events on its stream around its moments and its evaluation
+ * kernels for up to 16 calls, without waiting for them, and counts the
+ * (unit, patch) evaluations that its separable phase model declined
+ * (evaluated pixel by pixel).  out (host [3], may be NULL) receives the
+ * recorded calls' mean times in milliseconds (moments, evaluation) and the
+ * count over them, after waiting for their events; then timing stays on or
+ * goes off by enable.  No counterpart in the reference, which times its
+ * kernels from Python with CuPy events (hardware.py:1944-1980).
  */
 int mosaic_kin_timing(int enable, float* out);
 

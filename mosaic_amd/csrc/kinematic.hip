@@ -33,7 +33,6 @@
 // described at their kernels.
 
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <type_traits>
@@ -159,12 +158,10 @@ struct Layout {
     AtomRec* atoms;
     float* grp_r;           // [tile][2] radius of the pair / quad headed by the tile, else +inf
     uint32_t* tile_pos;     // [tile] index in its k-d node | node's tile count << 8
-    uint32_t* keys_a;       // species sort
-    uint32_t* keys_b;
+    uint32_t* spc_cnt;      // species pass: [block][MAX_SPECIES] counts
+    uint32_t* spc_off;      //   and each block's first slot per species
     uint32_t* vals_a;       // orders (atom index per position)
     uint32_t* vals_b;
-    void* sort_tmp;
-    size_t sort_bytes;
     unsigned int* kd_box;   // [node][6] tight bounding boxes of a level (order-preserving integers)
     float* kd_fbox[2];      // [node][6] float boxes (lo, hi) of two successive levels
     KdgNode* kdg_nodes;     // nodes of every global level
@@ -339,17 +336,13 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.grp_r = c.take<float>(2 * static_cast<size_t>(std::max(1, L.n_tiles)));
     L.tile_pos = c.take<uint32_t>(static_cast<size_t>(std::max(1, L.n_tiles)));
     size_t n = static_cast<size_t>(std::max<int64_t>(1, n_atoms));
-    L.keys_a = c.take<uint32_t>(n);
-    L.keys_b = c.take<uint32_t>(n);
     L.vals_a = c.take<uint32_t>(n);
     L.vals_b = c.take<uint32_t>(n);
-    size_t sort_bytes = 0;
-    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  static_cast<unsigned int>(n), 0, 4) != hipSuccess)
-        return MOSAIC_ERR_HIP;
-    L.sort_bytes = sort_bytes;
-    L.sort_tmp = c.take<char>(sort_bytes);
+    {
+        const size_t nblk = (n + KDG_CHUNK - 1) / KDG_CHUNK;
+        L.spc_cnt = c.take<uint32_t>(nblk * MAX_SPECIES);
+        L.spc_off = c.take<uint32_t>(nblk * MAX_SPECIES);
+    }
     L.kd_levels = kd_levels(n_species, counts);
     const size_t top = static_cast<size_t>(n_species) << std::max(0, L.kd_levels - 1);
     L.kd_box = c.take<unsigned int>(6 * top);
@@ -704,15 +697,6 @@ __global__ void k_stage_init(unsigned int* __restrict__ radius_bits) {
     if (threadIdx.x < 2) radius_bits[threadIdx.x] = 0u;
 }
 
-// Group by species (stable): key = species code.
-__global__ void k_species_keys(const uint8_t* __restrict__ spc, int64_t n,
-                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = spc[i] & 0xF;
-    vals[i] = static_cast<uint32_t>(i);
-}
-
 constexpr int KD_THREADS = 256;
 
 __global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
@@ -740,26 +724,14 @@ __global__ void k_kd_box_float(const unsigned int* __restrict__ box, int64_t wor
 }
 
 // The global levels move each position's coordinates and atom index
-// together, in four arrays (k_kdg_gather fills the first set from the order
-// of the species sort), so that every level reads and writes them
-// contiguously.
+// together, in four arrays (k_species_scatter fills the first set), so that
+// every level reads and writes them contiguously.
 struct KdgSet {
     float* x;
     float* y;
     float* z;
     uint32_t* idx;
 };
-
-__global__ void k_kdg_gather(const float* __restrict__ pos, const uint32_t* __restrict__ order,
-                             int64_t n, KdgSet out) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t a = order[i];
-    out.x[i] = pos[3 * static_cast<int64_t>(a)];
-    out.y[i] = pos[3 * static_cast<int64_t>(a) + 1];
-    out.z[i] = pos[3 * static_cast<int64_t>(a) + 2];
-    out.idx[i] = a;
-}
 
 // Global level: tight boxes of the nodes, one chunk per workgroup (6
 // atomics per chunk on order-preserving images of the floats).
@@ -1008,6 +980,110 @@ struct SpeciesSplit {
     int64_t atom_start[MAX_SPECIES + 1];
     int tile_start[MAX_SPECIES + 1];
 };
+
+// Grouping by species (stable) as a counting sort straight into the global
+// levels' first set: per block of KDG_CHUNK atoms its species counts
+// (k_species_count), their exclusive prefix per species over the blocks from
+// the species' first position (k_species_prefix), and the scatter of each
+// atom's coordinates and index (k_species_scatter; wave w takes the block's
+// atoms 512 w + 64 r + lane, ranks within a step from ballots).
+__global__ __launch_bounds__(KDG_THREADS) void k_species_count(const uint8_t* __restrict__ spc,
+                                                               int64_t n, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t h[MAX_SPECIES];
+    const int tid = threadIdx.x;
+    if (tid < MAX_SPECIES) h[tid] = 0u;
+    __syncthreads();
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * KDG_CHUNK;
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        const int64_t i = b0 + tid + r * KDG_THREADS;
+        if (i < n) atomicAdd(&h[spc[i] & 0xF], 1u);
+    }
+    __syncthreads();
+    if (tid < MAX_SPECIES) cnt[static_cast<int64_t>(blockIdx.x) * MAX_SPECIES + tid] = h[tid];
+}
+
+// one workgroup per species
+__global__ __launch_bounds__(KDG_THREADS) void k_species_prefix(const uint32_t* __restrict__ cnt,
+                                                                int nblocks, SpeciesSplit split,
+                                                                uint32_t* __restrict__ off) {
+    __shared__ uint32_t wsum[KDG_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sp = blockIdx.x;
+    uint32_t carry = static_cast<uint32_t>(split.atom_start[sp]);
+    for (int base = 0; base < nblocks; base += KDG_THREADS) {
+        const int b = base + tid;
+        const uint32_t v = b < nblocks ? cnt[static_cast<int64_t>(b) * MAX_SPECIES + sp] : 0u;
+        uint32_t inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (b < nblocks) off[static_cast<int64_t>(b) * MAX_SPECIES + sp] = before + inc - v;
+        for (int w = 0; w < KDG_THREADS / 64; ++w) carry += wsum[w];
+        __syncthreads();                              // wsum is rewritten next round
+    }
+}
+
+__global__ __launch_bounds__(KDG_THREADS) void k_species_scatter(
+    const float* __restrict__ pos, const uint8_t* __restrict__ spc, int64_t n, int n_species,
+    const uint32_t* __restrict__ off, KdgSet out) {
+    static_assert(KDG_CHUNK == 4 * 64 * KDG_PT && KDG_THREADS == 256, "four waves of 8 steps");
+    __shared__ uint32_t wcnt[4][MAX_SPECIES];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * KDG_CHUNK + 512 * wave + lane;
+    auto below_mask = [](uint64_t m) {             // set lanes below this one
+        return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                         __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    };
+    int sp[KDG_PT];
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        const int64_t e = e0 + 64 * r;
+        sp[r] = e < n ? static_cast<int>(spc[e] & 0xF) : MAX_SPECIES;
+    }
+    for (int s = 0; s < n_species; ++s) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < KDG_PT; ++r) c += __popcll(__ballot(sp[r] == s));
+        if (lane == 0) wcnt[wave][s] = c;
+    }
+    __syncthreads();
+    // this wave's first slot per species: the block's offset and the waves before it
+    uint32_t base[MAX_SPECIES];
+#pragma unroll
+    for (int s = 0; s < MAX_SPECIES; ++s) {
+        base[s] = 0u;
+        if (s < n_species) {
+            base[s] = off[static_cast<int64_t>(blockIdx.x) * MAX_SPECIES + s];
+            for (int w = 0; w < wave; ++w) base[s] += wcnt[w][s];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < KDG_PT; ++r) {
+        const int64_t e = e0 + 64 * r;
+        uint32_t dst = 0u;
+#pragma unroll
+        for (int s = 0; s < MAX_SPECIES; ++s) {
+            if (s >= n_species) break;
+            const uint64_t m = __ballot(sp[r] == s);
+            if (sp[r] == s) dst = base[s] + below_mask(m);
+            base[s] += __popcll(m);
+        }
+        // (codes that disagree with the host's species counts must not
+        // write outside the arrays)
+        if (e < n && dst < static_cast<uint64_t>(n)) {
+            out.x[dst] = pos[3 * e];
+            out.y[dst] = pos[3 * e + 1];
+            out.z[dst] = pos[3 * e + 2];
+            out.idx[dst] = static_cast<uint32_t>(e);
+        }
+    }
+}
 
 // Origin of a unit of 2 or 4 consecutive tiles: the mean of their origins,
 // taken pairwise (the moment sum and the staging both use this form).
@@ -2944,7 +3020,7 @@ struct GmArgs {
     int ngy;                  // regions of GM_REG x GM_REG patches along a row
     int n_groups;             // regions (one workgroup each)
     int accumulate;           // k_gm_eval adds to the partials (rounds after the first)
-    unsigned int* fallbacks;  // units evaluated pixel by pixel (counted when not null)
+    unsigned int* fallbacks;  // (unit, patch) evaluations taken pixel by pixel (counted when not null)
     int pixel_eval;           // evaluate every unit pixel by pixel
     double K;
 };
@@ -2960,9 +3036,8 @@ __device__ __forceinline__ void gm_wave_sync() {
 
 // Unit size of every head tile (0 for the others), from unit_of.
 __global__ void k_gm_heads(const float* __restrict__ grp_r, float rmax, int n_tiles,
-                           uint8_t* __restrict__ heads, unsigned int* __restrict__ fallbacks) {
+                           uint8_t* __restrict__ heads) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0 && fallbacks) *fallbacks = 0u;   // the evaluation's counter, when counting
     if (t >= n_tiles) return;
     int head, size;
     unit_of(t, n_tiles, grp_r, rmax, head, size);
@@ -4983,6 +5058,47 @@ extern "C" int mosaic_kin_workspace(int64_t atom_count, int n_species, const int
     return MOSAIC_OK;
 }
 
+// Page-locked host staging for the staging call's tables (the k-d plan's
+// nodes and chunks, the in-LDS nodes) and its read-back, one per host thread:
+// copies from pageable memory stall the stream.  An event after the last
+// copy that reads the buffer guards its reuse.
+namespace {
+struct PinnedStage {
+    char* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+thread_local PinnedStage g_pinned;
+
+// at least `bytes` of page-locked memory whose previous copies have finished
+char* pinned_stage(size_t bytes) {
+    PinnedStage& P = g_pinned;
+    if (P.pending) {
+        if (hipEventSynchronize(P.done) != hipSuccess) return nullptr;
+        P.pending = false;
+    }
+    if (!P.done && hipEventCreateWithFlags(&P.done, hipEventDisableTiming) != hipSuccess) {
+        P.done = nullptr;
+        return nullptr;
+    }
+    if (bytes > P.cap) {
+        if (P.buf) (void)hipHostFree(P.buf);
+        P.buf = nullptr;
+        P.cap = 0;
+        const size_t want = std::max<size_t>(bytes, size_t(1) << 20);
+        if (hipHostMalloc(reinterpret_cast<void**>(&P.buf), want, hipHostMallocDefault) != hipSuccess)
+            return nullptr;
+        P.cap = want;
+    }
+    return P.buf;
+}
+
+void pinned_stage_mark(hipStream_t st) {
+    if (g_pinned.done && hipEventRecord(g_pinned.done, st) == hipSuccess) g_pinned.pending = true;
+}
+}  // namespace
+
 extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
                                 const float* amp, int n_species, const int64_t* counts, double k,
                                 int n_frames, int Ny, int Nz, void* workspace,
@@ -5043,27 +5159,40 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
         }
     }
     if (lnodes.size() > static_cast<size_t>(std::max(1, L.n_tiles))) return MOSAIC_ERR_ARGS;
-    const unsigned int n32 = static_cast<unsigned int>(atom_count);
-    const int nb = blocks_for(atom_count, KD_THREADS);
     hipLaunchKernelGGL(k_stage_init, dim3(1), dim3(64), 0, st, L.radius_bits);
     // group by species (stable), then bisect each species level by level:
-    // the global levels by chunks, the last (at most KDL_LEVELS) in LDS
-    size_t sb = L.sort_bytes;
-    hipLaunchKernelGGL(k_species_keys, dim3(nb), dim3(KD_THREADS), 0, st, species, atom_count,
-                       L.keys_a, L.vals_a);
-    MOSAIC_HIP_CHECK(hipGetLastError());
-    MOSAIC_HIP_CHECK(rocprim::radix_sort_pairs(L.sort_tmp, sb, L.keys_a, L.keys_b, L.vals_a,
-                                               L.vals_b, n32, 0, 4, st));
-    // the coordinates in the order of the species sort (its order array is the
-    // first set's index array), then the levels move them between two sets
+    // the global levels by chunks, the last (at most KDL_LEVELS) in LDS.
+    // The grouping is a counting sort straight into the first coordinate set
+    // (its index array holds the order); the levels then move the
+    // coordinates between two sets
     KdgSet cur = {L.kdg_xyz[0][0], L.kdg_xyz[0][1], L.kdg_xyz[0][2], L.vals_b};
     KdgSet other = {L.kdg_xyz[1][0], L.kdg_xyz[1][1], L.kdg_xyz[1][2], L.vals_a};
-    hipLaunchKernelGGL(k_kdg_gather, dim3(nb), dim3(KD_THREADS), 0, st, pos, L.vals_b, atom_count, cur);
+    {
+        const int nblk = static_cast<int>((atom_count + KDG_CHUNK - 1) / KDG_CHUNK);
+        hipLaunchKernelGGL(k_species_count, dim3(nblk), dim3(KDG_THREADS), 0, st, species, atom_count,
+                           L.spc_cnt);
+        hipLaunchKernelGGL(k_species_prefix, dim3(n_species), dim3(KDG_THREADS), 0, st, L.spc_cnt, nblk,
+                           split, L.spc_off);
+        hipLaunchKernelGGL(k_species_scatter, dim3(nblk), dim3(KDG_THREADS), 0, st, pos, species,
+                           atom_count, n_species, L.spc_off, cur);
+        MOSAIC_HIP_CHECK(hipGetLastError());
+    }
+    // the plan's tables through page-locked memory: nodes, chunks, in-LDS nodes
+    auto round256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t gn_b = gnodes.size() * sizeof(KdgNode), gc_b = gchunks.size() * sizeof(KdgChunk);
+    const size_t ln_b = lnodes.size() * sizeof(KdNode);
+    char* pin = pinned_stage(round256(gn_b) + round256(gc_b) + round256(ln_b) + 256);
+    if (!pin) return MOSAIC_ERR_HIP;
+    std::memcpy(pin, gnodes.data(), gn_b);
+    std::memcpy(pin + round256(gn_b), gchunks.data(), gc_b);
+    std::memcpy(pin + round256(gn_b) + round256(gc_b), lnodes.data(), ln_b);
+    unsigned int* pin_bits = reinterpret_cast<unsigned int*>(pin + round256(gn_b) + round256(gc_b) +
+                                                             round256(ln_b));
     if (glevels > 0) {
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, gnodes.data(), gnodes.size() * sizeof(KdgNode),
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, pin, gn_b, hipMemcpyHostToDevice, st));
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_chunks, pin + round256(gn_b), gc_b,
                                         hipMemcpyHostToDevice, st));
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_chunks, gchunks.data(),
-                                        gchunks.size() * sizeof(KdgChunk), hipMemcpyHostToDevice, st));
+        pinned_stage_mark(st);
         auto tight = [&](int lv) { return lv == 0 || lv >= glevels - KD_TIGHT_LEVELS; };
         for (int l = 0; l < glevels; ++l) {
             const int64_t nodes = static_cast<int64_t>(node_off[l + 1] - node_off[l]);
@@ -5097,8 +5226,9 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     }
     const double K = k * INV_2PI_HI;
     if (!lnodes.empty()) {
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_nodes, lnodes.data(), lnodes.size() * sizeof(KdNode),
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_nodes, pin + round256(gn_b) + round256(gc_b), ln_b,
                                         hipMemcpyHostToDevice, st));
+        pinned_stage_mark(st);
         const float2* a2 = reinterpret_cast<const float2*>(amp);
         const KdNode* nds = L.kd_nodes;
         // the final order goes to vals_b (mosaic_kin_restage reads it there)
@@ -5116,11 +5246,12 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
                                L.grp_r, L.tile_pos, L.radius_bits);
     }
     MOSAIC_HIP_CHECK(hipGetLastError());
-    unsigned int bits[2] = {0u, 0u};
-    MOSAIC_HIP_CHECK(hipMemcpyAsync(bits, L.radius_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(pin_bits, L.radius_bits, 2 * sizeof(unsigned int),
+                                    hipMemcpyDeviceToHost, st));
+    pinned_stage_mark(st);
     MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
     float rf[2];
-    std::memcpy(rf, bits, sizeof(rf));
+    std::memcpy(rf, pin_bits, sizeof(rf));
     stage_stats[0] = static_cast<double>(rf[0]);
     stage_stats[1] = static_cast<double>(rf[1]);
     return MOSAIC_OK;
@@ -5291,17 +5422,21 @@ static bool gm_bounds(double r, const double* pixel_geom) {
 
 // Launch timing of the matrix-core moment sum, for measurement (enabled by
 // mosaic_kin_timing): HIP events on the sum's stream around each round's
-// moments and evaluation kernels, and a count of the units its evaluation
-// took pixel by pixel, per device.
+// moments and evaluation kernels of the first TIMING_CALLS calls since timing
+// was switched on, and a count of the (unit, patch) evaluations taken pixel by
+// pixel over those calls, per device.  Nothing waits for the events until
+// they are read.
 namespace {
 constexpr int TIMING_DEVICES = 16;
+constexpr int TIMING_CALLS = 16;
 constexpr int TIMING_ROUNDS = 16;
 struct KinTiming {
     bool on = false;
     bool made = false;
-    int rounds = 0;
-    hipEvent_t ev[TIMING_ROUNDS][4];
-    unsigned int* fallbacks = nullptr;   // device counter of the last call
+    int calls = 0;                         // calls recorded since timing was switched on
+    int rounds[TIMING_CALLS];              // rounds of each (0: more than TIMING_ROUNDS, not timed)
+    hipEvent_t ev[TIMING_CALLS][TIMING_ROUNDS][4];
+    unsigned int* fallbacks = nullptr;     // device counter since timing was switched on
 };
 KinTiming g_timing[TIMING_DEVICES];
 
@@ -5311,9 +5446,20 @@ KinTiming* timing_state() {
     return &g_timing[dev];
 }
 
-// event k (0, 1: moments start, stop; 2, 3: evaluation) of round r, if timing is on
-void timing_mark(KinTiming* T, int r, int k, hipStream_t st) {
-    if (T && T->on && T->made && r < TIMING_ROUNDS) (void)hipEventRecord(T->ev[r][k], st);
+// the recording slot of a new call, or -1
+int timing_slot(KinTiming* T) {
+    return (T && T->on && T->made && T->calls < TIMING_CALLS) ? T->calls : -1;
+}
+
+// event k (0, 1: moments start, stop; 2, 3: evaluation) of round r of slot c
+void timing_mark(KinTiming* T, int c, int r, int k, hipStream_t st) {
+    if (c >= 0 && r < TIMING_ROUNDS) (void)hipEventRecord(T->ev[c][r][k], st);
+}
+
+void timing_done(KinTiming* T, int c, int rounds) {
+    if (c < 0) return;
+    T->rounds[c] = rounds <= TIMING_ROUNDS ? rounds : 0;
+    T->calls = c + 1;
 }
 }  // namespace
 
@@ -5321,26 +5467,39 @@ extern "C" int mosaic_kin_timing(int enable, float* ms_out) {   // ms_out: host 
     KinTiming* T = timing_state();
     if (!T) return MOSAIC_ERR_HIP;
     if (enable && !T->made) {
-        for (int r = 0; r < TIMING_ROUNDS; ++r)
-            for (int k = 0; k < 4; ++k) MOSAIC_HIP_CHECK(hipEventCreate(&T->ev[r][k]));
+        for (int c = 0; c < TIMING_CALLS; ++c)
+            for (int r = 0; r < TIMING_ROUNDS; ++r)
+                for (int k = 0; k < 4; ++k) MOSAIC_HIP_CHECK(hipEventCreate(&T->ev[c][r][k]));
         MOSAIC_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&T->fallbacks), sizeof(unsigned int)));
-        MOSAIC_HIP_CHECK(hipMemset(T->fallbacks, 0, sizeof(unsigned int)));
         T->made = true;
     }
-    if (enable && !T->on) T->rounds = 0;   // no call timed since timing was off
+    if (enable && !T->on) {                // a fresh record
+        T->calls = 0;
+        MOSAIC_HIP_CHECK(hipMemset(T->fallbacks, 0, sizeof(unsigned int)));
+        MOSAIC_HIP_CHECK(hipDeviceSynchronize());
+    }
     if (ms_out) {
         ms_out[0] = 0.f;
         ms_out[1] = 0.f;
         ms_out[2] = 0.f;
-        for (int r = 0; r < T->rounds && r < TIMING_ROUNDS; ++r) {
-            MOSAIC_HIP_CHECK(hipEventSynchronize(T->ev[r][3]));
-            float a = 0.f, b = 0.f;
-            MOSAIC_HIP_CHECK(hipEventElapsedTime(&a, T->ev[r][0], T->ev[r][1]));
-            MOSAIC_HIP_CHECK(hipEventElapsedTime(&b, T->ev[r][2], T->ev[r][3]));
-            ms_out[0] += a;
-            ms_out[1] += b;
+        int timed = 0;
+        for (int c = 0; c < T->calls; ++c) {
+            if (T->rounds[c] <= 0) continue;
+            for (int r = 0; r < T->rounds[c]; ++r) {
+                MOSAIC_HIP_CHECK(hipEventSynchronize(T->ev[c][r][3]));
+                float a = 0.f, b = 0.f;
+                MOSAIC_HIP_CHECK(hipEventElapsedTime(&a, T->ev[c][r][0], T->ev[c][r][1]));
+                MOSAIC_HIP_CHECK(hipEventElapsedTime(&b, T->ev[c][r][2], T->ev[c][r][3]));
+                ms_out[0] += a;
+                ms_out[1] += b;
+            }
+            ++timed;
         }
-        if (T->made && T->rounds > 0) {
+        if (timed > 0) {
+            ms_out[0] /= static_cast<float>(timed);
+            ms_out[1] /= static_cast<float>(timed);
+        }
+        if (T->made && T->calls > 0) {
             unsigned int n = 0;
             MOSAIC_HIP_CHECK(hipMemcpy(&n, T->fallbacks, sizeof(n), hipMemcpyDeviceToHost));
             ms_out[2] = static_cast<float>(n);
@@ -5590,10 +5749,11 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
         // moments kernel: about 4096 waves over the round's tiles
         const int64_t waves = static_cast<int64_t>(A.n_groups) * GM_WAVES * n_frames;
         KinTiming* T = timing_state();
-        A.fallbacks = (T && T->on && T->made) ? T->fallbacks : nullptr;
+        const int slot = timing_slot(T);
+        A.fallbacks = slot >= 0 ? T->fallbacks : nullptr;
         A.pixel_eval = opts->gm_pixel_eval ? 1 : 0;
         hipLaunchKernelGGL(k_gm_heads, dim3(blocks_for(L.n_tiles, 256)), dim3(256), 0, st,
-                           L.grp_r, grp_rmax, L.n_tiles, L.gm_head, A.fallbacks);
+                           L.grp_r, grp_rmax, L.n_tiles, L.gm_head);
         e = hipGetLastError();
         int round = 0;
         for (int tb = 0; tb < L.n_tiles && e == hipSuccess; tb += tiles_per_round, ++round) {
@@ -5605,18 +5765,18 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
             A.t_begin = tb;
             A.t_end = te;
             A.accumulate = tb > 0 ? 1 : 0;
-            timing_mark(T, round, 0, st);
+            timing_mark(T, slot, round, 0, st);
             e = use_second ? launch_gm_moments<true>(A, slices_a, st)
                            : launch_gm_moments<false>(A, slices_a, st);
-            timing_mark(T, round, 1, st);
+            timing_mark(T, slot, round, 1, st);
             if (e != hipSuccess) break;
-            timing_mark(T, round, 2, st);
+            timing_mark(T, slot, round, 2, st);
             hipLaunchKernelGGL(k_gm_eval, dim3((n_patches + POLY_WAVES - 1) / POLY_WAVES, L.n_slices, n_frames),
                                dim3(64 * POLY_WAVES), 0, st, A);
-            timing_mark(T, round, 3, st);
+            timing_mark(T, slot, round, 3, st);
             e = hipGetLastError();
         }
-        if (T && T->on && T->made) T->rounds = std::min(round, TIMING_ROUNDS);   // events recorded
+        timing_done(T, slot, round);
     } else if (path == 6) {
         ProbeArgs A;
         A.atoms = L.atoms;

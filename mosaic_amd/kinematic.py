@@ -257,12 +257,14 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
 
 
 def sum_timing(enable=True, fallbacks=False):
-    """Per-kernel launch times of the matrix-core moment sum (path "gemm"):
-    enables or disables HIP events around its moments and evaluation kernels
-    on the current device, and returns the last call's (moments, evaluation)
-    times in milliseconds (waits for their events); with ``fallbacks`` also
-    the number of (unit, patch) evaluations its separable phase model
-    declined, which it evaluated pixel by pixel."""
+    """Per-kernel launch times of the matrix-core moment sum (path "gemm") on
+    the current device.  Switching timing on starts a fresh record of HIP
+    events around its moments and evaluation kernels (up to 16 calls, nothing
+    waits for them); each call returns the recorded calls' mean (moments,
+    evaluation) times in milliseconds, waiting for their events, and with
+    ``fallbacks`` also the number of (unit, patch) evaluations its separable
+    phase model declined over those calls (evaluated pixel by pixel); then
+    timing stays on or goes off by ``enable``."""
     ms = (ctypes.c_float * 3)()
     check(_lib.lib().mosaic_kin_timing(int(bool(enable)), ms), "mosaic_kin_timing")
     if fallbacks:
