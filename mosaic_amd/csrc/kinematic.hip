@@ -350,8 +350,10 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     L.kd_fbox[1] = c.take<float>(12 * top);
     {
         const KdgSizes z = kdg_sizes(n_species, counts);
+        // the staging plan's three tables side by side (one upload)
         L.kdg_nodes = c.take<KdgNode>(std::max<size_t>(1, z.nodes));
         L.kdg_chunks = c.take<KdgChunk>(std::max<size_t>(1, z.chunks));
+        L.kd_nodes = c.take<KdNode>(std::max(1, L.n_tiles));
         L.kdg_q8 = c.take<uint8_t>(n);
         for (int b = 0; b < 2; ++b)
             for (int d = 0; d < 3; ++d) L.kdg_xyz[b][d] = c.take<float>(n);
@@ -360,7 +362,6 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
         L.kdg_part = c.take<uint32_t>(z.max_part * KDG_BINS);
         L.kdg_sel = c.take<int2>(z.max_nodes);
     }
-    L.kd_nodes = c.take<KdNode>(std::max(1, L.n_tiles));
     L.radius_bits = c.take<unsigned int>(2);
     L.amax_blocks = c.take<float>(2 * AMAX_BLOCKS);
     L.frames = c.take<FrameRec>(std::max(1, n_frames));
@@ -5177,22 +5178,24 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
                            atom_count, n_species, L.spc_off, cur);
         MOSAIC_HIP_CHECK(hipGetLastError());
     }
-    // the plan's tables through page-locked memory: nodes, chunks, in-LDS nodes
-    auto round256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    // the plan's tables (global nodes, chunks, in-LDS nodes) through
+    // page-locked memory laid out as in the workspace, one upload
     const size_t gn_b = gnodes.size() * sizeof(KdgNode), gc_b = gchunks.size() * sizeof(KdgChunk);
     const size_t ln_b = lnodes.size() * sizeof(KdNode);
-    char* pin = pinned_stage(round256(gn_b) + round256(gc_b) + round256(ln_b) + 256);
+    const size_t off_c = static_cast<size_t>(reinterpret_cast<char*>(L.kdg_chunks) -
+                                              reinterpret_cast<char*>(L.kdg_nodes));
+    const size_t off_l = static_cast<size_t>(reinterpret_cast<char*>(L.kd_nodes) -
+                                              reinterpret_cast<char*>(L.kdg_nodes));
+    const size_t span = off_l + ln_b;
+    char* pin = pinned_stage((span + 255) / 256 * 256 + 256);
     if (!pin) return MOSAIC_ERR_HIP;
     std::memcpy(pin, gnodes.data(), gn_b);
-    std::memcpy(pin + round256(gn_b), gchunks.data(), gc_b);
-    std::memcpy(pin + round256(gn_b) + round256(gc_b), lnodes.data(), ln_b);
-    unsigned int* pin_bits = reinterpret_cast<unsigned int*>(pin + round256(gn_b) + round256(gc_b) +
-                                                             round256(ln_b));
+    std::memcpy(pin + off_c, gchunks.data(), gc_b);
+    std::memcpy(pin + off_l, lnodes.data(), ln_b);
+    unsigned int* pin_bits = reinterpret_cast<unsigned int*>(pin + (span + 255) / 256 * 256);
+    MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, pin, span, hipMemcpyHostToDevice, st));
+    pinned_stage_mark(st);
     if (glevels > 0) {
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_nodes, pin, gn_b, hipMemcpyHostToDevice, st));
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kdg_chunks, pin + round256(gn_b), gc_b,
-                                        hipMemcpyHostToDevice, st));
-        pinned_stage_mark(st);
         auto tight = [&](int lv) { return lv == 0 || lv >= glevels - KD_TIGHT_LEVELS; };
         for (int l = 0; l < glevels; ++l) {
             const int64_t nodes = static_cast<int64_t>(node_off[l + 1] - node_off[l]);
@@ -5226,9 +5229,6 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     }
     const double K = k * INV_2PI_HI;
     if (!lnodes.empty()) {
-        MOSAIC_HIP_CHECK(hipMemcpyAsync(L.kd_nodes, pin + round256(gn_b) + round256(gc_b), ln_b,
-                                        hipMemcpyHostToDevice, st));
-        pinned_stage_mark(st);
         const float2* a2 = reinterpret_cast<const float2*>(amp);
         const KdNode* nds = L.kd_nodes;
         // the final order goes to vals_b (mosaic_kin_restage reads it there)
