@@ -3073,14 +3073,19 @@ struct GmHeadWalk {
 typedef __fp16 halfx2 __attribute__((ext_vector_type(2)));
 typedef unsigned int uintx4 __attribute__((ext_vector_type(4)));
 
-// Two operand values into their hi and lo words (value 0 in the low halves).
+// Two operand values into their hi and lo words (value 0 in the low halves):
+// hi rounded to nearest, and the residual x - hi by one mixed-precision FMA
+// each (the FP16 hi half of the packed word times -1 plus the FP32 value,
+// rounded to FP16 into the low or high half of lo).  The round-toward-zero
+// packed convert is not used: with it the residual of small values came out
+// wrong (values counted twice), which a basis with many small entries showed
+// at 3e-4.
 __device__ __forceinline__ void gm_split_pair(float x0, float x1, unsigned int& hi,
                                              unsigned int& lo) {
-    const halfx2 h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+    halfx2 h;
+    h[0] = static_cast<__fp16>(x0);
+    h[1] = static_cast<__fp16>(x1);
     hi = __builtin_bit_cast(unsigned int, h);
-    // the residuals x - hi, each one mixed-precision FMA (FP16 hi half of the
-    // packed word times -1 plus the FP32 value, rounded to FP16 into the low
-    // or high half of lo)
     unsigned int l;
     asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
     asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
@@ -3474,9 +3479,37 @@ void k_gm_moments(const GmArgs A) {
             bh[col * GM_BS + lane] = h;
             bl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
         };
-        // part 0..3 of the main columns (m = 0; 1; 2; 3, 4) into buffer B;
-        // part 0 also writes the chunk's atom records
+        // Two columns' values of this lane's atom into LDS as 32-bit words of
+        // two atoms: lanes 2k and 2k + 1 swap one value (DPP quad_perm
+        // [1, 0, 3, 2]), then lane 2k stores column A's word of atoms 2k, 2k + 1
+        // and lane 2k + 1 column B's (hi and lo words for split columns, else
+        // the hi word rounded to nearest).
+        auto put_pair = [&](_Float16* bh, _Float16* bl, int colA, float vA, int colB, float vB,
+                            bool split) {
+            const bool odd = (lane & 1) != 0;
+            const float x = odd ? vA : vB;
+            const float y = __int_as_float(
+                __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+            const float first = odd ? y : vA, second = odd ? vB : y;
+            const int w = (odd ? colB : colA) * (GM_BS / 2) + (lane >> 1);
+            if (split) {
+                unsigned int hi, lo;
+                gm_split_pair(first, second, hi, lo);
+                reinterpret_cast<unsigned int*>(bh)[w] = hi;
+                reinterpret_cast<unsigned int*>(bl)[w] = lo;
+            } else {
+                halfx2 h;
+                h[0] = static_cast<__fp16>(first);
+                h[1] = static_cast<__fp16>(second);
+                reinterpret_cast<unsigned int*>(bh)[w] = __builtin_bit_cast(unsigned int, h);
+            }
+        };
+        // part 0..3 of the main columns (the 15 of m + n <= 4 in order, four
+        // to a part, the last alone) into buffer B; part 0 also writes the
+        // chunk's atom records
         auto build_main = [&](auto B, const Rows& r, int part) {
+            constexpr int KM[15] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 3, 3, 4};
+            constexpr int KN[15] = {0, 1, 2, 3, 4, 0, 1, 2, 3, 0, 1, 2, 0, 1, 0};
             constexpr int b = decltype(B)::value;
             _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
             _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
@@ -3486,23 +3519,21 @@ void k_gm_moments(const GmArgs A) {
                 ba[sh * GM_CHUNK + lane] = make_float4(r.sx, r.sy, r.sz, sh ? r.ph + 0.25f : r.ph);
             }
             const int col0 = wave == 0 ? 0 : GM_COL_S + GM_N4 * (wave - 1);
+            auto val = [&](int i) { return r.f * (r.pw[KM[i]] * r.rw[KN[i]]); };
 #pragma unroll
-            for (int m = 0; m <= 4; ++m) {
-                if (m != part && !(part == 3 && m == 4)) continue;
-#pragma unroll
-                for (int n = 0; n <= 4 - m; ++n)
-                    put_split(bh, bl, col0 + gm_k4(m, n), r.f * (r.pw[m] * r.rw[n]));
+            for (int i = 4 * part; i < 4 * part + 4 && i < 15; i += 2) {
+                if (i + 1 < 15) put_pair(bh, bl, col0 + i, val(i), col0 + i + 1, val(i + 1), true);
+                else put_split(bh, bl, col0 + i, val(i));
             }
         };
         // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
-        // waves 0..3 take eight each (the last one a pad column)
+        // waves 0..3 take eight each (the last one a pad column), in pairs
         auto build_rest = [&](auto B, const Rows& r) {
             constexpr int b = decltype(B)::value;
             _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
             _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
-            auto put = [&](int col, float v) {                   // col known at compile time
-                if (col < GM_NSPLIT) put_split(bh, bl, col, v);
-                else bh[col * GM_BS + lane] = static_cast<_Float16>(v);
+            auto pair = [&](int ca, float va, int cb, float vb) {   // both split or both not
+                put_pair(bh, bl, ca, va, cb, vb, ca < GM_NSPLIT);
             };
             const float ms = r.pw[0];
             const float ss[6] = {ms * r.sx * r.sx, ms * r.sx * r.sy, ms * r.sx * r.sz,
@@ -3510,23 +3541,25 @@ void k_gm_moments(const GmArgs A) {
             const float s3[3] = {1.f, r.Pb, r.Rb};
             const float* pw = r.pw;
             const float* rw = r.rw;
+            auto sig = [&](int i) { return ss[i / 3] * s3[i % 3]; };
+            static_assert(gm_colT(0, 5) + 4 == GM_NSPLIT, "split and hi-only pairs of wave 0");
             if (wave == 0) {
-                put(gm_colT(0, 5), pw[0] * rw[5]); put(gm_colT(1, 4), pw[1] * rw[4]);
-                put(gm_colT(2, 3), pw[2] * rw[3]); put(gm_colT(3, 2), pw[3] * rw[2]);
-                put(gm_colT(4, 1), pw[4] * rw[1]); put(gm_colT(5, 0), pw[5] * rw[0]);
-                put(gm_colT(0, 6), pw[0] * rw[6]); put(gm_colT(1, 5), pw[1] * rw[5]);
+                pair(gm_colT(0, 5), pw[0] * rw[5], gm_colT(1, 4), pw[1] * rw[4]);
+                pair(gm_colT(2, 3), pw[2] * rw[3], gm_colT(3, 2), pw[3] * rw[2]);
+                pair(gm_colT(4, 1), pw[4] * rw[1], gm_colT(5, 0), pw[5] * rw[0]);
+                pair(gm_colT(0, 6), pw[0] * rw[6], gm_colT(1, 5), pw[1] * rw[5]);
             } else if (wave == 1) {
-                put(gm_colT(2, 4), pw[2] * rw[4]); put(gm_colT(3, 3), pw[3] * rw[3]);
-                put(gm_colT(4, 2), pw[4] * rw[2]); put(gm_colT(5, 1), pw[5] * rw[1]);
-                put(gm_colT(6, 0), pw[6] * rw[0]);
-#pragma unroll
-                for (int i = 0; i < 3; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                pair(gm_colT(2, 4), pw[2] * rw[4], gm_colT(3, 3), pw[3] * rw[3]);
+                pair(gm_colT(4, 2), pw[4] * rw[2], gm_colT(5, 1), pw[5] * rw[1]);
+                pair(gm_colT(6, 0), pw[6] * rw[0], GM_COL_SS + 0, sig(0));
+                pair(GM_COL_SS + 1, sig(1), GM_COL_SS + 2, sig(2));
             } else if (wave == 2) {
 #pragma unroll
-                for (int i = 3; i < 11; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                for (int i = 3; i < 11; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
             } else {
 #pragma unroll
-                for (int i = 11; i < 18; ++i) put(GM_COL_SS + i, ss[i / 3] * s3[i % 3]);
+                for (int i = 11; i < 17; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
+                pair(GM_COL_SS + 17, sig(17), GM_NCOL, 0.f);   // with the first pad column
             }
         };
         // Product step s (16 atoms) of the chunk in buffer B: the carrier (one
