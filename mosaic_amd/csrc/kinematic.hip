@@ -3321,16 +3321,13 @@ __device__ __forceinline__ void gm_patch_model(const FrameRec& F, const double4*
 template <bool SECOND>
 __global__ __launch_bounds__(64 * GM_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void k_gm_moments(const GmArgs A) {
-    // Two chunk buffers, each: the basis hi halves [column][atom], the lo
-    // halves of blocks 0 and 1, and the chunk's atom records (offset
-    // [wavelengths], phase [turns]; then the same with the Re rows' quarter
-    // turn).  At a unit's end the same bytes hold each wave's products for
-    // the fold.
-    constexpr int SB_H = GM_NB * 32 * GM_BS * 2, SB_L = GM_NSPLIT * GM_BS * 2;
-    constexpr int SB_A = 2 * GM_CHUNK * 16, SB = SB_H + SB_L + SB_A;
-    static_assert(SB % 16 == 0 && 2 * SB >= GM_WAVES * 32 * GM_CS * 4,
-                  "the products at a unit's end fit the chunk buffers");
-    __shared__ __align__(16) unsigned char smem[2 * SB];
+    // Two chunk buffers, each: the basis hi halves [column][atom position],
+    // then the lo halves of blocks 0 and 1.  At a unit's end the same bytes
+    // hold each wave's products for the fold.
+    constexpr int SB_H = GM_NB * 32 * GM_BS * 2, SB_L = GM_NSPLIT * GM_BS * 2, SB = SB_H + SB_L;
+    constexpr int SMEM = 2 * SB > GM_WAVES * 32 * GM_CS * 4 ? 2 * SB : GM_WAVES * 32 * GM_CS * 4;
+    static_assert(SB % 16 == 0, "the chunk buffers stay 16-byte aligned");
+    __shared__ __align__(16) unsigned char smem[SMEM];
     __shared__ float sev[GM_WAVES][16][15];                      // each patch's E_c coefficients
     __shared__ double sgs[GM_WAVES][8];                          // the waves' gradient sums
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -3474,10 +3471,13 @@ void k_gm_moments(const GmArgs A) {
             r.f = wave == 0 ? 1.f : (wave == 1 ? r.sx : (wave == 2 ? r.sy : r.sz));
             return r;
         };
+        // Atom a of a chunk sits at position a with bits 2 and 3 swapped: the
+        // order in which the phase products leave the atoms in each lane.
+        const int apos = (lane & ~12) | ((lane & 4) << 1) | ((lane & 8) >> 1);
         auto put_split = [&](_Float16* bh, _Float16* bl, int col, float v) {   // col < GM_NSPLIT
             const _Float16 h = static_cast<_Float16>(v);
-            bh[col * GM_BS + lane] = h;
-            bl[col * GM_BS + lane] = static_cast<_Float16>(v - static_cast<float>(h));
+            bh[col * GM_BS + apos] = h;
+            bl[col * GM_BS + apos] = static_cast<_Float16>(v - static_cast<float>(h));
         };
         // Two columns' values of this lane's atom into LDS as 32-bit words of
         // two atoms: lanes 2k and 2k + 1 swap one value (DPP quad_perm
@@ -3491,7 +3491,7 @@ void k_gm_moments(const GmArgs A) {
             const float y = __int_as_float(
                 __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
             const float first = odd ? y : vA, second = odd ? vB : y;
-            const int w = (odd ? colB : colA) * (GM_BS / 2) + (lane >> 1);
+            const int w = (odd ? colB : colA) * (GM_BS / 2) + (apos >> 1);
             if (split) {
                 unsigned int hi, lo;
                 gm_split_pair(first, second, hi, lo);
@@ -3505,19 +3505,13 @@ void k_gm_moments(const GmArgs A) {
             }
         };
         // part 0..3 of the main columns (the 15 of m + n <= 4 in order, four
-        // to a part, the last alone) into buffer B; part 0 also writes the
-        // chunk's atom records
+        // to a part, the last alone) into buffer B
         auto build_main = [&](auto B, const Rows& r, int part) {
             constexpr int KM[15] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 3, 3, 4};
             constexpr int KN[15] = {0, 1, 2, 3, 4, 0, 1, 2, 3, 0, 1, 2, 0, 1, 0};
             constexpr int b = decltype(B)::value;
             _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
             _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
-            if (part == 0) {      // waves 2, 3 write the same values as waves 0, 1 (no branch)
-                float4* const ba = reinterpret_cast<float4*>(smem + b * SB + SB_H + SB_L);
-                const int sh = wave & 1;
-                ba[sh * GM_CHUNK + lane] = make_float4(r.sx, r.sy, r.sz, sh ? r.ph + 0.25f : r.ph);
-            }
             const int col0 = wave == 0 ? 0 : GM_COL_S + GM_N4 * (wave - 1);
             auto val = [&](int i) { return r.f * (r.pw[KM[i]] * r.rw[KN[i]]); };
 #pragma unroll
@@ -3562,31 +3556,73 @@ void k_gm_moments(const GmArgs A) {
                 pair(GM_COL_SS + 17, sig(17), GM_NCOL, 0.f);   // with the first pad column
             }
         };
+        // The chunk's phases th(atom, row) = v . s + ph (+ 1/4 on the Re rows
+        // q < 16; with SECOND + hh (|s|^2 - (xc . s)^2)) as products on the
+        // matrix cores, 32 atoms x 32 rows from K = 2 (FP32, one per atom
+        // pair of operands): lane l holds A[atom l & 31][k = l >> 5] and
+        // B[k = l >> 5][row l & 31], so the operands are the atom records'
+        // (x, y) and (z, ph) pairs with lanes 32..63 holding the second of
+        // each, for atoms 0..31 and 32..63 (two v_permlane32_swap), against
+        // the lane patch's (vx, vy), (vz, 1).  The result leaves lane l the
+        // phases of row l & 31 at atoms 8 (r >> 2) + 4 (l >> 5) + (r & 3),
+        // r = 0..15: steps 0, 1 (2, 3) take r = 0..7, 8..15 of the first
+        // (second) 32 atoms, the atoms at their basis positions.
+        struct PhOps {
+            float xy0, zp0, s0, xy1, zp1, s1;
+        };
+        auto ph_ops = [&](const Rows& r) {
+            PhOps o;
+            const auto xy = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sx), __float_as_uint(r.sy),
+                                                             false, false);
+            const auto zp = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sz), __float_as_uint(r.ph),
+                                                             false, false);
+            o.xy0 = __uint_as_float(xy[0]);
+            o.xy1 = __uint_as_float(xy[1]);
+            o.zp0 = __uint_as_float(zp[0]);
+            o.zp1 = __uint_as_float(zp[1]);
+            o.s0 = o.s1 = 0.f;
+            if (SECOND) {
+                const float s2 = fmaf(r.sz, r.sz, fmaf(r.sy, r.sy, r.sx * r.sx));
+                const auto ss = __builtin_amdgcn_permlane32_swap(__float_as_uint(s2), 0u, false, false);
+                o.s0 = __uint_as_float(ss[0]);
+                o.s1 = __uint_as_float(ss[1]);
+            }
+            return o;
+        };
+        const float pb0 = g == 0 ? U.vx : U.vy, pb1 = g == 0 ? U.vz : 1.f;
+        const float pq = q < 16 ? 0.25f : 0.f;
+        const float pw0 = g == 0 ? xc0 : xc1, pw1 = g == 0 ? xc2 : 0.f, ps = g == 0 ? hh : 0.f;
+        auto phases = [&](float xy, float zp, float s2) {
+            floatx16 c;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) c[r] = pq;
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pb0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pb1, c, 0, 0, 0);
+            if (SECOND) {
+                c = __builtin_amdgcn_mfma_f32_32x32x2f32(s2, ps, c, 0, 0, 0);
+                floatx16 w;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) w[r] = 0.f;
+                w = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pw0, w, 0, 0, 0);
+                w = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pw1, w, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) c[r] = fmaf(-hh * w[r], w[r], c[r]);
+            }
+            return c;
+        };
         // Product step s (16 atoms) of the chunk in buffer B: the carrier (one
-        // sine per operand value) against the three blocks.
-        auto product_step = [&](auto B, int s) {
+        // sine per operand value, from the phases th) against the three blocks.
+        auto product_step = [&](auto B, int s, const floatx16& th) {
             constexpr int b = decltype(B)::value;
             const _Float16* const bh = reinterpret_cast<const _Float16*>(smem + b * SB);
             const _Float16* const bl = reinterpret_cast<const _Float16*>(smem + b * SB + SB_H);
-            const float4* const sa =
-                reinterpret_cast<const float4*>(smem + b * SB + SB_H + SB_L) + (q < 16 ? GM_CHUNK : 0);
+            const int r0 = 8 * (s & 1);
             uintx4 hw, lw;
 #pragma unroll
             for (int u2 = 0; u2 < 4; ++u2) {
-                float v[2];
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const float4 ar = sa[16 * s + 8 * g + 2 * u2 + e];
-                    float th = fmaf(U.vz, ar.z, fmaf(U.vy, ar.y, fmaf(U.vx, ar.x, ar.w)));
-                    if (SECOND) {
-                        const float w = fmaf(xc2, ar.z, fmaf(xc1, ar.y, xc0 * ar.x));
-                        const float s2 = fmaf(ar.z, ar.z, fmaf(ar.y, ar.y, ar.x * ar.x));
-                        th = fmaf(hh, fmaf(-w, w, s2), th);
-                    }
-                    v[e] = __builtin_amdgcn_sinf(th);
-                }
                 unsigned int hv, lv;
-                gm_split_pair(v[0], v[1], hv, lv);
+                gm_split_pair(__builtin_amdgcn_sinf(th[r0 + 2 * u2]), __builtin_amdgcn_sinf(th[r0 + 2 * u2 + 1]),
+                              hv, lv);
                 hw[u2] = hv;
                 lw[u2] = lv;
             }
@@ -3621,11 +3657,15 @@ void k_gm_moments(const GmArgs A) {
         ChunkAt cn = first;
         bool more = advance(cn);                         // cn: the chunk after the first
         AtomRec nxt = rec(more ? cn : first);
+        PhOps pa;
+        floatx16 thA;                                  // the current chunk's first 32 atoms' phases
         {
             const Rows r0 = rows_of(rec(first), first);
 #pragma unroll
             for (int part = 0; part < 4; ++part) build_main(B0{}, r0, part);
             build_rest(B0{}, r0);
+            pa = ph_ops(r0);
+            thA = phases(pa.xy0, pa.zp0, pa.s0);
         }
         __syncthreads();
         // one chunk: products from buffer B, the next chunk's rows into the other
@@ -3635,12 +3675,16 @@ void k_gm_moments(const GmArgs A) {
             const bool more2 = more && advance(c2);
             const AtomRec nxt2 = rec(more2 ? c2 : first);
             const Rows r = rows_of(nxt, more ? cn : first);
+            floatx16 thB;                          // the second 32 atoms' phases
 #pragma unroll
             for (int s = 0; s < GM_CHUNK / 16; ++s) {
-                product_step(B, s);
+                if (s == 1) thB = phases(pa.xy1, pa.zp1, pa.s1);
+                product_step(B, s, s < 2 ? thA : thB);
                 build_main(NB{}, r, s);
             }
             build_rest(NB{}, r);
+            pa = ph_ops(r);
+            thA = phases(pa.xy0, pa.zp0, pa.s0);
             __syncthreads();                       // buffer B is free, the other complete
             const bool had = more;
             cn = c2;

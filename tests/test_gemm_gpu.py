@@ -145,6 +145,21 @@ def test_gemm_region_gradients_at_detector_corner(device):
     assert e <= TOL
 
 
+def test_gemm_second_order_close_detector(device):
+    """A 10 cm detector of 256^2 pixels of 10 um: the tiles' own second-order
+    phase is above the first-order bound, so the matrix-core moment sum runs
+    with it (path 176, the phase products with the |s|^2 and (xc . s)^2
+    terms); the field against the long-double oracle."""
+    w = W.fcc_au_bragg111(n=8, npix=256, seed=2)
+    d = w["detector"]
+    d.create_detector((256, 256), (1.0e5, 1.0e5))
+    d.position_detector_absolute(1.0e9, np.rad2deg(2 * w["theta_b"]), 0.0)
+    A = gpu_field(w)
+    assert w["beam"]._last_path == 176
+    e = report("10 cm detector, second order", A, oracle_field(w, "f80"))
+    assert e <= TOL
+
+
 def test_gemm_matches_valu_moment_sum(device):
     """A 200,000-atom FCC crystal on a 256^2 detector (one region of 8 x 8
     patches, units of up to four tiles): the automatic choice takes the
