@@ -162,7 +162,7 @@ struct Layout {
     uint32_t* spc_off;      //   and each block's first slot per species
     uint32_t* vals_a;       // orders (atom index per position)
     uint32_t* vals_b;
-    unsigned int* kd_box;   // [node][6] tight bounding boxes of a level (order-preserving integers)
+    unsigned int* kdg_cbox; // [chunk][6] a level's tight boxes per chunk (order-preserving integers)
     float* kd_fbox[2];      // [node][6] float boxes (lo, hi) of two successive levels
     KdgNode* kdg_nodes;     // nodes of every global level
     KdgChunk* kdg_chunks;   // chunks of every global level
@@ -345,7 +345,6 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
     }
     L.kd_levels = kd_levels(n_species, counts);
     const size_t top = static_cast<size_t>(n_species) << std::max(0, L.kd_levels - 1);
-    L.kd_box = c.take<unsigned int>(6 * top);
     L.kd_fbox[0] = c.take<float>(12 * top);
     L.kd_fbox[1] = c.take<float>(12 * top);
     {
@@ -359,6 +358,7 @@ int make_layout(void* base, int64_t n_atoms, int n_species, const int64_t* count
             for (int d = 0; d < 3; ++d) L.kdg_xyz[b][d] = c.take<float>(n);
         L.kdg_bh = c.take<uint32_t>(z.max_chunks * KDG_BINS);
         L.kdg_boff = c.take<uint2>(z.max_chunks);
+        L.kdg_cbox = c.take<unsigned int>(6 * z.max_chunks);
         L.kdg_part = c.take<uint32_t>(z.max_part * KDG_BINS);
         L.kdg_sel = c.take<int2>(z.max_nodes);
     }
@@ -700,11 +700,6 @@ __global__ void k_stage_init(unsigned int* __restrict__ radius_bits) {
 
 constexpr int KD_THREADS = 256;
 
-__global__ void k_kd_box_init(unsigned int* __restrict__ box, int64_t words) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < words) box[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
-}
-
 // Longest side of a float box (lo[3], hi[3]); ties go to the lower axis.
 __device__ __forceinline__ int kd_axis(const float* b, float (&lo)[3], float (&ext)[3]) {
     for (int d = 0; d < 3; ++d) {
@@ -717,13 +712,6 @@ __device__ __forceinline__ int kd_axis(const float* b, float (&lo)[3], float (&e
     return ax;
 }
 
-// Float boxes from the order-preserving integer boxes.
-__global__ void k_kd_box_float(const unsigned int* __restrict__ box, int64_t words,
-                               float* __restrict__ fbox) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < words) fbox[i] = ord2f(box[i]);
-}
-
 // The global levels move each position's coordinates and atom index
 // together, in four arrays (k_species_scatter fills the first set), so that
 // every level reads and writes them contiguously.
@@ -734,10 +722,13 @@ struct KdgSet {
     uint32_t* idx;
 };
 
-// Global level: tight boxes of the nodes, one chunk per workgroup (6
-// atomics per chunk on order-preserving images of the floats).
+// Global level: tight boxes, one chunk per workgroup, each chunk's own box
+// (order-preserving images of the floats; lo = ~0, hi = 0 for no positions)
+// into cbox[chunk]; k_kdg_box_node combines a node's chunks.  (One atomic
+// min / max per chunk on the node's words had every chunk of a level with
+// few nodes contend for the same six words.)
 __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
-    const KdgSet in, const KdgChunk* __restrict__ chunks, unsigned int* __restrict__ box) {
+    const KdgSet in, const KdgChunk* __restrict__ chunks, unsigned int* __restrict__ cbox) {
     __shared__ unsigned int red[KDG_THREADS / 64][6];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const KdgChunk C = chunks[blockIdx.x];
@@ -764,10 +755,32 @@ __global__ __launch_bounds__(KDG_THREADS) void k_kdg_box(
         unsigned int v = red[0][tid];
         for (int w = 1; w < KDG_THREADS / 64; ++w)
             v = tid < 3 ? min(v, red[w][tid]) : max(v, red[w][tid]);
-        unsigned int* dst = box + 6 * static_cast<int64_t>(C.node) + tid;
-        if (tid < 3) atomicMin(dst, v);
-        else atomicMax(dst, v);
+        cbox[6 * static_cast<int64_t>(blockIdx.x) + tid] = v;
     }
+}
+
+// A level's node boxes as floats (lo[3], hi[3]) from its chunks' boxes: one
+// wave per node over its chunks [blk0, blk0 + nblk).
+__global__ __launch_bounds__(64) void k_kdg_box_node(const KdgNode* __restrict__ nodes,
+                                                     const unsigned int* __restrict__ cbox,
+                                                     float* __restrict__ fbox) {
+    const KdgNode N = nodes[blockIdx.x];
+    const int lane = threadIdx.x;
+    unsigned int lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    for (uint32_t c = N.blk0 + lane; c < N.blk0 + N.nblk; c += 64) {
+        const unsigned int* b = cbox + 6 * static_cast<int64_t>(c);
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = min(lo[d], b[d]);
+            hi[d] = max(hi[d], b[3 + d]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = wave_umin(lo[d]);
+        hi[d] = wave_umax(hi[d]);
+    }
+    const unsigned int v = lane == 0 ? lo[0]
+                         : (lane == 1 ? lo[1] : (lane == 2 ? lo[2] : (lane == 3 ? hi[0] : (lane == 4 ? hi[1] : hi[2]))));
+    if (lane < 6) fbox[6 * static_cast<int64_t>(blockIdx.x) + lane] = ord2f(v);
 }
 
 // Global level: each position's bin (KDG_BINS over its node's longest side)
@@ -5283,12 +5296,10 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
             float* fnext = (l + 1 < glevels && !tight(l + 1)) ? L.kd_fbox[(l + 1) & 1] : nullptr;
             if (nchunks == 0) continue;
             if (tight(l)) {
-                hipLaunchKernelGGL(k_kd_box_init, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
-                                   st, L.kd_box, 6 * nodes);
                 hipLaunchKernelGGL(k_kdg_box, dim3(nchunks), dim3(KDG_THREADS), 0, st, cur, ch,
-                                   L.kd_box);
-                hipLaunchKernelGGL(k_kd_box_float, dim3(blocks_for(6 * nodes, 256)), dim3(256), 0,
-                                   st, L.kd_box, 6 * nodes, fbox);
+                                   L.kdg_cbox);
+                hipLaunchKernelGGL(k_kdg_box_node, dim3(static_cast<unsigned int>(nodes)), dim3(64), 0,
+                                   st, nd, L.kdg_cbox, fbox);
             }
             hipLaunchKernelGGL(k_kdg_bin, dim3(nchunks), dim3(KDG_THREADS), 0, st, cur, ch, nd,
                                fbox, L.kdg_q8, L.kdg_bh);
