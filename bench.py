@@ -48,6 +48,7 @@ PEAK_TRANS_LANE_OPS = 1.81e13
 MFMA_FLOP_PER_PAIR = 24.0
 # One v_mfma_f32_32x32x16_f16: 32 x 32 x 16 multiply-adds.
 MFMA_FLOP_PER_INST = 2.0 * 32 * 32 * 16
+MFMA_F32_FLOP_PER_INST = 2.0 * 32 * 32 * 2
 
 # Counters of one launch of the sum kernels on the 5M-atom, 512^2 workload
 # (rocprofv3 --pmc, same bench command; the profile files hold the commands):
@@ -60,9 +61,9 @@ COUNTERS = {
         add_f32=2.06103e8, mul_f32=8.25971e8, fma_f32=2.82295e9, trans_f32=9.04499e7),
     "k_gm_moments": dict(
         source="profiles/k_gm_moments_perovskite5M_512px.md",
-        fetch_kib=316290.0, write_kib=440380.0, valu=4.6777e8,
-        add_f32=1.0064e7, mul_f32=4.2031e7, fma_f32=1.6177e8, trans_f32=4.0004e7,
-        mfma=3.5e7),
+        fetch_kib=320410.0, write_kib=444450.0, valu=4.5234e8,
+        add_f32=8.7043e6, mul_f32=6.9233e7, fma_f32=4.1772e7, trans_f32=4.0004e7,
+        mfma=3.5e7, mfma_f32=5.22e6),
     "k_sum_mma": dict(
         source="profiles/k_sum_mma_perovskite5M_512px.md",
         fetch_kib=1831133.0, write_kib=2542443.8, valu=None,
@@ -470,7 +471,9 @@ def roofline(path_used, pairs, ksum_ms, workload, kernel_ms=None):
     wave-instruction, from the profile's SQ_INSTS_VALU_*_F32 counters of the
     same workload -- the counts are fixed by the workload), against the
     157.3 TFLOP/s FP32 peak; for the matrix-core kernel the FP16 matrix
-    flops it issues, against 2.5 PFLOP/s.  ``pair_equivalent_tflops`` is a
+    flops it issues, against 2.5 PFLOP/s (its FP32 phase products, 2 x 32 x
+    32 x 2 flop each, are reported apart: ``mfma_f32_flop_per_launch``).
+    ``pair_equivalent_tflops`` is a
     throughput figure, not a fraction of a roofline: 20 flop per pair (the
     per-pair algorithm) times the pairs the kernel's result stands for."""
     name = kernel_name(path_used)
@@ -493,6 +496,7 @@ def roofline(path_used, pairs, ksum_ms, workload, kernel_ms=None):
             r.update({"bound": "mfma", "achieved": flop / t / 1e12,
                       "peak": PEAK_FP16_MFMA / 1e12, "frac": flop / t / PEAK_FP16_MFMA,
                       "mfma_flop_per_launch": flop,
+                      "mfma_f32_flop_per_launch": c.get("mfma_f32", 0.0) * MFMA_F32_FLOP_PER_INST,
                       "fp32_valu_tflops": v32 / t / 1e12, "fp32_valu_frac": v32 / t / PEAK_FP32,
                       "valu_issue_frac": 64.0 * c["valu"] / t / PEAK_VALU_LANE_OPS,
                       "trans_frac": 64.0 * c["trans_f32"] / t / PEAK_TRANS_LANE_OPS})
