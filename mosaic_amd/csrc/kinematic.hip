@@ -3106,6 +3106,20 @@ __device__ __forceinline__ void gm_split_pair(float x0, float x1, unsigned int& 
     lo = l;
 }
 
+// hi + lo halves of a x0 and a x1 straight from the products (fused, one
+// rounding each): hi = f16(a x), lo = f16(a x - hi), four mixed-precision FMAs
+// for the pair against two multiplies, a convert and two FMAs.
+__device__ __forceinline__ void gm_split_prod_pair(float a, float x0, float x1, unsigned int& hi,
+                                                   unsigned int& lo) {
+    unsigned int h, l;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(h) : "v"(a), "v"(x0));
+    asm("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(h) : "v"(a), "v"(x1));
+    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(l) : "v"(a), "v"(x0), "v"(h));
+    asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(l) : "v"(a), "v"(x1), "v"(h));
+    hi = h;
+    lo = l;
+}
+
 // (re, im) times i^k
 __device__ __forceinline__ void gm_rot(int k, float& re, float& im) {
     const float r = re, i = im;
@@ -3767,7 +3781,13 @@ __global__ __launch_bounds__(64 * POLY_WAVES) __attribute__((amdgpu_waves_per_eu
 void k_gm_eval(const GmArgs A) {
     constexpr int D = 6;
     constexpr float TWO_PI_F = 6.283185307179586f;
-    __shared__ __align__(16) float srec[POLY_WAVES][2 * GM_REC];
+    // a record's copy in LDS: power m's moments M_mn from GM_PB[m], each run
+    // padded with zeros to the Horner length of the power's k slot (so the
+    // shorter half of a slot reads zeros past its own terms), complex entry
+    // 31 zero for the pad slot; the record's extra float2 from float GM_EVX
+    constexpr int GM_PB[8] = {0, 7, 14, 19, 24, 27, 30, 31};
+    constexpr int GM_EVX = 64;
+    __shared__ __align__(16) float srec[POLY_WAVES][GM_EVX + 2 * (GM_REC - GM_MOM)];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int patch = blockIdx.x * POLY_WAVES + wave;
@@ -3786,6 +3806,15 @@ void k_gm_eval(const GmArgs A) {
     auto arow = [&](int r) { return 8 * (r >> 2) + 4 * g + (r & 3); };
     auto pix = [&](int r) { return prow + min(arow(r), P.vy - 1); };
     float* const rw = srec[wave];
+    int dpos;                                          // this lane's float of the record, in rw
+    {
+        const int k = lane >> 1;
+        int m = 0;
+        while (m < 6 && gm_kT(m + 1, 0) <= k) ++m;
+        dpos = k < GM_MOM ? 2 * (GM_PB[m] + k - gm_kT(m, 0)) + (lane & 1) : GM_EVX + lane - 2 * GM_MOM;
+        const int zc = lane < 2 ? 13 : (lane < 4 ? 23 : (lane < 6 ? 29 : 31));   // the zero entries
+        if (lane < 8) rw[2 * zc + (lane & 1)] = 0.f;
+    }
 
     // this lane's operand roles: L row a = q (its x powers and middle-row
     // offset), N column b = q (its y powers and middle-column offset)
@@ -3800,7 +3829,7 @@ void k_gm_eval(const GmArgs A) {
     // largest n of each.
     constexpr int GM_MA[4] = {0, 3, 5, 6}, GM_MB[4] = {1, 2, 4, 6};
     constexpr int GM_HL[4] = {7, 5, 3, 1};         // Horner length of slot s (the larger half's)
-    int hbase[4], hlim[4];
+    int hbase[4];
     float lx[8];
     {
         float xp[8];
@@ -3811,8 +3840,7 @@ void k_gm_eval(const GmArgs A) {
         for (int j = 0; j < 4; ++j) {
             lx[j] = g ? xp[GM_MB[j]] : xp[GM_MA[j]];
             lx[4 + j] = g ? xp[GM_MB[j] + 1] : xp[GM_MA[j] + 1];
-            hbase[j] = g ? gm_kT(GM_MB[j], 0) : gm_kT(GM_MA[j], 0);
-            hlim[j] = g ? (j == 3 ? -1 : 6 - GM_MB[j]) : 6 - GM_MA[j];   // the pad: no term
+            hbase[j] = g ? GM_PB[j == 3 ? 7 : GM_MB[j]] : GM_PB[GM_MA[j]];   // the pad: the zero entry
         }
     }
     const int pmid = (P.z0 + P.mz) * Ny + P.y0 + P.my;
@@ -3894,13 +3922,13 @@ void k_gm_eval(const GmArgs A) {
         int tn, gn;
         const bool hn_ = walk.next(tn, gn);
         const float rec_next = hn_ ? rec_at(tn) : 0.f;     // in flight during this unit
-        rw[lane] = rec;
+        rw[dpos] = rec;
         gm_wave_sync();
-        const int species = __float_as_int(rw[2 * GM_MOM + 4]);
+        const int species = __float_as_int(rw[GM_EVX + 4]);
         if (since > 0 && species != acc_species) flush();
         acc_species = species;
-        const float pht = rw[2 * GM_MOM], kox = rw[2 * GM_MOM + 1];
-        const float koy = rw[2 * GM_MOM + 2], koz = rw[2 * GM_MOM + 3];
+        const float pht = rw[GM_EVX], kox = rw[GM_EVX + 1];
+        const float koy = rw[GM_EVX + 2], koz = rw[GM_EVX + 3];
         if (!ref_set) {
             kos_x = kox;
             kos_y = koy;
@@ -3914,21 +3942,16 @@ void k_gm_eval(const GmArgs A) {
         const bool sep = !A.pixel_eval && kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
         if (!sep && live && lane == 0 && A.fallbacks) atomicAdd(A.fallbacks, 1u);
         if (sep) {
-            // G of this lane's four powers at its column y = yb; a half's
-            // top term beyond its own degree is zero
+            // G of this lane's four powers at its column y = yb (a half's
+            // terms past its own degree read the zero pads)
             float Gr[4], Gi[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 float gr = 0.f, gi = 0.f;
 #pragma unroll
                 for (int n = GM_HL[j] - 1; n >= 0; --n) {
-                    float cr = rw[2 * (hbase[j] + n)], ci = rw[2 * (hbase[j] + n) + 1];
-                    if (n == GM_HL[j] - 1) {
-                        cr = n <= hlim[j] ? cr : 0.f;
-                        ci = n <= hlim[j] ? ci : 0.f;
-                    }
-                    gr = fmaf(gr, yb, cr);
-                    gi = fmaf(gi, yb, ci);
+                    gr = fmaf(gr, yb, rw[2 * (hbase[j] + n)]);
+                    gi = fmaf(gi, yb, rw[2 * (hbase[j] + n) + 1]);
                 }
                 Gr[j] = gr;
                 Gi[j] = gi;
@@ -3942,29 +3965,28 @@ void k_gm_eval(const GmArgs A) {
             const float cf = -256.f * kap * yb;
             // N(k, b): slot j < 4: T = beta G of the j-th power; slot 4 + j its
             // cross value cf i T = (-cf Ti, cf Tr)
-            float nr[8], ni[8], lr[8], li[8];
+            float tr[4], ti[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float tr = br * Gr[j] - bi * Gi[j];
-                const float ti = br * Gi[j] + bi * Gr[j];
-                nr[j] = tr;
-                ni[j] = ti;
-                nr[4 + j] = -cf * ti;
-                ni[4 + j] = cf * tr;
+                tr[j] = br * Gr[j] - bi * Gi[j];
+                ti[j] = br * Gi[j] + bi * Gr[j];
             }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                lr[j] = ar * lx[j];
-                li[j] = ai * lx[j];
-            }
+            // L = alpha x^m (slots 0..7), N: slots 0..3 T, slots 4..7 cf i T
+            // = (-cf Ti, cf Tr); the products split as they are formed
             uintx4 lrh, lrl, lih, lil, nrh, nrl, nih, nil;
 #pragma unroll
             for (int u2 = 0; u2 < 4; ++u2) {
                 unsigned int h, l;
-                gm_split_pair(lr[2 * u2], lr[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
-                gm_split_pair(li[2 * u2], li[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
-                gm_split_pair(nr[2 * u2], nr[2 * u2 + 1], h, l); nrh[u2] = h; nrl[u2] = l;
-                gm_split_pair(ni[2 * u2], ni[2 * u2 + 1], h, l); nih[u2] = h; nil[u2] = l;
+                gm_split_prod_pair(ar, lx[2 * u2], lx[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
+                gm_split_prod_pair(ai, lx[2 * u2], lx[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
+                if (u2 < 2) {
+                    gm_split_pair(tr[2 * u2], tr[2 * u2 + 1], h, l); nrh[u2] = h; nrl[u2] = l;
+                    gm_split_pair(ti[2 * u2], ti[2 * u2 + 1], h, l); nih[u2] = h; nil[u2] = l;
+                } else {
+                    const int j = 2 * (u2 - 2);
+                    gm_split_prod_pair(-cf, ti[j], ti[j + 1], h, l); nrh[u2] = h; nrl[u2] = l;
+                    gm_split_prod_pair(cf, tr[j], tr[j + 1], h, l); nih[u2] = h; nil[u2] = l;
+                }
             }
             const uintx4 sgn = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
             const halfx8 Lrh = __builtin_bit_cast(halfx8, lrh), Lrl = __builtin_bit_cast(halfx8, lrl);
@@ -3989,20 +4011,16 @@ void k_gm_eval(const GmArgs A) {
             // exact phase per pixel: G at this lane's column is G_m(yb) (the
             // lane's pixels share its column), Horner's rule in x per pixel
             float Gr[D + 1], Gi[D + 1];
-            {
-                int base = 0;
 #pragma unroll
-                for (int m = 0; m <= D; ++m) {
-                    float gr = 0.f, gi = 0.f;
+            for (int m = 0; m <= D; ++m) {
+                float gr = 0.f, gi = 0.f;
 #pragma unroll
-                    for (int n = D - m; n >= 0; --n) {
-                        gr = fmaf(gr, yb, rw[2 * (base + n)]);
-                        gi = fmaf(gi, yb, rw[2 * (base + n) + 1]);
-                    }
-                    Gr[m] = gr;
-                    Gi[m] = gi;
-                    base += D - m + 1;
+                for (int n = D - m; n >= 0; --n) {
+                    gr = fmaf(gr, yb, rw[2 * (GM_PB[m] + n)]);
+                    gi = fmaf(gi, yb, rw[2 * (GM_PB[m] + n) + 1]);
                 }
+                Gr[m] = gr;
+                Gi[m] = gi;
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
