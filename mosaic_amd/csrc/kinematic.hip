@@ -3874,7 +3874,10 @@ void k_gm_eval(const GmArgs A) {
         }
         for (int o = 32; o > 0; o >>= 1) dev = fmaxf(dev, __shfl_xor(dev, o, 64));
     }
-    const float hn = sqrtf(fmaf(hx, hx, fmaf(hy, hy, hz * hz)));
+    // the separable form's bounds per unit, on squares: K |dO| dev <= 1e-6
+    // and the cross term at the corners, (2 pi K |dO| |h| 256)^2 / 2 <= 1e-6
+    const float dev2 = dev * dev;
+    const float cx2 = 0.5f * (TWO_PI_F * 256.f) * (TWO_PI_F * 256.f) * fmaf(hx, hx, fmaf(hy, hy, hz * hz));
     (void)pmid;
 
     floatx16 sr, si;
@@ -3936,10 +3939,8 @@ void k_gm_eval(const GmArgs A) {
             ref_set = true;
         }
         const float dox = kox - kos_x, doy = koy - kos_y, doz = koz - kos_z;
-        const float kdo = sqrtf(fmaf(dox, dox, fmaf(doy, doy, doz * doz)));
-        // the unit's cross term at the patch corners [rad]
-        const float cmax = TWO_PI_F * kdo * hn * 256.f;
-        const bool sep = !A.pixel_eval && kdo * dev <= 1e-6f && 0.5f * cmax * cmax <= 1e-6f;
+        const float kdo2 = fmaf(dox, dox, fmaf(doy, doy, doz * doz));
+        const bool sep = !A.pixel_eval && kdo2 * dev2 <= 1e-12f && cx2 * kdo2 <= 1e-6f;
         if (!sep && live && lane == 0 && A.fallbacks) atomicAdd(A.fallbacks, 1u);
         if (sep) {
             // G of this lane's four powers at its column y = yb (a half's
