@@ -3349,12 +3349,15 @@ template <bool SECOND>
 __global__ __launch_bounds__(64 * GM_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void k_gm_moments(const GmArgs A) {
     // Two chunk buffers, each: the basis hi halves [column][atom position],
-    // then the lo halves of blocks 0 and 1.  At a unit's end the same bytes
-    // hold each wave's products for the fold.
-    constexpr int SB_H = GM_NB * 32 * GM_BS * 2, SB_L = GM_NSPLIT * GM_BS * 2, SB = SB_H + SB_L;
-    constexpr int SMEM = 2 * SB > GM_WAVES * 32 * GM_CS * 4 ? 2 * SB : GM_WAVES * 32 * GM_CS * 4;
-    static_assert(SB % 16 == 0, "the chunk buffers stay 16-byte aligned");
-    __shared__ __align__(16) unsigned char smem[SMEM];
+    // then the lo halves of blocks 0 and 1; and a third region.  At a unit's
+    // end waves 0, 1 put their products for the fold into the buffer the
+    // last chunk read, waves 2, 3 into the third region, while the other
+    // buffer holds the next unit's first chunk.
+    constexpr int SB_H = GM_NB * 32 * GM_BS * 2, SB_L = GM_NSPLIT * GM_BS * 2;
+    constexpr int SB_P = 2 * 32 * GM_CS * 4;
+    constexpr int SB = SB_H + SB_L > SB_P ? SB_H + SB_L : SB_P;
+    static_assert(SB % 16 == 0 && GM_WAVES == 4, "the regions stay 16-byte aligned, two waves to each");
+    __shared__ __align__(16) unsigned char smem[3 * SB];
     __shared__ float sev[GM_WAVES][16][15];                      // each patch's E_c coefficients
     __shared__ double sgs[GM_WAVES][8];                          // the waves' gradient sums
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -3421,308 +3424,342 @@ void k_gm_moments(const GmArgs A) {
     const float xc1 = static_cast<float>(F.xs[1] - vc0[1]);
     const float xc2 = static_cast<float>(F.xs[2] - vc0[2]);
     const float hh = SECOND ? static_cast<float>(um.w / A.K) : 0.f;
-    float* const cbuf = reinterpret_cast<float*>(smem) + wave * 32 * GM_CS;
 
     const int t0 = A.t_begin + slice * A.a_tiles_per_slice;
     const int t1 = min(A.t_end, t0 + A.a_tiles_per_slice);
     GmHeadWalk walk(A.heads, t0, t1, lane);      // every wave walks the same units
-    int t, gsz;
-    while (walk.next(t, gsz)) {
-        // the unit's tiles (a unit of 2 repeats its second tile), its origin
-        // and each tile's origin offset from it [wavelengths]
+    // a unit: head tile t, gsz tiles, its origin (a unit of 2 repeats its
+    // second tile in the mean) and its head's species
+    struct UnitAt {
+        int t, gsz, species;
+        double ox, oy, oz;
+    };
+    auto unit_at = [&](int t, int gsz) {
         TileRec trk[4];
         trk[0] = A.tiles[t];
 #pragma unroll
         for (int k = 1; k < 4; ++k) trk[k] = A.tiles[t + min(k, gsz - 1)];
-        const TileRec& tr = trk[0];
-        double uox = tr.ox, uoy = tr.oy, uoz = tr.oz;
+        UnitAt u;
+        u.t = t;
+        u.gsz = gsz;
+        u.species = trk[0].species;
+        u.ox = trk[0].ox;
+        u.oy = trk[0].oy;
+        u.oz = trk[0].oz;
         if (gsz > 1) {
-            uox = unit_origin(gsz, trk[0].ox, trk[1].ox, trk[2].ox, trk[3].ox);
-            uoy = unit_origin(gsz, trk[0].oy, trk[1].oy, trk[2].oy, trk[3].oy);
-            uoz = unit_origin(gsz, trk[0].oz, trk[1].oz, trk[2].oz, trk[3].oz);
+            u.ox = unit_origin(gsz, trk[0].ox, trk[1].ox, trk[2].ox, trk[3].ox);
+            u.oy = unit_origin(gsz, trk[0].oy, trk[1].oy, trk[2].oy, trk[3].oy);
+            u.oz = unit_origin(gsz, trk[0].oz, trk[1].oz, trk[2].oz, trk[3].oz);
         }
-        // a chunk's place in the unit (tile ti, offset co) with its tile's
-        // count and origin offset [wavelengths]
-        struct ChunkAt {
-            int ti, co, cnt;
-            float dx, dy, dz;
+        return u;
+    };
+    // a chunk's place (tile tl, ti in its unit, offset co) with its tile's
+    // count and origin offset from the unit origin [wavelengths]
+    struct ChunkAt {
+        int tl, ti, co, cnt;
+        float dx, dy, dz;
+    };
+    auto at_tile = [&](const UnitAt& u, int ti) {
+        const TileRec r = A.tiles[u.t + ti];
+        ChunkAt c;
+        c.tl = u.t + ti;
+        c.ti = ti;
+        c.co = 0;
+        c.cnt = r.count & 0xFFFF;
+        c.dx = static_cast<float>((r.ox - u.ox) * A.K);
+        c.dy = static_cast<float>((r.oy - u.oy) * A.K);
+        c.dz = static_cast<float>((r.oz - u.oz) * A.K);
+        return c;
+    };
+    // the chunk after c in unit u, false past the unit's last
+    auto advance = [&](const UnitAt& u, ChunkAt& c) {
+        c.co += GM_CHUNK;
+        if (c.co >= c.cnt) {
+            if (c.ti + 1 >= u.gsz) return false;
+            c = at_tile(u, c.ti + 1);
+        }
+        return true;
+    };
+    auto rec = [&](const ChunkAt& c) {
+        return A.atoms[static_cast<int64_t>(c.tl) * TILE + c.co + lane];
+    };
+    GmUnitPhase U;
+    floatx16 c0, c1, c2;
+
+    // A chunk's basis rows, each value times the atom's |w| (the carrier
+    // operand is then a plain sine): wave w writes the columns f_w P^m
+    // R^n (m + n <= 4), f = 1, sigma_x, sigma_y, sigma_z, in four parts
+    // placed between the previous chunk's product steps, and then eight of
+    // the rest (records past a tile's count are zero: weight 0).
+    struct Rows {
+        float sx, sy, sz, f, Pb, Rb, ph;
+        float pw[7], rw[7];
+    };
+    auto rows_of = [&](const AtomRec& at, const ChunkAt& ca) {
+        Rows r;
+        r.sx = at.sx + ca.dx;
+        r.sy = at.sy + ca.dy;
+        r.sz = at.sz + ca.dz;
+        r.ph = at.ph;
+        r.Pb = fmaf(gby[2], r.sz, fmaf(gby[1], r.sy, gby[0] * r.sx));
+        r.Rb = fmaf(gbz[2], r.sz, fmaf(gbz[1], r.sy, gbz[0] * r.sx));
+        r.pw[0] = at.mag;
+        r.rw[0] = 1.f;
+#pragma unroll
+        for (int m = 1; m <= 6; ++m) {
+            r.pw[m] = r.pw[m - 1] * r.Pb;
+            r.rw[m] = r.rw[m - 1] * r.Rb;
+        }
+        r.f = wave == 0 ? 1.f : (wave == 1 ? r.sx : (wave == 2 ? r.sy : r.sz));
+        return r;
+    };
+    // Atom a of a chunk sits at position a with bits 2 and 3 swapped: the
+    // order in which the phase products leave the atoms in each lane.
+    const int apos = (lane & ~12) | ((lane & 4) << 1) | ((lane & 8) >> 1);
+    auto put_split = [&](_Float16* bh, _Float16* bl, int col, float v) {   // col < GM_NSPLIT
+        const _Float16 h = static_cast<_Float16>(v);
+        bh[col * GM_BS + apos] = h;
+        bl[col * GM_BS + apos] = static_cast<_Float16>(v - static_cast<float>(h));
+    };
+    // Two columns' values of this lane's atom into LDS as 32-bit words of
+    // two atoms: lanes 2k and 2k + 1 swap one value (DPP quad_perm
+    // [1, 0, 3, 2]), then lane 2k stores column A's word of atoms 2k, 2k + 1
+    // and lane 2k + 1 column B's (hi and lo words for split columns, else
+    // the hi word rounded to nearest).
+    auto put_pair = [&](_Float16* bh, _Float16* bl, int colA, float vA, int colB, float vB,
+                        bool split) {
+        const bool odd = (lane & 1) != 0;
+        const float x = odd ? vA : vB;
+        const float y = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+        const float first = odd ? y : vA, second = odd ? vB : y;
+        const int w = (odd ? colB : colA) * (GM_BS / 2) + (apos >> 1);
+        if (split) {
+            unsigned int hi, lo;
+            gm_split_pair(first, second, hi, lo);
+            reinterpret_cast<unsigned int*>(bh)[w] = hi;
+            reinterpret_cast<unsigned int*>(bl)[w] = lo;
+        } else {
+            halfx2 h;
+            h[0] = static_cast<__fp16>(first);
+            h[1] = static_cast<__fp16>(second);
+            reinterpret_cast<unsigned int*>(bh)[w] = __builtin_bit_cast(unsigned int, h);
+        }
+    };
+    // part 0..3 of the main columns (the 15 of m + n <= 4 in order, four
+    // to a part, the last alone) into buffer B
+    auto build_main = [&](int b, const Rows& r, int part) {
+        constexpr int KM[15] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 3, 3, 4};
+        constexpr int KN[15] = {0, 1, 2, 3, 4, 0, 1, 2, 3, 0, 1, 2, 0, 1, 0};
+        _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
+        _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
+        const int col0 = wave == 0 ? 0 : GM_COL_S + GM_N4 * (wave - 1);
+        auto val = [&](int i) { return r.f * (r.pw[KM[i]] * r.rw[KN[i]]); };
+#pragma unroll
+        for (int i = 4 * part; i < 4 * part + 4 && i < 15; i += 2) {
+            if (i + 1 < 15) put_pair(bh, bl, col0 + i, val(i), col0 + i + 1, val(i + 1), true);
+            else put_split(bh, bl, col0 + i, val(i));
+        }
+    };
+    // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
+    // waves 0..3 take eight each (the last one a pad column), in pairs
+    auto build_rest = [&](int b, const Rows& r) {
+        _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
+        _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
+        auto pair = [&](int ca, float va, int cb, float vb) {   // both split or both not
+            put_pair(bh, bl, ca, va, cb, vb, ca < GM_NSPLIT);
         };
-        auto at_tile = [&](int ti, const TileRec& r) {
-            ChunkAt c;
-            c.ti = ti;
-            c.co = 0;
-            c.cnt = r.count & 0xFFFF;
-            c.dx = static_cast<float>((r.ox - uox) * A.K);
-            c.dy = static_cast<float>((r.oy - uoy) * A.K);
-            c.dz = static_cast<float>((r.oz - uoz) * A.K);
-            return c;
-        };
-        // the chunk after c, false past the unit's last
-        auto advance = [&](ChunkAt& c) {
-            c.co += GM_CHUNK;
-            if (c.co >= c.cnt) {
-                if (c.ti + 1 >= gsz) return false;
-                c = at_tile(c.ti + 1, A.tiles[t + c.ti + 1]);
-            }
-            return true;
-        };
-        const GmUnitPhase U = gm_unit_phase(F, uox, uoy, uoz, um, th_mid, vc0, A.K);
-        floatx16 c0, c1, c2;
+        const float ms = r.pw[0];
+        const float ss[6] = {ms * r.sx * r.sx, ms * r.sx * r.sy, ms * r.sx * r.sz,
+                             ms * r.sy * r.sy, ms * r.sy * r.sz, ms * r.sz * r.sz};
+        const float s3[3] = {1.f, r.Pb, r.Rb};
+        const float* pw = r.pw;
+        const float* rw = r.rw;
+        auto sig = [&](int i) { return ss[i / 3] * s3[i % 3]; };
+        static_assert(gm_colT(0, 5) + 4 == GM_NSPLIT, "split and hi-only pairs of wave 0");
+        if (wave == 0) {
+            pair(gm_colT(0, 5), pw[0] * rw[5], gm_colT(1, 4), pw[1] * rw[4]);
+            pair(gm_colT(2, 3), pw[2] * rw[3], gm_colT(3, 2), pw[3] * rw[2]);
+            pair(gm_colT(4, 1), pw[4] * rw[1], gm_colT(5, 0), pw[5] * rw[0]);
+            pair(gm_colT(0, 6), pw[0] * rw[6], gm_colT(1, 5), pw[1] * rw[5]);
+        } else if (wave == 1) {
+            pair(gm_colT(2, 4), pw[2] * rw[4], gm_colT(3, 3), pw[3] * rw[3]);
+            pair(gm_colT(4, 2), pw[4] * rw[2], gm_colT(5, 1), pw[5] * rw[1]);
+            pair(gm_colT(6, 0), pw[6] * rw[0], GM_COL_SS + 0, sig(0));
+            pair(GM_COL_SS + 1, sig(1), GM_COL_SS + 2, sig(2));
+        } else if (wave == 2) {
+#pragma unroll
+            for (int i = 3; i < 11; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
+        } else {
+#pragma unroll
+            for (int i = 11; i < 17; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
+            pair(GM_COL_SS + 17, sig(17), GM_NCOL, 0.f);   // with the first pad column
+        }
+    };
+    // The chunk's phases th(atom, row) = v . s + ph (+ 1/4 on the Re rows
+    // q < 16; with SECOND + hh (|s|^2 - (xc . s)^2)) as products on the
+    // matrix cores, 32 atoms x 32 rows from K = 2 (FP32, one per atom
+    // pair of operands): lane l holds A[atom l & 31][k = l >> 5] and
+    // B[k = l >> 5][row l & 31], so the operands are the atom records'
+    // (x, y) and (z, ph) pairs with lanes 32..63 holding the second of
+    // each, for atoms 0..31 and 32..63 (two v_permlane32_swap), against
+    // the lane patch's (vx, vy), (vz, 1).  The result leaves lane l the
+    // phases of row l & 31 at atoms 8 (r >> 2) + 4 (l >> 5) + (r & 3),
+    // r = 0..15: steps 0, 1 (2, 3) take r = 0..7, 8..15 of the first
+    // (second) 32 atoms, the atoms at their basis positions.
+    struct PhOps {
+        float xy0, zp0, s0, xy1, zp1, s1;
+    };
+    auto ph_ops = [&](const Rows& r) {
+        PhOps o;
+        const auto xy = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sx), __float_as_uint(r.sy),
+                                                         false, false);
+        const auto zp = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sz), __float_as_uint(r.ph),
+                                                         false, false);
+        o.xy0 = __uint_as_float(xy[0]);
+        o.xy1 = __uint_as_float(xy[1]);
+        o.zp0 = __uint_as_float(zp[0]);
+        o.zp1 = __uint_as_float(zp[1]);
+        o.s0 = o.s1 = 0.f;
+        if (SECOND) {
+            const float s2 = fmaf(r.sz, r.sz, fmaf(r.sy, r.sy, r.sx * r.sx));
+            const auto ss = __builtin_amdgcn_permlane32_swap(__float_as_uint(s2), 0u, false, false);
+            o.s0 = __uint_as_float(ss[0]);
+            o.s1 = __uint_as_float(ss[1]);
+        }
+        return o;
+    };
+    float pb0 = 0.f, pb1 = 0.f;                      // the unit's: g == 0 ? (vx, vz) : (vy, 1)
+    const float pq = q < 16 ? 0.25f : 0.f;
+    const float pw0 = g == 0 ? xc0 : xc1, pw1 = g == 0 ? xc2 : 0.f, ps = g == 0 ? hh : 0.f;
+    auto phases = [&](float xy, float zp, float s2) {
+        floatx16 c;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[r] = pq;
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pb0, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pb1, c, 0, 0, 0);
+        if (SECOND) {
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(s2, ps, c, 0, 0, 0);
+            floatx16 w;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) w[r] = 0.f;
+            w = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pw0, w, 0, 0, 0);
+            w = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pw1, w, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) c[r] = fmaf(-hh * w[r], w[r], c[r]);
+        }
+        return c;
+    };
+    // Product step s (16 atoms) of the chunk in buffer B: the carrier (one
+    // sine per operand value, from the phases th) against the three blocks.
+    auto product_step = [&](int b, int s, const floatx16& th) {
+        const _Float16* const bh = reinterpret_cast<const _Float16*>(smem + b * SB);
+        const _Float16* const bl = reinterpret_cast<const _Float16*>(smem + b * SB + SB_H);
+        const int r0 = 8 * (s & 1);
+        uintx4 hw, lw;
+#pragma unroll
+        for (int u2 = 0; u2 < 4; ++u2) {
+            unsigned int hv, lv;
+            gm_split_pair(__builtin_amdgcn_sinf(th[r0 + 2 * u2]), __builtin_amdgcn_sinf(th[r0 + 2 * u2 + 1]),
+                          hv, lv);
+            hw[u2] = hv;
+            lw[u2] = lv;
+        }
+        const halfx8 ah = __builtin_bit_cast(halfx8, hw);
+        const halfx8 al = __builtin_bit_cast(halfx8, lw);
+        const int boff = 16 * s + 8 * g;
+        const halfx8 bh0 = *reinterpret_cast<const halfx8*>(bh + q * GM_BS + boff);
+        const halfx8 bl0 = *reinterpret_cast<const halfx8*>(bl + q * GM_BS + boff);
+        const halfx8 bh1 = *reinterpret_cast<const halfx8*>(bh + (32 + q) * GM_BS + boff);
+        const halfx8 bl1 = *reinterpret_cast<const halfx8*>(bl + (32 + q) * GM_BS + boff);
+        const halfx8 bh2 = *reinterpret_cast<const halfx8*>(bh + (64 + q) * GM_BS + boff);
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
+    };
+
+    // The units in order.  A unit's 64-atom chunks go in order through the
+    // two chunk buffers: the next chunk's basis goes into the other buffer
+    // between the current chunk's product steps, one barrier per chunk; each
+    // chunk's records are loaded a chunk ahead of its basis (every wave loads
+    // all 64: a lane builds one atom's share).  During a unit's last chunk
+    // the next unit's first chunk is built, so only the first unit builds
+    // one on its own.  The products at a unit's end go to the buffer its last
+    // chunk read (waves 0, 1) and the third region (waves 2, 3) for the fold.
+    int t, gsz;
+    if (!walk.next(t, gsz)) return;                 // (uniform over the workgroup)
+    UnitAt cur = unit_at(t, gsz);
+    ChunkAt first = at_tile(cur, 0);
+    ChunkAt cn = first;
+    bool more = advance(cur, cn);                   // cn: the chunk after the first
+    AtomRec nxt;                                    // the record of more ? cn : fb
+    AtomRec pre = rec(cn);                          // cn's record, loaded ahead (when more)
+    auto unit_phase = [&]() {
+        U = gm_unit_phase(F, cur.ox, cur.oy, cur.oz, um, th_mid, vc0, A.K);
+        pb0 = g == 0 ? U.vx : U.vy;
+        pb1 = g == 0 ? U.vz : 1.f;
+    };
+    unit_phase();
+    PhOps pa;
+    floatx16 thA;                                  // the current chunk's first 32 atoms' phases
+    {
+        const Rows r0 = rows_of(rec(first), first);
+#pragma unroll
+        for (int part = 0; part < 4; ++part) build_main(0, r0, part);
+        build_rest(0, r0);
+        pa = ph_ops(r0);
+        thA = phases(pa.xy0, pa.zp0, pa.s0);
+    }
+    __syncthreads();
+    int b = 0;                                     // the current chunk's buffer
+    for (;;) {
+        // the next unit (its first chunk is built during this unit's last),
+        // else this unit's first chunk again into the buffer nothing reads
+        int tn, gn;
+        const bool have_next = walk.next(tn, gn);
+        UnitAt nu = cur;
+        ChunkAt fb = first;
+        if (have_next) {
+            nu = unit_at(tn, gn);
+            fb = at_tile(nu, 0);
+        }
+        ChunkAt cn1 = fb;                              // the next unit's second chunk
+        const bool more1 = have_next && advance(nu, cn1);
+        nxt = more ? pre : rec(fb);
 #pragma unroll
         for (int r = 0; r < 16; ++r) { c0[r] = 0.f; c1[r] = 0.f; c2[r] = 0.f; }
-
-        // A chunk's basis rows, each value times the atom's |w| (the carrier
-        // operand is then a plain sine): wave w writes the columns f_w P^m
-        // R^n (m + n <= 4), f = 1, sigma_x, sigma_y, sigma_z, in four parts
-        // placed between the previous chunk's product steps, and then eight of
-        // the rest (records past a tile's count are zero: weight 0).
-        struct Rows {
-            float sx, sy, sz, f, Pb, Rb, ph;
-            float pw[7], rw[7];
-        };
-        auto rows_of = [&](const AtomRec& at, const ChunkAt& ca) {
-            Rows r;
-            r.sx = at.sx + ca.dx;
-            r.sy = at.sy + ca.dy;
-            r.sz = at.sz + ca.dz;
-            r.ph = at.ph;
-            r.Pb = fmaf(gby[2], r.sz, fmaf(gby[1], r.sy, gby[0] * r.sx));
-            r.Rb = fmaf(gbz[2], r.sz, fmaf(gbz[1], r.sy, gbz[0] * r.sx));
-            r.pw[0] = at.mag;
-            r.rw[0] = 1.f;
-#pragma unroll
-            for (int m = 1; m <= 6; ++m) {
-                r.pw[m] = r.pw[m - 1] * r.Pb;
-                r.rw[m] = r.rw[m - 1] * r.Rb;
-            }
-            r.f = wave == 0 ? 1.f : (wave == 1 ? r.sx : (wave == 2 ? r.sy : r.sz));
-            return r;
-        };
-        // Atom a of a chunk sits at position a with bits 2 and 3 swapped: the
-        // order in which the phase products leave the atoms in each lane.
-        const int apos = (lane & ~12) | ((lane & 4) << 1) | ((lane & 8) >> 1);
-        auto put_split = [&](_Float16* bh, _Float16* bl, int col, float v) {   // col < GM_NSPLIT
-            const _Float16 h = static_cast<_Float16>(v);
-            bh[col * GM_BS + apos] = h;
-            bl[col * GM_BS + apos] = static_cast<_Float16>(v - static_cast<float>(h));
-        };
-        // Two columns' values of this lane's atom into LDS as 32-bit words of
-        // two atoms: lanes 2k and 2k + 1 swap one value (DPP quad_perm
-        // [1, 0, 3, 2]), then lane 2k stores column A's word of atoms 2k, 2k + 1
-        // and lane 2k + 1 column B's (hi and lo words for split columns, else
-        // the hi word rounded to nearest).
-        auto put_pair = [&](_Float16* bh, _Float16* bl, int colA, float vA, int colB, float vB,
-                            bool split) {
-            const bool odd = (lane & 1) != 0;
-            const float x = odd ? vA : vB;
-            const float y = __int_as_float(
-                __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
-            const float first = odd ? y : vA, second = odd ? vB : y;
-            const int w = (odd ? colB : colA) * (GM_BS / 2) + (apos >> 1);
-            if (split) {
-                unsigned int hi, lo;
-                gm_split_pair(first, second, hi, lo);
-                reinterpret_cast<unsigned int*>(bh)[w] = hi;
-                reinterpret_cast<unsigned int*>(bl)[w] = lo;
-            } else {
-                halfx2 h;
-                h[0] = static_cast<__fp16>(first);
-                h[1] = static_cast<__fp16>(second);
-                reinterpret_cast<unsigned int*>(bh)[w] = __builtin_bit_cast(unsigned int, h);
-            }
-        };
-        // part 0..3 of the main columns (the 15 of m + n <= 4 in order, four
-        // to a part, the last alone) into buffer B
-        auto build_main = [&](auto B, const Rows& r, int part) {
-            constexpr int KM[15] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 3, 3, 4};
-            constexpr int KN[15] = {0, 1, 2, 3, 4, 0, 1, 2, 3, 0, 1, 2, 0, 1, 0};
-            constexpr int b = decltype(B)::value;
-            _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
-            _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
-            const int col0 = wave == 0 ? 0 : GM_COL_S + GM_N4 * (wave - 1);
-            auto val = [&](int i) { return r.f * (r.pw[KM[i]] * r.rw[KN[i]]); };
-#pragma unroll
-            for (int i = 4 * part; i < 4 * part + 4 && i < 15; i += 2) {
-                if (i + 1 < 15) put_pair(bh, bl, col0 + i, val(i), col0 + i + 1, val(i + 1), true);
-                else put_split(bh, bl, col0 + i, val(i));
-            }
-        };
-        // P^m R^n, m + n = 5, 6 (13) and sigma_c sigma_d {1, P, R} (18):
-        // waves 0..3 take eight each (the last one a pad column), in pairs
-        auto build_rest = [&](auto B, const Rows& r) {
-            constexpr int b = decltype(B)::value;
-            _Float16* const bh = reinterpret_cast<_Float16*>(smem + b * SB);
-            _Float16* const bl = reinterpret_cast<_Float16*>(smem + b * SB + SB_H);
-            auto pair = [&](int ca, float va, int cb, float vb) {   // both split or both not
-                put_pair(bh, bl, ca, va, cb, vb, ca < GM_NSPLIT);
-            };
-            const float ms = r.pw[0];
-            const float ss[6] = {ms * r.sx * r.sx, ms * r.sx * r.sy, ms * r.sx * r.sz,
-                                 ms * r.sy * r.sy, ms * r.sy * r.sz, ms * r.sz * r.sz};
-            const float s3[3] = {1.f, r.Pb, r.Rb};
-            const float* pw = r.pw;
-            const float* rw = r.rw;
-            auto sig = [&](int i) { return ss[i / 3] * s3[i % 3]; };
-            static_assert(gm_colT(0, 5) + 4 == GM_NSPLIT, "split and hi-only pairs of wave 0");
-            if (wave == 0) {
-                pair(gm_colT(0, 5), pw[0] * rw[5], gm_colT(1, 4), pw[1] * rw[4]);
-                pair(gm_colT(2, 3), pw[2] * rw[3], gm_colT(3, 2), pw[3] * rw[2]);
-                pair(gm_colT(4, 1), pw[4] * rw[1], gm_colT(5, 0), pw[5] * rw[0]);
-                pair(gm_colT(0, 6), pw[0] * rw[6], gm_colT(1, 5), pw[1] * rw[5]);
-            } else if (wave == 1) {
-                pair(gm_colT(2, 4), pw[2] * rw[4], gm_colT(3, 3), pw[3] * rw[3]);
-                pair(gm_colT(4, 2), pw[4] * rw[2], gm_colT(5, 1), pw[5] * rw[1]);
-                pair(gm_colT(6, 0), pw[6] * rw[0], GM_COL_SS + 0, sig(0));
-                pair(GM_COL_SS + 1, sig(1), GM_COL_SS + 2, sig(2));
-            } else if (wave == 2) {
-#pragma unroll
-                for (int i = 3; i < 11; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
-            } else {
-#pragma unroll
-                for (int i = 11; i < 17; i += 2) pair(GM_COL_SS + i, sig(i), GM_COL_SS + i + 1, sig(i + 1));
-                pair(GM_COL_SS + 17, sig(17), GM_NCOL, 0.f);   // with the first pad column
-            }
-        };
-        // The chunk's phases th(atom, row) = v . s + ph (+ 1/4 on the Re rows
-        // q < 16; with SECOND + hh (|s|^2 - (xc . s)^2)) as products on the
-        // matrix cores, 32 atoms x 32 rows from K = 2 (FP32, one per atom
-        // pair of operands): lane l holds A[atom l & 31][k = l >> 5] and
-        // B[k = l >> 5][row l & 31], so the operands are the atom records'
-        // (x, y) and (z, ph) pairs with lanes 32..63 holding the second of
-        // each, for atoms 0..31 and 32..63 (two v_permlane32_swap), against
-        // the lane patch's (vx, vy), (vz, 1).  The result leaves lane l the
-        // phases of row l & 31 at atoms 8 (r >> 2) + 4 (l >> 5) + (r & 3),
-        // r = 0..15: steps 0, 1 (2, 3) take r = 0..7, 8..15 of the first
-        // (second) 32 atoms, the atoms at their basis positions.
-        struct PhOps {
-            float xy0, zp0, s0, xy1, zp1, s1;
-        };
-        auto ph_ops = [&](const Rows& r) {
-            PhOps o;
-            const auto xy = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sx), __float_as_uint(r.sy),
-                                                             false, false);
-            const auto zp = __builtin_amdgcn_permlane32_swap(__float_as_uint(r.sz), __float_as_uint(r.ph),
-                                                             false, false);
-            o.xy0 = __uint_as_float(xy[0]);
-            o.xy1 = __uint_as_float(xy[1]);
-            o.zp0 = __uint_as_float(zp[0]);
-            o.zp1 = __uint_as_float(zp[1]);
-            o.s0 = o.s1 = 0.f;
-            if (SECOND) {
-                const float s2 = fmaf(r.sz, r.sz, fmaf(r.sy, r.sy, r.sx * r.sx));
-                const auto ss = __builtin_amdgcn_permlane32_swap(__float_as_uint(s2), 0u, false, false);
-                o.s0 = __uint_as_float(ss[0]);
-                o.s1 = __uint_as_float(ss[1]);
-            }
-            return o;
-        };
-        const float pb0 = g == 0 ? U.vx : U.vy, pb1 = g == 0 ? U.vz : 1.f;
-        const float pq = q < 16 ? 0.25f : 0.f;
-        const float pw0 = g == 0 ? xc0 : xc1, pw1 = g == 0 ? xc2 : 0.f, ps = g == 0 ? hh : 0.f;
-        auto phases = [&](float xy, float zp, float s2) {
-            floatx16 c;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) c[r] = pq;
-            c = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pb0, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pb1, c, 0, 0, 0);
-            if (SECOND) {
-                c = __builtin_amdgcn_mfma_f32_32x32x2f32(s2, ps, c, 0, 0, 0);
-                floatx16 w;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) w[r] = 0.f;
-                w = __builtin_amdgcn_mfma_f32_32x32x2f32(xy, pw0, w, 0, 0, 0);
-                w = __builtin_amdgcn_mfma_f32_32x32x2f32(zp, pw1, w, 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 16; ++r) c[r] = fmaf(-hh * w[r], w[r], c[r]);
-            }
-            return c;
-        };
-        // Product step s (16 atoms) of the chunk in buffer B: the carrier (one
-        // sine per operand value, from the phases th) against the three blocks.
-        auto product_step = [&](auto B, int s, const floatx16& th) {
-            constexpr int b = decltype(B)::value;
-            const _Float16* const bh = reinterpret_cast<const _Float16*>(smem + b * SB);
-            const _Float16* const bl = reinterpret_cast<const _Float16*>(smem + b * SB + SB_H);
-            const int r0 = 8 * (s & 1);
-            uintx4 hw, lw;
-#pragma unroll
-            for (int u2 = 0; u2 < 4; ++u2) {
-                unsigned int hv, lv;
-                gm_split_pair(__builtin_amdgcn_sinf(th[r0 + 2 * u2]), __builtin_amdgcn_sinf(th[r0 + 2 * u2 + 1]),
-                              hv, lv);
-                hw[u2] = hv;
-                lw[u2] = lv;
-            }
-            const halfx8 ah = __builtin_bit_cast(halfx8, hw);
-            const halfx8 al = __builtin_bit_cast(halfx8, lw);
-            const int boff = 16 * s + 8 * g;
-            const halfx8 bh0 = *reinterpret_cast<const halfx8*>(bh + q * GM_BS + boff);
-            const halfx8 bl0 = *reinterpret_cast<const halfx8*>(bl + q * GM_BS + boff);
-            const halfx8 bh1 = *reinterpret_cast<const halfx8*>(bh + (32 + q) * GM_BS + boff);
-            const halfx8 bl1 = *reinterpret_cast<const halfx8*>(bl + (32 + q) * GM_BS + boff);
-            const halfx8 bh2 = *reinterpret_cast<const halfx8*>(bh + (64 + q) * GM_BS + boff);
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, c1, 0, 0, 0);
-            c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh2, c2, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, c1, 0, 0, 0);
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, c1, 0, 0, 0);
-        };
-        auto rec = [&](const ChunkAt& c) {
-            return A.atoms[static_cast<int64_t>(t + c.ti) * TILE + c.co + lane];
-        };
-        // The unit's 64-atom chunks in order: the next chunk's basis goes into
-        // the other buffer between the current chunk's product steps, one
-        // barrier per chunk; each chunk's records are loaded a chunk ahead of
-        // its basis (every wave loads all 64: a lane builds one atom's share).
-        // After the last chunk the first one is built again into the buffer
-        // that nothing reads any more.
-        using B0 = std::integral_constant<int, 0>;
-        using B1 = std::integral_constant<int, 1>;
-        const ChunkAt first = at_tile(0, tr);
-        ChunkAt cn = first;
-        bool more = advance(cn);                         // cn: the chunk after the first
-        AtomRec nxt = rec(more ? cn : first);
-        PhOps pa;
-        floatx16 thA;                                  // the current chunk's first 32 atoms' phases
-        {
-            const Rows r0 = rows_of(rec(first), first);
-#pragma unroll
-            for (int part = 0; part < 4; ++part) build_main(B0{}, r0, part);
-            build_rest(B0{}, r0);
-            pa = ph_ops(r0);
-            thA = phases(pa.xy0, pa.zp0, pa.s0);
-        }
-        __syncthreads();
-        // one chunk: products from buffer B, the next chunk's rows into the other
-        auto chunk = [&](auto B) {
-            using NB = std::integral_constant<int, 1 - decltype(B)::value>;
-            ChunkAt c2 = cn;
-            const bool more2 = more && advance(c2);
-            const AtomRec nxt2 = rec(more2 ? c2 : first);
-            const Rows r = rows_of(nxt, more ? cn : first);
+        // one chunk: products from buffer b, the next chunk's rows into the other
+        auto chunk = [&]() {
+            ChunkAt c2n = cn;
+            const bool more2 = more && advance(cur, c2n);
+            const AtomRec nxt2 = rec(more2 ? c2n : fb);
+            const Rows r = rows_of(nxt, more ? cn : fb);
+            if (!more && more1) pre = rec(cn1);    // the next unit's, a unit ahead
             floatx16 thB;                          // the second 32 atoms' phases
 #pragma unroll
             for (int s = 0; s < GM_CHUNK / 16; ++s) {
                 if (s == 1) thB = phases(pa.xy1, pa.zp1, pa.s1);
-                product_step(B, s, s < 2 ? thA : thB);
-                build_main(NB{}, r, s);
+                product_step(b, s, s < 2 ? thA : thB);
+                build_main(b ^ 1, r, s);
             }
-            build_rest(NB{}, r);
+            build_rest(b ^ 1, r);
             pa = ph_ops(r);
-            thA = phases(pa.xy0, pa.zp0, pa.s0);
-            __syncthreads();                       // buffer B is free, the other complete
+            if (more) thA = phases(pa.xy0, pa.zp0, pa.s0);   // (the next unit's: after its phase terms)
+            __syncthreads();                       // buffer b is free, the other complete
             const bool had = more;
-            cn = c2;
+            cn = c2n;
             more = more2;
             nxt = nxt2;
             return had;
         };
-        while (chunk(B0{}) && chunk(B1{})) {
-        }
+        while (chunk()) b ^= 1;
         // the products into this wave's LDS (rows: Re of patches 0..15, then
         // Im), then the fold
+        float* const cbuf = reinterpret_cast<float*>(smem + (wave < 2 ? b : 2) * SB) + (wave & 1) * 32 * GM_CS;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = 8 * (r >> 2) + 4 * g + (r & 3);
@@ -3733,13 +3770,21 @@ void k_gm_moments(const GmArgs A) {
         gm_wave_sync();
         {
             const int patch = ppz * A.pby + ppy;
-            float2* dst = plive ? A.mom + ((static_cast<int64_t>(t - A.t_begin) * A.n_frames + frame) *
+            float2* dst = plive ? A.mom + ((static_cast<int64_t>(cur.t - A.t_begin) * A.n_frames + frame) *
                                                A.n_patches + patch) * GM_REC
                                 : nullptr;
             gm_fold4(cbuf, lane, sev[wave], make_float2(U.pht, U.kox), make_float2(U.koy, U.koz),
-                     tr.species, dst);
+                     cur.species, dst);
         }
-        __syncthreads();                           // the next unit's first basis rewrites the products
+        __syncthreads();                           // buffer b takes the next unit's second chunk
+        if (!have_next) break;
+        cur = nu;
+        first = fb;
+        cn = cn1;
+        more = more1;
+        unit_phase();
+        thA = phases(pa.xy0, pa.zp0, pa.s0);
+        b ^= 1;
     }
 }
 

@@ -160,6 +160,21 @@ def test_gemm_second_order_close_detector(device):
     assert e <= TOL
 
 
+def test_gemm_units_across_slices(device):
+    """A 36^3-cell perovskite (233,280 atoms; Sr and Ti each 183 tiles, the
+    last of 64 atoms at an even place, so a unit of one tile and one chunk)
+    on a 512^2 detector (four regions, slices of about 15 tiles): each
+    unit's first chunk is built during the previous unit's last, and the
+    field agrees with the VALU moment sum."""
+    w = W.perovskite_screw_bragg200(n=36, npix=512)
+    A = gpu_field(w, path="auto")
+    assert w["beam"]._last_path in GEMM
+    B = gpu_field(w, path="poly")
+    assert w["beam"]._last_path % 100 in (45, 46)
+    e = report("perovskite36 x 512^2: gemm vs poly", A, B)
+    assert e <= TOL
+
+
 def test_gemm_matches_valu_moment_sum(device):
     """A 200,000-atom FCC crystal on a 256^2 detector (one region of 8 x 8
     patches, units of up to four tiles): the automatic choice takes the
