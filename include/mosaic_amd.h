@@ -90,8 +90,11 @@ int mosaic_kin_workspace(int64_t atom_count, int n_species,
  *   species  device (N,) uint8 species codes (< n_species)
  *   amp      device (N,) complex64 incident amplitude, or NULL for 1
  *   stage_stats  host out [2]: largest tile radius in wavelengths, largest
- *                tile-origin distance from the sample origin in metres
- * The call synchronizes the stream to return stage_stats.
+ *                tile-origin distance from the sample origin in metres, or
+ *                NULL to leave them in the workspace (mosaic_kin_stats,
+ *                mosaic_kin_sum read them there)
+ * The call synchronizes the stream to return stage_stats; with NULL it
+ * returns without waiting.
  */
 int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint8_t* species,
                      const float* amp, int n_species,
@@ -131,8 +134,10 @@ int mosaic_kin_restage(int64_t atom_count, const float* pos, const float* amp,
  * quadratic model of the far-field sums, the largest second and first
  * differences along a column, the largest deviation from the model with the
  * cross term a b g_yz (the moment expansion's), and the largest |g_yz|; the
- * call synchronizes the stream for them.  Must precede mosaic_kin_sum
- * on the same workspace.  `coords` is device (3, Ny*Nz) float64 metres.
+ * call synchronizes the stream for them.  With pixel_geom NULL they stay in
+ * the workspace and the call returns without waiting.  Must precede
+ * mosaic_kin_sum on the same workspace.  `coords` is device (3, Ny*Nz)
+ * float64 metres.
  */
 int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host */,
                       const mosaic_kin_opts_t* opts /* host */, int Ny, int Nz,
@@ -182,12 +187,15 @@ int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_tab /* host
  * A forced path (1-4, 6, 7) returns MOSAIC_ERR_UNSUPPORTED (nothing
  * written) when its bounds fail.
  * `stage_stats` and `pixel_geom` are the outputs of mosaic_kin_stage and
- * mosaic_kin_pixels; `path_used` (host out, may be NULL) reports 11/12
+ * mosaic_kin_pixels, or NULL where those calls left them in the workspace:
+ * the call then reads them there and synchronizes the stream once (one wait
+ * for both, in place of one in each of those calls); `path_used` (host out, may be NULL) reports 11/12
  * (per-tile, first/second order), 24/28 (row recurrence over 4/8 pixels),
  * 32 (matrix-core), 45/46 (moment expansion of degree 5/6), 76 (the same on
  * the matrix cores), 60 (probe), plus 100 for
  * the far-field forms with the atoms' second-order phase, or 50 + the
- * analyser kind (exact).  Stream-ordered, no host synchronization.
+ * analyser kind (exact).  Stream-ordered, no host synchronization when both
+ * statistics are given.
  */
 int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const int64_t* species_counts /* host */,
@@ -199,6 +207,20 @@ int mosaic_kin_sum(int64_t atom_count, int n_species,
                    const double* stage_stats /* host [2] */,
                    const double* pixel_geom /* host [8] */, void* workspace,
                    size_t workspace_bytes, int* path_used, mosaic_stream_t stream);
+
+/*
+ * The statistics that mosaic_kin_stage and mosaic_kin_pixels left in a
+ * workspace when called with NULL outputs: stage_stats (host out [2], or
+ * NULL) as mosaic_kin_stage returns them (zeros for an empty sample), and
+ * pixel_geom (host out [8], or NULL) as mosaic_kin_pixels returns them.
+ * Synchronizes the stream.  No counterpart in the reference: its staging
+ * and pixel tables return to Python at once (Beam.py:9265-9319, 9241-9262);
+ * here they may stay on the device until the sum needs them.
+ */
+int mosaic_kin_stats(int64_t atom_count, int n_species,
+                     const int64_t* species_counts /* host */, int n_frames, int Ny, int Nz,
+                     void* workspace, size_t workspace_bytes, double* stage_stats,
+                     double* pixel_geom, mosaic_stream_t stream);
 
 /*
  * Launch times of the kernels of mosaic_kin_sum calls on the calling

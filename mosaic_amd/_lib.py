@@ -31,6 +31,7 @@ EXPORTS = (
     "mosaic_kin_restage",
     "mosaic_kin_pixels",
     "mosaic_kin_sum",
+    "mosaic_kin_stats",
     "mosaic_kin_timing",
     "mosaic_kin_adjoint",
     "mosaic_kin_adjoint_scratch",
@@ -128,6 +129,7 @@ _SIGS = {
     "mosaic_kin_pixels": [_i, _vp, _vp, _i, _i, _vp, _vp, _sz, _vp, _vp],
     "mosaic_kin_sum": [_i64, _i, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _vp, _vp, _i, _vp, _vp,
                        _vp, _sz, _vp, _vp],
+    "mosaic_kin_stats": [_i64, _i, _vp, _i, _i, _i, _vp, _sz, _vp, _vp, _vp],
     "mosaic_kin_timing": [_i, _vp],
     "mosaic_kin_adjoint": [_i64, _i, _vp, _d, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
                            _sz, _vp, _sz, _i, _vp],

@@ -380,6 +380,13 @@ class beam:
                 off += n
         return amp
 
+    @property
+    def _last_stats(self):
+        """(largest tile radius [wavelengths], pixel geometry) of the last
+        staged sum: read from its workspace on first use (waits for it)."""
+        s = self._last_staged
+        return (s.radius_wl, tuple(s.pixel_geom))
+
     def _kin_field(self, sample, coords_A, shape, stage, frames, remove_forward,
                    apply_polarization, spherical_decay, out_dtype=torch.complex128,
                    analyser=None, use_depth_ein=False, ein_cache_dir=None,
@@ -467,7 +474,7 @@ class beam:
                     and staged_prev.counts.shape == staged.counts.shape):
                 # pixel tables sit at the head of the workspace and survive staging
                 staged.pixels_key = staged_prev.pixels_key
-                staged.pixel_geom = staged_prev.pixel_geom
+                staged.pixel_geom = staged_prev._pixel_geom
             self._workspace = staged.workspace
             if same:
                 K.kin_sum(staged, spc_tab, opts, rot, trans, coords, out)
@@ -478,7 +485,7 @@ class beam:
                     K.kin_sum(staged, spc_tab, opts, rot[f:f + 1], trans[f:f + 1], coords,
                               out[f:f + 1])
             self._last_path = staged.path_used
-            self._last_stats = (staged.radius_wl, tuple(staged.pixel_geom))
+            self._last_staged = staged
             staged_prev = staged
             del d_pos, d_cod, amp0
         return out

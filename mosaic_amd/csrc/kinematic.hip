@@ -5264,13 +5264,15 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
     if (rc != MOSAIC_OK) return rc;
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
     if (atom_count > 0 && (!pos || !species)) return MOSAIC_ERR_ARGS;
-    if (!(k > 0.0) || !stage_stats) return MOSAIC_ERR_ARGS;
+    if (!(k > 0.0)) return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
     rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
     if (rc != MOSAIC_OK) return rc;
-    stage_stats[0] = 0.0;
-    stage_stats[1] = 0.0;
+    if (stage_stats) {
+        stage_stats[0] = 0.0;
+        stage_stats[1] = 0.0;
+    }
     if (atom_count == 0) return MOSAIC_OK;
 
     SpeciesSplit split;
@@ -5398,6 +5400,7 @@ extern "C" int mosaic_kin_stage(int64_t atom_count, const float* pos, const uint
                                L.grp_r, L.tile_pos, L.radius_bits);
     }
     MOSAIC_HIP_CHECK(hipGetLastError());
+    if (!stage_stats) return MOSAIC_OK;             // left in the workspace, no wait
     MOSAIC_HIP_CHECK(hipMemcpyAsync(pin_bits, L.radius_bits, 2 * sizeof(unsigned int),
                                     hipMemcpyDeviceToHost, st));
     pinned_stage_mark(st);
@@ -5477,7 +5480,7 @@ extern "C" int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_
                                  const double* coords, void* workspace, size_t workspace_bytes,
                                  double* pixel_geom, mosaic_stream_t stream) {
     if (Ny < 1 || Nz < 1 || static_cast<int64_t>(Ny) * Nz > MAX_PIXELS || !coords || !opts ||
-        !(opts->k > 0.0) || !workspace || !pixel_geom || n_species < 1 ||
+        !(opts->k > 0.0) || !workspace || n_species < 1 ||
         n_species > MAX_SPECIES || !species_tab)
         return MOSAIC_ERR_ARGS;
     const int npix = Ny * Nz;
@@ -5499,6 +5502,7 @@ extern "C" int mosaic_kin_pixels(int n_species, const mosaic_species_t* species_
                        coords + 2 * (int64_t)npix, Ny, Nz, O, T);
     hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, st, T.stat_blocks, pb, T.stats);
     MOSAIC_HIP_CHECK(hipGetLastError());
+    if (!pixel_geom) return MOSAIC_OK;              // left in the workspace, no wait
     MOSAIC_HIP_CHECK(hipMemcpyAsync(pixel_geom, T.stats, N_GEOM * sizeof(double),
                                     hipMemcpyDeviceToHost, st));
     MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
@@ -5615,6 +5619,54 @@ void timing_done(KinTiming* T, int c, int rounds) {
 }
 }  // namespace
 
+namespace {
+// The stage statistics (largest tile radius [wavelengths], largest origin
+// distance [m]) and the pixel geometry that mosaic_kin_stage and
+// mosaic_kin_pixels left in workspace L, into host ss[2] / pg[N_GEOM]
+// (either may be null): one read through page-locked memory, one wait.
+int read_stats(const Layout& L, double* ss, double* pg, hipStream_t st) {
+    char* pin = pinned_stage(64 + N_GEOM * sizeof(double));
+    if (!pin) return MOSAIC_ERR_HIP;
+    if (ss)
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(pin, L.radius_bits, 2 * sizeof(unsigned int),
+                                        hipMemcpyDeviceToHost, st));
+    if (pg)
+        MOSAIC_HIP_CHECK(hipMemcpyAsync(pin + 64, L.px.stats, N_GEOM * sizeof(double),
+                                        hipMemcpyDeviceToHost, st));
+    pinned_stage_mark(st);
+    MOSAIC_HIP_CHECK(hipStreamSynchronize(st));
+    if (ss) {
+        float rf[2];
+        std::memcpy(rf, pin, sizeof(rf));
+        ss[0] = static_cast<double>(rf[0]);
+        ss[1] = static_cast<double>(rf[1]);
+    }
+    if (pg) std::memcpy(pg, pin + 64, N_GEOM * sizeof(double));
+    return MOSAIC_OK;
+}
+}  // namespace
+
+extern "C" int mosaic_kin_stats(int64_t atom_count, int n_species, const int64_t* counts,
+                                int n_frames, int Ny, int Nz, void* workspace,
+                                size_t workspace_bytes, double* stage_stats, double* pixel_geom,
+                                mosaic_stream_t stream) {
+    size_t need = 0;
+    int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
+    if (rc != MOSAIC_OK) return rc;
+    if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
+    Layout L;
+    rc = make_layout(workspace, atom_count, n_species, counts, n_frames, Ny, Nz, L);
+    if (rc != MOSAIC_OK) return rc;
+    double* ss = stage_stats;
+    if (ss && (atom_count == 0 || L.n_tiles == 0)) {   // nothing staged: no statistics written
+        ss[0] = 0.0;
+        ss[1] = 0.0;
+        ss = nullptr;
+    }
+    if (!ss && !pixel_geom) return MOSAIC_OK;
+    return read_stats(L, ss, pixel_geom, static_cast<hipStream_t>(stream));
+}
+
 extern "C" int mosaic_kin_timing(int enable, float* ms_out) {   // ms_out: host [3]
     KinTiming* T = timing_state();
     if (!T) return MOSAIC_ERR_HIP;
@@ -5672,11 +5724,10 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     int rc = mosaic_kin_workspace(atom_count, n_species, counts, n_frames, Ny, Nz, &need);
     if (rc != MOSAIC_OK) return rc;
     if (!workspace || workspace_bytes < need) return MOSAIC_ERR_WORKSPACE;
-    if (!species_tab || !opts || !rot || !trans || !coords || !out || !stage_stats ||
-        !pixel_geom)
+    if (!species_tab || !opts || !rot || !trans || !coords || !out)
         return MOSAIC_ERR_ARGS;
-    if (!(opts->k > 0.0) || !(stage_stats[0] >= 0.0) || opts->path < 0 || opts->path > 7 ||
-        opts->analyser_kind < 0 || opts->analyser_kind > 2)
+    if (!(opts->k > 0.0) || (stage_stats && !(stage_stats[0] >= 0.0)) || opts->path < 0 ||
+        opts->path > 7 || opts->analyser_kind < 0 || opts->analyser_kind > 2)
         return MOSAIC_ERR_ARGS;
     hipStream_t st = static_cast<hipStream_t>(stream);
     Layout L;
@@ -5686,6 +5737,15 @@ extern "C" int mosaic_kin_sum(int64_t atom_count, int n_species, const int64_t* 
     const double K = opts->k * INV_2PI_HI;
     if (path_used) *path_used = 0;
     if (atom_count == 0 || L.n_tiles == 0) return MOSAIC_OK;
+    // statistics the caller left in the workspace: one read, one wait
+    double ss_ws[2], pg_ws[N_GEOM];
+    if (!stage_stats || !pixel_geom) {
+        rc = read_stats(L, stage_stats ? nullptr : ss_ws, pixel_geom ? nullptr : pg_ws, st);
+        if (rc != MOSAIC_OK) return rc;
+        if (!stage_stats) stage_stats = ss_ws;
+        if (!pixel_geom) pixel_geom = pg_ws;
+        if (!(stage_stats[0] >= 0.0)) return MOSAIC_ERR_ARGS;
+    }
 
     // Error bounds.  Per-tile path: tile radius against the nearest pixel
     // distance (tiles move rigidly with the sample pose).  Where it fails,

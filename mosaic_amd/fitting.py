@@ -81,6 +81,7 @@ class KinematicModel:
         self._staged = None
         self._pos = None
         self._version = 0
+        self.pixel_geom = None
 
     def _forward(self, u):
         pos = (self.pos0 + u.detach().to(torch.float32) if u is not None
@@ -96,7 +97,8 @@ class KinematicModel:
         self._out.zero_()
         K.kin_sum(staged, self.spc, self.opts, self.rot, self.trans, self.coords, self._out)
         self.path_used = staged.path_used
-        self.pixel_geom = tuple(staged.pixel_geom)
+        if self.pixel_geom is None:             # the detector's, fixed: read once
+            self.pixel_geom = tuple(staged.pixel_geom)
         return staged
 
     def field(self, u):

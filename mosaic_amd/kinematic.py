@@ -104,20 +104,65 @@ def workspace_bytes(n_atoms, counts, n_frames, Ny, Nz):
 
 
 class StagedSample:
-    """A sample staged into tiles inside a caller-owned workspace."""
+    """A sample staged into tiles inside a caller-owned workspace.
 
-    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, stats, k):
+    Its statistics (``stage_stats``, ``radius_wl``, and ``pixel_geom`` once
+    the pixel prologue ran) may stay in the workspace: ``kin_sum`` reads
+    them there with a single wait, and the first read from Python fetches
+    them (``mosaic_kin_stats``, which waits for the stream).
+    """
+
+    def __init__(self, n_atoms, counts, n_frames, Ny, Nz, workspace, stats, k, stream=None):
         self.n_atoms = int(n_atoms)
         self.k = float(k)
         self.counts = np.asarray(counts, dtype=np.int64)
         self.n_frames = int(n_frames)
         self.Ny, self.Nz = int(Ny), int(Nz)
         self.workspace = workspace
-        self.stage_stats = (ctypes.c_double * 2)(*stats)
-        self.radius_wl = float(stats[0])
-        self.pixel_geom = None
+        self._stats = None if stats is None else (ctypes.c_double * 2)(*stats)
+        self._pixel_geom = None
         self.pixels_key = None
         self.path_used = None
+        self._stream = stream
+
+    def _fetch(self, stage, pixels):
+        ss = (ctypes.c_double * 2)() if stage else None
+        pg = (ctypes.c_double * 8)() if pixels else None
+        c, cp_ = _counts_arr(self.counts)
+        check(_lib.lib().mosaic_kin_stats(self.n_atoms, int(c.shape[0]), cp_, self.n_frames,
+                                          self.Ny, self.Nz, _ptr(self.workspace),
+                                          int(self.workspace.numel()), ss, pg,
+                                          _stream_handle(self._stream)),
+              "mosaic_kin_stats")
+        if stage:
+            self._stats = ss
+        if pixels:
+            self._pixel_geom = pg
+
+    @property
+    def stage_stats(self):
+        if self._stats is None:
+            self._fetch(True, False)
+        return self._stats
+
+    @stage_stats.setter
+    def stage_stats(self, v):
+        self._stats = v
+
+    @property
+    def radius_wl(self):
+        return float(self.stage_stats[0])
+
+    @property
+    def pixel_geom(self):
+        """The pixel prologue's geometry (None before it ran on this workspace)."""
+        if self._pixel_geom is None and self.pixels_key is not None:
+            self._fetch(False, True)
+        return self._pixel_geom
+
+    @pixel_geom.setter
+    def pixel_geom(self, v):
+        self._pixel_geom = v
 
 
 def ensure_workspace(ws, nbytes, device):
@@ -126,12 +171,14 @@ def ensure_workspace(ws, nbytes, device):
     return ws
 
 
-def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=None):
+def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=None,
+          readback=False):
     """Stage (N,3) float32 Angstrom sample-frame positions on the device.
 
     ``codes`` are uint8 species codes on the device, ``counts[s]`` the number
     of atoms with code s (host), ``amp`` an optional (N,) complex64 device
-    tensor of incident amplitudes.
+    tensor of incident amplitudes.  Returns without waiting for the device
+    unless ``readback``: the statistics stay in the workspace until read.
     """
     if not pos.is_cuda:
         raise ValueError("positions must be a device tensor")
@@ -151,12 +198,13 @@ def stage(pos, codes, amp, counts, k, n_frames, Ny, Nz, workspace=None, stream=N
         raise ValueError("species counts do not add up to the atom count")
     nbytes = workspace_bytes(n, c, n_frames, Ny, Nz)
     ws = ensure_workspace(workspace, nbytes, pos.device)
-    stats = (ctypes.c_double * 2)()
+    stats = (ctypes.c_double * 2)() if readback else None
     check(_lib.lib().mosaic_kin_stage(n, _ptr(pos), _ptr(codes), _ptr(amp), int(c.shape[0]),
                                       cp_, float(k), int(n_frames), int(Ny), int(Nz), _ptr(ws),
                                       int(ws.numel()), stats, _stream_handle(stream)),
           "mosaic_kin_stage")
-    return StagedSample(n, c, n_frames, Ny, Nz, ws, list(stats), k)
+    return StagedSample(n, c, n_frames, Ny, Nz, ws, None if stats is None else list(stats), k,
+                        stream)
 
 
 def restage(staged, pos, amp=None, stream=None):
@@ -179,19 +227,20 @@ def restage(staged, pos, amp=None, stream=None):
                                         _stream_handle(stream)),
           "mosaic_kin_restage")
     staged.stage_stats = stats
-    staged.radius_wl = float(stats[0])
     return staged
 
 
-def kin_pixels(staged, species_tab, opts, coords, stream=None):
-    """Pixel prologue into the staged sample's workspace; returns
-    (rho_min [m], max row step of the pixel direction, max row second
-    difference, max deviation from the matrix-core patch model)."""
+def kin_pixels(staged, species_tab, opts, coords, stream=None, readback=False):
+    """Pixel prologue into the staged sample's workspace.  With ``readback``
+    waits for it and returns its geometry (rho_min [m], max row step of the
+    pixel direction, max row second difference, max deviation from the
+    matrix-core patch model, ...); else returns None at once, the geometry
+    left in the workspace (``staged.pixel_geom`` fetches it)."""
     npix = staged.Ny * staged.Nz
     if coords.dtype != torch.float64 or tuple(coords.shape) != (3, npix) or not coords.is_cuda:
         raise ValueError("coords must be a (3, Ny*Nz) float64 device tensor")
     coords = coords.contiguous()
-    geom = (ctypes.c_double * 8)()
+    geom = (ctypes.c_double * 8)() if readback else None
     check(_lib.lib().mosaic_kin_pixels(len(species_tab), ctypes.cast(species_tab, ctypes.c_void_p),
                                        ctypes.byref(opts), staged.Ny, staged.Nz, _ptr(coords),
                                        _ptr(staged.workspace), int(staged.workspace.numel()),
@@ -199,7 +248,7 @@ def kin_pixels(staged, species_tab, opts, coords, stream=None):
           "mosaic_kin_pixels")
     staged.pixel_geom = geom
     staged.pixels_key = _pixels_key(species_tab, opts, coords)
-    return tuple(geom)
+    return None if geom is None else tuple(geom)
 
 
 def _pixels_key(species_tab, opts, coords):
@@ -249,7 +298,7 @@ def kin_sum(staged, species_tab, opts, rot, trans, coords, out, stream=None):
         staged.n_atoms, int(c.shape[0]), cp_, ctypes.cast(species_tab, ctypes.c_void_p),
         ctypes.byref(opts), F, rot.ctypes.data_as(ctypes.c_void_p),
         trans.ctypes.data_as(ctypes.c_void_p), staged.Ny, staged.Nz, _ptr(coords), _ptr(out),
-        int(out.dtype == torch.complex128), staged.stage_stats, staged.pixel_geom,
+        int(out.dtype == torch.complex128), staged._stats, staged._pixel_geom,
         _ptr(staged.workspace), int(staged.workspace.numel()), ctypes.byref(used),
         _stream_handle(stream)), "mosaic_kin_sum")
     staged.path_used = used.value

@@ -200,6 +200,38 @@ def test_restage_moved_atoms(device):
     assert e <= TOL
 
 
+def test_statistics_left_in_workspace(device):
+    """Staging and the pixel prologue without read-back (the sum reads their
+    statistics from the workspace, one wait) give the same field, bit for
+    bit, as with the statistics read back to the host, and the statistics
+    fetched afterwards equal those read back."""
+    from mosaic_amd import tables
+    w = W.perovskite_screw_bragg200(n=10, npix=64, blocks=(2, 2, 2))
+    b, s, d, st = w["beam"], w["sample"], w["detector"], w["stage"]
+    pos = torch.as_tensor(np.ascontiguousarray(s.load_chunk_positions(1), np.float32),
+                          device=device)
+    codes_h = np.asarray(s.load_chunk_species(1, codes=True)[0], np.uint8)
+    codes = torch.as_tensor(codes_h, device=device)
+    counts = np.bincount(codes_h, minlength=3).astype(np.int64)
+    k = 2 * np.pi / b.wavelength
+    Ny, Nz = d.shape
+    spc = K.species_table(tables.species_constants(["Sr", "Ti", "O"], b.energy))
+    opts = K.kin_opts(k, False, True, b._polarization_state(), False)
+    coords = torch.as_tensor(np.ascontiguousarray(d.pixel_coordinates / 1e10), device=device)
+    rot = np.asarray(st.rotation, np.float32)[None]
+    tr = np.asarray(st.translation, np.float32)[None]
+    fields, stats = [], []
+    for readback in (True, False):
+        staged = K.stage(pos, codes, None, counts, k, 1, Ny, Nz, readback=readback)
+        K.kin_pixels(staged, spc, opts, coords, readback=readback)
+        out = torch.zeros((1, Nz, Ny), dtype=torch.complex64, device=device)
+        K.kin_sum(staged, spc, opts, rot, tr, coords, out)
+        fields.append(out.cpu().numpy())
+        stats.append((tuple(staged.stage_stats), tuple(staged.pixel_geom), staged.path_used))
+    assert stats[0] == stats[1]
+    assert np.array_equal(fields[0], fields[1])
+
+
 def test_restage_keeps_units(device):
     """Restaging the staged positions rebuilds the same tiles and re-measures
     the radii of the pairs and quads of tiles, so the moment sum forms the
