@@ -61,9 +61,9 @@ COUNTERS = {
         add_f32=2.06103e8, mul_f32=8.25971e8, fma_f32=2.82295e9, trans_f32=9.04499e7),
     "k_gm_moments": dict(
         source="profiles/k_gm_moments_perovskite5M_512px.md",
-        fetch_kib=320410.0, write_kib=444450.0, valu=4.5234e8,
-        add_f32=8.7043e6, mul_f32=6.9233e7, fma_f32=4.1772e7, trans_f32=4.0004e7,
-        mfma=3.5e7, mfma_f32=5.22e6),
+        fetch_kib=315430.0, write_kib=440380.0, valu=4.3926e8,
+        add_f32=8.3862e6, mul_f32=6.4594e7, fma_f32=4.1348e7, trans_f32=4.0004e7,
+        mfma=3.5e7, mfma_f32=5.0e6),
     "k_sum_mma": dict(
         source="profiles/k_sum_mma_perovskite5M_512px.md",
         fetch_kib=1831133.0, write_kib=2542443.8, valu=None,
