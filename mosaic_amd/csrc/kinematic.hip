@@ -3106,6 +3106,15 @@ __device__ __forceinline__ void gm_split_pair(float x0, float x1, unsigned int& 
     lo = l;
 }
 
+// The hi word alone of a x0 and a x1 (fused, rounded to nearest): for
+// operand values small enough that their lo halves are below the error bound.
+__device__ __forceinline__ unsigned int gm_prod_hi_pair(float a, float x0, float x1) {
+    unsigned int h;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(h) : "v"(a), "v"(x0));
+    asm("v_fma_mixhi_f16 %0, %1, %2, 0" : "+v"(h) : "v"(a), "v"(x1));
+    return h;
+}
+
 // hi + lo halves of a x0 and a x1 straight from the products (fused, one
 // rounding each): hi = f16(a x), lo = f16(a x - hi), four mixed-precision FMAs
 // for the pair against two multiplies, a convert and two FMAs.
@@ -4018,20 +4027,29 @@ void k_gm_eval(const GmArgs A) {
                 ti[j] = br * Gi[j] + bi * Gr[j];
             }
             // L = alpha x^m (slots 0..7), N: slots 0..3 T, slots 4..7 cf i T
-            // = (-cf Ti, cf Tr); the products split as they are formed
+            // = (-cf Ti, cf Tr); the products split as they are formed.  The
+            // cross slots (4..7) take hi halves alone: their terms are at most
+            // cmax <= 1.4e-3 of the main terms (the bound above), so their lo
+            // halves, 2^-11 of that, stay below 1e-6 of the sum
             uintx4 lrh, lrl, lih, lil, nrh, nrl, nih, nil;
 #pragma unroll
             for (int u2 = 0; u2 < 4; ++u2) {
                 unsigned int h, l;
-                gm_split_prod_pair(ar, lx[2 * u2], lx[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
-                gm_split_prod_pair(ai, lx[2 * u2], lx[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
                 if (u2 < 2) {
+                    gm_split_prod_pair(ar, lx[2 * u2], lx[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
+                    gm_split_prod_pair(ai, lx[2 * u2], lx[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
                     gm_split_pair(tr[2 * u2], tr[2 * u2 + 1], h, l); nrh[u2] = h; nrl[u2] = l;
                     gm_split_pair(ti[2 * u2], ti[2 * u2 + 1], h, l); nih[u2] = h; nil[u2] = l;
                 } else {
                     const int j = 2 * (u2 - 2);
-                    gm_split_prod_pair(-cf, ti[j], ti[j + 1], h, l); nrh[u2] = h; nrl[u2] = l;
-                    gm_split_prod_pair(cf, tr[j], tr[j + 1], h, l); nih[u2] = h; nil[u2] = l;
+                    lrh[u2] = gm_prod_hi_pair(ar, lx[2 * u2], lx[2 * u2 + 1]);
+                    lih[u2] = gm_prod_hi_pair(ai, lx[2 * u2], lx[2 * u2 + 1]);
+                    nrh[u2] = gm_prod_hi_pair(-cf, ti[j], ti[j + 1]);
+                    nih[u2] = gm_prod_hi_pair(cf, tr[j], tr[j + 1]);
+                    lrl[u2] = 0u;
+                    lil[u2] = 0u;
+                    nrl[u2] = 0u;
+                    nil[u2] = 0u;
                 }
             }
             const uintx4 sgn = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
