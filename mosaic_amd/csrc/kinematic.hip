@@ -4027,19 +4027,36 @@ void k_gm_eval(const GmArgs A) {
                 ti[j] = br * Gi[j] + bi * Gr[j];
             }
             // L = alpha x^m (slots 0..7), N: slots 0..3 T, slots 4..7 cf i T
-            // = (-cf Ti, cf Tr); the products split as they are formed.  The
-            // cross slots (4..7) take hi halves alone: their terms are at most
-            // cmax <= 1.4e-3 of the main terms (the bound above), so their lo
-            // halves, 2^-11 of that, stay below 1e-6 of the sum
+            // = (-cf Ti, cf Tr); the products split as they are formed.  Only
+            // slots 0, 1 (powers 0, 3 and 1, 2) keep lo halves: slots 2, 3
+            // hold powers m >= 4 (5, 6 and 4, none), whose terms are at most
+            // phi^m / m! <= 2.0e-3 of the sum's scale (phi <= 0.47, the
+            // degree-6 bound), and the cross slots (4..7) terms at most cmax <=
+            // 1.4e-3 of the main ones (the bound above); their lo halves, 2^-11
+            // of that, stay near 1e-6 and below 7e-7 of it
             uintx4 lrh, lrl, lih, lil, nrh, nrl, nih, nil;
 #pragma unroll
             for (int u2 = 0; u2 < 4; ++u2) {
                 unsigned int h, l;
-                if (u2 < 2) {
-                    gm_split_prod_pair(ar, lx[2 * u2], lx[2 * u2 + 1], h, l); lrh[u2] = h; lrl[u2] = l;
-                    gm_split_prod_pair(ai, lx[2 * u2], lx[2 * u2 + 1], h, l); lih[u2] = h; lil[u2] = l;
-                    gm_split_pair(tr[2 * u2], tr[2 * u2 + 1], h, l); nrh[u2] = h; nrl[u2] = l;
-                    gm_split_pair(ti[2 * u2], ti[2 * u2 + 1], h, l); nih[u2] = h; nil[u2] = l;
+                if (u2 == 0) {
+                    gm_split_prod_pair(ar, lx[0], lx[1], h, l); lrh[0] = h; lrl[0] = l;
+                    gm_split_prod_pair(ai, lx[0], lx[1], h, l); lih[0] = h; lil[0] = l;
+                    gm_split_pair(tr[0], tr[1], h, l); nrh[0] = h; nrl[0] = l;
+                    gm_split_pair(ti[0], ti[1], h, l); nih[0] = h; nil[0] = l;
+                } else if (u2 == 1) {
+                    lrh[1] = gm_prod_hi_pair(ar, lx[2], lx[3]);
+                    lih[1] = gm_prod_hi_pair(ai, lx[2], lx[3]);
+                    halfx2 hr, hi2;
+                    hr[0] = static_cast<__fp16>(tr[2]);
+                    hr[1] = static_cast<__fp16>(tr[3]);
+                    hi2[0] = static_cast<__fp16>(ti[2]);
+                    hi2[1] = static_cast<__fp16>(ti[3]);
+                    nrh[1] = __builtin_bit_cast(unsigned int, hr);
+                    nih[1] = __builtin_bit_cast(unsigned int, hi2);
+                    lrl[1] = 0u;
+                    lil[1] = 0u;
+                    nrl[1] = 0u;
+                    nil[1] = 0u;
                 } else {
                     const int j = 2 * (u2 - 2);
                     lrh[u2] = gm_prod_hi_pair(ar, lx[2 * u2], lx[2 * u2 + 1]);
